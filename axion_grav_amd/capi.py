@@ -1,0 +1,271 @@
+"""ctypes binding of libaxgrav (include/axgrav.h) with numpy array conversion.
+
+This is the Python face of the same C-ABI a Fortran host binds through ISO_C_BINDING.  The
+library is loaded from ``axion_grav_amd/lib/libaxgrav.so`` (built in-tree by
+``python -m axion_grav_amd.build``); there is no fallback: if the library or a GPU is missing,
+the calls raise.
+
+Complex fields are numpy ``complex128`` arrays of shape ``(N, N, N)`` indexed ``[z, y, x]``
+(x fastest), byte-identical to the reference's Fortran ``data1(x, y, z)``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libaxgrav.so")
+
+FP64, FP32C, FP32 = 0, 1, 2
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ESTATE", -5: "ENUMERIC"}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("ngrid", C.c_int64),
+        ("precision", C.c_int32),
+        ("grav_spec", C.c_int32),
+        ("initial_loop", C.c_int32),
+        ("gw_fix_swap", C.c_int32),
+        ("num_bin_grav", C.c_int32),
+        ("num_bin", C.c_int32),
+        ("id_strings", C.c_int32),
+        ("id_walls", C.c_int32),
+        ("spec_est", C.c_int32),
+        ("device", C.c_int32),
+        ("dspa", C.c_double),
+        ("dwav", C.c_double),
+        ("dv_spa", C.c_double),
+        ("dv_wav", C.c_double),
+        ("sigma_cr", C.c_double),
+        ("tau_cr", C.c_double),
+        ("a_now", C.c_double),
+        ("kappa", C.c_double),
+        ("eps_dtau", C.c_double),
+        ("dtau_fixed", C.c_double),
+        ("potential_kappa0", C.c_double),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class Keys(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("g_rel", C.c_double), ("kappa", C.c_double),
+                ("tau_fin", C.c_double), ("boxsize", C.c_double), ("ngrid", C.c_int64)]
+
+
+class DefectOut(C.Structure):
+    _fields_ = [("n_string", C.c_int64), ("n_wall", C.c_int64), ("n_mask", C.c_int64)]
+
+
+EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
+           "axg_get_fields", "axg_init_flat", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
+           "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
+           "axg_field_spectrum", "axg_energy")
+
+_LIB = None
+
+
+def lib():
+    """Load libaxgrav.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m axion_grav_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        p, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sig = {
+            "axg_derive": [C.POINTER(Keys), C.POINTER(Params), p, p, p],
+            "axg_get_dtau": [C.POINTER(Params), d, d, p, p],
+            "axg_create": [C.POINTER(Params), C.POINTER(p)],
+            "axg_destroy": [p],
+            "axg_set_fields": [p, p, p, i64],
+            "axg_get_fields": [p, p, p, i64],
+            "axg_init_flat": [p, C.c_uint64, d],
+            "axg_get_gw_state": [p, p, i64],
+            "axg_set_gw_state": [p, p, i64],
+            "axg_advance": [p, d, d, p],
+            "axg_advance_steps": [p, d, d, i32],
+            "axg_gw_spectrum": [p, p, p, p, p, i32],
+            "axg_defects": [p, C.POINTER(DefectOut)],
+            "axg_get_mask": [p, p, i64, p],
+            "axg_get_flags": [p, p, i64],
+            "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
+            "axg_energy": [p, d, p],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.axg_last_error.argtypes = [p]
+        L.axg_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+class AxgError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def derive(sigma, g_rel, kappa, tau_fin, boxsize, ngrid, params=None):
+    """settings.f90:274-297 derived units; returns (Params, extras dict)."""
+    k = Keys(sigma, g_rel, kappa, tau_fin, boxsize, int(ngrid))
+    P = params if params is not None else Params()
+    z, w, t = C.c_double(), C.c_double(), C.c_double()
+    rc = lib().axg_derive(C.byref(k), C.byref(P), C.byref(z), C.byref(w), C.byref(t))
+    if rc != 0:
+        raise AxgError(f"axg_derive failed ({rc})")
+    return P, dict(zeta=z.value, width_cr=w.value, thickness_cr=t.value)
+
+
+def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initial_loop=True, kappa=1e-30,
+                num_bin_grav=25, num_bin=50, id_strings=True, id_walls=True, spec_est=True, tau_cr=1.0,
+                a_now=1.0, eps_dtau=0.0, dtau_fixed=0.0, gw_fix_swap=False, potential_kappa0=0.0, device=0):
+    """Params from a lattice spacing and sigma_cr (dwav, dv_spa, dv_wav as settings.f90:295-297)."""
+    twopi_t = 2 * 3.14159265359
+    P = Params()
+    P.ngrid = int(ngrid)
+    P.precision = precision
+    P.grav_spec = int(grav_spec)
+    P.initial_loop = int(initial_loop)
+    P.gw_fix_swap = int(gw_fix_swap)
+    P.num_bin_grav = num_bin_grav
+    P.num_bin = num_bin
+    P.id_strings, P.id_walls, P.spec_est = int(id_strings), int(id_walls), int(spec_est)
+    P.device = device
+    P.dspa = dspa
+    P.dwav = twopi_t / dspa / ngrid
+    P.dv_spa = dspa * dspa * dspa
+    r = P.dwav / twopi_t
+    P.dv_wav = r * r * r
+    P.sigma_cr = sigma_cr
+    P.tau_cr = tau_cr
+    P.a_now = a_now
+    P.kappa = kappa
+    P.eps_dtau = eps_dtau
+    P.dtau_fixed = dtau_fixed
+    P.potential_kappa0 = potential_kappa0
+    return P
+
+
+def get_dtau(P, t1, t2):
+    dt, n = C.c_double(), C.c_int32()
+    rc = lib().axg_get_dtau(C.byref(P), t1, t2, C.byref(dt), C.byref(n))
+    if rc != 0:
+        raise AxgError(f"axg_get_dtau failed ({rc})")
+    return dt.value, n.value
+
+
+class Lattice:
+    """One device-resident lattice (an axg_handle)."""
+
+    def __init__(self, params):
+        self.params = params
+        self.N = int(params.ngrid)
+        h = C.c_void_p()
+        rc = lib().axg_create(C.byref(params), C.byref(h))
+        if rc != 0:
+            msg = lib().axg_last_error(None).decode()
+            raise AxgError(f"axg_create failed ({ERRORS.get(rc, rc)}): {msg}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().axg_last_error(self._h).decode()
+            raise AxgError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def close(self):
+        if self._h:
+            lib().axg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n3(self):
+        return self.N ** 3
+
+    def set_fields(self, phi=None, pi=None):
+        a = np.ascontiguousarray(phi, np.complex128) if phi is not None else None
+        b = np.ascontiguousarray(pi, np.complex128) if pi is not None else None
+        for x in (a, b):
+            if x is not None and x.size != self.n3:
+                raise ValueError("field size must be N^3")
+        self._check(lib().axg_set_fields(self._h, _ptr(a), _ptr(b), self.n3), "axg_set_fields")
+
+    def get_fields(self):
+        phi = np.empty((self.N,) * 3, np.complex128)
+        pi = np.empty((self.N,) * 3, np.complex128)
+        self._check(lib().axg_get_fields(self._h, _ptr(phi), _ptr(pi), self.n3), "axg_get_fields")
+        return phi, pi
+
+    def init_flat(self, seed, kmax):
+        self._check(lib().axg_init_flat(self._h, int(seed), float(kmax)), "axg_init_flat")
+
+    def gw_state(self):
+        N = self.N
+        A = np.empty((4, N, N, N // 2 + 1), np.complex128)
+        self._check(lib().axg_get_gw_state(self._h, _ptr(A), A.size), "axg_get_gw_state")
+        return A
+
+    def set_gw_state(self, A):
+        A = np.ascontiguousarray(A, np.complex128)
+        self._check(lib().axg_set_gw_state(self._h, _ptr(A), A.size), "axg_set_gw_state")
+
+    def advance(self, t1, t2):
+        n = C.c_int32()
+        self._check(lib().axg_advance(self._h, t1, t2, C.byref(n)), "axg_advance")
+        return n.value
+
+    def advance_steps(self, t1, dt, nsteps):
+        self._check(lib().axg_advance_steps(self._h, t1, dt, int(nsteps)), "axg_advance_steps")
+
+    def gw_spectrum(self):
+        nb = self.params.num_bin_grav
+        q, pw, nu, de = (np.zeros(nb) for _ in range(4))
+        self._check(lib().axg_gw_spectrum(self._h, _ptr(q), _ptr(pw), _ptr(nu), _ptr(de), nb), "axg_gw_spectrum")
+        return dict(qmid=q, power=pw, numer=nu, denom=de)
+
+    def defects(self):
+        out = DefectOut()
+        self._check(lib().axg_defects(self._h, C.byref(out)), "axg_defects")
+        return np.array([out.n_string, out.n_wall, out.n_mask], np.int64)
+
+    def mask_list(self):
+        cnt = C.c_int64()
+        self._check(lib().axg_get_mask(self._h, None, 0, C.byref(cnt)), "axg_get_mask")
+        n = cnt.value
+        buf = np.zeros((max(n, 1), 3), np.int32)
+        self._check(lib().axg_get_mask(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_mask")
+        return buf[:n]
+
+    def flags(self):
+        f = np.empty((self.N,) * 3, np.int8)
+        self._check(lib().axg_get_flags(self._h, _ptr(f), self.n3), "axg_get_flags")
+        return f
+
+    def field_spectrum(self, tau):
+        nb = self.params.num_bin
+        q, pw, cv, nm = (np.zeros(nb) for _ in range(4))
+        rho = C.c_double()
+        self._check(lib().axg_field_spectrum(self._h, tau, _ptr(q), _ptr(pw), _ptr(cv), _ptr(nm), C.byref(rho), nb),
+                    "axg_field_spectrum")
+        return dict(qmid=q, power=pw, conv=cv, num=nm, rho=rho.value)
+
+    def energy(self, tau):
+        out = np.zeros(4)
+        self._check(lib().axg_energy(self._h, tau, _ptr(out)), "axg_energy")
+        return out

@@ -1,0 +1,124 @@
+// Host-side internals shared by the translation units of libaxgrav: the handle, the
+// kernel launchers and the small host math (get_dtau, potential scalars, bin tables).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/axgrav.h"
+#include "axg_device.h"
+
+// potential.f90:8 -- the truncated pi used by the reference for every physical constant,
+// bin edge and defect threshold.
+constexpr double AXG_PI_T = 3.14159265359;
+constexpr double AXG_TWOPI_T = 2 * AXG_PI_T;
+constexpr double AXG_THRM_SIGMA = 2.449489742783178;  // potential.f90:30
+
+#define AXG_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) throw axg::Error(AXG_EDEVICE, std::string(#call) + ": " + \
+                                               hipGetErrorString(e_));                  \
+    } while (0)
+
+namespace axg {
+
+struct Error {
+    int code;
+    std::string msg;
+    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// Scalars of one kick (evolve.f90:291-305).
+struct KickScalars {
+    double f1, f2, f3, m2, ma2, kt;  // kt: N_DW=2 tilt coefficient kappa0*t^4 (0 = off)
+};
+KickScalars kick_scalars(const axg_params& p, double t, double dt);
+double maxion2(double thrm, double kappa);
+double msquared(double thrm);
+
+FftPlan make_plan(int n);
+bool supported_n(int64_t n);
+
+// Column-tile width of the strided FFT passes: 128 bytes of complex per row.
+template <typename T> constexpr int col_tile() { return 128 / (int)sizeof(typename V2<T>::type); }
+
+// Row pitch (in complex elements) of the half-spectrum arrays: N/2+1 rounded up to 16.
+inline int half_pitch(int n) { return ((n / 2 + 1) + 15) / 16 * 16; }
+
+// ------------------------------------------------------------------ kernel launchers
+// T = double (FP64) or float (FP32 storage); COMP = FP32 with int16 compensation.
+
+template <typename T, bool COMP>
+void launch_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, const void* pi,
+                  const void* cphi_in, void* cphi_out, const void* cpi, double dt);
+
+template <typename T, bool COMP>
+void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi,
+                       const void* cphi_in, void* cphi_out, void* cpi, void* src,
+                       const KickScalars& k, double dspa, double dt_drift, bool gw);
+
+template <typename T>
+void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan,
+                     const void* tw);
+
+template <typename T, int SIGN>
+void launch_fft_cols(hipStream_t s, int N, void* data, int64_t es, int64_t ps, int nplanes, int ncol,
+                     const FftPlan& plan, const void* tw);
+
+template <typename T, int SIGN>
+void launch_fft_rows(hipStream_t s, int N, void* data, int64_t nrows, const FftPlan& plan, const void* tw);
+
+struct GwStepScalars {
+    double tau, dw, dwav;
+    int fix_swap;
+};
+template <typename T>
+void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const FftPlan& plan,
+                         const void* tw, const GwStepScalars& g);
+
+template <typename T>
+void launch_gw_bin(hipStream_t s, int N, int pitch, const void* A, const uint16_t* bin_of_n2, int nb,
+                   double* partial, int nwg, double* out);
+
+// diagnostics
+template <typename T, bool COMP>
+void launch_energy(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi,
+                   const void* cpi, double m2, double ma2, double* partial, int nwg, double* out);
+
+template <typename T, bool COMP>
+void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, int id_strings, int id_walls,
+                     int spec_est, int8_t* flags, int64_t* block_counts, int64_t nblk);
+
+void launch_mask_compact(hipStream_t s, int N, const int8_t* flags, const int64_t* block_offsets,
+                         int32_t* ixyz, int64_t nblk);
+
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total);
+
+template <typename T, bool COMP>
+void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags,
+                         double2* out, double fill_unmasked, bool use_pi);
+
+void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int nb,
+                     double scale, double* partial, int nwg, double* out);
+
+void launch_n2_hist(hipStream_t s, int N, const double2* data, double* plane_part, double* hist);
+
+void launch_deconv_mat(hipStream_t s, int nb, int num_cov, int nlq, const double* largeq, const double* mat,
+                       const double* dqs, const double* hist, const double* cnt, int64_t nn2, double dwav,
+                       double* mat0, double* tat0);
+
+template <typename T, bool COMP>
+void launch_store_field(hipStream_t s, int N, const double2* in, void* f, void* cf);
+
+template <typename T, bool COMP>
+void launch_load_field(hipStream_t s, int N, const void* f, const void* cf, double2* out);
+
+void launch_scatter_modes(hipStream_t s, int N, const int64_t* idx, const double2* val, int64_t nmodes,
+                          double2* out);
+
+void launch_sumsq(hipStream_t s, int64_t n, const double2* data, double* partial, int nwg, double* out);
+void launch_scale(hipStream_t s, int64_t n, double2* data, double scale);
+
+}  // namespace axg

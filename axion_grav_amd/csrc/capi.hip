@@ -1,0 +1,918 @@
+// C-ABI entry points of libaxgrav (declared in include/axgrav.h) and the host-side
+// scalars they need.  Every device buffer belongs to the handle; every entry point runs on
+// the handle's stream and synchronizes before returning.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "axg_internal.h"
+
+namespace axg {
+
+// ------------------------------------------------------------------ potential scalars
+
+double msquared(double thrm) { return (thrm * thrm - AXG_THRM_SIGMA * AXG_THRM_SIGMA) / 3; }  // potential.f90:34-41
+
+double maxion2(double thrm, double kappa)  // potential.f90:62-79
+{
+    const double n = 6.68, cT = 6.26, c0 = 1.0;
+    double thrm12 = kappa * pow(cT / c0, 1.0 / n);
+    double m = thrm > thrm12 ? cT * pow(kappa / thrm, n) : c0;
+    double k2 = kappa * kappa;
+    return m * (k2 * k2);
+}
+
+// evolve.f90:291-305 in the same FP64 operation order
+KickScalars kick_scalars(const axg_params& p, double t, double dt)
+{
+    KickScalars k;
+    double hubble = 1 / t;
+    double thrm = p.initial_loop ? 0.0 : AXG_THRM_SIGMA / t;
+    k.m2 = msquared(thrm);
+    k.ma2 = maxion2(thrm, p.kappa);
+    double fact0 = 1 + dt * hubble;
+    double fact1 = 1 - dt * hubble;
+    double fact2 = dt / (p.dspa * p.dspa);
+    double fact3 = -(dt * (t * t) * (p.sigma_cr * p.sigma_cr));
+    k.f1 = fact1 / fact0;
+    k.f2 = fact2 / fact0;
+    k.f3 = fact3 / fact0;
+    double t2 = t * t;
+    k.kt = p.potential_kappa0 != 0.0 ? p.potential_kappa0 * (t2 * t2) : 0.0;
+    return k;
+}
+
+bool supported_n(int64_t n)
+{
+    if (n < 8 || n > 1024 || (n & 1)) return false;
+    int64_t m = n;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    return m == 1;
+}
+
+FftPlan make_plan(int n)
+{
+    FftPlan p{};
+    p.n = n;
+    int m = n, k = 0;
+    while (m % 3 == 0) { p.radix[k++] = 3; m /= 3; }
+    while (m % 8 == 0) { p.radix[k++] = 8; m /= 8; }
+    if (m % 4 == 0) { p.radix[k++] = 4; m /= 4; }
+    if (m % 2 == 0) { p.radix[k++] = 2; m /= 2; }
+    p.nst = k;
+    return p;
+}
+
+}  // namespace axg
+
+using namespace axg;
+
+// ------------------------------------------------------------------ handle
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct axg_handle {
+    axg_params p{};
+    int N = 0;
+    int64_t n3 = 0;
+    int P = 0;        // half-spectrum row pitch
+    int nh = 0;       // N/2 + 1
+    int prec = AXG_FP64;
+    bool fp32 = false, comp = false;
+    size_t t2 = 16;   // bytes of one stored complex
+    hipStream_t stream = nullptr;
+    FftPlan plan{};
+    std::vector<DevBuf> bufs;
+    void* phi[2] = {nullptr, nullptr};
+    void* cphi[2] = {nullptr, nullptr};
+    void* pi = nullptr;
+    void* cpi = nullptr;
+    void* src = nullptr;
+    void* W = nullptr;
+    void* A = nullptr;
+    void* tw = nullptr;      // twiddles in the storage precision
+    void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
+    int cur = 0;
+    // diagnostics
+    uint16_t* gw_bin = nullptr;
+    uint16_t* spec_bin = nullptr;
+    int64_t nn2 = 0;          // entries of the n^2 tables (3 (N/2)^2 + 1)
+    double* red = nullptr;    // reduction scratch
+    size_t red_bytes = 0;
+    double2* work = nullptr;  // N^3 FP64 complex work array (spectra, IC, field copies)
+    int8_t* flags = nullptr;
+    int64_t* blk = nullptr;   // per-block counts / offsets of the classifier
+    int32_t* mask = nullptr;  // compacted mask list (3 ints per cell)
+    int64_t nmask = -1;       // -1: no classification yet
+    bool have_flags = false;
+    // spectrum tables (init_spectrum)
+    std::vector<double> wav;  // num_bin+1 edges
+    double* d_largeq = nullptr;
+    double* d_matspec = nullptr;
+    double* d_dqs = nullptr;
+    double* d_cnt = nullptr;  // lattice points per n^2
+    double* d_hist = nullptr;
+    double* d_plane = nullptr;
+    std::string err;
+
+    void* alloc(size_t bytes)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(AXG_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+        bufs.push_back({q, bytes});
+        return q;
+    }
+    void release()
+    {
+        for (auto& b : bufs) (void)hipFree(b.p);
+        bufs.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+static thread_local std::string g_create_err;
+
+template <typename F>
+static int guarded(axg_handle* h, F&& f)
+{
+    if (!h) return AXG_EINVAL;
+    try {
+        AXG_HIP(hipSetDevice(h->p.device));
+        f();
+        return AXG_OK;
+    } catch (const Error& e) {
+        h->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        h->err = "host allocation failed";
+        return AXG_ENOMEM;
+    } catch (...) {
+        h->err = "unexpected exception";
+        return AXG_EDEVICE;
+    }
+}
+
+static void require(bool ok, const std::string& msg)
+{
+    if (!ok) throw Error(AXG_EINVAL, msg);
+}
+
+// ------------------------------------------------------------------ settings
+
+extern "C" int axg_derive(const axg_keys* k, axg_params* p, double* zeta, double* width_cr, double* thickness_cr)
+{
+    if (!k || !p) return AXG_EINVAL;
+    // settings.f90:274-297, potential.f90:20-21
+    const double tcmb_in_mstar = 2.726 / 1.16045250061657e13 / 2.435e18;
+    double a_now = k->sigma * AXG_THRM_SIGMA / tcmb_in_mstar * pow(k->g_rel / 3.9091, 1.0 / 3);
+    double tau_cr = 1 / AXG_PI_T / sqrt(k->g_rel / 90.0) / (k->sigma * k->sigma) / (AXG_THRM_SIGMA * AXG_THRM_SIGMA);
+    double sigma_cr = tau_cr * k->sigma;
+    double dspa = k->boxsize * k->tau_fin / (double)k->ngrid;
+    double dwav = AXG_TWOPI_T / dspa / (double)k->ngrid;
+    double r = dwav / AXG_TWOPI_T;
+    p->ngrid = k->ngrid;
+    p->a_now = a_now;
+    p->tau_cr = tau_cr;
+    p->sigma_cr = sigma_cr;
+    p->kappa = k->kappa;
+    p->dspa = dspa;
+    p->dwav = dwav;
+    p->dv_spa = dspa * dspa * dspa;
+    p->dv_wav = r * r * r;
+    if (zeta) *zeta = sqrt(22.5 / k->g_rel) / k->sigma / AXG_PI_T;
+    if (width_cr) *width_cr = 1 / sqrt(2.0) / sigma_cr;
+    if (thickness_cr) *thickness_cr = 1 / sqrt(2.0) / (sqrt(maxion2(1.e-3 * k->kappa, k->kappa)) * sigma_cr);
+    return AXG_OK;
+}
+
+static void get_dtau(const axg_params& p, double t1, double t2, double* dt, int32_t* n)
+{
+    if (p.dtau_fixed > 0) {
+        // fixed step: the largest step <= dtau_fixed that divides the interval evenly
+        int32_t nn = (int32_t)ceil((t2 - t1) / p.dtau_fixed - 1e-9);
+        if (nn < 1) nn = 1;
+        *n = nn;
+        *dt = (t2 - t1) / nn;
+        return;
+    }
+    // evolve.f90:10-24
+    double omega = AXG_THRM_SIGMA * t2 * p.sigma_cr;
+    double kmax = p.dwav * (double)p.ngrid;
+    omega = sqrt(kmax * kmax + omega * omega);
+    double d = p.eps_dtau / omega;
+    int32_t nn = (int32_t)ceil((t2 - t1) / d);
+    *n = nn;
+    *dt = (t2 - t1) / nn;
+}
+
+extern "C" int axg_get_dtau(const axg_params* p, double t1, double t2, double* dtau, int32_t* nsteps)
+{
+    if (!p || !dtau || !nsteps || !(t2 > t1)) return AXG_EINVAL;
+    if (p->dtau_fixed <= 0 && !(p->eps_dtau > 0)) return AXG_EINVAL;
+    get_dtau(*p, t1, t2, dtau, nsteps);
+    return AXG_OK;
+}
+
+// Bin of a mode with |n|^2 = n2, in the reference's FP64 order: q = sqrt(n2)*dwav,
+// i = nint(q/dq + 0.5) with dq = wave(1) - wave(0) (grav_waves.f90:62-71,265,279;
+// spectrum.f90:63-72,476,538).  0 = outside 1..nb.
+static void bin_table(int64_t nn2, int nb, double dspa, double dwav, std::vector<uint16_t>& out,
+                      std::vector<double>* edges)
+{
+    double qmax = AXG_PI_T / dspa;
+    double dq0 = qmax / nb;
+    std::vector<double> wave(nb + 1);
+    wave[0] = 0;
+    for (int b = 1; b <= nb; b++) wave[b] = wave[b - 1] + dq0;
+    double dq = wave[1] - wave[0];
+    out.assign(nn2, 0);
+    for (int64_t n2 = 0; n2 < nn2; n2++) {
+        double q = sqrt((double)n2) * dwav;
+        long i = lround(q / dq + 0.5);
+        out[n2] = (i >= 1 && i <= nb) ? (uint16_t)i : 0;
+    }
+    if (edges) *edges = wave;
+}
+
+// spectrum.f90:117-171 integrate_p_check (3D branch)
+static double integrate_p(int nint_, double li, double si, double lj, double sj, double q)
+{
+    double maxp = std::min(lj, q + li);
+    double minp = std::max(q - li, si - q);
+    minp = std::max(minp, sj);
+    double dp = (maxp - minp) / nint_;
+    double x = 0;
+    if (dp > 0) {
+        for (int ip = 1; ip <= nint_; ip++) {
+            double pp = dp * (ip - 0.5) + minp;
+            double maxmu, minmu;
+            if (q == 0.0) { maxmu = 1; minmu = -1; }
+            else {
+                maxmu = (li * li - pp * pp - q * q) / 2 / pp / q;
+                minmu = (si * si - pp * pp - q * q) / 2 / pp / q;
+                maxmu = std::min(1.0, maxmu);
+                minmu = std::max(-1.0, minmu);
+            }
+            x = x + pp * pp * dp * (maxmu - minmu);
+        }
+    }
+    return x * AXG_TWOPI_T;
+}
+
+// ------------------------------------------------------------------ create / destroy
+
+extern "C" int axg_create(const axg_params* pin, axg_handle** out)
+{
+    g_create_err.clear();
+    if (!pin || !out) { g_create_err = "null argument"; return AXG_EINVAL; }
+    *out = nullptr;
+    std::unique_ptr<axg_handle> h(new (std::nothrow) axg_handle());
+    if (!h) { g_create_err = "host allocation failed"; return AXG_ENOMEM; }
+    h->p = *pin;
+    const axg_params& p = h->p;
+    try {
+        require(supported_n(p.ngrid), "ngrid must be even, 2^a 3^b, 8..1024");
+        require(p.precision == AXG_FP64 || p.precision == AXG_FP32C || p.precision == 2,
+                "precision must be AXG_FP64 or AXG_FP32C");
+        require(p.dspa > 0 && p.dwav > 0 && p.dv_spa > 0 && p.dv_wav > 0 && p.sigma_cr > 0, "derived units missing");
+        require(p.num_bin_grav >= 1 && p.num_bin_grav <= 4096, "num_bin_grav out of range");
+        require(!p.spec_est || (p.num_bin >= 1 && p.num_bin <= 1024), "num_bin out of range");
+        AXG_HIP(hipSetDevice(p.device));
+        h->N = (int)p.ngrid;
+        const int N = h->N;
+        h->n3 = (int64_t)N * N * N;
+        h->nh = N / 2 + 1;
+        h->P = half_pitch(N);
+        h->prec = p.precision;
+        h->fp32 = p.precision != AXG_FP64;
+        h->comp = p.precision == AXG_FP32C;
+        h->t2 = h->fp32 ? 8 : 16;
+        h->plan = make_plan(N);
+        AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        const int64_t n3 = h->n3;
+        for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
+        h->pi = h->alloc(n3 * h->t2);
+        if (h->comp) {
+            for (int i = 0; i < 2; i++) h->cphi[i] = h->alloc(n3 * 4);
+            h->cpi = h->alloc(n3 * 4);
+        }
+        if (p.grav_spec) {
+            h->src = h->alloc(3 * n3 * h->t2);
+            h->W = h->alloc((size_t)6 * N * N * h->P * h->t2);
+            h->A = h->alloc((size_t)4 * N * N * h->P * h->t2);
+            AXG_HIP(hipMemsetAsync(h->A, 0, (size_t)4 * N * N * h->P * h->t2, h->stream));
+        }
+        // twiddles e^{+2 pi i m / N}
+        {
+            std::vector<double> t64(2 * N);
+            std::vector<float> t32(2 * N);
+            const long double pl = 3.141592653589793238462643383279502884L;
+            for (int m = 0; m < N; m++) {
+                long double a = 2.0L * pl * (long double)m / (long double)N;
+                t64[2 * m] = (double)cosl(a);
+                t64[2 * m + 1] = (double)sinl(a);
+                t32[2 * m] = (float)cosl(a);
+                t32[2 * m + 1] = (float)sinl(a);
+            }
+            h->tw64 = h->alloc(N * 16);
+            AXG_HIP(hipMemcpy(h->tw64, t64.data(), N * 16, hipMemcpyHostToDevice));
+            if (h->fp32) {
+                h->tw = h->alloc(N * 8);
+                AXG_HIP(hipMemcpy(h->tw, t32.data(), N * 8, hipMemcpyHostToDevice));
+            } else {
+                h->tw = h->tw64;
+            }
+        }
+        h->nn2 = 3 * (int64_t)(N / 2) * (N / 2) + 1;
+        {
+            std::vector<uint16_t> tab;
+            bin_table(h->nn2, p.num_bin_grav, p.dspa, p.dwav, tab, nullptr);
+            h->gw_bin = (uint16_t*)h->alloc(h->nn2 * 2);
+            AXG_HIP(hipMemcpy(h->gw_bin, tab.data(), h->nn2 * 2, hipMemcpyHostToDevice));
+            if (p.spec_est) {
+                bin_table(h->nn2, p.num_bin, p.dspa, p.dwav, tab, &h->wav);
+                h->spec_bin = (uint16_t*)h->alloc(h->nn2 * 2);
+                AXG_HIP(hipMemcpy(h->spec_bin, tab.data(), h->nn2 * 2, hipMemcpyHostToDevice));
+            }
+        }
+        h->red_bytes = (size_t)2048 * 2 * (std::max(p.num_bin_grav, std::max(p.num_bin, 4)) + 8) * sizeof(double);
+        h->red = (double*)h->alloc(h->red_bytes);
+        h->work = (double2*)h->alloc(n3 * 16);
+        h->flags = (int8_t*)h->alloc(n3);
+        int64_t nblk = (n3 + 255) / 256;
+        h->blk = (int64_t*)h->alloc((3 * nblk + 8) * sizeof(int64_t) * 2);
+        AXG_HIP(hipStreamSynchronize(h->stream));
+    } catch (const Error& e) {
+        g_create_err = e.msg;
+        h->release();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_create_err = "host allocation failed";
+        h->release();
+        return AXG_ENOMEM;
+    }
+    *out = h.release();
+    return AXG_OK;
+}
+
+extern "C" int axg_destroy(axg_handle* h)
+{
+    if (!h) return AXG_EINVAL;
+    (void)hipSetDevice(h->p.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->release();
+    delete h;
+    return AXG_OK;
+}
+
+extern "C" const char* axg_last_error(const axg_handle* h)
+{
+    if (!h) return g_create_err.c_str();
+    return h->err.c_str();
+}
+
+// ------------------------------------------------------------------ fields
+
+static void store_from_work(axg_handle* h, void* f, void* cf)
+{
+    if (h->comp) launch_store_field<float, true>(h->stream, h->N, h->work, f, cf);
+    else if (h->fp32) launch_store_field<float, false>(h->stream, h->N, h->work, f, cf);
+    else launch_store_field<double, false>(h->stream, h->N, h->work, f, cf);
+}
+
+static void load_to_work(axg_handle* h, const void* f, const void* cf)
+{
+    if (h->comp) launch_load_field<float, true>(h->stream, h->N, f, cf, h->work);
+    else if (h->fp32) launch_load_field<float, false>(h->stream, h->N, f, cf, h->work);
+    else launch_load_field<double, false>(h->stream, h->N, f, cf, h->work);
+}
+
+extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi, int64_t nsite)
+{
+    return guarded(h, [&] {
+        require(nsite == h->n3, "nsite must be ngrid^3");
+        size_t bytes = (size_t)h->n3 * 16;
+        if (phi) {
+            AXG_HIP(hipMemcpyAsync(h->work, phi, bytes, hipMemcpyHostToDevice, h->stream));
+            store_from_work(h, h->phi[h->cur], h->cphi[h->cur]);
+        }
+        if (pi) {
+            AXG_HIP(hipMemcpyAsync(h->work, pi, bytes, hipMemcpyHostToDevice, h->stream));
+            store_from_work(h, h->pi, h->cpi);
+        }
+        h->have_flags = false;
+        AXG_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int axg_get_fields(axg_handle* h, double* phi, double* pi, int64_t nsite)
+{
+    return guarded(h, [&] {
+        require(nsite == h->n3, "nsite must be ngrid^3");
+        size_t bytes = (size_t)h->n3 * 16;
+        if (phi) {
+            load_to_work(h, h->phi[h->cur], h->cphi[h->cur]);
+            AXG_HIP(hipMemcpyAsync(phi, h->work, bytes, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+        if (pi) {
+            load_to_work(h, h->pi, h->cpi);
+            AXG_HIP(hipMemcpyAsync(pi, h->work, bytes, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+// ------------------------------------------------------------------ GW state
+
+extern "C" int axg_get_gw_state(axg_handle* h, double* A, int64_t count)
+{
+    return guarded(h, [&] {
+        const int N = h->N;
+        require(h->A != nullptr, "grav_spec is off: no GW state");
+        require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
+        size_t rows = (size_t)4 * N * N;
+        if (!h->fp32) {
+            AXG_HIP(hipMemcpy2DAsync(A, h->nh * 16, h->A, h->P * 16, h->nh * 16, rows, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        } else {
+            std::vector<float> tmp(rows * h->nh * 2);
+            AXG_HIP(hipMemcpy2DAsync(tmp.data(), h->nh * 8, h->A, h->P * 8, h->nh * 8, rows, hipMemcpyDeviceToHost,
+                                     h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < tmp.size(); i++) A[i] = tmp[i];
+        }
+    });
+}
+
+extern "C" int axg_set_gw_state(axg_handle* h, const double* A, int64_t count)
+{
+    return guarded(h, [&] {
+        const int N = h->N;
+        require(h->A != nullptr, "grav_spec is off: no GW state");
+        require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
+        size_t rows = (size_t)4 * N * N;
+        if (!h->fp32) {
+            AXG_HIP(hipMemcpy2DAsync(h->A, h->P * 16, A, h->nh * 16, h->nh * 16, rows, hipMemcpyHostToDevice, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        } else {
+            std::vector<float> tmp(rows * h->nh * 2);
+            for (size_t i = 0; i < tmp.size(); i++) tmp[i] = (float)A[i];
+            AXG_HIP(hipMemcpy2DAsync(h->A, h->P * 8, tmp.data(), h->nh * 8, h->nh * 8, rows, hipMemcpyHostToDevice,
+                                     h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+// ------------------------------------------------------------------ time stepping
+
+template <typename T, bool COMP>
+static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
+{
+    const int N = h->N;
+    const axg_params& p = h->p;
+    double t = t1;
+    for (int32_t it = 1; it <= nt; it++) {
+        if (it == 1) {
+            // first half drift, in place (each site reads and writes only itself)
+            launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
+                                  h->cphi[h->cur], h->cpi, dt / 2);
+            t = t + dt / 2;
+        }
+        KickScalars k = kick_scalars(p, t, dt);
+        double dtd = (it == nt) ? dt / 2 : dt;
+        int nxt = 1 - h->cur;
+        launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
+                                   h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
+        if (p.grav_spec) {
+            launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw);
+            launch_fft_cols<T, +1>(h->stream, N, h->W, h->P, (int64_t)N * h->P, 6 * N, h->nh, h->plan, h->tw);
+            GwStepScalars g;
+            g.tau = t;
+            g.dw = t * dt * p.dv_spa * (p.sigma_cr * p.sigma_cr);  // grav_waves.f90:209
+            g.dwav = p.dwav;
+            g.fix_swap = p.gw_fix_swap;
+            launch_gw_zpass_acc<T>(h->stream, N, h->P, h->W, h->A, h->plan, h->tw, g);
+        }
+        h->cur = nxt;
+        t = t + dtd;
+    }
+}
+
+static void advance(axg_handle* h, double t1, double dt, int32_t nt)
+{
+    if (h->comp) run_steps<float, true>(h, t1, dt, nt);
+    else if (h->fp32) run_steps<float, false>(h, t1, dt, nt);
+    else run_steps<double, false>(h, t1, dt, nt);
+    h->have_flags = false;
+    AXG_HIP(hipStreamSynchronize(h->stream));
+}
+
+extern "C" int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps)
+{
+    return guarded(h, [&] {
+        require(nsteps >= 1 && dt > 0 && t1 > 0, "need nsteps >= 1, dt > 0, t1 > 0");
+        advance(h, t1, dt, nsteps);
+    });
+}
+
+extern "C" int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_out)
+{
+    return guarded(h, [&] {
+        require(t2 > t1 && t1 > 0, "need t2 > t1 > 0");
+        require(h->p.dtau_fixed > 0 || h->p.eps_dtau > 0, "eps_dtau or dtau_fixed must be positive");
+        double dt;
+        int32_t n;
+        get_dtau(h->p, t1, t2, &dt, &n);
+        advance(h, t1, dt, n);
+        if (nsteps_out) *nsteps_out = n;
+    });
+}
+
+// ------------------------------------------------------------------ GW spectrum
+
+extern "C" int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, double* numer, double* denom, int32_t nbins)
+{
+    return guarded(h, [&] {
+        const axg_params& p = h->p;
+        const int nb = p.num_bin_grav;
+        require(h->A != nullptr, "grav_spec is off: no GW state");
+        require(nbins == nb, "nbins must equal num_bin_grav");
+        const int nwg = 1024;
+        require((size_t)(nwg + 1) * 2 * nb * sizeof(double) <= h->red_bytes, "reduction scratch too small");
+        double* out = h->red + (size_t)nwg * 2 * nb;
+        if (h->fp32) launch_gw_bin<float>(h->stream, h->N, h->P, h->A, h->gw_bin, nb, h->red, nwg, out);
+        else launch_gw_bin<double>(h->stream, h->N, h->P, h->A, h->gw_bin, nb, h->red, nwg, out);
+        std::vector<double> r(2 * nb);
+        AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        // grav_waves.f90:62-71, 265-267, 297-305
+        double qmax = AXG_PI_T / p.dspa, dq0 = qmax / nb;
+        std::vector<double> wave(nb + 1);
+        wave[0] = 0;
+        for (int b = 1; b <= nb; b++) wave[b] = wave[b - 1] + dq0;
+        const double bigh_in_mstar = 2.1332e-42 / 2.435e18;
+        double t6 = p.tau_cr * p.tau_cr * p.tau_cr;
+        t6 = t6 * t6;
+        double a4 = p.a_now * p.a_now;
+        a4 = a4 * a4;
+        double fact = 1 / t6 / (bigh_in_mstar * bigh_in_mstar) / a4 / 3 / (AXG_TWOPI_T * AXG_TWOPI_T) * p.dv_wav;
+        for (int b = 0; b < nb; b++) {
+            double qm = (wave[b + 1] + wave[b]) / 2;
+            double nu = r[b], de = r[nb + b];
+            if (qmid) qmid[b] = qm;
+            if (numer) numer[b] = nu;
+            if (denom) denom[b] = de;
+            if (power) power[b] = de > 0 ? nu / de * (qm * qm * qm) * fact : 0.0;
+        }
+    });
+}
+
+// ------------------------------------------------------------------ energy
+
+extern "C" int axg_energy(axg_handle* h, double tau, double out[4])
+{
+    return guarded(h, [&] {
+        require(out && tau > 0, "need out and tau > 0");
+        const axg_params& p = h->p;
+        double thrm = p.initial_loop ? 0.0 : AXG_THRM_SIGMA / tau;
+        double m2 = msquared(thrm), ma2 = maxion2(thrm, p.kappa);
+        const int nwg = 1024;
+        double* dout = h->red + (size_t)nwg * 4;
+        if (h->comp)
+            launch_energy<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi, m2, ma2, h->red,
+                                       nwg, dout);
+        else if (h->fp32)
+            launch_energy<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, h->red, nwg,
+                                        dout);
+        else
+            launch_energy<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, h->red, nwg,
+                                         dout);
+        double r[3];
+        AXG_HIP(hipMemcpyAsync(r, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        double dv = p.dspa * p.dspa * p.dspa;
+        out[0] = r[0] * dv;
+        out[1] = r[1] / (p.dspa * p.dspa) * dv;
+        out[2] = tau * tau * p.sigma_cr * p.sigma_cr * r[2] * dv;
+        out[3] = out[0] + out[1] + out[2];
+    });
+}
+
+// ------------------------------------------------------------------ defects
+
+extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
+{
+    return guarded(h, [&] {
+        require(out != nullptr, "null output");
+        const axg_params& p = h->p;
+        const int64_t nblk = (h->n3 + 255) / 256;
+        int64_t* counts = h->blk;             // [nblk][3]
+        int64_t* offs = h->blk + 3 * nblk;    // [nblk] exclusive mask offsets
+        int64_t* tot = offs + nblk;           // [4]
+        if (h->comp)
+            launch_classify<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], p.id_strings, p.id_walls,
+                                         p.spec_est, h->flags, counts, nblk);
+        else if (h->fp32)
+            launch_classify<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, p.id_strings, p.id_walls,
+                                          p.spec_est, h->flags, counts, nblk);
+        else
+            launch_classify<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, p.id_strings, p.id_walls,
+                                           p.spec_est, h->flags, counts, nblk);
+        launch_scan_counts(h->stream, counts, offs, nblk, tot);
+        int64_t t[4];
+        AXG_HIP(hipMemcpyAsync(t, tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        out->n_string = t[0];
+        out->n_wall = t[1];
+        out->n_mask = t[2];
+        h->nmask = t[2];
+        h->have_flags = true;
+    });
+}
+
+extern "C" int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* count)
+{
+    return guarded(h, [&] {
+        require(h->have_flags, "call axg_defects first");
+        require(count != nullptr, "null count");
+        *count = h->nmask;
+        if (h->nmask == 0) return;
+        require(ixyz != nullptr && cap >= h->nmask, "mask buffer too small");
+        const int64_t nblk = (h->n3 + 255) / 256;
+        int64_t* offs = h->blk + 3 * nblk;
+        int32_t* dm = (int32_t*)h->work;  // 3*nmask ints fit in the N^3 x 16-byte work array
+        launch_mask_compact(h->stream, h->N, h->flags, offs, dm, nblk);
+        AXG_HIP(hipMemcpyAsync(ixyz, dm, (size_t)3 * h->nmask * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite)
+{
+    return guarded(h, [&] {
+        require(h->have_flags, "call axg_defects first");
+        require(flags && nsite == h->n3, "nsite must be ngrid^3");
+        AXG_HIP(hipMemcpyAsync(flags, h->flags, h->n3, hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+// ------------------------------------------------------------------ field spectrum
+
+static void fft3d_work(axg_handle* h, double2* d)
+{
+    const int N = h->N;
+    launch_fft_rows<double, +1>(h->stream, N, d, (int64_t)N * N, h->plan, h->tw64);
+    launch_fft_cols<double, +1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw64);
+    launch_fft_cols<double, +1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw64);
+}
+
+static void ensure_spec_tables(axg_handle* h)
+{
+    if (h->d_largeq) return;
+    const int nb = h->p.num_bin, nlq = 500;
+    const int ncov = nb * (nb + 1) / 2;
+    // spectrum.f90:74-98
+    std::vector<double> largeq((size_t)ncov * (nlq + 1)), mat((size_t)ncov * (nlq + 1)), dqs(ncov);
+    const std::vector<double>& wav = h->wav;
+    for (int ij = 1; ij <= ncov; ij++) {
+        int i = (int)lround(sqrt(2.0 * ij));
+        int j = ij - i * (i - 1) / 2;
+        double lj = wav[j], sj = wav[j - 1], li = wav[i], si = wav[i - 1];
+        double maxq = li + lj;
+        double minq = std::max(si, sj) - std::min(li, lj);
+        if (minq < 0) minq = 0;
+        double d = maxq - minq;
+        d = d / nlq;
+        dqs[ij - 1] = d;
+        for (int iq = 0; iq <= nlq; iq++) {
+            double q = minq + d * iq;
+            largeq[(size_t)(ij - 1) * (nlq + 1) + iq] = q;
+            mat[(size_t)(ij - 1) * (nlq + 1) + iq] = integrate_p(200, li, si, lj, sj, q);
+        }
+    }
+    h->d_largeq = (double*)h->alloc(largeq.size() * 8);
+    h->d_matspec = (double*)h->alloc(mat.size() * 8);
+    h->d_dqs = (double*)h->alloc(dqs.size() * 8);
+    AXG_HIP(hipMemcpy(h->d_largeq, largeq.data(), largeq.size() * 8, hipMemcpyHostToDevice));
+    AXG_HIP(hipMemcpy(h->d_matspec, mat.data(), mat.size() * 8, hipMemcpyHostToDevice));
+    AXG_HIP(hipMemcpy(h->d_dqs, dqs.data(), dqs.size() * 8, hipMemcpyHostToDevice));
+    // lattice points per n^2 (host, once): r2[m] = #(ny,nz) with ny^2+nz^2 = m
+    const int N = h->N;
+    std::vector<double> r2((size_t)2 * (N / 2) * (N / 2) + 1, 0.0), cnt(h->nn2, 0.0);
+    for (int y = 0; y < N; y++)
+        for (int z = 0; z < N; z++) {
+            int ny = (y < N / 2) ? y : y - N, nz = (z < N / 2) ? z : z - N;
+            r2[(size_t)ny * ny + (size_t)nz * nz] += 1;
+        }
+    for (int x = 0; x < N; x++) {
+        int nx = (x < N / 2) ? x : x - N;
+        for (size_t m = 0; m < r2.size(); m++)
+            if (r2[m] != 0) cnt[(size_t)nx * nx + m] += r2[m];
+    }
+    h->d_cnt = (double*)h->alloc(h->nn2 * 8);
+    AXG_HIP(hipMemcpy(h->d_cnt, cnt.data(), h->nn2 * 8, hipMemcpyHostToDevice));
+    h->d_hist = (double*)h->alloc(h->nn2 * 8);
+    h->d_plane = (double*)h->alloc((size_t)N * ((size_t)(N / 2) * (N / 2) * 2 + 1) * 8);
+}
+
+extern "C" int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, double* conv, double* num,
+                                  double* rho, int32_t nbins)
+{
+    return guarded(h, [&] {
+        const axg_params& p = h->p;
+        const int nb = p.num_bin;
+        require(p.spec_est, "spec_est is off");
+        require(nbins == nb, "nbins must equal num_bin");
+        require(h->have_flags, "call axg_defects first (the mask)");
+        ensure_spec_tables(h);
+        const int ncov = nb * (nb + 1) / 2;
+        const int nwg = 1024;
+        // compute_matrix_deconv (spectrum.f90:358-433): window = dv_spa with mask cells zeroed
+        if (h->comp) launch_masked_field<float, true>(h->stream, h->N, nullptr, nullptr, h->flags, h->work, p.dv_spa, false);
+        else launch_masked_field<double, false>(h->stream, h->N, nullptr, nullptr, h->flags, h->work, p.dv_spa, false);
+        fft3d_work(h, h->work);
+        launch_n2_hist(h->stream, h->N, h->work, h->d_plane, h->d_hist);
+        double* mat0 = h->red;
+        double* tat0 = h->red + ncov;
+        require((size_t)2 * ncov * 8 <= h->red_bytes, "reduction scratch too small");
+        launch_deconv_mat(h->stream, nb, ncov, 500, h->d_largeq, h->d_matspec, h->d_dqs, h->d_hist, h->d_cnt, h->nn2,
+                          p.dwav, mat0, tat0);
+        std::vector<double> m0(ncov), t0(ncov);
+        AXG_HIP(hipMemcpyAsync(m0.data(), mat0, ncov * 8, hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipMemcpyAsync(t0.data(), tat0, ncov * 8, hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        // compute_conv_spec (spectrum.f90:474-563): masked pi -> FFT -> *dv_spa -> binned
+        if (h->comp) launch_masked_field<float, true>(h->stream, h->N, h->pi, h->cpi, h->flags, h->work, 0.0, true);
+        else if (h->fp32) launch_masked_field<float, false>(h->stream, h->N, h->pi, nullptr, h->flags, h->work, 0.0, true);
+        else launch_masked_field<double, false>(h->stream, h->N, h->pi, nullptr, h->flags, h->work, 0.0, true);
+        fft3d_work(h, h->work);
+        double* out = h->red + (size_t)nwg * 2 * nb;
+        require((size_t)(nwg + 1) * 2 * nb * 8 <= h->red_bytes, "reduction scratch too small");
+        launch_spec_bin(h->stream, h->N, h->work, h->spec_bin, nb, p.dv_spa, h->red, nwg, out);
+        std::vector<double> r(2 * nb);
+        AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * 8, hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        // host algebra: spectrum.f90:433-440, 267-275, 316-342
+        std::vector<double> mat(ncov), cs(nb), nm(nb);
+        for (int ij = 0; ij < ncov; ij++) mat[ij] = m0[ij] / t0[ij] * p.dv_wav * p.dv_wav;
+        // packed-upper Cholesky inverse (dpptrf + dpptri)
+        auto AP = [&](int r, int c) -> double& { return mat[(size_t)r + (size_t)c * (c - 1) / 2 - 1]; };
+        for (int j = 1; j <= nb; j++) {
+            for (int i = 1; i < j; i++) {
+                double s = AP(i, j);
+                for (int k = 1; k < i; k++) s -= AP(k, i) * AP(k, j);
+                AP(i, j) = s / AP(i, i);
+            }
+            double s = AP(j, j);
+            for (int k = 1; k < j; k++) s -= AP(k, j) * AP(k, j);
+            if (!(s > 0)) throw Error(AXG_ENUMERIC, "deconvolution matrix is not positive definite");
+            AP(j, j) = sqrt(s);
+        }
+        std::vector<double> col(nb);
+        for (int j = 1; j <= nb; j++) {
+            AP(j, j) = 1 / AP(j, j);
+            double ajj = -AP(j, j);
+            for (int i = 1; i < j; i++) {
+                double s = 0;
+                for (int k = i; k < j; k++) s += AP(i, k) * AP(k, j);
+                col[i - 1] = s * ajj;
+            }
+            for (int i = 1; i < j; i++) AP(i, j) = col[i - 1];
+        }
+        for (int i = 1; i <= nb; i++)
+            for (int j = i; j <= nb; j++) {
+                double s = 0;
+                for (int k = j; k <= nb; k++) s += AP(i, k) * AP(j, k);
+                AP(i, j) = s;
+            }
+        double t2 = tau * tau;
+        for (int i = 0; i < nb; i++) {
+            cs[i] = r[i] * p.dv_wav * t2;
+            nm[i] = r[nb + i];
+        }
+        for (int i = 1; i <= nb; i++)
+            for (int j = 1; j <= i; j++) {
+                int ij = i * (i - 1) / 2 + j;
+                mat[ij - 1] = mat[ij - 1] / nm[i - 1] / nm[j - 1];
+            }
+        std::vector<double> pw(nb, 0.0);
+        for (int i = 1; i <= nb; i++)
+            for (int j = 1; j <= nb; j++) {
+                int a = std::min(i, j), b = std::max(i, j);
+                pw[i - 1] += mat[(size_t)a + (size_t)b * (b - 1) / 2 - 1] * cs[j - 1];
+            }
+        double rh = 0;
+        for (int i = 0; i < nb; i++) {
+            double q = (h->wav[i + 1] + h->wav[i]) / 2;
+            double dq = h->wav[i + 1] - h->wav[i];
+            double q2dq = q * q * dq / AXG_TWOPI_T / AXG_PI_T;
+            rh = rh + pw[i] * q2dq;
+            if (qmid) qmid[i] = q;
+            if (power) power[i] = pw[i];
+            if (conv) conv[i] = cs[i];
+            if (num) num[i] = nm[i];
+        }
+        if (rho) *rho = rh;
+    });
+}
+
+// ------------------------------------------------------------------ initial field
+
+static void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+        uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0, h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+        uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = l1; c[2] = n2; c[3] = l0;
+        k0 += W0; k1 += W1;
+    }
+}
+
+extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
+{
+    return guarded(h, [&] {
+        require(kmax > 0, "kmax must be positive");
+        const int N = h->N;
+        const double dwav = h->p.dwav;
+        std::vector<int64_t> idx;
+        std::vector<double> val;
+        int rmax = (int)std::min<double>(N / 2, ceil(kmax / dwav) + 1);
+        for (int nz = -rmax; nz <= rmax; nz++)
+            for (int ny = -rmax; ny <= rmax; ny++)
+                for (int nx = -rmax; nx <= rmax; nx++) {
+                    if (nx < -N / 2 || nx >= N / 2 || ny < -N / 2 || ny >= N / 2 || nz < -N / 2 || nz >= N / 2) continue;
+                    double q = sqrt((double)((int64_t)nx * nx + (int64_t)ny * ny + (int64_t)nz * nz)) * dwav;
+                    if (!(q < kmax)) continue;
+                    int64_t ix = nx < 0 ? nx + N : nx, iy = ny < 0 ? ny + N : ny, iz = nz < 0 ? nz + N : nz;
+                    uint64_t lin = (uint64_t)ix + (uint64_t)N * ((uint64_t)iy + (uint64_t)N * (uint64_t)iz);
+                    uint32_t c[4] = {(uint32_t)lin, (uint32_t)(lin >> 32), 0x6178696fu, 0x6e677276u};
+                    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+                    uint64_t a = (((uint64_t)c[0] << 21) ^ (uint64_t)(c[1] >> 11)) & ((1ull << 53) - 1);
+                    uint64_t b = (((uint64_t)c[2] << 21) ^ (uint64_t)(c[3] >> 11)) & ((1ull << 53) - 1);
+                    double u1 = ((double)a + 1.0) / 9007199254740992.0;
+                    double u2 = ((double)b + 1.0) / 9007199254740992.0;
+                    double r = sqrt(-2.0 * log(u1));
+                    double th = 6.283185307179586 * u2;
+                    idx.push_back((int64_t)lin);
+                    val.push_back(r * cos(th));
+                    val.push_back(r * sin(th));
+                }
+        require(!idx.empty(), "no mode below kmax");
+        int64_t nm = (int64_t)idx.size();
+        AXG_HIP(hipMemsetAsync(h->work, 0, (size_t)h->n3 * 16, h->stream));
+        int64_t* d_idx = (int64_t*)h->red;
+        double2* d_val = nullptr;
+        std::vector<void*> tmp;
+        void* pidx = nullptr;
+        void* pval = nullptr;
+        AXG_HIP(hipMalloc(&pidx, nm * 8));
+        AXG_HIP(hipMalloc(&pval, nm * 16));
+        try {
+            AXG_HIP(hipMemcpyAsync(pidx, idx.data(), nm * 8, hipMemcpyHostToDevice, h->stream));
+            AXG_HIP(hipMemcpyAsync(pval, val.data(), nm * 16, hipMemcpyHostToDevice, h->stream));
+            d_idx = (int64_t*)pidx;
+            d_val = (double2*)pval;
+            launch_scatter_modes(h->stream, N, d_idx, d_val, nm, h->work);
+            fft3d_work(h, h->work);
+            const int nwg = 1024;
+            double* dsum = h->red + nwg;
+            launch_sumsq(h->stream, h->n3, h->work, h->red, nwg, dsum);
+            double s;
+            AXG_HIP(hipMemcpyAsync(&s, dsum, 8, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            double scale = 1.0 / sqrt(s / (double)h->n3);
+            launch_scale(h->stream, h->n3, h->work, scale);
+            store_from_work(h, h->phi[h->cur], h->cphi[h->cur]);
+            AXG_HIP(hipMemsetAsync(h->work, 0, (size_t)h->n3 * 16, h->stream));
+            store_from_work(h, h->pi, h->cpi);
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        } catch (...) {
+            (void)hipFree(pidx);
+            (void)hipFree(pval);
+            throw;
+        }
+        (void)hipFree(pidx);
+        (void)hipFree(pval);
+        h->have_flags = false;
+    });
+}

@@ -1,0 +1,520 @@
+// Checkpoint diagnostics: defect cell classification and mask compaction
+// (defects_3d.f90:67-103, 152-219, 309-349), the masked field spectrum and its
+// deconvolution reductions (spectrum.f90:344-567, 900-954), the lattice energy, and the
+// field conversions / initial-field helpers.  Every floating-point reduction is
+// deterministic: fixed per-block partial order, then a fixed-order sum over blocks.
+#include "axg_internal.h"
+
+namespace axg {
+
+constexpr int RED_NT = 256;
+
+// Fixed-order tree reduction of one double per thread; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void sum_columns_kernel(const double* __restrict__ partial, int nwg, int width, double* __restrict__ out)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= width) return;
+    double s = 0;
+    for (int w = 0; w < nwg; w++) s += partial[(int64_t)w * width + b];
+    out[b] = s;
+}
+
+static void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out)
+{
+    hipLaunchKernelGGL(sum_columns_kernel, dim3((width + 255) / 256), dim3(256), 0, s, partial, nwg, width, out);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ energy
+
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(RED_NT) energy_kernel(int N, const typename V2<T>::type* __restrict__ phi,
+                                                        const short2* __restrict__ cphi,
+                                                        const typename V2<T>::type* __restrict__ pi,
+                                                        const short2* __restrict__ cpi, double m2, double ma2,
+                                                        double* __restrict__ partial)
+{
+    __shared__ double sh[RED_NT];
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    double ek = 0, eg = 0, ev = 0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_NT + threadIdx.x; i < n3; i += (int64_t)gridDim.x * RED_NT) {
+        int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        double2 c = ldf<T, COMP>(phi, cphi, i);
+        double2 p = ldf<T, COMP>(pi, cpi, i);
+        ek += p.x * p.x + p.y * p.y;
+        double2 nb[3] = {ldf<T, COMP>(phi, cphi, i - x + (x + 1 == N ? 0 : x + 1)),
+                         ldf<T, COMP>(phi, cphi, i + ((y + 1 == N ? 0 : y + 1) - y) * (int64_t)N),
+                         ldf<T, COMP>(phi, cphi, i + ((z + 1 == N ? 0 : z + 1) - z) * NN)};
+        for (int d = 0; d < 3; d++) {
+            double dr = nb[d].x - c.x, di = nb[d].y - c.y;
+            eg += dr * dr + di * di;
+        }
+        double s2 = c.x * c.x + c.y * c.y;
+        double h = s2 + m2 / 2;
+        ev += h * h - ma2 * c.x;
+    }
+    double a = block_sum(ek, sh), b = block_sum(eg, sh), c = block_sum(ev, sh);
+    if (threadIdx.x == 0) {
+        partial[(int64_t)blockIdx.x * 3 + 0] = a;
+        partial[(int64_t)blockIdx.x * 3 + 1] = b;
+        partial[(int64_t)blockIdx.x * 3 + 2] = c;
+    }
+}
+
+template <typename T, bool COMP>
+void launch_energy(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi, double m2,
+                   double ma2, double* partial, int nwg, double* out)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((energy_kernel<T, COMP>), dim3(nwg), dim3(RED_NT), 0, s, N, (const T2*)phi, (const short2*)cphi,
+                       (const T2*)pi, (const short2*)cpi, m2, ma2, partial);
+    AXG_HIP(hipGetLastError());
+    sum_columns(s, partial, nwg, 3, out);
+}
+
+// ------------------------------------------------------------------ defect classification
+// Corner i-1 of the cell at (x,y,z) has offsets (bit0, bit1, bit2) in (x, y, z)
+// (defects_3d.f90:38-40); the sort below yields the same ordered list whatever the corner
+// order, so only the phases matter.
+
+__device__ __forceinline__ int id_with_phase(double* p, int id_strings, int id_walls, int spec_est)
+{
+    // defects_3d.f90:309-349 (bubble sort as there; same ordered result as any sort)
+    for (int i = 7; i >= 1; i--)
+        for (int j = 0; j < i; j++)
+            if (p[j] > p[j + 1]) { double t = p[j]; p[j] = p[j + 1]; p[j + 1] = t; }
+    int flag = 0;
+    double t = p[0] - p[7] + AXG_TWOPI_T;
+    double tmax = p[1] - p[0];
+    for (int i = 2; i < 8; i++) { double d = p[i] - p[i - 1]; tmax = d > tmax ? d : tmax; }
+    if (id_walls && tmax > t) flag += 2;
+    else tmax = tmax > t ? tmax : t;
+    if (id_strings && tmax < AXG_PI_T) flag += 1;
+    if (spec_est && tmax < 7 * AXG_PI_T / 4) flag += 4;
+    return flag;
+}
+
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(256) classify_kernel(int N, const typename V2<T>::type* __restrict__ phi,
+                                                       const short2* __restrict__ cphi, int id_strings, int id_walls,
+                                                       int spec_est, int8_t* __restrict__ flags,
+                                                       int64_t* __restrict__ counts)
+{
+#pragma clang fp contract(off)
+    __shared__ int sc[3];
+    if (threadIdx.x < 3) sc[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i < n3) {
+        int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        int x2 = x + 1 == N ? 0 : x + 1, y2 = y + 1 == N ? 0 : y + 1, z2 = z + 1 == N ? 0 : z + 1;
+        double ph[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            int xx = (c & 1) ? x2 : x, yy = (c & 2) ? y2 : y, zz = (c & 4) ? z2 : z;
+            double2 v = ldf<T, COMP>(phi, cphi, (int64_t)zz * NN + (int64_t)yy * N + xx);
+            ph[c] = atan2(v.y, v.x);  // aimag(log(phi)), defects_3d.f90:92
+        }
+        int fl = id_with_phase(ph, id_strings, id_walls, spec_est);
+        flags[i] = (int8_t)fl;
+        if (fl & 1) atomicAdd(&sc[0], 1);
+        if (fl & 2) atomicAdd(&sc[1], 1);
+        if (fl & 4) atomicAdd(&sc[2], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) counts[blockIdx.x * 3 + threadIdx.x] = sc[threadIdx.x];
+}
+
+template <typename T, bool COMP>
+void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, int id_strings, int id_walls, int spec_est,
+                     int8_t* flags, int64_t* block_counts, int64_t nblk)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((classify_kernel<T, COMP>), dim3((unsigned)nblk), dim3(256), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, id_strings, id_walls, spec_est, flags, block_counts);
+    AXG_HIP(hipGetLastError());
+}
+
+// Exclusive scan of the per-block mask counts (column 2 of in[nblk][3]) and the totals of
+// all three columns, by one block of 1024 threads over contiguous chunks.
+__global__ void __launch_bounds__(1024) scan_counts_kernel(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                                                           int64_t n, int64_t* __restrict__ total)
+{
+    __shared__ int64_t sh[1024];
+    __shared__ int64_t tot[3][1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024;
+    const int64_t b0 = t * chunk, b1 = min(n, b0 + chunk);
+    int64_t s = 0, s0 = 0, s1 = 0;
+    for (int64_t b = b0; b < b1; b++) {
+        s += in[b * 3 + 2];
+        s0 += in[b * 3 + 0];
+        s1 += in[b * 3 + 1];
+    }
+    sh[t] = s;
+    tot[0][t] = s0;
+    tot[1][t] = s1;
+    tot[2][t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int i = 0; i < 1024; i++) { int64_t v = sh[i]; sh[i] = run; run += v; }
+        int64_t a = 0, b = 0, c = 0;
+        for (int i = 0; i < 1024; i++) { a += tot[0][i]; b += tot[1][i]; c += tot[2][i]; }
+        total[0] = a; total[1] = b; total[2] = c; total[3] = 0;
+    }
+    __syncthreads();
+    int64_t run = sh[t];
+    for (int64_t b = b0; b < b1; b++) {
+        out[b] = run;
+        run += in[b * 3 + 2];
+    }
+}
+
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total)
+{
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, in, out, n, total);
+    AXG_HIP(hipGetLastError());
+}
+
+// Mask cells of block b go to offs[b] + (rank among the block's mask cells), so the list is
+// in site order: z outermost, x fastest -- the loop order of defects_3d.f90:167-217.
+__global__ void __launch_bounds__(256) mask_compact_kernel(int N, const int8_t* __restrict__ flags,
+                                                           const int64_t* __restrict__ offs, int32_t* __restrict__ ixyz)
+{
+    __shared__ int sh[256];
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    int m = (i < n3 && (flags[i] & 4)) ? 1 : 0;
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele) over 256 entries
+    for (int d = 1; d < 256; d <<= 1) {
+        int v = (int)threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (m) {
+        int64_t pos = offs[blockIdx.x] + sh[threadIdx.x] - 1;
+        int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        ixyz[3 * pos + 0] = x + 1;
+        ixyz[3 * pos + 1] = y + 1;
+        ixyz[3 * pos + 2] = z + 1;
+    }
+}
+
+void launch_mask_compact(hipStream_t s, int N, const int8_t* flags, const int64_t* block_offsets, int32_t* ixyz,
+                         int64_t nblk)
+{
+    hipLaunchKernelGGL(mask_compact_kernel, dim3((unsigned)nblk), dim3(256), 0, s, N, flags, block_offsets, ixyz);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ field spectrum
+// out = (mask ? 0 : pi) or (mask ? 0 : fill) as FP64 complex (spectrum.f90:358-374, 488-515).
+
+template <typename T, bool COMP>
+__global__ void masked_field_kernel(int64_t n3, const typename V2<T>::type* __restrict__ pi,
+                                    const short2* __restrict__ cpi, const int8_t* __restrict__ flags,
+                                    double2* __restrict__ out, double fill, int use_pi)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 v = use_pi ? ldf<T, COMP>(pi, cpi, i) : double2{fill, 0.0};
+        if (flags[i] & 4) v = double2{0.0, 0.0};
+        out[i] = v;
+    }
+}
+
+template <typename T, bool COMP>
+void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags, double2* out,
+                         double fill_unmasked, bool use_pi)
+{
+    using T2 = typename V2<T>::type;
+    int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL((masked_field_kernel<T, COMP>), dim3(65536), dim3(256), 0, s, n3, (const T2*)pi,
+                       (const short2*)cpi, flags, out, fill_unmasked, use_pi ? 1 : 0);
+    AXG_HIP(hipGetLastError());
+}
+
+// Shell binning over the full spectrum (spectrum.f90:530-556).  Row (ky,kz) is folded onto
+// j = |nx| = 0..N/2 (kx = j and kx = N-j), along which the bin is non-decreasing, then every
+// run of equal bins is summed by the thread at its start (same deterministic scheme as the
+// GW binning).  pow[b] += |d*amp|^2 (times dv_wav on the host), num[b] += 1 per mode.
+__global__ void spec_bin_kernel(int N, const double2* __restrict__ d, const uint16_t* __restrict__ bin_of_n2, int nb,
+                                double amp, double* __restrict__ partial)
+{
+    extern __shared__ double sh[];
+    double* bins = sh;
+    double* v = sh + 2 * nb;
+    double* cnt = v + (N / 2 + 1);
+    int* bb = (int*)(cnt + (N / 2 + 1));
+    const int nh = N / 2 + 1;
+    const int64_t NN = (int64_t)N * N;
+    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    for (int64_t row = blockIdx.x; row < NN; row += gridDim.x) {
+        int kz = (int)(row / N), ky = (int)(row - (int64_t)kz * N);
+        int ny = wrapn(ky, N), nz = wrapn(kz, N);
+        const double2* r = d + row * N;
+        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+            int64_t n2 = (int64_t)j * j + (int64_t)ny * ny + (int64_t)nz * nz;
+            int b = bin_of_n2[n2];
+            double s = 0, c = 0;
+            if (b > 0) {
+                double2 a = r[j];
+                double ax = a.x * amp, ay = a.y * amp;
+                s = ax * ax + ay * ay;
+                c = 1;
+                if (j > 0 && j < N / 2) {
+                    double2 e = r[N - j];
+                    double ex = e.x * amp, ey = e.y * amp;
+                    s += ex * ex + ey * ey;
+                    c = 2;
+                }
+            }
+            v[j] = s;
+            cnt[j] = c;
+            bb[j] = b;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+            int b = bb[j];
+            if (b > 0 && (j == 0 || bb[j - 1] != b)) {
+                double sn = 0, sd = 0;
+                for (int k = j; k < nh && bb[k] == b; k++) { sn += v[k]; sd += cnt[k]; }
+                bins[b - 1] += sn;
+                bins[nb + b - 1] += sd;
+            }
+        }
+        __syncthreads();
+    }
+    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) partial[(int64_t)blockIdx.x * 2 * nb + b] = bins[b];
+}
+
+void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int nb, double scale,
+                     double* partial, int nwg, double* out)
+{
+    int nh = N / 2 + 1;
+    size_t lds = (size_t)(2 * nb + 2 * nh) * sizeof(double) + (size_t)nh * sizeof(int);
+    hipLaunchKernelGGL(spec_bin_kernel, dim3(nwg), dim3(256), lds, s, N, data, bin_of_n2, nb, scale, partial);
+    AXG_HIP(hipGetLastError());
+    sum_columns(s, partial, nwg, 2 * nb, out);
+}
+
+// Histogram of |W|^2 by n^2 = |n|^2, deterministic: block kz walks its plane row by row
+// (fixed order); within one row the folded j = |nx| hit distinct m = j^2 + ny^2, so the
+// per-plane partial plane[kz][m] needs no atomics.  hist[n2] = sum over kz (in order) of
+// plane[kz][n2 - nz^2].
+__global__ void n2_plane_kernel(int N, const double2* __restrict__ d, double* __restrict__ plane)
+{
+    const int nh = N / 2 + 1;
+    const int64_t mm = (int64_t)2 * (N / 2) * (N / 2) + 1;
+    const int kz = blockIdx.x;
+    double* pp = plane + (int64_t)kz * mm;
+    for (int64_t m = threadIdx.x; m < mm; m += blockDim.x) pp[m] = 0;
+    __syncthreads();
+    for (int ky = 0; ky < N; ky++) {
+        int ny = wrapn(ky, N);
+        const double2* r = d + ((int64_t)kz * N + ky) * N;
+        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+            double2 a = r[j];
+            double s = a.x * a.x + a.y * a.y;
+            if (j > 0 && j < N / 2) {
+                double2 e = r[N - j];
+                s += e.x * e.x + e.y * e.y;
+            }
+            pp[(int64_t)j * j + (int64_t)ny * ny] += s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void n2_merge_kernel(int N, const double* __restrict__ plane, double* __restrict__ hist)
+{
+    const int64_t mm = (int64_t)2 * (N / 2) * (N / 2) + 1;
+    const int64_t nn2 = (int64_t)3 * (N / 2) * (N / 2) + 1;
+    int64_t n2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n2 >= nn2) return;
+    double s = 0;
+    for (int kz = 0; kz < N; kz++) {
+        int nz = wrapn(kz, N);
+        int64_t m = n2 - (int64_t)nz * nz;
+        if (m >= 0 && m < mm) s += plane[(int64_t)kz * mm + m];
+    }
+    hist[n2] = s;
+}
+
+void launch_n2_hist(hipStream_t s, int N, const double2* data, double* plane_part, double* hist)
+{
+    hipLaunchKernelGGL(n2_plane_kernel, dim3(N), dim3(256), 0, s, N, data, plane_part);
+    AXG_HIP(hipGetLastError());
+    int64_t nn2 = (int64_t)3 * (N / 2) * (N / 2) + 1;
+    hipLaunchKernelGGL(n2_merge_kernel, dim3((unsigned)((nn2 + 255) / 256)), dim3(256), 0, s, N, plane_part, hist);
+    AXG_HIP(hipGetLastError());
+}
+
+// mat0[ij] = sum_{n2 in pair ij's shell} hist[n2] u_ij(q), tat0[ij] = sum cnt[n2] u_ij(q),
+// q = sqrt(n2)*dwav, u_ij = get_mat_spec (spectrum.f90:385-428, 446-464).
+__global__ void __launch_bounds__(RED_NT) deconv_mat_kernel(int nlq, const double* __restrict__ largeq,
+                                                            const double* __restrict__ mat,
+                                                            const double* __restrict__ dqs,
+                                                            const double* __restrict__ hist,
+                                                            const double* __restrict__ cnt, int64_t nn2, double dwav,
+                                                            double* __restrict__ mat0, double* __restrict__ tat0)
+{
+    __shared__ double sh[RED_NT];
+    const int ij = blockIdx.x;
+    const double* lq = largeq + (int64_t)ij * (nlq + 1);
+    const double* ms = mat + (int64_t)ij * (nlq + 1);
+    double t1 = lq[nlq] / dwav, t0 = lq[0] / dwav;
+    int64_t imax2 = (int64_t)ceil(t1 * t1) - 1;
+    int64_t imin2 = (int64_t)ceil(t0 * t0);
+    if (imax2 > nn2 - 1) imax2 = nn2 - 1;
+    if (imin2 < 0) imin2 = 0;
+    const double dq = dqs[ij], q0 = lq[0];
+    double m = 0, t = 0;
+    for (int64_t n2 = imin2 + threadIdx.x; n2 <= imax2; n2 += RED_NT) {
+        double c = cnt[n2];
+        if (c == 0) continue;
+        double q = sqrt((double)n2) * dwav;
+        double d = (q - q0) / dq;
+        double fl = floor(d);
+        double u = 0;
+        if (fl < nlq && fl >= 0) {
+            int iq = (int)fl;
+            d = d - iq;
+            u = (1 - d) * ms[iq] + d * ms[iq + 1];
+        }
+        m += hist[n2] * u;
+        t += c * u;
+    }
+    double a = block_sum(m, sh), b = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+        mat0[ij] = a;
+        tat0[ij] = b;
+    }
+}
+
+void launch_deconv_mat(hipStream_t s, int nb, int num_cov, int nlq, const double* largeq, const double* mat,
+                       const double* dqs, const double* hist, const double* cnt, int64_t nn2, double dwav, double* mat0,
+                       double* tat0)
+{
+    (void)nb;
+    hipLaunchKernelGGL(deconv_mat_kernel, dim3(num_cov), dim3(RED_NT), 0, s, nlq, largeq, mat, dqs, hist, cnt, nn2, dwav,
+                       mat0, tat0);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ conversions / IC helpers
+
+template <typename T, bool COMP>
+__global__ void store_field_kernel(int64_t n3, const double2* __restrict__ in, typename V2<T>::type* __restrict__ f,
+                                   short2* __restrict__ cf)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
+        stf<T, COMP>(f, cf, i, in[i]);
+}
+
+template <typename T, bool COMP>
+__global__ void load_field_kernel(int64_t n3, const typename V2<T>::type* __restrict__ f, const short2* __restrict__ cf,
+                                  double2* __restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = ldf<T, COMP>(f, cf, i);
+}
+
+template <typename T, bool COMP>
+void launch_store_field(hipStream_t s, int N, const double2* in, void* f, void* cf)
+{
+    using T2 = typename V2<T>::type;
+    int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL((store_field_kernel<T, COMP>), dim3(65536), dim3(256), 0, s, n3, in, (T2*)f, (short2*)cf);
+    AXG_HIP(hipGetLastError());
+}
+
+template <typename T, bool COMP>
+void launch_load_field(hipStream_t s, int N, const void* f, const void* cf, double2* out)
+{
+    using T2 = typename V2<T>::type;
+    int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL((load_field_kernel<T, COMP>), dim3(65536), dim3(256), 0, s, n3, (const T2*)f,
+                       (const short2*)cf, out);
+    AXG_HIP(hipGetLastError());
+}
+
+__global__ void scatter_modes_kernel(const int64_t* __restrict__ idx, const double2* __restrict__ val, int64_t nm,
+                                     double2* __restrict__ out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < nm) out[idx[i]] = val[i];
+}
+
+void launch_scatter_modes(hipStream_t s, int N, const int64_t* idx, const double2* val, int64_t nmodes, double2* out)
+{
+    (void)N;
+    hipLaunchKernelGGL(scatter_modes_kernel, dim3((unsigned)((nmodes + 255) / 256)), dim3(256), 0, s, idx, val, nmodes,
+                       out);
+    AXG_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(RED_NT) sumsq_kernel(int64_t n, const double2* __restrict__ d,
+                                                       double* __restrict__ partial)
+{
+    __shared__ double sh[RED_NT];
+    double s = 0;
+    for (int64_t i = blockIdx.x * (int64_t)RED_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_NT)
+        s += d[i].x * d[i].x + d[i].y * d[i].y;
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+void launch_sumsq(hipStream_t s, int64_t n, const double2* data, double* partial, int nwg, double* out)
+{
+    hipLaunchKernelGGL(sumsq_kernel, dim3(nwg), dim3(RED_NT), 0, s, n, data, partial);
+    AXG_HIP(hipGetLastError());
+    sum_columns(s, partial, nwg, 1, out);
+}
+
+__global__ void scale_kernel(int64_t n, double2* __restrict__ d, double sc)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        d[i].x = d[i].x * sc;
+        d[i].y = d[i].y * sc;
+    }
+}
+
+void launch_scale(hipStream_t s, int64_t n, double2* data, double scale)
+{
+    hipLaunchKernelGGL(scale_kernel, dim3(65536), dim3(256), 0, s, n, data, scale);
+    AXG_HIP(hipGetLastError());
+}
+
+#define AXG_INST_DIAG(T, C)                                                                                      \
+    template void launch_energy<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*, double, \
+                                      double, double*, int, double*);                                            \
+    template void launch_classify<T, C>(hipStream_t, int, const void*, const void*, int, int, int, int8_t*,       \
+                                        int64_t*, int64_t);                                                      \
+    template void launch_masked_field<T, C>(hipStream_t, int, const void*, const void*, const int8_t*, double2*,  \
+                                            double, bool);                                                       \
+    template void launch_store_field<T, C>(hipStream_t, int, const double2*, void*, void*);                       \
+    template void launch_load_field<T, C>(hipStream_t, int, const void*, const void*, double2*);
+AXG_INST_DIAG(double, false)
+AXG_INST_DIAG(float, false)
+AXG_INST_DIAG(float, true)
+
+}  // namespace axg

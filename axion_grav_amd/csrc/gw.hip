@@ -1,0 +1,397 @@
+// GW sourcing kernels: the 3D FFT of the six source products (grav_waves.f90:225, an
+// unnormalized BACKWARD transform) as three passes over HBM, and the helicity projection +
+// Green's-function accumulation of grav_waves.f90:227-247 fused into the last pass, plus
+// the shell binning of grav_spec_estimation (grav_waves.f90:271-295).
+//
+// Layout.  The sources are real, so the transform is kept on the half spectrum
+// kx = 0..N/2.  W[c][z][y][kx] (c = 0..5 in the reference's (i,j) order (1,1),(2,1),(2,2),
+// (3,1),(3,2),(3,3); row pitch P = half_pitch(N)) holds the x-transformed sources, is
+// y-transformed in place, and is z-transformed on the fly by the accumulation pass.  The
+// accumulators A[m][kz][ky][kx] (m = arr_grav (1,1),(2,1),(1,2),(2,2)) use the same pitch.
+#include "axg_internal.h"
+
+namespace axg {
+
+constexpr int FFT_NT = 256;
+// complex elements per block held in LDS (64 KiB): 4096 FP64, 8192 FP32
+template <typename T> constexpr int fft_elems() { return 65536 / (int)sizeof(typename V2<T>::type); }
+// Column-tile width: 128 bytes of complex per row, fewer when N columns would not fit.
+template <typename T> int col_width(int N)
+{
+    int tx = col_tile<T>();
+    while (tx > 1 && N * tx > fft_elems<T>()) tx /= 2;
+    return N * tx <= fft_elems<T>() ? tx : 0;
+}
+
+// ------------------------------------------------------------------ x pass (R2C via pairing)
+// Block = TB rows of one packed component m at one z.  Row FFT of C_m = S_a + i S_b, then
+//   S_a^(k) = (X(k) + conj(X(N-k)))/2,  S_b^(k) = (X(k) - conj(X(N-k)))/(2i),  k = 0..N/2,
+// written to W[2m] and W[2m+1].
+
+template <typename T>
+__global__ void __launch_bounds__(FFT_NT) gw_xpass_kernel(int N, int P, int TB, const typename V2<T>::type* __restrict__ src,
+                                                         typename V2<T>::type* __restrict__ W, FftPlan plan,
+                                                         const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int m = blockIdx.z, z = blockIdx.y, y0 = blockIdx.x * TB;
+    const int nrow = min(TB, N - y0);
+    const int64_t NN = (int64_t)N * N;
+    const T2* in = src + (int64_t)m * NN * N + (int64_t)z * NN + (int64_t)y0 * N;
+    for (int e = threadIdx.x; e < nrow * N; e += FFT_NT) s[e] = in[e];
+    for (int e = nrow * N + threadIdx.x; e < TB * N; e += FFT_NT) s[e] = T2{0, 0};
+    __syncthreads();
+    fft_lds<T, +1, FFT_NT, fft_elems<T>(), false>(s, plan, TB, 1, N, tw);
+    const int nh = N / 2 + 1;
+    T2* wa = W + ((int64_t)(2 * m) * NN + (int64_t)z * N + y0) * P;
+    T2* wb = W + ((int64_t)(2 * m + 1) * NN + (int64_t)z * N + y0) * P;
+    for (int e = threadIdx.x; e < nrow * nh; e += FFT_NT) {
+        int r = e / nh, k = e - r * nh;
+        T2 xk = s[r * N + k];
+        T2 xm = s[r * N + (k == 0 ? 0 : N - k)];
+        T2 a = {(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+        // (xk - conj(xm)) / (2i) = (dr + i di) * (-i/2) = (di/2, -dr/2)
+        T2 b = {(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        wa[(int64_t)r * P + k] = a;
+        wb[(int64_t)r * P + k] = b;
+    }
+}
+
+template <typename T>
+void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan, const void* tw)
+{
+    using T2 = typename V2<T>::type;
+    int TB = fft_elems<T>() / N;
+    if (TB > N) TB = N;
+    dim3 grid((N + TB - 1) / TB, N, 3);
+    size_t lds = (size_t)TB * N * sizeof(T2);
+    hipLaunchKernelGGL((gw_xpass_kernel<T>), grid, dim3(FFT_NT), lds, s, N, pitch, TB, (const T2*)src, (T2*)W, plan,
+                       (const T2*)tw);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ strided (column) FFT
+// In place along an axis with element stride es; plane p starts at p*ps; columns are the
+// contiguous index 0..ncol-1, handled TX at a time (TX * sizeof(complex) = 128 bytes).
+
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(FFT_NT) fft_cols_kernel(int N, int TX, typename V2<T>::type* __restrict__ data, int64_t es,
+                                                         int64_t ps, int ncol, FftPlan plan,
+                                                         const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int c0 = blockIdx.x * TX;
+    T2* base = data + (int64_t)blockIdx.y * ps + c0;
+    for (int e = threadIdx.x; e < N * TX; e += FFT_NT) {
+        int n = e / TX, b = e - n * TX;
+        s[e] = (c0 + b < ncol) ? base[(int64_t)n * es + b] : T2{0, 0};
+    }
+    __syncthreads();
+    fft_lds<T, SIGN, FFT_NT, fft_elems<T>(), true>(s, plan, TX, TX, 1, tw);
+    for (int e = threadIdx.x; e < N * TX; e += FFT_NT) {
+        int n = e / TX, b = e - n * TX;
+        if (c0 + b < ncol) base[(int64_t)n * es + b] = s[e];
+    }
+}
+
+template <typename T, int SIGN>
+void launch_fft_cols(hipStream_t s, int N, void* data, int64_t es, int64_t ps, int nplanes, int ncol,
+                     const FftPlan& plan, const void* tw)
+{
+    using T2 = typename V2<T>::type;
+    int TX = col_width<T>(N);
+    if (TX < 1) throw Error(AXG_EINVAL, "column FFT: N too large for the LDS tile");
+    dim3 grid((ncol + TX - 1) / TX, nplanes);
+    size_t lds = (size_t)N * TX * sizeof(T2);
+    hipLaunchKernelGGL((fft_cols_kernel<T, SIGN>), grid, dim3(FFT_NT), lds, s, N, TX, (T2*)data, es, ps, ncol, plan,
+                       (const T2*)tw);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ contiguous (row) FFT
+
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(FFT_NT) fft_rows_kernel(int N, int TB, int64_t nrows, typename V2<T>::type* __restrict__ data,
+                                                         FftPlan plan, const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int64_t r0 = (int64_t)blockIdx.x * TB;
+    const int nrow = (int)min((int64_t)TB, nrows - r0);
+    T2* base = data + r0 * N;
+    for (int e = threadIdx.x; e < nrow * N; e += FFT_NT) s[e] = base[e];
+    for (int e = nrow * N + threadIdx.x; e < TB * N; e += FFT_NT) s[e] = T2{0, 0};
+    __syncthreads();
+    fft_lds<T, SIGN, FFT_NT, fft_elems<T>(), false>(s, plan, TB, 1, N, tw);
+    for (int e = threadIdx.x; e < nrow * N; e += FFT_NT) base[e] = s[e];
+}
+
+template <typename T, int SIGN>
+void launch_fft_rows(hipStream_t s, int N, void* data, int64_t nrows, const FftPlan& plan, const void* tw)
+{
+    using T2 = typename V2<T>::type;
+    int TB = fft_elems<T>() / N;
+    if (TB < 1) throw Error(AXG_EINVAL, "row FFT: N too large for the LDS tile");
+    int64_t nb = (nrows + TB - 1) / TB;
+    size_t lds = (size_t)TB * N * sizeof(T2);
+    hipLaunchKernelGGL((fft_rows_kernel<T, SIGN>), dim3((unsigned)nb), dim3(FFT_NT), lds, s, N, TB, nrows, (T2*)data,
+                       plan, (const T2*)tw);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ z pass + accumulation
+//
+// Block = one ky and TX consecutive kx.  For each source component c the z column tile is
+// transformed in LDS; every thread then owns k-points (kz, kx) and accumulates
+// SP = sum_c pol_c(k) D_c(k) and SN = sum_c conj(pol_c(k)) D_c(k).  With c = cos(q tau),
+// s = sin(q tau) and the step weight dw, the six sequential updates of
+// grav_waves.f90:239-242 (P = pol*dw, N = conj(P)) collapse to
+//   A11 += c dw SP;   A22 += s dw SN;   A12 += s dw SP + c dw SN;   A21 = A12 - c N6 D6,
+// where A12 is the updated value, A21's old value is discarded as in the reference, and
+// N6 = pol_33 dw = (r^2/2) dw is real; D6 (the (3,3) component, transformed last) is still in
+// LDS when the update runs.  With fix_swap: A21 += s dw SP, A12 += c dw SN.
+
+constexpr int ZNT = 512;  // threads of the accumulation pass
+
+template <typename T>
+__device__ __forceinline__ void pol_vec(int nx, int ny, int nz, typename V2<T>::type* p)
+{
+    // grav_waves.f90:165-183 (n != 0)
+    T qx = (T)nx, qy = (T)ny, qz = (T)nz;
+    T q = sqrt(qx * qx + qy * qy + qz * qz);
+    qx = qx / q; qy = qy / q; qz = qz / q;
+    T r = sqrt(qx * qx + qy * qy);
+    T c, s;
+    if (r > 0) { c = qx / r; s = qy / r; }
+    else { c = 1; s = 0; }
+    p[0] = {qz * c, -s};
+    p[1] = {qz * s, c};
+    p[2] = {-r, 0};
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ZNT) gw_zpass_acc_kernel(int N, int P, const typename V2<T>::type* __restrict__ W,
+                                                          typename V2<T>::type* __restrict__ A, FftPlan plan,
+                                                          const typename V2<T>::type* __restrict__ tw,
+                                                          GwStepScalars g, int TX)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    constexpr int KPT = fft_elems<T>() / ZNT;  // k-points per thread (upper bound)
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int nh = N / 2 + 1;
+    const int c0 = blockIdx.x * TX, ky = blockIdx.y;
+    const int64_t NN = (int64_t)N * N;
+    const int64_t es = (int64_t)N * P;  // z stride in W and A
+    const int nk = N * TX;
+    const int ny = wrapn(ky, N);
+
+    T2 sp[KPT], sn[KPT];
+#pragma unroll
+    for (int i = 0; i < KPT; i++) sp[i] = sn[i] = T2{0, 0};
+
+    for (int c = 0; c < 6; c++) {
+        // (i,j) of component c in the order (1,1),(2,1),(2,2),(3,1),(3,2),(3,3), 0-based
+        const int ii = c == 0 ? 0 : (c <= 2 ? 1 : 2);
+        const int jj = (c == 0 || c == 1 || c == 3) ? 0 : ((c == 2 || c == 4) ? 1 : 2);
+        if (c > 0) __syncthreads();
+        const T2* base = W + (int64_t)c * NN * P + (int64_t)ky * P + c0;
+        for (int e = threadIdx.x; e < nk; e += ZNT) {
+            int n = e / TX, b = e - n * TX;
+            s[e] = (c0 + b < nh) ? base[(int64_t)n * es + b] : T2{0, 0};
+        }
+        __syncthreads();
+        fft_lds<T, +1, ZNT, fft_elems<T>(), true>(s, plan, TX, TX, 1, tw);
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+            int e = threadIdx.x + i * ZNT;
+            if (e < nk && c0 + (e % TX) < nh) {
+                int kz = e / TX, kx = c0 + (e - kz * TX);
+                int nx = wrapn(kx, N), nz = wrapn(kz, N);
+                T2 pol = {0, 0};
+                if (nx != 0 || ny != 0 || nz != 0) {
+                    T2 pv[3];
+                    pol_vec<T>(nx, ny, nz, pv);
+                    T2 a = ii == 0 ? pv[0] : (ii == 1 ? pv[1] : pv[2]);
+                    T2 b = jj == 0 ? pv[0] : (jj == 1 ? pv[1] : pv[2]);
+                    pol = {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+                    if (ii == jj) { pol.x = pol.x / 2; pol.y = pol.y / 2; }
+                }
+                T2 d = s[e];
+                sp[i].x += pol.x * d.x - pol.y * d.y;
+                sp[i].y += pol.x * d.y + pol.y * d.x;
+                sn[i].x += pol.x * d.x + pol.y * d.y;
+                sn[i].y += pol.x * d.y - pol.y * d.x;
+            }
+        }
+    }
+    // s[] still holds D6 = the transformed (3,3) source
+
+#pragma unroll
+    for (int i = 0; i < KPT; i++) {
+        int e = threadIdx.x + i * ZNT;
+        if (e < nk && c0 + (e % TX) < nh) {
+            int kz = e / TX, kx = c0 + (e - kz * TX);
+            int nx = wrapn(kx, N), nz = wrapn(kz, N);
+            int64_t rho2 = (int64_t)nx * nx + (int64_t)ny * ny;
+            int64_t n2 = rho2 + (int64_t)nz * nz;
+            double qn = sqrt((double)n2);
+            double q = qn * g.dwav * g.tau;
+            double cq, sq;
+            if constexpr (sizeof(T) == 8) {
+                cq = cos(q);
+                sq = sin(q);
+            } else {
+                // FP32 storage: reduce the phase in FP64, evaluate in FP32
+                const double twopi = 6.283185307179586476925286766559;
+                float r = (float)(q - twopi * rint(q * (1.0 / twopi)));
+                float cf, sf;
+                __sincosf(r, &sf, &cf);
+                cq = cf;
+                sq = sf;
+            }
+            const double dw = g.dw;
+            double2 SP = {(double)sp[i].x * dw, (double)sp[i].y * dw};
+            double2 SN = {(double)sn[i].x * dw, (double)sn[i].y * dw};
+            int64_t ai = (int64_t)kz * es + (int64_t)ky * P + kx;
+            const int64_t ms = NN * P;
+            T2 a11 = A[ai], a21 = A[ms + ai], a12 = A[2 * ms + ai], a22 = A[3 * ms + ai];
+            double2 r11 = {(double)a11.x + cq * SP.x, (double)a11.y + cq * SP.y};
+            double2 r22 = {(double)a22.x + sq * SN.x, (double)a22.y + sq * SN.y};
+            double2 r21, r12;
+            if (!g.fix_swap) {
+                r12 = {(double)a12.x + (sq * SP.x + cq * SN.x), (double)a12.y + (sq * SP.y + cq * SN.y)};
+                double n6 = n2 > 0 ? ((double)rho2 / (double)n2) / 2 * dw : 0.0;
+                T2 d6 = s[e];
+                r21 = {r12.x - cq * (n6 * (double)d6.x), r12.y - cq * (n6 * (double)d6.y)};
+            } else {
+                r21 = {(double)a21.x + sq * SP.x, (double)a21.y + sq * SP.y};
+                r12 = {(double)a12.x + cq * SN.x, (double)a12.y + cq * SN.y};
+            }
+            A[ai] = T2{(T)r11.x, (T)r11.y};
+            A[ms + ai] = T2{(T)r21.x, (T)r21.y};
+            A[2 * ms + ai] = T2{(T)r12.x, (T)r12.y};
+            A[3 * ms + ai] = T2{(T)r22.x, (T)r22.y};
+        }
+    }
+}
+
+template <typename T>
+void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const FftPlan& plan, const void* tw,
+                         const GwStepScalars& g)
+{
+    using T2 = typename V2<T>::type;
+    int TX = col_width<T>(N);
+    if (TX < 1) throw Error(AXG_EINVAL, "z pass: N too large for the LDS tile");
+    int nh = N / 2 + 1;
+    dim3 grid((nh + TX - 1) / TX, N);
+    size_t lds = (size_t)N * TX * sizeof(T2);
+    hipLaunchKernelGGL((gw_zpass_acc_kernel<T>), grid, dim3(ZNT), lds, s, N, pitch, (const T2*)W, (T2*)A, plan,
+                       (const T2*)tw, g, TX);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ GW spectrum binning
+// bin_of_n2[n2] = reference bin (1..nb) of a mode with |n|^2 = n2, 0 = outside; bins are
+// non-decreasing in n2, so along a half-spectrum row (kx = 0..N/2, |nx| increasing) every
+// bin is one contiguous run.  Each block walks its rows in a fixed order; the thread at the
+// start of a run sums the run sequentially and adds it to the block's LDS bin (runs of one
+// row have distinct bins, so no two threads touch the same bin).  Block partials are then
+// summed in block order: the reduction is deterministic.  Weight 2 for 0 < kx < N/2 (the
+// mirrored half), 1 on the kx = 0 and kx = N/2 planes; k = 0 is excluded (q > 0).
+
+template <typename T>
+__global__ void gw_bin_kernel(int N, int P, const typename V2<T>::type* __restrict__ A,
+                              const uint16_t* __restrict__ bin_of_n2, int nb, double* __restrict__ partial)
+{
+    using T2 = typename V2<T>::type;
+    extern __shared__ double sh[];
+    double* bins = sh;              // 2*nb: numer, denom
+    double* v = sh + 2 * nb;        // N/2+1 values
+    int* bb = (int*)(v + (N / 2 + 1));
+    const int nh = N / 2 + 1;
+    const int64_t NN = (int64_t)N * N, ms = NN * P;
+    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    for (int64_t row = blockIdx.x; row < NN; row += gridDim.x) {
+        int kz = (int)(row / N), ky = (int)(row - (int64_t)kz * N);
+        int ny = wrapn(ky, N), nz = wrapn(kz, N);
+        for (int kx = threadIdx.x; kx < nh; kx += blockDim.x) {
+            int nx = wrapn(kx, N);
+            int64_t n2 = (int64_t)nx * nx + (int64_t)ny * ny + (int64_t)nz * nz;
+            int b = (n2 > 0) ? bin_of_n2[n2] : 0;
+            double s = 0;
+            if (b > 0) {
+                int64_t ai = (int64_t)kz * N * P + (int64_t)ky * P + kx;
+                for (int m = 0; m < 4; m++) {
+                    T2 a = A[m * ms + ai];
+                    s += (double)a.x * a.x + (double)a.y * a.y;
+                }
+            }
+            v[kx] = s;
+            bb[kx] = b;
+        }
+        __syncthreads();
+        for (int kx = threadIdx.x; kx < nh; kx += blockDim.x) {
+            int b = bb[kx];
+            if (b > 0 && (kx == 0 || bb[kx - 1] != b)) {
+                double sn = 0, sd = 0;
+                for (int j = kx; j < nh && bb[j] == b; j++) {
+                    double w = (j == 0 || j == N / 2) ? 1.0 : 2.0;
+                    sn += w * v[j];
+                    sd += w;
+                }
+                bins[b - 1] += sn;
+                bins[nb + b - 1] += sd;
+            }
+        }
+        __syncthreads();
+    }
+    for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) partial[(int64_t)blockIdx.x * 2 * nb + b] = bins[b];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ partial, int nwg, int width, double* __restrict__ out)
+{
+    // one thread per output column; fixed block order
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= width) return;
+    double s = 0;
+    for (int w = 0; w < nwg; w++) s += partial[(int64_t)w * width + b];
+    out[b] = s;
+}
+
+template <typename T>
+void launch_gw_bin(hipStream_t s, int N, int pitch, const void* A, const uint16_t* bin_of_n2, int nb, double* partial,
+                   int nwg, double* out)
+{
+    using T2 = typename V2<T>::type;
+    int nh = N / 2 + 1;
+    size_t lds = (size_t)(2 * nb + nh) * sizeof(double) + (size_t)nh * sizeof(int);
+    hipLaunchKernelGGL((gw_bin_kernel<T>), dim3(nwg), dim3(256), lds, s, N, pitch, (const T2*)A, bin_of_n2, nb,
+                       partial);
+    AXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((2 * nb + 255) / 256), dim3(256), 0, s, partial, nwg, 2 * nb, out);
+    AXG_HIP(hipGetLastError());
+}
+
+#define AXG_INST_GW(T)                                                                                          \
+    template void launch_gw_xpass<T>(hipStream_t, int, int, const void*, void*, const FftPlan&, const void*);   \
+    template void launch_fft_cols<T, +1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,   \
+                                         const void*);                                                          \
+    template void launch_fft_rows<T, +1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);        \
+    template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const FftPlan&, const void*, \
+                                         const GwStepScalars&);                                                 \
+    template void launch_gw_bin<T>(hipStream_t, int, int, const void*, const uint16_t*, int, double*, int, double*);
+AXG_INST_GW(double)
+AXG_INST_GW(float)
+template void launch_fft_cols<double, -1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,
+                                          const void*);
+template void launch_fft_rows<double, -1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);
+
+}  // namespace axg

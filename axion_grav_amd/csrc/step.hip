@@ -1,0 +1,137 @@
+// Lattice integrator kernels: the drift (evolve.f90:259-281) and the fused
+// kick + GW source + drift pass (evolve.f90:283-335 with potential.f90:43-60, the
+// gradient of grav_waves.f90:323-356 and the source products of grav_waves.f90:213-223).
+//
+// FP64 kernels keep the reference's operation order with contraction off, so one FP64 step
+// is bit-identical to the reference arithmetic.  The FP32 kernels store phi and pi as FP32
+// and (with COMP) an int16 residual per real that carries the bits below the FP32 value;
+// all arithmetic runs in FP64 registers.
+#include "axg_internal.h"
+
+namespace axg {
+
+// ------------------------------------------------------------------ drift
+// phi_out may equal phi_in (every thread reads and then writes only its own site).
+
+template <typename T, bool COMP>
+__global__ void drift_kernel(int64_t n3, const typename V2<T>::type* phi_in, typename V2<T>::type* phi_out,
+                             const typename V2<T>::type* __restrict__ pi, const short2* cphi_in, short2* cphi_out,
+                             const short2* __restrict__ cpi, double dt)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 f = ldf<T, COMP>(phi_in, cphi_in, i);
+        double2 p = ldf<T, COMP>(pi, cpi, i);
+        double2 r = {f.x + dt * p.x, f.y + dt * p.y};
+        stf<T, COMP>(phi_out, cphi_out, i, r);
+    }
+}
+
+template <typename T, bool COMP>
+void launch_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, const void* pi, const void* cphi_in,
+                  void* cphi_out, const void* cpi, double dt)
+{
+    using T2 = typename V2<T>::type;
+    int64_t n3 = (int64_t)N * N * N;
+    int64_t blocks = (n3 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((drift_kernel<T, COMP>), dim3((unsigned)blocks), dim3(256), 0, s, n3, (const T2*)phi_in,
+                       (T2*)phi_out, (const T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (const short2*)cpi,
+                       dt);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ kick + source + drift
+//
+// One thread per site.  Reads phi at the site and its 6 periodic neighbours and pi at the
+// site; writes pi (kicked), phi_out = phi + dt_drift * pi (the next drift, into the other
+// buffer so neighbours still see the old phi) and, when gw, the 6 real source products
+// S_ij = 2 Re(d_i phi conj(d_j phi)) packed as 3 complex planes
+//   C0 = S11 + i S21,  C1 = S22 + i S31,  C2 = S32 + i S33.
+
+template <typename T, bool COMP>
+__global__ void kick_drift_kernel(int N, const typename V2<T>::type* __restrict__ phi_in,
+                                  typename V2<T>::type* __restrict__ phi_out, typename V2<T>::type* __restrict__ pi,
+                                  const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
+                                  short2* __restrict__ cpi, typename V2<T>::type* __restrict__ src, KickScalars k,
+                                  double d, double dt_drift, int gw)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= N) return;
+    int y = blockIdx.y, z = blockIdx.z;
+    int xm = x == 0 ? N - 1 : x - 1, xp = x == N - 1 ? 0 : x + 1;
+    int ym = y == 0 ? N - 1 : y - 1, yp = y == N - 1 ? 0 : y + 1;
+    int zm = z == 0 ? N - 1 : z - 1, zp = z == N - 1 ? 0 : z + 1;
+    const int64_t NN = (int64_t)N * N;
+    const int64_t i = (int64_t)z * NN + (int64_t)y * N + x;
+    double2 c = ldf<T, COMP>(phi_in, cphi_in, i);
+    double2 a1 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)y * N + xm);
+    double2 a2 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)y * N + xp);
+    double2 b1 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)ym * N + x);
+    double2 b2 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)yp * N + x);
+    double2 c1 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x);
+    double2 c2 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zp * NN + (int64_t)y * N + x);
+    double2 p = ldf<T, COMP>(pi, cpi, i);
+
+    // 7-point Laplacian in the reference's summation order
+    double lr = a1.x + a2.x + b1.x + b2.x + c1.x + c2.x - 6 * c.x;
+    double li = a1.y + a2.y + b1.y + b2.y + c1.y + c2.y - 6 * c.y;
+    // dVdphi: (2|phi|^2 + m2) phi - ma2/2  [- kt conj(phi) for the N_DW=2 tilt]
+    double s2 = c.x * c.x + c.y * c.y;
+    double g = 2 * s2 + k.m2;
+    double dr = g * c.x - k.ma2 / 2;
+    double di = g * c.y;
+    if (k.kt != 0.0) {
+        dr = dr - k.kt * c.x;
+        di = di + k.kt * c.y;
+    }
+    p.x = k.f1 * p.x + k.f2 * lr + k.f3 * dr;
+    p.y = k.f1 * p.y + k.f2 * li + k.f3 * di;
+    stf<T, COMP>(pi, cpi, i, p);
+
+    if (gw) {
+        double2 gx = {(a2.x - a1.x) / d, (a2.y - a1.y) / d};
+        double2 gy = {(b2.x - b1.x) / d, (b2.y - b1.y) / d};
+        double2 gz = {(c2.x - c1.x) / d, (c2.y - c1.y) / d};
+        double s11 = 2 * (gx.x * gx.x + gx.y * gx.y);
+        double s21 = 2 * (gy.x * gx.x + gy.y * gx.y);
+        double s22 = 2 * (gy.x * gy.x + gy.y * gy.y);
+        double s31 = 2 * (gz.x * gx.x + gz.y * gx.y);
+        double s32 = 2 * (gz.x * gy.x + gz.y * gy.y);
+        double s33 = 2 * (gz.x * gz.x + gz.y * gz.y);
+        const int64_t n3 = NN * N;
+        src[i] = T2{(T)s11, (T)s21};
+        src[n3 + i] = T2{(T)s22, (T)s31};
+        src[2 * n3 + i] = T2{(T)s32, (T)s33};
+    }
+
+    double2 f = {c.x + dt_drift * p.x, c.y + dt_drift * p.y};
+    stf<T, COMP>(phi_out, cphi_out, i, f);
+}
+
+template <typename T, bool COMP>
+void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
+                       void* cphi_out, void* cpi, void* src, const KickScalars& k, double dspa, double dt_drift,
+                       bool gw)
+{
+    using T2 = typename V2<T>::type;
+    int bx = N < 256 ? N : 256;
+    dim3 grid((N + bx - 1) / bx, N, N);
+    hipLaunchKernelGGL((kick_drift_kernel<T, COMP>), grid, dim3(bx), 0, s, N, (const T2*)phi_in, (T2*)phi_out,
+                       (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)src, k, dspa * 2,
+                       dt_drift, gw ? 1 : 0);
+    AXG_HIP(hipGetLastError());
+}
+
+#define AXG_INST_STEP(T, C)                                                                                  \
+    template void launch_drift<T, C>(hipStream_t, int, const void*, void*, const void*, const void*, void*,   \
+                                     const void*, double);                                                   \
+    template void launch_kick_drift<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*,    \
+                                          void*, void*, const KickScalars&, double, double, bool);
+AXG_INST_STEP(double, false)
+AXG_INST_STEP(float, false)
+AXG_INST_STEP(float, true)
+
+}  // namespace axg

@@ -1,0 +1,146 @@
+/*
+ * axgrav.h -- C-ABI of the MI355X lattice step and diagnostics for axion_grav.
+ *
+ * A drop-in for the reference's hot path: a Fortran host keeps main.f90's driver, the
+ * inifile/settings keys and the output formats, and calls these entry points through
+ * ISO_C_BINDING (see axion_grav_amd/fortran/axgrav_capi.f90).  All entry points are
+ * extern "C", take plain pointers and sizes, and return an int status (0 = ok, < 0 =
+ * error); the message of the last failure is axg_last_error().
+ *
+ * Field layout at the boundary: exactly the reference's complex(C_DOUBLE_COMPLEX)
+ * data1/data2 interior, N^3 sites, x fastest, interleaved (re, im) doubles, no halo
+ * planes.  The handle owns every device buffer; fields stay resident in HBM between
+ * calls; host pointers are borrowed only for the duration of a call; every call
+ * synchronizes before it returns.  One host thread per handle; handles are independent.
+ */
+#ifndef AXGRAV_H
+#define AXGRAV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AXG_OK = 0,
+    AXG_EINVAL = -1,  /* bad argument (size, null pointer, unsupported N) */
+    AXG_ENOMEM = -2,  /* device allocation failed */
+    AXG_EDEVICE = -3, /* HIP runtime error */
+    AXG_ESTATE = -4,  /* call out of order (e.g. spectrum before defects) */
+    AXG_ENUMERIC = -5 /* numerical failure (e.g. deconvolution matrix not positive definite) */
+};
+
+enum { AXG_FP64 = 0, AXG_FP32C = 1 };
+
+/* Parameters: the settings.f90 globals the hot path reads (settings.f90:17-66,
+ * potential.f90:24) plus the keys this build adds.  Derive the physical ones with
+ * axg_derive() from the ini keys, or fill them in from an existing settings module. */
+typedef struct axg_params {
+    int64_t ngrid;          /* N (settings key ngrid); even, 2^a * 3^b, 8 <= N <= 1024 */
+    int32_t precision;      /* AXG_FP64 or AXG_FP32C (new key precision) */
+    int32_t grav_spec;      /* accumulate GW each kick (key grav_spec) */
+    int32_t initial_loop;   /* thermal mass off (key initial_loop; evolve.f90:292-296) */
+    int32_t gw_fix_swap;    /* 0: reproduce grav_waves.f90:239-242 exactly; 1: intended form */
+    int32_t num_bin_grav;   /* GW spectrum bins (grav_waves.f90:6 hard-codes 25) */
+    int32_t num_bin;        /* axion spectrum bins (key num_bin, default 50) */
+    int32_t id_strings;     /* keys id_strings / id_walls / spec_est (defect flag bits) */
+    int32_t id_walls;
+    int32_t spec_est;
+    int32_t device;         /* HIP device ordinal */
+    double dspa, dwav, dv_spa, dv_wav;   /* settings.f90:294-297 */
+    double sigma_cr, tau_cr, a_now;      /* settings.f90:274-276 */
+    double kappa;                        /* potential.f90:24 (key kappa) */
+    double eps_dtau;                     /* key eps_dtau (get_dtau, evolve.f90:10-24) */
+    double dtau_fixed;                   /* > 0: fixed step instead of get_dtau (new key) */
+    double potential_kappa0;             /* N_DW=2 tilt -kappa0 tau^4 (phi^2+phi*^2)/2; 0 = off */
+    int32_t reserved[8];
+} axg_params;
+
+/* The ini keys axg_derive() needs (settings.f90:269-297). */
+typedef struct axg_keys {
+    double sigma, g_rel, kappa, tau_fin, boxsize;
+    int64_t ngrid;
+} axg_keys;
+
+/* Derived units of settings.f90:274-297 in the reference's FP64 operation order (with the
+ * truncated pi of potential.f90:8).  Fills the physical fields of *p; leaves the rest. */
+int axg_derive(const axg_keys* k, axg_params* p, double* zeta, double* width_cr,
+               double* thickness_cr);
+
+/* get_dtau (evolve.f90:10-24): the step and step count for the interval [t1, t2]. */
+int axg_get_dtau(const axg_params* p, double t1, double t2, double* dtau, int32_t* nsteps);
+
+typedef struct axg_handle axg_handle;
+
+/* Allocate the lattice on the device (replaces init_grav/init_spectrum's allocations,
+ * grav_waves.f90:27-82, spectrum.f90:23-103, initial.f90:240-285).  GW accumulators
+ * start at zero (grav_waves.f90:77). */
+int axg_create(const axg_params* p, axg_handle** out);
+int axg_destroy(axg_handle* h);
+/* Message of the last failed call on h; h == NULL gives the last axg_create failure of
+ * the calling thread. */
+const char* axg_last_error(const axg_handle* h);
+
+/* Copy phi (data1) and pi = phi' (data2) in / out; nsite must be N^3.  In FP32C mode the
+ * FP64 values are split into an FP32 value and its compensation. */
+int axg_set_fields(axg_handle* h, const double* phi, const double* pi, int64_t nsite);
+int axg_get_fields(axg_handle* h, double* phi, double* pi, int64_t nsite);
+
+/* Seeded flat-spectrum initial field: complex Gaussian modes (Philox4x32-10 + Box-Muller)
+ * for |k| < kmax, zero above, BACKWARD FFT, rescaled to <|phi|^2> = 1; pi = 0. */
+int axg_init_flat(axg_handle* h, uint64_t seed, double kmax);
+
+/* GW accumulators arr_grav (grav_waves.f90:12,74) on the half spectrum kx = 0..N/2:
+ * count = 4 * N * N * (N/2+1) complex, ordered [m][kz][ky][kx] with m = (1,1),(2,1),(1,2),
+ * (2,2) of arr_grav(a,b,...), interleaved (re, im).  The other half is the complex
+ * conjugate (exact for every mode that reaches a spectrum bin). */
+int axg_get_gw_state(axg_handle* h, double* A, int64_t count);
+int axg_set_gw_state(axg_handle* h, const double* A, int64_t count);
+
+/* symplectic_integration(t1, t2) (evolve.f90:238-257): get_dtau (or dtau_fixed), a drift
+ * of dt/2, then n x (kick at the midpoint with the GW accumulation of grav_waves.f90:193-255
+ * when grav_spec, drift dt), the last drift being dt/2.  Returns the step count. */
+int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_out);
+/* The same DKD schedule with an explicit step: nsteps steps of dt from t1. */
+int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps);
+
+/* grav_spec_estimation (grav_waves.f90:257-321) without the file write: qmid, power,
+ * numer, denom per bin (nbins must equal num_bin_grav).  Any pointer may be NULL. */
+int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, double* numer, double* denom,
+                    int32_t nbins);
+
+typedef struct axg_defect_out {
+    int64_t n_string;   /* cells flagged string (defects_3d.f90:340-343) */
+    int64_t n_wall;     /* cells flagged wall (defects_3d.f90:334-336) */
+    int64_t n_mask;     /* cells flagged mask (defects_3d.f90:344-347) */
+} axg_defect_out;
+
+/* id_defects classification pass (defects_3d.f90:67-103, 309-349): flags every cell,
+ * counts them and compacts the mask list on the device.  Flags stay resident for the
+ * field spectrum. */
+int axg_defects(axg_handle* h, axg_defect_out* out);
+/* Compacted mask cells in the reference's loop order (z, then y, then x fastest), as
+ * 1-based (ix, iy, iz) triplets, the '(3I6)' rows of _mask_<idx>-<rank>.dat
+ * (defects_3d.f90:265-273).  cap = room in ixyz (triplets); *count = cells written. */
+int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* count);
+/* The per-cell flag bits (1 string, 2 wall, 4 mask), N^3 int8 in site order. */
+int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite);
+
+/* spectrum_estimation's deterministic part (spectrum.f90:242-286 with compute_matrix_deconv
+ * 344-444 and compute_conv_spec 466-567, degrade_spec = 0): the masked pi spectrum
+ * deconvolved with the mask window.  tau is tau_id(idx).  power = the deconvolved
+ * spectrum, conv = conv_spec (already times tau^2), num = num_conv_spec; rho = the
+ * compute_crho energy (spectrum.f90:316-342, its first output).  Needs axg_defects first
+ * when spec_est (the mask); nbins must equal num_bin. */
+int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, double* conv,
+                       double* num, double* rho, int32_t nbins);
+
+/* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
+ * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
+int axg_energy(axg_handle* h, double tau, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""CPU FP64 oracle for the axion_grav lattice step and diagnostics (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this package.  It is the checker, never the thing measured or shipped: the product in
+``axion_grav_amd/`` does not import it.
+
+The arithmetic lives in ``oracle.c`` (a restatement of the reference Fortran, each function
+citing the reference file:line it follows); this module is a numpy/ctypes veneer over it.
+
+Complex fields are numpy ``complex128`` arrays of shape ``(N, N, N)`` indexed ``[z, y, x]``
+(C order, x fastest), which is byte-identical to the reference's Fortran ``data1(x, y, z)``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+PI_T = 3.14159265359  # potential.f90:8
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        d, i64, i32, p = C.c_double, C.c_int64, C.c_int32, C.c_void_p
+        L.orc_msquared.restype = d
+        L.orc_msquared.argtypes = [d]
+        L.orc_maxion2.restype = d
+        L.orc_maxion2.argtypes = [d, d]
+        L.orc_derive.argtypes = [d, d, d, d, d, i64, p]
+        L.orc_get_dtau.argtypes = [d, d, d, d, i64, d, p, p]
+        L.orc_drift.argtypes = [p, p, i64, d]
+        L.orc_kick.argtypes = [p, p, i64, d, d, d, d, d, C.c_int]
+        L.orc_fft3d.argtypes = [p, p, i64, C.c_int]
+        L.orc_gradient.argtypes = [p, p, i64, d]
+        L.orc_gw_integrate.argtypes = [p, p, i64, d, d, d, d, d, d, C.c_int]
+        L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int]
+        L.orc_gw_spectrum.argtypes = [p, i64, C.c_int, d, d, d, d, d, p, p, p, p]
+        L.orc_classify.argtypes = [p, i64, C.c_int, C.c_int, C.c_int, p, p]
+        L.orc_conv_spectrum.argtypes = [p, p, i64, C.c_int, d, d, d, d, p, p]
+        L.orc_integrate_p_check.restype = d
+        L.orc_integrate_p_check.argtypes = [C.c_int, d, d, d, d, d]
+        L.orc_init_spectrum.argtypes = [C.c_int, d, C.c_int, p, p, p, p]
+        L.orc_pp_inverse.restype = C.c_int
+        L.orc_pp_inverse.argtypes = [C.c_int, p]
+        L.orc_matrix_deconv.restype = C.c_int
+        L.orc_matrix_deconv.argtypes = [p, i64, C.c_int, d, d, d, p, p, p, C.c_int, p, p]
+        L.orc_energy.argtypes = [p, p, i64, d, d, d, d, C.c_int, p]
+        L.orc_philox.argtypes = [p, C.c_uint32, C.c_uint32]
+        L.orc_ic_mode.argtypes = [C.c_uint64, i64, i64, i64, i64, p, p]
+        L.orc_ic_flat.argtypes = [p, p, i64, C.c_uint64, d, d]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    if not (a.flags["C_CONTIGUOUS"] and a.dtype in (np.complex128, np.float64, np.int8, np.int64, np.uint32)):
+        raise ValueError("oracle arrays must be C-contiguous complex128/float64/int8/int64/uint32")
+    if a.dtype == np.complex128 and a.ndim >= 3 and len(set(a.shape[-3:])) != 1:
+        raise ValueError(f"lattice arrays must be (..., N, N, N), got {a.shape}")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _field(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+# ------------------------------------------------------------------ settings / potential
+
+def msquared(thrm):
+    return lib().orc_msquared(thrm)
+
+
+def maxion2(thrm, kappa):
+    return lib().orc_maxion2(thrm, kappa)
+
+
+DERIVED_KEYS = ("a_now", "tau_cr", "sigma_cr", "zeta", "width_cr", "thickness_cr",
+                "dspa", "dwav", "dv_spa", "dv_wav")
+
+
+def derive(sigma, g_rel, kappa, tau_fin, boxsize, ngrid):
+    out = np.zeros(10)
+    lib().orc_derive(sigma, g_rel, kappa, tau_fin, boxsize, int(ngrid), _p(out))
+    return dict(zip(DERIVED_KEYS, out.tolist()))
+
+
+def get_dtau(tau1, tau2, sigma_cr, dwav, ngrid, eps_dtau):
+    dt = C.c_double()
+    n = C.c_int32()
+    lib().orc_get_dtau(tau1, tau2, sigma_cr, dwav, int(ngrid), eps_dtau, C.byref(dt), C.byref(n))
+    return dt.value, n.value
+
+
+# ------------------------------------------------------------------ integrator
+
+def drift(phi, pi, dt):
+    lib().orc_drift(_p(phi), _p(pi), phi.size, dt)
+
+
+def kick(phi, pi, t, dt, dspa, sigma_cr, kappa, initial_loop):
+    lib().orc_kick(_p(phi), _p(pi), phi.shape[0], t, dt, dspa, sigma_cr, kappa, int(initial_loop))
+
+
+def fft3d(a, sign=+1):
+    a = _field(a)
+    out = np.empty_like(a)
+    lib().orc_fft3d(_p(a), _p(out), a.shape[0], int(sign))
+    return out
+
+
+def gradient(phi, dspa):
+    N = phi.shape[0]
+    g = np.empty((3, N, N, N), np.complex128)
+    lib().orc_gradient(_p(phi), _p(g), N, dspa)
+    return g
+
+
+def gw_integrate(phi, A, tau, dtau, dspa, dwav, dv_spa, sigma_cr, fix_swap=False):
+    lib().orc_gw_integrate(_p(phi), _p(A), phi.shape[0], tau, dtau, dspa, dwav, dv_spa, sigma_cr,
+                           int(fix_swap))
+
+
+def symplectic(phi, pi, A, t1, dt, nsteps, P, fix_swap=False):
+    """Advance (phi, pi[, A]) in place by nsteps DKD steps of dt from t1.  P is a dict with
+    dspa, dwav, dv_spa, sigma_cr, kappa, initial_loop."""
+    lib().orc_symplectic(_p(phi), _p(pi), _p(A) if A is not None else None, phi.shape[0], t1, dt,
+                         int(nsteps), P["dspa"], P["dwav"], P["dv_spa"], P["sigma_cr"], P["kappa"],
+                         int(P["initial_loop"]), int(fix_swap))
+
+
+def new_gw_state(N):
+    return np.zeros((4, N, N, N), np.complex128)
+
+
+# ------------------------------------------------------------------ diagnostics
+
+def gw_spectrum(A, nb, dspa, dwav, tau_cr, a_now, dv_wav):
+    N = A.shape[1]
+    numer, denom, qmid, power = (np.zeros(nb) for _ in range(4))
+    lib().orc_gw_spectrum(_p(A), N, nb, dspa, dwav, tau_cr, a_now, dv_wav, _p(numer), _p(denom),
+                          _p(qmid), _p(power))
+    return dict(numer=numer, denom=denom, qmid=qmid, power=power)
+
+
+def classify(phi, id_strings=True, id_walls=True, spec_est=True):
+    N = phi.shape[0]
+    flags = np.zeros((N, N, N), np.int8)
+    counts = np.zeros(3, np.int64)
+    lib().orc_classify(_p(phi), N, int(id_strings), int(id_walls), int(spec_est), _p(flags), _p(counts))
+    return flags, counts
+
+
+def conv_spectrum(pi, flags, nb, dspa, dwav, dv_spa, dv_wav):
+    N = pi.shape[0]
+    pw, num = np.zeros(nb), np.zeros(nb)
+    lib().orc_conv_spectrum(_p(pi), _p(flags) if flags is not None else None, N, nb, dspa, dwav,
+                            dv_spa, dv_wav, _p(pw), _p(num))
+    return pw, num
+
+
+def integrate_p_check(nint, li, si, lj, sj, q):
+    return lib().orc_integrate_p_check(nint, li, si, lj, sj, q)
+
+
+def init_spectrum(nb, dspa, num_largeq=500):
+    ncov = nb * (nb + 1) // 2
+    wav = np.zeros(nb + 1)
+    largeq = np.zeros((ncov, num_largeq + 1))
+    mat = np.zeros((ncov, num_largeq + 1))
+    dqs = np.zeros(ncov)
+    lib().orc_init_spectrum(nb, dspa, num_largeq, _p(wav), _p(largeq), _p(mat), _p(dqs))
+    return dict(wav=wav, largeq=largeq, mat=mat, dqs=dqs, num_largeq=num_largeq)
+
+
+def pp_inverse(nb, ap):
+    ap = np.ascontiguousarray(ap, dtype=np.float64).copy()
+    info = lib().orc_pp_inverse(nb, _p(ap))
+    return ap, info
+
+
+def matrix_deconv(flags, nb, dwav, dv_spa, dv_wav, spec_tab):
+    N = flags.shape[0]
+    ncov = nb * (nb + 1) // 2
+    mat, inv = np.zeros(ncov), np.zeros(ncov)
+    info = lib().orc_matrix_deconv(_p(flags), N, nb, dwav, dv_spa, dv_wav, _p(spec_tab["largeq"]),
+                                   _p(spec_tab["mat"]), _p(spec_tab["dqs"]), spec_tab["num_largeq"],
+                                   _p(mat), _p(inv))
+    return mat, inv, info
+
+
+def energy(phi, pi, t, dspa, sigma_cr, kappa, initial_loop):
+    out = np.zeros(4)
+    lib().orc_energy(_p(phi), _p(pi), phi.shape[0], t, dspa, sigma_cr, kappa, int(initial_loop), _p(out))
+    return out
+
+
+def philox(ctr, k0, k1):
+    c = np.array(ctr, np.uint32)
+    lib().orc_philox(_p(c), k0, k1)
+    return c
+
+
+def ic_mode(seed, N, ix, iy, iz):
+    re, im = C.c_double(), C.c_double()
+    lib().orc_ic_mode(seed, N, ix, iy, iz, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def ic_flat(N, seed, kmax, dwav):
+    phi = np.zeros((N, N, N), np.complex128)
+    pi = np.zeros((N, N, N), np.complex128)
+    lib().orc_ic_flat(_p(phi), _p(pi), N, seed, kmax, dwav)
+    return phi, pi
+
+
+def sym_from_packed(ap, nb):
+    """Full symmetric matrix from LAPACK packed-upper storage (ij = j + i(i-1)/2, j <= i)."""
+    M = np.zeros((nb, nb))
+    for i in range(1, nb + 1):
+        for j in range(1, i + 1):
+            v = ap[i * (i - 1) // 2 + j - 1]
+            M[j - 1, i - 1] = v
+            M[i - 1, j - 1] = v
+    return M

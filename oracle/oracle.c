@@ -1,0 +1,764 @@
+/*
+ * oracle.c -- CPU FP64 restatement of axion_grav's lattice time step and its
+ * checkpoint diagnostics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py load it.  The
+ * product (axion_grav_amd/) never links or calls it.
+ *
+ * It restates the reference's algorithm in the reference's own dataflow (unfused:
+ * drift, kick, gradient, then 6 x [source -> 3D FFT -> accumulate]) and its FP64
+ * operation order, compiled with -ffp-contract=off (the shipped reference build is
+ * -O0, so it has no fused multiply-adds).  Each function cites the reference
+ * file:line it follows.  The FFT is this file's own mixed-radix DFT (the reference
+ * calls FFTW3-MPI through MKL, which is not available); it is pinned against a
+ * brute-force DFT and numpy.fft in tests/test_oracle.py.
+ *
+ * Layout: a complex field is interleaved (re, im) doubles, N^3 sites, x fastest:
+ * site (x, y, z) -> x + N*(y + N*z), exactly the Fortran complex(C_DOUBLE_COMPLEX)
+ * array data1(1:N,1:N,1:N) without halo planes.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+typedef struct { double re, im; } oc;
+
+/* potential.f90:8 -- the truncated pi the reference uses everywhere */
+#define PI_T 3.14159265359
+#define TWOPI_T (2 * PI_T)
+/* potential.f90:13-30 */
+#define TCMB_IN_K 2.726
+#define MSTAR_IN_GEV 2.435e18
+#define GEV_IN_K 1.16045250061657e13
+#define GEV_IN_HZ 1.5192488833521e24
+#define BIGH 2.1332e-42
+#define G_REL_NOWS 3.9091
+#define N_AXION 6.68
+#define CT_AXION 6.26
+#define C0_AXION 1.0
+#define THRM_SIGMA 2.449489742783178
+
+static inline int64_t wrapn(int64_t i, int64_t n) { return (i < n / 2) ? i : i - n; }
+
+/* ------------------------------------------------------------------ potential */
+
+/* potential.f90:34-41 msquared */
+double orc_msquared(double thrm) { return (thrm * thrm - THRM_SIGMA * THRM_SIGMA) / 3; }
+
+/* potential.f90:62-79 maxion2 (kappa is a module variable there, an argument here) */
+double orc_maxion2(double thrm, double kappa)
+{
+    double thrm12 = kappa * pow(CT_AXION / C0_AXION, 1.0 / N_AXION);
+    double m;
+    if (thrm > thrm12) m = CT_AXION * pow(kappa / thrm, N_AXION);
+    else m = C0_AXION;
+    double k2 = kappa * kappa;
+    return m * (k2 * k2);
+}
+
+/* ------------------------------------------------------------------- settings */
+
+/* settings.f90:274-297: derived units.  out = {a_now, tau_cr, sigma_cr, zeta, width_cr,
+ * thickness_cr, dspa, dwav, dv_spa, dv_wav} */
+void orc_derive(double sigma, double g_rel, double kappa, double tau_fin, double boxsize,
+                int64_t ngrid, double* out)
+{
+    const double tcmb_in_mstar = TCMB_IN_K / GEV_IN_K / MSTAR_IN_GEV;
+    double a_now = sigma * THRM_SIGMA / tcmb_in_mstar * pow(g_rel / G_REL_NOWS, 1.0 / 3);
+    double tau_cr = 1 / PI_T / sqrt(g_rel / 90.0) / (sigma * sigma) / (THRM_SIGMA * THRM_SIGMA);
+    double sigma_cr = tau_cr * sigma;
+    double zeta = sqrt(22.5 / g_rel) / sigma / PI_T;
+    double width_cr = 1 / sqrt(2.0) / sigma_cr;
+    double thickness_cr = 1 / sqrt(2.0) / (sqrt(orc_maxion2(1.e-3 * kappa, kappa)) * sigma_cr);
+    double dspa = boxsize * tau_fin / (double)ngrid;
+    double dwav = TWOPI_T / dspa / (double)ngrid;
+    double dv_spa = dspa * dspa * dspa;
+    double r = dwav / TWOPI_T;
+    double dv_wav = r * r * r;
+    out[0] = a_now; out[1] = tau_cr; out[2] = sigma_cr; out[3] = zeta; out[4] = width_cr;
+    out[5] = thickness_cr; out[6] = dspa; out[7] = dwav; out[8] = dv_spa; out[9] = dv_wav;
+}
+
+/* evolve.f90:10-24 get_dtau */
+void orc_get_dtau(double tau1, double tau2, double sigma_cr, double dwav, int64_t ngrid,
+                  double eps_dtau, double* dtau, int32_t* n)
+{
+    double omega = THRM_SIGMA * tau2 * sigma_cr;
+    double kmax = dwav * (double)ngrid;
+    omega = sqrt(kmax * kmax + omega * omega);
+    double dt = eps_dtau / omega;
+    int32_t nn = (int32_t)ceil((tau2 - tau1) / dt);
+    *n = nn;
+    *dtau = (tau2 - tau1) / nn;
+}
+
+/* ------------------------------------------------------------------ integrator */
+
+/* evolve.f90:259-281 evolve_phi: phi <- phi + dt*pi over every site */
+void orc_drift(double* phi, const double* pi, int64_t nsite, double dt)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nsite; i++) {
+        phi[2 * i] = phi[2 * i] + dt * pi[2 * i];
+        phi[2 * i + 1] = phi[2 * i + 1] + dt * pi[2 * i + 1];
+    }
+}
+
+/* evolve.f90:283-335 evolve_phidot (without the GW call at 337), with potential.f90:43-60
+ * dVdphi inlined.  Periodic in all three directions (update_margin's halo planes are the
+ * periodic wrap on one rank). */
+void orc_kick(const double* phi, double* pi, int64_t N, double t, double dt, double dspa,
+              double sigma_cr, double kappa, int initial_loop)
+{
+    double hubble = 1 / t;
+    double thrm = initial_loop ? 0.0 : THRM_SIGMA / t;
+    double m2 = orc_msquared(thrm);
+    double ma2 = orc_maxion2(thrm, kappa);
+    double fact0 = 1 + dt * hubble;
+    double fact1 = 1 - dt * hubble;
+    double fact2 = dt / (dspa * dspa);
+    double fact3 = -(dt * (t * t) * (sigma_cr * sigma_cr));
+    fact1 = fact1 / fact0;
+    fact2 = fact2 / fact0;
+    fact3 = fact3 / fact0;
+    const oc* f = (const oc*)phi;
+    oc* p = (oc*)pi;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++) {
+            int64_t z1 = (z + N - 1) % N, z2 = (z + 1) % N;
+            int64_t y1 = (y + N - 1) % N, y2 = (y + 1) % N;
+            for (int64_t x = 0; x < N; x++) {
+                int64_t x1 = (x + N - 1) % N, x2 = (x + 1) % N;
+                oc c = f[x + N * (y + N * z)];
+                oc a1 = f[x1 + N * (y + N * z)], a2 = f[x2 + N * (y + N * z)];
+                oc b1 = f[x + N * (y1 + N * z)], b2 = f[x + N * (y2 + N * z)];
+                oc c1 = f[x + N * (y + N * z1)], c2 = f[x + N * (y + N * z2)];
+                double lr = a1.re + a2.re + b1.re + b2.re + c1.re + c2.re - 6 * c.re;
+                double li = a1.im + a2.im + b1.im + b2.im + c1.im + c2.im - 6 * c.im;
+                double s2 = c.re * c.re + c.im * c.im;
+                double g = 2 * s2 + m2;
+                double dr = g * c.re - ma2 / 2;
+                double di = g * c.im;
+                oc* q = &p[x + N * (y + N * z)];
+                q->re = fact1 * q->re + fact2 * lr + fact3 * dr;
+                q->im = fact1 * q->im + fact2 * li + fact3 * di;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ FFT */
+
+/* e^{sign*2*pi*i*m/n}, m = 0..n-1, from the exact pi (the DFT itself does not use the
+ * truncated constant: FFTW's twiddles are exact). */
+static oc* make_twiddles(int64_t n, int sign)
+{
+    oc* tw = (oc*)malloc(sizeof(oc) * n);
+    const long double pl = 3.141592653589793238462643383279502884L;
+    for (int64_t m = 0; m < n; m++) {
+        long double a = 2.0L * pl * (long double)m / (long double)n;
+        tw[m].re = (double)cosl(a);
+        tw[m].im = (double)(sign * sinl(a));
+    }
+    return tw;
+}
+
+static int64_t small_factor(int64_t n)
+{
+    if (n % 4 == 0 && n != 8) return 4;
+    if (n % 2 == 0) return 2;
+    for (int64_t p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* y[k] = sum_j x[j*xs] * w^{jk}, mixed radix, decimation in time.  tw is the table of the
+ * top-level length ntop; tws = ntop/n. */
+static void fft_rec(const oc* x, oc* y, int64_t n, int64_t xs, const oc* tw, int64_t ntop,
+                    int64_t tws, oc* tmp)
+{
+    if (n == 1) { y[0] = x[0]; return; }
+    int64_t p = small_factor(n), m = n / p;
+    for (int64_t r = 0; r < p; r++) fft_rec(x + r * xs, y + r * m, m, xs * p, tw, ntop, tws * p, tmp);
+    for (int64_t k = 0; k < m; k++) {
+        for (int64_t r = 0; r < p; r++) tmp[r] = y[r * m + k];
+        for (int64_t q = 0; q < p; q++) {
+            double sr = 0, si = 0;
+            int64_t kk = k + q * m;
+            for (int64_t r = 0; r < p; r++) {
+                oc w = tw[((r * kk) % n) * tws];
+                sr += tmp[r].re * w.re - tmp[r].im * w.im;
+                si += tmp[r].re * w.im + tmp[r].im * w.re;
+            }
+            y[kk].re = sr;
+            y[kk].im = si;
+        }
+    }
+}
+
+/* Unnormalized 3D DFT out[k] = sum_x in[x] e^{sign*2*pi*i k.x/N}; sign=+1 is FFTW_BACKWARD
+ * (grav_waves.f90:117-118, spectrum.f90:210-211).  in and out may alias. */
+void orc_fft3d(const double* in, double* out, int64_t N, int sign)
+{
+    int64_t n3 = N * N * N;
+    oc* o = (oc*)out;
+    if (out != in) memcpy(out, in, sizeof(oc) * n3);
+    oc* tw = make_twiddles(N, sign);
+    for (int axis = 0; axis < 3; axis++) {
+        int64_t es = axis == 0 ? 1 : (axis == 1 ? N : N * N);
+#pragma omp parallel
+        {
+            oc* a = (oc*)malloc(sizeof(oc) * N);
+            oc* b = (oc*)malloc(sizeof(oc) * N);
+            oc* t = (oc*)malloc(sizeof(oc) * N);
+#pragma omp for schedule(static)
+            for (int64_t line = 0; line < N * N; line++) {
+                int64_t base;
+                if (axis == 0) base = line * N;
+                else if (axis == 1) base = (line % N) + (line / N) * N * N;
+                else base = line;
+                for (int64_t i = 0; i < N; i++) a[i] = o[base + i * es];
+                fft_rec(a, b, N, 1, tw, N, 1, t);
+                for (int64_t i = 0; i < N; i++) o[base + i * es] = b[i];
+            }
+            free(a); free(b); free(t);
+        }
+    }
+    free(tw);
+}
+
+/* ----------------------------------------------------------------- GW sourcing */
+
+/* grav_waves.f90:157-191 get_comwav_grav for one wavevector index (0-based ix,iy,iz).
+ * Returns q (= |n|*dwav) and, when i>0, pol = p_i*p_j (halved if i==j). */
+static double comwav_grav(int64_t ix, int64_t iy, int64_t iz, int64_t N, double dwav, int i, int j,
+                          oc* pol)
+{
+    double qh[3] = {(double)wrapn(ix, N), (double)wrapn(iy, N), (double)wrapn(iz, N)};
+    double q = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
+    if (i > 0) {
+        if (q > 0) {
+            qh[0] = qh[0] / q; qh[1] = qh[1] / q; qh[2] = qh[2] / q;
+            double r = sqrt(qh[0] * qh[0] + qh[1] * qh[1]);
+            double c, s;
+            if (r > 0) { c = qh[0] / r; s = qh[1] / r; }
+            else { c = 1; s = 0; }
+            oc p[3] = {{qh[2] * c, -s}, {qh[2] * s, c}, {-r, 0}};
+            oc a = p[i - 1], b = p[j - 1];
+            pol->re = a.re * b.re - a.im * b.im;
+            pol->im = a.re * b.im + a.im * b.re;
+            if (i == j) { pol->re = pol->re / 2; pol->im = pol->im / 2; }
+        } else {
+            pol->re = 0; pol->im = 0;
+        }
+    }
+    return q * dwav;
+}
+
+/* grav_waves.f90:323-356 data1_to_arr_dphi: central differences, dphi[d*N^3 + site] */
+void orc_gradient(const double* phi, double* dphi, int64_t N, double dspa)
+{
+    const oc* f = (const oc*)phi;
+    oc* g = (oc*)dphi;
+    double d = dspa * 2;
+    int64_t n3 = N * N * N;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++) {
+            int64_t z1 = (z + N - 1) % N, z2 = (z + 1) % N;
+            int64_t y1 = (y + N - 1) % N, y2 = (y + 1) % N;
+            for (int64_t x = 0; x < N; x++) {
+                int64_t x1 = (x + N - 1) % N, x2 = (x + 1) % N;
+                int64_t s = x + N * (y + N * z);
+                oc a = f[x2 + N * (y + N * z)], b = f[x1 + N * (y + N * z)];
+                g[s].re = (a.re - b.re) / d; g[s].im = (a.im - b.im) / d;
+                a = f[x + N * (y2 + N * z)]; b = f[x + N * (y1 + N * z)];
+                g[n3 + s].re = (a.re - b.re) / d; g[n3 + s].im = (a.im - b.im) / d;
+                a = f[x + N * (y + N * z2)]; b = f[x + N * (y + N * z1)];
+                g[2 * n3 + s].re = (a.re - b.re) / d; g[2 * n3 + s].im = (a.im - b.im) / d;
+            }
+        }
+}
+
+/* grav_waves.f90:193-255 grav_wave_integration at the kick's time tau with step dtau.
+ * A holds the four accumulators arr_grav(a,b,k) as planes m = (a-1) + 2*(b-1):
+ * m=0 (1,1) cos,+   m=1 (2,1) sin,+   m=2 (1,2) cos,-   m=3 (2,2) sin,-   (full N^3 each).
+ * fix_swap=0 reproduces lines 239-242 exactly (A(2,1) <- A(1,2)+..., A(1,2) <- new A(2,1)+...);
+ * fix_swap=1 applies the evidently intended A(2,1) += s*P*D, A(1,2) += c*N*D. */
+void orc_gw_integrate(const double* phi, double* A, int64_t N, double tau, double dtau,
+                      double dspa, double dwav, double dv_spa, double sigma_cr, int fix_swap)
+{
+    int64_t n3 = N * N * N;
+    double* dphi = (double*)malloc(sizeof(double) * 6 * n3);
+    double* src = (double*)malloc(sizeof(double) * 2 * n3);
+    orc_gradient(phi, dphi, N, dspa);
+    const oc* g = (const oc*)dphi;
+    oc* a = (oc*)A;
+    double dw = tau * dtau * dv_spa * (sigma_cr * sigma_cr);
+    for (int i = 1; i <= 3; i++)
+        for (int j = 1; j <= i; j++) {
+            oc* s = (oc*)src;
+            const oc* gi = g + (i - 1) * n3;
+            const oc* gj = g + (j - 1) * n3;
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < n3; k++) {
+                /* dphi_i*conj(dphi_j) + conj(dphi_i)*dphi_j, as the complex products */
+                double zr = gi[k].re * gj[k].re - gi[k].im * (-gj[k].im);
+                double zi = gi[k].im * gj[k].re + gi[k].re * (-gj[k].im);
+                double wr = gi[k].re * gj[k].re - (-gi[k].im) * gj[k].im;
+                double wi = gi[k].re * gj[k].im + (-gi[k].im) * gj[k].re;
+                s[k].re = zr + wr;
+                s[k].im = zi + wi;
+            }
+            orc_fft3d(src, src, N, +1);
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int64_t z = 0; z < N; z++)
+                for (int64_t y = 0; y < N; y++)
+                    for (int64_t x = 0; x < N; x++) {
+                        int64_t k = x + N * (y + N * z);
+                        oc polp = {0, 0};
+                        double q = comwav_grav(x, y, z, N, dwav, i, j, &polp);
+                        q = q * tau;
+                        double c = cos(q), sn = sin(q);
+                        polp.re = polp.re * dw; polp.im = polp.im * dw;
+                        oc poln = {polp.re, -polp.im};
+                        oc d = s[k];
+                        oc pd = {polp.re * d.re - polp.im * d.im, polp.re * d.im + polp.im * d.re};
+                        oc nd = {poln.re * d.re - poln.im * d.im, poln.re * d.im + poln.im * d.re};
+                        oc* a11 = &a[k]; oc* a21 = &a[n3 + k]; oc* a12 = &a[2 * n3 + k]; oc* a22 = &a[3 * n3 + k];
+                        a11->re = a11->re + c * pd.re; a11->im = a11->im + c * pd.im;
+                        if (!fix_swap) {
+                            a21->re = a12->re + sn * pd.re; a21->im = a12->im + sn * pd.im;
+                            a12->re = a21->re + c * nd.re; a12->im = a21->im + c * nd.im;
+                        } else {
+                            a21->re = a21->re + sn * pd.re; a21->im = a21->im + sn * pd.im;
+                            a12->re = a12->re + c * nd.re; a12->im = a12->im + c * nd.im;
+                        }
+                        a22->re = a22->re + sn * nd.re; a22->im = a22->im + sn * nd.im;
+                    }
+        }
+    free(dphi);
+    free(src);
+}
+
+/* evolve.f90:238-257 symplectic_integration over [t1, t2] with nt steps of dt (from get_dtau
+ * or a fixed step): drift dt/2, then nt x (kick at the midpoint [+GW], drift dt), the last
+ * drift dt/2.  A may be NULL when grav_spec is off. */
+void orc_symplectic(double* phi, double* pi, double* A, int64_t N, double t1, double dt, int32_t nt,
+                    double dspa, double dwav, double dv_spa, double sigma_cr, double kappa,
+                    int initial_loop, int fix_swap)
+{
+    int64_t n3 = N * N * N;
+    double t = t1;
+    for (int32_t it = 1; it <= nt; it++) {
+        if (it == 1) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
+        orc_kick(phi, pi, N, t, dt, dspa, sigma_cr, kappa, initial_loop);
+        if (A) orc_gw_integrate(phi, A, N, t, dt, dspa, dwav, dv_spa, sigma_cr, fix_swap);
+        if (it == nt) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
+        else { orc_drift(phi, pi, n3, dt); t = t + dt; }
+    }
+}
+
+/* ---------------------------------------------------------------- GW spectrum */
+
+/* grav_waves.f90:62-71 (bin width), 257-305 grav_spec_estimation: numer[b], denom[b] over the
+ * full k-space for q>0, b = nint(q/dq + 0.5) in 1..nb (returned 0-based), and the power
+ * P[b] = numer/denom * qmid^3 * fact (fact from line 267, qmid from the cumulative edges). */
+void orc_gw_spectrum(const double* A, int64_t N, int nb, double dspa, double dwav, double tau_cr,
+                     double a_now, double dv_wav, double* numer, double* denom, double* qmid,
+                     double* power)
+{
+    int64_t n3 = N * N * N;
+    const oc* a = (const oc*)A;
+    double qmax = PI_T / dspa;
+    double dq0 = qmax / nb;
+    double* wave = (double*)calloc(nb + 2, sizeof(double));
+    wave[0] = 0;
+    for (int b = 1; b <= nb; b++) wave[b] = wave[b - 1] + dq0;
+    double dq = wave[1] - wave[0];
+    for (int b = 0; b < nb; b++) { numer[b] = 0; denom[b] = 0; }
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                double q = comwav_grav(x, y, z, N, dwav, 0, 0, NULL);
+                long iq = lround(q / dq + 0.5);
+                if (q > 0 && iq >= 1 && iq <= nb) {
+                    int64_t k = x + N * (y + N * z);
+                    denom[iq - 1] = denom[iq - 1] + 1;
+                    double s = 0;
+                    for (int m = 0; m < 4; m++) {
+                        oc v = a[m * n3 + k];
+                        /* abs(z)**2 in the reference: hypot squared; |z|^2 = re^2+im^2 */
+                        s += v.re * v.re + v.im * v.im;
+                    }
+                    numer[iq - 1] += s;
+                }
+            }
+    const double bigh_in_mstar = BIGH / MSTAR_IN_GEV;
+    double t6 = tau_cr * tau_cr * tau_cr; t6 = t6 * t6;
+    double a4 = a_now * a_now; a4 = a4 * a4;
+    double fact = 1 / t6 / (bigh_in_mstar * bigh_in_mstar) / a4 / 3 / (TWOPI_T * TWOPI_T) * dv_wav;
+    for (int b = 1; b <= nb; b++) {
+        double qm = (wave[b] + wave[b - 1]) / 2;
+        qmid[b - 1] = qm;
+        power[b - 1] = denom[b - 1] > 0 ? numer[b - 1] / denom[b - 1] * (qm * qm * qm) * fact : 0.0;
+    }
+    free(wave);
+}
+
+/* --------------------------------------------------------------------- defects */
+
+/* defects_3d.f90:309-349 id_with_phase for one cell's 8 phases.  Returns the flag bits
+ * (string 1, wall 2, mask 4). */
+static int id_with_phase(const double* ph, int id_strings, int id_walls, int spec_est)
+{
+    double p[8];
+    for (int i = 0; i < 8; i++) p[i] = ph[i];
+    for (int i = 7; i >= 1; i--)
+        for (int j = 0; j < i; j++)
+            if (p[j] > p[j + 1]) { double t = p[j]; p[j] = p[j + 1]; p[j + 1] = t; }
+    int flag = 0;
+    double t = p[0] - p[7] + TWOPI_T;
+    double tmax = p[1] - p[0];
+    for (int i = 2; i < 8; i++) { double d = p[i] - p[i - 1]; if (d > tmax) tmax = d; }
+    if (id_walls && tmax > t) flag += 2;
+    else tmax = tmax > t ? tmax : t;
+    if (id_strings && tmax < PI_T) flag += 1;
+    if (spec_est && tmax < 7 * PI_T / 4) flag += 4;
+    return flag;
+}
+
+/* defects_3d.f90:67-103 classification pass: for every cell (x,y,z) with corners
+ * (x+{0,1}, y+{0,1}, z+{0,1}) (periodic), ph = aimag(log(phi)) = atan2(im, re).
+ * flags[cell] gets the bits; counts = {strings, walls, masks}. */
+void orc_classify(const double* phi, int64_t N, int id_strings, int id_walls, int spec_est,
+                  int8_t* flags, int64_t* counts)
+{
+    const oc* f = (const oc*)phi;
+    int64_t ns = 0, nw = 0, nm = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : ns, nw, nm)
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++) {
+            int64_t z2 = (z + 1) % N, y2 = (y + 1) % N;
+            for (int64_t x = 0; x < N; x++) {
+                int64_t x2 = (x + 1) % N;
+                int64_t xs[2] = {x, x2}, ys[2] = {y, y2}, zs[2] = {z, z2};
+                double ph[8];
+                for (int i = 0; i < 8; i++) {
+                    oc v = f[xs[i & 1] + N * (ys[(i >> 1) & 1] + N * zs[i >> 2])];
+                    ph[i] = atan2(v.im, v.re);
+                }
+                int fl = id_with_phase(ph, id_strings, id_walls, spec_est);
+                flags[x + N * (y + N * z)] = (int8_t)fl;
+                ns += fl & 1;
+                nw += (fl >> 1) & 1;
+                nm += (fl >> 2) & 1;
+            }
+        }
+    counts[0] = ns; counts[1] = nw; counts[2] = nm;
+}
+
+/* ---------------------------------------------------------- field (axion) spectrum */
+
+/* spectrum.f90:466-567 compute_conv_spec (+ masking 900-954, degrade_spec=0 copy 736-792,
+ * get_comwav_spec 233-240): pi with the mask cells (flag bit 4) zeroed -> BACKWARD FFT ->
+ * *dv_spa -> |.|^2*dv_wav binned with i = nint(q/dq+0.5), q=0 included (bin 1). */
+void orc_conv_spectrum(const double* pi, const int8_t* flags, int64_t N, int nb, double dspa,
+                       double dwav, double dv_spa, double dv_wav, double* pow_out, double* num)
+{
+    int64_t n3 = N * N * N;
+    double* w = (double*)malloc(sizeof(double) * 2 * n3);
+    memcpy(w, pi, sizeof(double) * 2 * n3);
+    if (flags)
+        for (int64_t k = 0; k < n3; k++)
+            if (flags[k] & 4) { w[2 * k] = 0; w[2 * k + 1] = 0; }
+    orc_fft3d(w, w, N, +1);
+    for (int64_t k = 0; k < 2 * n3; k++) w[k] = w[k] * dv_spa;
+    double qmax = PI_T / dspa, dq0 = qmax / nb;
+    double* wave = (double*)calloc(nb + 2, sizeof(double));
+    wave[0] = 0;
+    for (int b = 1; b <= nb; b++) wave[b] = wave[b - 1] + dq0;
+    double dq = wave[1] - wave[0];
+    for (int b = 0; b < nb; b++) { pow_out[b] = 0; num[b] = 0; }
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                int64_t nx = wrapn(x, N), ny = wrapn(y, N), nz = wrapn(z, N);
+                double q = (double)(nx * nx) + (double)(ny * ny) + (double)(nz * nz);
+                q = sqrt(q) * dwav;
+                long i = lround(q / dq + 0.5);
+                if (i >= 1 && i <= nb) {
+                    int64_t k = x + N * (y + N * z);
+                    double re = w[2 * k], im = w[2 * k + 1];
+                    pow_out[i - 1] = pow_out[i - 1] + (re * re + im * im) * dv_wav;
+                    num[i - 1] = num[i - 1] + 1.0;
+                }
+            }
+    free(w);
+    free(wave);
+}
+
+/* spectrum.f90:117-171 integrate_p_check (3D branch) */
+double orc_integrate_p_check(int nint_, double li, double si, double lj, double sj, double q)
+{
+    double maxp = lj < q + li ? lj : q + li;
+    double minp = (q - li) > (si - q) ? (q - li) : (si - q);
+    minp = minp > sj ? minp : sj;
+    double dp = (maxp - minp) / nint_;
+    double x = 0;
+    if (dp > 0) {
+        for (int ip = 1; ip <= nint_; ip++) {
+            double p = dp * (ip - 0.5) + minp;
+            double maxmu, minmu;
+            if (q == 0.0) { maxmu = 1; minmu = -1; }
+            else {
+                maxmu = (li * li - p * p - q * q) / 2 / p / q;
+                minmu = (si * si - p * p - q * q) / 2 / p / q;
+                if (maxmu > 1) maxmu = 1;
+                if (minmu < -1) minmu = -1;
+            }
+            x = x + p * p * dp * (maxmu - minmu);
+        }
+    }
+    return x * TWOPI_T;
+}
+
+/* spectrum.f90:23-103 init_spectrum: bin edges, and for every packed pair ij (1-based,
+ * i = nint(sqrt(2 ij)), j = ij - i(i-1)/2) the 501-point table of shell-overlap integrals.
+ * largeq/mat: (num_largeq+1) x num_cov, column-major per pair; dqs: num_cov. */
+void orc_init_spectrum(int nb, double dspa, int num_largeq, double* wav, double* largeq,
+                       double* mat, double* dqs)
+{
+    double qmax = PI_T / dspa, dq = qmax / nb;
+    wav[0] = 0;
+    for (int b = 1; b <= nb; b++) wav[b] = wav[b - 1] + dq;
+    int num_cov = nb * (nb + 1) / 2;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ij = 1; ij <= num_cov; ij++) {
+        int i = (int)lround(sqrt(2.0 * ij));
+        int j = ij - i * (i - 1) / 2;
+        double lj = wav[j], sj = wav[j - 1], li = wav[i], si = wav[i - 1];
+        double maxq = li + lj;
+        double mn = li < lj ? li : lj;
+        double mx = si > sj ? si : sj;
+        double minq = mx - mn;
+        if (minq < 0) minq = 0;
+        double d = maxq - minq;
+        d = d / num_largeq;
+        dqs[ij - 1] = d;
+        for (int iq = 0; iq <= num_largeq; iq++) {
+            double q = minq + d * iq;
+            largeq[(int64_t)(ij - 1) * (num_largeq + 1) + iq] = q;
+            mat[(int64_t)(ij - 1) * (num_largeq + 1) + iq] = orc_integrate_p_check(200, li, si, lj, sj, q);
+        }
+    }
+}
+
+/* spectrum.f90:446-464 get_mat_spec */
+static double get_mat_spec(const double* largeq, const double* mat, const double* dqs, int nlq, int ij0,
+                           double q)
+{
+    const double* lq = largeq + (int64_t)ij0 * (nlq + 1);
+    const double* ms = mat + (int64_t)ij0 * (nlq + 1);
+    double d = (q - lq[0]) / dqs[ij0];
+    double fl = floor(d);
+    if (fl < nlq && fl >= 0) {
+        int iq = (int)fl;
+        d = d - iq;
+        return (1 - d) * ms[iq] + d * ms[iq + 1];
+    }
+    return 0.0;
+}
+
+/* Packed-upper Cholesky inverse (the LAPACK dpptrf('U') + dpptri('U') pair used at
+ * spectrum.f90:437-438): ap holds A(r,c), r<=c (1-based) at r + c(c-1)/2 - 1; returns 0 on
+ * success, the failing column otherwise. */
+int orc_pp_inverse(int n, double* ap)
+{
+#define AP(r, c) ap[(r) + (int64_t)(c) * ((c) - 1) / 2 - 1]
+    /* A = U^T U */
+    for (int j = 1; j <= n; j++) {
+        for (int i = 1; i < j; i++) {
+            double s = AP(i, j);
+            for (int k = 1; k < i; k++) s -= AP(k, i) * AP(k, j);
+            AP(i, j) = s / AP(i, i);
+        }
+        double s = AP(j, j);
+        for (int k = 1; k < j; k++) s -= AP(k, j) * AP(k, j);
+        if (!(s > 0)) return j;
+        AP(j, j) = sqrt(s);
+    }
+    /* U <- inv(U) (upper triangular), column by column: the leading (j-1) block already
+     * holds inv(U); column j = -inv(U)(1:j-1,1:j-1) * U(1:j-1,j) / U(j,j). */
+    double* col = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int j = 1; j <= n; j++) {
+        AP(j, j) = 1 / AP(j, j);
+        double ajj = -AP(j, j);
+        for (int i = 1; i < j; i++) {
+            double s = 0;
+            for (int k = i; k < j; k++) s += AP(i, k) * AP(k, j);
+            col[i - 1] = s * ajj;
+        }
+        for (int i = 1; i < j; i++) AP(i, j) = col[i - 1];
+    }
+    free(col);
+    /* inv(A) = inv(U) inv(U)^T, upper part */
+    for (int i = 1; i <= n; i++)
+        for (int j = i; j <= n; j++) {
+            double s = 0;
+            for (int k = j; k <= n; k++) s += AP(i, k) * AP(j, k);
+            AP(i, j) = s;
+        }
+#undef AP
+    return 0;
+}
+
+/* spectrum.f90:344-444 compute_matrix_deconv: window W = BACKWARD-FFT(dv_spa with mask cells
+ * zeroed); mat(ij) = sum_k |W|^2 u_ij(|k|) / sum_k u_ij(|k|) * dv_wav^2 over each pair's
+ * shell; returns mat (before inversion) and matinv (packed-upper inverse). */
+int orc_matrix_deconv(const int8_t* flags, int64_t N, int nb, double dwav, double dv_spa,
+                      double dv_wav, const double* largeq, const double* matspec, const double* dqs,
+                      int nlq, double* mat, double* matinv)
+{
+    int64_t n3 = N * N * N;
+    int num_cov = nb * (nb + 1) / 2;
+    double* w = (double*)malloc(sizeof(double) * 2 * n3);
+    for (int64_t k = 0; k < n3; k++) {
+        w[2 * k] = dv_spa; w[2 * k + 1] = 0;
+        if (flags && (flags[k] & 4)) w[2 * k] = 0;
+    }
+    orc_fft3d(w, w, N, +1);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ij = 0; ij < num_cov; ij++) {
+        double ql = largeq[(int64_t)ij * (nlq + 1) + nlq], q0 = largeq[(int64_t)ij * (nlq + 1)];
+        double t1 = ql / dwav, t0 = q0 / dwav;
+        int64_t imax2 = (int64_t)ceil(t1 * t1) - 1;
+        int64_t imin2 = (int64_t)ceil(t0 * t0);
+        double m0 = 0, t = 0;
+        for (int64_t z = 0; z < N; z++) {
+            int64_t iqz2 = wrapn(z, N) * wrapn(z, N);
+            if (iqz2 > imax2) continue;
+            for (int64_t y = 0; y < N; y++) {
+                int64_t iqy2 = wrapn(y, N) * wrapn(y, N);
+                if (iqz2 + iqy2 > imax2) continue;
+                for (int64_t x = 0; x < N; x++) {
+                    int64_t iq2 = wrapn(x, N) * wrapn(x, N) + iqy2 + iqz2;
+                    if (!(iq2 <= imax2 && iq2 >= imin2)) continue;
+                    double q = sqrt((double)iq2) * dwav;
+                    double u = get_mat_spec(largeq, matspec, dqs, nlq, ij, q);
+                    int64_t k = x + N * (y + N * z);
+                    double v = w[2 * k] * w[2 * k] + w[2 * k + 1] * w[2 * k + 1];
+                    m0 = m0 + v * u;
+                    t = t + u;
+                }
+            }
+        }
+        mat[ij] = m0 / t * dv_wav * dv_wav;
+    }
+    free(w);
+    memcpy(matinv, mat, sizeof(double) * num_cov);
+    return orc_pp_inverse(nb, matinv);
+}
+
+/* ------------------------------------------------------------------ energy */
+
+/* Comoving energy of the lattice (a diagnostic the reference does not have): kinetic
+ * sum|pi|^2, gradient sum_d |phi(x+e_d)-phi(x)|^2/dspa^2 (the form whose variation is the
+ * 7-point Laplacian), potential tau^2 sigma_cr^2 sum V with V = (|phi|^2+m2/2)^2 - ma2*Re(phi)
+ * (dV/dphi* is potential.f90:43-60's dVdphi), each times dv_spa; out[3] = total. */
+void orc_energy(const double* phi, const double* pi, int64_t N, double t, double dspa,
+                double sigma_cr, double kappa, int initial_loop, double* out)
+{
+    const oc* f = (const oc*)phi;
+    const oc* p = (const oc*)pi;
+    double thrm = initial_loop ? 0.0 : THRM_SIGMA / t;
+    double m2 = orc_msquared(thrm), ma2 = orc_maxion2(thrm, kappa);
+    double ek = 0, eg = 0, ev = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : ek, eg, ev)
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                int64_t k = x + N * (y + N * z);
+                oc c = f[k];
+                ek += p[k].re * p[k].re + p[k].im * p[k].im;
+                oc nb3[3] = {f[(x + 1) % N + N * (y + N * z)], f[x + N * ((y + 1) % N + N * z)],
+                             f[x + N * (y + N * ((z + 1) % N))]};
+                for (int d = 0; d < 3; d++) {
+                    double dr = nb3[d].re - c.re, di = nb3[d].im - c.im;
+                    eg += dr * dr + di * di;
+                }
+                double s2 = c.re * c.re + c.im * c.im;
+                double h = s2 + m2 / 2;
+                ev += h * h - ma2 * c.re;
+            }
+    double dv = dspa * dspa * dspa;
+    out[0] = ek * dv;
+    out[1] = eg / (dspa * dspa) * dv;
+    out[2] = t * t * sigma_cr * sigma_cr * ev * dv;
+    out[3] = out[0] + out[1] + out[2];
+}
+
+/* ------------------------------------------------------------ initial condition */
+
+/* Philox4x32-10 (Salmon et al., SC'11), the counter-based generator used for the seeded
+ * initial field so that every backend draws the same numbers for the same mode. */
+static void philox4x32_10(uint32_t ctr[4], uint32_t key0, uint32_t key1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key0, k1 = key1;
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
+        uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+        c0 = h1 ^ c1 ^ k0; c1 = l1; c2 = h0 ^ c3 ^ k1; c3 = l0;
+        k0 += W0; k1 += W1;
+    }
+    ctr[0] = c0; ctr[1] = c1; ctr[2] = c2; ctr[3] = c3;
+}
+
+void orc_philox(uint32_t* ctr, uint32_t k0, uint32_t k1) { philox4x32_10(ctr, k0, k1); }
+
+/* Mode amplitude of the seeded flat-spectrum initial field for the wavevector index
+ * (0-based ix,iy,iz): complex Gaussian (Box-Muller on two 53-bit uniforms in (0,1]). */
+void orc_ic_mode(uint64_t seed, int64_t N, int64_t ix, int64_t iy, int64_t iz, double* re, double* im)
+{
+    uint64_t lin = (uint64_t)ix + (uint64_t)N * ((uint64_t)iy + (uint64_t)N * (uint64_t)iz);
+    uint32_t c[4] = {(uint32_t)lin, (uint32_t)(lin >> 32), 0x6178696fu, 0x6e677276u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)(c[1] >> 11);
+    uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)(c[3] >> 11);
+    a &= (1ull << 53) - 1; b &= (1ull << 53) - 1;
+    double u1 = ((double)a + 1.0) / 9007199254740992.0;
+    double u2 = ((double)b + 1.0) / 9007199254740992.0;
+    double r = sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586 * u2;
+    *re = r * cos(th);
+    *im = r * sin(th);
+}
+
+/* Seeded initial field: phi = BACKWARD-FFT of complex Gaussian modes for |k| < kmax
+ * (|k| = sqrt(n^2)*dwav), zero above, rescaled so that <|phi|^2> = 1; pi = 0. */
+void orc_ic_flat(double* phi, double* pi, int64_t N, uint64_t seed, double kmax, double dwav)
+{
+    int64_t n3 = N * N * N;
+    memset(phi, 0, sizeof(double) * 2 * n3);
+    memset(pi, 0, sizeof(double) * 2 * n3);
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                int64_t nx = wrapn(x, N), ny = wrapn(y, N), nz = wrapn(z, N);
+                double q = sqrt((double)(nx * nx + ny * ny + nz * nz)) * dwav;
+                if (q < kmax) {
+                    int64_t k = x + N * (y + N * z);
+                    orc_ic_mode(seed, N, x, y, z, &phi[2 * k], &phi[2 * k + 1]);
+                }
+            }
+    orc_fft3d(phi, phi, N, +1);
+    double s = 0;
+    for (int64_t k = 0; k < 2 * n3; k++) s += phi[k] * phi[k];
+    double scale = 1.0 / sqrt(s / (double)n3);
+    for (int64_t k = 0; k < 2 * n3; k++) phi[k] = phi[k] * scale;
+}

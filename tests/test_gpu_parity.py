@@ -1,0 +1,247 @@
+"""Parity of the HIP path (called through the C-ABI) with the CPU FP64 oracle.
+
+Every comparison runs the GPU and the oracle on the same seeded inputs.  Bars:
+  * integer work (defect flags, counts, mask list, bin counts): bit-exact;
+  * FP64 kick/drift without GW: bit-exact (same operation order, no contraction);
+  * FP64 GW accumulators / spectra / energy: relative error <= 1e-10 (stated per test);
+  * FP32C: per-output errors against FP64 with the tolerance written in the test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from axion_grav_amd import capi
+from tests.conftest import SQ_HALF, phys
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def params(N, **kw):
+    P = phys(N)
+    return capi.make_params(N, P["dspa"], SQ_HALF, **kw), P
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def ic(N, seed, kmax):
+    P = phys(N)
+    return O.ic_flat(N, seed, kmax, P["dwav"])
+
+
+# ---------------------------------------------------------------- fields / integrator
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_field_roundtrip(prec):
+    say("roundtrip", prec)
+    N = 16
+    prm, _ = params(N, precision=prec, grav_spec=False)
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    pi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        a, b = L.get_fields()
+    if prec == capi.FP64:
+        assert np.array_equal(a, phi) and np.array_equal(b, pi)
+    else:
+        # FP32 value + int16 residual: ~39 significant bits
+        assert relerr(a, phi) < 4e-11 and relerr(b, pi) < 4e-11
+
+
+@pytest.mark.parametrize("N,nsteps", [(16, 1), (24, 3), (32, 10)])
+def test_kick_drift_bitexact(N, nsteps):
+    say("kick_drift", N, nsteps)
+    prm, P = params(N, grav_spec=False)
+    phi, pi = ic(N, 11, 4.0)
+    rng = np.random.default_rng(N)
+    pi = 0.1 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    t1, dt = 1.0, 0.05
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(t1, dt, nsteps)
+        g_phi, g_pi = L.get_fields()
+    o_phi, o_pi = phi.copy(), pi.copy()
+    O.symplectic(o_phi, o_pi, None, t1, dt, nsteps, P)
+    assert np.array_equal(g_phi, o_phi), relerr(g_phi, o_phi)
+    assert np.array_equal(g_pi, o_pi), relerr(g_pi, o_pi)
+
+
+# ---------------------------------------------------------------- GW accumulation
+
+@pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False)])
+def test_gw_accumulators_fp64(N, nsteps, fix):
+    say("gw", N, nsteps, fix)
+    prm, P = params(N, gw_fix_swap=fix)
+    phi, pi = ic(N, 5, 4.0)
+    t1, dt = 1.0, 0.05
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(t1, dt, nsteps)
+        A = L.gw_state()
+        g_phi, _ = L.get_fields()
+    o_phi, o_pi = phi.copy(), pi.copy()
+    OA = O.new_gw_state(N)
+    O.symplectic(o_phi, o_pi, OA, t1, dt, nsteps, P, fix_swap=fix)
+    assert np.array_equal(g_phi, o_phi)
+    h = N // 2 + 1
+    for m in range(4):
+        e = relerr(A[m], OA[m][:, :, :h])
+        assert e < 1e-10, (m, e)
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_gw_spectrum_fp64(N):
+    say("gw_spectrum", N)
+    prm, P = params(N, num_bin_grav=25, tau_cr=9.42, a_now=5.4e30)
+    P.update(tau_cr=9.42, a_now=5.4e30)
+    phi, pi = ic(N, 21, 4.0)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(1.0, 0.05, 3)
+        s = L.gw_spectrum()
+    OA = O.new_gw_state(N)
+    O.symplectic(phi, pi, OA, 1.0, 0.05, 3, P)
+    o = O.gw_spectrum(OA, 25, P["dspa"], P["dwav"], 9.42, 5.4e30, P["dv_wav"])
+    assert np.array_equal(s["denom"], o["denom"])
+    assert relerr(s["numer"], o["numer"]) < 1e-10
+    assert relerr(s["power"], o["power"]) < 1e-10
+    assert np.array_equal(s["qmid"], o["qmid"])
+
+
+def test_gw_radix3_n96_short():
+    """N = 96 exercises the radix-3 FFT stage (96 = 3 * 8 * 4)."""
+    say("gw n96")
+    N = 96
+    prm, P = params(N)
+    phi, pi = ic(N, 4, 2.0)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(1.0, 0.05, 1)
+        A = L.gw_state()
+    OA = O.new_gw_state(N)
+    O.symplectic(phi, pi, OA, 1.0, 0.05, 1, P)
+    for m in range(4):
+        assert relerr(A[m], OA[m][:, :, : N // 2 + 1]) < 1e-10
+
+
+# ---------------------------------------------------------------- defects / spectra / energy
+
+@pytest.mark.parametrize("N,kmax", [(16, 6.0), (32, 6.0)])
+def test_defects_bitexact(N, kmax):
+    say("defects", N)
+    prm, P = params(N, grav_spec=False)
+    phi, pi = ic(N, 77, kmax)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        cnt = L.defects()
+        fl = L.flags()
+        ml = L.mask_list()
+    ofl, ocnt = O.classify(phi)
+    assert np.array_equal(cnt, ocnt), (cnt, ocnt)
+    assert np.array_equal(fl, ofl)
+    zz, yy, xx = np.nonzero(ofl & 4)
+    exp = np.stack([xx + 1, yy + 1, zz + 1], axis=1).astype(np.int32)
+    assert np.array_equal(ml, exp)
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_field_spectrum_fp64(N):
+    say("field_spectrum", N)
+    nb = 8 if N == 16 else 12
+    prm, P = params(N, grav_spec=False, num_bin=nb)
+    phi, _ = ic(N, 31, 6.0)
+    rng = np.random.default_rng(1)
+    pi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    tau = 3.0
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.defects()
+        s = L.field_spectrum(tau)
+    flags, _ = O.classify(phi)
+    pw, num = O.conv_spectrum(pi, flags, nb, P["dspa"], P["dwav"], P["dv_spa"], P["dv_wav"])
+    assert np.array_equal(s["num"], num)
+    assert relerr(s["conv"], pw * tau * tau) < 1e-10
+    tab = O.init_spectrum(nb, P["dspa"])
+    mat, inv, info = O.matrix_deconv(flags, nb, P["dwav"], P["dv_spa"], P["dv_wav"], tab)
+    assert info == 0
+    M = O.sym_from_packed(inv, nb) / np.outer(num, num)
+    power = M @ (pw * tau * tau)
+    assert relerr(s["power"], power) < 1e-8
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-12), (capi.FP32C, 1e-9)])
+def test_energy(prec, tol):
+    say("energy", prec)
+    N = 24
+    prm, P = params(N, grav_spec=False, precision=prec)
+    phi, _ = ic(N, 8, 4.0)
+    rng = np.random.default_rng(2)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        e = L.energy(2.5)
+    o = O.energy(phi, pi, 2.5, P["dspa"], P["sigma_cr"], P["kappa"], True)
+    assert relerr(e, o) < tol
+
+
+@pytest.mark.parametrize("N,seed,kmax", [(16, 1234, 4.0), (64, 1234, 1.0)])
+def test_initial_field(N, seed, kmax):
+    say("ic", N)
+    prm, P = params(N, grav_spec=False)
+    with capi.Lattice(prm) as L:
+        L.init_flat(seed, kmax)
+        phi, pi = L.get_fields()
+    ophi, opi = O.ic_flat(N, seed, kmax, P["dwav"])
+    assert relerr(phi, ophi) < 1e-12
+    assert not np.any(pi)
+
+
+# ---------------------------------------------------------------- FP32C vs FP64
+
+def test_fp32c_tracks_fp64_gw():
+    say("fp32c vs fp64")
+    N = 32
+    phi, pi = ic(N, 6, 3.0)
+    out = {}
+    for prec in (capi.FP64, capi.FP32C, capi.FP32):
+        prm, P = params(N, precision=prec)
+        with capi.Lattice(prm) as L:
+            L.set_fields(phi, pi)
+            L.advance_steps(1.0, 0.05, 20)
+            out[prec] = (L.get_fields()[0], L.gw_spectrum())
+    ref_phi, ref_s = out[capi.FP64]
+    c_phi, c_s = out[capi.FP32C]
+    # compensated storage keeps phi within ~1e-9 of FP64 over 20 steps
+    assert relerr(c_phi, ref_phi) < 1e-8
+    # GW spectrum with FP32 transforms/accumulators: per-bin relative error
+    nz = ref_s["numer"] > 0
+    err = np.abs(c_s["numer"][nz] - ref_s["numer"][nz]) / ref_s["numer"][nz]
+    assert err.max() < 1e-4, err.max()
+    assert np.array_equal(c_s["denom"], ref_s["denom"])
+
+
+def test_advance_get_dtau_interval():
+    say("advance interval")
+    N = 16
+    prm, P = params(N, eps_dtau=1.0)
+    dt, n = capi.get_dtau(prm, 1.0, 2.0)
+    odt, on = O.get_dtau(1.0, 2.0, P["sigma_cr"], P["dwav"], N, 1.0)
+    assert (dt, n) == (odt, on)
+    phi, pi = ic(N, 2, 4.0)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        assert L.advance(1.0, 2.0) == n
+        g_phi, g_pi = L.get_fields()
+        A = L.gw_state()
+    OA = O.new_gw_state(N)
+    O.symplectic(phi, pi, OA, 1.0, dt, n, P)
+    assert np.array_equal(g_phi, phi) and np.array_equal(g_pi, pi)
+    assert relerr(A[0], OA[0][:, :, : N // 2 + 1]) < 1e-10

@@ -1,0 +1,443 @@
+"""Pins for the CPU oracle (no GPU).  Each check is chosen so that a plausible mistake in the
+oracle (a dropped term, a wrong sign or index, a transposed operand) fails it: closed forms,
+brute force on tiny lattices, numpy cross-checks and invariants.  The reference ships no
+golden vectors, fixtures or tests for this path, so these are the pins."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import SQ_HALF, phys
+
+PI_T = 3.14159265359
+
+
+def brute_dft3(a, sign):
+    N = a.shape[0]
+    k = np.arange(N)
+    W = np.exp(sign * 2j * np.pi * np.outer(k, k) / N)
+    # a[z, y, x]: transform every axis
+    return np.einsum("az,by,cx,zyx->abc", W, W, W, a)
+
+
+# ------------------------------------------------------------------ FFT (grav_waves.f90:225)
+
+@pytest.mark.parametrize("N", [4, 6, 8, 12])
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_fft_brute_force(N, sign):
+    rng = np.random.default_rng(N)
+    a = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    ref = brute_dft3(a, sign)
+    assert np.abs(O.fft3d(a, sign) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("N", [16, 24, 32, 48])
+def test_fft_numpy(N):
+    rng = np.random.default_rng(N + 1)
+    a = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    # FFTW_BACKWARD = unnormalized e^{+i}: numpy's ifftn times N^3
+    ref = np.fft.ifftn(a) * N ** 3
+    assert np.abs(O.fft3d(a, +1) - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_fft_delta_plane_wave_parseval():
+    N = 8
+    d = np.zeros((N, N, N), complex)
+    d[0, 0, 0] = 1
+    assert np.allclose(O.fft3d(d, +1), 1)
+    x = np.arange(N)
+    kx, ky, kz = 1, 2, 3
+    pw = np.exp(-2j * np.pi * (kx * x[None, None, :] + ky * x[None, :, None] + kz * x[:, None, None]) / N)
+    out = O.fft3d(pw, +1)
+    assert abs(out[kz, ky, kx] - N ** 3) < 1e-9 and np.abs(out).sum() - N ** 3 < 1e-8
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    assert abs(np.sum(np.abs(O.fft3d(a)) ** 2) - N ** 3 * np.sum(np.abs(a) ** 2)) < 1e-9 * N ** 3 * np.sum(np.abs(a) ** 2)
+
+
+# ------------------------------------------------------------------ kick / drift (evolve.f90:259-335)
+
+def kick_scalars(t, dt, dspa, sigma_cr):
+    f0 = 1 + dt / t
+    return (1 - dt / t) / f0, dt / dspa ** 2 / f0, -dt * t * t * sigma_cr ** 2 / f0
+
+
+def test_drift_exact():
+    N = 4
+    rng = np.random.default_rng(0)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    pi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    ref = phi + 0.3 * pi
+    O.drift(phi, pi, 0.3)
+    assert np.array_equal(phi, ref)
+
+
+def test_kick_constant_field_closed_form():
+    """Constant field: the Laplacian vanishes, so pi' = f1 pi + f3 V'(phi) exactly."""
+    N, dspa, t, dt = 6, 0.25, 2.0, 0.05
+    phi = np.full((N, N, N), 0.6 + 0.3j)
+    pi = np.full((N, N, N), 0.1 - 0.2j)
+    O.kick(phi, pi, t, dt, dspa, SQ_HALF, 1e-30, True)
+    f1, f2, f3 = kick_scalars(t, dt, dspa, SQ_HALF)
+    m2 = (0.0 - 2.449489742783178 ** 2) / 3  # thrm = 0 (initial_loop)
+    c = 0.6 + 0.3j
+    dv = (2 * abs(c) ** 2 + m2) * c - O.maxion2(0.0, 1e-30) / 2
+    exp = f1 * (0.1 - 0.2j) + f3 * dv
+    assert np.abs(pi - exp).max() < 1e-15
+
+
+def test_kick_plane_wave_lattice_eigenvalue():
+    """|phi| = 1 plane wave at zero temperature: V' = 0 (m2 = -2 makes 2|phi|^2+m2 = 0), and the
+    7-point Laplacian of exp(i k.x) is exactly -sum_d (2 - 2 cos k_d dspa)/dspa^2 times it."""
+    N, dspa, t, dt = 8, 0.25, 1.5, 0.04
+    x = np.arange(N)
+    k = 2 * np.pi * np.array([1, 2, 3]) / (N * dspa)
+    ph = k[0] * dspa * x[None, None, :] + k[1] * dspa * x[None, :, None] + k[2] * dspa * x[:, None, None]
+    phi = np.exp(1j * ph)
+    pi = np.zeros_like(phi)
+    O.kick(phi, pi, t, dt, dspa, SQ_HALF, 1e-30, True)
+    f1, f2, f3 = kick_scalars(t, dt, dspa, SQ_HALF)
+    m2 = -2.449489742783178 ** 2 / 3
+    lam = -np.sum(2 - 2 * np.cos(k * dspa))
+    dv = (2 + m2) * phi - O.maxion2(0.0, 1e-30) / 2
+    exp = f2 * lam * phi + f3 * dv
+    assert np.abs(pi - exp).max() < 1e-12
+
+
+def test_massless_mode_converges_second_order():
+    """A small-amplitude mode of the linearised equation with H = 1/tau: the lattice solution
+    with dispersion w^2 = sum (2-2cos)/dspa^2 is j0-like, sin(w tau)/(w tau); the DKD
+    integrator converges to it at O(dt^2)."""
+    N, dspa = 8, 0.5
+    k = 2 * np.pi / (N * dspa)
+    w = np.sqrt((2 - 2 * np.cos(k * dspa))) / dspa
+    x = np.arange(N)
+    P = dict(phys(N, dspa), sigma_cr=1e-9)  # switch the potential off (sigma_cr -> 0)
+    t0, t1 = 1.0, 3.0
+    errs = []
+    for nsteps in (40, 80, 160):
+        dt = (t1 - t0) / nsteps
+        amp = np.sin(w * t0) / (w * t0)
+        damp = (np.cos(w * t0) * w * t0 - np.sin(w * t0)) / (w * t0 * t0)
+        mode = np.cos(k * dspa * x)[None, None, :] * np.ones((N, N, 1))
+        phi = (amp * mode).astype(complex)
+        pi = (damp * mode).astype(complex)
+        O.symplectic(phi, pi, None, t0, dt, nsteps, P)
+        exact = np.sin(w * t1) / (w * t1)
+        errs.append(abs(phi[0, 0, 0].real - exact))
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5, errs
+
+
+def test_time_reversal():
+    """The Crank-Nicolson friction form is reversible: n steps forward then n steps with -dt
+    (times run backwards) return the fields to round-off."""
+    N = 8
+    P = phys(N)
+    phi, pi = O.ic_flat(N, 3, 4.0, P["dwav"])
+    p0, q0 = phi.copy(), pi.copy()
+    O.symplectic(phi, pi, None, 2.0, 0.02, 25, P)
+    O.symplectic(phi, pi, None, 2.5, -0.02, 25, P)
+    assert np.abs(phi - p0).max() < 1e-11 and np.abs(pi - q0).max() < 1e-11
+
+
+# ------------------------------------------------------------------ gradient / source / accumulate
+
+def test_gradient_plane_wave():
+    N, dspa = 8, 0.25
+    x = np.arange(N)
+    k = 2 * np.pi * np.array([1, -2, 3]) / (N * dspa)
+    phi = np.exp(1j * (k[0] * dspa * x[None, None, :] + k[1] * dspa * x[None, :, None] + k[2] * dspa * x[:, None, None]))
+    g = O.gradient(phi, dspa)
+    for d in range(3):
+        assert np.abs(g[d] - 1j * np.sin(k[d] * dspa) / dspa * phi).max() < 1e-13
+
+
+def pol_reference(n, i, j):
+    """grav_waves.f90:165-186 restated in numpy for one wavevector."""
+    n = np.asarray(n, float)
+    q = np.sqrt(np.sum(n ** 2))
+    qh = n / q
+    r = np.hypot(qh[0], qh[1])
+    c, s = (qh[0] / r, qh[1] / r) if r > 0 else (1.0, 0.0)
+    p = np.array([qh[2] * c - 1j * s, qh[2] * s + 1j * c, -r + 0j])
+    v = p[i] * p[j]
+    return v / 2 if i == j else v, p, qh
+
+
+@pytest.mark.parametrize("n", [(1, 2, 3), (0, 0, 5), (0, 0, -4), (-3, 1, 0), (2, -2, -1)])
+def test_polarisation_invariants(n):
+    _, p, qh = pol_reference(n, 0, 0)
+    assert abs(np.dot(p, qh)) < 1e-15            # transverse
+    assert abs(np.dot(p, p)) < 1e-15             # traceless e_ij = p_i p_j
+    assert abs(np.vdot(p, p) - 2) < 1e-15        # normalisation
+    _, pm, _ = pol_reference(-np.asarray(n), 0, 0)
+    if np.hypot(n[0], n[1]) > 0:
+        assert np.abs(pm - np.conj(p)).max() < 1e-15   # pol(-k) = conj(pol(k))
+
+
+def test_gw_single_step_composition():
+    """One GW integration on a field whose source has one Fourier mode: checks the helicity
+    projection, the dw weight and the swapped accumulator order of grav_waves.f90:239-242
+    against a hand composition of the six sequential updates."""
+    N, dspa = 8, 0.25
+    P = phys(N, dspa)
+    x = np.arange(N)
+    k = 2 * np.pi * np.array([1, 0, 0]) / (N * dspa)
+    # phi = exp(i k x) + 0.5 exp(i 2k y): S_ij has a few modes
+    phi = (np.exp(1j * k[0] * dspa * x[None, None, :]) + 0.5 * np.exp(2j * k[0] * dspa * x[None, :, None])) \
+        * np.ones((N, 1, 1))
+    tau, dtau = 2.0, 0.05
+    A0 = np.random.default_rng(1).standard_normal((4, N, N, N)) * (1 + 0j)
+    for fix in (False, True):
+        A = A0.copy()
+        O.gw_integrate(phi, A, tau, dtau, dspa, P["dwav"], P["dv_spa"], P["sigma_cr"], fix_swap=fix)
+        g = O.gradient(phi, dspa)
+        dw = tau * dtau * P["dv_spa"] * P["sigma_cr"] ** 2
+        ref = A0.copy()
+        idx = np.arange(N)
+        nw = np.where(idx < N // 2, idx, idx - N)
+        for (i, j) in [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]:
+            S = g[i] * np.conj(g[j]) + np.conj(g[i]) * g[j]
+            D = np.fft.ifftn(S) * N ** 3
+            for kz in range(N):
+                for ky in range(N):
+                    for kx in range(N):
+                        nn = (nw[kx], nw[ky], nw[kz])
+                        if nn == (0, 0, 0):
+                            pol = 0
+                        else:
+                            pol, _, _ = pol_reference(nn, i, j)
+                        q = np.sqrt(sum(v * v for v in nn)) * P["dwav"] * tau
+                        c, s = np.cos(q), np.sin(q)
+                        pp = pol * dw
+                        pn = np.conj(pp)
+                        d = D[kz, ky, kx]
+                        a = ref[:, kz, ky, kx]
+                        a[0] += c * pp * d
+                        if not fix:
+                            a[1] = a[2] + s * pp * d
+                            a[2] = a[1] + c * pn * d
+                        else:
+                            a[1] += s * pp * d
+                            a[2] += c * pn * d
+                        a[3] += s * pn * d
+        assert np.abs(A - ref).max() < 1e-12 * np.abs(ref).max(), fix
+
+
+def test_gw_half_spectrum_is_exact_for_binned_modes():
+    """Every binned mode outside the stored half spectrum (kx > N/2) is the conjugate of its
+    mirror inside it, so storing kx = 0..N/2 loses nothing the spectrum reads."""
+    N = 16
+    P = phys(N)
+    phi, pi = O.ic_flat(N, 9, 4.0, P["dwav"])
+    A = O.new_gw_state(N)
+    O.symplectic(phi, pi, A, 1.0, 0.05, 2, P)
+    idx = np.arange(N)
+    n = np.where(idx < N // 2, idx, idx - N)
+    mirror = (-idx) % N
+    for kz in range(N):
+        for ky in range(N):
+            for kx in range(N // 2 + 1, N):
+                if n[kx] ** 2 + n[ky] ** 2 + n[kz] ** 2 > (N // 2) ** 2:
+                    continue
+                for m in range(4):
+                    a = A[m, kz, ky, kx]
+                    b = np.conj(A[m, mirror[kz], mirror[ky], mirror[kx]])
+                    assert abs(a - b) <= 1e-12 * np.abs(A[m]).max()
+
+
+# ------------------------------------------------------------------ spectra bins
+
+def nint(x):
+    return np.floor(np.abs(x) + 0.5) * np.sign(x)
+
+
+@pytest.mark.parametrize("N,nb", [(8, 25), (16, 25), (12, 7)])
+def test_gw_bin_counts_brute_force(N, nb):
+    P = phys(N)
+    A = np.ones((4, N, N, N), complex)
+    s = O.gw_spectrum(A, nb, P["dspa"], P["dwav"], 1.0, 1.0, P["dv_wav"])
+    dq = PI_T / P["dspa"] / nb
+    cnt = np.zeros(nb)
+    idx = np.arange(N)
+    n = np.where(idx < N // 2, idx, idx - N)
+    for a in n:
+        for b in n:
+            for c in n:
+                q = np.sqrt(float(a * a + b * b + c * c)) * P["dwav"]
+                i = int(nint(q / dq + 0.5))
+                if q > 0 and 1 <= i <= nb:
+                    cnt[i - 1] += 1
+    assert np.array_equal(s["denom"], cnt)
+    assert np.allclose(s["numer"], 4 * cnt)
+
+
+def test_conv_spectrum_single_mode():
+    """One Fourier mode of pi lands in one bin with |N^3 dv_spa|^2 dv_wav (q = 0 counted in
+    bin 1, spectrum.f90:538-551)."""
+    N, nb = 8, 10
+    P = phys(N)
+    x = np.arange(N)
+    pi = np.exp(-2j * np.pi * 2 * x[None, None, :] / N) * np.ones((N, N, 1))
+    pw, num = O.conv_spectrum(pi, None, nb, P["dspa"], P["dwav"], P["dv_spa"], P["dv_wav"])
+    q = 2 * P["dwav"]
+    b = int(nint(q / (PI_T / P["dspa"] / nb) + 0.5))
+    exp = np.zeros(nb)
+    exp[b - 1] = (N ** 3 * P["dv_spa"]) ** 2 * P["dv_wav"]
+    assert np.allclose(pw, exp, rtol=1e-13, atol=1e-30)
+    assert num[0] >= 1  # k = 0 is binned
+
+
+def test_integrate_p_check_shell_overlap():
+    """integrate_p_check(li,si,lj,sj,q) = 2 pi int p^2 dp dmu over p in shell j with |p+q| in
+    shell i; for q = 0 it is the full shell volume 4/3 pi (l^3 - s^3) of the overlap."""
+    v = O.integrate_p_check(200, 2.0, 1.0, 2.0, 1.0, 0.0)
+    assert abs(v - 4 / 3 * PI_T * (8 - 1)) < 1e-3 * v
+    # Monte Carlo of the overlap volume for q > 0
+    rng = np.random.default_rng(0)
+    li, si, lj, sj, q = 2.0, 1.5, 1.7, 1.2, 0.6
+    pts = rng.uniform(-lj, lj, size=(400000, 3))
+    r = np.linalg.norm(pts, axis=1)
+    inj = (r >= sj) & (r < lj)
+    rq = np.linalg.norm(pts + np.array([0, 0, q]), axis=1)
+    ini = (rq >= si) & (rq < li)
+    mc = (2 * lj) ** 3 * np.mean(inj & ini)
+    assert abs(O.integrate_p_check(200, li, si, lj, sj, q) - mc) < 0.02 * mc
+
+
+def test_packed_inverse():
+    rng = np.random.default_rng(0)
+    nb = 7
+    M = rng.standard_normal((nb, nb))
+    M = M @ M.T + nb * np.eye(nb)
+    ap = np.array([M[j - 1, i - 1] for i in range(1, nb + 1) for j in range(1, i + 1)])
+    inv, info = O.pp_inverse(nb, ap)
+    assert info == 0
+    assert np.allclose(O.sym_from_packed(inv, nb), np.linalg.inv(M), rtol=1e-12, atol=1e-14)
+
+
+def test_deconvolution_without_mask_is_diagonal():
+    """No masked cell: W = N^3 dv_spa delta_k0, so each pair's matrix element is
+    |W(0)|^2 u_ij(0) / sum u_ij * dv_wav^2, non-zero only where the pair's shells overlap at
+    q = 0, i.e. on the diagonal."""
+    N, nb = 8, 4
+    P = phys(N)
+    tab = O.init_spectrum(nb, P["dspa"])
+    flags = np.zeros((N, N, N), np.int8)
+    mat, inv, info = O.matrix_deconv(flags, nb, P["dwav"], P["dv_spa"], P["dv_wav"], tab)
+    M = O.sym_from_packed(mat, nb)
+    off = M - np.diag(np.diag(M))
+    assert np.abs(off).max() <= 1e-12 * np.abs(np.diag(M)).max()
+
+
+# ------------------------------------------------------------------ defects (defects_3d.f90:309-349)
+
+def classify_cell(ph, ids=True, idw=True, spec=True):
+    p = np.sort(ph)
+    t = p[0] - p[7] + 2 * PI_T
+    tmax = np.max(np.diff(p))
+    fl = 0
+    if idw and tmax > t:
+        fl += 2
+    else:
+        tmax = max(t, tmax)
+    if ids and tmax < PI_T:
+        fl += 1
+    if spec and tmax < 7 * PI_T / 4:
+        fl += 4
+    return fl
+
+
+def test_classify_uniform_phase():
+    N = 6
+    phi = np.full((N, N, N), np.exp(0.4j))
+    fl, cnt = O.classify(phi)
+    assert np.array_equal(cnt, [0, 0, 0])
+
+
+def test_classify_straight_string():
+    """phi = (x - x0) + i (y - y0) with x0, y0 inside one cell: exactly one string cell per
+    z plane (the winding cell), every z."""
+    N = 8
+    x = np.arange(N)
+    X = x[None, None, :] - 3.5
+    Y = x[None, :, None] - 4.5
+    phi = (X + 1j * Y) * np.ones((N, 1, 1))
+    fl, cnt = O.classify(phi, True, False, False)
+    s = fl & 1
+    # the periodic wrap adds the image windings at the box edges: count the cell at (3, 4)
+    assert all(s[z, 4, 3] == 1 for z in range(N))
+    assert s[:, 1:N - 1, 1:N - 1].sum() == N
+
+
+def test_classify_wall_plane():
+    """An N_DW = 1 wall is where the phase passes through +-pi: phi = -1 + 0.1 i sign, the sign
+    flipping across x = 3|4 (and, periodically, 7|0).  The cells straddling the flips hold
+    phases on both sides of the branch cut, so their largest gap is the one through 0: walls
+    there and nowhere else (defects_3d.f90:332-336)."""
+    N = 8
+    x = np.arange(N)
+    sgn = np.where(x < 4, 1.0, -1.0)
+    phi = (-1 + 0.1j * sgn)[None, None, :] * np.ones((N, N, 1))
+    fl, cnt = O.classify(phi, True, True, False)
+    w = (fl & 2) == 2
+    assert np.all(w[:, :, 3]) and np.all(w[:, :, 7])
+    assert w.sum() == 2 * N * N and cnt[1] == 2 * N * N
+    assert cnt[0] == 0
+
+
+def test_classify_random_matches_brute_force():
+    N = 6
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    fl, cnt = O.classify(phi)
+    exp = np.zeros_like(fl)
+    for z in range(N):
+        for y in range(N):
+            for x in range(N):
+                ph = [np.angle(phi[(z + dz) % N, (y + dy) % N, (x + dx) % N])
+                      for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)]
+                exp[z, y, x] = classify_cell(np.array(ph))
+    assert np.array_equal(fl, exp)
+    assert cnt[0] == np.sum(exp & 1) and cnt[1] == np.sum((exp >> 1) & 1) and cnt[2] == np.sum((exp >> 2) & 1)
+
+
+# ------------------------------------------------------------------ energy / IC / settings
+
+def test_energy_constant_field():
+    N, dspa, t = 4, 0.25, 2.0
+    c = 0.7 + 0.2j
+    phi = np.full((N, N, N), c)
+    pi = np.full((N, N, N), 0.3 - 0.1j)
+    e = O.energy(phi, pi, t, dspa, SQ_HALF, 1e-30, True)
+    m2 = -2.449489742783178 ** 2 / 3
+    V = (abs(c) ** 2 + m2 / 2) ** 2 - O.maxion2(0.0, 1e-30) * c.real
+    dv = dspa ** 3 * N ** 3
+    assert abs(e[0] - 0.1 * dv) < 1e-14
+    assert e[1] == 0
+    assert abs(e[2] - t * t * 0.5 * V * dv) < 1e-13
+    assert abs(e[3] - e[0] - e[1] - e[2]) < 1e-14
+
+
+def test_philox_known_answer():
+    # Random123 Philox4x32-10 known-answer vectors (counter, key) -> output
+    assert list(O.philox([0, 0, 0, 0], 0, 0)) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(O.philox([0xffffffff] * 4, 0xffffffff, 0xffffffff)) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert list(O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], 0xa4093822, 0x299f31d0)) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_ic_flat_mode_count_and_norm():
+    N = 64
+    P = phys(N)
+    phi, pi = O.ic_flat(N, 1234, 1.0, P["dwav"])
+    assert abs(np.mean(np.abs(phi) ** 2) - 1) < 1e-12
+    nz = np.abs(np.fft.fftn(phi)) > 1e-9 * np.abs(np.fft.fftn(phi)).max()
+    assert nz.sum() == 81  # |n|^2 <= 6 for |k| < 1 at L = 16
+    assert not np.any(pi)
+
+
+def test_get_dtau_hand_values():
+    # omega = sqrt((dwav N)^2 + (T_cr/sigma tau2 sigma_cr)^2), evolve.f90:17-21
+    dt, n = O.get_dtau(1.0, 6.0, SQ_HALF, 2 * PI_T / 16, 64, 1.0)
+    om = np.sqrt((2 * PI_T / 16 * 64) ** 2 + (2.449489742783178 * 6.0 * SQ_HALF) ** 2)
+    assert n == int(np.ceil(5.0 * om)) and abs(dt - 5.0 / n) < 1e-16
