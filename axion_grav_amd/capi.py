@@ -60,7 +60,7 @@ class DefectOut(C.Structure):
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_field_spectrum", "axg_energy")
+           "axg_field_spectrum", "axg_energy", "axg_profile")
 
 _LIB = None
 
@@ -91,6 +91,7 @@ def lib():
             "axg_get_flags": [p, p, i64],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_energy": [p, d, p],
+            "axg_profile": [p, i32, p, p, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -264,6 +265,15 @@ class Lattice:
         self._check(lib().axg_field_spectrum(self._h, tau, _ptr(q), _ptr(pw), _ptr(cv), _ptr(nm), C.byref(rho), nb),
                     "axg_field_spectrum")
         return dict(qmid=q, power=pw, conv=cv, num=nm, rho=rho.value)
+
+    KERNEL_SLOTS = ("drift", "kick_source_drift", "gw_xfft", "gw_yfft", "gw_zfft_accumulate")
+
+    def profile(self, enable):
+        """Start (True: reset) or stop kernel timing; returns {slot: (total ms, launches)}."""
+        ms = np.zeros(5)
+        n = np.zeros(5, np.int64)
+        self._check(lib().axg_profile(self._h, int(enable), _ptr(ms), _ptr(n), 5), "axg_profile")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_SLOTS)}
 
     def energy(self, tau):
         out = np.zeros(4)

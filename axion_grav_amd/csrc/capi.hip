@@ -75,6 +75,8 @@ using namespace axg;
 
 // ------------------------------------------------------------------ handle
 
+constexpr int AXG_NSLOT = 5;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -122,6 +124,13 @@ struct axg_handle {
     double* d_cnt = nullptr;  // lattice points per n^2
     double* d_hist = nullptr;
     double* d_plane = nullptr;
+    // kernel timing
+    bool prof = false;
+    std::vector<hipEvent_t> ev;      // event pool
+    std::vector<int> ev_slot;        // slot of each recorded (start, stop) pair
+    size_t ev_used = 0;
+    double prof_ms[AXG_NSLOT] = {0};
+    int64_t prof_n[AXG_NSLOT] = {0};
     std::string err;
 
     void* alloc(size_t bytes)
@@ -135,8 +144,37 @@ struct axg_handle {
         bufs.push_back({q, bytes});
         return q;
     }
+    // Record a start/stop event pair around launch() on the handle's stream when timing.
+    template <typename F>
+    void timed(int slot, F&& launch)
+    {
+        if (!prof) { launch(); return; }
+        while (ev.size() < 2 * (ev_used + 1)) {
+            hipEvent_t e;
+            AXG_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        AXG_HIP(hipEventRecord(ev[2 * ev_used], stream));
+        launch();
+        AXG_HIP(hipEventRecord(ev[2 * ev_used + 1], stream));
+        if (ev_slot.size() <= ev_used) ev_slot.resize(ev_used + 1);
+        ev_slot[ev_used++] = slot;
+    }
+    // Fold the recorded pairs into the totals (stream already synchronized).
+    void collect()
+    {
+        for (size_t i = 0; i < ev_used; i++) {
+            float ms = 0;
+            AXG_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            prof_ms[ev_slot[i]] += ms;
+            prof_n[ev_slot[i]] += 1;
+        }
+        ev_used = 0;
+    }
     void release()
     {
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
         for (auto& b : bufs) (void)hipFree(b.p);
         bufs.clear();
         if (stream) (void)hipStreamDestroy(stream);
@@ -490,24 +528,30 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
     for (int32_t it = 1; it <= nt; it++) {
         if (it == 1) {
             // first half drift, in place (each site reads and writes only itself)
-            launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
-                                  h->cphi[h->cur], h->cpi, dt / 2);
+            h->timed(0, [&] {
+                launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
+                                      h->cphi[h->cur], h->cpi, dt / 2);
+            });
             t = t + dt / 2;
         }
         KickScalars k = kick_scalars(p, t, dt);
         double dtd = (it == nt) ? dt / 2 : dt;
         int nxt = 1 - h->cur;
-        launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
-                                   h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
+        h->timed(1, [&] {
+            launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
+                                       h->cphi[nxt], h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
+        });
         if (p.grav_spec) {
-            launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw);
-            launch_fft_cols<T, +1>(h->stream, N, h->W, h->P, (int64_t)N * h->P, 6 * N, h->nh, h->plan, h->tw);
+            h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
+            h->timed(3, [&] {
+                launch_fft_cols<T, +1>(h->stream, N, h->W, h->P, (int64_t)N * h->P, 6 * N, h->nh, h->plan, h->tw);
+            });
             GwStepScalars g;
             g.tau = t;
             g.dw = t * dt * p.dv_spa * (p.sigma_cr * p.sigma_cr);  // grav_waves.f90:209
             g.dwav = p.dwav;
             g.fix_swap = p.gw_fix_swap;
-            launch_gw_zpass_acc<T>(h->stream, N, h->P, h->W, h->A, h->plan, h->tw, g);
+            h->timed(4, [&] { launch_gw_zpass_acc<T>(h->stream, N, h->P, h->W, h->A, h->plan, h->tw, g); });
         }
         h->cur = nxt;
         t = t + dtd;
@@ -521,6 +565,23 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     else run_steps<double, false>(h, t1, dt, nt);
     h->have_flags = false;
     AXG_HIP(hipStreamSynchronize(h->stream));
+    if (h->prof) h->collect();
+}
+
+extern "C" int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* launches, int32_t nslots)
+{
+    return guarded(h, [&] {
+        require(nslots >= 0 && nslots <= AXG_NSLOT, "nslots must be 0..5");
+        if (enable && !h->prof) {
+            for (int i = 0; i < AXG_NSLOT; i++) { h->prof_ms[i] = 0; h->prof_n[i] = 0; }
+            h->ev_used = 0;
+        }
+        h->prof = enable != 0;
+        for (int i = 0; i < nslots; i++) {
+            if (ms) ms[i] = h->prof_ms[i];
+            if (launches) launches[i] = h->prof_n[i];
+        }
+    });
 }
 
 extern "C" int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps)
@@ -652,8 +713,8 @@ extern "C" int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* 
         require(h->have_flags, "call axg_defects first");
         require(count != nullptr, "null count");
         *count = h->nmask;
-        if (h->nmask == 0) return;
-        require(ixyz != nullptr && cap >= h->nmask, "mask buffer too small");
+        if (h->nmask == 0 || ixyz == nullptr) return;  // size query
+        require(cap >= h->nmask, "mask buffer too small");
         const int64_t nblk = (h->n3 + 255) / 256;
         int64_t* offs = h->blk + 3 * nblk;
         int32_t* dm = (int32_t*)h->work;  // 3*nmask ints fit in the N^3 x 16-byte work array

@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the axion_grav lattice step with GW sourcing on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
+
+Workload: a 512^3 periodic lattice, comoving box L = 128 (dspa = 0.25), complex scalar with
+lambda = 2 (sigma_cr^2 = 1/2, zero-temperature potential), radiation era from tau = 1 with
+dtau = 0.2 dspa, the seeded flat-spectrum initial field (seed 7, |k| < 1), GW accumulation on.
+A step is one drift-kick-drift with the six source transforms and the projected accumulation.
+The headline value is the FP32-compensated path (FP32 storage + int16 residual for phi and
+pi, FP32 transforms and accumulators, FP64 stencil arithmetic); the FP64 path on the same
+GPU is reported beside it.  Inputs are resident in HBM before the timed region; the timed
+region is exactly K steps between device synchronizations.
+
+Prints ONE JSON line (rank 0).  `roofline` covers the dominant kernel, its achieved
+algorithmic bytes per launch over its mean duration measured with HIP events on the stream
+the kernels run on.  `cpu_baseline` is the CPU FP64 oracle (the reference dataflow restated
+in C, OpenMP) timed on this host on a bounded sample: one step at 512^3.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+TWOPI_T = 2 * 3.14159265359
+SIGMA_CR = float(np.sqrt(0.5))
+
+# Algorithmic reals moved per site by one launch of each step kernel (the model of 40 reals
+# per site-update splits as 14 + 12 + 14 when the x transform is fused into the stencil pass;
+# this build runs it as its own pass, 14 + 12 + 12 + 14).
+KERNEL_REALS = {
+    "drift": 6,                  # phi r/w, pi read
+    "kick_source_drift": 14,     # phi r/w, pi r/w, 6 source reals written
+    "gw_xfft": 12,               # 6 reals in, 6 half-spectrum reals out
+    "gw_yfft": 12,
+    "gw_zfft_accumulate": 14,    # 6 half-spectrum reals in, 4 accumulators read + written
+}
+MODEL_REALS = 40
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def params(capi, N, L, precision, nbins):
+    dspa = L / N
+    return capi.make_params(N, dspa, SIGMA_CR, precision=precision, grav_spec=True, initial_loop=True,
+                            kappa=1e-30, num_bin_grav=nbins, spec_est=False)
+
+
+def run_gpu(capi, N, L, precision, steps, warmup, seed):
+    P = params(capi, N, L, precision, 128)
+    real = 8 if precision == capi.FP64 else 4
+    dt = 0.2 * (L / N)
+    with capi.Lattice(P) as lat:
+        lat.init_flat(seed, 1.0)
+        t = 1.0
+        if warmup > 0:
+            lat.advance_steps(t, dt, warmup)
+            t += warmup * dt
+        lat.profile(True)
+        t0 = time.perf_counter()
+        lat.advance_steps(t, dt, steps)  # synchronizes before returning
+        el = time.perf_counter() - t0
+        prof = lat.profile(False)
+        spec = lat.gw_spectrum()
+    n3 = N ** 3
+    kern = {}
+    for k, (ms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        avg = ms / cnt
+        alg = KERNEL_REALS[k] * real * n3
+        kern[k] = dict(ms_avg=avg, launches=cnt, alg_bytes=alg, gbs=alg / (avg * 1e-3) / 1e9)
+    return dict(su_s=steps * n3 / el, ms_per_step=el / steps * 1e3, kernels=kern, real=real,
+                finite=bool(np.all(np.isfinite(spec["numer"]))))
+
+
+def run_cpu(N, L, seed):
+    import oracle as O
+
+    dspa = L / N
+    dwav = TWOPI_T / dspa / N
+    P = dict(dspa=dspa, dwav=dwav, dv_spa=dspa ** 3, sigma_cr=SIGMA_CR, kappa=1e-30, initial_loop=True)
+    phi, pi = O.ic_flat(N, seed, 1.0, dwav)
+    A = O.new_gw_state(N)
+    t0 = time.perf_counter()
+    O.symplectic(phi, pi, A, 1.0, 0.2 * dspa, 1, P)
+    el = time.perf_counter() - t0
+    return N ** 3 / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--box", type=float, default=128.0)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the lattice lives in one MI355X's HBM and this build has no "
+              f"multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    from axion_grav_amd import capi
+
+    N, L = args.n, args.box
+    log(f"FP32C {N}^3 L={L}: warmup {args.warmup}, timed {args.steps} steps")
+    c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed)
+    log(f"FP32C {c['su_s']:.4g} site-updates/s, {c['ms_per_step']:.3f} ms/step; kernels "
+        + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in c["kernels"].items()))
+    d = None
+    if not args.no_fp64:
+        log(f"FP64 {N}^3: warmup {args.warmup}, timed {args.steps} steps")
+        d = run_gpu(capi, N, L, capi.FP64, args.steps, args.warmup, args.seed)
+        log(f"FP64 {d['su_s']:.4g} site-updates/s, {d['ms_per_step']:.3f} ms/step; kernels "
+            + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in d["kernels"].items()))
+
+    dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
+    roof = dict(bound="hbm", kernel=dom_name, achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4))
+    step_frac = MODEL_REALS * c["real"] * c["su_s"] / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"CPU oracle: one step at {N}^3 on {cores} threads")
+        su, el = run_cpu(N, L, args.seed)
+        cpu = dict(value=su, unit="site-updates/s", cores=cores, kind="port",
+                   sample=f"1 DKD step with GW at {N}^3 (FP64 oracle, unfused reference dataflow), {el:.1f} s")
+        log(f"CPU {su:.4g} site-updates/s ({el:.1f} s)")
+
+    out = {
+        "metric": "site-updates/s at 512^3 GW on (FP32c)",
+        "value": c["su_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded flat-spectrum initial field, seed %d)" % args.seed,
+        "config": {"workload": f"{N}^3 lattice, L={L:g}, GW on, FP32 compensated storage", "N": N, "L": L,
+                   "dtau": 0.2 * L / N, "gw_bins": 128, "precision": "fp32c"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "step_model_frac": round(step_frac, 4),
+        "fp64": None if d is None else {"value": d["su_s"], "ms_per_step": d["ms_per_step"],
+                                        "step_model_frac": round(MODEL_REALS * 8 * d["su_s"] / 8e12, 4)},
+        "fp32c_over_fp64": None if d is None else round(c["su_s"] / d["su_s"], 3),
+        "kernels": {k: {"ms": round(v["ms_avg"], 4), "GB/s": round(v["gbs"], 1)} for k, v in c["kernels"].items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

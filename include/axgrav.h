@@ -122,7 +122,8 @@ typedef struct axg_defect_out {
 int axg_defects(axg_handle* h, axg_defect_out* out);
 /* Compacted mask cells in the reference's loop order (z, then y, then x fastest), as
  * 1-based (ix, iy, iz) triplets, the '(3I6)' rows of _mask_<idx>-<rank>.dat
- * (defects_3d.f90:265-273).  cap = room in ixyz (triplets); *count = cells written. */
+ * (defects_3d.f90:265-273).  cap = room in ixyz (triplets); *count = cells written.  With
+ * ixyz == NULL only *count is set (size query). */
 int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* count);
 /* The per-cell flag bits (1 string, 2 wall, 4 mask), N^3 int8 in site order. */
 int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite);
@@ -139,6 +140,13 @@ int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, d
 /* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);
+
+/* Kernel timing (no counterpart in the reference; for measurement).  enable = 1 resets the
+ * counters and starts timing every step kernel with HIP events on the handle's stream;
+ * enable = 0 stops.  ms[slot] / launches[slot] receive the totals so far for the slots
+ * 0 first drift, 1 kick + GW source + drift, 2 x-transform, 3 y-transform,
+ * 4 z-transform + accumulation (nslots <= 5; either pointer may be NULL). */
+int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* launches, int32_t nslots);
 
 #ifdef __cplusplus
 }
