@@ -9,6 +9,7 @@
 // y-transformed in place, and is z-transformed on the fly by the accumulation pass.  The
 // accumulators A[m][kz][ky][kx] (m = arr_grav (1,1),(2,1),(1,2),(2,2)) use the same pitch.
 #include "axg_internal.h"
+#include "fft_p2.h"
 
 namespace axg {
 
@@ -21,6 +22,207 @@ template <typename T> int col_width(int N)
     int tx = col_tile<T>();
     while (tx > 1 && N * tx > fft_elems<T>()) tx /= 2;
     return N * tx <= fft_elems<T>() ? tx : 0;
+}
+
+
+// ================================================================== N = 2^L specializations
+// The same three passes with the compile-time radix-8 engine of fft_p2.h: every thread
+// carries 8 elements of one transform in registers, the first load and last store go
+// directly between registers and HBM, and LDS is touched only between stages.
+
+template <int N> constexpr int p2_xrows() { return (2048 / N) < N ? (2048 / N < 1 ? 1 : 2048 / N) : N; }
+
+template <typename T, int N>
+__global__ void __launch_bounds__(p2_xrows<N>() * (N / 8)) gw_xpass_p2_kernel(const typename V2<T>::type* __restrict__ src,
+                                                                           typename V2<T>::type* __restrict__ W, int P,
+                                                                           const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int TPT = N / 8, XR = p2_xrows<N>();
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int rr = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+    const int y = blockIdx.x * XR + rr, z = blockIdx.y, m = blockIdx.z;
+    const int64_t NN = (int64_t)N * N;
+    const T2* in = src + (int64_t)m * NN * N + (int64_t)z * NN + (int64_t)y * N;
+    T2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = in[j0 + e * TPT];
+    fft_p2<T, N, +1>(v, j0, s, 1, rr * N, tw);
+    T2* row = s + rr * N;
+#pragma unroll
+    for (int e = 0; e < 8; e++) row[j0 + e * TPT] = v[e];
+    __syncthreads();
+    T2* wa = W + ((int64_t)(2 * m) * NN + (int64_t)z * N + y) * P;
+    T2* wb = W + ((int64_t)(2 * m + 1) * NN + (int64_t)z * N + y) * P;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+        const int k = j0 + e * TPT;
+        if (k <= N / 2) {
+            T2 xk = row[k];
+            T2 xm = row[(N - k) & (N - 1)];
+            wa[k] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+            wb[k] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        }
+    }
+}
+
+template <typename T, int N, int SIGN>
+__global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(typename V2<T>::type* __restrict__ data,
+                                                                           int64_t es, int64_t ps, int ncol,
+                                                                           const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int TX = p2_tx<T, N>(), TPT = N / 8;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
+    const int col = blockIdx.x * TX + b;
+    const bool ok = col < ncol;
+    T2* base = data + (int64_t)blockIdx.y * ps + col;
+    T2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * es] : T2{0, 0};
+    fft_p2<T, N, SIGN>(v, j0, s, TX, b, tw);
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) base[(int64_t)(j0 + e * TPT) * es] = v[e];
+    }
+}
+
+// The accumulation pass keeps 8 k-points' projected sums per thread: at most 512 threads per
+// block so the compiler has 256 VGPRs.
+template <typename T, int N> constexpr int p2_ztx()
+{
+    int tx = p2_tx<T, N>();
+    while (tx > 1 && tx * (N / 8) > 512) tx /= 2;
+    return tx;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
+                                                                           typename V2<T>::type* __restrict__ A, int P,
+                                                                           const typename V2<T>::type* __restrict__ tw,
+                                                                           GwStepScalars g)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    constexpr int TX = p2_ztx<T, N>(), TPT = N / 8, NH = N / 2 + 1;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
+    const int kx = blockIdx.x * TX + b, ky = blockIdx.y;
+    const bool ok = kx < NH;
+    const int64_t NN = (int64_t)N * N;
+    const int64_t es = (int64_t)N * P;
+    const int nx = wrapn(kx, N), ny = wrapn(ky, N);
+
+    // polarization geometry of grav_waves.f90:165-183, p1 = (qz c, -s), p2 = (qz s, c),
+    // p3 = (-r, 0) with qhat = n/|n|, r = |qhat_xy|, c = qhat_x/r = nx/rho, s = ny/rho
+    // (rho = |n_xy|; c = 1, s = 0 on the kz axis): c and s are the same for the thread's 8
+    // wavevectors, only qz and r vary with kz.
+    T gc, gs;
+    {
+        const T rho = sqrt((T)nx * (T)nx + (T)ny * (T)ny);
+        if (rho > 0) { gc = (T)nx / rho; gs = (T)ny / rho; }
+        else { gc = 1; gs = 0; }
+    }
+    T gz[8], gr[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int nz = wrapn(j0 + e * TPT, N);
+        const T q = sqrt((T)nx * (T)nx + (T)ny * (T)ny + (T)nz * (T)nz);
+        if (q > 0) {
+            const T qx = (T)nx / q, qy = (T)ny / q;
+            gz[e] = (T)nz / q;
+            gr[e] = sqrt(qx * qx + qy * qy);
+        } else {
+            gz[e] = gr[e] = 0;  // k = 0: pol = 0 (flagged by gr = gz = 0 below)
+        }
+    }
+
+    T2 sp[8], sn[8], v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) sp[e] = sn[e] = T2{0, 0};
+    const T2* colp = W + (int64_t)ky * P + kx;
+#pragma unroll 1
+    for (int c = 0; c < 6; c++) {
+        const T2* base = colp + (int64_t)c * NN * P;
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * es] : T2{0, 0};
+        if (c > 0) __syncthreads();
+        fft_p2<T, N, +1>(v, j0, s, TX, b, tw);
+        const int ii = c == 0 ? 0 : (c <= 2 ? 1 : 2);
+        const int jj = (c == 0 || c == 1 || c == 3) ? 0 : ((c == 2 || c == 4) ? 1 : 2);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const bool kz0 = (gz[e] == 0 && gr[e] == 0);
+            T2 p1 = {gz[e] * gc, -gs}, p2 = {gz[e] * gs, gc}, p3 = {-gr[e], 0};
+            T2 a = ii == 0 ? p1 : (ii == 1 ? p2 : p3);
+            T2 bb = jj == 0 ? p1 : (jj == 1 ? p2 : p3);
+            T2 pol = {a.x * bb.x - a.y * bb.y, a.x * bb.y + a.y * bb.x};
+            if (ii == jj) { pol.x = pol.x / 2; pol.y = pol.y / 2; }
+            if (kz0) pol = T2{0, 0};
+            T2 d = v[e];
+            sp[e].x += pol.x * d.x - pol.y * d.y;
+            sp[e].y += pol.x * d.y + pol.y * d.x;
+            sn[e].x += pol.x * d.x + pol.y * d.y;
+            sn[e].y += pol.x * d.y - pol.y * d.x;
+        }
+    }
+    if (!ok) return;
+    const int64_t ms = NN * P;
+    const T dw = (T)g.dw;
+    const int64_t rho2 = (int64_t)nx * nx + (int64_t)ny * ny;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int kz = j0 + e * TPT;
+        const int nz = wrapn(kz, N);
+        const int64_t n2 = rho2 + (int64_t)nz * nz;
+        // the phase q tau in FP64 (large arguments); FP32 storage evaluates the reduced phase in FP32
+        double q = sqrt((double)n2) * g.dwav * g.tau;
+        T cq, sq;
+        if constexpr (sizeof(T) == 8) {
+            cq = cos(q);
+            sq = sin(q);
+        } else {
+            const double twopi = 6.283185307179586476925286766559;
+            float r = (float)(q - twopi * rint(q * (1.0 / twopi)));
+            __sincosf(r, &sq, &cq);
+        }
+        const T2 SP = {sp[e].x * dw, sp[e].y * dw};
+        const T2 SN = {sn[e].x * dw, sn[e].y * dw};
+        const int64_t ai = (int64_t)kz * es + (int64_t)ky * P + kx;
+        T2 a11 = A[ai], a22 = A[3 * ms + ai], a12 = A[2 * ms + ai];
+        A[ai] = T2{a11.x + cq * SP.x, a11.y + cq * SP.y};
+        A[3 * ms + ai] = T2{a22.x + sq * SN.x, a22.y + sq * SN.y};
+        if (!g.fix_swap) {
+            T2 r12 = {a12.x + (sq * SP.x + cq * SN.x), a12.y + (sq * SP.y + cq * SN.y)};
+            const T n6 = n2 > 0 ? ((T)rho2 / (T)n2) / 2 * dw : (T)0;  // pol_33 dw = r^2/2 dw
+            A[2 * ms + ai] = r12;
+            A[ms + ai] = T2{r12.x - cq * (n6 * v[e].x), r12.y - cq * (n6 * v[e].y)};
+        } else {
+            T2 a21 = A[ms + ai];
+            A[ms + ai] = T2{a21.x + sq * SP.x, a21.y + sq * SP.y};
+            A[2 * ms + ai] = T2{a12.x + cq * SN.x, a12.y + cq * SN.y};
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the k-points' loads from piling up in registers
+    }
+}
+
+// Runs F.template operator()<N>() for a power-of-two N in 16..1024; returns false otherwise.
+template <typename F> bool p2_dispatch(int N, F&& f)
+{
+    switch (N) {
+    case 16: f.template operator()<16>(); return true;
+    case 32: f.template operator()<32>(); return true;
+    case 64: f.template operator()<64>(); return true;
+    case 128: f.template operator()<128>(); return true;
+    case 256: f.template operator()<256>(); return true;
+    case 512: f.template operator()<512>(); return true;
+    case 1024: f.template operator()<1024>(); return true;
+    default: return false;
+    }
 }
 
 // ------------------------------------------------------------------ x pass (R2C via pairing)
@@ -63,6 +265,12 @@ template <typename T>
 void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan, const void* tw)
 {
     using T2 = typename V2<T>::type;
+    bool done = p2_dispatch(N, [&]<int NN>() {
+        constexpr int XR = p2_xrows<NN>();
+        hipLaunchKernelGGL((gw_xpass_p2_kernel<T, NN>), dim3(NN / XR, NN, 3), dim3(XR * (NN / 8)),
+                           (size_t)XR * NN * sizeof(T2), s, (const T2*)src, (T2*)W, pitch, (const T2*)tw);
+    });
+    if (done) { AXG_HIP(hipGetLastError()); return; }
     int TB = fft_elems<T>() / N;
     if (TB > N) TB = N;
     dim3 grid((N + TB - 1) / TB, N, 3);
@@ -103,6 +311,13 @@ void launch_fft_cols(hipStream_t s, int N, void* data, int64_t es, int64_t ps, i
                      const FftPlan& plan, const void* tw)
 {
     using T2 = typename V2<T>::type;
+    bool done = p2_dispatch(N, [&]<int NN>() {
+        constexpr int TXP = p2_tx<T, NN>();
+        hipLaunchKernelGGL((fft_cols_p2_kernel<T, NN, SIGN>), dim3((ncol + TXP - 1) / TXP, nplanes),
+                           dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (T2*)data, es, ps, ncol,
+                           (const T2*)tw);
+    });
+    if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);
     if (TX < 1) throw Error(AXG_EINVAL, "column FFT: N too large for the LDS tile");
     dim3 grid((ncol + TX - 1) / TX, nplanes);
@@ -287,6 +502,12 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
                          const GwStepScalars& g)
 {
     using T2 = typename V2<T>::type;
+    bool done = p2_dispatch(N, [&]<int NN>() {
+        constexpr int TXP = p2_ztx<T, NN>();
+        hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3((NN / 2 + 1 + TXP - 1) / TXP, NN), dim3(TXP * (NN / 8)),
+                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, pitch, (const T2*)tw, g);
+    });
+    if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);
     if (TX < 1) throw Error(AXG_EINVAL, "z pass: N too large for the LDS tile");
     int nh = N / 2 + 1;
