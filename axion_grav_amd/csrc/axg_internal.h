@@ -59,6 +59,12 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
                        const void* cphi_in, void* cphi_out, void* cpi, void* src,
                        const KickScalars& k, double dspa, double dt_drift, bool gw);
 
+// Fused kick + source + x transform + drift for N = 2^k; false (nothing launched) otherwise.
+template <typename T, bool COMP>
+bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
+                             void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
+                             double dspa, double dt_drift);
+
 template <typename T>
 void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan,
                      const void* tw);

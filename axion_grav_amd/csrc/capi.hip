@@ -2,6 +2,7 @@
 // scalars they need.  Every device buffer belongs to the handle; every entry point runs on
 // the handle's stream and synchronizes before returning.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -104,6 +105,7 @@ struct axg_handle {
     void* tw = nullptr;      // twiddles in the storage precision
     void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
     int cur = 0;
+    bool fuse_x = true;      // x transform fused into the stencil pass when N = 2^k
     // diagnostics
     uint16_t* gw_bin = nullptr;
     uint16_t* spec_bin = nullptr;
@@ -340,6 +342,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         h->comp = p.precision == AXG_FP32C;
         h->t2 = h->fp32 ? 8 : 16;
         h->plan = make_plan(N);
+        // tuning knob: AXG_XFUSE=0 runs the x transform as its own pass (same results)
+        if (const char* e = getenv("AXG_XFUSE")) h->fuse_x = strcmp(e, "0") != 0;
         AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int64_t n3 = h->n3;
         for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
@@ -349,7 +353,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
             h->cpi = h->alloc(n3 * 4);
         }
         if (p.grav_spec) {
-            h->src = h->alloc(3 * n3 * h->t2);
+            // the packed source planes only exist when the x transform is not fused
+            if ((N & (N - 1)) != 0 || !h->fuse_x) h->src = h->alloc(3 * n3 * h->t2);
             h->W = h->alloc((size_t)6 * N * N * h->P * h->t2);
             h->A = h->alloc((size_t)4 * N * N * h->P * h->t2);
             AXG_HIP(hipMemsetAsync(h->A, 0, (size_t)4 * N * N * h->P * h->t2, h->stream));
@@ -537,12 +542,22 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
         KickScalars k = kick_scalars(p, t, dt);
         double dtd = (it == nt) ? dt / 2 : dt;
         int nxt = 1 - h->cur;
-        h->timed(1, [&] {
-            launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
-                                       h->cphi[nxt], h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
-        });
+        bool fused = false;
+        if (p.grav_spec && h->fuse_x) {
+            h->timed(1, [&] {
+                fused = launch_kick_source_xfft<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
+                                                         h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k,
+                                                         p.dspa, dtd);
+            });
+        }
+        if (!fused) {
+            h->timed(1, [&] {
+                launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
+                                           h->cphi[nxt], h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
+            });
+        }
         if (p.grav_spec) {
-            h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
+            if (!fused) h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
             h->timed(3, [&] {
                 launch_fft_cols<T, +1>(h->stream, N, h->W, h->P, (int64_t)N * h->P, 6 * N, h->nh, h->plan, h->tw);
             });

@@ -73,6 +73,42 @@ __device__ __forceinline__ void p2_exchange(typename V2<T>::type (&v)[8], int j0
     __syncthreads();
 }
 
+// Variant for blocks where only some threads carry a transform: inactive threads skip the
+// arithmetic and LDS traffic but take part in every barrier.
+template <typename T, int N, int S>
+__device__ __forceinline__ void p2_exchange_if(bool active, typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s,
+                                               int es, int off)
+{
+    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int j = j0 + m * TPT;
+            const int k = j & (Ns - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) s[(base + r * Ns) * es + off] = v[m + r * M];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = s[(j0 + e * TPT) * es + off];
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int SIGN, int S = 0>
+__device__ __forceinline__ void fft_p2_if(bool active, typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s,
+                                          int es, int off, const typename V2<T>::type* __restrict__ tw)
+{
+    if constexpr (S < P2<N>::NST) {
+        if (active) p2_stage<T, N, S, SIGN>(v, j0, tw);
+        if constexpr (S + 1 < P2<N>::NST) p2_exchange_if<T, N, S>(active, v, j0, s, es, off);
+        fft_p2_if<T, N, SIGN, S + 1>(active, v, j0, s, es, off, tw);
+    }
+}
+
 template <typename T, int N, int SIGN, int S = 0>
 __device__ __forceinline__ void fft_p2(typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s, int es, int off,
                                        const typename V2<T>::type* __restrict__ tw)

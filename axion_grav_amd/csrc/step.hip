@@ -7,6 +7,7 @@
 // and (with COMP) an int16 residual per real that carries the bits below the FP32 value;
 // all arithmetic runs in FP64 registers.
 #include "axg_internal.h"
+#include "fft_p2.h"
 
 namespace axg {
 
@@ -125,11 +126,146 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
     AXG_HIP(hipGetLastError());
 }
 
+
+// ------------------------------------------------------------------ fused pass for N = 2^L
+//
+// The same kick + source + drift, with the x transform of the three packed source planes
+// done in the same block (fft_p2.h): block = XR rows y at one z with one thread per site
+// (XR*N = 1024 threads), so the stencil phase has the memory parallelism of the plain
+// kernel.  Each thread leaves its packed sources C_m = S_a + i S_b in LDS (3 x XR x N
+// complex); then 3*XR*N/8 of the threads transform the 3*XR rows in place and every thread
+// takes part in splitting them into the two real sources' half spectra
+//   S_a^(k) = (X(k) + conj(X(N-k)))/2,  S_b^(k) = (X(k) - conj(X(N-k)))/(2i),  k = 0..N/2,
+// written to W[2m] and W[2m+1] (the y and z passes follow).  The six source reals per site
+// never reach HBM.
+
+template <int N> constexpr int fused_rows() { return N >= 1024 ? 1 : (1024 / N < N ? 1024 / N : N); }
+
+template <typename T, bool COMP, int N>
+__global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
+    const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
+    typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
+    short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, int P, const typename V2<T>::type* __restrict__ tw,
+    KickScalars k, double inv_d, double dt_drift)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    constexpr int TPT = N / 8, XR = fused_rows<N>(), NR = 3 * XR;  // NR rows to transform
+    constexpr int BS = XR * N;                                       // threads per block
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
+    const int64_t NN = (int64_t)N * N;
+    const int z = blockIdx.y;
+    {
+        const int rr = threadIdx.x / N, x = threadIdx.x % N;
+        const int y = blockIdx.x * XR + rr;
+        const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
+        const int zm = (z - 1) & (N - 1), zp = (z + 1) & (N - 1);
+        const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+        const int64_t row = (int64_t)z * NN + (int64_t)y * N;
+        const int64_t i = row + x;
+        double2 c = ldf<T, COMP>(phi_in, cphi_in, i);
+        double2 a1 = ldf<T, COMP>(phi_in, cphi_in, row + xm);
+        double2 a2 = ldf<T, COMP>(phi_in, cphi_in, row + xp);
+        double2 b1 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)ym * N + x);
+        double2 b2 = ldf<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)yp * N + x);
+        double2 c1v = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x);
+        double2 c2v = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zp * NN + (int64_t)y * N + x);
+        double2 p = ldf<T, COMP>(pi, cpi, i);
+        // evolve.f90:325-330 with potential.f90:43-60, in the reference's order
+        double lr = a1.x + a2.x + b1.x + b2.x + c1v.x + c2v.x - 6 * c.x;
+        double li = a1.y + a2.y + b1.y + b2.y + c1v.y + c2v.y - 6 * c.y;
+        double s2 = c.x * c.x + c.y * c.y;
+        double g = 2 * s2 + k.m2;
+        double dr = g * c.x - k.ma2 / 2;
+        double di = g * c.y;
+        if (k.kt != 0.0) {
+            dr = dr - k.kt * c.x;
+            di = di + k.kt * c.y;
+        }
+        p.x = k.f1 * p.x + k.f2 * lr + k.f3 * dr;
+        p.y = k.f1 * p.y + k.f2 * li + k.f3 * di;
+        stf<T, COMP>(pi, cpi, i, p);
+        double2 f = {c.x + dt_drift * p.x, c.y + dt_drift * p.y};
+        stf<T, COMP>(phi_out, cphi_out, i, f);
+        // grav_waves.f90:331-351 gradient (times 1/(2 dspa): exactly the division when dspa is
+        // a power of two) and 213-223 source products
+        double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+        double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+        double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
+        s[rr * N + x] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
+        s[(XR + rr) * N + x] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
+        s[(2 * XR + rr) * N + x] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
+    }
+    __syncthreads();
+    // transform the NR rows in place: row q = threadIdx / TPT
+    {
+        const int q = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+        const bool active = q < NR;
+        T2* rowp = s + (active ? q : 0) * N;
+        T2 v[8];
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = rowp[j0 + e * TPT];
+        }
+        __syncthreads();
+        fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, tw);
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) rowp[j0 + e * TPT] = v[e];
+        }
+        __syncthreads();
+    }
+    // split every transformed row into two half spectra: NR rows x (N/2+1) outputs
+    constexpr int NH = N / 2 + 1;
+    const int64_t cs = NN * P;  // component stride of W
+    for (int t = threadIdx.x; t < NR * NH; t += BS) {
+        const int q = t / NH, kk = t - q * NH;
+        const int m = q / XR, rr = q - m * XR;
+        const int y = blockIdx.x * XR + rr;
+        const T2* rowp = s + q * N;
+        T2 xk = rowp[kk];
+        T2 xm = rowp[(N - kk) & (N - 1)];
+        T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * P;
+        wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+        wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+    }
+}
+
+template <typename T, bool COMP>
+bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
+                             void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
+                             double dspa, double dt_drift)
+{
+    using T2 = typename V2<T>::type;
+    auto go = [&]<int NN>() {
+        constexpr int XR = fused_rows<NN>();
+        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN>), dim3(NN / XR, NN), dim3(XR * NN),
+                           (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
+                           (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
+                           1.0 / (dspa * 2), dt_drift);
+    };
+    switch (N) {
+    case 16: go.template operator()<16>(); break;
+    case 32: go.template operator()<32>(); break;
+    case 64: go.template operator()<64>(); break;
+    case 128: go.template operator()<128>(); break;
+    case 256: go.template operator()<256>(); break;
+    case 512: go.template operator()<512>(); break;
+    case 1024: go.template operator()<1024>(); break;
+    default: return false;
+    }
+    AXG_HIP(hipGetLastError());
+    return true;
+}
+
 #define AXG_INST_STEP(T, C)                                                                                  \
     template void launch_drift<T, C>(hipStream_t, int, const void*, void*, const void*, const void*, void*,   \
                                      const void*, double);                                                   \
     template void launch_kick_drift<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*,    \
-                                          void*, void*, const KickScalars&, double, double, bool);
+                                          void*, void*, const KickScalars&, double, double, bool);       \
+    template bool launch_kick_source_xfft<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*, \
+                                                void*, void*, int, const void*, const KickScalars&, double, double);
 AXG_INST_STEP(double, false)
 AXG_INST_STEP(float, false)
 AXG_INST_STEP(float, true)

@@ -83,6 +83,27 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed):
                 finite=bool(np.all(np.isfinite(spec["numer"]))))
 
 
+def copy_probe(nbytes=4 << 30, reps=10):
+    """Device-to-device copy of nbytes (read + write = 2 nbytes per copy): the achievable
+    streaming bandwidth on this GPU, measured with torch's copy kernel."""
+    import torch
+
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    gbs = 2 * nbytes * reps / el / 1e9
+    log(f"copy probe: {gbs:.0f} GB/s ({nbytes >> 20} MiB x {reps})")
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def run_cpu(N, L, seed):
     import oracle as O
 
@@ -107,6 +128,9 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
+                         "its kernel is the known-size reference for the FETCH/WRITE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the lattice lives in one MI355X's HBM and this build has no "
@@ -115,6 +139,7 @@ def main():
 
     from axion_grav_amd import capi
 
+    copy_gbs = copy_probe() if args.copy_probe else None
     N, L = args.n, args.box
     log(f"FP32C {N}^3 L={L}: warmup {args.warmup}, timed {args.steps} steps")
     c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed)
@@ -164,6 +189,7 @@ def main():
                                         "step_model_frac": round(MODEL_REALS * 8 * d["su_s"] / 8e12, 4)},
         "fp32c_over_fp64": None if d is None else round(c["su_s"] / d["su_s"], 3),
         "kernels": {k: {"ms": round(v["ms_avg"], 4), "GB/s": round(v["gbs"], 1)} for k, v in c["kernels"].items()},
+        "hbm_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
     }
     print(json.dumps(out), flush=True)
 
