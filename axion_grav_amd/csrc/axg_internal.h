@@ -76,17 +76,46 @@ void launch_fft_cols(hipStream_t s, int N, void* data, int64_t es, int64_t ps, i
 template <typename T, int SIGN>
 void launch_fft_rows(hipStream_t s, int N, void* data, int64_t nrows, const FftPlan& plan, const void* tw);
 
+// Layout of the half-spectrum arrays read by the accumulation pass (the y-transformed sources
+// and the accumulators).  zt == 0: rows [c][kz][ky][kx] with pitch P.  zt > 0 (N = 2^k):
+// tiles [c][ky][t][kz][zt] with t = kx / zt, so one accumulation block (one ky, one tile)
+// reads and writes contiguous N*zt runs whatever the tile width.
+struct GwLayout {
+    int N, P, zt, ntile;
+    __host__ __device__ int64_t comp() const
+    {
+        return zt ? (int64_t)N * ntile * N * zt : (int64_t)N * N * P;
+    }
+    __host__ __device__ int64_t idx(int c, int kz, int ky, int kx) const
+    {
+        if (zt == 0) return (int64_t)c * N * N * P + ((int64_t)kz * N + ky) * P + kx;
+        return (((int64_t)c * N + ky) * ntile + kx / zt) * N * zt + (int64_t)kz * zt + (kx % zt);
+    }
+};
+GwLayout gw_layout(int N, bool fp32);
+
 struct GwStepScalars {
     double tau, dw, dwav;
     int fix_swap;
 };
+// y transform of W ([c][z][y][kx], pitch P): in place for the row layout, or out of place
+// into W2 in the tiled layout.
 template <typename T>
-void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const FftPlan& plan,
-                         const void* tw, const GwStepScalars& g);
+void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
+                     const void* tw);
 
 template <typename T>
-void launch_gw_bin(hipStream_t s, int N, int pitch, const void* A, const uint16_t* bin_of_n2, int nb,
+void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
+                         const FftPlan& plan, const void* tw, const GwStepScalars& g);
+
+template <typename T>
+void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int nb,
                    double* partial, int nwg, double* out);
+
+// one accumulator component between the device layout and the compact [kz][ky][kx<=N/2]
+// FP64 form (dir > 0: device -> compact)
+template <typename T>
+void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, double2* compact, int dir);
 
 // diagnostics
 template <typename T, bool COMP>

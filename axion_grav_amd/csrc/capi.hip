@@ -101,7 +101,9 @@ struct axg_handle {
     void* cpi = nullptr;
     void* src = nullptr;
     void* W = nullptr;
+    void* W2 = nullptr;      // y-transformed sources in the tiled layout (N = 2^k)
     void* A = nullptr;
+    GwLayout L{};            // layout of W2 (or W) and A as the accumulation pass reads them
     void* tw = nullptr;      // twiddles in the storage precision
     void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
     int cur = 0;
@@ -355,9 +357,11 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         if (p.grav_spec) {
             // the packed source planes only exist when the x transform is not fused
             if ((N & (N - 1)) != 0 || !h->fuse_x) h->src = h->alloc(3 * n3 * h->t2);
+            h->L = gw_layout(N, h->fp32);
             h->W = h->alloc((size_t)6 * N * N * h->P * h->t2);
-            h->A = h->alloc((size_t)4 * N * N * h->P * h->t2);
-            AXG_HIP(hipMemsetAsync(h->A, 0, (size_t)4 * N * N * h->P * h->t2, h->stream));
+            if (h->L.zt) h->W2 = h->alloc((size_t)6 * h->L.comp() * h->t2);
+            h->A = h->alloc((size_t)4 * h->L.comp() * h->t2);
+            AXG_HIP(hipMemsetAsync(h->A, 0, (size_t)4 * h->L.comp() * h->t2, h->stream));
         }
         // twiddles e^{+2 pi i m / N}
         {
@@ -488,16 +492,12 @@ extern "C" int axg_get_gw_state(axg_handle* h, double* A, int64_t count)
         const int N = h->N;
         require(h->A != nullptr, "grav_spec is off: no GW state");
         require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
-        size_t rows = (size_t)4 * N * N;
-        if (!h->fp32) {
-            AXG_HIP(hipMemcpy2DAsync(A, h->nh * 16, h->A, h->P * 16, h->nh * 16, rows, hipMemcpyDeviceToHost, h->stream));
+        const size_t per = (size_t)N * N * h->nh;  // complex per component
+        for (int m = 0; m < 4; m++) {
+            if (h->fp32) launch_gw_pack<float>(h->stream, N, h->L, h->A, m, h->work, +1);
+            else launch_gw_pack<double>(h->stream, N, h->L, h->A, m, h->work, +1);
+            AXG_HIP(hipMemcpyAsync(A + 2 * per * m, h->work, per * 16, hipMemcpyDeviceToHost, h->stream));
             AXG_HIP(hipStreamSynchronize(h->stream));
-        } else {
-            std::vector<float> tmp(rows * h->nh * 2);
-            AXG_HIP(hipMemcpy2DAsync(tmp.data(), h->nh * 8, h->A, h->P * 8, h->nh * 8, rows, hipMemcpyDeviceToHost,
-                                     h->stream));
-            AXG_HIP(hipStreamSynchronize(h->stream));
-            for (size_t i = 0; i < tmp.size(); i++) A[i] = tmp[i];
         }
     });
 }
@@ -508,15 +508,11 @@ extern "C" int axg_set_gw_state(axg_handle* h, const double* A, int64_t count)
         const int N = h->N;
         require(h->A != nullptr, "grav_spec is off: no GW state");
         require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
-        size_t rows = (size_t)4 * N * N;
-        if (!h->fp32) {
-            AXG_HIP(hipMemcpy2DAsync(h->A, h->P * 16, A, h->nh * 16, h->nh * 16, rows, hipMemcpyHostToDevice, h->stream));
-            AXG_HIP(hipStreamSynchronize(h->stream));
-        } else {
-            std::vector<float> tmp(rows * h->nh * 2);
-            for (size_t i = 0; i < tmp.size(); i++) tmp[i] = (float)A[i];
-            AXG_HIP(hipMemcpy2DAsync(h->A, h->P * 8, tmp.data(), h->nh * 8, h->nh * 8, rows, hipMemcpyHostToDevice,
-                                     h->stream));
+        const size_t per = (size_t)N * N * h->nh;
+        for (int m = 0; m < 4; m++) {
+            AXG_HIP(hipMemcpyAsync(h->work, A + 2 * per * m, per * 16, hipMemcpyHostToDevice, h->stream));
+            if (h->fp32) launch_gw_pack<float>(h->stream, N, h->L, h->A, m, h->work, -1);
+            else launch_gw_pack<double>(h->stream, N, h->L, h->A, m, h->work, -1);
             AXG_HIP(hipStreamSynchronize(h->stream));
         }
     });
@@ -558,15 +554,14 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
         }
         if (p.grav_spec) {
             if (!fused) h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
-            h->timed(3, [&] {
-                launch_fft_cols<T, +1>(h->stream, N, h->W, h->P, (int64_t)N * h->P, 6 * N, h->nh, h->plan, h->tw);
-            });
+            h->timed(3, [&] { launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw); });
             GwStepScalars g;
             g.tau = t;
             g.dw = t * dt * p.dv_spa * (p.sigma_cr * p.sigma_cr);  // grav_waves.f90:209
             g.dwav = p.dwav;
             g.fix_swap = p.gw_fix_swap;
-            h->timed(4, [&] { launch_gw_zpass_acc<T>(h->stream, N, h->P, h->W, h->A, h->plan, h->tw, g); });
+            const void* wz = h->L.zt ? h->W2 : h->W;
+            h->timed(4, [&] { launch_gw_zpass_acc<T>(h->stream, N, h->P, wz, h->A, h->L, h->plan, h->tw, g); });
         }
         h->cur = nxt;
         t = t + dtd;
@@ -632,8 +627,8 @@ extern "C" int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, doubl
         const int nwg = 1024;
         require((size_t)(nwg + 1) * 2 * nb * sizeof(double) <= h->red_bytes, "reduction scratch too small");
         double* out = h->red + (size_t)nwg * 2 * nb;
-        if (h->fp32) launch_gw_bin<float>(h->stream, h->N, h->P, h->A, h->gw_bin, nb, h->red, nwg, out);
-        else launch_gw_bin<double>(h->stream, h->N, h->P, h->A, h->gw_bin, nb, h->red, nwg, out);
+        if (h->fp32) launch_gw_bin<float>(h->stream, h->N, h->L, h->A, h->gw_bin, nb, h->red, nwg, out);
+        else launch_gw_bin<double>(h->stream, h->N, h->L, h->A, h->gw_bin, nb, h->red, nwg, out);
         std::vector<double> r(2 * nb);
         AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));

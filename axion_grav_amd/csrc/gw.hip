@@ -90,6 +90,33 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
     }
 }
 
+// y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled
+// layout of the accumulation pass.  Block = TX kx columns of one (c, z) plane.
+template <typename T, int N>
+__global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
+                                                                             typename V2<T>::type* __restrict__ W2,
+                                                                             int P, GwLayout L,
+                                                                             const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int TX = p2_tx<T, N>(), TPT = N / 8, NH = N / 2 + 1;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
+    const int kx = blockIdx.x * TX + b;
+    const int c = blockIdx.y / N, z = blockIdx.y % N;
+    const bool ok = kx < NH;
+    const T2* base = W + ((int64_t)c * N + z) * (int64_t)N * P + kx;
+    T2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
+    fft_p2<T, N, +1>(v, j0, s, TX, b, tw);
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) W2[L.idx(c, z, j0 + e * TPT, kx)] = v[e];
+    }
+}
+
 // The accumulation pass keeps 8 k-points' projected sums per thread: at most 512 threads per
 // block so the compiler has 256 VGPRs.
 template <typename T, int N> constexpr int p2_ztx()
@@ -101,7 +128,7 @@ template <typename T, int N> constexpr int p2_ztx()
 
 template <typename T, int N>
 __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
-                                                                           typename V2<T>::type* __restrict__ A, int P,
+                                                                           typename V2<T>::type* __restrict__ A, GwLayout L,
                                                                            const typename V2<T>::type* __restrict__ tw,
                                                                            GwStepScalars g)
 {
@@ -111,11 +138,13 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
     extern __shared__ unsigned char smem_raw[];
     T2* s = reinterpret_cast<T2*>(smem_raw);
     const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
+    // the block is one tile (L.zt == TX columns) of one ky: contiguous N*TX runs
     const int kx = blockIdx.x * TX + b, ky = blockIdx.y;
     const bool ok = kx < NH;
-    const int64_t NN = (int64_t)N * N;
-    const int64_t es = (int64_t)N * P;
     const int nx = wrapn(kx, N), ny = wrapn(ky, N);
+    const int64_t tile0 = L.idx(0, 0, ky, blockIdx.x * TX) + b;
+    const int64_t cst = L.comp();
+    constexpr int zs = TX;
 
     // polarization geometry of grav_waves.f90:165-183, p1 = (qz c, -s), p2 = (qz s, c),
     // p3 = (-r, 0) with qhat = n/|n|, r = |qhat_xy|, c = qhat_x/r = nx/rho, s = ny/rho
@@ -144,12 +173,11 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
     T2 sp[8], sn[8], v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) sp[e] = sn[e] = T2{0, 0};
-    const T2* colp = W + (int64_t)ky * P + kx;
 #pragma unroll 1
     for (int c = 0; c < 6; c++) {
-        const T2* base = colp + (int64_t)c * NN * P;
+        const T2* base = W + c * cst + tile0;
 #pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * es] : T2{0, 0};
+        for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
         if (c > 0) __syncthreads();
         fft_p2<T, N, +1>(v, j0, s, TX, b, tw);
         const int ii = c == 0 ? 0 : (c <= 2 ? 1 : 2);
@@ -171,7 +199,7 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
         }
     }
     if (!ok) return;
-    const int64_t ms = NN * P;
+    const int64_t ms = cst;
     const T dw = (T)g.dw;
     const int64_t rho2 = (int64_t)nx * nx + (int64_t)ny * ny;
 #pragma unroll
@@ -192,7 +220,7 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
         }
         const T2 SP = {sp[e].x * dw, sp[e].y * dw};
         const T2 SN = {sn[e].x * dw, sn[e].y * dw};
-        const int64_t ai = (int64_t)kz * es + (int64_t)ky * P + kx;
+        const int64_t ai = tile0 + (int64_t)kz * zs;
         T2 a11 = A[ai], a22 = A[3 * ms + ai], a12 = A[2 * ms + ai];
         A[ai] = T2{a11.x + cq * SP.x, a11.y + cq * SP.y};
         A[3 * ms + ai] = T2{a22.x + sq * SN.x, a22.y + sq * SN.y};
@@ -497,15 +525,47 @@ __global__ void __launch_bounds__(ZNT) gw_zpass_acc_kernel(int N, int P, const t
     }
 }
 
+GwLayout gw_layout(int N, bool fp32)
+{
+    GwLayout L{N, half_pitch(N), 0, 0};
+    int zt = 0;
+    p2_dispatch(N, [&]<int NN>() { zt = fp32 ? p2_ztx<float, NN>() : p2_ztx<double, NN>(); });
+    if (zt) {
+        L.zt = zt;
+        L.ntile = (N / 2 + 1 + zt - 1) / zt;
+    }
+    return L;
+}
+
 template <typename T>
-void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const FftPlan& plan, const void* tw,
-                         const GwStepScalars& g)
+void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
+                     const void* tw)
 {
     using T2 = typename V2<T>::type;
-    bool done = p2_dispatch(N, [&]<int NN>() {
+    if (L.zt) {
+        bool done = p2_dispatch(N, [&]<int NN>() {
+            constexpr int TXP = p2_tx<T, NN>();
+            hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN>), dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN),
+                               dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, pitch, L,
+                               (const T2*)tw);
+        });
+        if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k");
+        AXG_HIP(hipGetLastError());
+        return;
+    }
+    launch_fft_cols<T, +1>(s, N, W, pitch, (int64_t)N * pitch, 6 * N, N / 2 + 1, plan, tw);
+}
+
+template <typename T>
+void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
+                         const FftPlan& plan, const void* tw, const GwStepScalars& g)
+{
+    using T2 = typename V2<T>::type;
+    bool done = L.zt && p2_dispatch(N, [&]<int NN>() {
         constexpr int TXP = p2_ztx<T, NN>();
-        hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3((NN / 2 + 1 + TXP - 1) / TXP, NN), dim3(TXP * (NN / 8)),
-                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, pitch, (const T2*)tw, g);
+        if (L.zt != TXP) throw Error(AXG_EINVAL, "accumulation layout does not match the z pass tile");
+        hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3(L.ntile, NN), dim3(TXP * (NN / 8)),
+                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, L, (const T2*)tw, g);
     });
     if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);
@@ -528,7 +588,7 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
 // mirrored half), 1 on the kx = 0 and kx = N/2 planes; k = 0 is excluded (q > 0).
 
 template <typename T>
-__global__ void gw_bin_kernel(int N, int P, const typename V2<T>::type* __restrict__ A,
+__global__ void gw_bin_kernel(int N, GwLayout L, const typename V2<T>::type* __restrict__ A,
                               const uint16_t* __restrict__ bin_of_n2, int nb, double* __restrict__ partial)
 {
     using T2 = typename V2<T>::type;
@@ -537,7 +597,7 @@ __global__ void gw_bin_kernel(int N, int P, const typename V2<T>::type* __restri
     double* v = sh + 2 * nb;        // N/2+1 values
     int* bb = (int*)(v + (N / 2 + 1));
     const int nh = N / 2 + 1;
-    const int64_t NN = (int64_t)N * N, ms = NN * P;
+    const int64_t NN = (int64_t)N * N, ms = L.comp();
     for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) bins[b] = 0;
     __syncthreads();
     for (int64_t row = blockIdx.x; row < NN; row += gridDim.x) {
@@ -549,7 +609,7 @@ __global__ void gw_bin_kernel(int N, int P, const typename V2<T>::type* __restri
             int b = (n2 > 0) ? bin_of_n2[n2] : 0;
             double s = 0;
             if (b > 0) {
-                int64_t ai = (int64_t)kz * N * P + (int64_t)ky * P + kx;
+                int64_t ai = L.idx(0, kz, ky, kx);
                 for (int m = 0; m < 4; m++) {
                     T2 a = A[m * ms + ai];
                     s += (double)a.x * a.x + (double)a.y * a.y;
@@ -588,16 +648,41 @@ __global__ void sum_partials_kernel(const double* __restrict__ partial, int nwg,
 }
 
 template <typename T>
-void launch_gw_bin(hipStream_t s, int N, int pitch, const void* A, const uint16_t* bin_of_n2, int nb, double* partial,
-                   int nwg, double* out)
+void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int nb,
+                   double* partial, int nwg, double* out)
 {
     using T2 = typename V2<T>::type;
     int nh = N / 2 + 1;
     size_t lds = (size_t)(2 * nb + nh) * sizeof(double) + (size_t)nh * sizeof(int);
-    hipLaunchKernelGGL((gw_bin_kernel<T>), dim3(nwg), dim3(256), lds, s, N, pitch, (const T2*)A, bin_of_n2, nb,
+    hipLaunchKernelGGL((gw_bin_kernel<T>), dim3(nwg), dim3(256), lds, s, N, L, (const T2*)A, bin_of_n2, nb,
                        partial);
     AXG_HIP(hipGetLastError());
     hipLaunchKernelGGL(sum_partials_kernel, dim3((2 * nb + 255) / 256), dim3(256), 0, s, partial, nwg, 2 * nb, out);
+    AXG_HIP(hipGetLastError());
+}
+
+// accumulator component m <-> compact FP64 [kz][ky][kx], kx = 0..N/2
+template <typename T>
+__global__ void gw_pack_kernel(int N, GwLayout L, typename V2<T>::type* A, int m, double2* compact, int dir)
+{
+    using T2 = typename V2<T>::type;
+    const int nh = N / 2 + 1;
+    const int64_t tot = (int64_t)N * N * nh;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        int kx = (int)(i % nh);
+        int64_t r = i / nh;
+        int ky = (int)(r % N), kz = (int)(r / N);
+        int64_t a = L.idx(m, kz, ky, kx);
+        if (dir > 0) compact[i] = double2{(double)A[a].x, (double)A[a].y};
+        else A[a] = T2{(T)compact[i].x, (T)compact[i].y};
+    }
+}
+
+template <typename T>
+void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, double2* compact, int dir)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((gw_pack_kernel<T>), dim3(16384), dim3(256), 0, s, N, L, (T2*)A, m, compact, dir);
     AXG_HIP(hipGetLastError());
 }
 
@@ -606,9 +691,13 @@ void launch_gw_bin(hipStream_t s, int N, int pitch, const void* A, const uint16_
     template void launch_fft_cols<T, +1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,   \
                                          const void*);                                                          \
     template void launch_fft_rows<T, +1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);        \
-    template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const FftPlan&, const void*, \
-                                         const GwStepScalars&);                                                 \
-    template void launch_gw_bin<T>(hipStream_t, int, int, const void*, const uint16_t*, int, double*, int, double*);
+    template void launch_gw_ypass<T>(hipStream_t, int, int, void*, void*, const GwLayout&, const FftPlan&,       \
+                                     const void*);                                                              \
+    template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const GwLayout&,             \
+                                         const FftPlan&, const void*, const GwStepScalars&);                    \
+    template void launch_gw_bin<T>(hipStream_t, int, const GwLayout&, const void*, const uint16_t*, int, double*, \
+                                   int, double*);                                                               \
+    template void launch_gw_pack<T>(hipStream_t, int, const GwLayout&, void*, int, double2*, int);
 AXG_INST_GW(double)
 AXG_INST_GW(float)
 template void launch_fft_cols<double, -1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,
