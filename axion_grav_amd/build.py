@@ -13,6 +13,9 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libaxgrav.so")
 SOURCES = ["capi.hip", "step.hip", "gw.hip", "diag.hip"]
+FORTRAN = ["axgrav_capi.f90", "axg_ini.f90", "axg_driver.f90"]
+BINDIR = os.path.join(HERE, "bin")
+FC = os.environ.get("AXG_FC", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AXG_OFFLOAD_ARCH", "gfx950")
@@ -52,7 +55,34 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose=True):
+    """The Fortran host driver (axion_grav_amd/bin/axion_3d) over the ISO_C_BINDING module."""
+    if not os.path.exists(FC):
+        if verbose:
+            print(f"no Fortran compiler at {FC}: skipping the Fortran host driver", file=sys.stderr)
+        return None
+    os.makedirs(BINDIR, exist_ok=True)
+    fdir = os.path.join(HERE, "fortran")
+    objs = []
+    for src in FORTRAN:
+        obj = os.path.join(BUILD, src.replace(".f90", ".o"))
+        cmd = [FC, "-O2", "-c", os.path.join(fdir, src), "-o", obj, "-module-dir", BUILD, "-I", BUILD]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed on {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    exe = os.path.join(BINDIR, "axion_3d")
+    cmd = [FC, "-o", exe, *objs, "-L", LIBDIR, "-laxgrav", f"-Wl,-rpath,$ORIGIN/../lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of axion_3d failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
 
 
 if __name__ == "__main__":

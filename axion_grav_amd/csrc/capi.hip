@@ -723,7 +723,7 @@ extern "C" int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* 
         require(h->have_flags, "call axg_defects first");
         require(count != nullptr, "null count");
         *count = h->nmask;
-        if (h->nmask == 0 || ixyz == nullptr) return;  // size query
+        if (h->nmask == 0 || ixyz == nullptr || cap == 0) return;  // size query
         require(cap >= h->nmask, "mask buffer too small");
         const int64_t nblk = (h->n3 + 255) / 256;
         int64_t* offs = h->blk + 3 * nblk;

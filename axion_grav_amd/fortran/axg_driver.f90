@@ -1,0 +1,483 @@
+! axion_3d on one MI355X: the reference's driver (main.f90), parameter keys (settings.f90
+! 112-297) and checkpoint schedule (evolve.f90 26-152, 154-235, 401-546), with the lattice and
+! its diagnostics resident on the GPU behind the C-ABI of libaxgrav (module axgrav_capi).
+!
+!   axion_3d <param.ini> [-n tag] [-t hours] [-b GB]
+!
+! Output files keep the reference names and layouts: <rootname>_gravspec_<idx>.dat
+! '(5E20.10e3)', _spectrum_<idx>.dat '(4E20.10e3)', _mask_<idx>-0.dat '(3I6)',
+! _string_wall_parameters.txt '(I12,11E20.10e3)', _crho.txt '(I12,3E20.10e3)', the resume
+! file _param.chk ('idx tau flag') and _phi_<idx>.chk (raw FP64 complex phi then pi).
+!
+! Keys this build adds (all optional): precision = fp64 | fp32c, num_bin_grav (25),
+! gw_fix_swap (F), dtau_fixed (0: use get_dtau), potential_kappa0 (0), ic_seed and ic_kmax
+! (the seeded flat-spectrum initial field used for a fresh start), ic_file (raw phi,pi file to
+! start from instead), chk_gw (F: also checkpoint the GW accumulators, which the reference
+! does not), device (0).
+program axion_3d
+  use, intrinsic :: iso_c_binding
+  use axgrav_capi
+  use axg_ini
+  implicit none
+
+  double precision, parameter :: pi_t = 3.14159265359d0, twopi_t = 2*pi_t  ! potential.f90:8
+  double precision, parameter :: mstar_in_hz = 2.435d18*1.5192488833521d24  ! potential.f90:14-23
+  double precision, parameter :: byte_dc = 16d0, hard_limit_cdata = 2*byte_dc  ! settings.f90:29-31
+
+  character(len=1024) :: paramfile, tag, arg, rootdir, rootname, tmprn, tmpdir, file_resume, ss, prec
+  character(len=1024) :: ic_file
+  integer :: nargs, i, ios, n_chk, n_id, num_bin, num_bin_grav, degrade_spec, n_mod_eps, idx, info
+  integer(c_int64_t) :: ngrid, n3, ic_seed
+  logical :: dry_run, initial_loop, id_strings, id_walls, spec_est, grav_spec, group_strings, group_walls
+  logical :: out_strings, out_walls, ok, chk, flag_defects, gw_fix_swap, chk_gw, exists
+  double precision :: tsim_max, limit_memory, b, sigma, g_rel, kappa, tau_ini, tau_fin, eps_dtau, boxsize
+  double precision :: zeta, width_cr, thickness_cr, tau0, dtau_chk, tau_start, tau_end, ic_kmax
+  double precision, allocatable :: tau_id(:)
+  real :: tsim_begin, tsim_now
+  type(axg_keys) :: keys
+  type(axg_params) :: prm
+  type(c_ptr) :: h
+  integer(c_int32_t) :: nsteps
+  integer(8) :: clock0, clock_rate
+
+  ! ------------------------------------------------------------ command line (main.f90:16-33)
+  nargs = command_argument_count()
+  if (nargs > 7) call die('error: too many arguments')
+  if (nargs < 1) call die('error: parameter file is absent')
+  call get_command_argument(1, paramfile)
+  tag = ''
+  tsim_max = 0d0
+  limit_memory = 0d0
+  do i = 2, nargs - 1, 2
+     call get_command_argument(i, arg)
+     select case (trim(adjustl(arg)))
+     case ('-n')
+        call get_command_argument(i + 1, tag)
+     case ('-t')
+        call get_command_argument(i + 1, arg)
+        read(arg, *) tsim_max
+     case ('-b')
+        call get_command_argument(i + 1, arg)
+        read(arg, *) limit_memory
+     end select
+  end do
+  call system_clock(clock0, clock_rate)
+  tsim_begin = 0.0
+
+  ! ------------------------------------------------------------ keys (settings.f90:112-297)
+  write(*, '(A)') ' #initialization'
+  call ini_open(trim(paramfile), ok)
+  if (.not. ok) then
+     write(*, *) 'Ini_Open: Error opening file '//trim(paramfile)
+     stop 1
+  end if
+  dry_run = ini_logical('dry_run')
+  rootdir = ini_string('rootdir')
+  if (len_trim(rootdir) == 0) then
+     rootdir = '/'
+  else if (rootdir(len_trim(rootdir):len_trim(rootdir)) /= '/') then
+     rootdir = trim(rootdir)//'/'
+  end if
+  tmprn = ini_string('rootname')
+  rootname = trim(rootdir)//trim(tmprn)
+  if (tag /= '') then
+     tmprn = trim(tmprn)//'_'//trim(tag)
+     rootname = trim(rootname)//'_'//trim(tag)
+  end if
+  write(*, '(A)') ' rootname = '//trim(rootname)
+  tmpdir = ini_string('tmpdir')
+  if (tmpdir /= '') then
+     if (tmpdir(len_trim(tmpdir):len_trim(tmpdir)) /= '/') tmpdir = trim(tmpdir)//'/'
+  else
+     tmpdir = rootdir
+  end if
+  ss = ini_string('ss')
+  ngrid = ini_int('ngrid')
+  if (mod(ngrid, 2_c_int64_t) /= 0) call die('error: ngrid should be even')
+  n3 = ngrid**3
+  n_chk = ini_int('n_chk')
+  if (.not. n_chk >= 1) call die('error: n_chk should be 1 or larger')
+  num_bin = ini_int('num_bin', 50)
+  initial_loop = ini_logical('initial_loop', .false.)
+  if (initial_loop) then
+     b = ini_double('loop_radius')
+     b = ini_double('loop_velocity')
+     b = ini_double('loop_velocity_in')
+  end if
+  id_strings = ini_logical('id_strings')
+  group_strings = .false.
+  out_strings = .false.
+  if (id_strings) then
+     group_strings = ini_logical('group_strings')
+     out_strings = ini_logical('out_strings')
+  end if
+  id_walls = ini_logical('id_walls')
+  group_walls = .false.
+  out_walls = .false.
+  if (id_walls) then
+     group_walls = ini_logical('group_walls')
+     out_walls = ini_logical('out_walls')
+  end if
+  spec_est = ini_logical('spec_est')
+  grav_spec = ini_logical('grav_spec')
+  n_id = 0
+  n_mod_eps = 0
+  if (id_strings .or. id_walls .or. spec_est .or. grav_spec) then
+     n_id = ini_int('n_id')
+     if (spec_est) n_mod_eps = ini_int('n_mod_eps')
+  end if
+  degrade_spec = 0
+  if (spec_est .or. grav_spec) then
+     allocate(tau_id(0:n_chk))
+     degrade_spec = ini_int('degrade_spec')
+     if (degrade_spec /= 0) call die('error: degrade_spec > 0 is not supported by this build')
+  end if
+  ! memory estimate (settings.f90:241-260), one task
+  b = byte_dc*2
+  if (spec_est) then
+     b = b + 2*byte_dc + byte_dc
+  else if (id_strings .or. id_walls) then
+     b = b + 1d0
+  end if
+  b = b*dble(n3)/1024/1024/1024
+  write(*, '(A,F12.4)') ' approx required total memory (GB) (without margin) = ', b
+  if (b >= 2*hard_limit_cdata .and. .not. dry_run) call die('error: to large cdata; increase the number of tasks')
+  b = b + 2*byte_dc*2*dble(ngrid*ngrid)/1024/1024/1024
+  if (b >= limit_memory .and. .not. dry_run) call die('error: not enough memory')
+  file_resume = trim(tmpdir)//trim(tmprn)//'_param.chk'
+  sigma = ini_double('sigma')
+  g_rel = ini_double('g_rel')
+  kappa = ini_double('kappa')
+  tau_ini = ini_double('tau_ini')
+  tau_fin = ini_double('tau_fin')
+  eps_dtau = ini_double('eps_dtau')
+  boxsize = ini_double('boxsize')
+  ! keys of this build
+  prec = ini_string('precision')
+  num_bin_grav = ini_int('num_bin_grav', 25)
+  gw_fix_swap = ini_logical('gw_fix_swap', .false.)
+  chk_gw = ini_logical('chk_gw', .false.)
+  ic_seed = ini_int('ic_seed', 1234)
+  ic_kmax = ini_double('ic_kmax', 1d0)
+  ic_file = ini_string('ic_file')
+
+  keys = axg_keys(sigma, g_rel, kappa, tau_fin, boxsize, ngrid)
+  call axg_check(axg_derive(keys, prm, zeta, width_cr, thickness_cr), c_null_ptr, 'axg_derive')
+  prm%precision = AXG_FP64
+  if (trim(prec) == 'fp32c') prm%precision = AXG_FP32C
+  prm%grav_spec = merge(1, 0, grav_spec)
+  prm%initial_loop = merge(1, 0, initial_loop)
+  prm%gw_fix_swap = merge(1, 0, gw_fix_swap)
+  prm%num_bin_grav = num_bin_grav
+  prm%num_bin = num_bin
+  prm%id_strings = merge(1, 0, id_strings)
+  prm%id_walls = merge(1, 0, id_walls)
+  prm%spec_est = merge(1, 0, spec_est)
+  prm%device = ini_int('device', 0)
+  prm%eps_dtau = eps_dtau
+  prm%dtau_fixed = ini_double('dtau_fixed', 0d0)
+  prm%potential_kappa0 = ini_double('potential_kappa0', 0d0)
+  call ini_close()
+  write(*, '(A,I0)') ' ngrid = ', ngrid
+  write(*, '(A,ES22.14)') ' sigma_cr = ', prm%sigma_cr
+  write(*, '(A,ES22.14)') ' delta_lattice (in units of tau_cr) = ', prm%dspa
+  if (dry_run) then
+     write(*, '(A)') ' *** dry run just for checking parameters ***'
+     stop
+  end if
+
+  call axg_check(axg_create(prm, h), c_null_ptr, 'axg_create')
+
+  ! ------------------------------------------------------------ initial field (initial.f90:21-78)
+  call set_initial_condition(chk)
+
+  ! ------------------------------------------------------------ evolution (evolve.f90:26-152)
+  tau0 = max(tau_ini, 1d0)
+  dtau_chk = (tau_fin - tau0)/n_chk
+  if (tau_ini + dtau_chk < 1d0) call die(' error: too large n_chk; tau at idx=1 is smaller than tau_at_pt')
+  if (spec_est .or. grav_spec) then
+     tau_id(0) = tau_ini
+     do idx = 1, n_chk
+        tau_id(idx) = tau0 + dtau_chk*idx
+     end do
+  end if
+  if (chk) then
+     call read_chk(tau_end, idx, flag_defects)
+  else
+     tau_end = tau_ini
+     idx = 0
+     flag_defects = .true.
+     tsim_now = wall_hours()
+     if ((tsim_now - tsim_begin)*(1 + 1d0/max(idx, 1)) > tsim_max) call write_chk(tau_end, idx, flag_defects)
+  end if
+  if (idx == 0) then
+     open(300, file=trim(rootname)//'_string_wall_parameters.txt', position='rewind', iostat=info)
+     open(400, file=trim(rootname)//'_crho.txt', position='rewind', iostat=info)
+  else
+     open(300, file=trim(rootname)//'_string_wall_parameters.txt', position='append', iostat=info)
+     open(400, file=trim(rootname)//'_crho.txt', position='append', iostat=info)
+  end if
+  if (info /= 0) call die('error: cannot open file string_wall_parameters.txt')
+  if (idx + n_id > n_chk .and. flag_defects) then
+     call defects_and_spectrum(idx, tau_end)
+     flag_defects = .false.
+     call write_chk(tau_end, idx, flag_defects)
+  end if
+  do while (idx < n_chk)
+     tau_start = tau_end
+     idx = idx + 1
+     tau_end = tau0 + dtau_chk*idx
+     write(*, '(A,I3)') ' #idx = ', idx
+     write(*, '(A,ES22.14,A,ES22.14)') '  tau_start = ', tau_start, ', tau_end = ', tau_end
+     call axg_check(axg_advance(h, tau_start, tau_end, nsteps), h, 'axg_advance')
+     write(*, '(A,I0)') '   iterations: ', nsteps
+     tsim_now = wall_hours()
+     if ((tsim_now - tsim_begin)*(1 + 1d0/idx) > tsim_max) then
+        flag_defects = .true.
+        call write_chk(tau_end, idx, flag_defects)
+     end if
+     if (idx + n_id > n_chk) then
+        call defects_and_spectrum(idx, tau_end)
+        if (tsim_now - tsim_begin > tsim_max) then
+           flag_defects = .false.
+           call write_chk(tau_end, idx, flag_defects)
+        end if
+     end if
+  end do
+  flag_defects = .false.
+  call write_chk(tau_end, idx, flag_defects)
+  close(300)
+  close(400)
+  call axg_check(axg_destroy(h), c_null_ptr, 'axg_destroy')
+  write(*, '(A)') ' ... done'
+
+contains
+
+  ! print and stop with a failure status (the reference's MpiStop aborts the run)
+  subroutine die(msg)
+    character(*), intent(in) :: msg
+    write(*, '(A)') ' '//msg
+    stop 1
+  end subroutine die
+
+  ! wall time since start, in hours (the reference's MKL second()/3600)
+  real function wall_hours()
+    integer(8) :: c
+    call system_clock(c)
+    wall_hours = real(dble(c - clock0)/dble(clock_rate)/3600d0)
+  end function wall_hours
+
+  function itoa(n) result(s)
+    integer, intent(in) :: n
+    character(len=32) :: s
+    write(s, '(I0)') n
+  end function itoa
+
+  function chk_name(n) result(s)
+    integer, intent(in) :: n
+    character(len=1100) :: s
+    s = trim(tmpdir)//trim(tmprn)//'_phi_'//trim(itoa(n))//'.chk'
+  end function chk_name
+
+  function grav_chk_name(n) result(s)
+    integer, intent(in) :: n
+    character(len=1100) :: s
+    s = trim(tmpdir)//trim(tmprn)//'_grav_'//trim(itoa(n))//'.chk'
+  end function grav_chk_name
+
+  ! initial.f90:21-78: resume when the param file and the named chk file exist; otherwise a
+  ! fresh field (ic_file, else the seeded flat spectrum) and an empty param file
+  subroutine set_initial_condition(chk)
+    logical, intent(out) :: chk
+    integer :: u, ios, idx0
+    double precision :: t
+    complex(c_double_complex), allocatable :: phi(:), pi(:)
+    chk = .false.
+    inquire(file=trim(file_resume), exist=exists)
+    if (exists) then
+       open(newunit=u, file=trim(file_resume), action='read', status='old', iostat=ios)
+       read(u, *, iostat=ios) idx0, t
+       close(u)
+       if (ios == 0 .and. idx0 >= 0 .and. idx0 <= n_chk) then
+          inquire(file=trim(chk_name(idx0)), exist=exists)
+          if (exists) then
+             write(*, '(A)') ' *** chk files EXIST!; restart from previous run ***'
+             chk = .true.
+             return
+          end if
+       end if
+    end if
+    write(*, '(A)') ' *generating new initial configuration ...'
+    if (ic_file /= '') then
+       allocate(phi(n3), pi(n3))
+       open(newunit=u, file=trim(ic_file), access='stream', form='unformatted', status='old', action='read', &
+            iostat=ios)
+       if (ios /= 0) call die('error: cannot open ic_file')
+       read(u) phi, pi
+       close(u)
+       call axg_check(axg_set_fields(h, phi, pi, n3), h, 'axg_set_fields')
+       deallocate(phi, pi)
+    else
+       call axg_check(axg_init_flat(h, ic_seed, ic_kmax), h, 'axg_init_flat')
+    end if
+    open(newunit=u, file=trim(file_resume), action='write', status='replace', iostat=ios)
+    close(u)
+    write(*, '(A)') '  *param.chk is created'
+  end subroutine set_initial_condition
+
+  ! evolve.f90:401-437: the param file is rewritten only if it exists (status='old'); fields
+  ! are written when flag_defects, then older chk files are deleted
+  subroutine write_chk(tau, idx, flag_defects)
+    double precision, intent(in) :: tau
+    integer, intent(in) :: idx
+    logical, intent(in) :: flag_defects
+    integer :: u, ios, iu
+    if (flag_defects) call io_chk(.false., idx)
+    open(newunit=u, file=trim(file_resume), action='write', status='old', iostat=ios)
+    if (ios == 0) then
+       write(u, '(I12,1X,ES25.16E3,1X,L2)', iostat=ios) idx, tau, flag_defects
+       close(u)
+    end if
+    if (flag_defects) then
+       do iu = 0, idx - 1
+          open(newunit=u, file=trim(chk_name(iu)), status='old', iostat=ios)
+          if (ios == 0) close(u, status='delete')
+          open(newunit=u, file=trim(grav_chk_name(iu)), status='old', iostat=ios)
+          if (ios == 0) close(u, status='delete')
+       end do
+    end if
+  end subroutine write_chk
+
+  ! evolve.f90:439-453
+  subroutine read_chk(tau, idx, flag_defects)
+    double precision, intent(out) :: tau
+    integer, intent(out) :: idx
+    logical, intent(out) :: flag_defects
+    integer :: u, ios
+    open(newunit=u, file=trim(file_resume), action='read', status='old', iostat=ios)
+    if (ios == 0) read(u, *, iostat=ios) idx, tau, flag_defects
+    if (ios /= 0) call die('error: cannot read param.chk')
+    close(u)
+    call io_chk(.true., idx)
+  end subroutine read_chk
+
+  ! evolve.f90:455-546: raw native FP64 complex, phi[N^3] then pi[N^3], x fastest
+  subroutine io_chk(rd, idx)
+    logical, intent(in) :: rd
+    integer, intent(in) :: idx
+    complex(c_double_complex), allocatable :: phi(:), pi(:), a(:)
+    integer :: u, ios
+    integer(c_int64_t) :: na
+    allocate(phi(n3), pi(n3))
+    if (rd) then
+       open(newunit=u, file=trim(chk_name(idx)), access='stream', form='unformatted', status='old', &
+            action='read', iostat=ios)
+       if (ios /= 0) call die('error: MPI_FILE_OPEN failed')
+       read(u) phi, pi
+       close(u)
+       call axg_check(axg_set_fields(h, phi, pi, n3), h, 'axg_set_fields')
+    else
+       call axg_check(axg_get_fields(h, phi, pi, n3), h, 'axg_get_fields')
+       open(newunit=u, file=trim(chk_name(idx)), access='stream', form='unformatted', status='replace', &
+            action='write', iostat=ios)
+       if (ios /= 0) call die('error: MPI_FILE_OPEN failed')
+       write(u) phi, pi
+       close(u)
+    end if
+    deallocate(phi, pi)
+    if (chk_gw .and. grav_spec) then
+       na = 4*ngrid*ngrid*(ngrid/2 + 1)
+       allocate(a(na))
+       if (rd) then
+          inquire(file=trim(grav_chk_name(idx)), exist=exists)
+          if (exists) then
+             open(newunit=u, file=trim(grav_chk_name(idx)), access='stream', form='unformatted', status='old', &
+                  action='read')
+             read(u) a
+             close(u)
+             call axg_check(axg_set_gw_state(h, a, na), h, 'axg_set_gw_state')
+          end if
+       else
+          call axg_check(axg_get_gw_state(h, a, na), h, 'axg_get_gw_state')
+          open(newunit=u, file=trim(grav_chk_name(idx)), access='stream', form='unformatted', status='replace', &
+               action='write')
+          write(u) a
+          close(u)
+       end if
+       deallocate(a)
+    end if
+  end subroutine io_chk
+
+  ! evolve.f90:154-235 (DIM = 3) with id_defects' outputs, spectrum_estimation and
+  ! grav_spec_estimation writing their files
+  subroutine defects_and_spectrum(idx, tau)
+    integer, intent(in) :: idx
+    double precision, intent(in) :: tau
+    type(axg_defect_out) :: dc
+    double precision :: xi(3), largea(3), vel(3), sw(11), crho(3), rho, xi0, xi1, gam
+    double precision, allocatable :: q(:), pw(:), nu(:), de(:), cv(:), nm(:)
+    integer(c_int32_t), allocatable :: ml(:)
+    integer(c_int64_t) :: nmask, k
+    integer :: u, iq
+    double precision :: freq
+    xi = 0d0
+    largea = 0d0
+    vel = 0d0
+    if (id_strings .or. id_walls .or. spec_est) then
+       call axg_check(axg_defects(h, dc), h, 'axg_defects')
+       write(*, '(A,3I12)') '  MpiRank    0 has found string, wall and mask points: ', dc%n_string, dc%n_wall, dc%n_mask
+       if (spec_est) then
+          allocate(ml(3))
+          call axg_check(axg_get_mask(h, ml, 0_c_int64_t, nmask), h, 'axg_get_mask')
+          deallocate(ml)
+          allocate(ml(3*max(nmask, 1_c_int64_t)))
+          call axg_check(axg_get_mask(h, ml, nmask, nmask), h, 'axg_get_mask')
+          open(newunit=u, file=trim(rootname)//'_mask_'//trim(itoa(idx))//'-0.dat', action='write', status='replace')
+          do k = 1, nmask
+             write(u, '(3I6)') ml(3*k-2), ml(3*k-1), ml(3*k)
+          end do
+          close(u)
+          deallocate(ml)
+       end if
+    end if
+    ! string length, wall area and velocities (defects_3d.f90:276-283) are not computed by this
+    ! build yet: their parameters are reported as zero
+    xi(1:3) = (tau/prm%dspa)**2*xi(1:3)/dble(n3)/4
+    largea(1:3) = (tau/prm%dspa)*largea(1:3)/dble(n3)/2
+    sw = [tau, tau**2*zeta, xi(1:3), largea(1:3), vel(1:3)]
+    crho = 0d0
+    if (spec_est) then
+       allocate(q(num_bin), pw(num_bin), cv(num_bin), nm(num_bin))
+       call axg_check(axg_field_spectrum(h, tau_id(idx), q, pw, cv, nm, rho, num_bin), h, 'axg_field_spectrum')
+       open(newunit=u, file=trim(rootname)//'_spectrum_'//trim(itoa(idx))//'.dat', action='write')
+       do iq = 1, num_bin
+          write(u, '(4E20.10e3)') q(iq), pw(iq), cv(iq)/nm(iq), nm(iq)
+       end do
+       close(u)
+       ! analyze_energy (evolve.f90:216-235)
+       xi0 = sw(5)
+       xi1 = sw(3) - sw(5)
+       gam = sw(11)
+       crho(1) = xi0*twopi_t*log(0.5d0*tau**2/sqrt(xi0)/width_cr)*4*gam
+       crho(2) = xi1*twopi_t*log(0.5d0*tau**2/sqrt(xi0)/width_cr)*4*gam
+       crho(3) = rho
+       deallocate(q, pw, cv, nm)
+    end if
+    if (grav_spec) then
+       allocate(q(num_bin_grav), pw(num_bin_grav), nu(num_bin_grav), de(num_bin_grav))
+       call axg_check(axg_gw_spectrum(h, q, pw, nu, de, num_bin_grav), h, 'axg_gw_spectrum')
+       freq = mstar_in_hz/prm%tau_cr/prm%a_now/twopi_t
+       open(newunit=u, file=trim(rootname)//'_gravspec_'//trim(itoa(idx))//'.dat', action='write')
+       do iq = 1, num_bin_grav
+          write(u, '(5E20.10e3)') tau_id(idx), q(iq), q(iq)*freq, pw(iq), de(iq)
+       end do
+       close(u)
+       deallocate(q, pw, nu, de)
+    end if
+    write(300, '(I12,11E20.10e3)') idx, sw(1:11)
+    write(400, '(I12,3E20.10e3)') idx, crho(1:3)
+    flush(300)
+    flush(400)
+  end subroutine defects_and_spectrum
+
+end program axion_3d

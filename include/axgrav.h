@@ -123,7 +123,7 @@ int axg_defects(axg_handle* h, axg_defect_out* out);
 /* Compacted mask cells in the reference's loop order (z, then y, then x fastest), as
  * 1-based (ix, iy, iz) triplets, the '(3I6)' rows of _mask_<idx>-<rank>.dat
  * (defects_3d.f90:265-273).  cap = room in ixyz (triplets); *count = cells written.  With
- * ixyz == NULL only *count is set (size query). */
+ * ixyz == NULL or cap == 0 only *count is set (size query). */
 int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* count);
 /* The per-cell flag bits (1 string, 2 wall, 4 mask), N^3 int8 in site order. */
 int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite);

@@ -1,0 +1,162 @@
+"""The Fortran host (axion_grav_amd/bin/axion_3d) keeps the reference's command line, ini keys,
+checkpoint schedule and output files, and drives the GPU through the ISO_C_BINDING module.
+CPU tests use dry_run (parameters only); the GPU test compares every output file of a short
+run with the CPU oracle and checks resume from a checkpoint (evolve.f90:439-546)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import ROOT
+
+EXE = os.path.join(ROOT, "axion_grav_amd", "bin", "axion_3d")
+PI_T = 3.14159265359
+SIGMA = float(1 / (6 * PI_T * np.sqrt(0.5)))  # sigma_cr^2 = 1/2 at g_rel = 90
+
+
+def ini_text(rootdir, **over):
+    keys = dict(dry_run="F", rootdir=rootdir, rootname="run", ss="1", ngrid=32, n_chk=2, num_bin=8,
+                initial_loop="T", loop_radius=0.5, loop_velocity=0.0, loop_velocity_in=0.0,
+                id_strings="T", group_strings="F", out_strings="F", id_walls="T", group_walls="F",
+                out_walls="F", spec_est="T", grav_spec="T", n_id=2, n_mod_eps=1, degrade_spec=0,
+                sigma=repr(SIGMA), g_rel=90.0, kappa="1d-30", tau_ini=1.0, tau_fin=2.0, eps_dtau=1.0,
+                boxsize=4.0, ic_seed=77, ic_kmax=6.0)
+    keys.update(over)
+    lines = ["# test run", "COMMENT this line is ignored = 1"] + [f"{k} = {v}" for k, v in keys.items()]
+    lines += ["ngrid = 64", "END", "ngrid = 16"]  # first occurrence wins; END stops reading
+    return "\n".join(lines) + "\n"
+
+
+def run(tmp_path, text, *args, check=True):
+    p = tmp_path / "p.ini"
+    p.write_text(text)
+    r = subprocess.run([EXE, str(p), *args], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    if check and r.returncode != 0:
+        raise AssertionError(r.stdout + r.stderr)
+    return r
+
+
+needs_exe = pytest.mark.skipif(not os.path.exists(EXE), reason="axion_3d not built (no amdflang)")
+
+
+@needs_exe
+def test_dry_run_parses_keys_and_derives(tmp_path):
+    r = run(tmp_path, ini_text(str(tmp_path), dry_run="T"), "-b", "100")
+    assert "dry run" in r.stdout
+    assert " ngrid = 32" in r.stdout  # the first 'ngrid' wins, 'END' stops reading
+    o = O.derive(SIGMA, 90.0, 1e-30, 2.0, 4.0, 32)
+    line = [l for l in r.stdout.splitlines() if "delta_lattice" in l][0]
+    assert float(line.split("=")[1]) == o["dspa"]
+    line = [l for l in r.stdout.splitlines() if "sigma_cr" in l][0]
+    assert abs(float(line.split("=")[1]) - o["sigma_cr"]) < 1e-13
+
+
+@needs_exe
+@pytest.mark.parametrize("key,val,msg", [("dry_run", "TRUE", "error reading logical"),
+                                         ("ngrid", "3.2e1", "error reading integer")])
+def test_bad_values_stop(tmp_path, key, val, msg):
+    over = dict(dry_run="T")
+    over[key] = val
+    r = run(tmp_path, ini_text(str(tmp_path), **over), "-b", "100", check=False)
+    assert r.returncode != 0 and msg in r.stdout + r.stderr
+
+
+@needs_exe
+def test_missing_key_and_memory_check(tmp_path):
+    text = ini_text(str(tmp_path), dry_run="T").replace("sigma = ", "sigmaX = ")
+    r = run(tmp_path, text, "-b", "100", check=False)
+    assert r.returncode != 0 and "key not found : sigma" in r.stdout
+    r = run(tmp_path, ini_text(str(tmp_path)), check=False)  # no -b: settings.f90:260
+    assert r.returncode != 0 and "not enough memory" in r.stdout + r.stderr
+
+
+def read_rows(path):
+    return np.array([[float(x) for x in l.split()] for l in open(path) if l.strip()])
+
+
+def oracle_run(N, P, seed, kmax, taus, eps):
+    """The same schedule with the oracle: IC, then get_dtau + DKD per interval."""
+    phi, pi = O.ic_flat(N, seed, kmax, P["dwav"])
+    A = O.new_gw_state(N)
+    states = {}
+    for idx in range(1, len(taus)):
+        dt, n = O.get_dtau(taus[idx - 1], taus[idx], P["sigma_cr"], P["dwav"], N, eps)
+        O.symplectic(phi, pi, A, taus[idx - 1], dt, n, P)
+        states[idx] = (phi.copy(), pi.copy(), A.copy())
+    return states
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_run_outputs_match_oracle_and_resume(tmp_path):
+    print("[case] fortran driver N=32", flush=True)
+    N, nb, nbg = 32, 8, 25
+    d = O.derive(SIGMA, 90.0, 1e-30, 2.0, 4.0, N)
+    # the oracle's seeded field enters through ic_file, so the fields can be compared bitwise
+    phi0, pi0 = O.ic_flat(N, 77, 6.0, d["dwav"])
+    np.concatenate([phi0.ravel(), pi0.ravel()]).tofile(tmp_path / "ic.raw")
+    run(tmp_path, ini_text(str(tmp_path), ic_file=str(tmp_path / "ic.raw")), "-b", "100")
+    P = dict(d, kappa=1e-30, initial_loop=True)
+    taus = [1.0, 1.5, 2.0]
+    states = oracle_run(N, P, 77, 6.0, taus, 1.0)
+    for idx in (1, 2):
+        phi, pi, A = states[idx]
+        # gravspec: tau_id, q, q f_cr, power, denom (grav_waves.f90:313-314)
+        g = read_rows(tmp_path / f"run_gravspec_{idx}.dat")
+        o = O.gw_spectrum(A, nbg, d["dspa"], d["dwav"], d["tau_cr"], d["a_now"], d["dv_wav"])
+        assert g.shape == (nbg, 5)
+        assert np.allclose(g[:, 0], taus[idx], rtol=1e-10)
+        assert np.allclose(g[:, 1], o["qmid"], rtol=1e-9)
+        assert np.array_equal(g[:, 4], o["denom"])
+        assert np.allclose(g[:, 3], o["power"], rtol=1e-8, atol=0)
+        # mask list (defects_3d.f90:265-273), bit-exact
+        flags, counts = O.classify(phi)
+        ml = read_rows(tmp_path / f"run_mask_{idx}-0.dat").astype(int) if counts[2] else np.zeros((0, 3))
+        zz, yy, xx = np.nonzero(flags & 4)
+        assert np.array_equal(ml, np.stack([xx + 1, yy + 1, zz + 1], axis=1))
+        # spectrum: qmid, power, conv/num, num (spectrum.f90:282-283)
+        s = read_rows(tmp_path / f"run_spectrum_{idx}.dat")
+        pw, num = O.conv_spectrum(pi, flags, nb, d["dspa"], d["dwav"], d["dv_spa"], d["dv_wav"])
+        conv = pw * taus[idx] ** 2
+        tab = O.init_spectrum(nb, d["dspa"])
+        _, inv, info = O.matrix_deconv(flags, nb, d["dwav"], d["dv_spa"], d["dv_wav"], tab)
+        power = (O.sym_from_packed(inv, nb) / np.outer(num, num)) @ conv
+        assert np.array_equal(s[:, 3], num)
+        assert np.allclose(s[:, 2], conv / num, rtol=1e-8)
+        assert np.allclose(s[:, 1], power, rtol=1e-6, atol=1e-12 * np.abs(power).max())
+    sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
+    assert sw.shape == (2, 12) and np.allclose(sw[:, 1], taus[1:])
+    # the final field checkpoint and the resume file (evolve.f90:401-437)
+    idx, tau, flag = open(tmp_path / "run_param.chk").read().split()
+    assert int(idx) == 2 and abs(float(tau) - 2.0) < 1e-12 and flag == "F"
+    raw = np.fromfile(tmp_path / "run_phi_2.chk", dtype=np.complex128)
+    phi2, pi2, _ = states[2]
+    assert np.array_equal(raw[: N ** 3].reshape(N, N, N), phi2)
+    assert np.array_equal(raw[N ** 3:].reshape(N, N, N), pi2)
+    assert not (tmp_path / "run_phi_0.chk").exists()
+
+    # resume from idx 1: the field comes back, the GW accumulators only with chk_gw
+    for chk_gw in ("F", "T"):
+        print(f"[case] fortran resume chk_gw={chk_gw}", flush=True)
+        rd = tmp_path / f"resume_{chk_gw}"
+        rd.mkdir()
+        phi1, pi1, A1 = states[1]
+        np.concatenate([phi1.ravel(), pi1.ravel()]).tofile(rd / "run_phi_1.chk")
+        if chk_gw == "T":
+            np.ascontiguousarray(A1[:, :, :, : N // 2 + 1]).tofile(rd / "run_grav_1.chk")
+        (rd / "run_param.chk").write_text("1 1.5 T\n")
+        r = run(rd, ini_text(str(rd), chk_gw=chk_gw), "-b", "100")
+        assert "chk files EXIST" in r.stdout
+        g = read_rows(rd / "run_gravspec_2.dat")
+        if chk_gw == "T":
+            A2 = states[2][2]
+        else:  # the reference restarts arr_grav from zero after a resume
+            phi, pi, A2 = phi1.copy(), pi1.copy(), O.new_gw_state(N)
+            dt, n = O.get_dtau(1.5, 2.0, P["sigma_cr"], P["dwav"], N, 1.0)
+            O.symplectic(phi, pi, A2, 1.5, dt, n, P)
+        o = O.gw_spectrum(A2, nbg, d["dspa"], d["dwav"], d["tau_cr"], d["a_now"], d["dv_wav"])
+        assert np.allclose(g[:, 3], o["power"], rtol=1e-8, atol=0)
+        raw = np.fromfile(rd / "run_phi_2.chk", dtype=np.complex128)
+        assert np.array_equal(raw[: N ** 3].reshape(N, N, N), phi2)
