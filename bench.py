@@ -14,8 +14,9 @@ region is exactly K steps between device synchronizations.
 
 Prints ONE JSON line (rank 0).  `roofline` covers the dominant kernel, its achieved
 algorithmic bytes per launch over its mean duration measured with HIP events on the stream
-the kernels run on.  `cpu_baseline` is the CPU FP64 oracle (the reference dataflow restated
-in C, OpenMP) timed on this host on a bounded sample: one step at 512^3.
+the kernels run on; `traffic` is its counter-measured HBM bytes per launch from the committed
+512^3 profile.  `cpu_baseline` is the CPU FP64 oracle (the reference dataflow restated in C,
+OpenMP) timed on this host on a bounded sample: two steps at 512^3.
 """
 import argparse
 import json
@@ -32,17 +33,40 @@ HBM_PEAK_GBS = 8000.0
 TWOPI_T = 2 * 3.14159265359
 SIGMA_CR = float(np.sqrt(0.5))
 
-# Algorithmic reals moved per site by one launch of each step kernel (the model of 40 reals
-# per site-update splits as 14 + 12 + 14 when the x transform is fused into the stencil pass;
-# this build runs it as its own pass, 14 + 12 + 12 + 14).
+# Algorithmic reals moved per site by one launch of each step kernel.  The model of 40 reals per
+# site-update splits as 14 + 12 + 14 over the three passes of a step: [kick + drift + sources +
+# x transform], [y transform], [z transform + projected accumulation].  gw_xfft only runs when
+# the x transform is not fused (lattice sizes without the compile-time transform).
 KERNEL_REALS = {
     "drift": 6,                  # phi r/w, pi read
-    "kick_source_drift": 14,     # phi r/w, pi r/w, 6 source reals written
+    "kick_source_drift": 14,     # phi r/w, pi r/w, 6 half-spectrum reals of the x-transformed sources
     "gw_xfft": 12,               # 6 reals in, 6 half-spectrum reals out
     "gw_yfft": 12,
     "gw_zfft_accumulate": 14,    # 6 half-spectrum reals in, 4 accumulators read + written
 }
 MODEL_REALS = 40
+
+# HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (calibrated against a copy of known
+# size) for the 512^3 workload; the profile summary holds the commands and the calibration.
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "lattice_step_512_hbm_traffic.md")
+TRAFFIC_KERNEL = {("kick_source_drift", "fp32c"): "kick_source_xfft_kernel<float, true, 512>",
+                  ("kick_source_drift", "fp64"): "kick_source_xfft_kernel<double, false, 512>",
+                  ("gw_yfft", "fp32c"): "gw_ypass_p2_kernel<float, 512>",
+                  ("gw_yfft", "fp64"): "gw_ypass_p2_kernel<double, 512>",
+                  ("gw_zfft_accumulate", "fp32c"): "gw_zpass_p2_kernel<float, 512>",
+                  ("gw_zfft_accumulate", "fp64"): "gw_zpass_p2_kernel<double, 512>"}
+
+
+def measured_traffic(slot, prec, N):
+    """Counter-measured HBM bytes per launch of a step kernel at 512^3, or None."""
+    name = TRAFFIC_KERNEL.get((slot, prec))
+    if N != 512 or name is None or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cols = [c.strip().strip("`") for c in line.split("|")]
+        if len(cols) > 8 and cols[1] == name:
+            return float(cols[8])
+    return None
 
 
 def log(*a):
@@ -104,7 +128,7 @@ def copy_probe(nbytes=4 << 30, reps=10):
     return gbs
 
 
-def run_cpu(N, L, seed):
+def run_cpu(N, L, seed, steps=2):
     import oracle as O
 
     dspa = L / N
@@ -113,9 +137,9 @@ def run_cpu(N, L, seed):
     phi, pi = O.ic_flat(N, seed, 1.0, dwav)
     A = O.new_gw_state(N)
     t0 = time.perf_counter()
-    O.symplectic(phi, pi, A, 1.0, 0.2 * dspa, 1, P)
+    O.symplectic(phi, pi, A, 1.0, 0.2 * dspa, steps, P)
     el = time.perf_counter() - t0
-    return N ** 3 / el, el
+    return steps * N ** 3 / el, el
 
 
 def main():
@@ -153,18 +177,20 @@ def main():
             + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in d["kernels"].items()))
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
+    traffic = measured_traffic(dom_name, "fp32c", N)
     roof = dict(bound="hbm", kernel=dom_name, achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
-                alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4))
+                frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4),
+                traffic_gbs=None if traffic is None else round(traffic / (dom["ms_avg"] * 1e-3) / 1e9, 1))
     step_frac = MODEL_REALS * c["real"] * c["su_s"] / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"CPU oracle: one step at {N}^3 on {cores} threads")
+        log(f"CPU oracle: two steps at {N}^3 on {cores} threads")
         su, el = run_cpu(N, L, args.seed)
         cpu = dict(value=su, unit="site-updates/s", cores=cores, kind="port",
-                   sample=f"1 DKD step with GW at {N}^3 (FP64 oracle, unfused reference dataflow), {el:.1f} s")
+                   sample=f"2 DKD steps with GW at {N}^3 (FP64 oracle, unfused reference dataflow), {el:.1f} s")
         log(f"CPU {su:.4g} site-updates/s ({el:.1f} s)")
 
     out = {
