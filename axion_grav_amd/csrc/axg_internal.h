@@ -35,6 +35,7 @@ struct KickScalars {
     double f1, f2, f3, m2, ma2, kt;  // kt: N_DW=2 tilt coefficient kappa0*t^4 (0 = off)
 };
 KickScalars kick_scalars(const axg_params& p, double t, double dt);
+double tilt_coef(const axg_params& p, double t);
 double maxion2(double thrm, double kappa);
 double msquared(double thrm);
 
@@ -120,7 +121,7 @@ void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, dou
 // diagnostics
 template <typename T, bool COMP>
 void launch_energy(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi,
-                   const void* cpi, double m2, double ma2, double* partial, int nwg, double* out);
+                   const void* cpi, double m2, double ma2, double kt, double* partial, int nwg, double* out);
 
 template <typename T, bool COMP>
 void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, int id_strings, int id_walls,

@@ -43,9 +43,18 @@ KickScalars kick_scalars(const axg_params& p, double t, double dt)
     k.f1 = fact1 / fact0;
     k.f2 = fact2 / fact0;
     k.f3 = fact3 / fact0;
-    double t2 = t * t;
-    k.kt = p.potential_kappa0 != 0.0 ? p.potential_kappa0 * (t2 * t2) : 0.0;
+    k.kt = tilt_coef(p, t);
     return k;
+}
+
+// N_DW=2 tilt V_t = -kappa0 t^4 (phi^2 + phi*^2)/2 (no counterpart in the reference).  The
+// reference's dVdphi is scaled so that sigma_cr^2 dVdphi is the force (potential.f90:43-60,
+// evolve.f90:303), so the tilt adds -kt conj(phi) with kt = kappa0 t^4 / sigma_cr^2; 0 = off.
+double tilt_coef(const axg_params& p, double t)
+{
+    if (p.potential_kappa0 == 0.0) return 0.0;
+    double t2 = t * t;
+    return p.potential_kappa0 * (t2 * t2) / (p.sigma_cr * p.sigma_cr);
 }
 
 bool supported_n(int64_t n)
@@ -662,18 +671,18 @@ extern "C" int axg_energy(axg_handle* h, double tau, double out[4])
         require(out && tau > 0, "need out and tau > 0");
         const axg_params& p = h->p;
         double thrm = p.initial_loop ? 0.0 : AXG_THRM_SIGMA / tau;
-        double m2 = msquared(thrm), ma2 = maxion2(thrm, p.kappa);
+        double m2 = msquared(thrm), ma2 = maxion2(thrm, p.kappa), kt = tilt_coef(p, tau);
         const int nwg = 1024;
         double* dout = h->red + (size_t)nwg * 4;
         if (h->comp)
-            launch_energy<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi, m2, ma2, h->red,
-                                       nwg, dout);
+            launch_energy<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi, m2, ma2, kt,
+                                       h->red, nwg, dout);
         else if (h->fp32)
-            launch_energy<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, h->red, nwg,
-                                        dout);
+            launch_energy<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, kt, h->red,
+                                        nwg, dout);
         else
-            launch_energy<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, h->red, nwg,
-                                         dout);
+            launch_energy<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, m2, ma2, kt, h->red,
+                                         nwg, dout);
         double r[3];
         AXG_HIP(hipMemcpyAsync(r, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));

@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(RED_NT) energy_kernel(int N, const typename V2
                                                         const short2* __restrict__ cphi,
                                                         const typename V2<T>::type* __restrict__ pi,
                                                         const short2* __restrict__ cpi, double m2, double ma2,
-                                                        double* __restrict__ partial)
+                                                        double kt, double* __restrict__ partial)
 {
     __shared__ double sh[RED_NT];
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
@@ -64,7 +64,9 @@ __global__ void __launch_bounds__(RED_NT) energy_kernel(int N, const typename V2
         }
         double s2 = c.x * c.x + c.y * c.y;
         double h = s2 + m2 / 2;
-        ev += h * h - ma2 * c.x;
+        double v = h * h - ma2 * c.x;
+        if (kt != 0.0) v = v - kt * (c.x * c.x - c.y * c.y);  // N_DW=2 tilt
+        ev += v;
     }
     double a = block_sum(ek, sh), b = block_sum(eg, sh), c = block_sum(ev, sh);
     if (threadIdx.x == 0) {
@@ -76,11 +78,11 @@ __global__ void __launch_bounds__(RED_NT) energy_kernel(int N, const typename V2
 
 template <typename T, bool COMP>
 void launch_energy(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi, double m2,
-                   double ma2, double* partial, int nwg, double* out)
+                   double ma2, double kt, double* partial, int nwg, double* out)
 {
     using T2 = typename V2<T>::type;
     hipLaunchKernelGGL((energy_kernel<T, COMP>), dim3(nwg), dim3(RED_NT), 0, s, N, (const T2*)phi, (const short2*)cphi,
-                       (const T2*)pi, (const short2*)cpi, m2, ma2, partial);
+                       (const T2*)pi, (const short2*)cpi, m2, ma2, kt, partial);
     AXG_HIP(hipGetLastError());
     sum_columns(s, partial, nwg, 3, out);
 }
@@ -506,7 +508,7 @@ void launch_scale(hipStream_t s, int64_t n, double2* data, double scale)
 
 #define AXG_INST_DIAG(T, C)                                                                                      \
     template void launch_energy<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*, double, \
-                                      double, double*, int, double*);                                            \
+                                      double, double, double*, int, double*);                                    \
     template void launch_classify<T, C>(hipStream_t, int, const void*, const void*, int, int, int, int8_t*,       \
                                         int64_t*, int64_t);                                                      \
     template void launch_masked_field<T, C>(hipStream_t, int, const void*, const void*, const int8_t*, double2*,  \

@@ -41,11 +41,11 @@ def lib():
         L.orc_derive.argtypes = [d, d, d, d, d, i64, p]
         L.orc_get_dtau.argtypes = [d, d, d, d, i64, d, p, p]
         L.orc_drift.argtypes = [p, p, i64, d]
-        L.orc_kick.argtypes = [p, p, i64, d, d, d, d, d, C.c_int]
+        L.orc_kick.argtypes = [p, p, i64, d, d, d, d, d, C.c_int, d]
         L.orc_fft3d.argtypes = [p, p, i64, C.c_int]
         L.orc_gradient.argtypes = [p, p, i64, d]
         L.orc_gw_integrate.argtypes = [p, p, i64, d, d, d, d, d, d, C.c_int]
-        L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int]
+        L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int, d]
         L.orc_gw_spectrum.argtypes = [p, i64, C.c_int, d, d, d, d, d, p, p, p, p]
         L.orc_classify.argtypes = [p, i64, C.c_int, C.c_int, C.c_int, p, p]
         L.orc_conv_spectrum.argtypes = [p, p, i64, C.c_int, d, d, d, d, p, p]
@@ -56,7 +56,7 @@ def lib():
         L.orc_pp_inverse.argtypes = [C.c_int, p]
         L.orc_matrix_deconv.restype = C.c_int
         L.orc_matrix_deconv.argtypes = [p, i64, C.c_int, d, d, d, p, p, p, C.c_int, p, p]
-        L.orc_energy.argtypes = [p, p, i64, d, d, d, d, C.c_int, p]
+        L.orc_energy.argtypes = [p, p, i64, d, d, d, d, C.c_int, d, p]
         L.orc_philox.argtypes = [p, C.c_uint32, C.c_uint32]
         L.orc_ic_mode.argtypes = [C.c_uint64, i64, i64, i64, i64, p, p]
         L.orc_ic_flat.argtypes = [p, p, i64, C.c_uint64, d, d]
@@ -110,8 +110,8 @@ def drift(phi, pi, dt):
     lib().orc_drift(_p(phi), _p(pi), phi.size, dt)
 
 
-def kick(phi, pi, t, dt, dspa, sigma_cr, kappa, initial_loop):
-    lib().orc_kick(_p(phi), _p(pi), phi.shape[0], t, dt, dspa, sigma_cr, kappa, int(initial_loop))
+def kick(phi, pi, t, dt, dspa, sigma_cr, kappa, initial_loop, kappa0=0.0):
+    lib().orc_kick(_p(phi), _p(pi), phi.shape[0], t, dt, dspa, sigma_cr, kappa, int(initial_loop), kappa0)
 
 
 def fft3d(a, sign=+1):
@@ -135,10 +135,10 @@ def gw_integrate(phi, A, tau, dtau, dspa, dwav, dv_spa, sigma_cr, fix_swap=False
 
 def symplectic(phi, pi, A, t1, dt, nsteps, P, fix_swap=False):
     """Advance (phi, pi[, A]) in place by nsteps DKD steps of dt from t1.  P is a dict with
-    dspa, dwav, dv_spa, sigma_cr, kappa, initial_loop."""
+    dspa, dwav, dv_spa, sigma_cr, kappa, initial_loop and optionally potential_kappa0."""
     lib().orc_symplectic(_p(phi), _p(pi), _p(A) if A is not None else None, phi.shape[0], t1, dt,
                          int(nsteps), P["dspa"], P["dwav"], P["dv_spa"], P["sigma_cr"], P["kappa"],
-                         int(P["initial_loop"]), int(fix_swap))
+                         int(P["initial_loop"]), int(fix_swap), float(P.get("potential_kappa0", 0.0)))
 
 
 def new_gw_state(N):
@@ -201,9 +201,9 @@ def matrix_deconv(flags, nb, dwav, dv_spa, dv_wav, spec_tab):
     return mat, inv, info
 
 
-def energy(phi, pi, t, dspa, sigma_cr, kappa, initial_loop):
+def energy(phi, pi, t, dspa, sigma_cr, kappa, initial_loop, kappa0=0.0):
     out = np.zeros(4)
-    lib().orc_energy(_p(phi), _p(pi), phi.shape[0], t, dspa, sigma_cr, kappa, int(initial_loop), _p(out))
+    lib().orc_energy(_p(phi), _p(pi), phi.shape[0], t, dspa, sigma_cr, kappa, int(initial_loop), kappa0, _p(out))
     return out
 
 

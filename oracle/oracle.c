@@ -107,12 +107,25 @@ void orc_drift(double* phi, const double* pi, int64_t nsite, double dt)
     }
 }
 
+/* Coefficient of the N_DW=2 tilt V_t = -kappa0 t^4 (phi^2 + phi*^2)/2, which has no
+ * counterpart in the reference.  The reference scales its potential so that sigma_cr^2 dV/dphi*
+ * is the force (potential.f90:43-60 with evolve.f90:303: lambda = 4 sigma_cr^2), so the tilt
+ * adds -kt conj(phi) to dVdphi and -kt (re^2 - im^2) to the energy's potential term, with
+ * kt = kappa0 t^4 / sigma_cr^2.  kappa0 = 0 leaves the reference's arithmetic untouched. */
+static double tilt_coef(double kappa0, double t, double sigma_cr)
+{
+    if (kappa0 == 0.0) return 0.0;
+    double t2 = t * t;
+    return kappa0 * (t2 * t2) / (sigma_cr * sigma_cr);
+}
+
 /* evolve.f90:283-335 evolve_phidot (without the GW call at 337), with potential.f90:43-60
  * dVdphi inlined.  Periodic in all three directions (update_margin's halo planes are the
  * periodic wrap on one rank). */
 void orc_kick(const double* phi, double* pi, int64_t N, double t, double dt, double dspa,
-              double sigma_cr, double kappa, int initial_loop)
+              double sigma_cr, double kappa, int initial_loop, double kappa0)
 {
+    const double kt = tilt_coef(kappa0, t, sigma_cr);
     double hubble = 1 / t;
     double thrm = initial_loop ? 0.0 : THRM_SIGMA / t;
     double m2 = orc_msquared(thrm);
@@ -143,6 +156,10 @@ void orc_kick(const double* phi, double* pi, int64_t N, double t, double dt, dou
                 double g = 2 * s2 + m2;
                 double dr = g * c.re - ma2 / 2;
                 double di = g * c.im;
+                if (kt != 0.0) {
+                    dr = dr - kt * c.re;
+                    di = di + kt * c.im;
+                }
                 oc* q = &p[x + N * (y + N * z)];
                 q->re = fact1 * q->re + fact2 * lr + fact3 * dr;
                 q->im = fact1 * q->im + fact2 * li + fact3 * di;
@@ -349,13 +366,13 @@ void orc_gw_integrate(const double* phi, double* A, int64_t N, double tau, doubl
  * drift dt/2.  A may be NULL when grav_spec is off. */
 void orc_symplectic(double* phi, double* pi, double* A, int64_t N, double t1, double dt, int32_t nt,
                     double dspa, double dwav, double dv_spa, double sigma_cr, double kappa,
-                    int initial_loop, int fix_swap)
+                    int initial_loop, int fix_swap, double kappa0)
 {
     int64_t n3 = N * N * N;
     double t = t1;
     for (int32_t it = 1; it <= nt; it++) {
         if (it == 1) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
-        orc_kick(phi, pi, N, t, dt, dspa, sigma_cr, kappa, initial_loop);
+        orc_kick(phi, pi, N, t, dt, dspa, sigma_cr, kappa, initial_loop, kappa0);
         if (A) orc_gw_integrate(phi, A, N, t, dt, dspa, dwav, dv_spa, sigma_cr, fix_swap);
         if (it == nt) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
         else { orc_drift(phi, pi, n3, dt); t = t + dt; }
@@ -670,8 +687,9 @@ int orc_matrix_deconv(const int8_t* flags, int64_t N, int nb, double dwav, doubl
  * 7-point Laplacian), potential tau^2 sigma_cr^2 sum V with V = (|phi|^2+m2/2)^2 - ma2*Re(phi)
  * (dV/dphi* is potential.f90:43-60's dVdphi), each times dv_spa; out[3] = total. */
 void orc_energy(const double* phi, const double* pi, int64_t N, double t, double dspa,
-                double sigma_cr, double kappa, int initial_loop, double* out)
+                double sigma_cr, double kappa, int initial_loop, double kappa0, double* out)
 {
+    const double kt = tilt_coef(kappa0, t, sigma_cr);
     const oc* f = (const oc*)phi;
     const oc* p = (const oc*)pi;
     double thrm = initial_loop ? 0.0 : THRM_SIGMA / t;
@@ -692,7 +710,9 @@ void orc_energy(const double* phi, const double* pi, int64_t N, double t, double
                 }
                 double s2 = c.re * c.re + c.im * c.im;
                 double h = s2 + m2 / 2;
-                ev += h * h - ma2 * c.re;
+                double v = h * h - ma2 * c.re;
+                if (kt != 0.0) v = v - kt * (c.re * c.re - c.im * c.im);
+                ev += v;
             }
     double dv = dspa * dspa * dspa;
     out[0] = ek * dv;

@@ -74,6 +74,29 @@ def test_kick_drift_bitexact(N, nsteps):
     assert np.array_equal(g_pi, o_pi), relerr(g_pi, o_pi)
 
 
+@pytest.mark.parametrize("N,gw", [(16, False), (32, True)])
+def test_kick_drift_tilt_bitexact(N, gw):
+    """N_DW=2 tilt on (kappa0 = 1e-3, large enough to move every site): the plain kick and the
+    fused kick+source kernel both match the oracle bit for bit."""
+    say("kick_drift tilt", N, gw)
+    prm, P = params(N, grav_spec=gw, potential_kappa0=1e-3)
+    P = dict(P, potential_kappa0=1e-3)
+    phi, pi = ic(N, 12, 4.0)
+    t1, dt, nsteps = 1.0, 0.05, 4
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(t1, dt, nsteps)
+        g_phi, g_pi = L.get_fields()
+    o_phi, o_pi = phi.copy(), pi.copy()
+    O.symplectic(o_phi, o_pi, O.new_gw_state(N) if gw else None, t1, dt, nsteps, P)
+    assert np.array_equal(g_phi, o_phi), relerr(g_phi, o_phi)
+    assert np.array_equal(g_pi, o_pi), relerr(g_pi, o_pi)
+    # the tilt changed the trajectory
+    n_phi, n_pi = phi.copy(), pi.copy()
+    O.symplectic(n_phi, n_pi, None, t1, dt, nsteps, dict(P, potential_kappa0=0.0))
+    assert relerr(n_pi, o_pi) > 1e-6
+
+
 # ---------------------------------------------------------------- GW accumulation
 
 @pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False)])
@@ -177,18 +200,19 @@ def test_field_spectrum_fp64(N):
     assert relerr(s["power"], power) < 1e-8
 
 
-@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-12), (capi.FP32C, 1e-9)])
-def test_energy(prec, tol):
-    say("energy", prec)
+@pytest.mark.parametrize("prec,tol,kappa0", [(capi.FP64, 1e-12, 0.0), (capi.FP32C, 1e-9, 0.0),
+                                              (capi.FP64, 1e-12, 1e-3)])
+def test_energy(prec, tol, kappa0):
+    say("energy", prec, kappa0)
     N = 24
-    prm, P = params(N, grav_spec=False, precision=prec)
+    prm, P = params(N, grav_spec=False, precision=prec, potential_kappa0=kappa0)
     phi, _ = ic(N, 8, 4.0)
     rng = np.random.default_rng(2)
     pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
     with capi.Lattice(prm) as L:
         L.set_fields(phi, pi)
         e = L.energy(2.5)
-    o = O.energy(phi, pi, 2.5, P["dspa"], P["sigma_cr"], P["kappa"], True)
+    o = O.energy(phi, pi, 2.5, P["dspa"], P["sigma_cr"], P["kappa"], True, kappa0)
     assert relerr(e, o) < tol
 
 

@@ -103,6 +103,49 @@ def test_kick_plane_wave_lattice_eigenvalue():
     assert np.abs(pi - exp).max() < 1e-12
 
 
+def test_tilt_vacua_are_static():
+    """N_DW=2 tilt -kappa0 t^4 (phi^2+phi*^2)/2 at zero temperature: with the reference's
+    normalisation the potential term is (|phi|^2-1)^2 - kt (re^2 - im^2), kt = kappa0 t^4/sigma_cr^2,
+    so the uniform field r with r^2 = 1 + kt/2 (minimum, real axis) and i s with s^2 = 1 - kt/2
+    (saddle, imaginary axis) feel no force.  A sign or factor error in the tilt moves them."""
+    N, dspa, t, dt, kappa0 = 4, 0.25, 3.0, 0.05, 2e-3
+    kt = kappa0 * t ** 4 / SQ_HALF ** 2
+    ma2 = O.maxion2(0.0, 1e-30)
+    for c in (np.sqrt(1 + kt / 2), 1j * np.sqrt(1 - kt / 2)):
+        phi = np.full((N, N, N), complex(c))
+        pi = np.zeros_like(phi)
+        O.kick(phi, pi, t, dt, dspa, SQ_HALF, 1e-30, True, kappa0)
+        assert np.abs(pi).max() < 1e-14 + abs(ma2) * dt * t * t
+        pi0 = np.zeros_like(phi)
+        O.kick(phi, pi0, t, dt, dspa, SQ_HALF, 1e-30, True)  # without the tilt the same field moves
+        assert np.abs(pi0).max() > 1e-3
+
+
+def test_tilt_force_is_gradient_of_energy():
+    """The kick's tilt force equals minus the derivative of the energy's tilt term: with
+    pi = 0 and a uniform field, the change of pi from the tilt over one kick, divided by f3,
+    is dV/dphi*, and E_pot = t^2 sigma_cr^2 dspa^3 N^3 V gives dE/d(re) = 2 Re(dV/dphi*) times
+    that prefactor (central differences of the oracle's own energy)."""
+    N, dspa, t, dt, kappa0 = 4, 0.25, 2.0, 0.05, 5e-3
+    c = 0.8 + 0.45j
+    f1, f2, f3 = kick_scalars(t, dt, dspa, SQ_HALF)
+    phi = np.full((N, N, N), c)
+    on, off = np.zeros_like(phi), np.zeros_like(phi)
+    O.kick(phi, on, t, dt, dspa, SQ_HALF, 1e-30, True, kappa0)
+    O.kick(phi, off, t, dt, dspa, SQ_HALF, 1e-30, True)
+    dvt = (on[0, 0, 0] - off[0, 0, 0]) / f3  # tilt part of dV/dphi*
+    pref = t * t * SQ_HALF ** 2 * dspa ** 3 * N ** 3
+    h = 1e-6
+    for d, part in ((1.0, dvt.real), (1j, dvt.imag)):
+        e = []
+        for sgn in (1, -1):
+            f = np.full((N, N, N), c + sgn * h * d)
+            e.append(O.energy(f, np.zeros_like(f), t, dspa, SQ_HALF, 1e-30, True, kappa0)[2]
+                     - O.energy(f, np.zeros_like(f), t, dspa, SQ_HALF, 1e-30, True)[2])
+        deriv = (e[0] - e[1]) / (2 * h)
+        assert abs(deriv - 2 * pref * part) < 1e-7 * abs(2 * pref * part)
+
+
 def test_massless_mode_converges_second_order():
     """A small-amplitude mode of the linearised equation with H = 1/tau: the lattice solution
     with dispersion w^2 = sum (2-2cos)/dspa^2 is j0-like, sin(w tau)/(w tau); the DKD
