@@ -606,7 +606,9 @@ extern "C" int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* l
 extern "C" int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps)
 {
     return guarded(h, [&] {
-        require(nsteps >= 1 && dt > 0 && t1 > 0, "need nsteps >= 1, dt > 0, t1 > 0");
+        // dt < 0 runs the schedule backwards in time (the time-reversal check of the integrator)
+        require(nsteps >= 1 && dt != 0 && t1 > 0 && t1 + nsteps * dt > 0,
+                "need nsteps >= 1, dt != 0, t1 > 0 and t1 + nsteps*dt > 0");
         advance(h, t1, dt, nsteps);
     });
 }

@@ -53,7 +53,8 @@ typedef struct axg_params {
     double kappa;                        /* potential.f90:24 (key kappa) */
     double eps_dtau;                     /* key eps_dtau (get_dtau, evolve.f90:10-24) */
     double dtau_fixed;                   /* > 0: fixed step instead of get_dtau (new key) */
-    double potential_kappa0;             /* N_DW=2 tilt -kappa0 tau^4 (phi^2+phi*^2)/2; 0 = off */
+    double potential_kappa0;             /* N_DW=2 tilt -kappa0 tau^4 (phi^2+phi*^2)/2 beside
+                                            lambda/4 (|phi|^2-1)^2, lambda = 4 sigma_cr^2; 0 = off */
     int32_t reserved[8];
 } axg_params;
 
@@ -102,7 +103,10 @@ int axg_set_gw_state(axg_handle* h, const double* A, int64_t count);
  * of dt/2, then n x (kick at the midpoint with the GW accumulation of grav_waves.f90:193-255
  * when grav_spec, drift dt), the last drift being dt/2.  Returns the step count. */
 int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_out);
-/* The same DKD schedule with an explicit step: nsteps steps of dt from t1. */
+/* The same DKD schedule with an explicit step: nsteps steps of dt from t1.  dt < 0 runs it
+ * backwards (t1 + nsteps*dt > 0): the leapfrog with the Crank-Nicolson friction of
+ * evolve.f90:299-303 is time-reversible, so forward then backward returns the fields to
+ * round-off (the reversibility residual of the energy diagnostics). */
 int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps);
 
 /* grav_spec_estimation (grav_waves.f90:257-321) without the file write: qmid, power,

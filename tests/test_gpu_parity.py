@@ -97,6 +97,26 @@ def test_kick_drift_tilt_bitexact(N, gw):
     assert relerr(n_pi, o_pi) > 1e-6
 
 
+@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-12), (capi.FP32C, 1e-9)])
+def test_time_reversal_residual(prec, tol):
+    """Forward 30 steps, then 30 steps of -dt from the end time: the fields come back to
+    round-off (the integrator is time-reversible; the fused GW path runs in both legs)."""
+    say("time reversal", prec)
+    N = 32
+    prm, P = params(N, precision=prec)
+    phi, pi = ic(N, 21, 4.0)
+    t1, dt, n = 1.0, 0.05, 30
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(t1, dt, n)
+        mid, _ = L.get_fields()
+        L.advance_steps(t1 + n * dt, -dt, n)
+        b_phi, b_pi = L.get_fields()
+    assert relerr(mid, phi) > 1e-2  # the fields did move
+    assert relerr(b_phi, phi) < tol, relerr(b_phi, phi)
+    assert np.abs(b_pi - pi).max() < tol * np.abs(phi).max(), np.abs(b_pi - pi).max()
+
+
 # ---------------------------------------------------------------- GW accumulation
 
 @pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False)])
