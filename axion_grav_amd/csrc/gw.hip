@@ -5,9 +5,11 @@
 //
 // Layout.  The sources are real, so the transform is kept on the half spectrum
 // kx = 0..N/2.  W[c][z][y][kx] (c = 0..5 in the reference's (i,j) order (1,1),(2,1),(2,2),
-// (3,1),(3,2),(3,3); row pitch P = half_pitch(N)) holds the x-transformed sources, is
-// y-transformed in place, and is z-transformed on the fly by the accumulation pass.  The
-// accumulators A[m][kz][ky][kx] (m = arr_grav (1,1),(2,1),(1,2),(2,2)) use the same pitch.
+// (3,1),(3,2),(3,3); row pitch P = half_pitch(N)) holds the x-transformed sources.  The y
+// transform writes them out of place into W2 in the tiled GwLayout [c][ky][kx/8][kz][kx%8]
+// (N = 2^k; in place in W with rows [c][kz][ky][kx] otherwise), and the accumulation pass
+// z-transforms them on the fly.  The accumulators A[m][...] (m = arr_grav (1,1),(2,1),(1,2),
+// (2,2)) use the same GwLayout, so that pass reads and writes contiguous runs.
 #include "axg_internal.h"
 #include "fft_p2.h"
 

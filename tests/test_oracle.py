@@ -2,6 +2,8 @@
 oracle (a dropped term, a wrong sign or index, a transposed operand) fails it: closed forms,
 brute force on tiny lattices, numpy cross-checks and invariants.  The reference ships no
 golden vectors, fixtures or tests for this path, so these are the pins."""
+import os
+
 import numpy as np
 import pytest
 
@@ -20,6 +22,15 @@ def brute_dft3(a, sign):
 
 
 # ------------------------------------------------------------------ FFT (grav_waves.f90:225)
+
+def test_fft_matches_committed_golden_vectors():
+    """numpy golden vectors (tests/golden/make_golden.py): the BACKWARD transform of seeded
+    12^3 (radix 3) and 16^3 inputs."""
+    for N in (12, 16):
+        z = np.load(os.path.join(os.path.dirname(__file__), "golden", f"fft_{N}.npz"))
+        out = O.fft3d(z["input"], +1)
+        assert np.abs(out - z["backward"]).max() < 1e-12 * np.abs(z["backward"]).max()
+
 
 @pytest.mark.parametrize("N", [4, 6, 8, 12])
 @pytest.mark.parametrize("sign", [+1, -1])
