@@ -57,10 +57,14 @@ class DefectOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_wall", C.c_int64), ("n_mask", C.c_int64)]
 
 
+class StringOut(C.Structure):
+    _fields_ = [("n_string", C.c_int64), ("n_vel", C.c_int64), ("length", C.c_double), ("vel", C.c_double * 3)]
+
+
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_field_spectrum", "axg_energy", "axg_profile")
+           "axg_strings", "axg_get_strings", "axg_field_spectrum", "axg_energy", "axg_profile")
 
 _LIB = None
 
@@ -89,6 +93,8 @@ def lib():
             "axg_defects": [p, C.POINTER(DefectOut)],
             "axg_get_mask": [p, p, i64, p],
             "axg_get_flags": [p, p, i64],
+            "axg_strings": [p, C.c_uint32, C.c_uint32, C.POINTER(StringOut)],
+            "axg_get_strings": [p, p, i64, p],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
@@ -257,6 +263,22 @@ class Lattice:
         f = np.empty((self.N,) * 3, np.int8)
         self._check(lib().axg_get_flags(self._h, _ptr(f), self.n3), "axg_get_flags")
         return f
+
+    def strings(self, seed0, seed1):
+        """Fit every string cell of the last defects() (needs id_strings): dict(n_string, n_vel,
+        length = sum of chord lengths in lattice units, vel = (mean |v|, mean v^2, mean gamma))."""
+        out = StringOut()
+        self._check(lib().axg_strings(self._h, seed0, seed1, C.byref(out)), "axg_strings")
+        return dict(n_string=out.n_string, n_vel=out.n_vel, length=out.length, vel=np.array(out.vel[:]))
+
+    def string_records(self):
+        """(n_string, 11): pos(3, site coordinates counted from 0), dir(3), gof, sl, vel(3)."""
+        cnt = C.c_int64()
+        self._check(lib().axg_get_strings(self._h, None, 0, C.byref(cnt)), "axg_get_strings")
+        n = cnt.value
+        buf = np.zeros((max(n, 1), 11))
+        self._check(lib().axg_get_strings(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_strings")
+        return buf[:n]
 
     def field_spectrum(self, tau):
         nb = self.params.num_bin

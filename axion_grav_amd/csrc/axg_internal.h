@@ -127,10 +127,21 @@ template <typename T, bool COMP>
 void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, int id_strings, int id_walls,
                      int spec_est, int8_t* flags, int64_t* block_counts, int64_t nblk);
 
-void launch_mask_compact(hipStream_t s, int N, const int8_t* flags, const int64_t* block_offsets,
-                         int32_t* ixyz, int64_t nblk);
+void launch_cell_compact(hipStream_t s, int N, const int8_t* flags, int bit, const int64_t* block_offsets,
+                         int32_t* out, int triplets, int64_t nblk);
+void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out);
 
-void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total);
+// String cells (flag bit 1) in site order `cells`: fit, chord length and velocity per cell.
+// rec: 11 doubles per string (pos(3) in site coordinates counted from 0, dir(3), gof, sl, vel(3));
+// partial: 5 doubles per block of STRING_NT strings (sum sl, n with |v| < 1, sum |v|, sum v^2,
+// sum gamma), to be summed with sum_columns.
+constexpr int STRING_NT = 64;
+template <typename T, bool COMP>
+void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
+                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, double* rec,
+                       double* partial);
+
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col);
 
 template <typename T, bool COMP>
 void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags,

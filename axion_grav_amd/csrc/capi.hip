@@ -128,7 +128,14 @@ struct axg_handle {
     int64_t* blk = nullptr;   // per-block counts / offsets of the classifier
     int32_t* mask = nullptr;  // compacted mask list (3 ints per cell)
     int64_t nmask = -1;       // -1: no classification yet
+    int64_t nstring = 0;      // string cells of the last classification
     bool have_flags = false;
+    // string fits (grown on demand): cell list, 11 doubles per string, per-block sums
+    int32_t* str_cells = nullptr;
+    double* str_rec = nullptr;
+    double* str_part = nullptr;
+    int64_t str_cap = 0;      // strings the three buffers hold
+    int64_t nstr_rec = -1;    // strings in str_rec (-1: no fit since the fields changed)
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
     double* d_largeq = nullptr;
@@ -156,6 +163,18 @@ struct axg_handle {
         }
         bufs.push_back({q, bytes});
         return q;
+    }
+    // Free one buffer of bufs (stream already idle).
+    void release_one(void*& q)
+    {
+        if (!q) return;
+        for (size_t i = 0; i < bufs.size(); i++)
+            if (bufs[i].p == q) {
+                (void)hipFree(q);
+                bufs.erase(bufs.begin() + i);
+                break;
+            }
+        q = nullptr;
     }
     // Record a start/stop event pair around launch() on the handle's stream when timing.
     template <typename F>
@@ -471,6 +490,7 @@ extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi
             store_from_work(h, h->pi, h->cpi);
         }
         h->have_flags = false;
+        h->nstr_rec = -1;
         AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
@@ -583,6 +603,7 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     else if (h->fp32) run_steps<float, false>(h, t1, dt, nt);
     else run_steps<double, false>(h, t1, dt, nt);
     h->have_flags = false;
+    h->nstr_rec = -1;
     AXG_HIP(hipStreamSynchronize(h->stream));
     if (h->prof) h->collect();
 }
@@ -716,7 +737,7 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         else
             launch_classify<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, p.id_strings, p.id_walls,
                                            p.spec_est, h->flags, counts, nblk);
-        launch_scan_counts(h->stream, counts, offs, nblk, tot);
+        launch_scan_counts(h->stream, counts, offs, nblk, tot, 2);
         int64_t t[4];
         AXG_HIP(hipMemcpyAsync(t, tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
@@ -724,7 +745,9 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         out->n_wall = t[1];
         out->n_mask = t[2];
         h->nmask = t[2];
+        h->nstring = t[0];
         h->have_flags = true;
+        h->nstr_rec = -1;
     });
 }
 
@@ -739,8 +762,74 @@ extern "C" int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* 
         const int64_t nblk = (h->n3 + 255) / 256;
         int64_t* offs = h->blk + 3 * nblk;
         int32_t* dm = (int32_t*)h->work;  // 3*nmask ints fit in the N^3 x 16-byte work array
-        launch_mask_compact(h->stream, h->N, h->flags, offs, dm, nblk);
+        launch_cell_compact(h->stream, h->N, h->flags, 4, offs, dm, 1, nblk);
         AXG_HIP(hipMemcpyAsync(ixyz, dm, (size_t)3 * h->nmask * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_string_out* out)
+{
+    return guarded(h, [&] {
+        require(out != nullptr, "null output");
+        require(h->have_flags, "call axg_defects first");
+        require(h->p.id_strings != 0, "id_strings is off: no string cells flagged");
+        const int64_t nblk = (h->n3 + 255) / 256;
+        const int64_t ns = h->nstring;
+        const int64_t nsb = (ns + STRING_NT - 1) / STRING_NT;
+        if (ns > h->str_cap) {
+            void* a = h->str_cells;
+            void* b = h->str_rec;
+            void* c = h->str_part;
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            h->release_one(a);
+            h->release_one(b);
+            h->release_one(c);
+            h->str_cap = 0;
+            h->str_cells = (int32_t*)h->alloc((size_t)ns * sizeof(int32_t));
+            h->str_rec = (double*)h->alloc((size_t)11 * ns * sizeof(double));
+            h->str_part = (double*)h->alloc((size_t)(5 * nsb + 8) * sizeof(double));
+            h->str_cap = ns;
+        }
+        double sums[5] = {0, 0, 0, 0, 0};
+        if (ns > 0) {
+            int64_t* counts = h->blk;               // [nblk][3] from the classification
+            int64_t* soffs = h->blk + 4 * nblk + 8;  // [nblk] exclusive string offsets
+            int64_t* stot = soffs + nblk;            // [4]
+            launch_scan_counts(h->stream, counts, soffs, nblk, stot, 0);
+            launch_cell_compact(h->stream, h->N, h->flags, 1, soffs, h->str_cells, 0, nblk);
+            double* dsum = h->str_part + 5 * nsb;
+            if (h->comp)
+                launch_string_fit<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi,
+                                               h->str_cells, ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+            else if (h->fp32)
+                launch_string_fit<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, h->str_cells,
+                                                ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+            else
+                launch_string_fit<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr,
+                                                 h->str_cells, ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+            sum_columns(h->stream, h->str_part, (int)nsb, 5, dsum);
+            AXG_HIP(hipMemcpyAsync(sums, dsum, sizeof sums, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+        out->n_string = ns;
+        out->n_vel = (int64_t)sums[1];
+        out->length = sums[0];
+        for (int k = 0; k < 3; k++) out->vel[k] = out->n_vel > 0 ? sums[2 + k] / (double)out->n_vel : 0.0;
+        h->nstr_rec = ns;
+    });
+}
+
+extern "C" int axg_get_strings(axg_handle* h, double* rec, int64_t cap, int64_t* count)
+{
+    return guarded(h, [&] {
+        require(count != nullptr, "null count");
+        require(h->nstr_rec >= 0, "call axg_strings first (after the last change of the fields)");
+        *count = h->nstr_rec;
+        if (h->nstr_rec == 0 || rec == nullptr || cap == 0) return;  // size query
+        require(cap >= h->nstr_rec, "string buffer too small");
+        AXG_HIP(hipMemcpyAsync(rec, h->str_rec, (size_t)11 * h->nstr_rec * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
@@ -996,5 +1085,6 @@ extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
         (void)hipFree(pidx);
         (void)hipFree(pval);
         h->have_flags = false;
+        h->nstr_rec = -1;
     });
 }

@@ -32,7 +32,7 @@ __global__ void sum_columns_kernel(const double* __restrict__ partial, int nwg, 
     out[b] = s;
 }
 
-static void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out)
+void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out)
 {
     hipLaunchKernelGGL(sum_columns_kernel, dim3((width + 255) / 256), dim3(256), 0, s, partial, nwg, width, out);
     AXG_HIP(hipGetLastError());
@@ -151,10 +151,11 @@ void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, in
     AXG_HIP(hipGetLastError());
 }
 
-// Exclusive scan of the per-block mask counts (column 2 of in[nblk][3]) and the totals of
-// all three columns, by one block of 1024 threads over contiguous chunks.
+// Exclusive scan of the per-block counts of column col of in[nblk][3] (0 strings, 1 walls,
+// 2 mask) and the totals of all three columns, by one block of 1024 threads over contiguous
+// chunks.
 __global__ void __launch_bounds__(1024) scan_counts_kernel(const int64_t* __restrict__ in, int64_t* __restrict__ out,
-                                                           int64_t n, int64_t* __restrict__ total)
+                                                           int64_t n, int64_t* __restrict__ total, int col)
 {
     __shared__ int64_t sh[1024];
     __shared__ int64_t tot[3][1024];
@@ -162,15 +163,17 @@ __global__ void __launch_bounds__(1024) scan_counts_kernel(const int64_t* __rest
     const int64_t chunk = (n + 1023) / 1024;
     const int64_t b0 = t * chunk, b1 = min(n, b0 + chunk);
     int64_t s = 0, s0 = 0, s1 = 0;
+    int64_t s2 = 0;
     for (int64_t b = b0; b < b1; b++) {
-        s += in[b * 3 + 2];
+        s += in[b * 3 + col];
         s0 += in[b * 3 + 0];
         s1 += in[b * 3 + 1];
+        s2 += in[b * 3 + 2];
     }
     sh[t] = s;
     tot[0][t] = s0;
     tot[1][t] = s1;
-    tot[2][t] = s;
+    tot[2][t] = s2;
     __syncthreads();
     if (t == 0) {
         int64_t run = 0;
@@ -183,25 +186,27 @@ __global__ void __launch_bounds__(1024) scan_counts_kernel(const int64_t* __rest
     int64_t run = sh[t];
     for (int64_t b = b0; b < b1; b++) {
         out[b] = run;
-        run += in[b * 3 + 2];
+        run += in[b * 3 + col];
     }
 }
 
-void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total)
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col)
 {
-    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, in, out, n, total);
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, in, out, n, total, col);
     AXG_HIP(hipGetLastError());
 }
 
-// Mask cells of block b go to offs[b] + (rank among the block's mask cells), so the list is
-// in site order: z outermost, x fastest -- the loop order of defects_3d.f90:167-217.
-__global__ void __launch_bounds__(256) mask_compact_kernel(int N, const int8_t* __restrict__ flags,
-                                                           const int64_t* __restrict__ offs, int32_t* __restrict__ ixyz)
+// Cells of block b with flag bit `bit` go to offs[b] + (rank among the block's such cells), so
+// the list is in site order: z outermost, x fastest -- the loop order of defects_3d.f90:167-217.
+// triplets != 0: 1-based (ix, iy, iz) per cell (the mask list); else the site index.
+__global__ void __launch_bounds__(256) cell_compact_kernel(int N, const int8_t* __restrict__ flags, int bit,
+                                                           const int64_t* __restrict__ offs, int32_t* __restrict__ out,
+                                                           int triplets)
 {
     __shared__ int sh[256];
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
     const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
-    int m = (i < n3 && (flags[i] & 4)) ? 1 : 0;
+    int m = (i < n3 && (flags[i] & bit)) ? 1 : 0;
     sh[threadIdx.x] = m;
     __syncthreads();
     // inclusive scan (Hillis-Steele) over 256 entries
@@ -213,17 +218,22 @@ __global__ void __launch_bounds__(256) mask_compact_kernel(int N, const int8_t* 
     }
     if (m) {
         int64_t pos = offs[blockIdx.x] + sh[threadIdx.x] - 1;
-        int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
-        ixyz[3 * pos + 0] = x + 1;
-        ixyz[3 * pos + 1] = y + 1;
-        ixyz[3 * pos + 2] = z + 1;
+        if (triplets) {
+            int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+            out[3 * pos + 0] = x + 1;
+            out[3 * pos + 1] = y + 1;
+            out[3 * pos + 2] = z + 1;
+        } else {
+            out[pos] = (int32_t)i;  // N <= 1024: site indices fit in 31 bits
+        }
     }
 }
 
-void launch_mask_compact(hipStream_t s, int N, const int8_t* flags, const int64_t* block_offsets, int32_t* ixyz,
-                         int64_t nblk)
+void launch_cell_compact(hipStream_t s, int N, const int8_t* flags, int bit, const int64_t* block_offsets,
+                         int32_t* out, int triplets, int64_t nblk)
 {
-    hipLaunchKernelGGL(mask_compact_kernel, dim3((unsigned)nblk), dim3(256), 0, s, N, flags, block_offsets, ixyz);
+    hipLaunchKernelGGL(cell_compact_kernel, dim3((unsigned)nblk), dim3(256), 0, s, N, flags, bit, block_offsets, out,
+                       triplets);
     AXG_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,444 @@
+// String cells: straight-line fit, chord length and velocity, one thread per string cell
+// (id_defects' string loop, defects_3d.f90:167-205, with calc_chi2_string 384-433,
+// position_string 435-603, string_length 1027-1056 and velocity_string 605-690 of the VEL = 0
+// build), plus the per-block sums of add_string_length (1654-1733, group_strings = F).
+//
+// The fit is a small sequential search per cell (1001 random starts, then up to 1000 simplex
+// iterations, 8 atan2 per chi2), so the kernel is bound by FP64 issue, not by HBM: it reads 8
+// corner values of phi and pi per string and writes 11 doubles.  Every array of the search
+// lives in registers: the insertion of a start point and the bubble sort are unrolled over the
+// five simplex slots, so no index into them depends on data.
+#include "axg_internal.h"
+
+namespace axg {
+
+namespace {
+
+constexpr int NSIMP = 4;       // num_simp_string_pos (defects_3d.f90:26)
+constexpr int MAXIT = 1000;    // max_iter_string_pos
+constexpr double PI_T = 3.14159265359;
+constexpr double TWOPI_T = 2 * PI_T;
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                                       uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t h0 = __umulhi(M0, c0), l0 = M0 * c0;
+        uint32_t h1 = __umulhi(M1, c2), l1 = M1 * c2;
+        uint32_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = l1;
+        c2 = n2;
+        c3 = l0;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+// The four uniforms of start point i of string s: one Philox block keyed by the seed, counter
+// (i, s, 'strg').  The reference reads them from an MKL VSL MCG59 stream positioned at
+// s * 4004 (defects_3d.f90:137-162, 465); a counter-based generator keeps one stream per string
+// independent of how the strings are scheduled.
+__device__ __forceinline__ void uniforms(uint32_t k0, uint32_t k1, int64_t s, int i, double u[4])
+{
+    uint32_t c0 = (uint32_t)i, c1 = (uint32_t)s, c2 = (uint32_t)((uint64_t)s >> 32), c3 = 0x73747267u;
+    philox(c0, c1, c2, c3, k0, k1);
+    const double sc = 2.3283064365386963e-10;  // 2^-32
+    u[0] = ((double)c0 + 0.5) * sc;
+    u[1] = ((double)c1 + 0.5) * sc;
+    u[2] = ((double)c2 + 0.5) * sc;
+    u[3] = ((double)c3 + 0.5) * sc;
+}
+
+// calc_chi2_string: variance over the corners of the phase minus the phase of the straight
+// string p = (mu, phi, rho, theta); corner i = (iz*2+iy)*2+ix sits at (ix,iy,iz) - 1/2.
+struct Phases {
+    double v[8];
+};
+
+// Out of line (one copy of its 8 atan2 instead of one per call site), with every argument
+// passed by value so that nothing of the caller's search state has to live in memory.
+__device__ __noinline__ double chi2_string_v(Phases ph, double p0, double p1, double p2, double p3)
+{
+#pragma clang fp contract(off)
+    double mu = p0, phi = p1, u0 = p2 * cos(p3), v0 = p2 * sin(p3);
+    double cm = mu, sm = sqrt(1.0 - mu * mu), cp = cos(phi), sp = sin(phi);
+    double a = 0, b = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        double x = (i & 1) - 0.5, y = ((i >> 1) & 1) - 0.5, z = (i >> 2) - 0.5;
+        double du = -u0 + cm * (x * cp + y * sp) - z * sm;
+        double dv = -v0 - x * sp + y * cp;
+        double d = ph.v[i] - atan2(dv, du);
+        if (d >= PI_T) d = d - TWOPI_T;
+        else if (d < -PI_T) d = d + TWOPI_T;
+        a = a + d * d;
+        b = b + d;
+    }
+    a = a / 8;
+    b = b / 8;
+    return a - b * b;
+}
+
+__device__ __forceinline__ double chi2_string(const Phases& ph, const double p[4])
+{
+    return chi2_string_v(ph, p[0], p[1], p[2], p[3]);
+}
+
+// string_length: chord of pos + t dir through [-1/2, 1/2]^3, zero when an exit lies beyond
+// the reference's cut (just below sqrt(3)/2).
+__device__ double string_length(const double pos[3], const double dir[3])
+{
+#pragma clang fp contract(off)
+    const double atmax = 0.867, atmax0 = 0.86602540378378444;
+    double tp = atmax, tn = -atmax;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            double w = j - 0.5;
+            w = w - pos[i];
+            if (fabs(w) > atmax * fabs(dir[i])) continue;
+            double t = w / dir[i];
+            if (t > 0) { if (t < tp) tp = t; }
+            else { if (t > tn) tn = t; }
+        }
+    return (tp < atmax0 && tn > -atmax0) ? tp - tn : 0.0;
+}
+
+__device__ __forceinline__ bool out_of_box(const double t[4])
+{
+    const double mx[4] = {1.0, PI_T, 0.86602540378443865, PI_T};
+    const double mn[4] = {-1.0, -PI_T, 0.0, -PI_T};
+    bool o = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o = o || t[k] > mx[k] || t[k] < mn[k];
+    return o;
+}
+
+// packed-upper Cholesky inverse of a 4x4 SPD matrix, A(r,c) at r + c(c-1)/2 - 1 (1-based),
+// in the operation order of the dpptrf('U') + dpptri('U') restatement of the oracle.
+__device__ void pp_inverse4(double ap[10])
+{
+#pragma clang fp contract(off)
+#define AP(r, c) ap[(r) + (c) * ((c) - 1) / 2 - 1]
+#pragma unroll
+    for (int j = 1; j <= 4; j++) {
+#pragma unroll
+        for (int i = 1; i < j; i++) {
+            double s = AP(i, j);
+#pragma unroll
+            for (int k = 1; k < i; k++) s -= AP(k, i) * AP(k, j);
+            AP(i, j) = s / AP(i, i);
+        }
+        double s = AP(j, j);
+#pragma unroll
+        for (int k = 1; k < j; k++) s -= AP(k, j) * AP(k, j);
+        AP(j, j) = sqrt(s);
+    }
+#pragma unroll
+    for (int j = 1; j <= 4; j++) {
+        AP(j, j) = 1 / AP(j, j);
+        double ajj = -AP(j, j);
+        double col[4];
+#pragma unroll
+        for (int i = 1; i < j; i++) {
+            double s = 0;
+#pragma unroll
+            for (int k = i; k < j; k++) s += AP(i, k) * AP(k, j);
+            col[i - 1] = s * ajj;
+        }
+#pragma unroll
+        for (int i = 1; i < j; i++) AP(i, j) = col[i - 1];
+    }
+#pragma unroll
+    for (int i = 1; i <= 4; i++)
+#pragma unroll
+        for (int j = i; j <= 4; j++) {
+            double s = 0;
+#pragma unroll
+            for (int k = j; k <= 4; k++) s += AP(i, k) * AP(j, k);
+            AP(i, j) = s;
+        }
+#undef AP
+}
+
+__device__ __forceinline__ void copy4(double* d, const double* s)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) d[k] = s[k];
+}
+
+// position_string: 1001 random starts keep the best five (the first two unsorted, as in the
+// reference), then the reference's simplex iteration until chi2 <= 0.05 or 1000 iterations.
+__device__ void position_string(const Phases& ph, uint32_t k0, uint32_t k1, int64_t s, double pos[3],
+                                double dir[3], double& gof, double& sl)
+{
+#pragma clang fp contract(off)
+    const double chi2max = 8 * PI_T * PI_T, tol = 0.05;
+    const double mx[4] = {1.0, PI_T, 0.86602540378443865, PI_T};
+    const double mn[4] = {-1.0, -PI_T, 0.0, -PI_T};
+    double p[NSIMP + 1][4], c2[NSIMP + 1], t[4], c[4], sv[4];
+#pragma unroll
+    for (int j = 0; j <= NSIMP; j++) {
+        c2[j] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) p[j][k] = 0;
+    }
+    int n0 = -1;
+    for (int i = 0; i <= MAXIT; i++) {
+        double x[4];
+        uniforms(k0, k1, s, i, x);
+#pragma unroll
+        for (int k = 0; k < 4; k++) t[k] = mx[k] * x[k] + mn[k] * (1 - x[k]);
+        double chi2 = chi2_string(ph, t);
+        if (n0 <= 0) {
+            n0 = n0 + 1;
+            if (n0 == 0) { c2[0] = chi2; copy4(p[0], t); }
+            else { c2[1] = chi2; copy4(p[1], t); }
+            continue;
+        }
+        // c2[n0] by selects (n0 is 1..NSIMP here), so that c2 stays in registers
+        double cn = n0 == 1 ? c2[1] : n0 == 2 ? c2[2] : n0 == 3 ? c2[3] : c2[4];
+        if (n0 < NSIMP || chi2 <= cn) {
+            n0 = n0 + 1 < NSIMP ? n0 + 1 : NSIMP;
+            // insertion from slot n0 downwards: shift while chi2 <= the slot above
+            bool placed = false;
+#pragma unroll
+            for (int j = NSIMP; j >= 1; j--) {
+                if (j <= n0 && !placed) {
+                    if (chi2 <= c2[j - 1]) {
+                        c2[j] = c2[j - 1];
+                        copy4(p[j], p[j - 1]);
+                    } else {
+                        c2[j] = chi2;
+                        copy4(p[j], t);
+                        placed = true;
+                    }
+                }
+            }
+            if (!placed) { c2[0] = chi2; copy4(p[0], t); }
+        }
+    }
+    int it = 0;
+    while (c2[0] > tol && it < MAXIT) {
+        it = it + 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = 0;
+#pragma unroll
+        for (int i0 = 0; i0 < NSIMP; i0++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) c[k] = c[k] + p[i0][k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = c[k] / NSIMP;
+#pragma unroll
+        for (int k = 0; k < 4; k++) t[k] = 2 * c[k] - p[NSIMP][k];
+        double chi2 = out_of_box(t) ? chi2max : chi2_string(ph, t);
+        if (chi2 < c2[NSIMP]) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) sv[k] = 3 * c[k] - 4 * p[NSIMP][k];
+            double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+            if (chi2 <= chi2s) { c2[NSIMP] = chi2; copy4(p[NSIMP], t); }
+            else { c2[NSIMP] = chi2s; copy4(p[NSIMP], sv); }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) sv[k] = 1.5 * c[k] - 0.5 * p[NSIMP][k];
+            double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+            if (chi2s <= c2[NSIMP]) {
+                c2[NSIMP] = chi2s;
+                copy4(p[NSIMP], sv);
+            } else {
+#pragma unroll
+                for (int i0 = 1; i0 <= NSIMP; i0++) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) p[i0][k] = 0.5 * (p[i0][k] + p[0][k]);
+                    c2[i0] = chi2_string(ph, p[i0]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i0 = NSIMP - 1; i0 >= 0; i0--)
+#pragma unroll
+            for (int j = 0; j <= i0; j++)
+                if (c2[j] > c2[j + 1]) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        double q = p[j][k];
+                        p[j][k] = p[j + 1][k];
+                        p[j + 1][k] = q;
+                    }
+                    double q = c2[j];
+                    c2[j] = c2[j + 1];
+                    c2[j + 1] = q;
+                }
+    }
+    (void)mx;
+    (void)mn;
+    double mu = p[0][0], phi = p[0][1];
+    double u0 = p[0][2] * cos(p[0][3]), v0 = p[0][2] * sin(p[0][3]);
+    double sm = sqrt(1.0 - mu * mu);
+    dir[0] = sm * cos(phi);
+    dir[1] = sm * sin(phi);
+    dir[2] = mu;
+    pos[0] = u0 * mu * cos(phi) - v0 * sin(phi);
+    pos[1] = u0 * mu * sin(phi) + v0 * cos(phi);
+    pos[2] = -u0 * sm;
+    gof = exp(-c2[0] / 2);
+    sl = string_length(pos, dir);
+}
+
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const typename V2<T>::type* __restrict__ phi,
+                                                               const short2* __restrict__ cphi,
+                                                               const typename V2<T>::type* __restrict__ pi,
+                                                               const short2* __restrict__ cpi,
+                                                               const int32_t* __restrict__ cells, int64_t ns,
+                                                               uint32_t k0, uint32_t k1, double dspa,
+                                                               double* __restrict__ rec, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[5][STRING_NT];
+    const int64_t s = blockIdx.x * (int64_t)STRING_NT + threadIdx.x;
+    double acc[5] = {0, 0, 0, 0, 0};
+    if (s < ns) {
+        const int64_t NN = (int64_t)N * N;
+        const int64_t i = cells[s];
+        const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        auto corner = [&](int c) {
+            int cx = x + (c & 1), cy = y + ((c >> 1) & 1), cz = z + (c >> 2);
+            cx = cx == N ? 0 : cx;
+            cy = cy == N ? 0 : cy;
+            cz = cz == N ? 0 : cz;
+            return (int64_t)cz * NN + (int64_t)cy * N + cx;
+        };
+        Phases ph;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            double2 f = ldf<T, COMP>(phi, cphi, corner(c));
+            ph.v[c] = atan2(f.y, f.x);
+        }
+        double pos[3], dir[3], gof, sl;
+        position_string(ph, k0, k1, s, pos, dir, gof, sl);
+        // the corner values are read again here rather than kept live through the fit
+        double2 f[8], q[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            f[c] = ldf<T, COMP>(phi, cphi, corner(c));
+            q[c] = ldf<T, COMP>(pi, cpi, corner(c));
+        }
+
+        // velocity_string: least-squares fit of phi and pi to a + b.d over the corners
+        double wr[4] = {0, 0, 0, 0}, wi[4] = {0, 0, 0, 0}, dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
+        double mat[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            double d[4] = {1.0, (c & 1) - pos[0] - 0.5, ((c >> 1) & 1) - pos[1] - 0.5, (c >> 2) - pos[2] - 0.5};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                wr[m] = wr[m] + d[m] * f[c].x;
+                wi[m] = wi[m] + d[m] * f[c].y;
+                dr[m] = dr[m] + d[m] * q[c].x;
+                di[m] = di[m] + d[m] * q[c].y;
+            }
+#pragma unroll
+            for (int cc = 1; cc <= 4; cc++)
+#pragma unroll
+                for (int r = 1; r <= cc; r++) mat[r + cc * (cc - 1) / 2 - 1] += d[r - 1] * d[cc - 1];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            wr[m] /= 8;
+            wi[m] /= 8;
+            dr[m] /= 8;
+            di[m] /= 8;
+        }
+#pragma unroll
+        for (int m = 0; m < 10; m++) mat[m] /= 8;
+        pp_inverse4(mat);
+        double w0r[4] = {0, 0, 0, 0}, w0i[4] = {0, 0, 0, 0}, d0r = 0, d0i = 0;
+#pragma unroll
+        for (int ii = 1; ii <= 4; ii++)
+#pragma unroll
+            for (int jj = 1; jj <= 4; jj++) {
+                int kk = ii > jj ? ii : jj;
+                kk = kk * (kk - 1) / 2 + (ii < jj ? ii : jj) - 1;
+                w0r[ii - 1] += mat[kk] * wr[jj - 1];
+                w0i[ii - 1] += mat[kk] * wi[jj - 1];
+                if (ii == 1) {
+                    d0r += mat[kk] * dr[jj - 1];
+                    d0i += mat[kk] * di[jj - 1];
+                }
+            }
+        double a = 0, b = 0, cc = 0;
+#pragma unroll
+        for (int m = 1; m < 4; m++) {
+            a += w0r[m] * w0r[m];
+            b += w0r[m] * w0i[m];
+            cc += w0i[m] * w0i[m];
+        }
+        double det = a * cc - b * b;
+        double vel[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            double v = (d0r * cc - d0i * b) * w0r[m + 1] + (-d0r * b + d0i * a) * w0i[m + 1];
+            vel[m] = -v / det * dspa;
+        }
+
+        double* o = rec + 11 * s;
+        o[0] = pos[0] + ((double)(x + 1) - 0.5);
+        o[1] = pos[1] + ((double)(y + 1) - 0.5);
+        o[2] = pos[2] + ((double)(z + 1) - 0.5);
+        o[3] = dir[0];
+        o[4] = dir[1];
+        o[5] = dir[2];
+        o[6] = gof;
+        o[7] = sl;
+        o[8] = vel[0];
+        o[9] = vel[1];
+        o[10] = vel[2];
+        acc[0] = sl;
+        double v2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+        if (v2 < 1.0) {
+            acc[1] = 1;
+            acc[2] = sqrt(v2);
+            acc[3] = v2;
+            acc[4] = 1 / sqrt(1 - v2);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double t = 0;
+        for (int j = 0; j < STRING_NT; j++) t += sh[threadIdx.x][j];
+        partial[(int64_t)blockIdx.x * 5 + threadIdx.x] = t;
+    }
+}
+
+}  // namespace
+
+template <typename T, bool COMP>
+void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
+                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, double* rec,
+                       double* partial)
+{
+    using T2 = typename V2<T>::type;
+    if (ns <= 0) return;
+    const int64_t nblk = (ns + STRING_NT - 1) / STRING_NT;
+    hipLaunchKernelGGL((string_fit_kernel<T, COMP>), dim3((unsigned)nblk), dim3(STRING_NT), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, key0, key1, dspa, rec,
+                       partial);
+    AXG_HIP(hipGetLastError());
+}
+
+template void launch_string_fit<double, false>(hipStream_t, int, const void*, const void*, const void*, const void*,
+                                               const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
+template void launch_string_fit<float, false>(hipStream_t, int, const void*, const void*, const void*, const void*,
+                                              const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
+template void launch_string_fit<float, true>(hipStream_t, int, const void*, const void*, const void*, const void*,
+                                             const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
+
+}  // namespace axg

@@ -38,6 +38,8 @@ program axion_3d
   type(axg_params) :: prm
   type(c_ptr) :: h
   integer(c_int32_t) :: nsteps
+  integer(c_int32_t) :: seed(2)   ! the two integers of key ss (subroutines.f90:49-95)
+  integer(8) :: clk
   integer(8) :: clock0, clock_rate
 
   ! ------------------------------------------------------------ command line (main.f90:16-33)
@@ -92,6 +94,17 @@ program axion_3d
      tmpdir = rootdir
   end if
   ss = ini_string('ss')
+  ! seed of the string fits' generator: two integers, or the clock when empty
+  ! (init_rand_gen, subroutines.f90:61-74)
+  if (ss /= '') then
+     read(ss, *, iostat=info) seed(1:2)
+     if (info /= 0) call die('error: cannot read seed; we need two integer*4')
+  else
+     call system_clock(count=clk)
+     clk = mod(clk, 2147483648_8*2147483648_8)
+     seed(2) = int(clk/2147483648_8, c_int32_t)
+     seed(1) = int(clk - 2147483648_8*seed(2), c_int32_t)
+  end if
   ngrid = ini_int('ngrid')
   if (mod(ngrid, 2_c_int64_t) /= 0) call die('error: ngrid should be even')
   n3 = ngrid**3
@@ -414,6 +427,7 @@ contains
     integer, intent(in) :: idx
     double precision, intent(in) :: tau
     type(axg_defect_out) :: dc
+    type(axg_string_out) :: so
     double precision :: xi(3), largea(3), vel(3), sw(11), crho(3), rho, xi0, xi1, gam
     double precision, allocatable :: q(:), pw(:), nu(:), de(:), cv(:), nm(:)
     integer(c_int32_t), allocatable :: ml(:)
@@ -440,10 +454,22 @@ contains
           deallocate(ml)
        end if
     end if
-    ! string length, wall area and velocities (defects_3d.f90:276-283) are not computed by this
-    ! build yet: their parameters are reported as zero
+    ! string length and velocities (add_string_length, defects_3d.f90:276-279, group_strings
+    ! = F: the three lengths coincide); the generator's stream moves on with idx as the
+    ! reference's does between calls.  Wall areas (position_wall) are not computed by this
+    ! build: their parameters are reported as zero.
+    if (id_strings) then
+       call axg_check(axg_strings(h, seed(1), seed(2) + int(idx, c_int32_t), so), h, 'axg_strings')
+       xi(1:3) = so%length
+       vel(1:3) = so%vel
+       if (so%n_vel == 0) write(*, '(A)') ' velocity cannot be computed because there are no strings'
+    end if
     xi(1:3) = (tau/prm%dspa)**2*xi(1:3)/dble(n3)/4
     largea(1:3) = (tau/prm%dspa)*largea(1:3)/dble(n3)/2
+    if (id_strings) then  ! evolve.f90:192-196
+       write(*, '(A,F14.6,3E20.10e3)') '  tau, string parameter: ', tau, xi(1:3)
+       write(*, '(A,F14.6,3E20.10e3)') '  tau, string velocity parameter: ', tau, vel(1:3)
+    end if
     sw = [tau, tau**2*zeta, xi(1:3), largea(1:3), vel(1:3)]
     crho = 0d0
     if (spec_est) then

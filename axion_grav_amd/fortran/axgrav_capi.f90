@@ -43,6 +43,13 @@ module axgrav_capi
      integer(c_int64_t) :: n_string = 0, n_wall = 0, n_mask = 0
   end type axg_defect_out
 
+  ! struct axg_string_out
+  type, bind(C) :: axg_string_out
+     integer(c_int64_t) :: n_string = 0, n_vel = 0
+     real(c_double) :: length = 0
+     real(c_double) :: vel(3) = 0
+  end type axg_string_out
+
   interface
      integer(c_int) function axg_derive(k, p, zeta, width_cr, thickness_cr) bind(C, name="axg_derive")
        import :: c_int, c_double, axg_keys, axg_params
@@ -144,6 +151,21 @@ module axgrav_capi
        integer(c_int64_t), value :: cap
        integer(c_int64_t), intent(out) :: count
      end function axg_get_mask
+
+     integer(c_int) function axg_strings(h, seed0, seed1, out) bind(C, name="axg_strings")
+       import :: c_int, c_ptr, c_int32_t, axg_string_out
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: seed0, seed1   ! uint32_t in C: the same bits
+       type(axg_string_out), intent(out) :: out
+     end function axg_strings
+
+     integer(c_int) function axg_get_strings(h, rec, cap, count) bind(C, name="axg_get_strings")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: rec(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: count
+     end function axg_get_strings
 
      integer(c_int) function axg_field_spectrum(h, tau, qmid, power, conv, num, rho, nbins) &
           bind(C, name="axg_field_spectrum")

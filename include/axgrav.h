@@ -132,6 +132,30 @@ int axg_get_mask(axg_handle* h, int32_t* ixyz, int64_t cap, int64_t* count);
 /* The per-cell flag bits (1 string, 2 wall, 4 mask), N^3 int8 in site order. */
 int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite);
 
+typedef struct axg_string_out {
+    int64_t n_string;   /* string cells fitted */
+    int64_t n_vel;      /* strings with |v| < 1 (the ones add_string_length averages over) */
+    double length;      /* sum of the chord lengths sl over the string cells, in lattice units:
+                           id_defects' sl(1) = sl(2) = sl(3) with group_strings = F */
+    double vel[3];      /* mean |v|, mean v^2, mean 1/sqrt(1-v^2) over those (svel) */
+} axg_string_out;
+
+/* String length and velocity (defects_3d.f90:167-205 with position_string 435-603,
+ * calc_chi2_string 384-433, string_length 1027-1056, velocity_string 605-690 of the VEL = 0
+ * build, and add_string_length 1654-1733 with group_strings = F), one GPU thread per string
+ * cell of the last axg_defects.  Each fit draws its 1001 starting points from Philox4x32-10
+ * keyed by (seed0, seed1) with the string's index in site order as counter (the reference's
+ * MKL VSL stream, positioned per string, is not reproduced: results agree with the reference's
+ * algorithm to the fit's tolerance, not bit for bit).  The driver's string parameter is
+ * (tau/dspa)^2 * length / N^3 / 4 (evolve.f90:191). */
+int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_string_out* out);
+/* The per-string records of the last axg_strings: 11 doubles per string, pos(3), dir(3), gof,
+ * sl, vel(3), in site order (the arr_string list of defects_3d.f90:8-20).  pos is the point of
+ * the fitted line nearest the cell centre plus ixyz - 1/2 with ixyz the cell's 1-based corner
+ * (defects_3d.f90:202), i.e. in site coordinates counted from 0.  cap = room in strings;
+ * *count = strings written; rec == NULL or cap == 0: size query. */
+int axg_get_strings(axg_handle* h, double* rec, int64_t cap, int64_t* count);
+
 /* spectrum_estimation's deterministic part (spectrum.f90:242-286 with compute_matrix_deconv
  * 344-444 and compute_conv_spec 466-567, degrade_spec = 0): the masked pi spectrum
  * deconvolved with the mask window.  tau is tau_id(idx).  power = the deconvolved

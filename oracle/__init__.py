@@ -60,6 +60,14 @@ def lib():
         L.orc_philox.argtypes = [p, C.c_uint32, C.c_uint32]
         L.orc_ic_mode.argtypes = [C.c_uint64, i64, i64, i64, i64, p, p]
         L.orc_ic_flat.argtypes = [p, p, i64, C.c_uint64, d, d]
+        L.orc_chi2_string.restype = d
+        L.orc_chi2_string.argtypes = [p, p]
+        L.orc_string_length.restype = d
+        L.orc_string_length.argtypes = [p, p]
+        L.orc_position_string.argtypes = [p, C.c_uint32, C.c_uint32, i64, p, p, p, p]
+        L.orc_velocity_string.argtypes = [p, p, i64, i64, i64, i64, p, d, p]
+        L.orc_strings.restype = i64
+        L.orc_strings.argtypes = [p, p, i64, p, C.c_uint32, C.c_uint32, d, p, p]
         _LIB = L
     return _LIB
 
@@ -235,3 +243,39 @@ def sym_from_packed(ap, nb):
             M[j - 1, i - 1] = v
             M[i - 1, j - 1] = v
     return M
+
+
+# ------------------------------------------------------------------ strings
+
+def chi2_string(ph, prm):
+    return lib().orc_chi2_string(_p(np.ascontiguousarray(ph, np.float64)), _p(np.ascontiguousarray(prm, np.float64)))
+
+
+def string_length(pos, dirv):
+    return lib().orc_string_length(_p(np.ascontiguousarray(pos, np.float64)), _p(np.ascontiguousarray(dirv, np.float64)))
+
+
+def position_string(ph, key0, key1, s):
+    """Fit of one string cell: (pos relative to the cell centre, dir, gof, sl)."""
+    pos, dirv, g, sl = np.zeros(3), np.zeros(3), np.zeros(1), np.zeros(1)
+    lib().orc_position_string(_p(np.ascontiguousarray(ph, np.float64)), key0, key1, s, _p(pos), _p(dirv), _p(g), _p(sl))
+    return pos, dirv, float(g[0]), float(sl[0])
+
+
+def velocity_string(phi, pi, x, y, z, pos, dspa):
+    vel = np.zeros(3)
+    lib().orc_velocity_string(_p(phi), _p(pi), phi.shape[0], x, y, z, _p(np.ascontiguousarray(pos, np.float64)),
+                              dspa, _p(vel))
+    return vel
+
+
+def strings(phi, pi, flags, key0, key1, dspa):
+    """All string cells (flag bit 1) in site order: records (ns, 11) = pos(3, site coordinates
+    counted from 0), dir(3), gof, sl, vel(3); sums = (sum sl, n |v|<1, sum |v|, sum v^2, sum gamma)."""
+    N = phi.shape[0]
+    ns = int(np.count_nonzero(flags & 1))
+    rec = np.zeros((max(ns, 1), 11))
+    sums = np.zeros(5)
+    n = lib().orc_strings(_p(phi), _p(pi), N, _p(np.ascontiguousarray(flags, np.int8)), key0, key1, dspa,
+                          _p(rec), _p(sums))
+    return rec[:n], sums

@@ -782,3 +782,273 @@ void orc_ic_flat(double* phi, double* pi, int64_t N, uint64_t seed, double kmax,
     double scale = 1.0 / sqrt(s / (double)n3);
     for (int64_t k = 0; k < 2 * n3; k++) phi[k] = phi[k] * scale;
 }
+
+/* ------------------------------------------------------------ string fit and length */
+
+#define STR_NSIMP 4      /* num_simp_string_pos (defects_3d.f90:26) */
+#define STR_MAXIT 1000   /* max_iter_string_pos */
+
+/* defects_3d.f90:384-433 calc_chi2_string: variance over the 8 corners of the phase minus the
+ * phase of a straight string with direction (sqrt(1-mu^2) cos phi, sqrt(1-mu^2) sin phi, mu)
+ * at distance rho, angle theta from the cell centre; p = (mu, phi, rho, theta), corner
+ * i = (iz*2+iy)*2+ix at (ix,iy,iz) - 1/2. */
+double orc_chi2_string(const double* ph, const double* p)
+{
+    double mu = p[0], phi = p[1], u0 = p[2] * cos(p[3]), v0 = p[2] * sin(p[3]);
+    double cm = mu, sm = sqrt(1.0 - mu * mu), cp = cos(phi), sp = sin(phi);
+    double a = 0, b = 0;
+    for (int iz = 0; iz < 2; iz++)
+        for (int iy = 0; iy < 2; iy++)
+            for (int ix = 0; ix < 2; ix++) {
+                double x = ix - 0.5, y = iy - 0.5, z = iz - 0.5;
+                int i = (iz * 2 + iy) * 2 + ix;
+                double du = -u0 + cm * (x * cp + y * sp) - z * sm;
+                double dv = -v0 - x * sp + y * cp;
+                double d = ph[i] - atan2(dv, du);
+                if (d >= PI_T) d = d - TWOPI_T;
+                else if (d < -PI_T) d = d + TWOPI_T;
+                a = a + d * d;
+                b = b + d;
+            }
+    a = a / 8;
+    b = b / 8;
+    return a - b * b;
+}
+
+/* defects_3d.f90:1027-1056 string_length: the chord of the line pos + t dir through the cell
+ * [-1/2, 1/2]^3 (zero when an exit lies beyond sqrt(3)/2). */
+double orc_string_length(const double* pos, const double* dir)
+{
+    const double atmax = 0.867, atmax0 = 0.86602540378378444;
+    double tp = atmax, tn = -atmax;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 2; j++) {
+            double w = j - 0.5;
+            w = w - pos[i];
+            if (fabs(w) > atmax * fabs(dir[i])) continue;
+            double t = w / dir[i];
+            if (t > 0) { if (t < tp) tp = t; }
+            else { if (t > tn) tn = t; }
+        }
+    return (tp < atmax0 && tn > -atmax0) ? tp - tn : 0.0;
+}
+
+/* The uniforms of start point i of string s: one Philox4x32-10 block keyed by the seed.  The
+ * reference draws them from an MKL VSL MCG59 stream positioned at s*4004 (defects_3d.f90:
+ * 137-162, 465); this keeps the same "stream per string" structure with a counter-based
+ * generator, so the result does not depend on the thread count. */
+static void string_uniforms(uint32_t key0, uint32_t key1, int64_t s, int i, double* u)
+{
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)((uint64_t)s >> 32), 0x73747267u};
+    philox4x32_10(c, key0, key1);
+    for (int k = 0; k < 4; k++) u[k] = ((double)c[k] + 0.5) * 2.3283064365386963e-10;
+}
+
+static int str_out_of_box(const double* t, const double* mx, const double* mn)
+{
+    for (int k = 0; k < STR_NSIMP; k++)
+        if (t[k] > mx[k] || t[k] < mn[k]) return 1;
+    return 0;
+}
+
+/* defects_3d.f90:435-603 position_string: 1001 random starting points keep the best five
+ * (the first two unsorted, as there), then the simplex iteration of the reference (reflection
+ * 2c - p, expansion 3c - 4p, contraction 1.5c - 0.5p, else shrink towards the best point,
+ * bubble sort) until chi2 <= 0.05 or 1000 iterations; then position, direction, goodness of
+ * fit and the chord length.  s = the string's index in site order (its RNG stream). */
+void orc_position_string(const double* ph, uint32_t key0, uint32_t key1, int64_t s, double* pos,
+                         double* dir, double* gof, double* sl)
+{
+    const double chi2max = 8 * PI_T * PI_T, tol = 0.05;
+    const double mx[STR_NSIMP] = {1.0, PI_T, 0.86602540378443865, PI_T};
+    const double mn[STR_NSIMP] = {-1.0, -PI_T, 0.0, -PI_T};
+    double p[STR_NSIMP + 1][STR_NSIMP], c2[STR_NSIMP + 1], t[STR_NSIMP], c[STR_NSIMP], sv[STR_NSIMP];
+    memset(p, 0, sizeof p);
+    memset(c2, 0, sizeof c2);
+    int n0 = -1;
+    for (int i = 0; i <= STR_MAXIT; i++) {
+        double x[4];
+        string_uniforms(key0, key1, s, i, x);
+        for (int k = 0; k < STR_NSIMP; k++) t[k] = mx[k] * x[k] + mn[k] * (1 - x[k]);
+        double chi2 = orc_chi2_string(ph, t);
+        if (n0 <= 0) {
+            n0 = n0 + 1;
+            c2[n0] = chi2;
+            memcpy(p[n0], t, sizeof t);
+            continue;
+        }
+        if (n0 < STR_NSIMP || chi2 <= c2[n0]) {
+            n0 = n0 + 1 < STR_NSIMP ? n0 + 1 : STR_NSIMP;
+            int i0 = n0;
+            while (chi2 <= c2[i0 - 1]) {
+                if (i0 <= STR_NSIMP) {
+                    c2[i0] = c2[i0 - 1];
+                    memcpy(p[i0], p[i0 - 1], sizeof t);
+                }
+                i0 = i0 - 1;
+                if (i0 == 0) break;
+            }
+            c2[i0] = chi2;
+            memcpy(p[i0], t, sizeof t);
+        }
+    }
+    int it = 0;
+    while (c2[0] > tol && it < STR_MAXIT) {
+        it = it + 1;
+        for (int k = 0; k < STR_NSIMP; k++) c[k] = 0;
+        for (int i0 = 0; i0 < STR_NSIMP; i0++)
+            for (int k = 0; k < STR_NSIMP; k++) c[k] = c[k] + p[i0][k];
+        for (int k = 0; k < STR_NSIMP; k++) c[k] = c[k] / STR_NSIMP;
+        for (int k = 0; k < STR_NSIMP; k++) t[k] = 2 * c[k] - p[STR_NSIMP][k];
+        double chi2 = str_out_of_box(t, mx, mn) ? chi2max : orc_chi2_string(ph, t);
+        if (chi2 < c2[STR_NSIMP]) {
+            for (int k = 0; k < STR_NSIMP; k++) sv[k] = 3 * c[k] - 4 * p[STR_NSIMP][k];
+            double chi2s = str_out_of_box(sv, mx, mn) ? chi2max : orc_chi2_string(ph, sv);
+            if (chi2 <= chi2s) { c2[STR_NSIMP] = chi2; memcpy(p[STR_NSIMP], t, sizeof t); }
+            else { c2[STR_NSIMP] = chi2s; memcpy(p[STR_NSIMP], sv, sizeof t); }
+        } else {
+            for (int k = 0; k < STR_NSIMP; k++) sv[k] = 1.5 * c[k] - 0.5 * p[STR_NSIMP][k];
+            double chi2s = str_out_of_box(sv, mx, mn) ? chi2max : orc_chi2_string(ph, sv);
+            if (chi2s <= c2[STR_NSIMP]) {
+                c2[STR_NSIMP] = chi2s;
+                memcpy(p[STR_NSIMP], sv, sizeof t);
+            } else {
+                for (int i0 = 1; i0 <= STR_NSIMP; i0++) {
+                    for (int k = 0; k < STR_NSIMP; k++) p[i0][k] = 0.5 * (p[i0][k] + p[0][k]);
+                    c2[i0] = orc_chi2_string(ph, p[i0]);
+                }
+            }
+        }
+        for (int i0 = STR_NSIMP - 1; i0 >= 0; i0--)
+            for (int j = 0; j <= i0; j++)
+                if (c2[j] > c2[j + 1]) {
+                    double tmp[STR_NSIMP];
+                    memcpy(tmp, p[j], sizeof tmp);
+                    memcpy(p[j], p[j + 1], sizeof tmp);
+                    memcpy(p[j + 1], tmp, sizeof tmp);
+                    double q = c2[j]; c2[j] = c2[j + 1]; c2[j + 1] = q;
+                }
+    }
+    double mu = p[0][0], phi = p[0][1];
+    double u0 = p[0][2] * cos(p[0][3]), v0 = p[0][2] * sin(p[0][3]);
+    double sm = sqrt(1.0 - mu * mu);
+    dir[0] = sm * cos(phi);
+    dir[1] = sm * sin(phi);
+    dir[2] = mu;
+    pos[0] = u0 * mu * cos(phi) - v0 * sin(phi);
+    pos[1] = u0 * mu * sin(phi) + v0 * cos(phi);
+    pos[2] = -u0 * sm;
+    *gof = exp(-c2[0] / 2);
+    *sl = orc_string_length(pos, dir);
+}
+
+/* defects_3d.f90:605-690 velocity_string (the VEL = 0 build): least-squares fit of phi and pi
+ * to a + b.d over the 8 corners, d = corner - (pos + 1/2), through the packed-upper inverse of
+ * the 4x4 normal matrix; vel = -[(x c - y b) u + (-x b + y a) v] / det * dspa with x + i y the
+ * fitted pi, u + i v the fitted gradient, a = u.u, b = u.v, c = v.v, det = a c - b^2.
+ * (x,y,z) is the cell's corner (0-based); pos is relative to the cell centre. */
+void orc_velocity_string(const double* phi, const double* pi, int64_t N, int64_t x, int64_t y, int64_t z,
+                         const double* pos, double dspa, double* vel)
+{
+    const oc* f = (const oc*)phi;
+    const oc* q = (const oc*)pi;
+    double wr[4] = {0}, wi[4] = {0}, dr[4] = {0}, di[4] = {0}, mat[10] = {0}, d[4];
+    d[0] = 1.0;
+    for (int k = 0; k < 2; k++) {
+        int64_t iz = (z + k) % N;
+        d[3] = k - pos[2] - 0.5;
+        for (int j = 0; j < 2; j++) {
+            int64_t iy = (y + j) % N;
+            d[2] = j - pos[1] - 0.5;
+            for (int i = 0; i < 2; i++) {
+                int64_t ix = (x + i) % N;
+                d[1] = i - pos[0] - 0.5;
+                oc a = f[ix + N * (iy + N * iz)], b = q[ix + N * (iy + N * iz)];
+                for (int m = 0; m < 4; m++) {
+                    wr[m] = wr[m] + d[m] * a.re;
+                    wi[m] = wi[m] + d[m] * a.im;
+                    dr[m] = dr[m] + d[m] * b.re;
+                    di[m] = di[m] + d[m] * b.im;
+                }
+                /* dspr('U'): A(r,c) += d_r d_c for r <= c, packed at r + c(c-1)/2 (1-based) */
+                for (int cc = 1; cc <= 4; cc++)
+                    for (int r = 1; r <= cc; r++) mat[r + cc * (cc - 1) / 2 - 1] += d[r - 1] * d[cc - 1];
+            }
+        }
+    }
+    for (int m = 0; m < 4; m++) { wr[m] /= 8; wi[m] /= 8; dr[m] /= 8; di[m] /= 8; }
+    for (int m = 0; m < 10; m++) mat[m] /= 8;
+    orc_pp_inverse(4, mat);
+    double w0r[4] = {0}, w0i[4] = {0}, d0r[4] = {0}, d0i[4] = {0};
+    for (int ii = 1; ii <= 4; ii++)
+        for (int jj = 1; jj <= 4; jj++) {
+            int kk = ii > jj ? ii : jj;
+            kk = kk * (kk - 1) / 2 + (ii < jj ? ii : jj) - 1;
+            w0r[ii - 1] += mat[kk] * wr[jj - 1];
+            w0i[ii - 1] += mat[kk] * wi[jj - 1];
+            d0r[ii - 1] += mat[kk] * dr[jj - 1];
+            d0i[ii - 1] += mat[kk] * di[jj - 1];
+        }
+    double xx = d0r[0], yy = d0i[0];
+    double a = 0, b = 0, c = 0;
+    for (int m = 1; m < 4; m++) {
+        a += w0r[m] * w0r[m];
+        b += w0r[m] * w0i[m];
+        c += w0i[m] * w0i[m];
+    }
+    double det = a * c - b * b;
+    for (int m = 0; m < 3; m++) {
+        double v = (xx * c - yy * b) * w0r[m + 1] + (-xx * b + yy * a) * w0i[m + 1];
+        vel[m] = -v / det * dspa;
+    }
+}
+
+/* id_defects' string loop (defects_3d.f90:167-205: cells with the string flag in site order,
+ * 8 corner phases atan2(Im, Re), position_string, velocity_string, pos + ixyz - 1/2 with ixyz
+ * the 1-based corner, i.e. site coordinates counted from 0) and add_string_length (1654-1733, group_strings = F).  rec (or NULL)
+ * receives 11 doubles per string: pos(3), dir(3), gof, sl, vel(3).  sums = {sum sl, n with
+ * |v| < 1, sum |v|, sum v^2, sum gamma} over those.  Returns the string count. */
+int64_t orc_strings(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
+                    uint32_t key1, double dspa, double* rec, double* sums)
+{
+    const oc* f = (const oc*)phi;
+    int64_t n3 = N * N * N, ns = 0;
+    for (int64_t i = 0; i < n3; i++) ns += (flags[i] & 1) != 0;
+    int64_t* cell = (int64_t*)malloc(sizeof(int64_t) * (ns > 0 ? ns : 1));
+    double* r = (double*)malloc(sizeof(double) * 11 * (ns > 0 ? ns : 1));
+    ns = 0;
+    for (int64_t i = 0; i < n3; i++)
+        if (flags[i] & 1) cell[ns++] = i;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t s = 0; s < ns; s++) {
+        int64_t i = cell[s], x = i % N, y = (i / N) % N, z = i / (N * N);
+        double ph[8];
+        for (int c = 0; c < 8; c++) {
+            int64_t cx = (x + (c & 1)) % N, cy = (y + ((c >> 1) & 1)) % N, cz = (z + ((c >> 2) & 1)) % N;
+            oc v = f[cx + N * (cy + N * cz)];
+            ph[c] = atan2(v.im, v.re);
+        }
+        double* o = r + 11 * s;
+        orc_position_string(ph, key0, key1, s, o, o + 3, o + 6, o + 7);
+        orc_velocity_string(phi, pi, N, x, y, z, o, dspa, o + 8);
+        o[0] += (double)(x + 1) - 0.5;
+        o[1] += (double)(y + 1) - 0.5;
+        o[2] += (double)(z + 1) - 0.5;
+    }
+    double sl = 0, nv = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int64_t s = 0; s < ns; s++) {
+        const double* o = r + 11 * s;
+        sl += o[7];
+        double v2 = o[8] * o[8] + o[9] * o[9] + o[10] * o[10];
+        if (v2 >= 1.0) continue;
+        nv += 1;
+        s1 += sqrt(v2);
+        s2 += v2;
+        s3 += 1 / sqrt(1 - v2);
+    }
+    sums[0] = sl; sums[1] = nv; sums[2] = s1; sums[3] = s2; sums[4] = s3;
+    if (rec) memcpy(rec, r, sizeof(double) * 11 * ns);
+    free(cell);
+    free(r);
+    return ns;
+}

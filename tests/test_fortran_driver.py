@@ -17,7 +17,7 @@ SIGMA = float(1 / (6 * PI_T * np.sqrt(0.5)))  # sigma_cr^2 = 1/2 at g_rel = 90
 
 
 def ini_text(rootdir, **over):
-    keys = dict(dry_run="F", rootdir=rootdir, rootname="run", ss="1", ngrid=32, n_chk=2, num_bin=8,
+    keys = dict(dry_run="F", rootdir=rootdir, rootname="run", ss="1 2", ngrid=32, n_chk=2, num_bin=8,
                 initial_loop="T", loop_radius=0.5, loop_velocity=0.0, loop_velocity_in=0.0,
                 id_strings="T", group_strings="F", out_strings="F", id_walls="T", group_walls="F",
                 out_walls="F", spec_est="T", grav_spec="T", n_id=2, n_mod_eps=1, degrade_spec=0,
@@ -128,6 +128,14 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
         assert np.allclose(s[:, 1], power, rtol=1e-6, atol=1e-12 * np.abs(power).max())
     sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
     assert sw.shape == (2, 12) and np.allclose(sw[:, 1], taus[1:])
+    # string parameter (evolve.f90:191) and velocities from the fits seeded by ss = "1 2" and idx
+    for idx in (1, 2):
+        phi, pi, _ = states[idx]
+        flags, _ = O.classify(phi)
+        _, sums = O.strings(phi, pi, flags, 1, 2 + idx, d["dspa"])
+        xi = (taus[idx] / d["dspa"]) ** 2 * sums[0] / N ** 3 / 4
+        assert np.allclose(sw[idx - 1, 3:6], xi, rtol=0.02), (sw[idx - 1, 3:6], xi)
+        assert np.allclose(sw[idx - 1, 9:12], sums[2:5] / sums[1], rtol=0.02)
     # the final field checkpoint and the resume file (evolve.f90:401-437)
     idx, tau, flag = open(tmp_path / "run_param.chk").read().split()
     assert int(idx) == 2 and abs(float(tau) - 2.0) < 1e-12 and flag == "F"

@@ -248,6 +248,67 @@ def test_initial_field(N, seed, kmax):
     assert not np.any(pi)
 
 
+# ---------------------------------------------------------------- string fits
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_strings_match_oracle(prec):
+    """String fits on the GPU against the oracle on the same field, cell list and seed.  The
+    cell list is bit-exact; each fit runs the same FP64 search from the same Philox starting
+    points, so the searches take the same path and the records differ only by last-bit
+    differences of atan2/sin/cos (GPU vs libm) carried through sqrt(1 - mu^2) near |mu| = 1 and
+    the velocity fit's 1/det (measured: <= 1.2e-6 relative, ~10 % of records above 1e-9).
+    Bars: every record within 1e-5 (a search that took another path differs by O(0.1)), the
+    summed length within 1e-9, the mean velocities within 1e-8, n_vel equal."""
+    say("strings", prec)
+    N = 32
+    prm, P = params(N, grav_spec=False, precision=prec, id_strings=True, id_walls=True, spec_est=True)
+    phi, _ = ic(N, 31, 6.0)
+    rng = np.random.default_rng(4)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        f_phi, f_pi = L.get_fields()       # the values the GPU holds (FP32C rounds them)
+        cnt = L.defects()
+        out = L.strings(1234, 99)
+        rec = L.string_records()
+        flags = L.flags()
+    oflags, ocnt = O.classify(f_phi)
+    assert np.array_equal(flags, oflags) and cnt[0] == ocnt[0] == out["n_string"] > 50
+    orec, osums = O.strings(f_phi, f_pi, oflags, 1234, 99, P["dspa"])
+    assert rec.shape == orec.shape
+    rel = np.abs(rec - orec) / np.maximum(1, np.abs(orec))
+    print(f"  strings {len(rec)}: max rel diff {rel.max():.2e}; length {out['length']:.10f} vs {osums[0]:.10f}")
+    assert rel.max() < 1e-5
+    assert abs(out["length"] - osums[0]) <= 1e-9 * osums[0]
+    assert out["n_vel"] == osums[1]
+    ovel = osums[2:5] / osums[1]
+    assert np.all(np.abs(out["vel"] - ovel) <= 1e-8 * np.abs(ovel))
+
+
+def test_string_velocity_of_moving_string():
+    """A straight string along z moving in x at 0.3: phi = (X - X0 - u tau) + i (Y - Y0) in
+    physical units (pi = -u), so the fitted velocity of each of its cells is (u, 0, 0) to
+    round-off and the chord about one cell."""
+    say("moving string")
+    N, u = 16, 0.3
+    prm, P = params(N, grav_spec=False, id_strings=True)
+    d = P["dspa"]
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    x0, y0 = 7.3, 8.6
+    phi = np.ascontiguousarray((x - x0) * d + 1j * (y - y0) * d)
+    pi = np.full((N, N, N), -u + 0j)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.defects()
+        L.strings(5, 6)
+        rec = L.string_records()
+    on_line = (np.abs(rec[:, 0] - x0) < 1) & (np.abs(rec[:, 1] - y0) < 1)   # site coordinates from 0
+    assert on_line.sum() == N   # one cell per z-plane (other cells sit on the periodic seams)
+    r = rec[on_line]
+    assert np.abs(r[:, 8] - u).max() < 1e-12 and np.abs(r[:, 9:11]).max() < 1e-12
+    assert np.abs(r[:, 7] - 1).max() < 0.15
+
+
 # ---------------------------------------------------------------- FP32C vs FP64
 
 def test_fp32c_tracks_fp64_gw():

@@ -495,3 +495,102 @@ def test_get_dtau_hand_values():
     dt, n = O.get_dtau(1.0, 6.0, SQ_HALF, 2 * PI_T / 16, 64, 1.0)
     om = np.sqrt((2 * PI_T / 16 * 64) ** 2 + (2.449489742783178 * 6.0 * SQ_HALF) ** 2)
     assert n == int(np.ceil(5.0 * om)) and abs(dt - 5.0 / n) < 1e-16
+
+
+# ------------------------------------------------------------------ string fit (defects_3d.f90:384-603, 1027-1056)
+
+def corner_xyz():
+    """Corner i = (iz*2+iy)*2+ix of a cell, relative to the cell centre."""
+    return np.array([[ix - 0.5, iy - 0.5, iz - 0.5] for iz in (0, 1) for iy in (0, 1) for ix in (0, 1)])
+
+
+def test_string_length_closed_forms():
+    r2 = 2 ** -0.5
+    assert O.string_length([0, 0, 0], [0, 0, 1]) == 1.0                      # through the centre along z
+    assert abs(O.string_length([0, 0, 0], [r2, r2, 0]) - 2 ** 0.5) < 1e-15   # face diagonal
+    assert O.string_length([0.3, -0.2, 0], [0, 0, 1]) == 1.0                 # off centre, still crosses z faces
+    assert abs(O.string_length([0, 0.2, 0.1], [1, 0, 0]) - 1.0) < 1e-15
+    # chord from (0.5, 0) to (0, -0.5) of a diagonal line through (0.25, -0.25)
+    assert abs(O.string_length([0.25, -0.25, 0], [r2, r2, 0]) - r2) < 1e-15
+    # the body diagonal exits at sqrt(3)/2, just beyond the reference's cut 0.86602540378378444
+    s3 = 3 ** -0.5
+    assert O.string_length([0, 0, 0], [s3, s3, s3]) == 0.0
+
+
+def test_chi2_vanishes_for_straight_strings():
+    """Phases of a physical straight string through the cell: phi = (x-x0) + i(y-y0) along z, and
+    (y-y0) + i(z-z0) along x.  The reference's model reproduces them up to a constant phase,
+    so its chi2 (a variance) is zero at the string's parameters and positive elsewhere."""
+    c = corner_xyz()
+    x0, y0, z0 = 0.17, -0.08, 0.11
+    ph = np.arctan2(c[:, 1] - y0, c[:, 0] - x0)
+    assert O.chi2_string(ph, [1.0, 0.0, np.hypot(x0, y0), np.arctan2(y0, x0)]) < 1e-28
+    assert O.chi2_string(ph, [1.0, 0.0, np.hypot(x0, y0) + 0.2, np.arctan2(y0, x0)]) > 1e-3
+    # along x: dir = (1,0,0) is mu = 0, phi = 0; the model's (du, dv) = (-u0 - z, -v0 + y)
+    ph = np.arctan2(c[:, 2] - z0, c[:, 1] - y0)
+    u0, v0 = -z0, y0
+    assert O.chi2_string(ph, [0.0, 0.0, np.hypot(u0, v0), np.arctan2(v0, u0)]) < 1e-28
+    assert O.chi2_string(ph, [0.3, 0.0, np.hypot(u0, v0), np.arctan2(v0, u0)]) > 1e-3
+
+
+def test_position_string_recovers_straight_strings():
+    """The fit (random starts + the reference's simplex, stopping at chi2 <= 0.05) finds strings
+    along z through random points: direction within ~20 degrees of z, the chord close to 1."""
+    c = corner_xyz()
+    rng = np.random.default_rng(5)
+    sls, dz = [], []
+    for s in range(200):
+        x0, y0 = rng.uniform(-0.3, 0.3, 2)
+        ph = np.arctan2(c[:, 1] - y0, c[:, 0] - x0)
+        pos, dirv, gof, sl = O.position_string(ph, 11, 22, s)
+        assert gof >= np.exp(-0.05 / 2) * (1 - 1e-12)           # converged: chi2 <= 0.05
+        assert abs(np.linalg.norm(dirv) - 1) < 1e-12
+        sls.append(sl)
+        dz.append(abs(dirv[2]))
+    assert np.median(dz) > 0.97 and np.min(dz) > 0.85
+    assert abs(np.median(sls) - 1) < 0.05
+
+
+def test_velocity_of_moving_string():
+    """A straight string moving with velocity u perpendicular to it: phi = (r-q-u tau).e1 +
+    i (r-q-u tau).e2 in physical units, so pi = -(u.e1) - i (u.e2); the fitted velocity is u
+    exactly (the field is linear, so the least-squares fit is exact)."""
+    N, dspa = 6, 0.25
+    rng = np.random.default_rng(9)
+    for trial in range(5):
+        n = rng.standard_normal(3); n /= np.linalg.norm(n)
+        e1 = np.cross(n, rng.standard_normal(3)); e1 /= np.linalg.norm(e1)
+        e2 = np.cross(n, e1)
+        u = 0.4 * e1 - 0.25 * e2
+        q = rng.uniform(1, 3, 3) * dspa
+        z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+        r = np.stack([x, y, z], -1) * dspa - q
+        phi = np.ascontiguousarray(r @ e1 + 1j * (r @ e2))
+        pi = np.full((N, N, N), -(u @ e1) - 1j * (u @ e2))
+        vel = O.velocity_string(phi, pi, 2, 1, 2, [0.1, -0.2, 0.05], dspa)
+        assert np.abs(vel - u).max() < 1e-12, (vel, u)
+
+
+def test_strings_records_and_sums():
+    """orc_strings over a random field: one record per string cell in site order, positions
+    within sqrt(3)/2 of their cell centre, sums equal to the records' sums."""
+    N = 16
+    P = phys(N)
+    phi, _ = O.ic_flat(N, 3, 6.0, P["dwav"])
+    rng = np.random.default_rng(1)
+    pi = 0.2 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    flags, counts = O.classify(phi)
+    rec, sums = O.strings(phi, pi, flags, 7, 8, P["dspa"])
+    assert rec.shape == (counts[0], 11) and counts[0] > 10
+    zz, yy, xx = np.nonzero(flags & 1)
+    centre = np.stack([xx, yy, zz], 1) + 0.5
+    assert np.abs(rec[:, :3] - centre).max() <= 0.8661
+    assert abs(sums[0] - rec[:, 7].sum()) < 1e-12 * max(1.0, sums[0])
+    v2 = (rec[:, 8:11] ** 2).sum(1)
+    ok = v2 < 1
+    assert sums[1] == ok.sum()
+    assert abs(sums[2] - np.sqrt(v2[ok]).sum()) < 1e-10 * max(1.0, sums[2])
+    rec2, _ = O.strings(phi, pi, flags, 7, 8, P["dspa"])
+    assert np.array_equal(rec, rec2)
+    rec3, _ = O.strings(phi, pi, flags, 7, 9, P["dspa"])   # another seed: other starting points
+    assert not np.array_equal(rec[:, 7], rec3[:, 7])
