@@ -95,6 +95,22 @@ struct GwLayout {
 };
 GwLayout gw_layout(int N, bool fp32);
 
+// half_pitch(N) as a compile-time constant for the N = 2^k kernels.
+template <int N> constexpr int half_pitch_c() { return ((N / 2 + 1) + 15) / 16 * 16; }
+
+// The tiled GwLayout with compile-time N and tile width ZT (powers of two), so index
+// arithmetic in the N = 2^k kernels reduces to shifts and masks.  Launchers check that the
+// handle's runtime layout is this one.
+template <int N, int ZT> struct TiledLayout {
+    static constexpr int NT = (N / 2 + 1 + ZT - 1) / ZT;
+    static constexpr int64_t COMP = (int64_t)N * NT * N * ZT;
+    __host__ __device__ static int64_t idx(int c, int kz, int ky, int kx)
+    {
+        return (((int64_t)c * N + ky) * NT + kx / ZT) * (int64_t)(N * ZT) + (int64_t)kz * ZT + (kx % ZT);
+    }
+    static bool matches(const GwLayout& L) { return L.N == N && L.zt == ZT && L.ntile == NT; }
+};
+
 struct GwStepScalars {
     double tau, dw, dwav;
     int fix_swap;

@@ -28,25 +28,81 @@ template <int N> struct P2 {
     }
 };
 
-template <typename T, int N, int S, int SIGN>
-__device__ __forceinline__ void p2_stage(typename V2<T>::type (&v)[8], int j0, const typename V2<T>::type* __restrict__ tw)
+// The twiddles a thread needs depend only on j0: stage S >= 1 uses M*(R-1) <= 7 of them.
+// Loading them all before the first stage (next to the data loads, or before a barrier)
+// takes their latency and address arithmetic out of the stages.
+template <typename T, int N> struct P2Tw {
+    static constexpr int NS = P2<N>::NST > 1 ? P2<N>::NST - 1 : 1;
+    typename V2<T>::type w[NS][7];
+};
+
+template <typename T, int N, int SIGN, int S = 1>
+__device__ __forceinline__ void p2_twiddles(P2Tw<T, N>& t, int j0, const typename V2<T>::type* __restrict__ tw)
+{
+    if constexpr (S < P2<N>::NST) {
+        using T2 = typename V2<T>::type;
+        constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
+        constexpr int TS = N / (Ns * R);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (j0 + m * TPT) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                T2 w = tw[r * k * TS];
+                if constexpr (SIGN < 0) w.y = -w.y;
+                t.w[S - 1][m * (R - 1) + r - 1] = w;
+            }
+        }
+        p2_twiddles<T, N, SIGN, S + 1>(t, j0, tw);
+    }
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ P2Tw<T, N> p2_load_twiddles(int j0, const typename V2<T>::type* __restrict__ tw)
+{
+    P2Tw<T, N> t;
+    p2_twiddles<T, N, SIGN>(t, j0, tw);
+    return t;
+}
+
+// Twiddles read from the table inside each stage instead (fewer registers held: for FP64
+// kernels whose register budget is already tight).
+template <typename T, int N, int SIGN> struct P2TwTable {
+    const typename V2<T>::type* __restrict__ tw;
+    int j0;
+};
+
+template <int S, typename T, int N>
+__device__ __forceinline__ typename V2<T>::type p2_tw(const P2Tw<T, N>& t, int m, int r)
+{
+    constexpr int R = P2<N>::radix(S);
+    return t.w[S - 1][m * (R - 1) + r - 1];
+}
+
+template <int S, typename T, int N, int SIGN>
+__device__ __forceinline__ typename V2<T>::type p2_tw(const P2TwTable<T, N, SIGN>& t, int m, int r)
+{
+    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT;
+    constexpr int TS = N / (Ns * R);
+    const int k = (t.j0 + m * TPT) & (Ns - 1);
+    typename V2<T>::type w = t.tw[r * k * TS];
+    if constexpr (SIGN < 0) w.y = -w.y;
+    return w;
+}
+
+template <typename T, int N, int S, int SIGN, typename TW>
+__device__ __forceinline__ void p2_stage(typename V2<T>::type (&v)[8], const TW& tws)
 {
     using T2 = typename V2<T>::type;
-    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
-    constexpr int TS = N / (Ns * R);
+    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), M = 8 / R;
 #pragma unroll
     for (int m = 0; m < M; m++) {
         T2 u[R];
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] = v[m + r * M];
         if constexpr (Ns > 1) {
-            const int k = (j0 + m * TPT) & (Ns - 1);
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                T2 w = tw[r * k * TS];
-                if constexpr (SIGN < 0) w.y = -w.y;
-                u[r] = cmul(u[r], w);
-            }
+            for (int r = 1; r < R; r++) u[r] = cmul(u[r], p2_tw<S>(tws, m, r));
         }
         dft_small<R, SIGN>(u);
 #pragma unroll
@@ -98,25 +154,27 @@ __device__ __forceinline__ void p2_exchange_if(bool active, typename V2<T>::type
     __syncthreads();
 }
 
-template <typename T, int N, int SIGN, int S = 0>
+// tws: p2_load_twiddles<T, N, SIGN>(j0, tw) of the same thread (registers), or
+// P2TwTable<T, N, SIGN>{tw, j0} (read in the stages).
+template <typename T, int N, int SIGN, int S = 0, typename TW>
 __device__ __forceinline__ void fft_p2_if(bool active, typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s,
-                                          int es, int off, const typename V2<T>::type* __restrict__ tw)
+                                          int es, int off, const TW& tws)
 {
     if constexpr (S < P2<N>::NST) {
-        if (active) p2_stage<T, N, S, SIGN>(v, j0, tw);
+        if (active) p2_stage<T, N, S, SIGN>(v, tws);
         if constexpr (S + 1 < P2<N>::NST) p2_exchange_if<T, N, S>(active, v, j0, s, es, off);
-        fft_p2_if<T, N, SIGN, S + 1>(active, v, j0, s, es, off, tw);
+        fft_p2_if<T, N, SIGN, S + 1>(active, v, j0, s, es, off, tws);
     }
 }
 
-template <typename T, int N, int SIGN, int S = 0>
+template <typename T, int N, int SIGN, int S = 0, typename TW>
 __device__ __forceinline__ void fft_p2(typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s, int es, int off,
-                                       const typename V2<T>::type* __restrict__ tw)
+                                       const TW& tws)
 {
     if constexpr (S < P2<N>::NST) {
-        p2_stage<T, N, S, SIGN>(v, j0, tw);
+        p2_stage<T, N, S, SIGN>(v, tws);
         if constexpr (S + 1 < P2<N>::NST) p2_exchange<T, N, S>(v, j0, s, es, off);
-        fft_p2<T, N, SIGN, S + 1>(v, j0, s, es, off, tw);
+        fft_p2<T, N, SIGN, S + 1>(v, j0, s, es, off, tws);
     }
 }
 

@@ -50,7 +50,8 @@ __global__ void __launch_bounds__(p2_xrows<N>() * (N / 8)) gw_xpass_p2_kernel(co
     T2 v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) v[e] = in[j0 + e * TPT];
-    fft_p2<T, N, +1>(v, j0, s, 1, rr * N, tw);
+    const P2Tw<T, N> tws = p2_load_twiddles<T, N, +1>(j0, tw);
+    fft_p2<T, N, +1>(v, j0, s, 1, rr * N, tws);
     T2* row = s + rr * N;
 #pragma unroll
     for (int e = 0; e < 8; e++) row[j0 + e * TPT] = v[e];
@@ -85,37 +86,11 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
     T2 v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * es] : T2{0, 0};
-    fft_p2<T, N, SIGN>(v, j0, s, TX, b, tw);
+    const P2Tw<T, N> tws = p2_load_twiddles<T, N, SIGN>(j0, tw);
+    fft_p2<T, N, SIGN>(v, j0, s, TX, b, tws);
     if (ok) {
 #pragma unroll
         for (int e = 0; e < 8; e++) base[(int64_t)(j0 + e * TPT) * es] = v[e];
-    }
-}
-
-// y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled
-// layout of the accumulation pass.  Block = TX kx columns of one (c, z) plane.
-template <typename T, int N>
-__global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
-                                                                             typename V2<T>::type* __restrict__ W2,
-                                                                             int P, GwLayout L,
-                                                                             const typename V2<T>::type* __restrict__ tw)
-{
-    using T2 = typename V2<T>::type;
-    constexpr int TX = p2_tx<T, N>(), TPT = N / 8, NH = N / 2 + 1;
-    extern __shared__ unsigned char smem_raw[];
-    T2* s = reinterpret_cast<T2*>(smem_raw);
-    const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
-    const int kx = blockIdx.x * TX + b;
-    const int c = blockIdx.y / N, z = blockIdx.y % N;
-    const bool ok = kx < NH;
-    const T2* base = W + ((int64_t)c * N + z) * (int64_t)N * P + kx;
-    T2 v[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
-    fft_p2<T, N, +1>(v, j0, s, TX, b, tw);
-    if (ok) {
-#pragma unroll
-        for (int e = 0; e < 8; e++) W2[L.idx(c, z, j0 + e * TPT, kx)] = v[e];
     }
 }
 
@@ -128,13 +103,43 @@ template <typename T, int N> constexpr int p2_ztx()
     return tx;
 }
 
+// y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled
+// layout of the accumulation pass.  Block = TX kx columns of one (c, z) plane.
+template <typename T, int N>
+__global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
+                                                                             typename V2<T>::type* __restrict__ W2,
+                                                                             const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int TX = p2_tx<T, N>(), TPT = N / 8, NH = N / 2 + 1, P = half_pitch_c<N>();
+    using L = TiledLayout<N, p2_ztx<T, N>()>;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
+    const int kx = blockIdx.x * TX + b;
+    const int c = blockIdx.y / N, z = blockIdx.y % N;
+    const bool ok = kx < NH;
+    const T2* base = W + ((int64_t)c * N + z) * (int64_t)N * P + kx;
+    T2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
+    // twiddles read inside the stages (measured: no faster with them held in registers)
+    fft_p2<T, N, +1>(v, j0, s, TX, b, P2TwTable<T, N, +1>{tw, j0});
+    if (ok) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) W2[L::idx(c, z, j0 + e * TPT, kx)] = v[e];
+    }
+}
+
+
 template <typename T, int N>
 __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
-                                                                           typename V2<T>::type* __restrict__ A, GwLayout L,
+                                                                           typename V2<T>::type* __restrict__ A,
                                                                            const typename V2<T>::type* __restrict__ tw,
                                                                            GwStepScalars g)
 {
 #pragma clang fp contract(off)
+    using L = TiledLayout<N, p2_ztx<T, N>()>;
     using T2 = typename V2<T>::type;
     constexpr int TX = p2_ztx<T, N>(), TPT = N / 8, NH = N / 2 + 1;
     extern __shared__ unsigned char smem_raw[];
@@ -144,8 +149,8 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
     const int kx = blockIdx.x * TX + b, ky = blockIdx.y;
     const bool ok = kx < NH;
     const int nx = wrapn(kx, N), ny = wrapn(ky, N);
-    const int64_t tile0 = L.idx(0, 0, ky, blockIdx.x * TX) + b;
-    const int64_t cst = L.comp();
+    const int64_t tile0 = L::idx(0, 0, ky, blockIdx.x * TX) + b;
+    constexpr int64_t cst = L::COMP;
     constexpr int zs = TX;
 
     // polarization geometry of grav_waves.f90:165-183, p1 = (qz c, -s), p2 = (qz s, c),
@@ -158,49 +163,78 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
         if (rho > 0) { gc = (T)nx / rho; gs = (T)ny / rho; }
         else { gc = 1; gs = 0; }
     }
-    T gz[8], gr[8];
+    T2 sp[8], sn[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int nz = wrapn(j0 + e * TPT, N);
-        const T q = sqrt((T)nx * (T)nx + (T)ny * (T)ny + (T)nz * (T)nz);
-        if (q > 0) {
-            const T qx = (T)nx / q, qy = (T)ny / q;
-            gz[e] = (T)nz / q;
-            gr[e] = sqrt(qx * qx + qy * qy);
-        } else {
-            gz[e] = gr[e] = 0;  // k = 0: pol = 0 (flagged by gr = gz = 0 below)
+    for (int e = 0; e < 8; e++) sp[e] = sn[e] = T2{0, 0};
+    // per wavevector: qz = nz/|n| and r = |n_xy|/|n| (one reciprocal; k = 0 is zeroed after the
+    // component loop)
+    T gz[8], gr[8];
+    {
+        const T rho2 = (T)nx * (T)nx + (T)ny * (T)ny, rho = sqrt(rho2);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int nz = wrapn(j0 + e * TPT, N);
+            const T q = sqrt(rho2 + (T)nz * (T)nz);
+            const T iq = q > 0 ? (T)1 / q : (T)0;
+            gz[e] = (T)nz * iq;
+            gr[e] = rho * iq;
         }
     }
 
-    T2 sp[8], sn[8], v[8];
+    // FP32: twiddles in registers and component c+1 loaded while component c is transformed
+    // (software pipelining); FP64 has no registers to spare for either.
+    constexpr bool PIPE = sizeof(T) == 4, PTW = sizeof(T) == 4;
+    T2 v[8], nxt[8];
+    if constexpr (PIPE) {
 #pragma unroll
-    for (int e = 0; e < 8; e++) sp[e] = sn[e] = T2{0, 0};
+        for (int e = 0; e < 8; e++) nxt[e] = ok ? W[tile0 + (int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+    }
+    P2Tw<T, N> twr;
+    if constexpr (PTW) twr = p2_load_twiddles<T, N, +1>(j0, tw);
+    const P2TwTable<T, N, +1> twt{tw, j0};
 #pragma unroll 1
     for (int c = 0; c < 6; c++) {
-        const T2* base = W + c * cst + tile0;
+        if constexpr (PIPE) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+            for (int e = 0; e < 8; e++) v[e] = nxt[e];
+            if (c + 1 < 6) {
+                const T2* base = W + (c + 1) * cst + tile0;
+#pragma unroll
+                for (int e = 0; e < 8; e++) nxt[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+            }
+        } else {
+            const T2* base = W + c * cst + tile0;
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+        }
         if (c > 0) __syncthreads();
-        fft_p2<T, N, +1>(v, j0, s, TX, b, tw);
-        const int ii = c == 0 ? 0 : (c <= 2 ? 1 : 2);
-        const int jj = (c == 0 || c == 1 || c == 3) ? 0 : ((c == 2 || c == 4) ? 1 : 2);
+        if constexpr (PTW) fft_p2<T, N, +1>(v, j0, s, TX, b, twr);
+        else fft_p2<T, N, +1>(v, j0, s, TX, b, twt);
+        {
+            // pol_ij d into sp (pol d) and sn (conj(pol) d), (i,j) in the reference's order
+            // (1,1),(2,1),(2,2),(3,1),(3,2),(3,3) (grav_waves.f90:213-247)
+            const int ii = c == 0 ? 0 : (c <= 2 ? 1 : 2);
+            const int jj = (c == 0 || c == 1 || c == 3) ? 0 : ((c == 2 || c == 4) ? 1 : 2);
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const bool kz0 = (gz[e] == 0 && gr[e] == 0);
-            T2 p1 = {gz[e] * gc, -gs}, p2 = {gz[e] * gs, gc}, p3 = {-gr[e], 0};
-            T2 a = ii == 0 ? p1 : (ii == 1 ? p2 : p3);
-            T2 bb = jj == 0 ? p1 : (jj == 1 ? p2 : p3);
-            T2 pol = {a.x * bb.x - a.y * bb.y, a.x * bb.y + a.y * bb.x};
-            if (ii == jj) { pol.x = pol.x / 2; pol.y = pol.y / 2; }
-            if (kz0) pol = T2{0, 0};
-            T2 d = v[e];
-            sp[e].x += pol.x * d.x - pol.y * d.y;
-            sp[e].y += pol.x * d.y + pol.y * d.x;
-            sn[e].x += pol.x * d.x + pol.y * d.y;
-            sn[e].y += pol.x * d.y - pol.y * d.x;
+            for (int e = 0; e < 8; e++) {
+                const T2 p1 = {gz[e] * gc, -gs}, p2 = {gz[e] * gs, gc}, p3 = {-gr[e], 0};
+                const T2 a = ii == 0 ? p1 : (ii == 1 ? p2 : p3);
+                const T2 bb = jj == 0 ? p1 : (jj == 1 ? p2 : p3);
+                T2 pol = {a.x * bb.x - a.y * bb.y, a.x * bb.y + a.y * bb.x};
+                if (ii == jj) {
+                    pol.x = pol.x / 2;
+                    pol.y = pol.y / 2;
+                }
+                const T2 d = v[e];
+                sp[e].x += pol.x * d.x - pol.y * d.y;
+                sp[e].y += pol.x * d.y + pol.y * d.x;
+                sn[e].x += pol.x * d.x + pol.y * d.y;
+                sn[e].y += pol.x * d.y - pol.y * d.x;
+            }
         }
     }
     if (!ok) return;
+    if (kx == 0 && ky == 0 && j0 == 0) sp[0] = sn[0] = T2{0, 0};  // k = 0: pol = 0
     const int64_t ms = cst;
     const T dw = (T)g.dw;
     const int64_t rho2 = (int64_t)nx * nx + (int64_t)ny * ny;
@@ -547,8 +581,10 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
     if (L.zt) {
         bool done = p2_dispatch(N, [&]<int NN>() {
             constexpr int TXP = p2_tx<T, NN>();
+            if (!TiledLayout<NN, p2_ztx<T, NN>()>::matches(L) || pitch != half_pitch_c<NN>())
+                throw Error(AXG_EINVAL, "y pass: layout does not match the compiled tile");
             hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN>), dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN),
-                               dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, pitch, L,
+                               dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2,
                                (const T2*)tw);
         });
         if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k");
@@ -565,9 +601,10 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
     using T2 = typename V2<T>::type;
     bool done = L.zt && p2_dispatch(N, [&]<int NN>() {
         constexpr int TXP = p2_ztx<T, NN>();
-        if (L.zt != TXP) throw Error(AXG_EINVAL, "accumulation layout does not match the z pass tile");
+        if (!TiledLayout<NN, TXP>::matches(L))
+            throw Error(AXG_EINVAL, "accumulation layout does not match the z pass tile");
         hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3(L.ntile, NN), dim3(TXP * (NN / 8)),
-                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, L, (const T2*)tw, g);
+                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, (const T2*)tw, g);
     });
     if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);

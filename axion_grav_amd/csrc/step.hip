@@ -131,18 +131,24 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
 //
 // The same kick + source + drift, with the x transform of the three packed source planes
 // done in the same block (fft_p2.h): block = XR rows y at one z with one thread per site
-// (XR*N = 1024 threads), so the stencil phase has the memory parallelism of the plain
-// kernel.  Each thread leaves its packed sources C_m = S_a + i S_b in LDS (3 x XR x N
+// (XR*N = 512 threads, or N when N > 512), so the stencil phase has the memory parallelism of
+// the plain kernel.  Each thread leaves its packed sources C_m = S_a + i S_b in LDS (3 x XR x N
 // complex); then 3*XR*N/8 of the threads transform the 3*XR rows in place and every thread
 // takes part in splitting them into the two real sources' half spectra
 //   S_a^(k) = (X(k) + conj(X(N-k)))/2,  S_b^(k) = (X(k) - conj(X(N-k)))/(2i),  k = 0..N/2,
 // written to W[2m] and W[2m+1] (the y and z passes follow).  The six source reals per site
 // never reach HBM.
 
-template <int N> constexpr int fused_rows() { return N >= 1024 ? 1 : (1024 / N < N ? 1024 / N : N); }
+// rows per block: blocks of 512 threads (measured at 512^3: one row per block beats two by
+// 5-7 % in both precisions), one row when N >= 512
+template <int N> constexpr int fused_rows() { return N >= 512 ? 1 : (512 / N < N ? 512 / N : N); }
+template <int N> constexpr int fused_rows_wide() { return N >= 1024 ? 1 : (1024 / N < N ? 1024 / N : N); }
 
-template <typename T, bool COMP, int N>
-__global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
+// XCD-aware block order (REMAP): the dispatcher deals consecutive blocks to the 8 XCDs in
+// turn; remapped, XCD k walks a contiguous run of planes, so the blocks it holds at once are
+// neighbouring rows of one plane and share their halo rows in its L2.
+template <typename T, bool COMP, int N, int XR, bool REMAP>
+__global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, int P, const typename V2<T>::type* __restrict__ tw,
@@ -150,15 +156,22 @@ __global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
 {
 #pragma clang fp contract(off)
     using T2 = typename V2<T>::type;
-    constexpr int TPT = N / 8, XR = fused_rows<N>(), NR = 3 * XR;  // NR rows to transform
-    constexpr int BS = XR * N;                                       // threads per block
+    constexpr int TPT = N / 8, NR = 3 * XR;  // NR rows to transform
+    constexpr int BS = XR * N;                 // threads per block
+    constexpr int NG = N / XR;                 // row groups per plane
     extern __shared__ unsigned char smem_raw[];
     T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
     const int64_t NN = (int64_t)N * N;
-    const int z = blockIdx.y;
+    int yg = blockIdx.x, z = blockIdx.y;
+    if constexpr (REMAP) {
+        const int lin = blockIdx.y * NG + blockIdx.x, nb = NG * N;
+        const int logical = (lin & 7) * (nb >> 3) + (lin >> 3);
+        yg = logical % NG;
+        z = logical / NG;
+    }
     {
         const int rr = threadIdx.x / N, x = threadIdx.x % N;
-        const int y = blockIdx.x * XR + rr;
+        const int y = yg * XR + rr;
         const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
         const int zm = (z - 1) & (N - 1), zp = (z + 1) & (N - 1);
         const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
@@ -197,11 +210,18 @@ __global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
         s[(XR + rr) * N + x] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
         s[(2 * XR + rr) * N + x] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
     }
+    // FP32: twiddles of the row transforms requested before the barrier, so their latency
+    // overlaps it; FP64 reads them inside the stages (fewer registers held)
+    const int q = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+    const bool active = q < NR;
+    P2Tw<T, N> twr;
+    if constexpr (sizeof(T) == 4) {
+        if (active) twr = p2_load_twiddles<T, N, +1>(j0, tw);
+    }
+    const P2TwTable<T, N, +1> twt{tw, j0};
     __syncthreads();
     // transform the NR rows in place: row q = threadIdx / TPT
     {
-        const int q = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
-        const bool active = q < NR;
         T2* rowp = s + (active ? q : 0) * N;
         T2 v[8];
         if (active) {
@@ -209,7 +229,8 @@ __global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
             for (int e = 0; e < 8; e++) v[e] = rowp[j0 + e * TPT];
         }
         __syncthreads();
-        fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, tw);
+        if constexpr (sizeof(T) == 4) fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, twr);
+        else fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, twt);
         if (active) {
 #pragma unroll
             for (int e = 0; e < 8; e++) rowp[j0 + e * TPT] = v[e];
@@ -217,16 +238,16 @@ __global__ void __launch_bounds__(1024) kick_source_xfft_kernel(
         __syncthreads();
     }
     // split every transformed row into two half spectra: NR rows x (N/2+1) outputs
-    constexpr int NH = N / 2 + 1;
-    const int64_t cs = NN * P;  // component stride of W
+    constexpr int NH = N / 2 + 1, PC = half_pitch_c<N>();
+    constexpr int64_t cs = (int64_t)N * N * PC;  // component stride of W
     for (int t = threadIdx.x; t < NR * NH; t += BS) {
         const int q = t / NH, kk = t - q * NH;
         const int m = q / XR, rr = q - m * XR;
-        const int y = blockIdx.x * XR + rr;
+        const int y = yg * XR + rr;
         const T2* rowp = s + q * N;
         T2 xk = rowp[kk];
         T2 xm = rowp[(N - kk) & (N - 1)];
-        T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * P;
+        T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
         wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
         wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
     }
@@ -238,12 +259,30 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                              double dspa, double dt_drift)
 {
     using T2 = typename V2<T>::type;
-    auto go = [&]<int NN>() {
-        constexpr int XR = fused_rows<NN>();
-        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN>), dim3(NN / XR, NN), dim3(XR * NN),
+    if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
+    // tuning knob AXG_KICK_VARIANT: bit 0 = 1024-thread blocks (1024/N rows), bit 1 = XCD
+    // remap (same results; the block shape and order only)
+    static const int variant = [] {
+        const char* e = getenv("AXG_KICK_VARIANT");
+        return e ? atoi(e) : 0;
+    }();
+    auto launch = [&]<int NN, int XR, bool RM>() {
+        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
                            (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
                            1.0 / (dspa * 2), dt_drift);
+    };
+    auto go = [&]<int NN>() {
+        constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
+        if constexpr (NN >= 64) {
+            switch (variant & 3) {
+            case 1: launch.template operator()<NN, XW, false>(); return;
+            case 2: launch.template operator()<NN, XR, true>(); return;
+            case 3: launch.template operator()<NN, XW, true>(); return;
+            default: break;
+            }
+        }
+        launch.template operator()<NN, XR, false>();
     };
     switch (N) {
     case 16: go.template operator()<16>(); break;

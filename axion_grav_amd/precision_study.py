@@ -7,10 +7,12 @@
 Both precisions start from the same seeded flat-spectrum field (|k| < 1, <|phi|^2> = 1,
 pi = 0, lambda = 2 at zero temperature, radiation era from tau = 1) and advance in lockstep
 with GW accumulation on.  Every `--every` steps each lattice reports its 128-bin GW spectrum,
-its energy (FP64 reductions in both precisions) and its string / wall / mask cell counts, and
+its energy (FP64 reductions in both precisions), its string / wall / mask cell counts and its
+string length and velocities, and
 one JSON line gives the FP32C error against FP64: the per-bin relative error of the GW power
-(max and median over the bins that hold modes), the relative error of the energy terms and
-the difference of the cell counts.
+(max and median over the bins that hold modes), the relative error of the energy terms, the
+cell counts, and the relative error of the summed string length and mean string velocities
+(fits seeded alike in both precisions).
 
 The step is dt = 0.01 by default: with lambda = 2 the radiation-era potential term grows as
 tau^2, and the leapfrog passes its stability bound near tau = 23 (at dt = 0.05 the FP64 oracle
@@ -34,6 +36,10 @@ def lattice(N, box, precision, bins):
     return capi.Lattice(P)
 
 
+def relerr(a, b):
+    return float(abs(a - b) / abs(b)) if b != 0 else float(abs(a))
+
+
 def compare(t, step, d64, d32):
     s64, s32 = d64["spec"], d32["spec"]
     live = s64["denom"] > 0
@@ -44,12 +50,16 @@ def compare(t, step, d64, d32):
                 gw_relerr_max=float(rel.max()) if rel.size else None,
                 gw_relerr_median=float(np.median(rel)) if rel.size else None,
                 gw_denom_equal=bool(np.array_equal(s32["denom"], s64["denom"])),
-                energy_relerr=[float(abs(a - b) / abs(b)) if b != 0 else 0.0 for a, b in zip(e32, e64)],
-                cells_fp64=[int(x) for x in d64["cells"]], cells_fp32c=[int(x) for x in d32["cells"]])
+                energy_relerr=[relerr(a, b) for a, b in zip(e32, e64)],
+                cells_fp64=[int(x) for x in d64["cells"]], cells_fp32c=[int(x) for x in d32["cells"]],
+                string_length_fp64=d64["str"]["length"],
+                string_length_relerr=relerr(d32["str"]["length"], d64["str"]["length"]),
+                string_vel_relerr=[relerr(a, b) for a, b in zip(d32["str"]["vel"], d64["str"]["vel"])])
 
 
-def diag(L, t):
-    return dict(spec=L.gw_spectrum(), energy=L.energy(t), cells=L.defects())
+def diag(L, t, step):
+    cells = L.defects()
+    return dict(spec=L.gw_spectrum(), energy=L.energy(t), cells=cells, str=L.strings(7, step))
 
 
 def main():
@@ -76,7 +86,8 @@ def main():
         L64.advance_steps(t, a.dt, a.every)
         L32.advance_steps(t, a.dt, a.every)
         t = 1.0 + (blk + 1) * a.every * a.dt
-        r = compare(t, (blk + 1) * a.every, diag(L64, t), diag(L32, t))
+        step = (blk + 1) * a.every
+        r = compare(t, step, diag(L64, t, step), diag(L32, t, step))
         r["wall_s"] = round(time.perf_counter() - t0, 1)
         if r["gw_relerr_max"] is not None:
             worst = max(worst, r["gw_relerr_max"])

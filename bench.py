@@ -76,10 +76,32 @@ def log(*a):
 def params(capi, N, L, precision, nbins):
     dspa = L / N
     return capi.make_params(N, dspa, SIGMA_CR, precision=precision, grav_spec=True, initial_loop=True,
-                            kappa=1e-30, num_bin_grav=nbins, spec_est=False)
+                            kappa=1e-30, num_bin_grav=nbins, id_strings=True, id_walls=True, spec_est=True)
 
 
-def run_gpu(capi, N, L, precision, steps, warmup, seed):
+def time_ms(fn, reps=3):
+    """Mean wall time of fn() (each call synchronizes before returning), after one untimed call."""
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    return (time.perf_counter() - t0) / reps * 1e3, out
+
+
+def run_diagnostics(lat, t):
+    """Wall time of each diagnostic on the evolved field (outside the timed step region)."""
+    d = {}
+    d["gw_spectrum_ms"], _ = time_ms(lat.gw_spectrum)
+    d["defects_ms"], cells = time_ms(lat.defects)
+    d["strings_ms"], st = time_ms(lambda: lat.strings(1, 2), reps=1)
+    d["field_spectrum_ms"], _ = time_ms(lambda: lat.field_spectrum(t))
+    d["energy_ms"], _ = time_ms(lambda: lat.energy(t))
+    d["n_string_cells"] = int(cells[0])
+    d["gw_spectra_per_s"] = 1e3 / d["gw_spectrum_ms"]
+    return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False):
     P = params(capi, N, L, precision, 128)
     real = 8 if precision == capi.FP64 else 4
     dt = 0.2 * (L / N)
@@ -95,6 +117,8 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed):
         el = time.perf_counter() - t0
         prof = lat.profile(False)
         spec = lat.gw_spectrum()
+        t = t + steps * dt
+        dg = run_diagnostics(lat, t) if diag else None
     n3 = N ** 3
     kern = {}
     for k, (ms, cnt) in prof.items():
@@ -104,7 +128,7 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed):
         alg = KERNEL_REALS[k] * real * n3
         kern[k] = dict(ms_avg=avg, launches=cnt, alg_bytes=alg, gbs=alg / (avg * 1e-3) / 1e9)
     return dict(su_s=steps * n3 / el, ms_per_step=el / steps * 1e3, kernels=kern, real=real,
-                finite=bool(np.all(np.isfinite(spec["numer"]))))
+                finite=bool(np.all(np.isfinite(spec["numer"]))), diagnostics=dg)
 
 
 def copy_probe(nbytes=4 << 30, reps=10):
@@ -152,6 +176,7 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-diag", action="store_true", help="skip timing the diagnostics after the FP32C run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
                          "its kernel is the known-size reference for the FETCH/WRITE counters)")
@@ -166,7 +191,9 @@ def main():
     copy_gbs = copy_probe() if args.copy_probe else None
     N, L = args.n, args.box
     log(f"FP32C {N}^3 L={L}: warmup {args.warmup}, timed {args.steps} steps")
-    c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed)
+    c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed, diag=not args.no_diag)
+    if c["diagnostics"]:
+        log("FP32C diagnostics: " + ", ".join(f"{k} {v}" for k, v in c["diagnostics"].items()))
     log(f"FP32C {c['su_s']:.4g} site-updates/s, {c['ms_per_step']:.3f} ms/step; kernels "
         + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in c["kernels"].items()))
     d = None
@@ -216,6 +243,7 @@ def main():
         "fp32c_over_fp64": None if d is None else round(c["su_s"] / d["su_s"], 3),
         "kernels": {k: {"ms": round(v["ms_avg"], 4), "GB/s": round(v["gbs"], 1)} for k, v in c["kernels"].items()},
         "hbm_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
+        "diagnostics": c["diagnostics"],
     }
     print(json.dumps(out), flush=True)
 

@@ -119,7 +119,8 @@ def test_time_reversal_residual(prec, tol):
 
 # ---------------------------------------------------------------- GW accumulation
 
-@pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False)])
+@pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False),
+                                           (64, 2, False)])
 def test_gw_accumulators_fp64(N, nsteps, fix):
     say("gw", N, nsteps, fix)
     prm, P = params(N, gw_fix_swap=fix)
