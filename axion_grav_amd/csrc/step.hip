@@ -144,6 +144,39 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
 template <int N> constexpr int fused_rows() { return N >= 512 ? 1 : (512 / N < N ? 512 / N : N); }
 template <int N> constexpr int fused_rows_wide() { return N >= 1024 ? 1 : (1024 / N < N ? 1024 / N : N); }
 
+// One site of the fused pass: kick of p (evolve.f90:325-330 with potential.f90:43-60, in the
+// reference's order), the next drift f = c + dt_drift p, and the gradient (grav_waves.f90:
+// 331-351, times 1/(2 dspa): exactly the division when dspa is a power of two) with the packed
+// source products of grav_waves.f90:213-223, C0 = S11 + i S21, C1 = S22 + i S31,
+// C2 = S32 + i S33.
+template <typename T>
+__device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, double2 b1, double2 b2, double2 c1v,
+                                          double2 c2v, double2& p, double2& f, typename V2<T>::type (&src)[3],
+                                          const KickScalars& k, double inv_d, double dt_drift)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    double lr = a1.x + a2.x + b1.x + b2.x + c1v.x + c2v.x - 6 * c.x;
+    double li = a1.y + a2.y + b1.y + b2.y + c1v.y + c2v.y - 6 * c.y;
+    double s2 = c.x * c.x + c.y * c.y;
+    double g = 2 * s2 + k.m2;
+    double dr = g * c.x - k.ma2 / 2;
+    double di = g * c.y;
+    if (k.kt != 0.0) {
+        dr = dr - k.kt * c.x;
+        di = di + k.kt * c.y;
+    }
+    p.x = k.f1 * p.x + k.f2 * lr + k.f3 * dr;
+    p.y = k.f1 * p.y + k.f2 * li + k.f3 * di;
+    f = double2{c.x + dt_drift * p.x, c.y + dt_drift * p.y};
+    double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+    double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+    double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
+    src[0] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
+    src[1] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
+    src[2] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
+}
+
 // XCD-aware block order (REMAP): the dispatcher deals consecutive blocks to the 8 XCDs in
 // turn; remapped, XCD k walks a contiguous run of planes, so the blocks it holds at once are
 // neighbouring rows of one plane and share their halo rows in its L2.
@@ -185,30 +218,14 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
         double2 c1v = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x);
         double2 c2v = ldf<T, COMP>(phi_in, cphi_in, (int64_t)zp * NN + (int64_t)y * N + x);
         double2 p = ldf<T, COMP>(pi, cpi, i);
-        // evolve.f90:325-330 with potential.f90:43-60, in the reference's order
-        double lr = a1.x + a2.x + b1.x + b2.x + c1v.x + c2v.x - 6 * c.x;
-        double li = a1.y + a2.y + b1.y + b2.y + c1v.y + c2v.y - 6 * c.y;
-        double s2 = c.x * c.x + c.y * c.y;
-        double g = 2 * s2 + k.m2;
-        double dr = g * c.x - k.ma2 / 2;
-        double di = g * c.y;
-        if (k.kt != 0.0) {
-            dr = dr - k.kt * c.x;
-            di = di + k.kt * c.y;
-        }
-        p.x = k.f1 * p.x + k.f2 * lr + k.f3 * dr;
-        p.y = k.f1 * p.y + k.f2 * li + k.f3 * di;
+        double2 f;
+        T2 src[3];
+        kick_site<T>(c, a1, a2, b1, b2, c1v, c2v, p, f, src, k, inv_d, dt_drift);
         stf<T, COMP>(pi, cpi, i, p);
-        double2 f = {c.x + dt_drift * p.x, c.y + dt_drift * p.y};
         stf<T, COMP>(phi_out, cphi_out, i, f);
-        // grav_waves.f90:331-351 gradient (times 1/(2 dspa): exactly the division when dspa is
-        // a power of two) and 213-223 source products
-        double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
-        double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
-        double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
-        s[rr * N + x] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
-        s[(XR + rr) * N + x] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
-        s[(2 * XR + rr) * N + x] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
+        s[rr * N + x] = src[0];
+        s[(XR + rr) * N + x] = src[1];
+        s[(2 * XR + rr) * N + x] = src[2];
     }
     // FP32: twiddles of the row transforms requested before the barrier, so their latency
     // overlaps it; FP64 reads them inside the stages (fewer registers held)
@@ -253,6 +270,138 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     }
 }
 
+// Two neighbouring sites x0 = 2t, 2t+1 per thread: phi and pi pairs move as one 16-byte
+// (FP32) or 32-byte (FP64) access and one 8-byte residual access, the outer x neighbours as
+// single values, and the block of XR*N/2 threads keeps 3/4 of its threads busy in the row
+// transforms.  Same arithmetic per site (kick_site) as the one-site kernel.
+template <typename T> struct alignas(2 * sizeof(typename V2<T>::type)) CPair {
+    typename V2<T>::type a, b;
+};
+struct alignas(8) RPair {
+    short2 a, b;
+};
+
+template <typename T, bool COMP>
+__device__ __forceinline__ void ld_pair(const typename V2<T>::type* __restrict__ f, const short2* __restrict__ cf,
+                                        int64_t i, double2& u, double2& w)
+{
+    const CPair<T> v = reinterpret_cast<const CPair<T>*>(f + i)[0];
+    if constexpr (sizeof(T) == 8) {
+        u = v.a;
+        w = v.b;
+    } else if constexpr (COMP) {
+        const RPair r = reinterpret_cast<const RPair*>(cf + i)[0];
+        u = double2{comp_get(v.a.x, r.a.x), comp_get(v.a.y, r.a.y)};
+        w = double2{comp_get(v.b.x, r.b.x), comp_get(v.b.y, r.b.y)};
+    } else {
+        u = double2{(double)v.a.x, (double)v.a.y};
+        w = double2{(double)v.b.x, (double)v.b.y};
+    }
+}
+
+template <typename T, bool COMP>
+__device__ __forceinline__ void st_pair(typename V2<T>::type* __restrict__ f, short2* __restrict__ cf, int64_t i,
+                                        double2 u, double2 w)
+{
+    using T2 = typename V2<T>::type;
+    CPair<T> v;
+    if constexpr (sizeof(T) == 8) {
+        v.a = u;
+        v.b = w;
+    } else if constexpr (COMP) {
+        RPair r;
+        v.a = T2{comp_put(u.x, r.a.x), comp_put(u.y, r.a.y)};
+        v.b = T2{comp_put(w.x, r.b.x), comp_put(w.y, r.b.y)};
+        reinterpret_cast<RPair*>(cf + i)[0] = r;
+    } else {
+        v.a = T2{(float)u.x, (float)u.y};
+        v.b = T2{(float)w.x, (float)w.y};
+    }
+    reinterpret_cast<CPair<T>*>(f + i)[0] = v;
+}
+
+template <typename T, bool COMP, int N, int XR>
+__global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
+    const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
+    typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
+    short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, const typename V2<T>::type* __restrict__ tw,
+    KickScalars k, double inv_d, double dt_drift)
+{
+#pragma clang fp contract(off)
+    using T2 = typename V2<T>::type;
+    constexpr int TPT = N / 8, NR = 3 * XR, BS = XR * N / 2, H = N / 2;
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
+    const int64_t NN = (int64_t)N * N;
+    const int yg = blockIdx.x, z = blockIdx.y;
+    {
+        const int rr = threadIdx.x / H, x0 = 2 * (threadIdx.x % H);
+        const int y = yg * XR + rr;
+        const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
+        const int zm = (z - 1) & (N - 1), zp = (z + 1) & (N - 1);
+        const int64_t row = (int64_t)z * NN + (int64_t)y * N;
+        double2 c0, c1, l, r, b10, b11, b20, b21, z10, z11, z20, z21, p0, p1;
+        ld_pair<T, COMP>(phi_in, cphi_in, row + x0, c0, c1);
+        l = ldf<T, COMP>(phi_in, cphi_in, row + ((x0 - 1) & (N - 1)));
+        r = ldf<T, COMP>(phi_in, cphi_in, row + ((x0 + 2) & (N - 1)));
+        ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)ym * N + x0, b10, b11);
+        ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)yp * N + x0, b20, b21);
+        ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x0, z10, z11);
+        ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)zp * NN + (int64_t)y * N + x0, z20, z21);
+        ld_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
+        double2 f0, f1;
+        T2 sa[3], sb[3];
+        kick_site<T>(c0, l, c1, b10, b20, z10, z20, p0, f0, sa, k, inv_d, dt_drift);
+        kick_site<T>(c1, c0, r, b11, b21, z11, z21, p1, f1, sb, k, inv_d, dt_drift);
+        st_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
+        st_pair<T, COMP>(phi_out, cphi_out, row + x0, f0, f1);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            CPair<T> v;
+            v.a = sa[m];
+            v.b = sb[m];
+            reinterpret_cast<CPair<T>*>(s + (m * XR + rr) * N + x0)[0] = v;
+        }
+    }
+    const int q = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+    const bool active = q < NR;
+    P2Tw<T, N> twr;
+    if constexpr (sizeof(T) == 4) {
+        if (active) twr = p2_load_twiddles<T, N, +1>(j0, tw);
+    }
+    const P2TwTable<T, N, +1> twt{tw, j0};
+    __syncthreads();
+    {
+        T2* rowp = s + (active ? q : 0) * N;
+        T2 v[8];
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = rowp[j0 + e * TPT];
+        }
+        __syncthreads();
+        if constexpr (sizeof(T) == 4) fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, twr);
+        else fft_p2_if<T, N, +1>(active, v, j0, rowp, 1, 0, twt);
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) rowp[j0 + e * TPT] = v[e];
+        }
+        __syncthreads();
+    }
+    constexpr int NH = N / 2 + 1, PC = half_pitch_c<N>();
+    constexpr int64_t cs = (int64_t)N * N * PC;
+    for (int t = threadIdx.x; t < NR * NH; t += BS) {
+        const int qq = t / NH, kk = t - qq * NH;
+        const int m = qq / XR, rr = qq - m * XR;
+        const int y = yg * XR + rr;
+        const T2* rowp = s + qq * N;
+        T2 xk = rowp[kk];
+        T2 xm = rowp[(N - kk) & (N - 1)];
+        T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
+        wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+        wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+    }
+}
+
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
@@ -261,7 +410,8 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
     // tuning knob AXG_KICK_VARIANT: bit 0 = 1024-thread blocks (1024/N rows), bit 1 = XCD
-    // remap (same results; the block shape and order only)
+    // remap, bit 2 = swap the sites-per-thread choice (same results; the block shape and
+    // order only)
     static const int variant = [] {
         const char* e = getenv("AXG_KICK_VARIANT");
         return e ? atoi(e) : 0;
@@ -275,6 +425,17 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     auto go = [&]<int NN>() {
         constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
         if constexpr (NN >= 64) {
+            // two sites per thread (blocks of XR*N/2 threads) for FP32 storage; one for FP64
+            // (measured at 512^3: FP32C 2.95 vs 3.39 ms, FP64 4.97 vs 3.64 ms)
+            const bool two = (sizeof(T) == 4) != ((variant & 4) != 0);
+            if (two) {
+                constexpr int X2 = XR;
+                hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2>), dim3(NN / X2, NN), dim3(X2 * NN / 2),
+                                   (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
+                                   (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, (const T2*)tw, k,
+                                   1.0 / (dspa * 2), dt_drift);
+                return;
+            }
             switch (variant & 3) {
             case 1: launch.template operator()<NN, XW, false>(); return;
             case 2: launch.template operator()<NN, XR, true>(); return;
