@@ -62,7 +62,7 @@ class StringOut(C.Structure):
 
 
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
-           "axg_get_fields", "axg_init_flat", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
+           "axg_get_fields", "axg_init_flat", "axg_init_thermal", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
            "axg_strings", "axg_get_strings", "axg_field_spectrum", "axg_energy", "axg_profile")
 
@@ -85,6 +85,7 @@ def lib():
             "axg_set_fields": [p, p, p, i64],
             "axg_get_fields": [p, p, p, i64],
             "axg_init_flat": [p, C.c_uint64, d],
+            "axg_init_thermal": [p, C.c_uint64, d],
             "axg_get_gw_state": [p, p, i64],
             "axg_set_gw_state": [p, p, i64],
             "axg_advance": [p, d, d, p],
@@ -221,6 +222,10 @@ class Lattice:
 
     def init_flat(self, seed, kmax):
         self._check(lib().axg_init_flat(self._h, int(seed), float(kmax)), "axg_init_flat")
+
+    def init_thermal(self, seed, tau_ini):
+        """Thermal state at tau_ini < 1 (get_initial_thermal, initial.f90:80-238)."""
+        self._check(lib().axg_init_thermal(self._h, seed, tau_ini), "axg_init_thermal")
 
     def gw_state(self):
         N = self.N

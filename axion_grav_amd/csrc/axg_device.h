@@ -229,3 +229,22 @@ __device__ __forceinline__ void stf(typename V2<T>::type* __restrict__ f, short2
     }
 }
 
+// Philox4x32-10 (Salmon et al., SC'11): the counter-based generator of the seeded initial
+// states and of the string fits' starting points.
+__device__ __forceinline__ void philox4x32(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                                           uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t h0 = __umulhi(M0, c0), l0 = M0 * c0;
+        uint32_t h1 = __umulhi(M1, c2), l1 = M1 * c2;
+        uint32_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = l1;
+        c2 = n2;
+        c3 = l0;
+        k0 += W0;
+        k1 += W1;
+    }
+}

@@ -146,6 +146,8 @@ void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, in
 void launch_cell_compact(hipStream_t s, int N, const int8_t* flags, int bit, const int64_t* block_offsets,
                          int32_t* out, int triplets, int64_t nblk);
 void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out);
+void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+                          double dv_wav, double thrm_sigma, double2* out);
 
 // String cells (flag bit 1) in site order `cells`: fit, chord length and velocity per cell.
 // rec: 11 doubles per string (pos(3) in site coordinates counted from 0, dir(3), gof, sl, vel(3));

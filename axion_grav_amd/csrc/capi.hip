@@ -846,12 +846,20 @@ extern "C" int axg_get_flags(axg_handle* h, int8_t* flags, int64_t nsite)
 
 // ------------------------------------------------------------------ field spectrum
 
-static void fft3d_work(axg_handle* h, double2* d)
+// In-place unnormalised 3D transform of an N^3 FP64 array: sign +1 is FFTW_BACKWARD (the
+// sign of every diagnostic transform), -1 FFTW_FORWARD (the thermal initial state).
+static void fft3d_work(axg_handle* h, double2* d, int sign = +1)
 {
     const int N = h->N;
-    launch_fft_rows<double, +1>(h->stream, N, d, (int64_t)N * N, h->plan, h->tw64);
-    launch_fft_cols<double, +1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw64);
-    launch_fft_cols<double, +1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw64);
+    if (sign > 0) {
+        launch_fft_rows<double, +1>(h->stream, N, d, (int64_t)N * N, h->plan, h->tw64);
+        launch_fft_cols<double, +1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw64);
+        launch_fft_cols<double, +1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw64);
+    } else {
+        launch_fft_rows<double, -1>(h->stream, N, d, (int64_t)N * N, h->plan, h->tw64);
+        launch_fft_cols<double, -1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw64);
+        launch_fft_cols<double, -1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw64);
+    }
 }
 
 static void ensure_spec_tables(axg_handle* h)
@@ -1017,6 +1025,26 @@ static void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
         c[0] = n0; c[1] = l1; c[2] = n2; c[3] = l0;
         k0 += W0; k1 += W1;
     }
+}
+
+extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
+{
+    return guarded(h, [&] {
+        // at tau_ini >= 1 the temperature is at or below the transition, m^2 <= 0, and the
+        // reference's zero mode has no finite amplitude (osc = 0 or imaginary)
+        require(tau_ini > 0 && tau_ini < 1, "the thermal initial state needs 0 < tau_ini < 1 (T above T_cr)");
+        const axg_params& p = h->p;
+        for (int ip = 1; ip <= 2; ip++) {
+            launch_thermal_modes(h->stream, h->N, seed, ip, tau_ini, p.sigma_cr, p.dwav, p.dv_wav, AXG_THRM_SIGMA,
+                                 h->work);
+            fft3d_work(h, h->work, -1);
+            if (ip == 1) store_from_work(h, h->phi[h->cur], h->cphi[h->cur]);
+            else store_from_work(h, h->pi, h->cpi);
+        }
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        h->have_flags = false;
+        h->nstr_rec = -1;
+    });
 }
 
 extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)

@@ -516,6 +516,53 @@ void launch_scale(hipStream_t s, int64_t n, double2* data, double scale)
     AXG_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ thermal initial state
+// One mode per thread (initial.f90:105-135 with initial_phi_phidot 372-390): dv_wav times a
+// unit complex Gaussian (Philox4x32-10 keyed by the seed, counter = (mode index, 'thml', ip);
+// Box-Muller on two 53-bit uniforms in (0,1]) times the Bose-Einstein amplitude of the mode,
+// in the reference's operation order.  out is then transformed with the FORWARD sign.
+__global__ void thermal_modes_kernel(int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+                                     double dv_wav, double thrm_sigma, double2* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
+        const int64_t nx = x < N / 2 ? x : x - N, ny = y < N / 2 ? y : y - N, nz = z < N / 2 ? z : z - N;
+        uint32_t c0 = (uint32_t)lin, c1 = (uint32_t)((uint64_t)lin >> 32), c2 = 0x74686d6cu, c3 = (uint32_t)ip;
+        philox4x32(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+        uint64_t a = (((uint64_t)c0 << 21) ^ (uint64_t)(c1 >> 11)) & ((1ull << 53) - 1);
+        uint64_t b = (((uint64_t)c2 << 21) ^ (uint64_t)(c3 >> 11)) & ((1ull << 53) - 1);
+        double u1 = ((double)a + 1.0) / 9007199254740992.0;
+        double u2 = ((double)b + 1.0) / 9007199254740992.0;
+        double r = sqrt(-2.0 * log(u1));
+        double th = 6.283185307179586 * u2;
+        double g1 = r * cos(th), g2 = r * sin(th);
+        // initial_phi_phidot
+        double q = (double)(nx * nx + ny * ny + nz * nz);
+        q = sqrt(q) * dwav;
+        double wav2 = q / tau_ini / sigma_cr;
+        wav2 = wav2 * wav2;
+        double thrm = thrm_sigma / tau_ini;
+        double osc = sqrt((thrm * thrm - thrm_sigma * thrm_sigma) / 3 + wav2);
+        double fbe = exp(-osc / thrm);
+        fbe = fbe / (1 - fbe);
+        double var = fbe * osc / dv_wav / sigma_cr;
+        double f = sqrt(var / 2);
+        if (ip == 1) f = f / (osc * sigma_cr);
+        out[lin] = double2{(g1 * dv_wav) * f, (g2 * dv_wav) * f};
+    }
+}
+
+void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+                          double dv_wav, double thrm_sigma, double2* out)
+{
+    hipLaunchKernelGGL(thermal_modes_kernel, dim3(4096), dim3(256), 0, s, N, seed, ip, tau_ini, sigma_cr, dwav, dv_wav,
+                       thrm_sigma, out);
+    AXG_HIP(hipGetLastError());
+}
+
 #define AXG_INST_DIAG(T, C)                                                                                      \
     template void launch_energy<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*, double, \
                                       double, double, double*, int, double*);                                    \

@@ -19,25 +19,6 @@ constexpr int MAXIT = 1000;    // max_iter_string_pos
 constexpr double PI_T = 3.14159265359;
 constexpr double TWOPI_T = 2 * PI_T;
 
-// Philox4x32-10 (Salmon et al., SC'11).
-__device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
-                                       uint32_t k1)
-{
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        uint32_t h0 = __umulhi(M0, c0), l0 = M0 * c0;
-        uint32_t h1 = __umulhi(M1, c2), l1 = M1 * c2;
-        uint32_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = l1;
-        c2 = n2;
-        c3 = l0;
-        k0 += W0;
-        k1 += W1;
-    }
-}
-
 // The four uniforms of start point i of string s: one Philox block keyed by the seed, counter
 // (i, s, 'strg').  The reference reads them from an MKL VSL MCG59 stream positioned at
 // s * 4004 (defects_3d.f90:137-162, 465); a counter-based generator keeps one stream per string
@@ -45,7 +26,7 @@ __device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2,
 __device__ __forceinline__ void uniforms(uint32_t k0, uint32_t k1, int64_t s, int i, double u[4])
 {
     uint32_t c0 = (uint32_t)i, c1 = (uint32_t)s, c2 = (uint32_t)((uint64_t)s >> 32), c3 = 0x73747267u;
-    philox(c0, c1, c2, c3, k0, k1);
+    philox4x32(c0, c1, c2, c3, k0, k1);
     const double sc = 2.3283064365386963e-10;  // 2^-32
     u[0] = ((double)c0 + 0.5) * sc;
     u[1] = ((double)c1 + 0.5) * sc;

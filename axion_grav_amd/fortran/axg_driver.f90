@@ -10,9 +10,9 @@
 ! file _param.chk ('idx tau flag') and _phi_<idx>.chk (raw FP64 complex phi then pi).
 !
 ! Keys this build adds (all optional): precision = fp64 | fp32c, num_bin_grav (25),
-! gw_fix_swap (F), dtau_fixed (0: use get_dtau), potential_kappa0 (0), ic_seed and ic_kmax
-! (the seeded flat-spectrum initial field used for a fresh start), ic_file (raw phi,pi file to
-! start from instead), chk_gw (F: also checkpoint the GW accumulators, which the reference
+! gw_fix_swap (F), dtau_fixed (0: use get_dtau), potential_kappa0 (0), ic_seed (seed of a
+! fresh start: the thermal state at tau_ini when initial_loop = F, else the flat-spectrum field
+! up to ic_kmax), ic_file (raw phi,pi file to start from instead), chk_gw (F: also checkpoint the GW accumulators, which the reference
 ! does not), device (0).
 program axion_3d
   use, intrinsic :: iso_c_binding
@@ -330,7 +330,13 @@ contains
        close(u)
        call axg_check(axg_set_fields(h, phi, pi, n3), h, 'axg_set_fields')
        deallocate(phi, pi)
+    else if (.not. initial_loop) then
+       ! initial.f90:62-66: the thermal state before the transition
+       write(*, '(A)') ' #starting from thermal distribution before phase transition'
+       call axg_check(axg_init_thermal(h, ic_seed, tau_ini), h, 'axg_init_thermal')
     else
+       ! in place of get_initial_loop (a relaxed string loop, not built here): the seeded
+       ! flat-spectrum field
        call axg_check(axg_init_flat(h, ic_seed, ic_kmax), h, 'axg_init_flat')
     end if
     open(newunit=u, file=trim(file_resume), action='write', status='replace', iostat=ios)

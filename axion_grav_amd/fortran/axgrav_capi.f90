@@ -103,6 +103,13 @@ module axgrav_capi
        real(c_double), value :: kmax
      end function axg_init_flat
 
+     integer(c_int) function axg_init_thermal(h, seed, tau_ini) bind(C, name="axg_init_thermal")
+       import :: c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: seed
+       real(c_double), value :: tau_ini
+     end function axg_init_thermal
+
      integer(c_int) function axg_get_gw_state(h, a, count) bind(C, name="axg_get_gw_state")
        import :: c_int, c_ptr, c_int64_t, c_double_complex
        type(c_ptr), value :: h

@@ -92,6 +92,15 @@ int axg_get_fields(axg_handle* h, double* phi, double* pi, int64_t nsite);
  * for |k| < kmax, zero above, BACKWARD FFT, rescaled to <|phi|^2> = 1; pi = 0. */
 int axg_init_flat(axg_handle* h, uint64_t seed, double kmax);
 
+/* Thermal initial state before the transition (get_initial_thermal, initial.f90:80-238, with
+ * initial_phi_phidot 372-390): every mode of phi and phidot gets dv_wav times a unit complex
+ * Gaussian times its Bose-Einstein amplitude at temperature thrm_sigma/tau_ini, then the
+ * unnormalised FORWARD transform.  The Gaussians come from Philox4x32-10 keyed by seed, one
+ * draw per mode (the reference's MKL VSL stream is not reproduced: the state agrees in
+ * distribution, not draw for draw).  Needs 0 < tau_ini < 1 (AXG_EINVAL otherwise: at
+ * tau_ini >= 1 the reference's zero mode is not finite). */
+int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini);
+
 /* GW accumulators arr_grav (grav_waves.f90:12,74) on the half spectrum kx = 0..N/2:
  * count = 4 * N * N * (N/2+1) complex, ordered [m][kz][ky][kx] with m = (1,1),(2,1),(1,2),
  * (2,2) of arr_grav(a,b,...), interleaved (re, im).  The other half is the complex

@@ -68,6 +68,10 @@ def lib():
         L.orc_velocity_string.argtypes = [p, p, i64, i64, i64, i64, p, d, p]
         L.orc_strings.restype = i64
         L.orc_strings.argtypes = [p, p, i64, p, C.c_uint32, C.c_uint32, d, p, p]
+        L.orc_thermal_factor.restype = d
+        L.orc_thermal_factor.argtypes = [C.c_int, i64, d, d, d, d]
+        L.orc_thermal_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
+        L.orc_ic_thermal.argtypes = [p, p, i64, C.c_uint64, d, d, d, d]
         _LIB = L
     return _LIB
 
@@ -279,3 +283,22 @@ def strings(phi, pi, flags, key0, key1, dspa):
     n = lib().orc_strings(_p(phi), _p(pi), N, _p(np.ascontiguousarray(flags, np.int8)), key0, key1, dspa,
                           _p(rec), _p(sums))
     return rec[:n], sums
+
+
+# ------------------------------------------------------------------ thermal initial state
+
+def thermal_factor(ip, n2, tau_ini, sigma_cr, dwav, dv_wav):
+    return lib().orc_thermal_factor(ip, n2, tau_ini, sigma_cr, dwav, dv_wav)
+
+
+def thermal_gauss(seed, ip, lin):
+    re, im = C.c_double(), C.c_double()
+    lib().orc_thermal_gauss(seed, ip, lin, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def ic_thermal(N, seed, tau_ini, sigma_cr, dwav, dv_wav):
+    phi = np.zeros((N, N, N), np.complex128)
+    pi = np.zeros((N, N, N), np.complex128)
+    lib().orc_ic_thermal(_p(phi), _p(pi), N, seed, tau_ini, sigma_cr, dwav, dv_wav)
+    return phi, pi

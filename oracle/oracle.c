@@ -1052,3 +1052,71 @@ int64_t orc_strings(const double* phi, const double* pi, int64_t N, const int8_t
     free(r);
     return ns;
 }
+
+/* ------------------------------------------------------------ thermal initial state */
+
+/* initial.f90:372-390 initial_phi_phidot with get_comwav_initial (363-370): the factor that
+ * turns a unit complex Gaussian times dv_wav into the amplitude of the mode |n|^2 = n2 of phi
+ * (ip = 1) or phidot (ip = 2) at tau_ini: Bose-Einstein occupation of a mode of frequency
+ * osc = sqrt(m^2(T) + k_phys^2), T = thrm_sigma / tau_ini, in the reference's order. */
+double orc_thermal_factor(int ip, int64_t n2, double tau_ini, double sigma_cr, double dwav, double dv_wav)
+{
+    double q = (double)n2;
+    q = sqrt(q) * dwav;
+    double wav2 = q / tau_ini / sigma_cr;
+    wav2 = wav2 * wav2;
+    double thrm = THRM_SIGMA / tau_ini;
+    double osc = sqrt(orc_msquared(thrm) + wav2);
+    double fbe = exp(-osc / thrm);
+    fbe = fbe / (1 - fbe);
+    double var = fbe * osc / dv_wav / sigma_cr;
+    double f = sqrt(var / 2);
+    if (ip == 1) f = f / (osc * sigma_cr);
+    return f;
+}
+
+/* Unit complex Gaussian of mode lin (site-order index of the wavevector) for field ip, from
+ * Philox4x32-10 keyed by the seed (Box-Muller on two 53-bit uniforms in (0,1]).  The
+ * reference draws these from an MKL VSL MCG59 stream (initial.f90:105-135); this keeps one
+ * draw per mode, independent of thread and rank counts. */
+void orc_thermal_gauss(uint64_t seed, int ip, uint64_t lin, double* re, double* im)
+{
+    uint32_t c[4] = {(uint32_t)lin, (uint32_t)(lin >> 32), 0x74686d6cu, (uint32_t)ip};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)(c[1] >> 11);
+    uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)(c[3] >> 11);
+    a &= (1ull << 53) - 1; b &= (1ull << 53) - 1;
+    double u1 = ((double)a + 1.0) / 9007199254740992.0;
+    double u2 = ((double)b + 1.0) / 9007199254740992.0;
+    double r = sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586 * u2;
+    *re = r * cos(th);
+    *im = r * sin(th);
+}
+
+/* initial.f90:80-238 get_initial_thermal (one rank): for phi then phidot, every mode of the
+ * wavevector grid (x fastest) gets dv_wav times a unit complex Gaussian times
+ * orc_thermal_factor, then the unnormalised FORWARD transform (FFTW_FORWARD, 332-333) gives
+ * the field in real space. */
+void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double tau_ini, double sigma_cr,
+                    double dwav, double dv_wav)
+{
+    int64_t n3 = N * N * N;
+    oc* buf = (oc*)malloc(sizeof(oc) * n3);
+    for (int ip = 1; ip <= 2; ip++) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int64_t z = 0; z < N; z++)
+            for (int64_t y = 0; y < N; y++)
+                for (int64_t x = 0; x < N; x++) {
+                    int64_t lin = x + N * (y + N * z);
+                    int64_t nx = wrapn(x, N), ny = wrapn(y, N), nz = wrapn(z, N);
+                    double g1, g2;
+                    orc_thermal_gauss(seed, ip, (uint64_t)lin, &g1, &g2);
+                    double f = orc_thermal_factor(ip, nx * nx + ny * ny + nz * nz, tau_ini, sigma_cr, dwav, dv_wav);
+                    buf[lin].re = (g1 * dv_wav) * f;
+                    buf[lin].im = (g2 * dv_wav) * f;
+                }
+        orc_fft3d((const double*)buf, ip == 1 ? phi : pi, N, -1);
+    }
+    free(buf);
+}

@@ -310,6 +310,29 @@ def test_string_velocity_of_moving_string():
     assert np.abs(r[:, 7] - 1).max() < 0.15
 
 
+# ---------------------------------------------------------------- thermal initial state
+
+@pytest.mark.parametrize("N,prec", [(32, capi.FP64), (48, capi.FP64), (32, capi.FP32C)])
+def test_thermal_state_matches_oracle(N, prec):
+    """The GPU thermal state (same Philox draws, device exp/log/sqrt, FP64 transform) against
+    the oracle: <= 1e-12 relative in FP64, the FP32C storage rounding otherwise."""
+    say("thermal", N, prec)
+    prm, P = params(N, grav_spec=False, initial_loop=False, precision=prec)
+    with capi.Lattice(prm) as L:
+        L.init_thermal(77, 0.2)
+        phi, pi = L.get_fields()
+    ophi, opi = O.ic_thermal(N, 77, 0.2, P["sigma_cr"], P["dwav"], P["dv_wav"])
+    tol = 1e-12 if prec == capi.FP64 else 1e-10
+    assert relerr(phi, ophi) < tol and relerr(pi, opi) < tol
+
+
+def test_thermal_state_rejects_tau_ini_at_or_above_one():
+    prm, _ = params(16, grav_spec=False, initial_loop=False)
+    with capi.Lattice(prm) as L:
+        with pytest.raises(capi.AxgError, match="tau_ini"):
+            L.init_thermal(1, 1.0)
+
+
 # ---------------------------------------------------------------- FP32C vs FP64
 
 def test_fp32c_tracks_fp64_gw():

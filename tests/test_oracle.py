@@ -594,3 +594,47 @@ def test_strings_records_and_sums():
     assert np.array_equal(rec, rec2)
     rec3, _ = O.strings(phi, pi, flags, 7, 9, P["dspa"])   # another seed: other starting points
     assert not np.array_equal(rec[:, 7], rec3[:, 7])
+
+
+# ------------------------------------------------------------------ thermal initial state (initial.f90:80-238, 372-390)
+
+def thermal_continuum(tau, sigma):
+    """<|phi|^2> = tau^3 int d^3p/(2pi)^3 n(w)/w and <|phidot|^2> = tau^3 sigma^2 int d^3p/(2pi)^3
+    n(w) w for the Bose-Einstein occupation n at T = thrm_sigma/tau with the thermal mass
+    m^2 = (T^2 - T_cr^2)/3 (at tau = 1, m = 0 these are the reference's own check values
+    T^2/12 and (T^2 pi sigma)^2/30, initial.f90:426-433)."""
+    from scipy.integrate import quad
+    T0 = 2.449489742783178
+    T = T0 / tau
+    m2 = (T * T - T0 * T0) / 3
+    w = lambda p: np.sqrt(m2 + p * p)
+    n = lambda p: 1 / np.expm1(w(p) / T)
+    i1 = quad(lambda p: p * p * n(p) / w(p), 0, 60 * T, limit=400)[0] / (2 * np.pi ** 2)
+    i2 = quad(lambda p: p * p * n(p) * w(p), 0, 60 * T, limit=400)[0] / (2 * np.pi ** 2)
+    return tau ** 3 * i1, tau ** 3 * sigma ** 2 * i2
+
+
+@pytest.mark.parametrize("tau", [0.1, 0.3])
+def test_thermal_state_matches_continuum_variances(tau):
+    """The lattice thermal state's variances against the continuum integrals (the lattice
+    covers the thermal momenta: p_max/T ~ 7 at dspa = 0.25): phi to 3 %, phidot to 5 % (UV
+    cutoff), real and imaginary parts alike and uncorrelated."""
+    N = 64
+    P = phys(N)
+    phi, pi = O.ic_thermal(N, 5, tau, SQ_HALF, P["dwav"], P["dv_wav"])
+    e1, e2 = thermal_continuum(tau, SQ_HALF)
+    assert abs(np.mean(np.abs(phi) ** 2) / e1 - 1) < 0.03
+    assert abs(np.mean(np.abs(pi) ** 2) / e2 - 1) < 0.05
+    for f in (phi, pi):
+        v = np.mean(np.abs(f) ** 2)
+        assert abs(2 * np.mean(f.real ** 2) / v - 1) < 0.05
+        assert abs(np.mean(f.real * f.imag)) < 0.03 * v
+
+
+def test_thermal_gaussians_are_unit_normal():
+    g = np.array([O.thermal_gauss(9, ip, lin) for ip in (1, 2) for lin in range(20000)])
+    for part in (g.real, g.imag):
+        assert abs(part.mean()) < 0.02 and abs(part.var() - 1) < 0.03
+        assert abs(np.mean(part ** 4) - 3) < 0.15          # Gaussian kurtosis
+    assert abs(np.mean(g.real * g.imag)) < 0.02
+    assert O.thermal_gauss(9, 1, 5) != O.thermal_gauss(9, 2, 5) != O.thermal_gauss(10, 1, 5)
