@@ -57,6 +57,10 @@ class DefectOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_wall", C.c_int64), ("n_mask", C.c_int64)]
 
 
+class WallOut(C.Structure):
+    _fields_ = [("n_wall", C.c_int64), ("area", C.c_double)]
+
+
 class StringOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_vel", C.c_int64), ("length", C.c_double), ("vel", C.c_double * 3)]
 
@@ -64,7 +68,7 @@ class StringOut(C.Structure):
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_init_thermal", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_strings", "axg_get_strings", "axg_field_spectrum", "axg_energy", "axg_profile")
+           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum", "axg_energy", "axg_profile")
 
 _LIB = None
 
@@ -96,6 +100,8 @@ def lib():
             "axg_get_flags": [p, p, i64],
             "axg_strings": [p, C.c_uint32, C.c_uint32, C.POINTER(StringOut)],
             "axg_get_strings": [p, p, i64, p],
+            "axg_walls": [p, C.POINTER(WallOut)],
+            "axg_get_walls": [p, p, i64, p],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
@@ -283,6 +289,22 @@ class Lattice:
         n = cnt.value
         buf = np.zeros((max(n, 1), 11))
         self._check(lib().axg_get_strings(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_strings")
+        return buf[:n]
+
+    def walls(self):
+        """Fit every wall cell of the last defects() (needs id_walls): dict(n_wall, area = sum of
+        the cells' wall areas in lattice units)."""
+        out = WallOut()
+        self._check(lib().axg_walls(self._h, C.byref(out)), "axg_walls")
+        return dict(n_wall=out.n_wall, area=out.area)
+
+    def wall_records(self):
+        """(n_wall, 8): pos(3, site coordinates counted from 0), dir(3), gof, wa."""
+        cnt = C.c_int64()
+        self._check(lib().axg_get_walls(self._h, None, 0, C.byref(cnt)), "axg_get_walls")
+        n = cnt.value
+        buf = np.zeros((max(n, 1), 8))
+        self._check(lib().axg_get_walls(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_walls")
         return buf[:n]
 
     def field_spectrum(self, tau):

@@ -146,6 +146,13 @@ void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, in
 void launch_cell_compact(hipStream_t s, int N, const int8_t* flags, int bit, const int64_t* block_offsets,
                          int32_t* out, int triplets, int64_t nblk);
 void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out);
+// Wall cells (flag bit 2) in site order: plane fit and area per cell.  rec: 8 doubles per
+// wall (pos(3) in site coordinates counted from 0, dir(3), gof, wa); partial: 2 doubles per
+// block of WALL_NT walls (sum wa, cells with fewer than 3 edge points).
+constexpr int WALL_NT = 64;
+template <typename T, bool COMP>
+void launch_wall_fit(hipStream_t s, int N, const void* phi, const void* cphi, const int32_t* cells, int64_t nw,
+                     double* rec, double* partial);
 void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
                           double dv_wav, double thrm_sigma, double2* out);
 

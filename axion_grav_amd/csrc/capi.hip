@@ -136,6 +136,12 @@ struct axg_handle {
     double* str_part = nullptr;
     int64_t str_cap = 0;      // strings the three buffers hold
     int64_t nstr_rec = -1;    // strings in str_rec (-1: no fit since the fields changed)
+    int64_t nwall = 0;        // wall cells of the last classification
+    int32_t* wall_cells = nullptr;
+    double* wall_rec = nullptr;
+    double* wall_part = nullptr;
+    int64_t wall_cap = 0;
+    int64_t nwall_rec = -1;
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
     double* d_largeq = nullptr;
@@ -491,6 +497,7 @@ extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi
         }
         h->have_flags = false;
         h->nstr_rec = -1;
+        h->nwall_rec = -1;
         AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
@@ -604,6 +611,7 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     else run_steps<double, false>(h, t1, dt, nt);
     h->have_flags = false;
     h->nstr_rec = -1;
+    h->nwall_rec = -1;
     AXG_HIP(hipStreamSynchronize(h->stream));
     if (h->prof) h->collect();
 }
@@ -746,8 +754,10 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         out->n_mask = t[2];
         h->nmask = t[2];
         h->nstring = t[0];
+        h->nwall = t[1];
         h->have_flags = true;
         h->nstr_rec = -1;
+        h->nwall_rec = -1;
     });
 }
 
@@ -817,6 +827,73 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
         out->length = sums[0];
         for (int k = 0; k < 3; k++) out->vel[k] = out->n_vel > 0 ? sums[2 + k] / (double)out->n_vel : 0.0;
         h->nstr_rec = ns;
+    });
+}
+
+extern "C" int axg_walls(axg_handle* h, axg_wall_out* out)
+{
+    return guarded(h, [&] {
+        require(out != nullptr, "null output");
+        require(h->have_flags, "call axg_defects first");
+        require(h->p.id_walls != 0, "id_walls is off: no wall cells flagged");
+        const int64_t nblk = (h->n3 + 255) / 256;
+        const int64_t nw = h->nwall;
+        const int64_t nwb = (nw + WALL_NT - 1) / WALL_NT;
+        if (nw > h->wall_cap) {
+            void* a = h->wall_cells;
+            void* b = h->wall_rec;
+            void* c = h->wall_part;
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            h->release_one(a);
+            h->release_one(b);
+            h->release_one(c);
+            h->wall_cap = 0;
+            h->wall_cells = (int32_t*)h->alloc((size_t)nw * sizeof(int32_t));
+            h->wall_rec = (double*)h->alloc((size_t)8 * nw * sizeof(double));
+            h->wall_part = (double*)h->alloc((size_t)(2 * nwb + 8) * sizeof(double));
+            h->wall_cap = nw;
+        }
+        double sums[2] = {0, 0};
+        if (nw > 0) {
+            int64_t* counts = h->blk;
+            int64_t* woffs = h->blk + 4 * nblk + 8;  // shares the string offsets' room
+            int64_t* wtot = woffs + nblk;
+            launch_scan_counts(h->stream, counts, woffs, nblk, wtot, 1);
+            launch_cell_compact(h->stream, h->N, h->flags, 2, woffs, h->wall_cells, 0, nblk);
+            double* dsum = h->wall_part + 2 * nwb;
+            if (h->comp)
+                launch_wall_fit<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->wall_cells, nw,
+                                             h->wall_rec, h->wall_part);
+            else if (h->fp32)
+                launch_wall_fit<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->wall_cells, nw, h->wall_rec,
+                                              h->wall_part);
+            else
+                launch_wall_fit<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->wall_cells, nw,
+                                               h->wall_rec, h->wall_part);
+            sum_columns(h->stream, h->wall_part, (int)nwb, 2, dsum);
+            AXG_HIP(hipMemcpyAsync(sums, dsum, sizeof sums, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+        out->n_wall = nw;
+        out->area = sums[0];
+        h->nwall_rec = nw;
+        if (sums[1] > 0)
+            throw Error(AXG_ENUMERIC, "position_wall: " + std::to_string((int64_t)sums[1]) +
+                                          " wall cells have fewer than 3 edges where Im(phi) changes sign");
+    });
+}
+
+extern "C" int axg_get_walls(axg_handle* h, double* rec, int64_t cap, int64_t* count)
+{
+    return guarded(h, [&] {
+        require(count != nullptr, "null count");
+        require(h->nwall_rec >= 0, "call axg_walls first (after the last change of the fields)");
+        *count = h->nwall_rec;
+        if (h->nwall_rec == 0 || rec == nullptr || cap == 0) return;  // size query
+        require(cap >= h->nwall_rec, "wall buffer too small");
+        AXG_HIP(hipMemcpyAsync(rec, h->wall_rec, (size_t)8 * h->nwall_rec * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -1044,6 +1121,7 @@ extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
         AXG_HIP(hipStreamSynchronize(h->stream));
         h->have_flags = false;
         h->nstr_rec = -1;
+        h->nwall_rec = -1;
     });
 }
 
@@ -1114,5 +1192,6 @@ extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
         (void)hipFree(pval);
         h->have_flags = false;
         h->nstr_rec = -1;
+        h->nwall_rec = -1;
     });
 }

@@ -434,6 +434,7 @@ contains
     double precision, intent(in) :: tau
     type(axg_defect_out) :: dc
     type(axg_string_out) :: so
+    type(axg_wall_out) :: wo
     double precision :: xi(3), largea(3), vel(3), sw(11), crho(3), rho, xi0, xi1, gam
     double precision, allocatable :: q(:), pw(:), nu(:), de(:), cv(:), nm(:)
     integer(c_int32_t), allocatable :: ml(:)
@@ -462,13 +463,16 @@ contains
     end if
     ! string length and velocities (add_string_length, defects_3d.f90:276-279, group_strings
     ! = F: the three lengths coincide); the generator's stream moves on with idx as the
-    ! reference's does between calls.  Wall areas (position_wall) are not computed by this
-    ! build: their parameters are reported as zero.
+    ! reference's does between calls.  Wall areas: add_wall_area (280-283, group_walls = F).
     if (id_strings) then
        call axg_check(axg_strings(h, seed(1), seed(2) + int(idx, c_int32_t), so), h, 'axg_strings')
        xi(1:3) = so%length
        vel(1:3) = so%vel
        if (so%n_vel == 0) write(*, '(A)') ' velocity cannot be computed because there are no strings'
+    end if
+    if (id_walls) then
+       call axg_check(axg_walls(h, wo), h, 'axg_walls')
+       largea(1:3) = wo%area
     end if
     xi(1:3) = (tau/prm%dspa)**2*xi(1:3)/dble(n3)/4
     largea(1:3) = (tau/prm%dspa)*largea(1:3)/dble(n3)/2
@@ -476,6 +480,7 @@ contains
        write(*, '(A,F14.6,3E20.10e3)') '  tau, string parameter: ', tau, xi(1:3)
        write(*, '(A,F14.6,3E20.10e3)') '  tau, string velocity parameter: ', tau, vel(1:3)
     end if
+    if (id_walls) write(*, '(A,F14.6,3E20.10e3)') '  tau, wall parameter: ', tau, largea(1:3)
     sw = [tau, tau**2*zeta, xi(1:3), largea(1:3), vel(1:3)]
     crho = 0d0
     if (spec_est) then

@@ -50,6 +50,12 @@ module axgrav_capi
      real(c_double) :: vel(3) = 0
   end type axg_string_out
 
+  ! struct axg_wall_out
+  type, bind(C) :: axg_wall_out
+     integer(c_int64_t) :: n_wall = 0
+     real(c_double) :: area = 0
+  end type axg_wall_out
+
   interface
      integer(c_int) function axg_derive(k, p, zeta, width_cr, thickness_cr) bind(C, name="axg_derive")
        import :: c_int, c_double, axg_keys, axg_params
@@ -173,6 +179,20 @@ module axgrav_capi
        integer(c_int64_t), value :: cap
        integer(c_int64_t), intent(out) :: count
      end function axg_get_strings
+
+     integer(c_int) function axg_walls(h, out) bind(C, name="axg_walls")
+       import :: c_int, c_ptr, axg_wall_out
+       type(c_ptr), value :: h
+       type(axg_wall_out), intent(out) :: out
+     end function axg_walls
+
+     integer(c_int) function axg_get_walls(h, rec, cap, count) bind(C, name="axg_get_walls")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: rec(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: count
+     end function axg_get_walls
 
      integer(c_int) function axg_field_spectrum(h, tau, qmid, power, conv, num, rho, nbins) &
           bind(C, name="axg_field_spectrum")

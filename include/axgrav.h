@@ -165,6 +165,26 @@ int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_string_out* o
  * *count = strings written; rec == NULL or cap == 0: size query. */
 int axg_get_strings(axg_handle* h, double* rec, int64_t cap, int64_t* count);
 
+typedef struct axg_wall_out {
+    int64_t n_wall;     /* wall cells fitted */
+    double area;        /* sum of the cells' wall areas wa, in lattice units: id_defects' wa(1) =
+                           wa(2) = wa(3) with group_walls = F */
+} axg_wall_out;
+
+/* Wall area (defects_3d.f90:183-188 with position_wall 1058-1202 and add_wall_area 2145-2204,
+ * group_walls = F), one GPU thread per wall cell of the last axg_defects: the points where
+ * sin(ph) changes sign on the cell's edges, the plane normal from the reference's 20 x 20 grid
+ * search, the polygon area in that plane.  No random numbers: the GPU and the reference
+ * algorithm agree to round-off.  A wall cell with fewer than 3 such edges is the reference's
+ * stop: AXG_ENUMERIC (after *out is filled).  The driver's wall parameter is
+ * (tau/dspa) * area / N^3 / 2 (evolve.f90:192). */
+int axg_walls(axg_handle* h, axg_wall_out* out);
+/* The per-wall records of the last axg_walls: 8 doubles per wall, pos(3) (the fitted plane's
+ * point nearest the cell centre plus ixyz - 1/2, site coordinates counted from 0), dir(3) (the
+ * normal), gof, wa, in site order (the arr_wall list of defects_3d.f90:8-20).  Size query as
+ * axg_get_strings. */
+int axg_get_walls(axg_handle* h, double* rec, int64_t cap, int64_t* count);
+
 /* spectrum_estimation's deterministic part (spectrum.f90:242-286 with compute_matrix_deconv
  * 344-444 and compute_conv_spec 466-567, degrade_spec = 0): the masked pi spectrum
  * deconvolved with the mask window.  tau is tau_id(idx).  power = the deconvolved

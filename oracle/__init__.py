@@ -72,6 +72,10 @@ def lib():
         L.orc_thermal_factor.argtypes = [C.c_int, i64, d, d, d, d]
         L.orc_thermal_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
         L.orc_ic_thermal.argtypes = [p, p, i64, C.c_uint64, d, d, d, d]
+        L.orc_position_wall.restype = C.c_int
+        L.orc_position_wall.argtypes = [p, p, p, p, p]
+        L.orc_walls.restype = i64
+        L.orc_walls.argtypes = [p, i64, p, p, p]
         _LIB = L
     return _LIB
 
@@ -302,3 +306,22 @@ def ic_thermal(N, seed, tau_ini, sigma_cr, dwav, dv_wav):
     pi = np.zeros((N, N, N), np.complex128)
     lib().orc_ic_thermal(_p(phi), _p(pi), N, seed, tau_ini, sigma_cr, dwav, dv_wav)
     return phi, pi
+
+
+# ------------------------------------------------------------------ walls
+
+def position_wall(ph):
+    """Fit of one wall cell: (m edge points, pos relative to the cell centre, dir, gof, wa)."""
+    pos, dirv, g, wa = np.zeros(3), np.zeros(3), np.zeros(1), np.zeros(1)
+    m = lib().orc_position_wall(_p(np.ascontiguousarray(ph, np.float64)), _p(pos), _p(dirv), _p(g), _p(wa))
+    return m, pos, dirv, float(g[0]), float(wa[0])
+
+
+def walls(phi, flags):
+    """All wall cells (flag bit 2) in site order: records (nw, 8) = pos(3, site coordinates
+    counted from 0), dir(3), gof, wa; sums = (sum wa, cells with < 3 edge points)."""
+    nw = int(np.count_nonzero(flags & 2))
+    rec = np.zeros((max(nw, 1), 8))
+    sums = np.zeros(2)
+    n = lib().orc_walls(_p(phi), phi.shape[0], _p(np.ascontiguousarray(flags, np.int8)), _p(rec), _p(sums))
+    return rec[:n], sums

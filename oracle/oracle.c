@@ -1120,3 +1120,133 @@ void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double ta
     }
     free(buf);
 }
+
+/* ------------------------------------------------------------ wall fit and area */
+
+/* defects_3d.f90:1058-1202 position_wall: the points where sin(ph) changes sign on the cell's
+ * edges (linear interpolation), in the reference's edge order; the plane normal by a 20 x 20
+ * grid search in (mu, phi) minimising the spread of the points' distance along it; the
+ * orientation by the corners' cos(ph); and the area of the polygon the points span in that
+ * plane (angular sort, triangle fan).  Returns the number of edge points m; m < 3 is the
+ * reference's stop ('something wrong in position_wall') and leaves the outputs unset. */
+int orc_position_wall(const double* ph, double* pos, double* dir, double* gof, double* wa)
+{
+    double pp[12][3], th[12], uv[12][2];
+    int m = 0;
+    for (int iz = 0; iz < 2; iz++)
+        for (int iy = 0; iy < 2; iy++) {
+            double u = sin(ph[(iz * 2 + iy) * 2 + 0]), v = sin(ph[(iz * 2 + iy) * 2 + 1]);
+            if (u * v < 0) { pp[m][0] = (u + v) / (u - v) / 2; pp[m][1] = iy - 0.5; pp[m][2] = iz - 0.5; m++; }
+        }
+    for (int iy = 0; iy < 2; iy++)
+        for (int ix = 0; ix < 2; ix++) {
+            double u = sin(ph[(0 * 2 + iy) * 2 + ix]), v = sin(ph[(1 * 2 + iy) * 2 + ix]);
+            if (u * v < 0) { pp[m][0] = ix - 0.5; pp[m][1] = iy - 0.5; pp[m][2] = (u + v) / (u - v) / 2; m++; }
+        }
+    for (int iz = 0; iz < 2; iz++)
+        for (int ix = 0; ix < 2; ix++) {
+            double u = sin(ph[(iz * 2 + 0) * 2 + ix]), v = sin(ph[(iz * 2 + 1) * 2 + ix]);
+            if (u * v < 0) { pp[m][0] = ix - 0.5; pp[m][1] = (u + v) / (u - v) / 2; pp[m][2] = iz - 0.5; m++; }
+        }
+    if (m < 3) return m;
+    double chi2min = 6.0, mut = 0, phit = 0, wt = 0;
+    for (int imu = 1; imu <= 20; imu++) {
+        double mu = (-20 + 2 * imu - 1.0) / 20;
+        for (int iphi = 1; iphi <= 20; iphi++) {
+            double phi = PI_T * (2 * iphi - 1.0 - 20) / 20;
+            double d0 = sqrt(1 - mu * mu) * cos(phi), d1 = sqrt(1 - mu * mu) * sin(phi), d2 = mu;
+            double w = 0, chi2 = 0;
+            for (int i = 0; i < m; i++) {
+                double d = pp[i][0] * d0 + pp[i][1] * d1 + pp[i][2] * d2;
+                w = w + d;
+                chi2 = chi2 + d * d;
+            }
+            if (w < 0) w = 0;
+            else w = w / m;
+            chi2 = chi2 - m * w * w;
+            if (chi2 < chi2min) { chi2min = chi2; mut = mu; phit = phi; wt = w; }
+        }
+    }
+    double dr[3] = {sqrt(1 - mut * mut) * cos(phit), sqrt(1 - mut * mut) * sin(phit), mut};
+    int num = 0;
+    for (int iz = 0; iz < 2; iz++)
+        for (int iy = 0; iy < 2; iy++)
+            for (int ix = 0; ix < 2; ix++) {
+                double d = (ix - 0.5) * dr[0] + (iy - 0.5) * dr[1] + (iz - 0.5) * dr[2] - wt;
+                if (d * cos(ph[(iz * 2 + iy) * 2 + ix]) > 0) num++;
+            }
+    if (num <= 4) { dr[0] = -dr[0]; dr[1] = -dr[1]; dr[2] = -dr[2]; wt = -wt; }
+    for (int k = 0; k < 3; k++) { dir[k] = dr[k]; pos[k] = wt * dr[k]; }
+    *gof = exp(-chi2min / 2);
+    double e1[3] = {mut * cos(phit), mut * sin(phit), -sqrt(1 - mut * mut)};
+    double e2[3] = {-sin(phit), cos(phit), 0.0};
+    for (int i = 0; i < m; i++) {
+        for (int k = 0; k < 3; k++) pp[i][k] = pp[i][k] - pos[k];
+        double u = e1[0] * pp[i][0] + e1[1] * pp[i][1] + e1[2] * pp[i][2];
+        double v = e2[0] * pp[i][0] + e2[1] * pp[i][1] + e2[2] * pp[i][2];
+        th[i] = atan2(v, u);
+        uv[i][0] = u;
+        uv[i][1] = v;
+    }
+    for (int i = m - 2; i >= 0; i--)
+        for (int j = 0; j <= i; j++)
+            if (th[j] > th[j + 1]) {
+                double t = th[j]; th[j] = th[j + 1]; th[j + 1] = t;
+                double a = uv[j][0], b = uv[j][1];
+                uv[j][0] = uv[j + 1][0]; uv[j][1] = uv[j + 1][1];
+                uv[j + 1][0] = a; uv[j + 1][1] = b;
+            }
+    double area = 0, w = th[m - 1] - TWOPI_T, u = sqrt(uv[m - 1][0] * uv[m - 1][0] + uv[m - 1][1] * uv[m - 1][1]);
+    for (int i = 0; i < m; i++) {
+        double v = sqrt(uv[i][0] * uv[i][0] + uv[i][1] * uv[i][1]);
+        w = th[i] - w;
+        area = area + 0.5 * u * v * sin(w);
+        u = v;
+        w = th[i];
+    }
+    *wa = area;
+    return m;
+}
+
+/* id_defects' wall loop (defects_3d.f90:183-188: cells with the wall flag in site order,
+ * position_wall, pos + ixyz - 1/2) and add_wall_area (2145-2204, group_walls = F).  rec (or
+ * NULL) receives 8 doubles per wall: pos(3), dir(3), gof, wa.  sums = {sum wa, cells with
+ * fewer than 3 edge points}.  Returns the wall count. */
+int64_t orc_walls(const double* phi, int64_t N, const int8_t* flags, double* rec, double* sums)
+{
+    const oc* f = (const oc*)phi;
+    int64_t n3 = N * N * N, nw = 0;
+    for (int64_t i = 0; i < n3; i++) nw += (flags[i] & 2) != 0;
+    int64_t* cell = (int64_t*)malloc(sizeof(int64_t) * (nw > 0 ? nw : 1));
+    double* r = (double*)malloc(sizeof(double) * 8 * (nw > 0 ? nw : 1));
+    int* bad = (int*)calloc(nw > 0 ? nw : 1, sizeof(int));
+    nw = 0;
+    for (int64_t i = 0; i < n3; i++)
+        if (flags[i] & 2) cell[nw++] = i;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < nw; s++) {
+        int64_t i = cell[s], x = i % N, y = (i / N) % N, z = i / (N * N);
+        double ph[8];
+        for (int c = 0; c < 8; c++) {
+            int64_t cx = (x + (c & 1)) % N, cy = (y + ((c >> 1) & 1)) % N, cz = (z + ((c >> 2) & 1)) % N;
+            oc v = f[cx + N * (cy + N * cz)];
+            ph[c] = atan2(v.im, v.re);
+        }
+        double* o = r + 8 * s;
+        if (orc_position_wall(ph, o, o + 3, o + 6, o + 7) < 3) {
+            bad[s] = 1;
+            for (int k = 0; k < 8; k++) o[k] = 0;
+            continue;
+        }
+        o[0] += (double)(x + 1) - 0.5;
+        o[1] += (double)(y + 1) - 0.5;
+        o[2] += (double)(z + 1) - 0.5;
+    }
+    double wa = 0, nb = 0;
+    for (int64_t s = 0; s < nw; s++) { wa += r[8 * s + 7]; nb += bad[s]; }
+    sums[0] = wa;
+    sums[1] = nb;
+    if (rec) memcpy(rec, r, sizeof(double) * 8 * nw);
+    free(cell); free(r); free(bad);
+    return nw;
+}

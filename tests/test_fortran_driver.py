@@ -136,6 +136,10 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
         xi = (taus[idx] / d["dspa"]) ** 2 * sums[0] / N ** 3 / 4
         assert np.allclose(sw[idx - 1, 3:6], xi, rtol=0.02), (sw[idx - 1, 3:6], xi)
         assert np.allclose(sw[idx - 1, 9:12], sums[2:5] / sums[1], rtol=0.02)
+        # wall parameter (evolve.f90:192), no random numbers
+        _, wsums = O.walls(phi, flags)
+        largea = (taus[idx] / d["dspa"]) * wsums[0] / N ** 3 / 2
+        assert np.allclose(sw[idx - 1, 6:9], largea, rtol=1e-9), (sw[idx - 1, 6:9], largea)
     # the final field checkpoint and the resume file (evolve.f90:401-437)
     idx, tau, flag = open(tmp_path / "run_param.chk").read().split()
     assert int(idx) == 2 and abs(float(tau) - 2.0) < 1e-12 and flag == "F"

@@ -286,6 +286,30 @@ def test_strings_match_oracle(prec):
     assert np.all(np.abs(out["vel"] - ovel) <= 1e-8 * np.abs(ovel))
 
 
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_walls_match_oracle(prec):
+    """Wall fits on the GPU against the oracle on the same field and cell list: no random
+    numbers, the same grid search and polygon; records differ by last-bit libm differences
+    only.  Bars: every record within 1e-9, the summed area within 1e-12."""
+    say("walls", prec)
+    N = 32
+    prm, P = params(N, grav_spec=False, precision=prec, id_strings=True, id_walls=True, spec_est=True)
+    phi, pi = ic(N, 31, 6.0)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        f_phi, _ = L.get_fields()
+        cnt = L.defects()
+        out = L.walls()
+        rec = L.wall_records()
+    flags, ocnt = O.classify(f_phi)
+    orec, osums = O.walls(f_phi, flags)
+    assert cnt[1] == ocnt[1] == out["n_wall"] > 50
+    rel = np.abs(rec - orec) / np.maximum(1, np.abs(orec))
+    print(f"  walls {len(rec)}: max rel diff {rel.max():.2e}; area {out['area']:.10f} vs {osums[0]:.10f}")
+    assert rel.max() < 1e-9
+    assert abs(out["area"] - osums[0]) <= 1e-12 * osums[0]
+
+
 def test_string_velocity_of_moving_string():
     """A straight string along z moving in x at 0.3: phi = (X - X0 - u tau) + i (Y - Y0) in
     physical units (pi = -u), so the fitted velocity of each of its cells is (u, 0, 0) to

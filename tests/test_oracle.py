@@ -638,3 +638,49 @@ def test_thermal_gaussians_are_unit_normal():
         assert abs(np.mean(part ** 4) - 3) < 0.15          # Gaussian kurtosis
     assert abs(np.mean(g.real * g.imag)) < 0.02
     assert O.thermal_gauss(9, 1, 5) != O.thermal_gauss(9, 2, 5) != O.thermal_gauss(10, 1, 5)
+
+
+# ------------------------------------------------------------------ wall fit (defects_3d.f90:1058-1202)
+
+def planar_wall_phases(n, d, eps=1e-3):
+    """Corner phases of phi = -1 + i eps (r.n - d): Im(phi) changes sign on the plane r.n = d."""
+    n = np.asarray(n, float)
+    n = n / np.linalg.norm(n)
+    return np.arctan2(eps * (corner_xyz() @ n - d), -np.ones(8))
+
+
+@pytest.mark.parametrize("n,d,m,area", [([1, 0, 0], 0.1, 4, 1.0),
+                                        ([1, 1, 0], 0.0, 4, 2 ** 0.5),
+                                        ([1, 1, 1], 0.0, 6, 3 * 3 ** 0.5 / 4),   # central hexagon
+                                        ([0.3, -0.2, 1.0], 0.12, 4, np.sqrt(1.13))])
+def test_wall_area_of_planar_walls(n, d, m, area):
+    """A planar wall through the cell: the edge points are exact (to O(eps^2)), the polygon
+    is the plane's cross-section of the cube, whose area is a closed form.  The fitted
+    normal comes from a 20 x 20 grid of directions (up to ~9 degrees off), so the area
+    agrees to 2 % and the normal to within the grid spacing."""
+    mm, pos, dirv, gof, wa = O.position_wall(planar_wall_phases(n, d))
+    assert mm == m
+    assert abs(wa - area) < 0.02 * area
+    nn = np.asarray(n, float) / np.linalg.norm(n)
+    assert abs(abs(dirv @ nn) - 1) < 0.02 and abs(np.linalg.norm(dirv) - 1) < 1e-12
+    assert gof > 0.95
+
+
+def test_wall_needs_three_edge_points():
+    # a uniform phase has no sign change of Im(phi) on any edge: the reference stops
+    assert O.position_wall(np.full(8, 0.3))[0] == 0
+
+
+def test_walls_records_and_sums():
+    N = 16
+    P = phys(N)
+    phi, _ = O.ic_flat(N, 3, 6.0, P["dwav"])
+    flags, counts = O.classify(phi)
+    rec, sums = O.walls(phi, flags)
+    assert rec.shape == (counts[1], 8) and counts[1] > 10 and sums[1] == 0
+    zz, yy, xx = np.nonzero(flags & 2)
+    assert np.abs(rec[:, :3] - (np.stack([xx, yy, zz], 1) + 0.5)).max() <= 0.87
+    assert abs(sums[0] - rec[:, 7].sum()) < 1e-12 * max(1.0, sums[0])
+    # a plane cuts a unit cube in at most ~1.41 (sqrt(2)); for non-planar edge points the triangle
+    # fan of the reference can give a slightly negative area
+    assert np.all(rec[:, 7] > -0.1) and np.all(rec[:, 7] < 1.5) and np.median(rec[:, 7]) > 0.3
