@@ -23,18 +23,27 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
     return r;
 }
 
-__global__ void sum_columns_kernel(const double* __restrict__ partial, int nwg, int width, double* __restrict__ out)
+// out[b] = sum over the nwg rows of column b of partial[nwg][width]: one block per column,
+// thread t sums rows t, t + 256, ... in order, then a fixed-shape tree (deterministic).
+__global__ void __launch_bounds__(256) sum_columns_kernel(const double* __restrict__ partial, int nwg, int width,
+                                                          double* __restrict__ out)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= width) return;
+    __shared__ double sh[256];
+    const int b = blockIdx.x;
     double s = 0;
-    for (int w = 0; w < nwg; w++) s += partial[(int64_t)w * width + b];
-    out[b] = s;
+    for (int w = threadIdx.x; w < nwg; w += 256) s += partial[(int64_t)w * width + b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = sh[0];
 }
 
 void sum_columns(hipStream_t s, const double* partial, int nwg, int width, double* out)
 {
-    hipLaunchKernelGGL(sum_columns_kernel, dim3((width + 255) / 256), dim3(256), 0, s, partial, nwg, width, out);
+    hipLaunchKernelGGL(sum_columns_kernel, dim3(width), dim3(256), 0, s, partial, nwg, width, out);
     AXG_HIP(hipGetLastError());
 }
 

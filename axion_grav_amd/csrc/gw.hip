@@ -32,6 +32,15 @@ template <typename T> int col_width(int N)
 // carries 8 elements of one transform in registers, the first load and last store go
 // directly between registers and HBM, and LDS is touched only between stages.
 
+// The accumulation pass keeps 8 k-points' projected sums per thread: at most 512 threads per
+// block so the compiler has 256 VGPRs.
+template <typename T, int N> constexpr int p2_ztx()
+{
+    int tx = p2_tx<T, N>();
+    while (tx > 1 && tx * (N / 8) > 512) tx /= 2;
+    return tx;
+}
+
 template <int N> constexpr int p2_xrows() { return (2048 / N) < N ? (2048 / N < 1 ? 1 : 2048 / N) : N; }
 
 template <typename T, int N>
@@ -92,15 +101,6 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
 #pragma unroll
         for (int e = 0; e < 8; e++) base[(int64_t)(j0 + e * TPT) * es] = v[e];
     }
-}
-
-// The accumulation pass keeps 8 k-points' projected sums per thread: at most 512 threads per
-// block so the compiler has 256 VGPRs.
-template <typename T, int N> constexpr int p2_ztx()
-{
-    int tx = p2_tx<T, N>();
-    while (tx > 1 && tx * (N / 8) > 512) tx /= 2;
-    return tx;
 }
 
 // y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled

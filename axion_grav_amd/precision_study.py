@@ -7,12 +7,12 @@
 Both precisions start from the same seeded flat-spectrum field (|k| < 1, <|phi|^2> = 1,
 pi = 0, lambda = 2 at zero temperature, radiation era from tau = 1) and advance in lockstep
 with GW accumulation on.  Every `--every` steps each lattice reports its 128-bin GW spectrum,
-its energy (FP64 reductions in both precisions), its string / wall / mask cell counts and its
-string length and velocities, and
+its energy (FP64 reductions in both precisions), its string / wall / mask cell counts, its
+string length and velocities and its wall area, and
 one JSON line gives the FP32C error against FP64: the per-bin relative error of the GW power
 (max and median over the bins that hold modes), the relative error of the energy terms, the
 cell counts, and the relative error of the summed string length and mean string velocities
-(fits seeded alike in both precisions).
+(fits seeded alike in both precisions) and of the summed wall area.
 
 The step is dt = 0.01 by default: with lambda = 2 the radiation-era potential term grows as
 tau^2, and the leapfrog passes its stability bound near tau = 23 (at dt = 0.05 the FP64 oracle
@@ -54,12 +54,14 @@ def compare(t, step, d64, d32):
                 cells_fp64=[int(x) for x in d64["cells"]], cells_fp32c=[int(x) for x in d32["cells"]],
                 string_length_fp64=d64["str"]["length"],
                 string_length_relerr=relerr(d32["str"]["length"], d64["str"]["length"]),
-                string_vel_relerr=[relerr(a, b) for a, b in zip(d32["str"]["vel"], d64["str"]["vel"])])
+                string_vel_relerr=[relerr(a, b) for a, b in zip(d32["str"]["vel"], d64["str"]["vel"])],
+                wall_area_fp64=d64["wall"]["area"],
+                wall_area_relerr=relerr(d32["wall"]["area"], d64["wall"]["area"]))
 
 
 def diag(L, t, step):
     cells = L.defects()
-    return dict(spec=L.gw_spectrum(), energy=L.energy(t), cells=cells, str=L.strings(7, step))
+    return dict(spec=L.gw_spectrum(), energy=L.energy(t), cells=cells, str=L.strings(7, step), wall=L.walls())
 
 
 def main():

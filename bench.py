@@ -49,8 +49,8 @@ MODEL_REALS = 40
 # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (calibrated against a copy of known
 # size) for the 512^3 workload; the profile summary holds the commands and the calibration.
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "lattice_step_512_hbm_traffic.md")
-TRAFFIC_KERNEL = {("kick_source_drift", "fp32c"): "kick_source_xfft_kernel<float, true, 512>",
-                  ("kick_source_drift", "fp64"): "kick_source_xfft_kernel<double, false, 512>",
+TRAFFIC_KERNEL = {("kick_source_drift", "fp32c"): "kick_source_xfft2_kernel<float, true, 512, 1>",
+                  ("kick_source_drift", "fp64"): "kick_source_xfft_kernel<double, false, 512, 1, false>",
                   ("gw_yfft", "fp32c"): "gw_ypass_p2_kernel<float, 512>",
                   ("gw_yfft", "fp64"): "gw_ypass_p2_kernel<double, 512>",
                   ("gw_zfft_accumulate", "fp32c"): "gw_zpass_p2_kernel<float, 512>",
@@ -94,9 +94,11 @@ def run_diagnostics(lat, t):
     d["gw_spectrum_ms"], _ = time_ms(lat.gw_spectrum)
     d["defects_ms"], cells = time_ms(lat.defects)
     d["strings_ms"], st = time_ms(lambda: lat.strings(1, 2), reps=1)
+    d["walls_ms"], wl = time_ms(lat.walls, reps=1)
     d["field_spectrum_ms"], _ = time_ms(lambda: lat.field_spectrum(t))
     d["energy_ms"], _ = time_ms(lambda: lat.energy(t))
     d["n_string_cells"] = int(cells[0])
+    d["n_wall_cells"] = int(cells[1])
     d["gw_spectra_per_s"] = 1e3 / d["gw_spectrum_ms"]
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
 
