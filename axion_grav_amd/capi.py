@@ -68,7 +68,7 @@ class StringOut(C.Structure):
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_init_thermal", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum", "axg_energy", "axg_profile")
+           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum", "axg_energy", "axg_tune", "axg_profile")
 
 _LIB = None
 
@@ -105,6 +105,7 @@ def lib():
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
+            "axg_tune": [p, i32, p, p, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -314,6 +315,15 @@ class Lattice:
         self._check(lib().axg_field_spectrum(self._h, tau, _ptr(q), _ptr(pw), _ptr(cv), _ptr(nm), C.byref(rho), nb),
                     "axg_field_spectrum")
         return dict(qmid=q, power=pw, conv=cv, num=nm, rho=rho.value)
+
+    TUNE_CONFIGS = ("built-in", "wide blocks", "other sites per thread", "other sites, wide blocks")
+
+    def tune(self, reps=5):
+        """Autotune the fused pass on this lattice: (chosen code, {config: ms per launch})."""
+        v = C.c_int32()
+        ms = np.zeros(len(self.TUNE_CONFIGS))
+        self._check(lib().axg_tune(self._h, reps, C.byref(v), _ptr(ms), len(ms)), "axg_tune")
+        return v.value, dict(zip(self.TUNE_CONFIGS, (float(x) for x in ms)))
 
     KERNEL_SLOTS = ("drift", "kick_source_drift", "gw_xfft", "gw_yfft", "gw_zfft_accumulate")
 

@@ -64,7 +64,7 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift);
+                             double dspa, double dt_drift, int variant);
 
 template <typename T>
 void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan,

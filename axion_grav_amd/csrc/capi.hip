@@ -117,6 +117,7 @@ struct axg_handle {
     void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
     int cur = 0;
     bool fuse_x = true;      // x transform fused into the stencil pass when N = 2^k
+    int kick_variant = 0;    // configuration of the fused pass (axg_tune; 0 = built-in choice)
     // diagnostics
     uint16_t* gw_bin = nullptr;
     uint16_t* spec_bin = nullptr;
@@ -380,6 +381,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         h->plan = make_plan(N);
         // tuning knob: AXG_XFUSE=0 runs the x transform as its own pass (same results)
         if (const char* e = getenv("AXG_XFUSE")) h->fuse_x = strcmp(e, "0") != 0;
+        // tuning knob AXG_KICK_VARIANT: the fused pass's initial configuration (axg_tune's codes)
+        if (const char* e = getenv("AXG_KICK_VARIANT")) h->kick_variant = atoi(e);
         AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int64_t n3 = h->n3;
         for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
@@ -579,7 +582,7 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
             h->timed(1, [&] {
                 fused = launch_kick_source_xfft<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
                                                          h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k,
-                                                         p.dspa, dtd);
+                                                         p.dspa, dtd, h->kick_variant);
             });
         }
         if (!fused) {
@@ -629,6 +632,82 @@ extern "C" int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* l
             if (ms) ms[i] = h->prof_ms[i];
             if (launches) launches[i] = h->prof_n[i];
         }
+    });
+}
+
+// Configurations of the fused kick / source / x-transform pass the tuner tries: the built-in
+// choice, wide blocks (twice the rows), the other sites-per-thread choice, and both.
+static const int kTuneVariants[] = {0, 1, 4, 5};
+
+extern "C" int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms)
+{
+    return guarded(h, [&] {
+        require(reps >= 1 && reps <= 1000, "reps must be 1..1000");
+        const int N = h->N;
+        const int nv = (int)(sizeof kTuneVariants / sizeof kTuneVariants[0]);
+        const bool tunable = h->p.grav_spec && h->fuse_x && (N & (N - 1)) == 0 && N >= 64;
+        if (!tunable) {  // one configuration only
+            if (variant) *variant = h->kick_variant;
+            for (int i = 0; i < nms; i++) ms[i] = 0;
+            return;
+        }
+        // pi is kicked in place by every trial launch: keep it in the work array meanwhile
+        const size_t pib = (size_t)h->n3 * h->t2;
+        AXG_HIP(hipMemcpyAsync(h->work, h->pi, pib, hipMemcpyDeviceToDevice, h->stream));
+        if (h->comp) AXG_HIP(hipMemcpyAsync((char*)h->work + pib, h->cpi, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice,
+                                            h->stream));
+        KickScalars k = kick_scalars(h->p, 1.0, 1e-3);
+        hipEvent_t e0, e1;
+        AXG_HIP(hipEventCreate(&e0));
+        AXG_HIP(hipEventCreate(&e1));
+        double best = 1e300;
+        int bestv = h->kick_variant;
+        const int nxt = 1 - h->cur;
+        try {
+            for (int iv = 0; iv < nv; iv++) {
+                const int v = kTuneVariants[iv];
+                auto once = [&] {
+                    if (h->comp)
+                        launch_kick_source_xfft<float, true>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
+                                                             h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw,
+                                                             k, h->p.dspa, 1e-3, v);
+                    else if (h->fp32)
+                        launch_kick_source_xfft<float, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
+                                                              nullptr, nullptr, nullptr, h->W, h->P, h->tw, k,
+                                                              h->p.dspa, 1e-3, v);
+                    else
+                        launch_kick_source_xfft<double, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
+                                                               nullptr, nullptr, nullptr, h->W, h->P, h->tw, k,
+                                                               h->p.dspa, 1e-3, v);
+                };
+                once();  // warm-up
+                AXG_HIP(hipEventRecord(e0, h->stream));
+                for (int r = 0; r < reps; r++) once();
+                AXG_HIP(hipEventRecord(e1, h->stream));
+                AXG_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                AXG_HIP(hipEventElapsedTime(&t, e0, e1));
+                const double avg = (double)t / reps;
+                if (iv < nms && ms) ms[iv] = avg;
+                if (avg < best) {
+                    best = avg;
+                    bestv = v;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        AXG_HIP(hipMemcpyAsync(h->pi, h->work, pib, hipMemcpyDeviceToDevice, h->stream));
+        if (h->comp) AXG_HIP(hipMemcpyAsync(h->cpi, (char*)h->work + pib, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice,
+                                            h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        h->kick_variant = bestv;
+        if (variant) *variant = bestv;
+        for (int i = nv; i < nms; i++) ms[i] = 0;
     });
 }
 

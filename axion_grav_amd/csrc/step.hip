@@ -405,17 +405,13 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift)
+                             double dspa, double dt_drift, int variant)
 {
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
-    // tuning knob AXG_KICK_VARIANT: bit 0 = 1024-thread blocks (1024/N rows), bit 1 = XCD
-    // remap, bit 2 = swap the sites-per-thread choice (same results; the block shape and
-    // order only)
-    static const int variant = [] {
-        const char* e = getenv("AXG_KICK_VARIANT");
-        return e ? atoi(e) : 0;
-    }();
+    // configuration (same results; the block shape and order only): bit 0 = wide blocks (twice
+    // the rows: 1024 threads), bit 1 = XCD remap (one site per thread only), bit 2 = swap the
+    // sites-per-thread choice
     auto launch = [&]<int NN, int XR, bool RM>() {
         hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
@@ -429,11 +425,15 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
             // (measured at 512^3: FP32C 2.95 vs 3.39 ms, FP64 4.97 vs 3.64 ms)
             const bool two = (sizeof(T) == 4) != ((variant & 4) != 0);
             if (two) {
-                constexpr int X2 = XR;
-                hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2>), dim3(NN / X2, NN), dim3(X2 * NN / 2),
-                                   (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
-                                   (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, (const T2*)tw, k,
-                                   1.0 / (dspa * 2), dt_drift);
+                auto launch2 = [&]<int X2>() {
+                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2>), dim3(NN / X2, NN),
+                                       dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
+                                       (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
+                                       (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift);
+                };
+                constexpr int X2W = 2 * XR <= NN ? 2 * XR : XR;
+                if (variant & 1) launch2.template operator()<X2W>();
+                else launch2.template operator()<XR>();
                 return;
             }
             switch (variant & 3) {
@@ -465,7 +465,8 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     template void launch_kick_drift<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*,    \
                                           void*, void*, const KickScalars&, double, double, bool);       \
     template bool launch_kick_source_xfft<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*, \
-                                                void*, void*, int, const void*, const KickScalars&, double, double);
+                                                void*, void*, int, const void*, const KickScalars&, double, double, \
+                                                int);
 AXG_INST_STEP(double, false)
 AXG_INST_STEP(float, false)
 AXG_INST_STEP(float, true)

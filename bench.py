@@ -49,17 +49,29 @@ MODEL_REALS = 40
 # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (calibrated against a copy of known
 # size) for the 512^3 workload; the profile summary holds the commands and the calibration.
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "lattice_step_512_hbm_traffic.md")
-TRAFFIC_KERNEL = {("kick_source_drift", "fp32c"): "kick_source_xfft2_kernel<float, true, 512, 1>",
-                  ("kick_source_drift", "fp64"): "kick_source_xfft_kernel<double, false, 512, 1, false>",
-                  ("gw_yfft", "fp32c"): "gw_ypass_p2_kernel<float, 512>",
+# the fused pass's kernel for each tuner choice (0 built-in, 1 wide blocks, 4 / 5 the other
+# sites-per-thread choice), FP32C and FP64 at 512^3
+FUSED_KERNEL = {("fp32c", 0): "kick_source_xfft2_kernel<float, true, 512, 1>",
+                ("fp32c", 1): "kick_source_xfft2_kernel<float, true, 512, 2>",
+                ("fp32c", 4): "kick_source_xfft_kernel<float, true, 512, 1, false>",
+                ("fp32c", 5): "kick_source_xfft_kernel<float, true, 512, 2, false>",
+                ("fp64", 0): "kick_source_xfft_kernel<double, false, 512, 1, false>",
+                ("fp64", 1): "kick_source_xfft_kernel<double, false, 512, 2, false>",
+                ("fp64", 4): "kick_source_xfft2_kernel<double, false, 512, 1>",
+                ("fp64", 5): "kick_source_xfft2_kernel<double, false, 512, 2>"}
+TRAFFIC_KERNEL = {("gw_yfft", "fp32c"): "gw_ypass_p2_kernel<float, 512>",
                   ("gw_yfft", "fp64"): "gw_ypass_p2_kernel<double, 512>",
                   ("gw_zfft_accumulate", "fp32c"): "gw_zpass_p2_kernel<float, 512>",
                   ("gw_zfft_accumulate", "fp64"): "gw_zpass_p2_kernel<double, 512>"}
 
 
-def measured_traffic(slot, prec, N):
-    """Counter-measured HBM bytes per launch of a step kernel at 512^3, or None."""
-    name = TRAFFIC_KERNEL.get((slot, prec))
+def measured_traffic(slot, prec, N, choice=0):
+    """Counter-measured HBM bytes per launch of a step kernel at 512^3 (the fused pass in the
+    tuner's configuration `choice`), or None when the profile does not hold that kernel."""
+    if slot == "kick_source_drift":
+        name = FUSED_KERNEL.get((prec, choice))
+    else:
+        name = TRAFFIC_KERNEL.get((slot, prec))
     if N != 512 or name is None or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
@@ -103,12 +115,13 @@ def run_diagnostics(lat, t):
     return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
 
 
-def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False):
+def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False, tune=True):
     P = params(capi, N, L, precision, 128)
     real = 8 if precision == capi.FP64 else 4
     dt = 0.2 * (L / N)
     with capi.Lattice(P) as lat:
         lat.init_flat(seed, 1.0)
+        tuned = lat.tune(5) if tune else None  # picks the fused pass's configuration; results unchanged
         t = 1.0
         if warmup > 0:
             lat.advance_steps(t, dt, warmup)
@@ -130,7 +143,9 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False):
         alg = KERNEL_REALS[k] * real * n3
         kern[k] = dict(ms_avg=avg, launches=cnt, alg_bytes=alg, gbs=alg / (avg * 1e-3) / 1e9)
     return dict(su_s=steps * n3 / el, ms_per_step=el / steps * 1e3, kernels=kern, real=real,
-                finite=bool(np.all(np.isfinite(spec["numer"]))), diagnostics=dg)
+                finite=bool(np.all(np.isfinite(spec["numer"]))), diagnostics=dg,
+                tuned=None if tuned is None else {"choice": tuned[0],
+                                                  "ms": {k: round(v, 4) for k, v in tuned[1].items()}})
 
 
 def copy_probe(nbytes=4 << 30, reps=10):
@@ -179,6 +194,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-diag", action="store_true", help="skip timing the diagnostics after the FP32C run")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in configuration of the fused pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
                          "its kernel is the known-size reference for the FETCH/WRITE counters)")
@@ -193,7 +209,9 @@ def main():
     copy_gbs = copy_probe() if args.copy_probe else None
     N, L = args.n, args.box
     log(f"FP32C {N}^3 L={L}: warmup {args.warmup}, timed {args.steps} steps")
-    c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed, diag=not args.no_diag)
+    c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed, diag=not args.no_diag,
+                tune=not args.no_tune)
+    log(f"FP32C tuning: {c['tuned']}")
     if c["diagnostics"]:
         log("FP32C diagnostics: " + ", ".join(f"{k} {v}" for k, v in c["diagnostics"].items()))
     log(f"FP32C {c['su_s']:.4g} site-updates/s, {c['ms_per_step']:.3f} ms/step; kernels "
@@ -201,12 +219,12 @@ def main():
     d = None
     if not args.no_fp64:
         log(f"FP64 {N}^3: warmup {args.warmup}, timed {args.steps} steps")
-        d = run_gpu(capi, N, L, capi.FP64, args.steps, args.warmup, args.seed)
+        d = run_gpu(capi, N, L, capi.FP64, args.steps, args.warmup, args.seed, tune=not args.no_tune)
         log(f"FP64 {d['su_s']:.4g} site-updates/s, {d['ms_per_step']:.3f} ms/step; kernels "
             + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in d["kernels"].items()))
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
-    traffic = measured_traffic(dom_name, "fp32c", N)
+    traffic = measured_traffic(dom_name, "fp32c", N, c["tuned"]["choice"] if c["tuned"] else 0)
     roof = dict(bound="hbm", kernel=dom_name, achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                 alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4),
@@ -246,6 +264,7 @@ def main():
         "kernels": {k: {"ms": round(v["ms_avg"], 4), "GB/s": round(v["gbs"], 1)} for k, v in c["kernels"].items()},
         "hbm_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "diagnostics": c["diagnostics"],
+        "tuning": {"fp32c": c["tuned"], "fp64": None if d is None else d["tuned"]},
     }
     print(json.dumps(out), flush=True)
 

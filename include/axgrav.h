@@ -198,6 +198,16 @@ int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, d
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);
 
+/* Autotuner (no counterpart in the reference).  Times each configuration of the fused
+ * kick / GW source / x-transform pass (block shape, sites per thread, block order) on this
+ * handle's lattice, reps launches each, and keeps the fastest for the handle's later steps.
+ * Every configuration gives bit-identical results; the fields are left as they were (pi is
+ * saved and restored; the other buffers it writes are rewritten by every step).  *variant =
+ * the chosen configuration code; ms[i] (i < nms, may be NULL) = average ms of configuration i
+ * in the order built-in, wide blocks (twice the rows), other sites per thread, both (0
+ * when the lattice has a single configuration: grav_spec off or N not a power of two >= 64). */
+int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms);
+
 /* Kernel timing (no counterpart in the reference; for measurement).  enable = 1 resets the
  * counters and starts timing every step kernel with HIP events on the handle's stream;
  * enable = 0 stops.  ms[slot] / launches[slot] receive the totals so far for the slots

@@ -117,6 +117,32 @@ def test_time_reversal_residual(prec, tol):
     assert np.abs(b_pi - pi).max() < tol * np.abs(phi).max(), np.abs(b_pi - pi).max()
 
 
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_tune_keeps_fields_and_results(prec):
+    """axg_tune times the fused pass's configurations on the live lattice: the fields are the
+    same before and after, and the steps after tuning match untuned steps bit for bit."""
+    say("tune", prec)
+    N = 64
+    prm, P = params(N, precision=prec)
+    phi, pi = ic(N, 9, 4.0)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        a_phi, a_pi = L.get_fields()
+        v, ms = L.tune(3)
+        b_phi, b_pi = L.get_fields()
+        assert np.array_equal(a_phi, b_phi) and np.array_equal(a_pi, b_pi)
+        assert v in (0, 1, 4, 5) and all(t > 0 for t in ms.values())
+        L.advance_steps(1.0, 0.05, 3)
+        t_phi, t_pi = L.get_fields()
+        t_A = L.gw_state()
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.advance_steps(1.0, 0.05, 3)
+        u_phi, u_pi = L.get_fields()
+        u_A = L.gw_state()
+    assert np.array_equal(t_phi, u_phi) and np.array_equal(t_pi, u_pi) and np.array_equal(t_A, u_A)
+
+
 # ---------------------------------------------------------------- GW accumulation
 
 @pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False),
