@@ -316,14 +316,16 @@ class Lattice:
                     "axg_field_spectrum")
         return dict(qmid=q, power=pw, conv=cv, num=nm, rho=rho.value)
 
-    TUNE_CONFIGS = ("built-in", "wide blocks", "other sites per thread", "other sites, wide blocks")
+    TUNE_CONFIGS = ("fused: built-in", "fused: wide blocks", "fused: other sites per thread",
+                    "fused: other sites, wide blocks", "y: 128-byte tiles", "y: 64-byte tiles")
 
     def tune(self, reps=5):
-        """Autotune the fused pass on this lattice: (chosen code, {config: ms per launch})."""
+        """Autotune the fused pass and the y transform on this lattice:
+        (fused-pass code, y-pass code, {config: ms per launch})."""
         v = C.c_int32()
         ms = np.zeros(len(self.TUNE_CONFIGS))
         self._check(lib().axg_tune(self._h, reps, C.byref(v), _ptr(ms), len(ms)), "axg_tune")
-        return v.value, dict(zip(self.TUNE_CONFIGS, (float(x) for x in ms)))
+        return v.value & 0xFF, v.value >> 8, dict(zip(self.TUNE_CONFIGS, (float(x) for x in ms)))
 
     KERNEL_SLOTS = ("drift", "kick_source_drift", "gw_xfft", "gw_yfft", "gw_zfft_accumulate")
 

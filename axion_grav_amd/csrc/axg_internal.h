@@ -119,7 +119,7 @@ struct GwStepScalars {
 // into W2 in the tiled layout.
 template <typename T>
 void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
-                     const void* tw);
+                     const void* tw, int variant = 0);
 
 template <typename T>
 void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,

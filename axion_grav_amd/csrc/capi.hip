@@ -118,6 +118,7 @@ struct axg_handle {
     int cur = 0;
     bool fuse_x = true;      // x transform fused into the stencil pass when N = 2^k
     int kick_variant = 0;    // configuration of the fused pass (axg_tune; 0 = built-in choice)
+    int ypass_variant = 0;   // configuration of the y pass (axg_tune; 0 = 128-byte column tiles)
     // diagnostics
     uint16_t* gw_bin = nullptr;
     uint16_t* spec_bin = nullptr;
@@ -381,8 +382,9 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         h->plan = make_plan(N);
         // tuning knob: AXG_XFUSE=0 runs the x transform as its own pass (same results)
         if (const char* e = getenv("AXG_XFUSE")) h->fuse_x = strcmp(e, "0") != 0;
-        // tuning knob AXG_KICK_VARIANT: the fused pass's initial configuration (axg_tune's codes)
+        // tuning knobs AXG_KICK_VARIANT / AXG_YPASS_VARIANT: initial configurations (axg_tune's codes)
         if (const char* e = getenv("AXG_KICK_VARIANT")) h->kick_variant = atoi(e);
+        if (const char* e = getenv("AXG_YPASS_VARIANT")) h->ypass_variant = atoi(e);
         AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int64_t n3 = h->n3;
         for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
@@ -593,7 +595,9 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
         }
         if (p.grav_spec) {
             if (!fused) h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
-            h->timed(3, [&] { launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw); });
+            h->timed(3, [&] {
+                launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, h->ypass_variant);
+            });
             GwStepScalars g;
             g.tau = t;
             g.dw = t * dt * p.dv_spa * (p.sigma_cr * p.sigma_cr);  // grav_waves.f90:209
@@ -639,75 +643,99 @@ extern "C" int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* l
 // choice, wide blocks (twice the rows), the other sites-per-thread choice, and both.
 static const int kTuneVariants[] = {0, 1, 4, 5};
 
+// A start/stop event pair, destroyed with the scope.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair()
+    {
+        AXG_HIP(hipEventCreate(&a));
+        AXG_HIP(hipEventCreate(&b));
+    }
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+};
+
 extern "C" int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms)
 {
     return guarded(h, [&] {
         require(reps >= 1 && reps <= 1000, "reps must be 1..1000");
         const int N = h->N;
         const int nv = (int)(sizeof kTuneVariants / sizeof kTuneVariants[0]);
+        for (int i = 0; ms && i < nms; i++) ms[i] = 0;
         const bool tunable = h->p.grav_spec && h->fuse_x && (N & (N - 1)) == 0 && N >= 64;
         if (!tunable) {  // one configuration only
-            if (variant) *variant = h->kick_variant;
-            for (int i = 0; i < nms; i++) ms[i] = 0;
+            if (variant) *variant = h->kick_variant | (h->ypass_variant << 8);
             return;
         }
-        // pi is kicked in place by every trial launch: keep it in the work array meanwhile
+        EventPair ev;
+        // mean ms per launch of once() over reps launches after one warm-up launch
+        auto time_avg = [&](auto&& once) {
+            once();
+            AXG_HIP(hipEventRecord(ev.a, h->stream));
+            for (int r = 0; r < reps; r++) once();
+            AXG_HIP(hipEventRecord(ev.b, h->stream));
+            AXG_HIP(hipEventSynchronize(ev.b));
+            float t = 0;
+            AXG_HIP(hipEventElapsedTime(&t, ev.a, ev.b));
+            return (double)t / reps;
+        };
+        // fused pass: pi is kicked in place by every trial launch, so it waits in the work array
         const size_t pib = (size_t)h->n3 * h->t2;
         AXG_HIP(hipMemcpyAsync(h->work, h->pi, pib, hipMemcpyDeviceToDevice, h->stream));
-        if (h->comp) AXG_HIP(hipMemcpyAsync((char*)h->work + pib, h->cpi, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice,
-                                            h->stream));
-        KickScalars k = kick_scalars(h->p, 1.0, 1e-3);
-        hipEvent_t e0, e1;
-        AXG_HIP(hipEventCreate(&e0));
-        AXG_HIP(hipEventCreate(&e1));
+        if (h->comp)
+            AXG_HIP(hipMemcpyAsync((char*)h->work + pib, h->cpi, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice, h->stream));
+        const KickScalars k = kick_scalars(h->p, 1.0, 1e-3);
+        const int nxt = 1 - h->cur;
         double best = 1e300;
         int bestv = h->kick_variant;
-        const int nxt = 1 - h->cur;
-        try {
-            for (int iv = 0; iv < nv; iv++) {
-                const int v = kTuneVariants[iv];
-                auto once = [&] {
-                    if (h->comp)
-                        launch_kick_source_xfft<float, true>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
-                                                             h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw,
-                                                             k, h->p.dspa, 1e-3, v);
-                    else if (h->fp32)
-                        launch_kick_source_xfft<float, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
-                                                              nullptr, nullptr, nullptr, h->W, h->P, h->tw, k,
-                                                              h->p.dspa, 1e-3, v);
-                    else
-                        launch_kick_source_xfft<double, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
-                                                               nullptr, nullptr, nullptr, h->W, h->P, h->tw, k,
-                                                               h->p.dspa, 1e-3, v);
-                };
-                once();  // warm-up
-                AXG_HIP(hipEventRecord(e0, h->stream));
-                for (int r = 0; r < reps; r++) once();
-                AXG_HIP(hipEventRecord(e1, h->stream));
-                AXG_HIP(hipEventSynchronize(e1));
-                float t = 0;
-                AXG_HIP(hipEventElapsedTime(&t, e0, e1));
-                const double avg = (double)t / reps;
-                if (iv < nms && ms) ms[iv] = avg;
-                if (avg < best) {
-                    best = avg;
-                    bestv = v;
+        for (int iv = 0; iv < nv; iv++) {
+            const int v = kTuneVariants[iv];
+            const double avg = time_avg([&] {
+                if (h->comp)
+                    launch_kick_source_xfft<float, true>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
+                                                         h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k,
+                                                         h->p.dspa, 1e-3, v);
+                else if (h->fp32)
+                    launch_kick_source_xfft<float, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, nullptr,
+                                                          nullptr, nullptr, h->W, h->P, h->tw, k, h->p.dspa, 1e-3, v);
+                else
+                    launch_kick_source_xfft<double, false>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, nullptr,
+                                                           nullptr, nullptr, h->W, h->P, h->tw, k, h->p.dspa, 1e-3, v);
+            });
+            if (ms && iv < nms) ms[iv] = avg;
+            if (avg < best) {
+                best = avg;
+                bestv = v;
+            }
+        }
+        AXG_HIP(hipMemcpyAsync(h->pi, h->work, pib, hipMemcpyDeviceToDevice, h->stream));
+        if (h->comp)
+            AXG_HIP(hipMemcpyAsync(h->cpi, (char*)h->work + pib, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice, h->stream));
+        // y pass: reads W and writes W2, both rewritten by every step
+        double ybest = 1e300;
+        int ybestv = h->ypass_variant;
+        if (h->L.zt) {
+            for (int yv = 0; yv < 2; yv++) {
+                const double avg = time_avg([&] {
+                    if (h->fp32) launch_gw_ypass<float>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, yv);
+                    else launch_gw_ypass<double>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, yv);
+                });
+                if (ms && nv + yv < nms) ms[nv + yv] = avg;
+                if (avg < ybest) {
+                    ybest = avg;
+                    ybestv = yv;
                 }
             }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        AXG_HIP(hipMemcpyAsync(h->pi, h->work, pib, hipMemcpyDeviceToDevice, h->stream));
-        if (h->comp) AXG_HIP(hipMemcpyAsync(h->cpi, (char*)h->work + pib, (size_t)h->n3 * 4, hipMemcpyDeviceToDevice,
-                                            h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
         h->kick_variant = bestv;
-        if (variant) *variant = bestv;
-        for (int i = nv; i < nms; i++) ms[i] = 0;
+        h->ypass_variant = ybestv;
+        if (variant) *variant = bestv | (ybestv << 8);
     });
 }
 

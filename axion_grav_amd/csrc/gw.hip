@@ -105,13 +105,13 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
 
 // y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled
 // layout of the accumulation pass.  Block = TX kx columns of one (c, z) plane.
-template <typename T, int N>
-__global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
-                                                                             typename V2<T>::type* __restrict__ W2,
-                                                                             const typename V2<T>::type* __restrict__ tw)
+template <typename T, int N, int TX>
+__global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
+                                                                    typename V2<T>::type* __restrict__ W2,
+                                                                    const typename V2<T>::type* __restrict__ tw)
 {
     using T2 = typename V2<T>::type;
-    constexpr int TX = p2_tx<T, N>(), TPT = N / 8, NH = N / 2 + 1, P = half_pitch_c<N>();
+    constexpr int TPT = N / 8, NH = N / 2 + 1, P = half_pitch_c<N>();
     using L = TiledLayout<N, p2_ztx<T, N>()>;
     extern __shared__ unsigned char smem_raw[];
     T2* s = reinterpret_cast<T2*>(smem_raw);
@@ -575,17 +575,22 @@ GwLayout gw_layout(int N, bool fp32)
 
 template <typename T>
 void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
-                     const void* tw)
+                     const void* tw, int variant)
 {
     using T2 = typename V2<T>::type;
     if (L.zt) {
         bool done = p2_dispatch(N, [&]<int NN>() {
-            constexpr int TXP = p2_tx<T, NN>();
             if (!TiledLayout<NN, p2_ztx<T, NN>()>::matches(L) || pitch != half_pitch_c<NN>())
                 throw Error(AXG_EINVAL, "y pass: layout does not match the compiled tile");
-            hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN>), dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN),
-                               dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2,
-                               (const T2*)tw);
+            // variant 1: half-width column tiles (more, smaller blocks per CU)
+            auto go = [&]<int TXP>() {
+                hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP>), dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN),
+                                   dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2,
+                                   (const T2*)tw);
+            };
+            constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1;
+            if (variant == 1) go.template operator()<TXH>();
+            else go.template operator()<TXP>();
         });
         if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k");
         AXG_HIP(hipGetLastError());
@@ -731,7 +736,7 @@ void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, dou
                                          const void*);                                                          \
     template void launch_fft_rows<T, +1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);        \
     template void launch_gw_ypass<T>(hipStream_t, int, int, void*, void*, const GwLayout&, const FftPlan&,       \
-                                     const void*);                                                              \
+                                     const void*, int);                                                              \
     template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const GwLayout&,             \
                                          const FftPlan&, const void*, const GwStepScalars&);                    \
     template void launch_gw_bin<T>(hipStream_t, int, const GwLayout&, const void*, const uint16_t*, int, double*, \

@@ -70,6 +70,8 @@ def measured_traffic(slot, prec, N, choice=0):
     tuner's configuration `choice`), or None when the profile does not hold that kernel."""
     if slot == "kick_source_drift":
         name = FUSED_KERNEL.get((prec, choice))
+    elif slot == "gw_yfft" and choice != 0:
+        name = None  # the half-width y pass is not in the profile
     else:
         name = TRAFFIC_KERNEL.get((slot, prec))
     if N != 512 or name is None or not os.path.exists(TRAFFIC_FILE):
@@ -144,8 +146,8 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False, tune=True):
         kern[k] = dict(ms_avg=avg, launches=cnt, alg_bytes=alg, gbs=alg / (avg * 1e-3) / 1e9)
     return dict(su_s=steps * n3 / el, ms_per_step=el / steps * 1e3, kernels=kern, real=real,
                 finite=bool(np.all(np.isfinite(spec["numer"]))), diagnostics=dg,
-                tuned=None if tuned is None else {"choice": tuned[0],
-                                                  "ms": {k: round(v, 4) for k, v in tuned[1].items()}})
+                tuned=None if tuned is None else {"choice": tuned[0], "ypass": tuned[1],
+                                                  "ms": {k: round(v, 4) for k, v in tuned[2].items()}})
 
 
 def copy_probe(nbytes=4 << 30, reps=10):
@@ -224,7 +226,8 @@ def main():
             + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in d["kernels"].items()))
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
-    traffic = measured_traffic(dom_name, "fp32c", N, c["tuned"]["choice"] if c["tuned"] else 0)
+    tc = c["tuned"] or {"choice": 0, "ypass": 0}
+    traffic = measured_traffic(dom_name, "fp32c", N, tc["choice"] if dom_name == "kick_source_drift" else tc["ypass"])
     roof = dict(bound="hbm", kernel=dom_name, achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                 alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4),

@@ -199,13 +199,14 @@ int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, d
 int axg_energy(axg_handle* h, double tau, double out[4]);
 
 /* Autotuner (no counterpart in the reference).  Times each configuration of the fused
- * kick / GW source / x-transform pass (block shape, sites per thread, block order) on this
- * handle's lattice, reps launches each, and keeps the fastest for the handle's later steps.
- * Every configuration gives bit-identical results; the fields are left as they were (pi is
- * saved and restored; the other buffers it writes are rewritten by every step).  *variant =
- * the chosen configuration code; ms[i] (i < nms, may be NULL) = average ms of configuration i
- * in the order built-in, wide blocks (twice the rows), other sites per thread, both (0
- * when the lattice has a single configuration: grav_spec off or N not a power of two >= 64). */
+ * kick / GW source / x-transform pass (block rows, sites per thread) and of the y transform
+ * (column-tile width) on this handle's lattice, reps launches each, and keeps the fastest for
+ * the handle's later steps.  Every configuration gives bit-identical results; the fields are
+ * left as they were (pi is saved and restored; the other buffers written are rewritten by
+ * every step).  *variant = fused-pass code | (y-pass code << 8); ms[i] (i < nms, may be NULL)
+ * = average ms per launch of: fused pass built-in, wide blocks (twice the rows), other sites
+ * per thread, both; y pass 128-byte tiles, 64-byte tiles (all 0 when the lattice has a single
+ * configuration: grav_spec off or N not a power of two >= 64). */
 int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms);
 
 /* Kernel timing (no counterpart in the reference; for measurement).  enable = 1 resets the
