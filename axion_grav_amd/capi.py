@@ -68,7 +68,8 @@ class StringOut(C.Structure):
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_init_thermal", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum", "axg_energy", "axg_tune", "axg_profile")
+           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum",
+           "axg_field_covariance", "axg_energy", "axg_tune", "axg_profile")
 
 _LIB = None
 
@@ -103,6 +104,7 @@ def lib():
             "axg_walls": [p, C.POINTER(WallOut)],
             "axg_get_walls": [p, p, i64, p],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
+            "axg_field_covariance": [p, C.c_uint64, i32, i32, d, p, i64, p],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
             "axg_tune": [p, i32, p, p, i32],
@@ -315,6 +317,16 @@ class Lattice:
         self._check(lib().axg_field_spectrum(self._h, tau, _ptr(q), _ptr(pw), _ptr(cv), _ptr(nm), C.byref(rho), nb),
                     "axg_field_spectrum")
         return dict(qmid=q, power=pw, conv=cv, num=nm, rho=rho.value)
+
+    def field_covariance(self, seed, min_mc=10, max_mc=100, tol=0.1):
+        """Monte Carlo covariance of the last field_spectrum: (packed-upper matrix, realisations;
+        negative: -(max_mc + 1) without convergence)."""
+        nb = self.params.num_bin
+        cov = np.zeros(nb * (nb + 1) // 2)
+        it = C.c_int32()
+        self._check(lib().axg_field_covariance(self._h, seed, min_mc, max_mc, tol, _ptr(cov), len(cov), C.byref(it)),
+                    "axg_field_covariance")
+        return cov, it.value
 
     TUNE_CONFIGS = ("fused: built-in", "fused: wide blocks", "fused: other sites per thread",
                     "fused: other sites, wide blocks", "y: 128-byte tiles", "y: 64-byte tiles")

@@ -248,3 +248,22 @@ __device__ __forceinline__ void philox4x32(uint32_t& c0, uint32_t& c1, uint32_t&
         k1 += W1;
     }
 }
+
+// Unit complex Gaussian (each part unit variance) number `lin` of the stream (seed, tag, word):
+// Philox4x32-10 on the counter (lin, tag, word) keyed by the seed, then Box-Muller on two
+// 53-bit uniforms in (0,1].
+__device__ __forceinline__ void unit_gauss(uint64_t seed, uint32_t tag, uint32_t word, uint64_t lin, double& re,
+                                           double& im)
+{
+#pragma clang fp contract(off)
+    uint32_t c0 = (uint32_t)lin, c1 = (uint32_t)(lin >> 32), c2 = tag, c3 = word;
+    philox4x32(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t a = (((uint64_t)c0 << 21) ^ (uint64_t)(c1 >> 11)) & ((1ull << 53) - 1);
+    const uint64_t b = (((uint64_t)c2 << 21) ^ (uint64_t)(c3 >> 11)) & ((1ull << 53) - 1);
+    const double u1 = ((double)a + 1.0) / 9007199254740992.0;
+    const double u2 = ((double)b + 1.0) / 9007199254740992.0;
+    const double r = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586 * u2;
+    re = r * cos(th);
+    im = r * sin(th);
+}

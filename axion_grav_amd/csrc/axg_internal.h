@@ -153,6 +153,9 @@ constexpr int WALL_NT = 64;
 template <typename T, bool COMP>
 void launch_wall_fit(hipStream_t s, int N, const void* phi, const void* cphi, const int32_t* cells, int64_t nw,
                      double* rec, double* partial);
+void launch_cov_modes(hipStream_t s, int N, uint64_t seed, int iter, const uint16_t* bin_of_n2, const double* sigma,
+                      double2* out);
+void launch_mask_zero(hipStream_t s, int64_t n3, const int8_t* flags, double2* data);
 void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
                           double dv_wav, double thrm_sigma, double2* out);
 

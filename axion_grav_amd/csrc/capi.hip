@@ -146,6 +146,10 @@ struct axg_handle {
     int64_t nwall_rec = -1;
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
+    // the last field spectrum: deconvolved power and the count-normalised deconvolution matrix
+    // (estimate_covmat's inputs); valid while the fields and flags are unchanged
+    std::vector<double> spec_power, spec_mat;
+    bool have_spec = false;
     double* d_largeq = nullptr;
     double* d_matspec = nullptr;
     double* d_dqs = nullptr;
@@ -501,6 +505,7 @@ extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi
             store_from_work(h, h->pi, h->cpi);
         }
         h->have_flags = false;
+        h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
         AXG_HIP(hipStreamSynchronize(h->stream));
@@ -617,6 +622,7 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     else if (h->fp32) run_steps<float, false>(h, t1, dt, nt);
     else run_steps<double, false>(h, t1, dt, nt);
     h->have_flags = false;
+    h->have_spec = false;
     h->nstr_rec = -1;
     h->nwall_rec = -1;
     AXG_HIP(hipStreamSynchronize(h->stream));
@@ -1193,6 +1199,89 @@ extern "C" int axg_field_spectrum(axg_handle* h, double tau, double* qmid, doubl
             if (num) num[i] = nm[i];
         }
         if (rho) *rho = rh;
+        h->spec_power = pw;
+        h->spec_mat = mat;
+        h->have_spec = true;
+    });
+}
+
+// estimate_covmat (spectrum.f90:569-734): Monte Carlo covariance of the deconvolved spectrum of
+// the last axg_field_spectrum.  Each realisation: Gaussian modes with amplitude
+// sqrt(power/2)/N^3 per bin, FORWARD transform, mask cells zeroed, BACKWARD transform, binned
+// |.|^2, deconvolved, minus the power; outer products summed, with leave-one-out sums for the
+// jackknife test on the diagonal after min_mc realisations (stop when below tol).
+extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc, int32_t max_mc, double tol,
+                                    double* cov, int64_t ncov_out, int32_t* iters)
+{
+    return guarded(h, [&] {
+        const int nb = h->p.num_bin;
+        const int ncov = nb * (nb + 1) / 2;
+        require(h->have_spec && h->have_flags, "call axg_field_spectrum first (after the last change of the fields)");
+        require(cov != nullptr && ncov_out == ncov, "cov needs num_bin*(num_bin+1)/2 entries");
+        require(min_mc >= 2 && max_mc >= min_mc && max_mc <= 100000, "need 2 <= min_mc <= max_mc");
+        const int N = h->N;
+        const double ntot = (double)h->n3;
+        std::vector<double> sig(nb);
+        for (int i = 0; i < nb; i++) sig[i] = sqrt(h->spec_power[i] / 2) / ntot;
+        double* d_sig = h->red;
+        const int nwg = 1024;
+        double* out = h->red + nb + (size_t)nwg * 2 * nb;
+        double* part = h->red + nb;
+        require((size_t)(nb + (size_t)(nwg + 1) * 2 * nb) * 8 <= h->red_bytes, "reduction scratch too small");
+        AXG_HIP(hipMemcpyAsync(d_sig, sig.data(), nb * 8, hipMemcpyHostToDevice, h->stream));
+        std::vector<double> pc(ncov, 0.0), tmp((size_t)ncov * max_mc, 0.0), r(2 * nb), pw(nb);
+        const std::vector<double>& mat = h->spec_mat;
+        int iter = 1;
+        bool converged = false;
+        for (; iter <= max_mc; iter++) {
+            launch_cov_modes(h->stream, N, seed, iter, h->spec_bin, d_sig, h->work);
+            fft3d_work(h, h->work, -1);
+            launch_mask_zero(h->stream, h->n3, h->flags, h->work);
+            fft3d_work(h, h->work, +1);
+            launch_spec_bin(h->stream, N, h->work, h->spec_bin, nb, 1.0, part, nwg, out);
+            AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * 8, hipMemcpyDeviceToHost, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            // pow = mat_deconv pow0 - power (dspmv with beta = 1, spectrum.f90:675-677)
+            for (int i = 1; i <= nb; i++) {
+                double acc = -h->spec_power[i - 1];
+                for (int j = 1; j <= nb; j++) {
+                    int a = std::min(i, j), b = std::max(i, j);
+                    acc += mat[(size_t)a + (size_t)b * (b - 1) / 2 - 1] * r[j - 1];
+                }
+                pw[i - 1] = acc;
+            }
+            for (int i = 1; i <= nb; i++)
+                for (int j = 1; j <= i; j++) {
+                    const int ij = i * (i - 1) / 2 + j - 1;
+                    const double u = pw[i - 1] * pw[j - 1];
+                    tmp[(size_t)ij * max_mc + (iter - 1)] = pc[ij];
+                    for (int k = 0; k < iter - 1; k++) tmp[(size_t)ij * max_mc + k] += u;
+                    pc[ij] += u;
+                }
+            if (iter < min_mc) continue;
+            double diff = 0;
+            for (int i = 1; i <= nb; i++) {
+                const int ij = i * (i + 1) / 2 - 1;
+                double u = 0, v = 0;
+                for (int k = 0; k < iter; k++) {
+                    double x = tmp[(size_t)ij * max_mc + k] / (iter - 1);
+                    u = u + x;
+                    v = v + x * x;
+                }
+                u = u / iter;
+                v = v / iter;
+                v = sqrt(v - u * u);
+                double x = pc[ij] / iter;
+                diff = std::max(diff, v / x);
+            }
+            if (diff < tol) {
+                converged = true;
+                break;
+            }
+        }
+        // without convergence the loop counter ends at max_mc + 1, which the reference divides by
+        for (int ij = 0; ij < ncov; ij++) cov[ij] = pc[ij] / iter;
+        if (iters) *iters = converged ? iter : -iter;
     });
 }
 
@@ -1227,6 +1316,7 @@ extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
         }
         AXG_HIP(hipStreamSynchronize(h->stream));
         h->have_flags = false;
+        h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
     });
@@ -1298,6 +1388,7 @@ extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
         (void)hipFree(pidx);
         (void)hipFree(pval);
         h->have_flags = false;
+        h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
     });

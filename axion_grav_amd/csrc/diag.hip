@@ -539,15 +539,8 @@ __global__ void thermal_modes_kernel(int N, uint64_t seed, int ip, double tau_in
          lin += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
         const int64_t nx = x < N / 2 ? x : x - N, ny = y < N / 2 ? y : y - N, nz = z < N / 2 ? z : z - N;
-        uint32_t c0 = (uint32_t)lin, c1 = (uint32_t)((uint64_t)lin >> 32), c2 = 0x74686d6cu, c3 = (uint32_t)ip;
-        philox4x32(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
-        uint64_t a = (((uint64_t)c0 << 21) ^ (uint64_t)(c1 >> 11)) & ((1ull << 53) - 1);
-        uint64_t b = (((uint64_t)c2 << 21) ^ (uint64_t)(c3 >> 11)) & ((1ull << 53) - 1);
-        double u1 = ((double)a + 1.0) / 9007199254740992.0;
-        double u2 = ((double)b + 1.0) / 9007199254740992.0;
-        double r = sqrt(-2.0 * log(u1));
-        double th = 6.283185307179586 * u2;
-        double g1 = r * cos(th), g2 = r * sin(th);
+        double g1, g2;
+        unit_gauss(seed, 0x74686d6cu, (uint32_t)ip, (uint64_t)lin, g1, g2);
         // initial_phi_phidot
         double q = (double)(nx * nx + ny * ny + nz * nz);
         q = sqrt(q) * dwav;
@@ -569,6 +562,51 @@ void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double ta
 {
     hipLaunchKernelGGL(thermal_modes_kernel, dim3(4096), dim3(256), 0, s, N, seed, ip, tau_ini, sigma_cr, dwav, dv_wav,
                        thrm_sigma, out);
+    AXG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ spectrum covariance (MC)
+// One Monte Carlo realisation of estimate_covmat (spectrum.f90:592-641): mode k gets a unit
+// complex Gaussian (Philox4x32-10 keyed by the seed, counter = (mode index, 'cov ', iter)) times
+// sigma[b] of its spectrum bin b, or 0 outside the bins.
+__global__ void cov_modes_kernel(int N, uint64_t seed, int iter, const uint16_t* __restrict__ bin_of_n2,
+                                 const double* __restrict__ sigma, double2* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
+        const int64_t nx = x < N / 2 ? x : x - N, ny = y < N / 2 ? y : y - N, nz = z < N / 2 ? z : z - N;
+        const int b = bin_of_n2[nx * nx + ny * ny + nz * nz];
+        if (b == 0) {
+            out[lin] = double2{0.0, 0.0};
+            continue;
+        }
+        double g1, g2;
+        unit_gauss(seed, 0x636f7620u, (uint32_t)iter, (uint64_t)lin, g1, g2);
+        const double sg = sigma[b - 1];
+        out[lin] = double2{g1 * sg, g2 * sg};
+    }
+}
+
+// masking (spectrum.f90:900-954): the mask cells of the real-space field set to zero.
+__global__ void mask_zero_kernel(int64_t n3, const int8_t* __restrict__ flags, double2* __restrict__ data)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i] & 4) data[i] = double2{0.0, 0.0};
+}
+
+void launch_cov_modes(hipStream_t s, int N, uint64_t seed, int iter, const uint16_t* bin_of_n2, const double* sigma,
+                      double2* out)
+{
+    hipLaunchKernelGGL(cov_modes_kernel, dim3(4096), dim3(256), 0, s, N, seed, iter, bin_of_n2, sigma, out);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_mask_zero(hipStream_t s, int64_t n3, const int8_t* flags, double2* data)
+{
+    hipLaunchKernelGGL(mask_zero_kernel, dim3(4096), dim3(256), 0, s, n3, flags, data);
     AXG_HIP(hipGetLastError());
 }
 

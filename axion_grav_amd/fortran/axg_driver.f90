@@ -32,7 +32,7 @@ program axion_3d
   logical :: out_strings, out_walls, ok, chk, flag_defects, gw_fix_swap, chk_gw, exists
   double precision :: tsim_max, limit_memory, b, sigma, g_rel, kappa, tau_ini, tau_fin, eps_dtau, boxsize
   double precision :: zeta, width_cr, thickness_cr, tau0, dtau_chk, tau_start, tau_end, ic_kmax
-  double precision, allocatable :: tau_id(:)
+  double precision, allocatable :: tau_id(:), crho_spec(:, :)
   real :: tsim_begin, tsim_now
   type(axg_keys) :: keys
   type(axg_params) :: prm
@@ -144,6 +144,13 @@ program axion_3d
      allocate(tau_id(0:n_chk))
      degrade_spec = ini_int('degrade_spec')
      if (degrade_spec /= 0) call die('error: degrade_spec > 0 is not supported by this build')
+  end if
+  if (spec_est) then
+     if (n_mod_eps < 1) call die('error: n_mod_eps must be positive')
+     ! crho and delta crho per snapshot (spectrum.f90:38-39), for get_epsilon at the end; a
+     ! snapshot spectrum_estimation did not run for in this process (before a resume) keeps 0
+     allocate(crho_spec(2, 0:n_chk))
+     crho_spec = 0d0
   end if
   ! memory estimate (settings.f90:241-260), one task
   b = byte_dc*2
@@ -261,6 +268,7 @@ program axion_3d
   call write_chk(tau_end, idx, flag_defects)
   close(300)
   close(400)
+  if (spec_est) call get_epsilon()   ! main.f90:58 fin_spectrum
   call axg_check(axg_destroy(h), c_null_ptr, 'axg_destroy')
   write(*, '(A)') ' ... done'
 
@@ -491,6 +499,7 @@ contains
           write(u, '(4E20.10e3)') q(iq), pw(iq), cv(iq)/nm(iq), nm(iq)
        end do
        close(u)
+       call covariance(idx, rho)
        ! analyze_energy (evolve.f90:216-235)
        xi0 = sw(5)
        xi1 = sw(3) - sw(5)
@@ -516,5 +525,139 @@ contains
     flush(300)
     flush(400)
   end subroutine defects_and_spectrum
+
+  ! The rest of spectrum_estimation (spectrum.f90:290-309): estimate_covmat with min_mc = 10,
+  ! max_mc = 100, tol = 0.1 on the GPU (its generator keyed by ss and idx), the covariance
+  ! file '(2I12,E20.10e3)' (pairs j <= i), and compute_crho (316-342) for delta crho.
+  subroutine covariance(idx, rho)
+    integer, intent(in) :: idx
+    double precision, intent(in) :: rho
+    double precision, allocatable :: cov(:)
+    integer(c_int32_t) :: iters
+    integer(c_int64_t) :: ncov, key
+    integer :: u, i, j, ij
+    double precision :: q, dq, q2dq, ddrho, drho
+    ncov = int(num_bin, c_int64_t)*(num_bin + 1)/2
+    allocate(cov(ncov))
+    write(*, '(A)') ' *estimating covariance matrix using MC (Gaussianity assumed)'
+    key = ior(ishft(int(seed(1), c_int64_t), 32), iand(int(seed(2), c_int64_t) + idx, int(z'FFFFFFFF', c_int64_t)))
+    call axg_check(axg_field_covariance(h, key, 10_c_int32_t, 100_c_int32_t, 1d-1, cov, ncov, iters), h, &
+         'axg_field_covariance')
+    if (iters > 0) then
+       write(*, '(A,I0,A)') '  MC converged (', iters, ' realisations)'
+    else
+       write(*, '(A)') '  WARNING: MC failed to converge'
+    end if
+    open(newunit=u, file=trim(rootname)//'_covariance_'//trim(itoa(idx))//'.dat', action='write', iostat=info)
+    if (info /= 0) call die('cannot open covariance.dat')
+    do i = 1, num_bin
+       do j = 1, i
+          ij = i*(i - 1)/2 + j
+          write(u, '(2I12,E20.10e3)') i, j, cov(ij)
+       end do
+    end do
+    close(u)
+    drho = 0d0
+    do i = 1, num_bin
+       ddrho = 0d0
+       do j = 1, i
+          ij = i*(i - 1)/2 + j
+          q = (wav_edge(j) + wav_edge(j - 1))/2
+          dq = wav_edge(j) - wav_edge(j - 1)
+          q2dq = q*q*dq/pi_t/pi_t
+          if (j == i) q2dq = q2dq/2
+          ddrho = ddrho + cov(ij)*q2dq
+       end do
+       q = (wav_edge(i) + wav_edge(i - 1))/2
+       dq = wav_edge(i) - wav_edge(i - 1)
+       q2dq = q*q*dq/twopi_t/pi_t
+       drho = drho + ddrho*q2dq
+    end do
+    drho = sqrt(drho)
+    crho_spec(1, idx) = rho
+    crho_spec(2, idx) = drho
+    write(*, '(A,3E16.4)') '  tau, crho, delta crho: ', tau_id(idx), crho_spec(1:2, idx)
+    deallocate(cov)
+  end subroutine covariance
+
+  ! Bin edge i of the axion spectrum (init_spectrum, spectrum.f90:58-61): i dq, dq = pi/dspa/num_bin.
+  double precision function wav_edge(i)
+    integer, intent(in) :: i
+    integer :: k
+    double precision :: dq
+    dq = pi_t/prm%dspa/num_bin
+    wav_edge = 0d0
+    do k = 1, i
+       wav_edge = wav_edge + dq
+    end do
+  end function wav_edge
+
+  ! get_epsilon (spectrum.f90:956-1040): the last n_id spectra and covariances read back from
+  ! their files; for each pair (idx, idx + n_mod_eps) the energy-weighted slope eps of the
+  ! differential spectrum, its error deps, and the changes of crho, into _eps.txt
+  ! '(2I12,7E20.10e3)'.
+  subroutine get_epsilon()
+    integer :: id, i, j, ij, u, lo
+    double precision, allocatable :: ps(:, :), pc(:, :)
+    double precision :: eps, deps, num, den, d, q, tau, dum1, dum2, f
+    integer :: ncov
+    write(*, *)
+    write(*, '(A)') ' #epsilon'
+    lo = n_chk - n_id + 1
+    ncov = num_bin*(num_bin + 1)/2
+    allocate(ps(num_bin, lo:n_chk), pc(ncov, lo:n_chk))
+    do id = lo, n_chk
+       open(newunit=u, file=trim(rootname)//'_spectrum_'//trim(itoa(id))//'.dat', action='read', status='old', &
+            iostat=info)
+       if (info /= 0) call die('error: cannot open spectrum.dat')
+       do i = 1, num_bin
+          read(u, *, iostat=info) dum1, ps(i, id), dum2
+          if (info /= 0) call die('error: cannot read spectrum.dat')
+       end do
+       close(u)
+       open(newunit=u, file=trim(rootname)//'_covariance_'//trim(itoa(id))//'.dat', action='read', status='old', &
+            iostat=info)
+       if (info /= 0) call die('error: cannot open covariance.dat')
+       do i = 1, num_bin
+          do j = 1, i
+             ij = i*(i - 1)/2 + j
+             read(u, *, iostat=info) dum1, dum2, pc(ij, id)
+             if (info /= 0) call die('error: cannot read spectrum.dat')
+          end do
+       end do
+       close(u)
+    end do
+    open(newunit=u, file=trim(rootname)//'_eps.txt', action='write', status='replace', iostat=info)
+    if (info /= 0) call die('error: cannot open eps.txt')
+    do id = lo, n_chk - n_mod_eps, n_mod_eps
+       tau = (tau_id(id + n_mod_eps) + tau_id(id))/2
+       f = tau/2/twopi_t
+       num = 0d0
+       den = 0d0
+       do i = 1, num_bin
+          d = ps(i, id + n_mod_eps) - ps(i, id)
+          q = (wav_edge(i - 1) + wav_edge(i))/2
+          num = num + d*q*q
+          den = den + d*q
+       end do
+       eps = num/den*f
+       deps = 0d0
+       do i = 1, num_bin
+          dum1 = (wav_edge(i - 1) + wav_edge(i))/2
+          do j = 1, i
+             ij = i*(i - 1)/2 + j
+             dum2 = (wav_edge(j - 1) + wav_edge(j))/2
+             d = (1 - eps/dum1/f)*(1 - eps/dum2/f)*(pc(ij, id + n_mod_eps) + pc(ij, id))*dum1**2*dum2**2
+             if (j /= i) d = d*2
+             deps = deps + d
+          end do
+       end do
+       deps = sqrt(deps/den/den)*f
+       write(u, '(2I12,7E20.10e3)') id, id + n_mod_eps, tau_id(id), tau_id(id + n_mod_eps), tau, eps, deps, &
+            crho_spec(1, id + n_mod_eps) - crho_spec(1, id), sqrt(crho_spec(2, id + n_mod_eps)**2 + crho_spec(2, id)**2)
+    end do
+    close(u)
+    deallocate(ps, pc)
+  end subroutine get_epsilon
 
 end program axion_3d

@@ -203,6 +203,18 @@ module axgrav_capi
        integer(c_int32_t), value :: nbins
      end function axg_field_spectrum
 
+     integer(c_int) function axg_field_covariance(h, seed, min_mc, max_mc, tol, cov, ncov, iters) &
+          bind(C, name="axg_field_covariance")
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), value :: min_mc, max_mc
+       real(c_double), value :: tol
+       real(c_double), intent(out) :: cov(*)
+       integer(c_int64_t), value :: ncov
+       integer(c_int32_t), intent(out) :: iters
+     end function axg_field_covariance
+
      integer(c_int) function axg_energy(h, tau, out) bind(C, name="axg_energy")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h

@@ -194,6 +194,18 @@ int axg_get_walls(axg_handle* h, double* rec, int64_t cap, int64_t* count);
 int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, double* conv,
                        double* num, double* rho, int32_t nbins);
 
+/* estimate_covmat (spectrum.f90:569-734; spectrum_estimation calls it with min_mc = 10,
+ * max_mc = 100, tol = 0.1): Monte Carlo covariance of the power of the last
+ * axg_field_spectrum (an ESTATE error if the fields changed since).  Each realisation draws
+ * Gaussian modes of that power (Philox4x32-10 keyed by seed; the reference uses its MCG59
+ * stream, so realisations differ from its own), masks them in real space and deconvolves.
+ * cov: the packed-upper num_bin x num_bin matrix (ncov = num_bin (num_bin+1)/2 entries, pair
+ * (i, j <= i) at i(i-1)/2 + j - 1), the reference's power_cov(:, idx).  iters: the number of
+ * realisations when the jackknife test met tol, else -(max_mc + 1), the count the reference
+ * divides by in that case.  Clobbers nothing but the work buffer. */
+int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc, int32_t max_mc, double tol,
+                         double* cov, int64_t ncov, int32_t* iters);
+
 /* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);

@@ -72,6 +72,9 @@ def lib():
         L.orc_thermal_factor.argtypes = [C.c_int, i64, d, d, d, d]
         L.orc_thermal_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
         L.orc_ic_thermal.argtypes = [p, p, i64, C.c_uint64, d, d, d, d]
+        L.orc_cov_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
+        L.orc_field_covariance.restype = C.c_int
+        L.orc_field_covariance.argtypes = [p, i64, C.c_int, d, d, p, p, C.c_uint64, C.c_int, C.c_int, d, p, p]
         L.orc_position_wall.restype = C.c_int
         L.orc_position_wall.argtypes = [p, p, p, p, p]
         L.orc_walls.restype = i64
@@ -306,6 +309,28 @@ def ic_thermal(N, seed, tau_ini, sigma_cr, dwav, dv_wav):
     pi = np.zeros((N, N, N), np.complex128)
     lib().orc_ic_thermal(_p(phi), _p(pi), N, seed, tau_ini, sigma_cr, dwav, dv_wav)
     return phi, pi
+
+
+# ------------------------------------------------------------------ spectrum covariance
+
+def cov_gauss(seed, it, lin):
+    re, im = C.c_double(), C.c_double()
+    lib().orc_cov_gauss(seed, it, lin, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def field_covariance(flags, nb, dspa, dwav, power, matn, seed, min_mc, max_mc, tol, want_pows=False):
+    """Monte Carlo covariance of the deconvolved spectrum: (packed cov, iteration counter
+    (negative without convergence), per-realisation pow (max_mc, nb) or None)."""
+    N = flags.shape[0]
+    ncov = nb * (nb + 1) // 2
+    cov = np.zeros(ncov)
+    pows = np.zeros((max_mc, nb)) if want_pows else None
+    it = lib().orc_field_covariance(_p(np.ascontiguousarray(flags, np.int8)), N, nb, dspa, dwav,
+                                    _p(np.ascontiguousarray(power, np.float64)),
+                                    _p(np.ascontiguousarray(matn, np.float64)), seed, min_mc, max_mc, tol,
+                                    _p(cov), _p(pows) if want_pows else None)
+    return cov, it, pows
 
 
 # ------------------------------------------------------------------ walls

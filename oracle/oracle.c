@@ -1075,13 +1075,11 @@ double orc_thermal_factor(int ip, int64_t n2, double tau_ini, double sigma_cr, d
     return f;
 }
 
-/* Unit complex Gaussian of mode lin (site-order index of the wavevector) for field ip, from
- * Philox4x32-10 keyed by the seed (Box-Muller on two 53-bit uniforms in (0,1]).  The
- * reference draws these from an MKL VSL MCG59 stream (initial.f90:105-135); this keeps one
- * draw per mode, independent of thread and rank counts. */
-void orc_thermal_gauss(uint64_t seed, int ip, uint64_t lin, double* re, double* im)
+/* Unit complex Gaussian number lin of the stream (seed, tag, word): Philox4x32-10 on the
+ * counter (lin, tag, word) keyed by the seed, Box-Muller on two 53-bit uniforms in (0,1]. */
+static void unit_gauss(uint64_t seed, uint32_t tag, uint32_t word, uint64_t lin, double* re, double* im)
 {
-    uint32_t c[4] = {(uint32_t)lin, (uint32_t)(lin >> 32), 0x74686d6cu, (uint32_t)ip};
+    uint32_t c[4] = {(uint32_t)lin, (uint32_t)(lin >> 32), tag, word};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)(c[1] >> 11);
     uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)(c[3] >> 11);
@@ -1092,6 +1090,21 @@ void orc_thermal_gauss(uint64_t seed, int ip, uint64_t lin, double* re, double* 
     double th = 6.283185307179586 * u2;
     *re = r * cos(th);
     *im = r * sin(th);
+}
+
+/* Unit complex Gaussian of mode lin (site-order index of the wavevector) for field ip.  The
+ * reference draws these from an MKL VSL MCG59 stream (initial.f90:105-135); this keeps one
+ * draw per mode, independent of thread and rank counts. */
+void orc_thermal_gauss(uint64_t seed, int ip, uint64_t lin, double* re, double* im)
+{
+    unit_gauss(seed, 0x74686d6cu, (uint32_t)ip, lin, re, im);
+}
+
+/* Unit complex Gaussian of mode lin in Monte Carlo realisation iter of the covariance
+ * estimate (the reference draws from its MCG59 stream, spectrum.f90:592-630). */
+void orc_cov_gauss(uint64_t seed, int iter, uint64_t lin, double* re, double* im)
+{
+    unit_gauss(seed, 0x636f7620u, (uint32_t)iter, lin, re, im);
 }
 
 /* initial.f90:80-238 get_initial_thermal (one rank): for phi then phidot, every mode of the
@@ -1119,6 +1132,108 @@ void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double ta
         orc_fft3d((const double*)buf, ip == 1 ? phi : pi, N, -1);
     }
     free(buf);
+}
+
+/* ------------------------------------------------------ spectrum covariance (MC) */
+
+/* spectrum.f90:569-734 estimate_covmat (degrade_spec = 0, one rank).  power: the deconvolved
+ * spectrum; matn: the packed-upper deconvolution matrix after division by num_i num_j
+ * (mat_deconv as spectrum_estimation leaves it, 316-323).  Realisation iter: mode k (bins by
+ * i = nint(q/dq+0.5) <= nb) = orc_cov_gauss * sqrt(power/2)/N^3, else 0; FORWARD FFT; mask
+ * cells (flag bit 4) zeroed; BACKWARD FFT; pow0 = binned |.|^2; pow = matn pow0 - power;
+ * cov += pow pow^T, with the leave-one-out sums tmp(:,k) of the jackknife; from iter = min_mc
+ * on, stop when max over the diagonal of sd(tmp/(iter-1))/(cov/iter) < tol.  cov is divided
+ * by the final loop counter, max_mc + 1 without convergence.  Returns the counter, negated
+ * without convergence.  pows (optional, max_mc x nb) receives each realisation's pow. */
+int orc_field_covariance(const int8_t* flags, int64_t N, int nb, double dspa, double dwav, const double* power,
+                         const double* matn, uint64_t seed, int min_mc, int max_mc, double tol, double* cov,
+                         double* pows)
+{
+    const int64_t n3 = N * N * N;
+    const int ncov = nb * (nb + 1) / 2;
+    double qmax = PI_T / dspa, dq0 = qmax / nb;
+    double* wave = (double*)calloc(nb + 2, sizeof(double));
+    for (int b = 1; b <= nb; b++) wave[b] = wave[b - 1] + dq0;
+    const double dq = wave[1] - wave[0];
+    free(wave);
+    int* bin = (int*)malloc(sizeof(int) * n3);
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                int64_t nx = wrapn(x, N), ny = wrapn(y, N), nz = wrapn(z, N);
+                double q = (double)(nx * nx) + (double)(ny * ny) + (double)(nz * nz);
+                q = sqrt(q) * dwav;
+                long i = lround(q / dq + 0.5);
+                bin[x + N * (y + N * z)] = (i >= 1 && i <= nb) ? (int)i : 0;
+            }
+    double* sigma = (double*)malloc(sizeof(double) * nb);
+    for (int b = 0; b < nb; b++) sigma[b] = sqrt(power[b] / 2) / (double)n3;
+    double* w = (double*)malloc(sizeof(double) * 2 * n3);
+    double* pow0 = (double*)malloc(sizeof(double) * nb);
+    double* pw = (double*)malloc(sizeof(double) * nb);
+    double* tmp = (double*)calloc((size_t)ncov * max_mc, sizeof(double));
+    for (int ij = 0; ij < ncov; ij++) cov[ij] = 0;
+    int iter, converged = 0;
+    for (iter = 1; iter <= max_mc; iter++) {
+        for (int64_t k = 0; k < n3; k++) {
+            double g1, g2;
+            orc_cov_gauss(seed, iter, (uint64_t)k, &g1, &g2);
+            if (bin[k]) {
+                w[2 * k] = g1 * sigma[bin[k] - 1];
+                w[2 * k + 1] = g2 * sigma[bin[k] - 1];
+            } else {
+                w[2 * k] = 0;
+                w[2 * k + 1] = 0;
+            }
+        }
+        orc_fft3d(w, w, N, -1);
+        if (flags)
+            for (int64_t k = 0; k < n3; k++)
+                if (flags[k] & 4) { w[2 * k] = 0; w[2 * k + 1] = 0; }
+        orc_fft3d(w, w, N, +1);
+        for (int b = 0; b < nb; b++) pow0[b] = 0;
+        for (int64_t k = 0; k < n3; k++)
+            if (bin[k]) pow0[bin[k] - 1] += w[2 * k] * w[2 * k] + w[2 * k + 1] * w[2 * k + 1];
+        /* dspmv('U', nb, 1, matn, pow0, 1, 1, pow, 1) with pow = -power */
+        for (int i = 1; i <= nb; i++) {
+            double acc = -power[i - 1];
+            for (int j = 1; j <= nb; j++) {
+                int a = i < j ? i : j, b = i < j ? j : i;
+                acc += matn[a + b * (b - 1) / 2 - 1] * pow0[j - 1];
+            }
+            pw[i - 1] = acc;
+        }
+        if (pows)
+            for (int b = 0; b < nb; b++) pows[(size_t)(iter - 1) * nb + b] = pw[b];
+        for (int i = 1; i <= nb; i++)
+            for (int j = 1; j <= i; j++) {
+                int ij = i * (i - 1) / 2 + j - 1;
+                double u = pw[i - 1] * pw[j - 1];
+                tmp[(size_t)ij * max_mc + iter - 1] = cov[ij];
+                for (int k = 0; k < iter - 1; k++) tmp[(size_t)ij * max_mc + k] += u;
+                cov[ij] += u;
+            }
+        if (iter < min_mc) continue;
+        double diff = 0;
+        for (int i = 1; i <= nb; i++) {
+            int ij = i * (i + 1) / 2 - 1;
+            double u = 0, v = 0;
+            for (int k = 0; k < iter; k++) {
+                double x = tmp[(size_t)ij * max_mc + k] / (iter - 1);
+                u = u + x;
+                v = v + x * x;
+            }
+            u = u / iter;
+            v = v / iter;
+            v = sqrt(v - u * u);
+            double x = cov[ij] / iter;
+            if (v / x > diff) diff = v / x;
+        }
+        if (diff < tol) { converged = 1; break; }
+    }
+    for (int ij = 0; ij < ncov; ij++) cov[ij] = cov[ij] / iter;
+    free(bin); free(sigma); free(w); free(pow0); free(pw); free(tmp);
+    return converged ? iter : -iter;
 }
 
 /* ------------------------------------------------------------ wall fit and area */

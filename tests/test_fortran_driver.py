@@ -76,9 +76,9 @@ def read_rows(path):
     return np.array([[float(x) for x in l.split()] for l in open(path) if l.strip()])
 
 
-def oracle_run(N, P, seed, kmax, taus, eps):
-    """The same schedule with the oracle: IC, then get_dtau + DKD per interval."""
-    phi, pi = O.ic_flat(N, seed, kmax, P["dwav"])
+def oracle_run(N, P, phi0, pi0, taus, eps):
+    """The same schedule with the oracle: from (phi0, pi0), get_dtau + DKD per interval."""
+    phi, pi = phi0.copy(), pi0.copy()
     A = O.new_gw_state(N)
     states = {}
     for idx in range(1, len(taus)):
@@ -94,13 +94,18 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
     print("[case] fortran driver N=32", flush=True)
     N, nb, nbg = 32, 8, 25
     d = O.derive(SIGMA, 90.0, 1e-30, 2.0, 4.0, N)
-    # the oracle's seeded field enters through ic_file, so the fields can be compared bitwise
-    phi0, pi0 = O.ic_flat(N, 77, 6.0, d["dwav"])
+    # the oracle's seeded field enters through ic_file, so the fields can be compared bitwise.
+    # A smooth phi (few defects, a mask of 10-20% of the cells) and a broadband pi keep the
+    # deconvolved power positive in every bin at both snapshots, so the covariance is finite
+    # (a negative bin makes estimate_covmat's sqrt(power/2), and so the covariance, NaN).
+    phi0, _ = O.ic_flat(N, 77, 1.0, d["dwav"])
+    rng = np.random.default_rng(5)
+    pi0 = 2.0 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
     np.concatenate([phi0.ravel(), pi0.ravel()]).tofile(tmp_path / "ic.raw")
     run(tmp_path, ini_text(str(tmp_path), ic_file=str(tmp_path / "ic.raw")), "-b", "100")
     P = dict(d, kappa=1e-30, initial_loop=True)
     taus = [1.0, 1.5, 2.0]
-    states = oracle_run(N, P, 77, 6.0, taus, 1.0)
+    states = oracle_run(N, P, phi0, pi0, taus, 1.0)
     for idx in (1, 2):
         phi, pi, A = states[idx]
         # gravspec: tau_id, q, q f_cr, power, denom (grav_waves.f90:313-314)
@@ -126,6 +131,48 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
         assert np.array_equal(s[:, 3], num)
         assert np.allclose(s[:, 2], conv / num, rtol=1e-8)
         assert np.allclose(s[:, 1], power, rtol=1e-6, atol=1e-12 * np.abs(power).max())
+        # covariance (spectrum.f90:295-304, estimate_covmat 569-734): pairs (i, j <= i); the
+        # MC generator is keyed by ss = "1 2" and idx
+        c = read_rows(tmp_path / f"run_covariance_{idx}.dat")
+        ii, jj = np.tril_indices(nb)
+        order = np.lexsort((jj, ii))
+        assert c.shape == (nb * (nb + 1) // 2, 3)
+        assert np.array_equal(c[:, 0], ii[order] + 1) and np.array_equal(c[:, 1], jj[order] + 1)
+        matn = [(O.sym_from_packed(inv, nb) / np.outer(num, num))[j, i] for i, j in zip(ii[order], jj[order])]
+        ocov, oit, _ = O.field_covariance(flags, nb, d["dspa"], d["dwav"], power, np.array(matn), (1 << 32) | (2 + idx),
+                                          10, 100, 0.1)
+        assert power.min() > 0 and np.all(np.isfinite(ocov))
+        assert np.allclose(c[:, 2], ocov, rtol=1e-5, atol=1e-6 * np.abs(ocov).max())
+    # get_epsilon (spectrum.f90:956-1040) from the files: eps = f sum d q^2 / sum d q with
+    # d the differential spectrum, and its delta-method error sqrt(w^T (C1 + C2) w) f/|den|,
+    # w = q^2 - eps q / f; delta crho = sqrt(a^T C a) / (2 pi^2), a = q^2 dq, in quadrature
+    e = read_rows(tmp_path / "run_eps.txt")
+    assert e.shape == (1, 9) and list(e[0, :2]) == [1, 2]
+    edges = np.zeros(nb + 1)
+    for b in range(1, nb + 1):
+        edges[b] = edges[b - 1] + PI_T / d["dspa"] / nb
+    q, dq = (edges[1:] + edges[:-1]) / 2, np.diff(edges)
+    spec = {i: read_rows(tmp_path / f"run_spectrum_{i}.dat")[:, 1] for i in (1, 2)}
+    C = {}
+    for i in (1, 2):
+        rows = read_rows(tmp_path / f"run_covariance_{i}.dat")
+        M = np.zeros((nb, nb))
+        M[rows[:, 0].astype(int) - 1, rows[:, 1].astype(int) - 1] = rows[:, 2]
+        C[i] = M + np.tril(M, -1).T
+    tau = (taus[1] + taus[2]) / 2
+    f = tau / 2 / (2 * PI_T)
+    dd = spec[2] - spec[1]
+    den = (dd * q).sum()
+    eps = (dd * q * q).sum() / den * f
+    w = q * q - eps * q / f
+    deps = np.sqrt(w @ (C[1] + C[2]) @ w / den ** 2) * f
+    a = q * q * dq
+    drho = {i: np.sqrt(a @ C[i] @ a) / (2 * PI_T * PI_T) for i in (1, 2)}
+    cr = read_rows(tmp_path / "run_crho.txt")
+    assert np.allclose(e[0, 2:5], [taus[1], taus[2], tau], rtol=1e-9)
+    assert np.allclose(e[0, 5:7], [eps, deps], rtol=1e-7)
+    assert np.allclose(e[0, 7], cr[1, 3] - cr[0, 3], rtol=1e-6, atol=1e-9 * abs(cr[1, 3]))
+    assert np.allclose(e[0, 8], np.hypot(drho[1], drho[2]), rtol=1e-7)
     sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
     assert sw.shape == (2, 12) and np.allclose(sw[:, 1], taus[1:])
     # string parameter (evolve.f90:191) and velocities from the fits seeded by ss = "1 2" and idx

@@ -247,6 +247,54 @@ def test_field_spectrum_fp64(N):
     assert relerr(s["power"], power) < 1e-8
 
 
+@pytest.mark.parametrize("N,kmax,min_mc,max_mc,tol", [(16, 2.0, 10, 100, 0.1), (32, 2.0, 4, 12, 0.0),
+                                                       (16, 6.0, 10, 20, 0.1)])
+def test_field_covariance_fp64(N, kmax, min_mc, max_mc, tol):
+    """estimate_covmat on the GPU against the oracle's restatement on the same mask, power (the
+    GPU's) and deconvolution matrix (the oracle's, which agrees with the GPU's to 1e-8): the
+    same realisation count, and the covariance to 1e-6.  The second case never converges
+    (tol = 0): counter -(max_mc + 1).  In the third, most cells are masked and a bin's
+    deconvolved power is negative: sqrt(power/2) is NaN there, and so is every covariance
+    entry, on both sides."""
+    say("field_covariance", N, kmax, min_mc, max_mc, tol)
+    nb = 8 if N == 16 else 12
+    prm, P = params(N, grav_spec=False, num_bin=nb)
+    phi, _ = ic(N, 31, kmax)
+    rng = np.random.default_rng(1)
+    pi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    tau = 3.0
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.defects()
+        with pytest.raises(RuntimeError):
+            L.field_covariance(5, min_mc, max_mc, tol)  # no spectrum yet
+        s = L.field_spectrum(tau)
+        cov, it = L.field_covariance(5, min_mc, max_mc, tol)
+        cov2, it2 = L.field_covariance(5, min_mc, max_mc, tol)
+    assert it == it2 and np.array_equal(cov, cov2, equal_nan=True)  # deterministic
+    flags, _ = O.classify(phi)
+    assert np.count_nonzero(flags & 4) > 0
+    assert (s["power"].min() < 0) == (kmax == 6.0)
+    pw, num = O.conv_spectrum(pi, flags, nb, P["dspa"], P["dwav"], P["dv_spa"], P["dv_wav"])
+    tab = O.init_spectrum(nb, P["dspa"])
+    _, inv, info = O.matrix_deconv(flags, nb, P["dwav"], P["dv_spa"], P["dv_wav"], tab)
+    M = O.sym_from_packed(inv, nb) / np.outer(num, num)
+    ocov, oit, _ = O.field_covariance(flags, nb, P["dspa"], P["dwav"], s["power"], packed(M), 5, min_mc,
+                                      max_mc, tol)
+    assert it == oit
+    if tol == 0.0:
+        assert it == -(max_mc + 1)
+    if kmax == 6.0:
+        assert np.all(np.isnan(cov)) and np.all(np.isnan(ocov))
+    else:
+        assert np.all(np.isfinite(cov)) and relerr(cov, ocov) < 1e-6
+
+
+def packed(M):
+    nb = M.shape[0]
+    return np.array([M[j - 1, i - 1] for i in range(1, nb + 1) for j in range(1, i + 1)])
+
+
 @pytest.mark.parametrize("prec,tol,kappa0", [(capi.FP64, 1e-12, 0.0), (capi.FP32C, 1e-9, 0.0),
                                               (capi.FP64, 1e-12, 1e-3)])
 def test_energy(prec, tol, kappa0):

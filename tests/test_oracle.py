@@ -383,6 +383,113 @@ def test_deconvolution_without_mask_is_diagonal():
     assert np.abs(off).max() <= 1e-12 * np.abs(np.diag(M)).max()
 
 
+# ------------------------------------------------------------------ spectrum covariance (spectrum.f90:569-734)
+
+def packed(M):
+    nb = M.shape[0]
+    return np.array([M[j - 1, i - 1] for i in range(1, nb + 1) for j in range(1, i + 1)])
+
+
+def bin_index(N, nb, P):
+    n = np.fft.fftfreq(N, 1.0 / N)
+    n2 = n[:, None, None] ** 2 + n[None, :, None] ** 2 + n[None, None, :] ** 2
+    q = np.sqrt(n2) * P["dwav"]
+    dq = PI_T / P["dspa"] / nb
+    i = np.floor(q / dq + 0.5 + 0.5).astype(int)  # nint of a non-negative argument
+    return np.where(i <= nb, i, 0)
+
+
+def test_cov_without_mask_is_binned_mode_power():
+    """No mask: the FORWARD/BACKWARD pair returns N^3 times the modes, so realisation n's
+    pow0_b = sum over bin b of |g_k|^2 power_b/2 with the realisation's unit Gaussians g_k;
+    with matn = diag(1/num) its pow_b = pow0_b/num_b - power_b."""
+    N, nb = 8, 4
+    P = phys(N)
+    b = bin_index(N, nb, P)
+    num = np.array([np.count_nonzero(b == i) for i in range(1, nb + 1)], float)
+    power = np.array([3.0, 0.5, 2.0, 1.25])
+    matn = packed(np.diag(1 / num))
+    flags = np.zeros((N, N, N), np.int8)
+    cov, it, pows = O.field_covariance(flags, nb, P["dspa"], P["dwav"], power, matn, 77, 3, 3, 0.0, True)
+    assert it == -4  # tol 0 never converges: the counter ends at max_mc + 1
+    for n in range(1, 4):
+        g2 = np.array([abs(O.cov_gauss(77, n, k)) ** 2 for k in range(N ** 3)]).reshape(N, N, N)
+        pow0 = np.array([(g2[b == i] * power[i - 1] / 2).sum() for i in range(1, nb + 1)])
+        assert np.allclose(pows[n - 1], pow0 / num - power, rtol=1e-11, atol=1e-12 * power.max())
+    S = sum(np.outer(p, p) for p in pows)
+    assert np.allclose(cov, packed(S) / 4, rtol=1e-13, atol=0)
+
+
+def test_cov_fully_masked_is_outer_product():
+    """Every cell masked: each realisation's pow is -power, so cov = power power^T; the
+    jackknife spread is zero, so the loop stops at min_mc, and without convergence (tol = 0)
+    the sum of max_mc products is divided by max_mc + 1."""
+    N, nb = 8, 4
+    P = phys(N)
+    power = np.array([1.0, 2.0, 0.25, 4.0])
+    matn = packed(np.diag([0.3, 0.2, 0.1, 0.05]))
+    flags = np.full((N, N, N), 4, np.int8)
+    cov, it, _ = O.field_covariance(flags, nb, P["dspa"], P["dwav"], power, matn, 5, 6, 20, 0.1)
+    assert it == 6
+    assert np.allclose(cov, packed(np.outer(power, power)), rtol=1e-14, atol=0)
+    cov, it, _ = O.field_covariance(flags, nb, P["dspa"], P["dwav"], power, matn, 5, 6, 20, 0.0)
+    assert it == -21
+    assert np.allclose(cov, packed(np.outer(power, power)) * 20 / 21, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("tol,stop_at", [(0.1, 13), (0.05, 22), (0.01, None)])
+def test_cov_jackknife_stop(tol, stop_at):
+    """The leave-one-out bookkeeping against a direct restatement: after each realisation
+    n >= min_mc, x_k = (S_n - p_k p_k^T)_bb / (n-1) for k <= n; diff = max_b sd_k(x_k) /
+    (S_n,bb / n); the loop stops at the first n with diff < tol and returns S_n / n."""
+    N, nb = 8, 4
+    P = phys(N)
+    rng = np.random.default_rng(4)
+    flags = np.where(rng.random((N, N, N)) < 0.3, 4, 0).astype(np.int8)
+    power = np.array([2.0, 1.0, 0.5, 0.8])
+    A = rng.standard_normal((nb, nb))
+    matn = packed((A @ A.T + nb * np.eye(nb)) / 50)
+    min_mc, max_mc = 4, 60
+    cov, it, pows = O.field_covariance(flags, nb, P["dspa"], P["dwav"], power, matn, 9, min_mc, max_mc, tol, True)
+    stop = None
+    for n in range(min_mc, max_mc + 1):
+        p = pows[:n]
+        S = p.T @ p
+        loo = np.diag(S)[None, :] - p * p
+        x = loo / (n - 1)
+        sd = np.sqrt((x * x).mean(0) - x.mean(0) ** 2)
+        if (sd / (np.diag(S) / n)).max() < tol:
+            stop = n
+            break
+    assert stop == stop_at  # the seeded case stops mid-loop (or never, for the smallest tol)
+    if stop is None:
+        assert it == -(max_mc + 1)
+        assert np.allclose(cov, packed(pows.T @ pows) / (max_mc + 1), rtol=1e-12, atol=0)
+    else:
+        assert it == stop
+        p = pows[:stop]
+        assert np.allclose(cov, packed(p.T @ p) / stop, rtol=1e-12, atol=0)
+
+
+def test_cov_matches_gaussian_variance():
+    """Statistics: without a mask and with matn = diag(1/num), pow_b is the mean of num_b
+    exponential variables of mean power_b minus power_b, so cov_bb -> power_b^2 / num_b and
+    the off-diagonal terms -> 0 (400 realisations: about 7% per bin)."""
+    N, nb = 8, 4
+    P = phys(N)
+    b = bin_index(N, nb, P)
+    num = np.array([np.count_nonzero(b == i) for i in range(1, nb + 1)], float)
+    power = np.array([1.0, 3.0, 2.0, 0.5])
+    matn = packed(np.diag(1 / num))
+    flags = np.zeros((N, N, N), np.int8)
+    cov, it, _ = O.field_covariance(flags, nb, P["dspa"], P["dwav"], power, matn, 2024, 400, 400, 0.0)
+    C = O.sym_from_packed(cov, nb) * 401 / 400
+    r = np.diag(C) * num / power ** 2
+    assert np.all(np.abs(r - 1) < 0.3) and abs(r.mean() - 1) < 0.12
+    corr = C / np.sqrt(np.outer(np.diag(C), np.diag(C)))
+    assert np.abs(corr - np.eye(nb)).max() < 0.2
+
+
 # ------------------------------------------------------------------ defects (defects_3d.f90:309-349)
 
 def classify_cell(ph, ids=True, idw=True, spec=True):
