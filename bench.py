@@ -111,6 +111,9 @@ def run_diagnostics(lat, t):
     d["walls_ms"], wl = time_ms(lat.walls, reps=1)
     d["field_spectrum_ms"], _ = time_ms(lambda: lat.field_spectrum(t))
     d["energy_ms"], _ = time_ms(lambda: lat.energy(t))
+    # estimate_covmat: 10 realisations (tol 0 runs all of them), reported per realisation
+    ms, _ = time_ms(lambda: lat.field_covariance(3, 10, 10, 0.0), reps=1)
+    d["covariance_ms_per_realisation"] = ms / 10
     d["n_string_cells"] = int(cells[0])
     d["n_wall_cells"] = int(cells[1])
     d["gw_spectra_per_s"] = 1e3 / d["gw_spectrum_ms"]
