@@ -61,6 +61,10 @@ class WallOut(C.Structure):
     _fields_ = [("n_wall", C.c_int64), ("area", C.c_double)]
 
 
+class GroupOut(C.Structure):
+    _fields_ = [("n_group", C.c_int64), ("th_dist", C.c_double), ("sum", C.c_double * 3)]
+
+
 class StringOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_vel", C.c_int64), ("length", C.c_double), ("vel", C.c_double * 3)]
 
@@ -68,7 +72,8 @@ class StringOut(C.Structure):
 EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_error", "axg_set_fields",
            "axg_get_fields", "axg_init_flat", "axg_init_thermal", "axg_get_gw_state", "axg_set_gw_state", "axg_advance",
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
-           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls", "axg_field_spectrum",
+           "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls",
+           "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
            "axg_field_covariance", "axg_energy", "axg_tune", "axg_profile")
 
 _LIB = None
@@ -103,6 +108,9 @@ def lib():
             "axg_get_strings": [p, p, i64, p],
             "axg_walls": [p, C.POINTER(WallOut)],
             "axg_get_walls": [p, p, i64, p],
+            "axg_group_strings": [p, d, d, C.POINTER(GroupOut)],
+            "axg_group_walls": [p, d, d, C.POINTER(GroupOut)],
+            "axg_get_groups": [p, i32, p, i64, p],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_field_covariance": [p, C.c_uint64, i32, i32, d, p, i64, p],
             "axg_energy": [p, d, p],
@@ -309,6 +317,19 @@ class Lattice:
         buf = np.zeros((max(n, 1), 8))
         self._check(lib().axg_get_walls(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_walls")
         return buf[:n]
+
+    def group(self, kind, width, hor_size):
+        """Networks of the last strings (kind 0) or walls (kind 1): dict(n_group, th_dist,
+        sum (3,), ids (group number per record))."""
+        o = GroupOut()
+        fn = lib().axg_group_strings if kind == 0 else lib().axg_group_walls
+        self._check(fn(self._h, width, hor_size, C.byref(o)), "axg_group_strings" if kind == 0 else "axg_group_walls")
+        cnt = C.c_int64()
+        self._check(lib().axg_get_groups(self._h, kind, None, 0, C.byref(cnt)), "axg_get_groups")
+        ids = np.zeros(max(cnt.value, 1), np.int32)
+        self._check(lib().axg_get_groups(self._h, kind, ids.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt)),
+                    "axg_get_groups")
+        return dict(n_group=o.n_group, th_dist=o.th_dist, sum=np.array(o.sum[:]), ids=ids[: cnt.value])
 
     def field_spectrum(self, tau):
         nb = self.params.num_bin

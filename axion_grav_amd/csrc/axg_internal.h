@@ -153,6 +153,19 @@ constexpr int WALL_NT = 64;
 template <typename T, bool COMP>
 void launch_wall_fit(hipStream_t s, int N, const void* phi, const void* cphi, const int32_t* cells, int64_t nw,
                      double* rec, double* partial);
+// Grouping of defect points (groups.hip): clears the cell map, scatters the points into it
+// (points more than one cell from their cell centre go to the outlier list instead) and
+// returns the largest per-axis offset of a mapped point; then links the points within th and
+// writes each point's label (the smallest index of its group).  scalars: 2 device words
+// (offset bits; outlier count and first mapped point); the failure flag is separate.  mode 0:
+// cell box of half-width R; 1: all mapped points in one group (centre = the first mapped
+// point); 2: every pair tested.
+double group_prepare(hipStream_t s, int N, const double* rec, int stride, int col, const int32_t* cells, int64_t n,
+                     int32_t* map, int32_t* parent, double* val, unsigned long long* scalars, int32_t* outliers,
+                     int* n_out, int* first_mapped);
+void group_link(hipStream_t s, int N, const double* rec, int stride, const int32_t* cells, int64_t n,
+                const int32_t* map, int32_t* parent, int32_t* label, double th, double maxoff, int R, int mode,
+                int centre, const int32_t* outliers, int n_out, int* fail);
 void launch_cov_modes(hipStream_t s, int N, uint64_t seed, int iter, const uint16_t* bin_of_n2, const double* sigma,
                       double2* out);
 void launch_mask_zero(hipStream_t s, int64_t n3, const int8_t* flags, double2* data);

@@ -144,6 +144,12 @@ struct axg_handle {
     double* wall_part = nullptr;
     int64_t wall_cap = 0;
     int64_t nwall_rec = -1;
+    // grouping scratch (parent, label, value per point; grown on demand) and the last groups
+    void* grp = nullptr;
+    int64_t grp_cap = 0;
+    std::vector<int32_t> grp_ids[2];  // group number (1-based) per string / wall record
+    bool have_grp[2] = {false, false};
+    bool wall_bad = false;  // the last axg_walls met cells with fewer than 3 edge points
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
     // the last field spectrum: deconvolved power and the count-normalised deconvolution matrix
@@ -508,6 +514,7 @@ extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
+        h->have_grp[0] = h->have_grp[1] = false;
         AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
@@ -625,6 +632,7 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     h->have_spec = false;
     h->nstr_rec = -1;
     h->nwall_rec = -1;
+    h->have_grp[0] = h->have_grp[1] = false;
     AXG_HIP(hipStreamSynchronize(h->stream));
     if (h->prof) h->collect();
 }
@@ -871,6 +879,7 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         h->have_flags = true;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
+        h->have_grp[0] = h->have_grp[1] = false;
     });
 }
 
@@ -940,6 +949,7 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
         out->length = sums[0];
         for (int k = 0; k < 3; k++) out->vel[k] = out->n_vel > 0 ? sums[2 + k] / (double)out->n_vel : 0.0;
         h->nstr_rec = ns;
+        h->have_grp[0] = false;
     });
 }
 
@@ -990,9 +1000,129 @@ extern "C" int axg_walls(axg_handle* h, axg_wall_out* out)
         out->n_wall = nw;
         out->area = sums[0];
         h->nwall_rec = nw;
+        h->have_grp[1] = false;
+        h->wall_bad = sums[1] > 0;
         if (sums[1] > 0)
             throw Error(AXG_ENUMERIC, "position_wall: " + std::to_string((int64_t)sums[1]) +
                                           " wall cells have fewer than 3 edges where Im(phi) changes sign");
+    });
+}
+
+// grouping_strings / grouping_walls (defects_3d.f90:1235-1260, 1735-1760) on the records of the
+// last fit, and the grouped sums of add_string_length / add_wall_area (1654-1733, 2145-2204).
+// Groups are numbered 1, 2, ... in the order of their first record (site order); the sums run
+// over each group's records in site order, then over the groups in that order.
+static void group_records(axg_handle* h, int kind, double width, double hor_size, axg_group_out* out)
+{
+    const bool st = kind == 0;
+    const int64_t n = st ? h->nstr_rec : h->nwall_rec;
+    require(n >= 0, st ? "call axg_strings first (after the last change of the fields)"
+                       : "call axg_walls first (after the last change of the fields)");
+    require(out != nullptr, "null output");
+    require(width >= 0 && hor_size > 0 && std::isfinite(width) && std::isfinite(hor_size),
+            "need width >= 0 and hor_size > 0");
+    // threshold: the weighted geometric mean of width and separation, at least 2 (1245-1246)
+    double th = pow(pow(width, 2.0) * hor_size, 1 / (2.0 + 1));
+    th = std::max(2.0, th);
+    require(st || !h->wall_bad, "the last axg_walls failed (cells with fewer than 3 edge points)");
+    const double* rec = st ? h->str_rec : h->wall_rec;
+    const int32_t* cells = st ? h->str_cells : h->wall_cells;
+    const int stride = st ? 11 : 8, col = 7;  // sl and wa are both column 7
+    std::vector<int32_t> lab(n);
+    std::vector<double> val(n);
+    if (n > 0) {
+        if (n > h->grp_cap) {
+            void* q = h->grp;
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            h->release_one(q);
+            h->grp = nullptr;
+            h->grp_cap = 0;
+            h->grp = h->alloc((size_t)n * 16 + 256);  // val, parent, label; scalars
+            h->grp_cap = n;
+        }
+        double* d_val = (double*)h->grp;
+        int32_t* d_parent = (int32_t*)(d_val + n);
+        int32_t* d_label = d_parent + n;
+        unsigned long long* d_scal = (unsigned long long*)((char*)h->grp + (((size_t)n * 16 + 63) & ~(size_t)63));
+        int* d_fail = (int*)(d_scal + 2);
+        int32_t* map = (int32_t*)h->work;  // one int per cell
+        int n_out = 0, centre = 0;
+        const double maxoff = group_prepare(h->stream, h->N, rec, stride, col, cells, n, map, d_parent, d_val, d_scal,
+                                            d_label, &n_out, &centre);
+        require(n_out <= 65536, "more than 65536 defect positions lie over a cell from their cell");
+        // route: the cell box (th_dist + 2 offsets, each at most one cell, up to 16 cells), one
+        // group when no two mapped points can be farther apart than th_dist, else every pair
+        int mode = 0, R = 0;
+        if (th >= sqrt(3.0) * h->N / 2) {
+            mode = n_out < n ? 1 : 2;
+        } else if (th + 2 * maxoff <= 16) {
+            R = std::min((int)ceil(th + 2 * maxoff), h->N);
+        } else {
+            require(n <= 262144, "grouping threshold " + std::to_string(th) +
+                                     " cells needs the all-pairs test, limited to 262144 points");
+            mode = 2;
+        }
+        const int zero = 0;
+        AXG_HIP(hipMemcpyAsync(d_fail, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
+        group_link(h->stream, h->N, rec, stride, cells, n, map, d_parent, d_label, th, maxoff, R, mode, centre,
+                   d_label, n_out, d_fail);
+        int fail = 0;
+        AXG_HIP(hipMemcpyAsync(&fail, d_fail, sizeof fail, hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipMemcpyAsync(lab.data(), d_label, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipMemcpyAsync(val.data(), d_val, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        require(fail == 0, "grouping did not converge");
+    }
+    std::vector<int32_t>& ids = h->grp_ids[kind];
+    ids.assign(n, 0);
+    std::vector<double> gsum;
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t r = lab[i];
+        require(r >= 0 && r <= i, "grouping produced an invalid label");
+        if (r == i) {
+            gsum.push_back(0.0);
+            ids[i] = (int32_t)gsum.size();
+        } else {
+            ids[i] = ids[r];
+        }
+        gsum[ids[i] - 1] += val[i];
+    }
+    // thresholds of add_string_length (2 hor, 2 pi hor) and add_wall_area (pi hor^2, 4 pi hor^2)
+    const double t2 = st ? 2 * hor_size : AXG_PI_T * (hor_size * hor_size);
+    const double t3 = st ? AXG_TWOPI_T * hor_size : 4 * AXG_PI_T * (hor_size * hor_size);
+    double s[3] = {0, 0, 0};
+    for (double g : gsum) {
+        s[0] = s[0] + g;
+        if (g > t2) s[1] = s[1] + g;
+        if (g > t3) s[2] = s[2] + g;
+    }
+    out->n_group = (int64_t)gsum.size();
+    out->th_dist = th;
+    for (int k = 0; k < 3; k++) out->sum[k] = s[k];
+    h->have_grp[kind] = true;
+}
+
+extern "C" int axg_group_strings(axg_handle* h, double width, double hor_size, axg_group_out* out)
+{
+    return guarded(h, [&] { group_records(h, 0, width, hor_size, out); });
+}
+
+extern "C" int axg_group_walls(axg_handle* h, double thickness, double hor_size, axg_group_out* out)
+{
+    return guarded(h, [&] { group_records(h, 1, thickness, hor_size, out); });
+}
+
+extern "C" int axg_get_groups(axg_handle* h, int32_t kind, int32_t* ids, int64_t cap, int64_t* count)
+{
+    return guarded(h, [&] {
+        require(count != nullptr, "null count");
+        require(kind == 0 || kind == 1, "kind must be 0 (strings) or 1 (walls)");
+        require(h->have_grp[kind], kind == 0 ? "call axg_group_strings first" : "call axg_group_walls first");
+        const std::vector<int32_t>& g = h->grp_ids[kind];
+        *count = (int64_t)g.size();
+        if (g.empty() || ids == nullptr || cap == 0) return;  // size query
+        require(cap >= (int64_t)g.size(), "id buffer too small");
+        memcpy(ids, g.data(), g.size() * sizeof(int32_t));
     });
 }
 
@@ -1319,6 +1449,7 @@ extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
+        h->have_grp[0] = h->have_grp[1] = false;
     });
 }
 
@@ -1391,5 +1522,6 @@ extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
+        h->have_grp[0] = h->have_grp[1] = false;
     });
 }

@@ -443,6 +443,7 @@ contains
     type(axg_defect_out) :: dc
     type(axg_string_out) :: so
     type(axg_wall_out) :: wo
+    type(axg_group_out) :: go
     double precision :: xi(3), largea(3), vel(3), sw(11), crho(3), rho, xi0, xi1, gam
     double precision, allocatable :: q(:), pw(:), nu(:), de(:), cv(:), nm(:)
     integer(c_int32_t), allocatable :: ml(:)
@@ -469,18 +470,37 @@ contains
           deallocate(ml)
        end if
     end if
-    ! string length and velocities (add_string_length, defects_3d.f90:276-279, group_strings
-    ! = F: the three lengths coincide); the generator's stream moves on with idx as the
-    ! reference's does between calls.  Wall areas: add_wall_area (280-283, group_walls = F).
+    ! string length and velocities (add_string_length, defects_3d.f90:276-279; without
+    ! grouping the three lengths coincide); the generator's stream moves on with idx as the
+    ! reference's does between calls.  Wall areas: add_wall_area (280-283).  With grouping
+    ! (226-227) the second and third entries keep only the long strings / large walls.
     if (id_strings) then
        call axg_check(axg_strings(h, seed(1), seed(2) + int(idx, c_int32_t), so), h, 'axg_strings')
        xi(1:3) = so%length
        vel(1:3) = so%vel
+       if (group_strings) then
+          call axg_check(axg_group_strings(h, width_cr/prm%dspa/tau, tau/prm%dspa, go), h, 'axg_group_strings')
+          write(*, '(A,F14.6)') ' size of string width in lattice units:', width_cr/prm%dspa/tau
+          write(*, '(A,F14.6)') ' size of string separation in lattice units:', tau/prm%dspa
+          write(*, '(A,F14.6)') ' threshold for string grouping in lattice units:', go%th_dist
+          write(*, '(A,I0)') '  strings grouped into networks: ', go%n_group
+          xi(1:3) = go%sum
+       end if
+       if (out_strings) call write_defects(0, idx, int(so%n_string, c_int64_t), group_strings)
        if (so%n_vel == 0) write(*, '(A)') ' velocity cannot be computed because there are no strings'
     end if
     if (id_walls) then
        call axg_check(axg_walls(h, wo), h, 'axg_walls')
        largea(1:3) = wo%area
+       if (group_walls) then
+          call axg_check(axg_group_walls(h, thickness_cr/prm%dspa/tau, tau/prm%dspa, go), h, 'axg_group_walls')
+          write(*, '(A,F14.6)') ' size of wall width in lattice units:', thickness_cr/prm%dspa/tau
+          write(*, '(A,F14.6)') ' size of wall separation in lattice units:', tau/prm%dspa
+          write(*, '(A,F14.6)') ' threshold for wall grouping in lattice units:', go%th_dist
+          write(*, '(A,I0)') '  walls grouped into networks: ', go%n_group
+          largea(1:3) = go%sum
+       end if
+       if (out_walls) call write_defects(1, idx, int(wo%n_wall, c_int64_t), group_walls)
     end if
     xi(1:3) = (tau/prm%dspa)**2*xi(1:3)/dble(n3)/4
     largea(1:3) = (tau/prm%dspa)*largea(1:3)/dble(n3)/2
@@ -525,6 +545,50 @@ contains
     flush(300)
     flush(400)
   end subroutine defects_and_spectrum
+
+  ! out_strings / out_walls (defects_3d.f90:232-260): one line per string, pos(3) dir(3) vel(3)
+  ! gof sl '(3F10.3,8F8.3,2I4)', or per wall, pos(3) dir(3) gof wa '(3F10.3,5F8.3,2I4)', then the
+  ! local and global group numbers (equal on one GPU; 0 without grouping).  The MPI build's
+  ! concatenated file name _string_<idx>.dat / _wall_<idx>.dat is kept.  The reference writes
+  ! the wall lines' group numbers from the wrong index (arr_wall(ns)); here each wall has its own.
+  subroutine write_defects(kind, idx, n, grouped)
+    integer, intent(in) :: kind, idx
+    integer(c_int64_t), intent(in) :: n
+    logical, intent(in) :: grouped
+    double precision, allocatable :: rec(:)
+    integer(c_int32_t), allocatable :: ids(:)
+    integer(c_int64_t) :: cnt, k, w
+    integer :: u
+    w = merge(11_c_int64_t, 8_c_int64_t, kind == 0)
+    allocate(rec(w*max(n, 1_c_int64_t)), ids(max(n, 1_c_int64_t)))
+    ids = 0
+    if (n > 0) then
+       if (kind == 0) then
+          call axg_check(axg_get_strings(h, rec, n, cnt), h, 'axg_get_strings')
+       else
+          call axg_check(axg_get_walls(h, rec, n, cnt), h, 'axg_get_walls')
+       end if
+       if (grouped) call axg_check(axg_get_groups(h, int(kind, c_int32_t), ids, n, cnt), h, 'axg_get_groups')
+    end if
+    if (kind == 0) then
+       open(newunit=u, file=trim(rootname)//'_string_'//trim(itoa(idx))//'.dat', action='write', &
+            status='replace', iostat=info)
+       if (info /= 0) call die(' error: cannot create string.dat')
+       do k = 1, n
+          write(u, '(3F10.3,8F8.3,2I4)') rec(w*(k-1)+1:w*(k-1)+6), rec(w*(k-1)+9:w*(k-1)+11), &
+               rec(w*(k-1)+7:w*(k-1)+8), ids(k), ids(k)
+       end do
+    else
+       open(newunit=u, file=trim(rootname)//'_wall_'//trim(itoa(idx))//'.dat', action='write', &
+            status='replace', iostat=info)
+       if (info /= 0) call die(' error: cannot create wall.dat')
+       do k = 1, n
+          write(u, '(3F10.3,5F8.3,2I4)') rec(w*(k-1)+1:w*(k-1)+8), ids(k), ids(k)
+       end do
+    end if
+    close(u)
+    deallocate(rec, ids)
+  end subroutine write_defects
 
   ! The rest of spectrum_estimation (spectrum.f90:290-309): estimate_covmat with min_mc = 10,
   ! max_mc = 100, tol = 0.1 on the GPU (its generator keyed by ss and idx), the covariance

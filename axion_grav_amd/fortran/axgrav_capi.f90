@@ -56,6 +56,13 @@ module axgrav_capi
      real(c_double) :: area = 0
   end type axg_wall_out
 
+  ! struct axg_group_out
+  type, bind(C) :: axg_group_out
+     integer(c_int64_t) :: n_group = 0
+     real(c_double) :: th_dist = 0
+     real(c_double) :: sum(3) = 0
+  end type axg_group_out
+
   interface
      integer(c_int) function axg_derive(k, p, zeta, width_cr, thickness_cr) bind(C, name="axg_derive")
        import :: c_int, c_double, axg_keys, axg_params
@@ -193,6 +200,29 @@ module axgrav_capi
        integer(c_int64_t), value :: cap
        integer(c_int64_t), intent(out) :: count
      end function axg_get_walls
+
+     integer(c_int) function axg_group_strings(h, width, hor_size, out) bind(C, name="axg_group_strings")
+       import :: c_int, c_ptr, c_double, axg_group_out
+       type(c_ptr), value :: h
+       real(c_double), value :: width, hor_size
+       type(axg_group_out), intent(out) :: out
+     end function axg_group_strings
+
+     integer(c_int) function axg_group_walls(h, thickness, hor_size, out) bind(C, name="axg_group_walls")
+       import :: c_int, c_ptr, c_double, axg_group_out
+       type(c_ptr), value :: h
+       real(c_double), value :: thickness, hor_size
+       type(axg_group_out), intent(out) :: out
+     end function axg_group_walls
+
+     integer(c_int) function axg_get_groups(h, kind, ids, cap, count) bind(C, name="axg_get_groups")
+       import :: c_int, c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: kind
+       integer(c_int32_t), intent(out) :: ids(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: count
+     end function axg_get_groups
 
      integer(c_int) function axg_field_spectrum(h, tau, qmid, power, conv, num, rho, nbins) &
           bind(C, name="axg_field_spectrum")

@@ -185,6 +185,29 @@ int axg_walls(axg_handle* h, axg_wall_out* out);
  * axg_get_strings. */
 int axg_get_walls(axg_handle* h, double* rec, int64_t cap, int64_t* count);
 
+/* Networks of strings or walls (grouping_strings / grouping_walls, defects_3d.f90:1235-1260,
+ * 1441-1652, 1735-1760, with the grouped sums of add_string_length 1654-1733 and add_wall_area
+ * 2145-2204).  The points (the records of the last axg_strings / axg_walls) fall into the
+ * connected components of the graph linking two points at most th_dist apart in the periodic
+ * distance lat_dist_periodic, th_dist = max(2, (width^2 hor_size)^(1/3)).  Arguments as the
+ * reference passes them: width = width_cr/dspa/tau (strings) or thickness_cr/dspa/tau (walls),
+ * hor_size = tau/dspa.  sum[0] = the total length (area); sum[1], sum[2] = the total over groups
+ * longer than 2 hor_size and 2 pi hor_size (walls: larger than pi hor_size^2 and 4 pi
+ * hor_size^2).  Groups are numbered 1.. in the order of their first record (site order).  The
+ * reference's single-rank run also merges groups through its cross-rank step (its pair index
+ * assumes the two ranks differ); this build gives the components the multi-rank run finds. */
+typedef struct {
+    int64_t n_group;
+    double th_dist;
+    double sum[3];
+} axg_group_out;
+
+int axg_group_strings(axg_handle* h, double width, double hor_size, axg_group_out* out);
+int axg_group_walls(axg_handle* h, double thickness, double hor_size, axg_group_out* out);
+/* Group number of each record of the last axg_group_strings (kind 0) or axg_group_walls
+ * (kind 1), in record order.  Size query as axg_get_strings. */
+int axg_get_groups(axg_handle* h, int32_t kind, int32_t* ids, int64_t cap, int64_t* count);
+
 /* spectrum_estimation's deterministic part (spectrum.f90:242-286 with compute_matrix_deconv
  * 344-444 and compute_conv_spec 466-567, degrade_spec = 0): the masked pi spectrum
  * deconvolved with the mask window.  tau is tau_id(idx).  power = the deconvolved

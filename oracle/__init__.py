@@ -72,6 +72,11 @@ def lib():
         L.orc_thermal_factor.argtypes = [C.c_int, i64, d, d, d, d]
         L.orc_thermal_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
         L.orc_ic_thermal.argtypes = [p, p, i64, C.c_uint64, d, d, d, d]
+        L.orc_lat_dist.restype = d
+        L.orc_lat_dist.argtypes = [p, p, i64]
+        L.orc_group_points.argtypes = [p, C.c_int, i64, i64, d, p]
+        L.orc_group_sums.restype = i64
+        L.orc_group_sums.argtypes = [p, p, i64, d, d, p, p]
         L.orc_cov_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
         L.orc_field_covariance.restype = C.c_int
         L.orc_field_covariance.argtypes = [p, i64, C.c_int, d, d, p, p, C.c_uint64, C.c_int, C.c_int, d, p, p]
@@ -331,6 +336,37 @@ def field_covariance(flags, nb, dspa, dwav, power, matn, seed, min_mc, max_mc, t
                                     _p(np.ascontiguousarray(matn, np.float64)), seed, min_mc, max_mc, tol,
                                     _p(cov), _p(pows) if want_pows else None)
     return cov, it, pows
+
+
+# ------------------------------------------------------------------ string / wall networks
+
+def lat_dist(p1, p2, N):
+    return lib().orc_lat_dist(_p(np.ascontiguousarray(p1, np.float64)), _p(np.ascontiguousarray(p2, np.float64)), N)
+
+
+def group_points(rec, N, th):
+    """Labels (smallest record index of each record's group) of records (n, >= 3) by position."""
+    rec = np.ascontiguousarray(rec, np.float64)
+    n = rec.shape[0]
+    lab = np.zeros(max(n, 1), np.int32)
+    lib().orc_group_points(_p(rec), rec.shape[1] if n else 3, n, N, th, lab.ctypes.data_as(C.c_void_p))
+    return lab[:n]
+
+
+def group_sums(label, val, t2, t3):
+    """(number of groups, group number per record (1-based), [total, above t2, above t3])."""
+    label = np.ascontiguousarray(label, np.int32)
+    n = label.shape[0]
+    ids = np.zeros(max(n, 1), np.int32)
+    out = np.zeros(3)
+    ng = lib().orc_group_sums(label.ctypes.data_as(C.c_void_p), _p(np.ascontiguousarray(val, np.float64)), n, t2, t3,
+                              ids.ctypes.data_as(C.c_void_p), _p(out))
+    return ng, ids[:n], out
+
+
+def group_threshold(width, hor_size):
+    """grouping_strings / grouping_walls threshold (defects_3d.f90:1245-1246, 1745-1746)."""
+    return max(2.0, (width ** 2.0 * hor_size) ** (1 / (2.0 + 1)))
 
 
 # ------------------------------------------------------------------ walls

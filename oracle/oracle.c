@@ -1365,3 +1365,74 @@ int64_t orc_walls(const double* phi, int64_t N, const int8_t* flags, double* rec
     free(cell); free(r); free(bad);
     return nw;
 }
+
+/* ------------------------------------------------------------ string / wall networks */
+
+/* defects_3d.f90:1645-1652 lat_dist_periodic */
+double orc_lat_dist(const double* p1, const double* p2, int64_t N)
+{
+    double s = 0;
+    for (int k = 0; k < 3; k++) {
+        double dp = fabs(p1[k] - p2[k]);
+        double m = dp < (double)N - dp ? dp : (double)N - dp;
+        s = s + m * m;
+    }
+    return sqrt(s);
+}
+
+static int64_t uf_find(int64_t* par, int64_t i)
+{
+    while (par[i] != i) {
+        par[i] = par[par[i]];
+        i = par[i];
+    }
+    return i;
+}
+
+/* grouping_strings / grouping_walls (defects_3d.f90:1235-1260, 1441-1643, 1735-1760): the
+ * connected components of the graph linking records (pos = the first 3 of `stride` doubles)
+ * at most th apart in lat_dist_periodic, by brute force over all pairs.  label[i] = the
+ * smallest record index of i's group. */
+void orc_group_points(const double* rec, int stride, int64_t n, int64_t N, double th, int32_t* label)
+{
+    int64_t* par = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) par[i] = i;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) {
+            if (!(orc_lat_dist(rec + stride * i, rec + stride * j, N) <= th)) continue;
+            int64_t a = uf_find(par, i), b = uf_find(par, j);
+            if (a == b) continue;
+            if (a < b) par[b] = a;
+            else par[a] = b;
+        }
+    for (int64_t i = 0; i < n; i++) label[i] = (int32_t)uf_find(par, i);
+    free(par);
+}
+
+/* add_string_length / add_wall_area with grouping (defects_3d.f90:1676-1702, 2172-2200):
+ * groups numbered by their first record, each group's values summed in record order, then
+ * out[0] = sum of all groups, out[1] = of groups above t2, out[2] = of groups above t3.
+ * Returns the number of groups. */
+int64_t orc_group_sums(const int32_t* label, const double* val, int64_t n, double t2, double t3, int32_t* ids,
+                       double* out)
+{
+    double* g = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    int64_t ng = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (label[i] == i) {
+            g[ng] = 0;
+            ids[i] = (int32_t)(++ng);
+        } else {
+            ids[i] = ids[label[i]];
+        }
+        g[ids[i] - 1] += val[i];
+    }
+    out[0] = out[1] = out[2] = 0;
+    for (int64_t k = 0; k < ng; k++) {
+        out[0] = out[0] + g[k];
+        if (g[k] > t2) out[1] = out[1] + g[k];
+        if (g[k] > t3) out[2] = out[2] + g[k];
+    }
+    free(g);
+    return ng;
+}

@@ -219,3 +219,53 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
         assert np.allclose(g[:, 3], o["power"], rtol=1e-8, atol=0)
         raw = np.fromfile(rd / "run_phi_2.chk", dtype=np.complex128)
         assert np.array_equal(raw[: N ** 3].reshape(N, N, N), phi2)
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_grouped_networks_and_defect_files(tmp_path):
+    """group_strings = group_walls = T with out_strings = out_walls = T: the string and wall
+    parameter columns hold the grouped sums (all, long strings / large walls, longer still) of
+    the oracle's grouping of the oracle's fits with the driver's thresholds, and
+    _string_<idx>.dat / _wall_<idx>.dat list every fit with its group number."""
+    print("[case] fortran driver grouping N=32", flush=True)
+    N = 32
+    d = O.derive(SIGMA, 90.0, 1e-30, 2.0, 4.0, N)
+    phi0, _ = O.ic_flat(N, 77, 1.0, d["dwav"])
+    rng = np.random.default_rng(5)
+    pi0 = 2.0 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    np.concatenate([phi0.ravel(), pi0.ravel()]).tofile(tmp_path / "ic.raw")
+    r = run(tmp_path, ini_text(str(tmp_path), ic_file=str(tmp_path / "ic.raw"), group_strings="T", group_walls="T",
+                               out_strings="T", out_walls="T", spec_est="F", grav_spec="F"), "-b", "100")
+    assert "threshold for string grouping" in r.stdout and "threshold for wall grouping" in r.stdout
+    P = dict(d, kappa=1e-30, initial_loop=True)
+    taus = [1.0, 1.5, 2.0]
+    states = oracle_run(N, P, phi0, pi0, taus, 1.0)
+    sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
+    for idx in (1, 2):
+        phi, pi, _ = states[idx]
+        tau = taus[idx]
+        flags, counts = O.classify(phi)
+        srec, _ = O.strings(phi, pi, flags, 1, 2 + idx, d["dspa"])
+        wrec, _ = O.walls(phi, flags)
+        hor = tau / d["dspa"]
+        th_s = O.group_threshold(d["width_cr"] / d["dspa"] / tau, hor)
+        th_w = O.group_threshold(d["thickness_cr"] / d["dspa"] / tau, hor)
+        ng_s, ids_s, s_s = O.group_sums(O.group_points(srec, N, th_s), srec[:, 7], 2 * hor, 2 * PI_T * hor)
+        ng_w, ids_w, s_w = O.group_sums(O.group_points(wrec, N, th_w), wrec[:, 7], PI_T * hor ** 2,
+                                        4 * PI_T * hor ** 2)
+        # kappa ~ 0 makes the wall thickness, and so the wall threshold, astronomically large:
+        # one network of walls; the strings split into networks at idx 2
+        assert th_w > 1e30 and ng_w == 1 and ng_s == idx
+        xi = (tau / d["dspa"]) ** 2 * s_s / N ** 3 / 4
+        largea = (tau / d["dspa"]) * s_w / N ** 3 / 2
+        assert np.allclose(sw[idx - 1, 3:6], xi, rtol=1e-6), (sw[idx - 1, 3:6], xi)
+        assert np.allclose(sw[idx - 1, 6:9], largea, rtol=1e-9), (sw[idx - 1, 6:9], largea)
+        # the per-defect files: the same fits to the printed 3 decimals, the same group numbers
+        fs = read_rows(tmp_path / f"run_string_{idx}.dat")
+        fw = read_rows(tmp_path / f"run_wall_{idx}.dat")
+        assert fs.shape == (counts[0], 13) and fw.shape == (counts[1], 10)
+        assert np.allclose(fs[:, 0:6], srec[:, 0:6], atol=1.5e-3) and np.allclose(fs[:, 9:11], srec[:, 6:8], atol=1.5e-3)
+        assert np.allclose(fw[:, 0:8], wrec[:, 0:8], atol=1.5e-3)
+        assert np.array_equal(fs[:, 11], ids_s) and np.array_equal(fs[:, 12], ids_s)
+        assert np.array_equal(fw[:, 8], ids_w) and np.array_equal(fw[:, 9], ids_w)

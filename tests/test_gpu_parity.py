@@ -384,6 +384,53 @@ def test_walls_match_oracle(prec):
     assert abs(out["area"] - osums[0]) <= 1e-12 * osums[0]
 
 
+@pytest.mark.parametrize("N,kmax,width,hor,route", [(32, 1.0, 1.0, 2.0, "box"), (64, 1.5, 1.0, 3.0, "box"),
+                                                      (64, 2.0, 3.0, 2.0, "box"), (64, None, 40.0, 3.0, "pairs"),
+                                                      (32, 1.0, 1e6, 2.0, "one group")])
+def test_grouping_matches_oracle(N, kmax, width, hor, route):
+    """String and wall networks on the GPU against the oracle's brute-force grouping of the
+    same records: identical group numbers per record (integer work, bit-exact) and identical
+    grouped sums (the same values summed in the same order).  The thresholds exercise the
+    three routes: the cell box (th_dist of 2-3.2 cells), every pair (16.9 cells, on a field of
+    four straight strings 32 cells apart and two wall sheets) and the single group (th_dist
+    beyond the largest periodic distance, as for walls at kappa ~ 0)."""
+    say("grouping", N, kmax, width, hor, route)
+    prm, P = params(N, grav_spec=False, id_strings=True, id_walls=True, spec_est=True)
+    rng = np.random.default_rng(6)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    if kmax is None:
+        x = np.arange(N) + 0.3
+        phi = np.sin(2 * np.pi * x[None, None, :] / N) + 1j * np.sin(2 * np.pi * x[None, :, None] / N)
+        phi = np.ascontiguousarray(np.broadcast_to(phi, (N, N, N)))
+    else:
+        phi, _ = ic(N, 17, kmax)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        L.defects()
+        with pytest.raises(RuntimeError):
+            L.group(0, width, hor)  # no fits yet
+        L.strings(5, 6)
+        srec = L.string_records()
+        gs = L.group(0, width, hor)
+        L.walls()
+        wrec = L.wall_records()
+        gw = L.group(1, width, hor)
+        gs2 = L.group(0, width, hor)
+    assert np.array_equal(gs["ids"], gs2["ids"]) and np.array_equal(gs["sum"], gs2["sum"])
+    th = O.group_threshold(width, hor)
+    for g, rec, t2, t3 in ((gs, srec, 2 * hor, 2 * PI_T * hor), (gw, wrec, PI_T * hor ** 2, 4 * PI_T * hor ** 2)):
+        assert g["th_dist"] == th and len(g["ids"]) == len(rec) > 50
+        lab = O.group_points(rec, N, th)
+        ng, ids, s = O.group_sums(lab, rec[:, 7], t2, t3)
+        print(f"  {len(rec)} records, {ng} groups, sums {s}")
+        assert g["n_group"] == ng and np.array_equal(g["ids"], ids)
+        assert (ng == 1) == (route == "one group")
+        assert np.array_equal(g["sum"], s)
+
+
+PI_T = 3.14159265359
+
+
 def test_string_velocity_of_moving_string():
     """A straight string along z moving in x at 0.3: phi = (X - X0 - u tau) + i (Y - Y0) in
     physical units (pi = -u), so the fitted velocity of each of its cells is (u, 0, 0) to

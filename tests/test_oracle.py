@@ -490,6 +490,73 @@ def test_cov_matches_gaussian_variance():
     assert np.abs(corr - np.eye(nb)).max() < 0.2
 
 
+# ------------------------------------------------------------------ networks (defects_3d.f90:1235-2204)
+
+def test_lat_dist_periodic():
+    """The nearest periodic image on each axis, also for positions half a cell outside [0, N)."""
+    N = 16
+    assert O.lat_dist([0.5, 0.5, 0.5], [15.5, 0.5, 0.5], N) == 1.0
+    assert abs(O.lat_dist([-0.25, 3.0, 8.0], [16.25, 4.0, 8.0], N) - np.sqrt(0.25 + 1.0)) < 1e-15
+    assert abs(O.lat_dist([1.0, 1.0, 1.0], [4.0, 5.0, 13.0], N) - np.sqrt(9 + 16 + 16)) < 1e-14
+
+
+def components(pos, N, th):
+    """scipy's periodic k-d tree pairs (distance <= th) and connected components, as labels =
+    the smallest index of each component."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    from scipy.spatial import cKDTree
+
+    n = len(pos)
+    pairs = cKDTree(np.mod(pos, N), boxsize=N).query_pairs(th, output_type="ndarray")
+    A = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n))
+    _, comp = connected_components(A, directed=False)
+    first = {}
+    for i, c in enumerate(comp):
+        first.setdefault(c, i)
+    return np.array([first[c] for c in comp], np.int32)
+
+
+@pytest.mark.parametrize("n,th", [(300, 2.0), (800, 1.3), (1500, 1.05)])
+def test_group_points_matches_periodic_kdtree(n, th):
+    """Brute-force grouping against scipy's periodic k-d tree and connected components; points
+    sit up to 0.45 outside their cell like fitted defect positions, some beyond [0, N)."""
+    N = 16
+    rng = np.random.default_rng(n)
+    cells = rng.choice(N ** 3, size=n, replace=False)
+    base = np.stack([cells % N, (cells // N) % N, cells // (N * N)], axis=1) + 0.5
+    pos = base + rng.uniform(-0.95, 0.95, size=(n, 3))
+    rec = np.concatenate([pos, rng.random((n, 8))], axis=1)
+    lab = O.group_points(rec, N, th)
+    assert np.array_equal(lab, components(pos, N, th))
+    assert 1 < len(set(lab)) < n  # neither all apart nor all joined
+
+
+def test_group_points_chains_and_threshold():
+    """Points 1.5 apart along a line join into one chain through the periodic boundary; a
+    parallel line 2.01 away stays apart at th = 2 and joins at th = 2.02."""
+    N = 16
+    a = np.array([[x + 0.25, 4.0, 4.0] for x in np.arange(0, 16, 1.5)])
+    b = a + [0.0, 2.01, 0.0]
+    rec = np.concatenate([a, b])
+    lab = O.group_points(rec, N, 2.0)
+    assert len(set(lab)) == 2 and np.all(lab[: len(a)] == 0) and np.all(lab[len(a):] == len(a))
+    assert len(set(O.group_points(rec, N, 2.02))) == 1
+    # the chain closes through the boundary: its first and last points are 16 - 15.25 + 0.25 = 1 apart
+    assert O.lat_dist(a[0], a[-1], N) == 1.0
+
+
+def test_group_sums_nested_thresholds():
+    """Per-group sums in record order, then totals over all groups and over those above each
+    threshold; groups numbered by their first record."""
+    label = np.array([0, 1, 0, 3, 1, 3, 3], np.int32)
+    val = np.array([1.0, 2.0, 3.0, 0.5, 4.0, 0.25, 0.25])
+    ng, ids, s = O.group_sums(label, val, 2.0, 5.0)
+    assert ng == 3 and list(ids) == [1, 2, 1, 3, 2, 3, 3]
+    assert list(s) == [11.0, 10.0, 6.0]  # groups: 4, 6, 1
+    assert O.group_threshold(0.5, 10.0) == 2.0 and abs(O.group_threshold(3.0, 8.0) - 72 ** (1 / 3)) < 1e-14
+
+
 # ------------------------------------------------------------------ defects (defects_3d.f90:309-349)
 
 def classify_cell(ph, ids=True, idw=True, spec=True):
