@@ -65,6 +65,11 @@ class GroupOut(C.Structure):
     _fields_ = [("n_group", C.c_int64), ("th_dist", C.c_double), ("sum", C.c_double * 3)]
 
 
+class GradOut(C.Structure):
+    _fields_ = [("J", C.c_double), ("dJ_dsigma2", C.c_double), ("dJ_dkappa0", C.c_double),
+                ("dJ_dlnamp", C.c_double), ("forward_ms", C.c_double), ("adjoint_ms", C.c_double)]
+
+
 class StringOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_vel", C.c_int64), ("length", C.c_double), ("vel", C.c_double * 3)]
 
@@ -74,7 +79,7 @@ EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
            "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls",
            "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
-           "axg_field_covariance", "axg_energy", "axg_tune", "axg_profile")
+           "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_energy", "axg_tune", "axg_profile")
 
 _LIB = None
 
@@ -113,6 +118,8 @@ def lib():
             "axg_get_groups": [p, i32, p, i64, p],
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_field_covariance": [p, C.c_uint64, i32, i32, d, p, i64, p],
+            "axg_gradient": [p, d, d, i32, i32, d, d, C.POINTER(GradOut)],
+            "axg_get_adjoint": [p, p, p, i64],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
             "axg_tune": [p, i32, p, p, i32],
@@ -348,6 +355,22 @@ class Lattice:
         self._check(lib().axg_field_covariance(self._h, seed, min_mc, max_mc, tol, _ptr(cov), len(cov), C.byref(it)),
                     "axg_field_covariance")
         return cov, it.value
+
+    def gradient(self, t1, dt, nsteps, K, qlo, qhi):
+        """Run nsteps from t1 with the GW accumulators from zero and differentiate the GW band
+        energy J (power summed over qlo <= qmid <= qhi): dict(J, dJ_dsigma2, dJ_dkappa0,
+        dJ_dlnamp, forward_ms, adjoint_ms)."""
+        o = GradOut()
+        self._check(lib().axg_gradient(self._h, t1, dt, nsteps, K, qlo, qhi, C.byref(o)), "axg_gradient")
+        return {k: getattr(o, k) for k, _ in GradOut._fields_}
+
+    def adjoint(self):
+        """(phibar, pibar) of the last gradient: dJ/dRe + i dJ/dIm of the initial fields."""
+        N = self.params.ngrid
+        a = np.zeros((N, N, N), np.complex128)
+        b = np.zeros((N, N, N), np.complex128)
+        self._check(lib().axg_get_adjoint(self._h, _ptr(a), _ptr(b), N ** 3), "axg_get_adjoint")
+        return a, b
 
     TUNE_CONFIGS = ("fused: built-in", "fused: wide blocks", "fused: other sites per thread",
                     "fused: other sites, wide blocks", "y: 128-byte tiles", "y: 64-byte tiles")

@@ -1,0 +1,369 @@
+// Adjoint of the leapfrog + GW step for the gradient of a band of the GW spectrum
+// J = sum over bins b in the band of fact q_b^3 / denom_b * numer_b, numer_b the sum of |A_m(k)|^2
+// over the bin's modes (grav_spec_estimation, grav_waves.f90:257-305).
+//
+// Conventions: a complex field z = a + ib has the adjoint zbar = dJ/da + i dJ/db, so the adjoint
+// of multiplication by a complex c is multiplication by conj(c), and of a real-linear map its
+// transpose.  The GW accumulators are linear in the six real sources S_c(x) = 2 Re(g_i conj g_j)
+// (g = central-difference gradient of phi) through D_c = sum_x S_c(x) e^{+ikx} on the half
+// spectrum kx <= N/2 and the per-component update of grav_waves.f90:228-242.  The adjoint of
+// one kick's update, for the accumulator adjoints abar of the final state:
+//   pdbar = c abar11 + s (abar21 [+ abar12 with the index quirk]),  ndbar = c abar12 + s abar22,
+//   Dbar  = conj(P) pdbar + P ndbar,  P = pol dw,  c, s = cos, sin(q tau);
+// under the quirk (A(2,1) <- A(1,2) + ..., A(1,2) <- new A(2,1) + ...) the adjoint state after
+// the last update going backward becomes abar21 = 0, abar12 = abar21 + abar12, constant
+// thereafter.  Sbar_c(x) = Re sum_half Dbar_c(k) e^{-ikx} = sum_full H_c(k) e^{-ikx} with the
+// Hermitian H_c(k) = (Dbar_c(k)[k in half] + conj Dbar_c(-k)[-k in half]) / 2, so the pair
+// H_a + i H_b transforms to Sbar_a + i Sbar_b in one complex FFT.
+#include "axg_internal.h"
+
+namespace axg {
+
+namespace {
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// pol_c = p_i p_j (halved when i == j) of grav_waves.f90:157-191 for the six (i, j) in the
+// order (1,1) (2,1) (2,2) (3,1) (3,2) (3,3); q = |n| dwav.  pol = 0 at k = 0.
+__device__ void pol_all(int nx, int ny, int nz, double dwav, double2 pol[6], double& q)
+{
+#pragma clang fp contract(off)
+    double qh[3] = {(double)nx, (double)ny, (double)nz};
+    const double qn = sqrt(qh[0] * qh[0] + qh[1] * qh[1] + qh[2] * qh[2]);
+    q = qn * dwav;
+    if (!(qn > 0)) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) pol[c] = double2{0, 0};
+        return;
+    }
+    qh[0] = qh[0] / qn;
+    qh[1] = qh[1] / qn;
+    qh[2] = qh[2] / qn;
+    const double r = sqrt(qh[0] * qh[0] + qh[1] * qh[1]);
+    double cc = 1, ss = 0;
+    if (r > 0) {
+        cc = qh[0] / r;
+        ss = qh[1] / r;
+    }
+    const double2 p[3] = {{qh[2] * cc, -ss}, {qh[2] * ss, cc}, {-r, 0}};
+    const int ii[6] = {0, 1, 1, 2, 2, 2}, jj[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const double2 a = p[ii[c]], b = p[jj[c]];
+        double2 v = {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+        if (ii[c] == jj[c]) v = double2{v.x / 2, v.y / 2};
+        pol[c] = v;
+    }
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ double2 cconj(double2 a) { return {a.x, -a.y}; }
+
+// Dbar_c at one stored mode (half index hk) for all six components.
+__device__ void dbar_all(int N, int kx, int ky, int kz, const double2* __restrict__ abar, int64_t ms, int64_t hk,
+                         const AdjStepScalars& g, double2 out[6])
+{
+#pragma clang fp contract(off)
+    double2 pol[6];
+    double q;
+    pol_all(wrapi(kx, N), wrapi(ky, N), wrapi(kz, N), g.dwav, pol, q);
+    const double ph = q * g.tau;
+    const double c = cos(ph), s = sin(ph);
+    const double2 a11 = abar[hk], a21 = abar[ms + hk], a12 = abar[2 * ms + hk], a22 = abar[3 * ms + hk];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double2 pdb, ndb;
+        if (g.fix_swap) {
+            pdb = {c * a11.x + s * a21.x, c * a11.y + s * a21.y};
+            ndb = {c * a12.x + s * a22.x, c * a12.y + s * a22.y};
+        } else if (g.last && k == 5) {
+            // the last update of the run: the final adjoints themselves
+            pdb = {c * a11.x + s * (a21.x + a12.x), c * a11.y + s * (a21.y + a12.y)};
+            ndb = {c * a12.x + s * a22.x, c * a12.y + s * a22.y};
+        } else {
+            // every earlier update: abar21 = 0, abar12 = abar21 + abar12 of the final state
+            const double2 m12 = {a21.x + a12.x, a21.y + a12.y};
+            pdb = {c * a11.x + s * m12.x, c * a11.y + s * m12.y};
+            ndb = {c * m12.x + s * a22.x, c * m12.y + s * a22.y};
+        }
+        const double2 P = {pol[k].x * g.dw, pol[k].y * g.dw};
+        const double2 u = cmul(cconj(P), pdb), v = cmul(P, ndb);
+        out[k] = double2{u.x + v.x, u.y + v.y};
+    }
+}
+
+// W_p(k) = H_{2p}(k) + i H_{2p+1}(k) over the full grid (x fastest), p = 0, 1, 2.
+__global__ void adj_modes_kernel(int N, const double2* __restrict__ abar, AdjStepScalars g, double2* __restrict__ W0,
+                                 double2* __restrict__ W1, double2* __restrict__ W2)
+{
+#pragma clang fp contract(off)
+    const int nh = N / 2 + 1;
+    const int64_t NN = (int64_t)N * N, n3 = NN * N, ms = NN * nh;
+    for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
+        double2 H[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) H[c] = double2{0, 0};
+        double2 d[6];
+        if (x <= N / 2) {
+            dbar_all(N, x, y, z, abar, ms, ((int64_t)z * N + y) * nh + x, g, d);
+#pragma unroll
+            for (int c = 0; c < 6; c++) H[c] = d[c];
+        }
+        const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
+        if (xm <= N / 2) {
+            dbar_all(N, xm, ym, zm, abar, ms, ((int64_t)zm * N + ym) * nh + xm, g, d);
+#pragma unroll
+            for (int c = 0; c < 6; c++) H[c] = double2{H[c].x + d[c].x, H[c].y - d[c].y};
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) H[c] = double2{H[c].x / 2, H[c].y / 2};
+        W0[lin] = double2{H[0].x - H[1].y, H[0].y + H[1].x};
+        W1[lin] = double2{H[2].x - H[3].y, H[2].y + H[3].x};
+        W2[lin] = double2{H[4].x - H[5].y, H[4].y + H[5].x};
+    }
+}
+
+// abar_m(k) = 2 w_b mult(kx) A_m(k) in place on the packed FP64 copies [m][kz][ky][kx <= N/2]:
+// d|A|^2 = 2 Re(conj A dA), summed over the full spectrum, whose modes -k mirror the stored
+// ones with 0 < kx < N/2 (mult 2) and are stored themselves on the planes kx = 0, N/2 (mult 1).
+__global__ void adj_seed_kernel(int N, double2* __restrict__ abar, const uint16_t* __restrict__ bin_of_n2,
+                                const double* __restrict__ w)
+{
+    const int nh = N / 2 + 1;
+    const int64_t tot = (int64_t)N * N * nh;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % nh);
+        const int64_t r = i / nh;
+        const int ky = (int)(r % N), kz = (int)(r / N);
+        const int64_t nx = wrapi(kx, N), ny = wrapi(ky, N), nz = wrapi(kz, N);
+        const int64_t n2 = nx * nx + ny * ny + nz * nz;
+        const int b = n2 > 0 ? bin_of_n2[n2] : 0;
+        const double f = b > 0 ? 2 * w[b - 1] * ((kx == 0 || kx == N / 2) ? 1.0 : 2.0) : 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const double2 a = abar[m * tot + i];
+            abar[m * tot + i] = double2{a.x * f, a.y * f};
+        }
+    }
+}
+
+template <typename T, bool COMP>
+__device__ __forceinline__ double2 ldphi(const typename V2<T>::type* phi, const short2* cphi, int64_t i)
+{
+    return ldf<T, COMP>(phi, cphi, i);
+}
+
+// gbar_i(x) = sum_j dS_ij/dg_i Sbar_ij: 4 Sbar_ii g_i + 2 sum_{j != i} Sbar_ij g_j, written over
+// W (W0 <- gbar_x, W1 <- gbar_y, W2 <- gbar_z); W holds Sbar as (S11 + i S21, S22 + i S31,
+// S32 + i S33).
+template <typename T, bool COMP>
+__global__ void adj_gbar_kernel(int N, const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi,
+                                double inv_d, double2* __restrict__ W0, double2* __restrict__ W1,
+                                double2* __restrict__ W2)
+{
+#pragma clang fp contract(off)
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        const int64_t base = i - x;
+        const int xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+        const int yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+        const int zp = z + 1 == N ? 0 : z + 1, zm = z == 0 ? N - 1 : z - 1;
+        const double2 a1 = ldphi<T, COMP>(phi, cphi, base + xm), a2 = ldphi<T, COMP>(phi, cphi, base + xp);
+        const double2 b1 = ldphi<T, COMP>(phi, cphi, (int64_t)z * NN + (int64_t)ym * N + x);
+        const double2 b2 = ldphi<T, COMP>(phi, cphi, (int64_t)z * NN + (int64_t)yp * N + x);
+        const double2 c1 = ldphi<T, COMP>(phi, cphi, (int64_t)zm * NN + (int64_t)y * N + x);
+        const double2 c2 = ldphi<T, COMP>(phi, cphi, (int64_t)zp * NN + (int64_t)y * N + x);
+        const double2 g1 = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+        const double2 g2 = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+        const double2 g3 = {(c2.x - c1.x) * inv_d, (c2.y - c1.y) * inv_d};
+        const double2 w0 = W0[i], w1 = W1[i], w2 = W2[i];
+        const double s11 = w0.x, s21 = w0.y, s22 = w1.x, s31 = w1.y, s32 = w2.x, s33 = w2.y;
+        W0[i] = double2{4 * s11 * g1.x + 2 * s21 * g2.x + 2 * s31 * g3.x, 4 * s11 * g1.y + 2 * s21 * g2.y + 2 * s31 * g3.y};
+        W1[i] = double2{2 * s21 * g1.x + 4 * s22 * g2.x + 2 * s32 * g3.x, 2 * s21 * g1.y + 4 * s22 * g2.y + 2 * s32 * g3.y};
+        W2[i] = double2{2 * s31 * g1.x + 2 * s32 * g2.x + 4 * s33 * g3.x, 2 * s31 * g1.y + 2 * s32 * g2.y + 4 * s33 * g3.y};
+    }
+}
+
+// Adjoint of one kick (and of the source's central differences):
+//   phibar(y) += inv_d sum_i (gbar_i(y - e_i) - gbar_i(y + e_i)) + f2 Lap(pibar)(y) + f3 H(phi(y)) pibar(y),
+// H the symmetric Jacobian of dVdphi as a map of (Re, Im).  Block partial sums of the parameter
+// derivatives: [0] = sum <pibar, V'_0(phi)> (V' without the tilt), [1] = sum <pibar, (-a, b)>.
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(256) adj_kick_kernel(int N, const typename V2<T>::type* __restrict__ phi,
+                                                       const short2* __restrict__ cphi,
+                                                       const double2* __restrict__ pibar, double2* __restrict__ phibar,
+                                                       const double2* __restrict__ G0, const double2* __restrict__ G1,
+                                                       const double2* __restrict__ G2, KickScalars k, double inv_d,
+                                                       int use_g, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[2][256];
+    const int64_t NN = (int64_t)N * N, n3 = NN * N;
+    double acc0 = 0, acc1 = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
+        const int64_t base = i - x;
+        const int xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+        const int yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+        const int zp = z + 1 == N ? 0 : z + 1, zm = z == 0 ? N - 1 : z - 1;
+        const int64_t ixm = base + xm, ixp = base + xp;
+        const int64_t iym = (int64_t)z * NN + (int64_t)ym * N + x, iyp = (int64_t)z * NN + (int64_t)yp * N + x;
+        const int64_t izm = (int64_t)zm * NN + (int64_t)y * N + x, izp = (int64_t)zp * NN + (int64_t)y * N + x;
+        const double2 pb = pibar[i];
+        const double2 pa1 = pibar[ixm], pa2 = pibar[ixp], pb1 = pibar[iym], pb2 = pibar[iyp], pc1 = pibar[izm],
+                      pc2 = pibar[izp];
+        const double lr = pa1.x + pa2.x + pb1.x + pb2.x + pc1.x + pc2.x - 6 * pb.x;
+        const double li = pa1.y + pa2.y + pb1.y + pb2.y + pc1.y + pc2.y - 6 * pb.y;
+        const double2 c = ldphi<T, COMP>(phi, cphi, i);
+        const double a = c.x, b = c.y;
+        const double s2 = a * a + b * b;
+        const double g = 2 * s2 + k.m2;
+        const double hrr = g + 4 * (a * a) - k.kt, hii = g + 4 * (b * b) + k.kt, hri = 4 * (a * b);
+        double fr = phibar[i].x + k.f2 * lr + k.f3 * (hrr * pb.x + hri * pb.y);
+        double fi = phibar[i].y + k.f2 * li + k.f3 * (hri * pb.x + hii * pb.y);
+        if (use_g) {
+            const double2 gx1 = G0[ixm], gx2 = G0[ixp], gy1 = G1[iym], gy2 = G1[iyp], gz1 = G2[izm], gz2 = G2[izp];
+            fr = fr + inv_d * ((gx1.x - gx2.x) + (gy1.x - gy2.x) + (gz1.x - gz2.x));
+            fi = fi + inv_d * ((gx1.y - gx2.y) + (gy1.y - gy2.y) + (gz1.y - gz2.y));
+        }
+        phibar[i] = double2{fr, fi};
+        const double vr = g * a - k.ma2 / 2, vi = g * b;
+        acc0 = acc0 + (pb.x * vr + pb.y * vi);
+        acc1 = acc1 + (pb.y * b - pb.x * a);
+    }
+    sh[0][threadIdx.x] = acc0;
+    sh[1][threadIdx.x] = acc1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) partial[(int64_t)blockIdx.x * 2 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// pibar <- f1 pibar + h phibar (the kick's pibar scaling and the preceding drift's adjoint).
+__global__ void adj_drift_kernel(int64_t n3, double f1, double hd, const double2* __restrict__ phibar,
+                                 double2* __restrict__ pibar)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 p = pibar[i], f = phibar[i];
+        pibar[i] = double2{f1 * p.x + hd * f.x, f1 * p.y + hd * f.y};
+    }
+}
+
+// total[j] += scale[j] * col[j], j = 0, 1 (one thread: a fixed order).
+__global__ void adj_accumulate_kernel(const double* __restrict__ col, double s0, double s1, double* __restrict__ total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        total[0] = total[0] + s0 * col[0];
+        total[1] = total[1] + s1 * col[1];
+    }
+}
+
+// Block partial sums of <phibar, phi> + <pibar, pi>.
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(256) adj_dot_kernel(int64_t n3, const typename V2<T>::type* __restrict__ phi,
+                                                      const short2* __restrict__ cphi,
+                                                      const typename V2<T>::type* __restrict__ pi,
+                                                      const short2* __restrict__ cpi, const double2* __restrict__ phibar,
+                                                      const double2* __restrict__ pibar, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[256];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 f = ldphi<T, COMP>(phi, cphi, i), p = ldphi<T, COMP>(pi, cpi, i);
+        const double2 fb = phibar[i], pb = pibar[i];
+        acc = acc + (fb.x * f.x + fb.y * f.y) + (pb.x * p.x + pb.y * p.y);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+}  // namespace
+
+constexpr int ADJ_GRID = ADJ_GRID_H;
+
+void launch_adj_seed(hipStream_t s, int N, double2* abar, const uint16_t* bin_of_n2, const double* w)
+{
+    hipLaunchKernelGGL(adj_seed_kernel, dim3(ADJ_GRID), dim3(256), 0, s, N, abar, bin_of_n2, w);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, double2* W0, double2* W1,
+                      double2* W2)
+{
+    hipLaunchKernelGGL(adj_modes_kernel, dim3(ADJ_GRID), dim3(256), 0, s, N, abar, g, W0, W1, W2);
+    AXG_HIP(hipGetLastError());
+}
+
+template <typename T, bool COMP>
+void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, double2* W0, double2* W1,
+                     double2* W2)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((adj_gbar_kernel<T, COMP>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, inv_d, W0, W1, W2);
+    AXG_HIP(hipGetLastError());
+}
+
+template <typename T, bool COMP>
+void launch_adj_kick(hipStream_t s, int N, const void* phi, const void* cphi, const double2* pibar, double2* phibar,
+                     const double2* G0, const double2* G1, const double2* G2, const KickScalars& k, double inv_d,
+                     bool use_g, double* partial, double* colsum)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((adj_kick_kernel<T, COMP>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, pibar, phibar, G0, G1, G2, k, inv_d, (int)use_g, partial);
+    AXG_HIP(hipGetLastError());
+    sum_columns(s, partial, ADJ_GRID, 2, colsum);
+}
+
+void launch_adj_drift(hipStream_t s, int64_t n3, double f1, double hd, const double2* phibar, double2* pibar)
+{
+    hipLaunchKernelGGL(adj_drift_kernel, dim3(ADJ_GRID), dim3(256), 0, s, n3, f1, hd, phibar, pibar);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_adj_accumulate(hipStream_t s, const double* col, double s0, double s1, double* total)
+{
+    hipLaunchKernelGGL(adj_accumulate_kernel, dim3(1), dim3(64), 0, s, col, s0, s1, total);
+    AXG_HIP(hipGetLastError());
+}
+
+template <typename T, bool COMP>
+void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi, const void* pi, const void* cpi,
+                    const double2* phibar, const double2* pibar, double* partial, double* out)
+{
+    using T2 = typename V2<T>::type;
+    hipLaunchKernelGGL((adj_dot_kernel<T, COMP>), dim3(ADJ_GRID), dim3(256), 0, s, n3, (const T2*)phi,
+                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, phibar, pibar, partial);
+    AXG_HIP(hipGetLastError());
+    sum_columns(s, partial, ADJ_GRID, 1, out);
+}
+
+#define AXG_INST_ADJ(T, C)                                                                                           \
+    template void launch_adj_gbar<T, C>(hipStream_t, int, const void*, const void*, double, double2*, double2*,      \
+                                        double2*);                                                                  \
+    template void launch_adj_kick<T, C>(hipStream_t, int, const void*, const void*, const double2*, double2*,        \
+                                        const double2*, const double2*, const double2*, const KickScalars&, double, \
+                                        bool, double*, double*);                                                     \
+    template void launch_adj_dot<T, C>(hipStream_t, int64_t, const void*, const void*, const void*, const void*,     \
+                                       const double2*, const double2*, double*, double*);
+AXG_INST_ADJ(double, false)
+AXG_INST_ADJ(float, false)
+AXG_INST_ADJ(float, true)
+
+}  // namespace axg

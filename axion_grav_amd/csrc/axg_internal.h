@@ -115,6 +115,27 @@ struct GwStepScalars {
     double tau, dw, dwav;
     int fix_swap;
 };
+// One kick's scalars for the adjoint of its GW update (adjoint.hip); last = the run's final kick.
+struct AdjStepScalars {
+    double tau, dw, dwav;
+    int fix_swap, last;
+};
+constexpr int ADJ_GRID_H = 1024;  // blocks of the adjoint's reductions
+void launch_adj_seed(hipStream_t s, int N, double2* abar, const uint16_t* bin_of_n2, const double* w);
+void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, double2* W0, double2* W1,
+                      double2* W2);
+template <typename T, bool COMP>
+void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, double2* W0, double2* W1,
+                     double2* W2);
+template <typename T, bool COMP>
+void launch_adj_kick(hipStream_t s, int N, const void* phi, const void* cphi, const double2* pibar, double2* phibar,
+                     const double2* G0, const double2* G1, const double2* G2, const KickScalars& k, double inv_d,
+                     bool use_g, double* partial, double* colsum);
+void launch_adj_drift(hipStream_t s, int64_t n3, double f1, double hd, const double2* phibar, double2* pibar);
+void launch_adj_accumulate(hipStream_t s, const double* col, double s0, double s1, double* total);
+template <typename T, bool COMP>
+void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi, const void* pi, const void* cpi,
+                    const double2* phibar, const double2* pibar, double* partial, double* out);
 // y transform of W ([c][z][y][kx], pitch P): in place for the row layout, or out of place
 // into W2 in the tiled layout.
 template <typename T>

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+
 #include "axg_internal.h"
 
 namespace axg {
@@ -150,6 +152,13 @@ struct axg_handle {
     std::vector<int32_t> grp_ids[2];  // group number (1-based) per string / wall record
     bool have_grp[2] = {false, false};
     bool wall_bad = false;  // the last axg_walls met cells with fewer than 3 edge points
+    // adjoint (axg_gradient): phibar, pibar, two more N^3 FP64 complex arrays (the third is work),
+    // the accumulator adjoints, and the checkpoints (grown on demand)
+    double2* adj = nullptr;
+    double2* abar = nullptr;
+    void* ckpt = nullptr;
+    int64_t ckpt_cap = 0;  // checkpoints the buffer holds
+    bool have_adj = false;
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
     // the last field spectrum: deconvolved power and the count-normalised deconvolution matrix
@@ -573,8 +582,10 @@ extern "C" int axg_set_gw_state(axg_handle* h, const double* A, int64_t count)
 
 // ------------------------------------------------------------------ time stepping
 
-template <typename T, bool COMP>
-static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
+// The DKD loop; before_kick(s, t) runs before kick s (0-based) at its time t, when phi has
+// reached t and pi is still the previous kick's.
+template <typename T, bool COMP, typename Hook>
+static void run_steps_hooked(axg_handle* h, double t1, double dt, int32_t nt, Hook&& before_kick)
 {
     const int N = h->N;
     const axg_params& p = h->p;
@@ -588,6 +599,7 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
             });
             t = t + dt / 2;
         }
+        before_kick(it - 1, t);
         KickScalars k = kick_scalars(p, t, dt);
         double dtd = (it == nt) ? dt / 2 : dt;
         int nxt = 1 - h->cur;
@@ -621,6 +633,12 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
         h->cur = nxt;
         t = t + dtd;
     }
+}
+
+template <typename T, bool COMP>
+static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
+{
+    run_steps_hooked<T, COMP>(h, t1, dt, nt, [](int32_t, double) {});
 }
 
 static void advance(axg_handle* h, double t1, double dt, int32_t nt)
@@ -1412,6 +1430,193 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
         // without convergence the loop counter ends at max_mc + 1, which the reference divides by
         for (int ij = 0; ij < ncov; ij++) cov[ij] = pc[ij] / iter;
         if (iters) *iters = converged ? iter : -iter;
+    });
+}
+
+// ------------------------------------------------------------------ gradient (adjoint)
+
+template <typename T, bool COMP>
+static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, int32_t K, double qlo, double qhi,
+                          axg_grad_out* out)
+{
+    const axg_params& p = h->p;
+    const int N = h->N;
+    const int64_t n3 = h->n3;
+    const int nh = N / 2 + 1;
+    const int64_t hs = (int64_t)N * N * nh;
+    const double s2c = p.sigma_cr * p.sigma_cr;
+    const size_t fsz = h->fp32 ? 8 : 16, csz = h->comp ? 4 : 0;
+    const size_t state = (size_t)n3 * (2 * fsz + 2 * csz);
+    const int64_t nck = (nsteps - 1) / K + 1 + 2;  // kicks 0, K, 2K, ...; the initial and final states
+    if (!h->adj) {
+        h->adj = (double2*)h->alloc((size_t)4 * n3 * sizeof(double2));
+        h->abar = (double2*)h->alloc((size_t)4 * hs * sizeof(double2));
+    }
+    if (nck > h->ckpt_cap) {
+        void* q = h->ckpt;
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        h->release_one(q);
+        h->ckpt = nullptr;
+        h->ckpt_cap = 0;
+        h->ckpt = h->alloc(state * (size_t)nck);
+        h->ckpt_cap = nck;
+    }
+    double2* phibar = h->adj;
+    double2* pibar = h->adj + n3;
+    double2* W0 = h->work;
+    double2* W1 = h->adj + 2 * n3;
+    double2* W2 = h->adj + 3 * n3;
+    auto slot = [&](int64_t j) { return (char*)h->ckpt + state * (size_t)j; };
+    auto save = [&](int64_t j) {
+        char* b = slot(j);
+        AXG_HIP(hipMemcpyAsync(b, h->phi[h->cur], n3 * fsz, hipMemcpyDeviceToDevice, h->stream));
+        AXG_HIP(hipMemcpyAsync(b + n3 * fsz, h->pi, n3 * fsz, hipMemcpyDeviceToDevice, h->stream));
+        if (csz) {
+            AXG_HIP(hipMemcpyAsync(b + 2 * n3 * fsz, h->cphi[h->cur], n3 * csz, hipMemcpyDeviceToDevice, h->stream));
+            AXG_HIP(hipMemcpyAsync(b + 2 * n3 * fsz + n3 * csz, h->cpi, n3 * csz, hipMemcpyDeviceToDevice, h->stream));
+        }
+    };
+    auto load = [&](int64_t j) {
+        const char* b = slot(j);
+        AXG_HIP(hipMemcpyAsync(h->phi[h->cur], b, n3 * fsz, hipMemcpyDeviceToDevice, h->stream));
+        AXG_HIP(hipMemcpyAsync(h->pi, b + n3 * fsz, n3 * fsz, hipMemcpyDeviceToDevice, h->stream));
+        if (csz) {
+            AXG_HIP(hipMemcpyAsync(h->cphi[h->cur], b + 2 * n3 * fsz, n3 * csz, hipMemcpyDeviceToDevice, h->stream));
+            AXG_HIP(hipMemcpyAsync(h->cpi, b + 2 * n3 * fsz + n3 * csz, n3 * csz, hipMemcpyDeviceToDevice, h->stream));
+        }
+    };
+    const int64_t J_INIT = nck - 2, J_FINAL = nck - 1;
+
+    // forward: the accumulators from zero, checkpoints before kicks 0, K, 2K, ...
+    AXG_HIP(hipStreamSynchronize(h->stream));
+    const auto c0 = std::chrono::steady_clock::now();
+    AXG_HIP(hipMemsetAsync(h->A, 0, (size_t)4 * h->L.comp() * fsz, h->stream));
+    save(J_INIT);
+    std::vector<double> ts(nsteps);
+    run_steps_hooked<T, COMP>(h, t1, dt, nsteps, [&](int32_t s, double t) {
+        ts[s] = t;
+        if (s % K == 0) save(s / K);
+    });
+    save(J_FINAL);
+    AXG_HIP(hipStreamSynchronize(h->stream));
+    const auto c1 = std::chrono::steady_clock::now();
+
+    // J and its seed: w_b = fact q_b^3 / denom_b over the band, abar = 2 w mult A
+    const int nb = p.num_bin_grav;
+    std::vector<double> qm(nb), pw(nb), nu(nb), de(nb), w(nb, 0.0);
+    if (axg_gw_spectrum(h, qm.data(), pw.data(), nu.data(), de.data(), nb) != 0) throw Error(AXG_EDEVICE, h->err);
+    double J = 0;
+    {
+        const double bigh_in_mstar = 2.1332e-42 / 2.435e18;
+        double t6 = p.tau_cr * p.tau_cr * p.tau_cr;
+        t6 = t6 * t6;
+        double a4 = p.a_now * p.a_now;
+        a4 = a4 * a4;
+        const double fact = 1 / t6 / (bigh_in_mstar * bigh_in_mstar) / a4 / 3 / (AXG_TWOPI_T * AXG_TWOPI_T) * p.dv_wav;
+        for (int b = 0; b < nb; b++)
+            if (qm[b] >= qlo && qm[b] <= qhi && de[b] > 0) {
+                w[b] = fact * (qm[b] * qm[b] * qm[b]) / de[b];
+                J = J + pw[b];
+            }
+    }
+    double* d_w = h->red;
+    double* d_tot = h->red + nb;      // 2 running totals
+    double* d_col = h->red + nb + 2;  // 2 column sums
+    double* d_part = h->red + nb + 4;
+    require((size_t)(nb + 4 + 2 * ADJ_GRID_H) * 8 <= h->red_bytes, "reduction scratch too small");
+    AXG_HIP(hipMemcpyAsync(d_w, w.data(), nb * 8, hipMemcpyHostToDevice, h->stream));
+    for (int m = 0; m < 4; m++) launch_gw_pack<T>(h->stream, N, h->L, h->A, m, h->abar + m * hs, +1);
+    launch_adj_seed(h->stream, N, h->abar, h->gw_bin, d_w);
+    AXG_HIP(hipMemsetAsync(phibar, 0, (size_t)2 * n3 * sizeof(double2), h->stream));
+    AXG_HIP(hipMemsetAsync(d_tot, 0, 2 * sizeof(double), h->stream));
+
+    // backward: from the final state, undo the last half drift, then per kick s (last first):
+    // the adjoint of its GW update and of the kick, the kick undone (-dt), the exact state from
+    // the checkpoint when there is one, and the preceding drift undone with its adjoint
+    const double inv_d = 1 / (2 * p.dspa);
+    launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur], h->cphi[h->cur],
+                          h->cpi, -dt / 2);
+    for (int32_t s = nsteps - 1; s >= 0; s--) {
+        const double t = ts[s];
+        AdjStepScalars g;
+        g.tau = t;
+        g.dw = t * dt * p.dv_spa * s2c;
+        g.dwav = p.dwav;
+        g.fix_swap = p.gw_fix_swap;
+        g.last = s == nsteps - 1;
+        launch_adj_modes(h->stream, N, h->abar, g, W0, W1, W2);
+        fft3d_work(h, W0, -1);
+        fft3d_work(h, W1, -1);
+        fft3d_work(h, W2, -1);
+        launch_adj_gbar<T, COMP>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2);
+        const KickScalars k = kick_scalars(p, t, dt);
+        launch_adj_kick<T, COMP>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], pibar, phibar, W0, W1, W2, k, inv_d,
+                                 true, d_part, d_col);
+        const double t2 = t * t;
+        launch_adj_accumulate(h->stream, d_col, k.f3 / s2c, k.f3 * (t2 * t2) / s2c, d_tot);
+        // undo the kick: the same kick with -dt at the same time, phi unchanged (drift of 0)
+        const int nxt = 1 - h->cur;
+        launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
+                                   h->cpi, nullptr, kick_scalars(p, t, -dt), p.dspa, 0.0, false);
+        h->cur = nxt;
+        if (s % K == 0) load(s / K);
+        const double hd = s == 0 ? dt / 2 : dt;
+        launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
+                              h->cphi[h->cur], h->cpi, -hd);
+        launch_adj_drift(h->stream, n3, k.f1, hd, phibar, pibar);
+    }
+    // gradients: the dynamics' parts plus the explicit sigma_cr^2 in dw (J is quadratic in A)
+    load(J_INIT);
+    double* d_dot = d_col;
+    launch_adj_dot<T, COMP>(h->stream, n3, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi, phibar, pibar, d_part, d_dot);
+    double tot[2], dot = 0;
+    AXG_HIP(hipMemcpyAsync(tot, d_tot, sizeof tot, hipMemcpyDeviceToHost, h->stream));
+    AXG_HIP(hipMemcpyAsync(&dot, d_dot, sizeof dot, hipMemcpyDeviceToHost, h->stream));
+    AXG_HIP(hipStreamSynchronize(h->stream));
+    const auto c2 = std::chrono::steady_clock::now();
+    load(J_FINAL);
+    AXG_HIP(hipStreamSynchronize(h->stream));
+    h->have_adj = true;
+    out->J = J;
+    out->dJ_dsigma2 = tot[0] + 2 * J / s2c;
+    out->dJ_dkappa0 = tot[1];
+    out->dJ_dlnamp = dot;
+    out->forward_ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
+    out->adjoint_ms = std::chrono::duration<double, std::milli>(c2 - c1).count();
+}
+
+extern "C" int axg_gradient(axg_handle* h, double t1, double dt, int32_t nsteps, int32_t K, double qlo, double qhi,
+                            axg_grad_out* out)
+{
+    return guarded(h, [&] {
+        require(out != nullptr, "null output");
+        require(h->p.grav_spec && h->A != nullptr, "grav_spec is off: no GW spectrum to differentiate");
+        require(nsteps >= 1 && K >= 1, "need nsteps >= 1 and K >= 1");
+        require(dt > 0 && t1 > 0 && std::isfinite(dt) && std::isfinite(t1), "need t1 > 0 and dt > 0");
+        require(qlo <= qhi, "need qlo <= qhi");
+        h->have_adj = false;
+        if (h->comp) gradient_impl<float, true>(h, t1, dt, nsteps, K, qlo, qhi, out);
+        else if (h->fp32) gradient_impl<float, false>(h, t1, dt, nsteps, K, qlo, qhi, out);
+        else gradient_impl<double, false>(h, t1, dt, nsteps, K, qlo, qhi, out);
+        h->have_flags = false;
+        h->have_spec = false;
+        h->nstr_rec = -1;
+        h->nwall_rec = -1;
+        h->have_grp[0] = h->have_grp[1] = false;
+    });
+}
+
+extern "C" int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite)
+{
+    return guarded(h, [&] {
+        require(h->have_adj, "call axg_gradient first");
+        require(nsite == h->n3, "nsite must be N^3");
+        if (phibar)
+            AXG_HIP(hipMemcpyAsync(phibar, h->adj, (size_t)h->n3 * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        if (pibar)
+            AXG_HIP(hipMemcpyAsync(pibar, h->adj + h->n3, (size_t)h->n3 * sizeof(double2), hipMemcpyDeviceToHost,
+                                   h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
 

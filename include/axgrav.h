@@ -229,6 +229,34 @@ int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, d
 int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc, int32_t max_mc, double tol,
                          double* cov, int64_t ncov, int32_t* iters);
 
+/* Gradient of a band of the GW spectrum by the adjoint of the leapfrog + GW step (no
+ * counterpart in the reference; the step it differentiates is evolve.f90:238-339 with
+ * grav_waves.f90:193-255).  Runs nsteps DKD steps of dt from the current fields at t1 exactly as
+ * axg_advance_steps does, with the GW accumulators starting from zero, keeping (phi, pi) on the
+ * device before kicks 0, K, 2K, ...; then J = the sum of the axg_gw_spectrum power over the bins
+ * with qlo <= qmid <= qhi, and a backward sweep that undoes the steps by time reversal (each
+ * segment restarts from its checkpoint, bounding the reversal drift) while it propagates the
+ * adjoints.  dJ_dsigma2: d/d(sigma_cr^2) with dspa and the spectrum's prefactor fixed (through
+ * the kick's force, the tilt and the GW weight; for V = lambda/4 (|phi|^2 - 1)^2 in these
+ * units sigma_cr^2 = lambda/4); dJ_dkappa0: d/d(potential_kappa0); dJ_dlnamp: d/d(ln A) for
+ * the initial (phi, pi) scaled by A.  On return the fields and accumulators are those at the
+ * end of the run; the adjoints of the initial fields are available from axg_get_adjoint.
+ * forward_ms / adjoint_ms: wall time of the two sweeps. */
+typedef struct {
+    double J;
+    double dJ_dsigma2;
+    double dJ_dkappa0;
+    double dJ_dlnamp;
+    double forward_ms;
+    double adjoint_ms;
+} axg_grad_out;
+
+int axg_gradient(axg_handle* h, double t1, double dt, int32_t nsteps, int32_t K, double qlo, double qhi,
+                 axg_grad_out* out);
+/* dJ/dRe + i dJ/dIm of the initial phi and pi of the last axg_gradient, as N^3 complex
+ * (re, im) pairs, x fastest; either pointer may be NULL. */
+int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite);
+
 /* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);
