@@ -68,7 +68,8 @@ __device__ void dbar_all(int N, int kx, int ky, int kz, const double2* __restric
     double q;
     pol_all(wrapi(kx, N), wrapi(ky, N), wrapi(kz, N), g.dwav, pol, q);
     const double ph = q * g.tau;
-    const double c = cos(ph), s = sin(ph);
+    double s, c;
+    sincos(ph, &s, &c);
     const double2 a11 = abar[hk], a21 = abar[ms + hk], a12 = abar[2 * ms + hk], a22 = abar[3 * ms + hk];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -92,36 +93,50 @@ __device__ void dbar_all(int N, int kx, int ky, int kz, const double2* __restric
     }
 }
 
-// W_p(k) = H_{2p}(k) + i H_{2p+1}(k) over the full grid (x fastest), p = 0, 1, 2.
-__global__ void adj_modes_kernel(int N, const double2* __restrict__ abar, AdjStepScalars g, double2* __restrict__ W0,
-                                 double2* __restrict__ W1, double2* __restrict__ W2)
+template <typename TW>
+__device__ __forceinline__ void store_pairs(typename V2<TW>::type* __restrict__ W0, typename V2<TW>::type* __restrict__ W1,
+                                            typename V2<TW>::type* __restrict__ W2, int64_t lin, const double2 H[6])
+{
+    using W2T = typename V2<TW>::type;
+    W0[lin] = W2T{(TW)(H[0].x - H[1].y), (TW)(H[0].y + H[1].x)};
+    W1[lin] = W2T{(TW)(H[2].x - H[3].y), (TW)(H[2].y + H[3].x)};
+    W2[lin] = W2T{(TW)(H[4].x - H[5].y), (TW)(H[4].y + H[5].x)};
+}
+
+// W_p(k) = H_{2p}(k) + i H_{2p+1}(k) on the full grid (x fastest), p = 0, 1, 2, one thread per
+// stored mode (kz, ky, kx <= N/2): for 0 < kx < N/2 the mode fills H(k) = Dbar(k)/2 and its
+// mirror H(-k) = conj Dbar(k)/2; on the planes kx = 0, N/2 (closed under k -> -k) each mode fills
+// its own H(k) = (Dbar(k) + conj Dbar(-k))/2.
+template <typename TW>
+__global__ void adj_modes_kernel(int N, const double2* __restrict__ abar, AdjStepScalars g,
+                                 typename V2<TW>::type* __restrict__ W0, typename V2<TW>::type* __restrict__ W1,
+                                 typename V2<TW>::type* __restrict__ W2)
 {
 #pragma clang fp contract(off)
     const int nh = N / 2 + 1;
-    const int64_t NN = (int64_t)N * N, n3 = NN * N, ms = NN * nh;
-    for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
-         lin += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
-        double2 H[6];
+    const int64_t NN = (int64_t)N * N, ms = NN * nh;
+    for (int64_t hk = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; hk < ms;
+         hk += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(hk % nh);
+        const int64_t r = hk / nh;
+        const int y = (int)(r % N), z = (int)(r / N);
+        const int ym = (N - y) % N, zm = (N - z) % N;
+        double2 d[6], H[6];
+        dbar_all(N, x, y, z, abar, ms, hk, g, d);
+        if (x > 0 && x < N / 2) {
 #pragma unroll
-        for (int c = 0; c < 6; c++) H[c] = double2{0, 0};
-        double2 d[6];
-        if (x <= N / 2) {
-            dbar_all(N, x, y, z, abar, ms, ((int64_t)z * N + y) * nh + x, g, d);
+            for (int c = 0; c < 6; c++) H[c] = double2{d[c].x / 2, d[c].y / 2};
+            store_pairs<TW>(W0, W1, W2, ((int64_t)z * N + y) * N + x, H);
 #pragma unroll
-            for (int c = 0; c < 6; c++) H[c] = d[c];
+            for (int c = 0; c < 6; c++) H[c] = double2{d[c].x / 2, -d[c].y / 2};
+            store_pairs<TW>(W0, W1, W2, ((int64_t)zm * N + ym) * N + (N - x), H);
+        } else {
+            double2 e[6];
+            dbar_all(N, x, ym, zm, abar, ms, ((int64_t)zm * N + ym) * nh + x, g, e);
+#pragma unroll
+            for (int c = 0; c < 6; c++) H[c] = double2{(d[c].x + e[c].x) / 2, (d[c].y - e[c].y) / 2};
+            store_pairs<TW>(W0, W1, W2, ((int64_t)z * N + y) * N + x, H);
         }
-        const int xm = (N - x) % N, ym = (N - y) % N, zm = (N - z) % N;
-        if (xm <= N / 2) {
-            dbar_all(N, xm, ym, zm, abar, ms, ((int64_t)zm * N + ym) * nh + xm, g, d);
-#pragma unroll
-            for (int c = 0; c < 6; c++) H[c] = double2{H[c].x + d[c].x, H[c].y - d[c].y};
-        }
-#pragma unroll
-        for (int c = 0; c < 6; c++) H[c] = double2{H[c].x / 2, H[c].y / 2};
-        W0[lin] = double2{H[0].x - H[1].y, H[0].y + H[1].x};
-        W1[lin] = double2{H[2].x - H[3].y, H[2].y + H[3].x};
-        W2[lin] = double2{H[4].x - H[5].y, H[4].y + H[5].x};
     }
 }
 
@@ -158,12 +173,13 @@ __device__ __forceinline__ double2 ldphi(const typename V2<T>::type* phi, const 
 // gbar_i(x) = sum_j dS_ij/dg_i Sbar_ij: 4 Sbar_ii g_i + 2 sum_{j != i} Sbar_ij g_j, written over
 // W (W0 <- gbar_x, W1 <- gbar_y, W2 <- gbar_z); W holds Sbar as (S11 + i S21, S22 + i S31,
 // S32 + i S33).
-template <typename T, bool COMP>
+template <typename T, bool COMP, typename TW>
 __global__ void adj_gbar_kernel(int N, const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi,
-                                double inv_d, double2* __restrict__ W0, double2* __restrict__ W1,
-                                double2* __restrict__ W2)
+                                double inv_d, typename V2<TW>::type* __restrict__ W0, typename V2<TW>::type* __restrict__ W1,
+                                typename V2<TW>::type* __restrict__ W2)
 {
 #pragma clang fp contract(off)
+    using W2T = typename V2<TW>::type;
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
@@ -179,11 +195,14 @@ __global__ void adj_gbar_kernel(int N, const typename V2<T>::type* __restrict__ 
         const double2 g1 = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
         const double2 g2 = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
         const double2 g3 = {(c2.x - c1.x) * inv_d, (c2.y - c1.y) * inv_d};
-        const double2 w0 = W0[i], w1 = W1[i], w2 = W2[i];
+        const W2T w0 = W0[i], w1 = W1[i], w2 = W2[i];
         const double s11 = w0.x, s21 = w0.y, s22 = w1.x, s31 = w1.y, s32 = w2.x, s33 = w2.y;
-        W0[i] = double2{4 * s11 * g1.x + 2 * s21 * g2.x + 2 * s31 * g3.x, 4 * s11 * g1.y + 2 * s21 * g2.y + 2 * s31 * g3.y};
-        W1[i] = double2{2 * s21 * g1.x + 4 * s22 * g2.x + 2 * s32 * g3.x, 2 * s21 * g1.y + 4 * s22 * g2.y + 2 * s32 * g3.y};
-        W2[i] = double2{2 * s31 * g1.x + 2 * s32 * g2.x + 4 * s33 * g3.x, 2 * s31 * g1.y + 2 * s32 * g2.y + 4 * s33 * g3.y};
+        W0[i] = W2T{(TW)(4 * s11 * g1.x + 2 * s21 * g2.x + 2 * s31 * g3.x),
+                    (TW)(4 * s11 * g1.y + 2 * s21 * g2.y + 2 * s31 * g3.y)};
+        W1[i] = W2T{(TW)(2 * s21 * g1.x + 4 * s22 * g2.x + 2 * s32 * g3.x),
+                    (TW)(2 * s21 * g1.y + 4 * s22 * g2.y + 2 * s32 * g3.y)};
+        W2[i] = W2T{(TW)(2 * s31 * g1.x + 2 * s32 * g2.x + 4 * s33 * g3.x),
+                    (TW)(2 * s31 * g1.y + 2 * s32 * g2.y + 4 * s33 * g3.y)};
     }
 }
 
@@ -191,15 +210,19 @@ __global__ void adj_gbar_kernel(int N, const typename V2<T>::type* __restrict__ 
 //   phibar(y) += inv_d sum_i (gbar_i(y - e_i) - gbar_i(y + e_i)) + f2 Lap(pibar)(y) + f3 H(phi(y)) pibar(y),
 // H the symmetric Jacobian of dVdphi as a map of (Re, Im).  Block partial sums of the parameter
 // derivatives: [0] = sum <pibar, V'_0(phi)> (V' without the tilt), [1] = sum <pibar, (-a, b)>.
-template <typename T, bool COMP>
+template <typename T, bool COMP, typename TW>
 __global__ void __launch_bounds__(256) adj_kick_kernel(int N, const typename V2<T>::type* __restrict__ phi,
                                                        const short2* __restrict__ cphi,
                                                        const double2* __restrict__ pibar, double2* __restrict__ phibar,
-                                                       const double2* __restrict__ G0, const double2* __restrict__ G1,
-                                                       const double2* __restrict__ G2, KickScalars k, double inv_d,
-                                                       int use_g, double* __restrict__ partial)
+                                                       double2* __restrict__ pibar_out,
+                                                       const typename V2<TW>::type* __restrict__ G0,
+                                                       const typename V2<TW>::type* __restrict__ G1,
+                                                       const typename V2<TW>::type* __restrict__ G2, KickScalars k,
+                                                       double inv_d, double hd, double s0, double s1, int use_g,
+                                                       double* __restrict__ partial)
 {
 #pragma clang fp contract(off)
+    using W2T = typename V2<TW>::type;
     __shared__ double sh[2][256];
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
     double acc0 = 0, acc1 = 0;
@@ -225,11 +248,13 @@ __global__ void __launch_bounds__(256) adj_kick_kernel(int N, const typename V2<
         double fr = phibar[i].x + k.f2 * lr + k.f3 * (hrr * pb.x + hri * pb.y);
         double fi = phibar[i].y + k.f2 * li + k.f3 * (hri * pb.x + hii * pb.y);
         if (use_g) {
-            const double2 gx1 = G0[ixm], gx2 = G0[ixp], gy1 = G1[iym], gy2 = G1[iyp], gz1 = G2[izm], gz2 = G2[izp];
-            fr = fr + inv_d * ((gx1.x - gx2.x) + (gy1.x - gy2.x) + (gz1.x - gz2.x));
-            fi = fi + inv_d * ((gx1.y - gx2.y) + (gy1.y - gy2.y) + (gz1.y - gz2.y));
+            const W2T gx1 = G0[ixm], gx2 = G0[ixp], gy1 = G1[iym], gy2 = G1[iyp], gz1 = G2[izm], gz2 = G2[izp];
+            fr = fr + inv_d * (((double)gx1.x - gx2.x) + ((double)gy1.x - gy2.x) + ((double)gz1.x - gz2.x));
+            fi = fi + inv_d * (((double)gx1.y - gx2.y) + ((double)gy1.y - gy2.y) + ((double)gz1.y - gz2.y));
         }
         phibar[i] = double2{fr, fi};
+        // the kick scales pibar by f1; the drift before it adds hd phibar
+        pibar_out[i] = double2{k.f1 * pb.x + hd * fr, k.f1 * pb.y + hd * fi};
         const double vr = g * a - k.ma2 / 2, vi = g * b;
         acc0 = acc0 + (pb.x * vr + pb.y * vi);
         acc1 = acc1 + (pb.y * b - pb.x * a);
@@ -244,26 +269,10 @@ __global__ void __launch_bounds__(256) adj_kick_kernel(int N, const typename V2<
         }
         __syncthreads();
     }
-    if (threadIdx.x < 2) partial[(int64_t)blockIdx.x * 2 + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-// pibar <- f1 pibar + h phibar (the kick's pibar scaling and the preceding drift's adjoint).
-__global__ void adj_drift_kernel(int64_t n3, double f1, double hd, const double2* __restrict__ phibar,
-                                 double2* __restrict__ pibar)
-{
-#pragma clang fp contract(off)
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 p = pibar[i], f = phibar[i];
-        pibar[i] = double2{f1 * p.x + hd * f.x, f1 * p.y + hd * f.y};
-    }
-}
-
-// total[j] += scale[j] * col[j], j = 0, 1 (one thread: a fixed order).
-__global__ void adj_accumulate_kernel(const double* __restrict__ col, double s0, double s1, double* __restrict__ total)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        total[0] = total[0] + s0 * col[0];
-        total[1] = total[1] + s1 * col[1];
+    // each block adds its scaled sums to its own slot: a fixed order, step after step
+    if (threadIdx.x < 2) {
+        const int64_t j = (int64_t)blockIdx.x * 2 + threadIdx.x;
+        partial[j] = partial[j] + (threadIdx.x == 0 ? s0 : s1) * sh[threadIdx.x][0];
     }
 }
 
@@ -302,44 +311,38 @@ void launch_adj_seed(hipStream_t s, int N, double2* abar, const uint16_t* bin_of
     AXG_HIP(hipGetLastError());
 }
 
-void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, double2* W0, double2* W1,
-                      double2* W2)
+template <typename TW>
+void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, void* W0, void* W1, void* W2)
 {
-    hipLaunchKernelGGL(adj_modes_kernel, dim3(ADJ_GRID), dim3(256), 0, s, N, abar, g, W0, W1, W2);
+    using W2T = typename V2<TW>::type;
+    hipLaunchKernelGGL((adj_modes_kernel<TW>), dim3(ADJ_GRID), dim3(256), 0, s, N, abar, g, (W2T*)W0, (W2T*)W1,
+                       (W2T*)W2);
     AXG_HIP(hipGetLastError());
 }
+template void launch_adj_modes<float>(hipStream_t, int, const double2*, const AdjStepScalars&, void*, void*, void*);
+template void launch_adj_modes<double>(hipStream_t, int, const double2*, const AdjStepScalars&, void*, void*, void*);
 
-template <typename T, bool COMP>
-void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, double2* W0, double2* W1,
-                     double2* W2)
+template <typename T, bool COMP, typename TW>
+void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, void* W0, void* W1,
+                     void* W2)
 {
     using T2 = typename V2<T>::type;
-    hipLaunchKernelGGL((adj_gbar_kernel<T, COMP>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, inv_d, W0, W1, W2);
+    using W2T = typename V2<TW>::type;
+    hipLaunchKernelGGL((adj_gbar_kernel<T, COMP, TW>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, inv_d, (W2T*)W0, (W2T*)W1, (W2T*)W2);
     AXG_HIP(hipGetLastError());
 }
 
-template <typename T, bool COMP>
+template <typename T, bool COMP, typename TW>
 void launch_adj_kick(hipStream_t s, int N, const void* phi, const void* cphi, const double2* pibar, double2* phibar,
-                     const double2* G0, const double2* G1, const double2* G2, const KickScalars& k, double inv_d,
-                     bool use_g, double* partial, double* colsum)
+                     double2* pibar_out, const void* G0, const void* G1, const void* G2, const KickScalars& k,
+                     double inv_d, double hd, double s0, double s1, bool use_g, double* partial)
 {
     using T2 = typename V2<T>::type;
-    hipLaunchKernelGGL((adj_kick_kernel<T, COMP>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, pibar, phibar, G0, G1, G2, k, inv_d, (int)use_g, partial);
-    AXG_HIP(hipGetLastError());
-    sum_columns(s, partial, ADJ_GRID, 2, colsum);
-}
-
-void launch_adj_drift(hipStream_t s, int64_t n3, double f1, double hd, const double2* phibar, double2* pibar)
-{
-    hipLaunchKernelGGL(adj_drift_kernel, dim3(ADJ_GRID), dim3(256), 0, s, n3, f1, hd, phibar, pibar);
-    AXG_HIP(hipGetLastError());
-}
-
-void launch_adj_accumulate(hipStream_t s, const double* col, double s0, double s1, double* total)
-{
-    hipLaunchKernelGGL(adj_accumulate_kernel, dim3(1), dim3(64), 0, s, col, s0, s1, total);
+    using W2T = typename V2<TW>::type;
+    hipLaunchKernelGGL((adj_kick_kernel<T, COMP, TW>), dim3(ADJ_GRID), dim3(256), 0, s, N, (const T2*)phi,
+                       (const short2*)cphi, pibar, phibar, pibar_out, (const W2T*)G0, (const W2T*)G1, (const W2T*)G2, k,
+                       inv_d, hd, s0, s1, (int)use_g, partial);
     AXG_HIP(hipGetLastError());
 }
 
@@ -354,16 +357,15 @@ void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi
     sum_columns(s, partial, ADJ_GRID, 1, out);
 }
 
-#define AXG_INST_ADJ(T, C)                                                                                           \
-    template void launch_adj_gbar<T, C>(hipStream_t, int, const void*, const void*, double, double2*, double2*,      \
-                                        double2*);                                                                  \
-    template void launch_adj_kick<T, C>(hipStream_t, int, const void*, const void*, const double2*, double2*,        \
-                                        const double2*, const double2*, const double2*, const KickScalars&, double, \
-                                        bool, double*, double*);                                                     \
+#define AXG_INST_ADJ(T, C, TW)                                                                                       \
+    template void launch_adj_gbar<T, C, TW>(hipStream_t, int, const void*, const void*, double, void*, void*, void*); \
+    template void launch_adj_kick<T, C, TW>(hipStream_t, int, const void*, const void*, const double2*, double2*,     \
+                                            double2*, const void*, const void*, const void*, const KickScalars&,      \
+                                            double, double, double, double, bool, double*);                           \
     template void launch_adj_dot<T, C>(hipStream_t, int64_t, const void*, const void*, const void*, const void*,     \
                                        const double2*, const double2*, double*, double*);
-AXG_INST_ADJ(double, false)
-AXG_INST_ADJ(float, false)
-AXG_INST_ADJ(float, true)
+AXG_INST_ADJ(double, false, double)
+AXG_INST_ADJ(float, false, float)
+AXG_INST_ADJ(float, true, float)
 
 }  // namespace axg

@@ -122,17 +122,18 @@ struct AdjStepScalars {
 };
 constexpr int ADJ_GRID_H = 1024;  // blocks of the adjoint's reductions
 void launch_adj_seed(hipStream_t s, int N, double2* abar, const uint16_t* bin_of_n2, const double* w);
-void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, double2* W0, double2* W1,
-                      double2* W2);
-template <typename T, bool COMP>
-void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, double2* W0, double2* W1,
-                     double2* W2);
-template <typename T, bool COMP>
+// W0..W2: the three packed adjoint-source arrays, N^3 complex in the precision TW
+template <typename TW>
+void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepScalars& g, void* W0, void* W1, void* W2);
+template <typename T, bool COMP, typename TW>
+void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, void* W0, void* W1,
+                     void* W2);
+// phibar += adjoint of the kick and of the source's differences; pibar_out = f1 pibar + hd phibar;
+// partial[block][2] += (s0, s1) times the block's sums of <pibar, V'_0(phi)> and <pibar, (-Re, Im)>.
+template <typename T, bool COMP, typename TW>
 void launch_adj_kick(hipStream_t s, int N, const void* phi, const void* cphi, const double2* pibar, double2* phibar,
-                     const double2* G0, const double2* G1, const double2* G2, const KickScalars& k, double inv_d,
-                     bool use_g, double* partial, double* colsum);
-void launch_adj_drift(hipStream_t s, int64_t n3, double f1, double hd, const double2* phibar, double2* pibar);
-void launch_adj_accumulate(hipStream_t s, const double* col, double s0, double s1, double* total);
+                     double2* pibar_out, const void* G0, const void* G1, const void* G2, const KickScalars& k,
+                     double inv_d, double hd, double s0, double s1, bool use_g, double* partial);
 template <typename T, bool COMP>
 void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi, const void* pi, const void* cpi,
                     const double2* phibar, const double2* pibar, double* partial, double* out);

@@ -159,6 +159,8 @@ struct axg_handle {
     void* ckpt = nullptr;
     int64_t ckpt_cap = 0;  // checkpoints the buffer holds
     bool have_adj = false;
+    double2* adj_pibar = nullptr;  // which of the two pibar buffers holds the result
+    double adj_scale = 1;          // the adjoint fields are held divided by this
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
     // the last field spectrum: deconvolved power and the count-normalised deconvolution matrix
@@ -1449,7 +1451,7 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
     const size_t state = (size_t)n3 * (2 * fsz + 2 * csz);
     const int64_t nck = (nsteps - 1) / K + 1 + 2;  // kicks 0, K, 2K, ...; the initial and final states
     if (!h->adj) {
-        h->adj = (double2*)h->alloc((size_t)4 * n3 * sizeof(double2));
+        h->adj = (double2*)h->alloc((size_t)5 * n3 * sizeof(double2));
         h->abar = (double2*)h->alloc((size_t)4 * hs * sizeof(double2));
     }
     if (nck > h->ckpt_cap) {
@@ -1462,10 +1464,17 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
         h->ckpt_cap = nck;
     }
     double2* phibar = h->adj;
-    double2* pibar = h->adj + n3;
-    double2* W0 = h->work;
-    double2* W1 = h->adj + 2 * n3;
-    double2* W2 = h->adj + 3 * n3;
+    double2* pibar = h->adj + n3;       // pibar ping-pongs with pibar2 across the adjoint kicks
+    double2* pibar2 = h->adj + 4 * n3;
+    // the adjoint sources in the storage precision (FP32 for FP32 and FP32C lattices)
+    void* W0 = h->work;
+    void* W1 = h->adj + 2 * n3;
+    void* W2 = h->adj + 3 * n3;
+    auto fft_back = [&](void* d) {
+        launch_fft_rows<T, -1>(h->stream, N, d, (int64_t)N * N, h->plan, h->tw);
+        launch_fft_cols<T, -1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw);
+        launch_fft_cols<T, -1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw);
+    };
     auto slot = [&](int64_t j) { return (char*)h->ckpt + state * (size_t)j; };
     auto save = [&](int64_t j) {
         char* b = slot(j);
@@ -1519,16 +1528,22 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
                 J = J + pw[b];
             }
     }
+    // the adjoint is linear in its seed: run it on w / wmax (the spectrum's prefactor can be
+    // ~1e120, beyond FP32) and scale the results back
+    double wmax = 0;
+    for (int b = 0; b < nb; b++) wmax = std::max(wmax, w[b]);
+    const double scale = wmax > 0 ? wmax : 1.0;
+    for (int b = 0; b < nb; b++) w[b] = w[b] / scale;
     double* d_w = h->red;
-    double* d_tot = h->red + nb;      // 2 running totals
-    double* d_col = h->red + nb + 2;  // 2 column sums
-    double* d_part = h->red + nb + 4;
+    double* d_tot = h->red + nb;      // 2 totals
+    double* d_col = h->red + nb + 2;  // the dot product
+    double* d_part = h->red + nb + 4;  // [block][2] sums over the steps
     require((size_t)(nb + 4 + 2 * ADJ_GRID_H) * 8 <= h->red_bytes, "reduction scratch too small");
     AXG_HIP(hipMemcpyAsync(d_w, w.data(), nb * 8, hipMemcpyHostToDevice, h->stream));
     for (int m = 0; m < 4; m++) launch_gw_pack<T>(h->stream, N, h->L, h->A, m, h->abar + m * hs, +1);
     launch_adj_seed(h->stream, N, h->abar, h->gw_bin, d_w);
     AXG_HIP(hipMemsetAsync(phibar, 0, (size_t)2 * n3 * sizeof(double2), h->stream));
-    AXG_HIP(hipMemsetAsync(d_tot, 0, 2 * sizeof(double), h->stream));
+    AXG_HIP(hipMemsetAsync(d_part, 0, (size_t)2 * ADJ_GRID_H * sizeof(double), h->stream));
 
     // backward: from the final state, undo the last half drift, then per kick s (last first):
     // the adjoint of its GW update and of the kick, the kick undone (-dt), the exact state from
@@ -1544,27 +1559,31 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
         g.dwav = p.dwav;
         g.fix_swap = p.gw_fix_swap;
         g.last = s == nsteps - 1;
-        launch_adj_modes(h->stream, N, h->abar, g, W0, W1, W2);
-        fft3d_work(h, W0, -1);
-        fft3d_work(h, W1, -1);
-        fft3d_work(h, W2, -1);
-        launch_adj_gbar<T, COMP>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2);
+        launch_adj_modes<T>(h->stream, N, h->abar, g, W0, W1, W2);
+        fft_back(W0);
+        fft_back(W1);
+        fft_back(W2);
+        launch_adj_gbar<T, COMP, T>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2);
         const KickScalars k = kick_scalars(p, t, dt);
-        launch_adj_kick<T, COMP>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], pibar, phibar, W0, W1, W2, k, inv_d,
-                                 true, d_part, d_col);
+        const double hd = s == 0 ? dt / 2 : dt;  // the drift before kick s
         const double t2 = t * t;
-        launch_adj_accumulate(h->stream, d_col, k.f3 / s2c, k.f3 * (t2 * t2) / s2c, d_tot);
-        // undo the kick: the same kick with -dt at the same time, phi unchanged (drift of 0)
+        launch_adj_kick<T, COMP, T>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], pibar, phibar, pibar2, W0, W1, W2, k,
+                                 inv_d, hd, k.f3 / s2c, k.f3 * (t2 * t2) / s2c, true, d_part);
+        std::swap(pibar, pibar2);
+        // undo the kick (the same kick with -dt at the same time) and the drift before it; at a
+        // checkpoint the exact state comes back between the two
+        const bool ck = s % K == 0;
         const int nxt = 1 - h->cur;
         launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
-                                   h->cpi, nullptr, kick_scalars(p, t, -dt), p.dspa, 0.0, false);
+                                   h->cpi, nullptr, kick_scalars(p, t, -dt), p.dspa, ck ? 0.0 : -hd, false);
         h->cur = nxt;
-        if (s % K == 0) load(s / K);
-        const double hd = s == 0 ? dt / 2 : dt;
-        launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
-                              h->cphi[h->cur], h->cpi, -hd);
-        launch_adj_drift(h->stream, n3, k.f1, hd, phibar, pibar);
+        if (ck) {
+            load(s / K);
+            launch_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[h->cur], h->pi, h->cphi[h->cur],
+                                  h->cphi[h->cur], h->cpi, -hd);
+        }
     }
+    sum_columns(h->stream, d_part, ADJ_GRID_H, 2, d_tot);
     // gradients: the dynamics' parts plus the explicit sigma_cr^2 in dw (J is quadratic in A)
     load(J_INIT);
     double* d_dot = d_col;
@@ -1576,11 +1595,13 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
     const auto c2 = std::chrono::steady_clock::now();
     load(J_FINAL);
     AXG_HIP(hipStreamSynchronize(h->stream));
+    h->adj_pibar = pibar;
+    h->adj_scale = scale;
     h->have_adj = true;
     out->J = J;
-    out->dJ_dsigma2 = tot[0] + 2 * J / s2c;
-    out->dJ_dkappa0 = tot[1];
-    out->dJ_dlnamp = dot;
+    out->dJ_dsigma2 = tot[0] * scale + 2 * J / s2c;
+    out->dJ_dkappa0 = tot[1] * scale;
+    out->dJ_dlnamp = dot * scale;
     out->forward_ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
     out->adjoint_ms = std::chrono::duration<double, std::milli>(c2 - c1).count();
 }
@@ -1614,9 +1635,13 @@ extern "C" int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int
         if (phibar)
             AXG_HIP(hipMemcpyAsync(phibar, h->adj, (size_t)h->n3 * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
         if (pibar)
-            AXG_HIP(hipMemcpyAsync(pibar, h->adj + h->n3, (size_t)h->n3 * sizeof(double2), hipMemcpyDeviceToHost,
+            AXG_HIP(hipMemcpyAsync(pibar, h->adj_pibar, (size_t)h->n3 * sizeof(double2), hipMemcpyDeviceToHost,
                                    h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
+        for (int64_t i = 0; i < 2 * h->n3; i++) {
+            if (phibar) phibar[i] *= h->adj_scale;
+            if (pibar) pibar[i] *= h->adj_scale;
+        }
     });
 }
 

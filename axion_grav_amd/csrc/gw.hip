@@ -410,10 +410,43 @@ __global__ void __launch_bounds__(FFT_NT) fft_rows_kernel(int N, int TB, int64_t
     for (int e = threadIdx.x; e < nrow * N; e += FFT_NT) base[e] = s[e];
 }
 
+// Rows of N = 2^k: RB rows per block of RB * N/8 threads, each thread holding 8 elements of its
+// row in registers (fft_p2.h), loads and stores coalesced along the row.
+template <int N> constexpr int p2_rows_per_block() { return (N / 8) >= 256 ? 1 : 256 / (N / 8); }
+
+template <typename T, int N, int SIGN>
+__global__ void __launch_bounds__(p2_rows_per_block<N>() * (N / 8))
+    fft_rows_p2_kernel(typename V2<T>::type* __restrict__ data, int64_t nrows, const typename V2<T>::type* __restrict__ tw)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int TPT = N / 8, RB = p2_rows_per_block<N>();
+    extern __shared__ unsigned char smem_raw[];
+    T2* s = reinterpret_cast<T2*>(smem_raw);
+    const int rr = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+    const int64_t row = (int64_t)blockIdx.x * RB + rr;
+    const bool active = row < nrows;
+    T2* base = data + (active ? row : 0) * N;
+    T2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = active ? base[j0 + e * TPT] : T2{0, 0};
+    const P2Tw<T, N> tws = p2_load_twiddles<T, N, SIGN>(j0, tw);
+    fft_p2_if<T, N, SIGN>(active, v, j0, s + rr * N, 1, 0, tws);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) base[j0 + e * TPT] = v[e];
+    }
+}
+
 template <typename T, int SIGN>
 void launch_fft_rows(hipStream_t s, int N, void* data, int64_t nrows, const FftPlan& plan, const void* tw)
 {
     using T2 = typename V2<T>::type;
+    bool done = p2_dispatch(N, [&]<int NN>() {
+        constexpr int RB = p2_rows_per_block<NN>();
+        hipLaunchKernelGGL((fft_rows_p2_kernel<T, NN, SIGN>), dim3((unsigned)((nrows + RB - 1) / RB)),
+                           dim3(RB * (NN / 8)), (size_t)RB * NN * sizeof(T2), s, (T2*)data, nrows, (const T2*)tw);
+    });
+    if (done) { AXG_HIP(hipGetLastError()); return; }
     int TB = fft_elems<T>() / N;
     if (TB < 1) throw Error(AXG_EINVAL, "row FFT: N too large for the LDS tile");
     int64_t nb = (nrows + TB - 1) / TB;
@@ -747,5 +780,8 @@ AXG_INST_GW(float)
 template void launch_fft_cols<double, -1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,
                                           const void*);
 template void launch_fft_rows<double, -1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);
+template void launch_fft_cols<float, -1>(hipStream_t, int, void*, int64_t, int64_t, int, int, const FftPlan&,
+                                         const void*);
+template void launch_fft_rows<float, -1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);
 
 }  // namespace axg

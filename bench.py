@@ -153,6 +153,32 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False, tune=True):
                                                   "ms": {k: round(v, 4) for k, v in tuned[2].items()}})
 
 
+def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_end=16.0):
+    """Cost of the gradient of a GW band energy (k in [0.5, 2]) by the adjoint: a plain FP32C
+    forward of `steps` steps, then the same steps with checkpoints every K and the backward sweep.
+    The steps span tau = 1 .. t_end: a fixed step of 0.2 dspa would reach tau = 41, past the
+    leapfrog's stability limit (near tau = 23 for lambda = 2 at dt = 0.05)."""
+    P = params(capi, N, L, capi.FP32C, 32)
+    P.potential_kappa0 = kappa0
+    dt = (t_end - 1.0) / steps
+    with capi.Lattice(P) as lat:
+        lat.init_flat(seed, 1.0)
+        phi0, pi0 = lat.get_fields()
+        lat.advance_steps(1.0, dt, 3)  # warm the kernels up
+        lat.set_fields(phi0, pi0)
+        t0 = time.perf_counter()
+        lat.advance_steps(1.0, dt, steps)
+        fwd = (time.perf_counter() - t0) * 1e3
+        lat.set_fields(phi0, pi0)
+        g = lat.gradient(1.0, dt, steps, K, 0.5, 2.0)
+    return dict(workload=f"{N}^3, L={L:g}, FP32C, kappa0={kappa0:g}, seed {seed}, {steps} steps of {dt:g} from tau 1, "
+                         f"checkpoints every {K}",
+                J=g["J"], dJ_dlambda=g["dJ_dsigma2"] / 4, dJ_dkappa0=g["dJ_dkappa0"], dJ_dlnamp=g["dJ_dlnamp"],
+                forward_ms=round(fwd, 2), forward_with_checkpoints_ms=round(g["forward_ms"], 2),
+                adjoint_ms=round(g["adjoint_ms"], 2), adjoint_over_forward=round(g["adjoint_ms"] / fwd, 3),
+                gradient_over_forward=round((g["forward_ms"] + g["adjoint_ms"]) / fwd, 3))
+
+
 def copy_probe(nbytes=4 << 30, reps=10):
     """Device-to-device copy of nbytes (read + write = 2 nbytes per copy): the achievable
     streaming bandwidth on this GPU, measured with torch's copy kernel."""
@@ -200,6 +226,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-diag", action="store_true", help="skip timing the diagnostics after the FP32C run")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in configuration of the fused pass")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip timing the gradient (128^3, 800 steps)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
                          "its kernel is the known-size reference for the FETCH/WRITE counters)")
@@ -227,6 +254,12 @@ def main():
         d = run_gpu(capi, N, L, capi.FP64, args.steps, args.warmup, args.seed, tune=not args.no_tune)
         log(f"FP64 {d['su_s']:.4g} site-updates/s, {d['ms_per_step']:.3f} ms/step; kernels "
             + ", ".join(f"{k} {v['ms_avg']:.3f} ms {v['gbs']:.0f} GB/s" for k, v in d["kernels"].items()))
+
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint: 128^3 FP32C, 800 steps, checkpoints every 50")
+        adj = run_adjoint(capi)
+        log(f"adjoint: {adj}")
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
     tc = c["tuned"] or {"choice": 0, "ypass": 0}
@@ -271,6 +304,7 @@ def main():
         "hbm_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "diagnostics": c["diagnostics"],
         "tuning": {"fp32c": c["tuned"], "fp64": None if d is None else d["tuned"]},
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -20,8 +20,8 @@ def fields(N, seed=3):
     return phi, pi
 
 
-def oparams(N, kappa0):
-    return dict(phys(N), potential_kappa0=kappa0, tau_cr=9.42, a_now=5.4e30)
+def oparams(N, kappa0, tau_cr=9.42, a_now=5.4e30):
+    return dict(phys(N), potential_kappa0=kappa0, tau_cr=tau_cr, a_now=a_now)
 
 
 @pytest.mark.parametrize("fix", [False, True])
@@ -55,19 +55,23 @@ def test_oracle_adjoint_matches_finite_differences(fix):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,N,K,fix,tol", [("fp64", 16, 4, False, 1e-7), ("fp64", 16, 5, True, 1e-7),
-                                              ("fp32c", 32, 4, False, 2e-4)])
-def test_gpu_gradient_matches_oracle(prec, N, K, fix, tol):
+@pytest.mark.parametrize("prec,N,K,fix,tol,units", [("fp64", 16, 4, False, 1e-7, "phys"),
+                                                    ("fp64", 16, 5, True, 1e-7, "phys"),
+                                                    ("fp32c", 32, 4, False, 2e-4, "phys"),
+                                                    ("fp32c", 32, 6, False, 2e-4, "unit")])
+def test_gpu_gradient_matches_oracle(prec, N, K, fix, tol, units):
     """axg_gradient against the oracle's adjoint on the same fields: J, the three parameter
     derivatives and the adjoint fields (FP64 within 1e-7: the GPU's time reversal and FFT
     rounding; FP32C storage within 2e-4).  The run leaves the fields and accumulators of a plain
-    axg_advance_steps of the same steps."""
+    axg_advance_steps of the same steps.  With tau_cr = a_now = 1 the spectrum's prefactor is
+    ~1e120: the FP32 adjoint must not overflow."""
     from axion_grav_amd import capi
 
-    print(f"[case] gradient {prec} N={N} K={K} fix={fix}", flush=True)
+    print(f"[case] gradient {prec} N={N} K={K} fix={fix} {units}", flush=True)
     c = CASE
-    P = oparams(N, 1e-3)
-    prm = capi.make_params(N, P["dspa"], SQ_HALF, num_bin_grav=c["nb"], tau_cr=9.42, a_now=5.4e30,
+    tc, an = (9.42, 5.4e30) if units == "phys" else (1.0, 1.0)
+    P = oparams(N, 1e-3, tc, an)
+    prm = capi.make_params(N, P["dspa"], SQ_HALF, num_bin_grav=c["nb"], tau_cr=tc, a_now=an,
                            potential_kappa0=1e-3, gw_fix_swap=fix,
                            precision=capi.FP64 if prec == "fp64" else capi.FP32C)
     phi0, pi0 = fields(N)
@@ -95,3 +99,46 @@ def test_gpu_gradient_matches_oracle(prec, N, K, fix, tol):
     assert abs(g["dJ_dlnamp"] / dlna - 1) < tol
     for a, b in ((phib, ophib), (pib, opib)):
         assert np.abs(a - b).max() < tol * np.abs(b).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-6), ("fp32c", 2e-3)])
+def test_torch_harness_against_finite_differences(prec, tol):
+    """The torch.autograd wrapper: J(A phi_seed, A pi_seed, sigma2, kappa0).backward() against
+    central differences (relative step 1e-4) of the oracle's FP64 J -- the reference path -- in
+    lambda (through sigma_cr^2 = lambda/4), kappa0 and the amplitude A; the FP64 GPU's own
+    differences agree as well."""
+    import torch
+
+    from axion_grav_amd import capi, gw_grad
+
+    print(f"[case] torch harness {prec}", flush=True)
+    N, c = 16, CASE
+    P = oparams(N, 1e-3)
+    run = dict(N=N, dspa=P["dspa"], t1=c["t1"], dt=c["dt"], nsteps=c["nsteps"], K=4, qlo=c["qlo"], qhi=c["qhi"],
+               params=dict(num_bin_grav=c["nb"], tau_cr=9.42, a_now=5.4e30,
+                           precision=capi.FP64 if prec == "fp64" else capi.FP32C))
+    phi_s, pi_s = fields(N)
+    lam = torch.tensor(4 * SQ_HALF ** 2, dtype=torch.float64, requires_grad=True)
+    k0 = torch.tensor(1e-3, dtype=torch.float64, requires_grad=True)
+    amp = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    J = gw_grad.band_energy(amp * torch.from_numpy(phi_s), amp * torch.from_numpy(pi_s), lam / 4, k0, run)
+    J.backward()
+    grads = {"lambda": lam.grad.item(), "kappa0": k0.grad.item(), "A": amp.grad.item()}
+
+    def oJ(lmb=2.0, kap=1e-3, a=1.0):
+        P2 = dict(P, sigma_cr=(lmb / 4) ** 0.5, potential_kappa0=kap)
+        return OA.band_energy(a * phi_s, a * pi_s, P2, c["t1"], c["dt"], c["nsteps"], c["nb"], c["qlo"], c["qhi"])
+
+    h = 1e-4
+    fd = {"lambda": (oJ(lmb=2 * (1 + h)) - oJ(lmb=2 * (1 - h))) / (4 * h),
+          "kappa0": (oJ(kap=1e-3 * (1 + h)) - oJ(kap=1e-3 * (1 - h))) / (2e-3 * h),
+          "A": (oJ(a=1 + h) - oJ(a=1 - h)) / (2 * h)}
+    print("  torch", grads, "\n  oracle FD", fd)
+    assert abs(J.item() / oJ() - 1) < tol
+    for k in grads:
+        assert abs(grads[k] / fd[k] - 1) < tol, k
+    if prec == "fp64":
+        gfd = (gw_grad.forward_only(phi_s, pi_s, 0.5 * (1 + h), 1e-3, run) -
+               gw_grad.forward_only(phi_s, pi_s, 0.5 * (1 - h), 1e-3, run)) / (2 * h * 0.5) / 4
+        assert abs(grads["lambda"] / gfd - 1) < 1e-5
