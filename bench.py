@@ -153,11 +153,14 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False, tune=True):
                                                   "ms": {k: round(v, 4) for k, v in tuned[2].items()}})
 
 
-def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_end=16.0):
+def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_end=16.0, check_steps=400):
     """Cost of the gradient of a GW band energy (k in [0.5, 2]) by the adjoint: a plain FP32C
     forward of `steps` steps, then the same steps with checkpoints every K and the backward sweep.
     The steps span tau = 1 .. t_end: a fixed step of 0.2 dspa would reach tau = 41, past the
-    leapfrog's stability limit (near tau = 23 for lambda = 2 at dt = 0.05)."""
+    leapfrog's stability limit (near tau = 23 for lambda = 2 at dt = 0.05).  The gradients are
+    checked against FP64 central differences (relative step 1e-4) over the first `check_steps`
+    steps: over the whole run J varies too roughly with the parameters for differences to
+    converge (at 800 steps steps of 1e-4 and 1e-6 disagree with each other)."""
     P = params(capi, N, L, capi.FP32C, 32)
     P.potential_kappa0 = kappa0
     dt = (t_end - 1.0) / steps
@@ -171,12 +174,34 @@ def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_en
         fwd = (time.perf_counter() - t0) * 1e3
         lat.set_fields(phi0, pi0)
         g = lat.gradient(1.0, dt, steps, K, 0.5, 2.0)
+        lat.set_fields(phi0, pi0)
+        gc = lat.gradient(1.0, dt, check_steps, K, 0.5, 2.0)
+
+    def J64(s2f=1.0, k0f=1.0, a=1.0):
+        P64 = params(capi, N, L, capi.FP64, 32)
+        P64.sigma_cr = SIGMA_CR * s2f ** 0.5
+        P64.potential_kappa0 = kappa0 * k0f
+        with capi.Lattice(P64) as lat:
+            lat.set_fields(a * phi0, a * pi0)
+            lat.advance_steps(1.0, dt, check_steps)
+            s = lat.gw_spectrum()
+        band = (s["qmid"] >= 0.5) & (s["qmid"] <= 2.0) & (s["denom"] > 0)
+        return float(s["power"][band].sum())
+
+    h = 1e-4
+    lam = 4 * SIGMA_CR ** 2
+    fd = {"dJ_dlambda": (J64(s2f=1 + h) - J64(s2f=1 - h)) / (2 * h * lam),
+          "dJ_dkappa0": (J64(k0f=1 + h) - J64(k0f=1 - h)) / (2 * h * kappa0),
+          "dJ_dA": (J64(a=1 + h) - J64(a=1 - h)) / (2 * h)}
+    ad = {"dJ_dlambda": gc["dJ_dsigma2"] / 4, "dJ_dkappa0": gc["dJ_dkappa0"], "dJ_dA": gc["dJ_dlnamp"]}
+    rel = {k: float(f"{abs(ad[k] / fd[k] - 1):.3g}") for k in fd}
     return dict(workload=f"{N}^3, L={L:g}, FP32C, kappa0={kappa0:g}, seed {seed}, {steps} steps of {dt:g} from tau 1, "
                          f"checkpoints every {K}",
-                J=g["J"], dJ_dlambda=g["dJ_dsigma2"] / 4, dJ_dkappa0=g["dJ_dkappa0"], dJ_dlnamp=g["dJ_dlnamp"],
+                J=g["J"], dJ_dlambda=g["dJ_dsigma2"] / 4, dJ_dkappa0=g["dJ_dkappa0"], dJ_dA=g["dJ_dlnamp"],
                 forward_ms=round(fwd, 2), forward_with_checkpoints_ms=round(g["forward_ms"], 2),
                 adjoint_ms=round(g["adjoint_ms"], 2), adjoint_over_forward=round(g["adjoint_ms"] / fwd, 3),
-                gradient_over_forward=round((g["forward_ms"] + g["adjoint_ms"]) / fwd, 3))
+                gradient_over_forward=round((g["forward_ms"] + g["adjoint_ms"]) / fwd, 3),
+                check=dict(steps=check_steps, adjoint_fp32c=ad, fd_fp64=fd, rel_err=rel))
 
 
 def copy_probe(nbytes=4 << 30, reps=10):
