@@ -175,9 +175,11 @@ __device__ __forceinline__ int wrapn(int i, int n) { return (i < n / 2) ? i : i 
 // residual x - hi is at most half an FP32 ulp (2^(e-25)), so lo fits in 15 bits and the
 // pair carries ~39 significant bits.
 
+// Both directions are branch-free: a per-element branch here costs the callers (the kick
+// passes) more in exec-mask handling than the arithmetic it would skip.  lo = 0 adds +0, which
+// leaves every value but -0 unchanged (that one reads back as +0).
 __device__ __forceinline__ double comp_get(float hi, short lo)
 {
-    if (lo == 0) return (double)hi;
     int e;
     (void)frexpf(hi, &e);
     return (double)hi + ldexp((double)lo, e - 39);
@@ -185,13 +187,12 @@ __device__ __forceinline__ double comp_get(float hi, short lo)
 
 __device__ __forceinline__ float comp_put(double x, short& lo)
 {
-    float hi = (float)x;
-    if (hi == 0.0f || !isfinite(hi)) { lo = 0; return hi; }
+    const float hi = (float)x;
     int e;
     (void)frexpf(hi, &e);
     double r = rint(ldexp(x - (double)hi, 39 - e));
     r = fmin(fmax(r, -32767.0), 32767.0);
-    lo = (short)r;
+    lo = (hi == 0.0f || !isfinite(hi)) ? (short)0 : (short)r;
     return hi;
 }
 

@@ -169,12 +169,23 @@ __device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, dou
     p.x = k.f1 * p.x + k.f2 * lr + k.f3 * dr;
     p.y = k.f1 * p.y + k.f2 * li + k.f3 * di;
     f = double2{c.x + dt_drift * p.x, c.y + dt_drift * p.y};
-    double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
-    double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
-    double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
-    src[0] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
-    src[1] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
-    src[2] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
+    if constexpr (sizeof(T) == 4) {
+        // FP32 storage: the differences in FP64 (they cancel), the gradient and the products in
+        // FP32, the precision the sources are stored in
+        const float sx = (float)((a2.x - a1.x) * inv_d), sy = (float)((a2.y - a1.y) * inv_d);
+        const float tx = (float)((b2.x - b1.x) * inv_d), ty = (float)((b2.y - b1.y) * inv_d);
+        const float ux = (float)((c2v.x - c1v.x) * inv_d), uy = (float)((c2v.y - c1v.y) * inv_d);
+        src[0] = T2{2 * (sx * sx + sy * sy), 2 * (tx * sx + ty * sy)};
+        src[1] = T2{2 * (tx * tx + ty * ty), 2 * (ux * sx + uy * sy)};
+        src[2] = T2{2 * (ux * tx + uy * ty), 2 * (ux * ux + uy * uy)};
+    } else {
+        double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+        double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+        double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
+        src[0] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
+        src[1] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
+        src[2] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
+    }
 }
 
 // XCD-aware block order (REMAP): the dispatcher deals consecutive blocks to the 8 XCDs in
@@ -320,7 +331,7 @@ __device__ __forceinline__ void st_pair(typename V2<T>::type* __restrict__ f, sh
     reinterpret_cast<CPair<T>*>(f + i)[0] = v;
 }
 
-template <typename T, bool COMP, int N, int XR>
+template <typename T, bool COMP, int N, int XR, bool STRIP>
 __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
@@ -333,7 +344,17 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     extern __shared__ unsigned char smem_raw[];
     T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
     const int64_t NN = (int64_t)N * N;
-    const int yg = blockIdx.x, z = blockIdx.y;
+    int yg = blockIdx.x, z = blockIdx.y;
+    if constexpr (STRIP) {
+        // XCD-aware strips: block lin goes to XCD lin % 8 as its (lin / 8)-th block; each XCD
+        // takes a strip of NG/8 row groups and walks it plane by plane, so the y and z
+        // neighbour rows its blocks read were fetched by its own recent blocks (its L2)
+        constexpr int NG = N / XR, SG = NG >= 8 ? NG / 8 : 1;  // launched only when NG % 8 == 0
+        const int lin = blockIdx.y * NG + blockIdx.x;
+        const int l = lin >> 3;
+        yg = (lin & 7) * SG + l % SG;
+        z = l / SG;
+    }
     {
         const int rr = threadIdx.x / H, x0 = 2 * (threadIdx.x % H);
         const int y = yg * XR + rr;
@@ -410,8 +431,8 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
     // configuration (same results; the block shape and order only): bit 0 = wide blocks (twice
-    // the rows: 1024 threads), bit 1 = XCD remap (one site per thread only), bit 2 = swap the
-    // sites-per-thread choice
+    // the rows: 1024 threads), bit 1 = XCD remap by planes (one site per thread only), bit 2 =
+    // swap the sites-per-thread choice, bit 3 = XCD strips (two sites per thread only)
     auto launch = [&]<int NN, int XR, bool RM>() {
         hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
@@ -425,15 +446,21 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
             // (measured at 512^3: FP32C 2.95 vs 3.39 ms, FP64 4.97 vs 3.64 ms)
             const bool two = (sizeof(T) == 4) != ((variant & 4) != 0);
             if (two) {
-                auto launch2 = [&]<int X2>() {
-                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2>), dim3(NN / X2, NN),
+                auto launch2 = [&]<int X2, bool ST>() {
+                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN),
                                        dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
                                        (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
                                        (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift);
                 };
                 constexpr int X2W = 2 * XR <= NN ? 2 * XR : XR;
-                if (variant & 1) launch2.template operator()<X2W>();
-                else launch2.template operator()<XR>();
+                const bool strip = (variant & 8) != 0 && (NN / X2W) % 8 == 0;
+                if (variant & 1) {
+                    if (strip) launch2.template operator()<X2W, true>();
+                    else launch2.template operator()<X2W, false>();
+                } else {
+                    if (strip && (NN / XR) % 8 == 0) launch2.template operator()<XR, true>();
+                    else launch2.template operator()<XR, false>();
+                }
                 return;
             }
             switch (variant & 3) {

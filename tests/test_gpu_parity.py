@@ -131,7 +131,7 @@ def test_tune_keeps_fields_and_results(prec):
         v, yv, ms = L.tune(3)
         b_phi, b_pi = L.get_fields()
         assert np.array_equal(a_phi, b_phi) and np.array_equal(a_pi, b_pi)
-        assert v in (0, 1, 4, 5) and yv in (0, 1) and all(t > 0 for t in ms.values())
+        assert v in (0, 1, 4, 5, 8, 9, 12, 13) and yv in (0, 1) and all(t > 0 for t in ms.values())
         L.advance_steps(1.0, 0.05, 3)
         t_phi, t_pi = L.get_fields()
         t_A = L.gw_state()
