@@ -51,29 +51,30 @@ MODEL_REALS = 40
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "lattice_step_512_hbm_traffic.md")
 # the fused pass's kernel for each tuner choice (0 built-in, 1 wide blocks, 4 / 5 the other
 # sites-per-thread choice), FP32C and FP64 at 512^3
-FUSED_KERNEL = {("fp32c", 0): "kick_source_xfft2_kernel<float, true, 512, 1>",
-                ("fp32c", 1): "kick_source_xfft2_kernel<float, true, 512, 2>",
-                ("fp32c", 4): "kick_source_xfft_kernel<float, true, 512, 1, false>",
-                ("fp32c", 5): "kick_source_xfft_kernel<float, true, 512, 2, false>",
-                ("fp64", 0): "kick_source_xfft_kernel<double, false, 512, 1, false>",
-                ("fp64", 1): "kick_source_xfft_kernel<double, false, 512, 2, false>",
-                ("fp64", 4): "kick_source_xfft2_kernel<double, false, 512, 1>",
-                ("fp64", 5): "kick_source_xfft2_kernel<double, false, 512, 2>"}
-TRAFFIC_KERNEL = {("gw_yfft", "fp32c"): "gw_ypass_p2_kernel<float, 512>",
-                  ("gw_yfft", "fp64"): "gw_ypass_p2_kernel<double, 512>",
-                  ("gw_zfft_accumulate", "fp32c"): "gw_zpass_p2_kernel<float, 512>",
-                  ("gw_zfft_accumulate", "fp64"): "gw_zpass_p2_kernel<double, 512>"}
+def _fused_name(prec, v):
+    """Kernel name of the fused pass for tuner code v at 512^3 (see launch_kick_source_xfft)."""
+    T, C = ("float", "true") if prec == "fp32c" else ("double", "false")
+    two = (prec == "fp32c") != bool(v & 4)
+    xr = 2 if v & 1 else 1
+    if two:
+        return f"kick_source_xfft2_kernel<{T}, {C}, 512, {xr}, {'true' if v & 8 else 'false'}>"
+    return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {'true' if v & 2 else 'false'}>"
+
+
+YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16>", ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8>",
+                ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8>", ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4>"}
+ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512>", "fp64": "gw_zpass_p2_kernel<double, 512>"}
 
 
 def measured_traffic(slot, prec, N, choice=0):
     """Counter-measured HBM bytes per launch of a step kernel at 512^3 (the fused pass in the
     tuner's configuration `choice`), or None when the profile does not hold that kernel."""
     if slot == "kick_source_drift":
-        name = FUSED_KERNEL.get((prec, choice))
-    elif slot == "gw_yfft" and choice != 0:
-        name = None  # the half-width y pass is not in the profile
+        name = _fused_name(prec, choice)
+    elif slot == "gw_yfft":
+        name = YPASS_KERNEL.get((prec, choice))
     else:
-        name = TRAFFIC_KERNEL.get((slot, prec))
+        name = ZPASS_KERNEL.get(prec)
     if N != 512 or name is None or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
