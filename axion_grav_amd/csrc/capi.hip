@@ -675,7 +675,7 @@ extern "C" int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* l
 
 // Configurations of the fused kick / source / x-transform pass the tuner tries: the built-in
 // choice, wide blocks (twice the rows), the other sites-per-thread choice, and both; then the
-// same four with XCD strips (which apply to the two-sites-per-thread kernel).
+// same four with XCD strips.
 static const int kTuneVariants[] = {0, 1, 4, 5, 8, 9, 12, 13};
 
 // A start/stop event pair, destroyed with the scope.

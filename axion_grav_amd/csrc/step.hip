@@ -188,10 +188,12 @@ __device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, dou
     }
 }
 
-// XCD-aware block order (REMAP): the dispatcher deals consecutive blocks to the 8 XCDs in
-// turn; remapped, XCD k walks a contiguous run of planes, so the blocks it holds at once are
-// neighbouring rows of one plane and share their halo rows in its L2.
-template <typename T, bool COMP, int N, int XR, bool REMAP>
+// XCD-aware block orders: the dispatcher deals consecutive blocks to the 8 XCDs in turn.
+// REMAP 1 (planes): XCD k walks a contiguous run of planes, so the blocks it holds at once are
+// neighbouring rows of one plane and share their y halo rows in its L2.  REMAP 2 (strips): XCD
+// k takes a strip of NG/8 row groups and walks it plane by plane, so its blocks share both
+// the y and the z halo rows.
+template <typename T, bool COMP, int N, int XR, int REMAP>
 __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
@@ -207,11 +209,17 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
     const int64_t NN = (int64_t)N * N;
     int yg = blockIdx.x, z = blockIdx.y;
-    if constexpr (REMAP) {
+    if constexpr (REMAP == 1) {
         const int lin = blockIdx.y * NG + blockIdx.x, nb = NG * N;
         const int logical = (lin & 7) * (nb >> 3) + (lin >> 3);
         yg = logical % NG;
         z = logical / NG;
+    } else if constexpr (REMAP == 2) {
+        constexpr int SG = NG >= 8 ? NG / 8 : 1;  // launched only when NG % 8 == 0
+        const int lin = blockIdx.y * NG + blockIdx.x;
+        const int l = lin >> 3;
+        yg = (lin & 7) * SG + l % SG;
+        z = l / SG;
     }
     {
         const int rr = threadIdx.x / N, x = threadIdx.x % N;
@@ -432,8 +440,8 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
     // configuration (same results; the block shape and order only): bit 0 = wide blocks (twice
     // the rows: 1024 threads), bit 1 = XCD remap by planes (one site per thread only), bit 2 =
-    // swap the sites-per-thread choice, bit 3 = XCD strips (two sites per thread only)
-    auto launch = [&]<int NN, int XR, bool RM>() {
+    // swap the sites-per-thread choice, bit 3 = XCD strips
+    auto launch = [&]<int NN, int XR, int RM>() {
         hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
                            (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
@@ -463,14 +471,20 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                 }
                 return;
             }
+            const bool wide = (variant & 1) != 0;
+            if ((variant & 8) && (NN / (wide ? XW : XR)) % 8 == 0) {
+                if (wide) launch.template operator()<NN, XW, 2>();
+                else launch.template operator()<NN, XR, 2>();
+                return;
+            }
             switch (variant & 3) {
-            case 1: launch.template operator()<NN, XW, false>(); return;
-            case 2: launch.template operator()<NN, XR, true>(); return;
-            case 3: launch.template operator()<NN, XW, true>(); return;
+            case 1: launch.template operator()<NN, XW, 0>(); return;
+            case 2: launch.template operator()<NN, XR, 1>(); return;
+            case 3: launch.template operator()<NN, XW, 1>(); return;
             default: break;
             }
         }
-        launch.template operator()<NN, XR, false>();
+        launch.template operator()<NN, XR, 0>();
     };
     switch (N) {
     case 16: go.template operator()<16>(); break;

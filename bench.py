@@ -58,7 +58,7 @@ def _fused_name(prec, v):
     xr = 2 if v & 1 else 1
     if two:
         return f"kick_source_xfft2_kernel<{T}, {C}, 512, {xr}, {'true' if v & 8 else 'false'}>"
-    return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {'true' if v & 2 else 'false'}>"
+    return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {2 if v & 8 else 1 if v & 2 else 0}>"
 
 
 YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16>", ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8>",
