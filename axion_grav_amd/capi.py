@@ -79,7 +79,7 @@ EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
            "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls",
            "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
-           "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_energy", "axg_tune", "axg_profile")
+           "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_batch_advance", "axg_energy", "axg_tune", "axg_profile")
 
 _LIB = None
 
@@ -119,6 +119,7 @@ def lib():
             "axg_field_spectrum": [p, d, p, p, p, p, p, i32],
             "axg_field_covariance": [p, C.c_uint64, i32, i32, d, p, i64, p],
             "axg_gradient": [p, d, d, i32, i32, d, d, C.POINTER(GradOut)],
+            "axg_batch_advance": [p, i32, d, d, i32, i32, p],
             "axg_get_adjoint": [p, p, p, i64],
             "axg_energy": [p, d, p],
             "axg_profile": [p, i32, p, p, i32],
@@ -151,6 +152,17 @@ def derive(sigma, g_rel, kappa, tau_fin, boxsize, ngrid, params=None):
     if rc != 0:
         raise AxgError(f"axg_derive failed ({rc})")
     return P, dict(zeta=z.value, width_cr=w.value, thickness_cr=t.value)
+
+
+def batch_advance(lattices, t1, dt, nsteps, chunk=10):
+    """Advance every Lattice of the list by nsteps steps in one captured graph of `chunk` steps
+    of all of them (0: direct launches); returns the wall time in ms."""
+    hs = (C.c_void_p * len(lattices))(*[L._h for L in lattices])
+    ms = C.c_double()
+    rc = lib().axg_batch_advance(hs, len(lattices), t1, dt, nsteps, chunk, C.byref(ms))
+    if rc != 0:
+        raise AxgError(f"axg_batch_advance failed ({ERRORS.get(rc, rc)}): {lib().axg_last_error(lattices[0]._h).decode()}")
+    return ms.value
 
 
 def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initial_loop=True, kappa=1e-30,

@@ -35,6 +35,14 @@ struct KickScalars {
     double f1, f2, f3, m2, ma2, kt;  // kt: N_DW=2 tilt coefficient kappa0*t^4 (0 = off)
 };
 KickScalars kick_scalars(const axg_params& p, double t, double dt);
+// One step's scalars in device memory, for steps replayed from a captured graph: the step
+// kernels take them from here instead of their by-value arguments when the pointer is set.
+struct StepEntry {
+    KickScalars k;
+    double dtd;       // the drift after the kick
+    double tau, dw;   // the GW update's time and weight
+};
+void launch_step_select(hipStream_t s, const StepEntry* tab, int* counter, StepEntry* cur);
 double tilt_coef(const axg_params& p, double t);
 double maxion2(double thrm, double kappa);
 double msquared(double thrm);
@@ -58,13 +66,14 @@ void launch_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, const
 template <typename T, bool COMP>
 void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi,
                        const void* cphi_in, void* cphi_out, void* cpi, void* src,
-                       const KickScalars& k, double dspa, double dt_drift, bool gw);
+                       const KickScalars& k, double dspa, double dt_drift, bool gw,
+                       const StepEntry* se = nullptr);
 
 // Fused kick + source + x transform + drift for N = 2^k; false (nothing launched) otherwise.
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift, int variant);
+                             double dspa, double dt_drift, int variant, const StepEntry* se = nullptr);
 
 template <typename T>
 void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan,
@@ -145,7 +154,8 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
 
 template <typename T>
 void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
-                         const FftPlan& plan, const void* tw, const GwStepScalars& g);
+                         const FftPlan& plan, const void* tw, const GwStepScalars& g,
+                         const StepEntry* se = nullptr);
 
 template <typename T>
 void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int nb,

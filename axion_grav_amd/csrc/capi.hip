@@ -160,6 +160,10 @@ struct axg_handle {
     int64_t ckpt_cap = 0;  // checkpoints the buffer holds
     bool have_adj = false;
     double2* adj_pibar = nullptr;  // which of the two pibar buffers holds the result
+    // table-driven steps (axg_batch_advance): per-step scalars, the counter and the current entry
+    StepEntry* steptab = nullptr;
+    int64_t steptab_cap = 0;
+    StepEntry* stepcur = nullptr;  // [0] the current entry; the counter follows it
     double adj_scale = 1;          // the adjoint fields are held divided by this
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
@@ -641,6 +645,47 @@ template <typename T, bool COMP>
 static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
 {
     run_steps_hooked<T, COMP>(h, t1, dt, nt, [](int32_t, double) {});
+}
+
+// One DKD step whose scalars come from the handle's step table (the entry the head kernel
+// selects): the same kernels as run_steps, so the fields match it bit for bit.  Launches only,
+// so it can be captured into a graph; flips h->cur like a step does.
+template <typename T, bool COMP>
+static void table_step(axg_handle* h)
+{
+    const int N = h->N;
+    const axg_params& p = h->p;
+    StepEntry* cur = h->stepcur;
+    launch_step_select(h->stream, h->steptab, (int*)(cur + 1), cur);
+    const KickScalars k{};  // replaced by the table's entry in the kernels
+    const int nxt = 1 - h->cur;
+    bool fused = false;
+    if (p.grav_spec && h->fuse_x)
+        fused = launch_kick_source_xfft<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
+                                                 h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k, p.dspa, 0.0,
+                                                 h->kick_variant, cur);
+    if (!fused)
+        launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
+                                   h->cpi, h->src, k, p.dspa, 0.0, p.grav_spec != 0, cur);
+    if (p.grav_spec) {
+        if (!fused) launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw);
+        launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, h->ypass_variant);
+        GwStepScalars g;
+        g.tau = 0;
+        g.dw = 0;
+        g.dwav = p.dwav;
+        g.fix_swap = p.gw_fix_swap;
+        const void* wz = h->L.zt ? h->W2 : h->W;
+        launch_gw_zpass_acc<T>(h->stream, N, h->P, wz, h->A, h->L, h->plan, h->tw, g, cur);
+    }
+    h->cur = nxt;
+}
+
+static void table_step_any(axg_handle* h)
+{
+    if (h->comp) table_step<float, true>(h);
+    else if (h->fp32) table_step<float, false>(h);
+    else table_step<double, false>(h);
 }
 
 static void advance(axg_handle* h, double t1, double dt, int32_t nt)
@@ -1433,6 +1478,135 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
         // without convergence the loop counter ends at max_mc + 1, which the reference divides by
         for (int ij = 0; ij < ncov; ij++) cov[ij] = pc[ij] / iter;
         if (iters) *iters = converged ? iter : -iter;
+    });
+}
+
+// ------------------------------------------------------------------ batches of lattices
+
+// A captured graph and its instance, destroyed with the scope.
+struct GraphPair {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphPair()
+    {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    explicit EventSet(size_t n)
+    {
+        for (size_t i = 0; i < n; i++) {
+            hipEvent_t v;
+            AXG_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+            e.push_back(v);
+        }
+    }
+    ~EventSet()
+    {
+        for (hipEvent_t v : e) (void)hipEventDestroy(v);
+    }
+};
+
+extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, int32_t nsteps,
+                                 int32_t chunk, double* ms)
+{
+    axg_handle* h0 = (hs && n > 0) ? hs[0] : nullptr;
+    return guarded(h0, [&] {
+        require(hs != nullptr && n >= 1, "need at least one handle");
+        require(nsteps >= 1 && dt > 0 && t1 > 0 && std::isfinite(dt) && std::isfinite(t1),
+                "need nsteps >= 1, dt > 0 and t1 > 0");
+        require(chunk == 0 || (chunk >= 2 && chunk % 2 == 0 && chunk <= 64),
+                "chunk: 0 (no graph) or an even number of steps from 2 to 64");
+        for (int i = 0; i < n; i++) {
+            require(hs[i] != nullptr, "null handle");
+            require(hs[i]->p.device == h0->p.device, "all handles must be on one device");
+            for (int j = 0; j < i; j++) require(hs[j] != hs[i], "a handle appears twice");
+        }
+        // each lattice's table: the scalars run_steps would pass, step by step
+        for (int i = 0; i < n; i++) {
+            axg_handle* h = hs[i];
+            const axg_params& p = h->p;
+            std::vector<StepEntry> tab(nsteps);
+            double t = t1 + dt / 2;
+            for (int32_t s = 0; s < nsteps; s++) {
+                const double dtd = s == nsteps - 1 ? dt / 2 : dt;
+                tab[s].k = kick_scalars(p, t, dt);
+                tab[s].dtd = dtd;
+                tab[s].tau = t;
+                tab[s].dw = t * dt * p.dv_spa * (p.sigma_cr * p.sigma_cr);
+                t = t + dtd;
+            }
+            if (nsteps > h->steptab_cap) {
+                void* q = h->steptab;
+                AXG_HIP(hipStreamSynchronize(h->stream));
+                h->release_one(q);
+                h->steptab = nullptr;
+                h->steptab_cap = 0;
+                h->steptab = (StepEntry*)h->alloc((size_t)nsteps * sizeof(StepEntry));
+                h->steptab_cap = nsteps;
+            }
+            if (!h->stepcur) h->stepcur = (StepEntry*)h->alloc(2 * sizeof(StepEntry));
+            AXG_HIP(hipMemcpyAsync(h->steptab, tab.data(), (size_t)nsteps * sizeof(StepEntry), hipMemcpyHostToDevice,
+                                   h->stream));
+            const int zero = 0;
+            AXG_HIP(hipMemcpyAsync(h->stepcur + 1, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
+            AXG_HIP(hipStreamSynchronize(h->stream));
+        }
+        const auto c0 = std::chrono::steady_clock::now();
+        // first half drifts
+        for (int i = 0; i < n; i++) {
+            axg_handle* h = hs[i];
+            if (h->comp) launch_drift<float, true>(h->stream, h->N, h->phi[h->cur], h->phi[h->cur], h->pi,
+                                                   h->cphi[h->cur], h->cphi[h->cur], h->cpi, dt / 2);
+            else if (h->fp32) launch_drift<float, false>(h->stream, h->N, h->phi[h->cur], h->phi[h->cur], h->pi,
+                                                         nullptr, nullptr, nullptr, dt / 2);
+            else launch_drift<double, false>(h->stream, h->N, h->phi[h->cur], h->phi[h->cur], h->pi, nullptr,
+                                             nullptr, nullptr, dt / 2);
+        }
+        int32_t done = 0;
+        if (chunk > 0 && nsteps >= chunk) {
+            // one graph of `chunk` steps of every lattice, each lattice on its own stream (forked
+            // from and joined into the first), replayed; an even chunk returns the ping-pong
+            // buffers to where the capture found them
+            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+            hipStream_t s0 = h0->stream;
+            EventSet ev((size_t)n + 1);
+            GraphPair gp;
+            AXG_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+            AXG_HIP(hipEventRecord(ev.e[n], s0));
+            for (int i = 1; i < n; i++) AXG_HIP(hipStreamWaitEvent(hs[i]->stream, ev.e[n], 0));
+            for (int i = 0; i < n; i++)
+                for (int32_t c = 0; c < chunk; c++) table_step_any(hs[i]);
+            for (int i = 1; i < n; i++) {
+                AXG_HIP(hipEventRecord(ev.e[i], hs[i]->stream));
+                AXG_HIP(hipStreamWaitEvent(s0, ev.e[i], 0));
+            }
+            AXG_HIP(hipStreamEndCapture(s0, &gp.g));
+            AXG_HIP(hipGraphInstantiate(&gp.x, gp.g, nullptr, nullptr, 0));
+            for (; done + chunk <= nsteps; done += chunk) AXG_HIP(hipGraphLaunch(gp.x, s0));
+            AXG_HIP(hipEventRecord(ev.e[0], s0));
+            for (int i = 1; i < n; i++) AXG_HIP(hipStreamWaitEvent(hs[i]->stream, ev.e[0], 0));
+            for (int i = 0; i < n; i++)
+                for (int32_t s = done; s < nsteps; s++) table_step_any(hs[i]);
+            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+        } else {
+            for (int32_t s = 0; s < nsteps; s++)
+                for (int i = 0; i < n; i++) table_step_any(hs[i]);
+            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+        }
+        const auto c1 = std::chrono::steady_clock::now();
+        for (int i = 0; i < n; i++) {
+            axg_handle* h = hs[i];
+            h->have_flags = false;
+            h->have_spec = false;
+            h->nstr_rec = -1;
+            h->nwall_rec = -1;
+            h->have_grp[0] = h->have_grp[1] = false;
+        }
+        if (ms) *ms = std::chrono::duration<double, std::milli>(c1 - c0).count();
     });
 }
 

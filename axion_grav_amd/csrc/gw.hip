@@ -136,9 +136,14 @@ template <typename T, int N>
 __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                            typename V2<T>::type* __restrict__ A,
                                                                            const typename V2<T>::type* __restrict__ tw,
-                                                                           GwStepScalars g)
+                                                                           GwStepScalars g,
+                                                                           const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
+    if (se) {
+        g.tau = se->tau;
+        g.dw = se->dw;
+    }
     using L = TiledLayout<N, p2_ztx<T, N>()>;
     using T2 = typename V2<T>::type;
     constexpr int TX = p2_ztx<T, N>(), TPT = N / 8, NH = N / 2 + 1;
@@ -490,9 +495,13 @@ template <typename T>
 __global__ void __launch_bounds__(ZNT) gw_zpass_acc_kernel(int N, int P, const typename V2<T>::type* __restrict__ W,
                                                           typename V2<T>::type* __restrict__ A, FftPlan plan,
                                                           const typename V2<T>::type* __restrict__ tw,
-                                                          GwStepScalars g, int TX)
+                                                          GwStepScalars g, int TX, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
+    if (se) {
+        g.tau = se->tau;
+        g.dw = se->dw;
+    }
     using T2 = typename V2<T>::type;
     constexpr int KPT = fft_elems<T>() / ZNT;  // k-points per thread (upper bound)
     extern __shared__ unsigned char smem_raw[];
@@ -634,7 +643,7 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
 
 template <typename T>
 void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
-                         const FftPlan& plan, const void* tw, const GwStepScalars& g)
+                         const FftPlan& plan, const void* tw, const GwStepScalars& g, const StepEntry* se)
 {
     using T2 = typename V2<T>::type;
     bool done = L.zt && p2_dispatch(N, [&]<int NN>() {
@@ -642,7 +651,7 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
         if (!TiledLayout<NN, TXP>::matches(L))
             throw Error(AXG_EINVAL, "accumulation layout does not match the z pass tile");
         hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3(L.ntile, NN), dim3(TXP * (NN / 8)),
-                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, (const T2*)tw, g);
+                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, (const T2*)tw, g, se);
     });
     if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);
@@ -651,7 +660,7 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
     dim3 grid((nh + TX - 1) / TX, N);
     size_t lds = (size_t)N * TX * sizeof(T2);
     hipLaunchKernelGGL((gw_zpass_acc_kernel<T>), grid, dim3(ZNT), lds, s, N, pitch, (const T2*)W, (T2*)A, plan,
-                       (const T2*)tw, g, TX);
+                       (const T2*)tw, g, TX, se);
     AXG_HIP(hipGetLastError());
 }
 
@@ -771,7 +780,7 @@ void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, dou
     template void launch_gw_ypass<T>(hipStream_t, int, int, void*, void*, const GwLayout&, const FftPlan&,       \
                                      const void*, int);                                                              \
     template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const GwLayout&,             \
-                                         const FftPlan&, const void*, const GwStepScalars&);                    \
+                                         const FftPlan&, const void*, const GwStepScalars&, const StepEntry*);  \
     template void launch_gw_bin<T>(hipStream_t, int, const GwLayout&, const void*, const uint16_t*, int, double*, \
                                    int, double*);                                                               \
     template void launch_gw_pack<T>(hipStream_t, int, const GwLayout&, void*, int, double2*, int);

@@ -55,9 +55,13 @@ __global__ void kick_drift_kernel(int N, const typename V2<T>::type* __restrict_
                                   typename V2<T>::type* __restrict__ phi_out, typename V2<T>::type* __restrict__ pi,
                                   const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
                                   short2* __restrict__ cpi, typename V2<T>::type* __restrict__ src, KickScalars k,
-                                  double d, double dt_drift, int gw)
+                                  double d, double dt_drift, int gw, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
+    if (se) {
+        k = se->k;
+        dt_drift = se->dtd;
+    }
     using T2 = typename V2<T>::type;
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= N) return;
@@ -112,17 +116,34 @@ __global__ void kick_drift_kernel(int N, const typename V2<T>::type* __restrict_
     stf<T, COMP>(phi_out, cphi_out, i, f);
 }
 
+// cur = tab[*counter]; ++*counter (one thread, vector stores): the head of a table-driven step.
+__global__ void step_select_kernel(const StepEntry* __restrict__ tab, int* __restrict__ counter,
+                                   StepEntry* __restrict__ cur)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int c = *counter;
+        *cur = tab[c];
+        *counter = c + 1;
+    }
+}
+
+void launch_step_select(hipStream_t s, const StepEntry* tab, int* counter, StepEntry* cur)
+{
+    hipLaunchKernelGGL(step_select_kernel, dim3(1), dim3(64), 0, s, tab, counter, cur);
+    AXG_HIP(hipGetLastError());
+}
+
 template <typename T, bool COMP>
 void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                        void* cphi_out, void* cpi, void* src, const KickScalars& k, double dspa, double dt_drift,
-                       bool gw)
+                       bool gw, const StepEntry* se)
 {
     using T2 = typename V2<T>::type;
     int bx = N < 256 ? N : 256;
     dim3 grid((N + bx - 1) / bx, N, N);
     hipLaunchKernelGGL((kick_drift_kernel<T, COMP>), grid, dim3(bx), 0, s, N, (const T2*)phi_in, (T2*)phi_out,
                        (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)src, k, dspa * 2,
-                       dt_drift, gw ? 1 : 0);
+                       dt_drift, gw ? 1 : 0, se);
     AXG_HIP(hipGetLastError());
 }
 
@@ -198,9 +219,13 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, int P, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double dt_drift)
+    KickScalars k, double inv_d, double dt_drift, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
+    if (se) {
+        k = se->k;
+        dt_drift = se->dtd;
+    }
     using T2 = typename V2<T>::type;
     constexpr int TPT = N / 8, NR = 3 * XR;  // NR rows to transform
     constexpr int BS = XR * N;                 // threads per block
@@ -344,9 +369,13 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double dt_drift)
+    KickScalars k, double inv_d, double dt_drift, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
+    if (se) {
+        k = se->k;
+        dt_drift = se->dtd;
+    }
     using T2 = typename V2<T>::type;
     constexpr int TPT = N / 8, NR = 3 * XR, BS = XR * N / 2, H = N / 2;
     extern __shared__ unsigned char smem_raw[];
@@ -434,7 +463,7 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift, int variant)
+                             double dspa, double dt_drift, int variant, const StepEntry* se)
 {
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
@@ -445,7 +474,7 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
         hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
                            (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
-                           1.0 / (dspa * 2), dt_drift);
+                           1.0 / (dspa * 2), dt_drift, se);
     };
     auto go = [&]<int NN>() {
         constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
@@ -458,7 +487,7 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                     hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN),
                                        dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
                                        (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
-                                       (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift);
+                                       (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift, se);
                 };
                 constexpr int X2W = 2 * XR <= NN ? 2 * XR : XR;
                 const bool strip = (variant & 8) != 0 && (NN / X2W) % 8 == 0;
@@ -504,10 +533,11 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     template void launch_drift<T, C>(hipStream_t, int, const void*, void*, const void*, const void*, void*,   \
                                      const void*, double);                                                   \
     template void launch_kick_drift<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*,    \
-                                          void*, void*, const KickScalars&, double, double, bool);       \
+                                          void*, void*, const KickScalars&, double, double, bool,        \
+                                          const StepEntry*);                                              \
     template bool launch_kick_source_xfft<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*, \
                                                 void*, void*, int, const void*, const KickScalars&, double, double, \
-                                                int);
+                                                int, const StepEntry*);
 AXG_INST_STEP(double, false)
 AXG_INST_STEP(float, false)
 AXG_INST_STEP(float, true)

@@ -257,6 +257,17 @@ int axg_gradient(axg_handle* h, double t1, double dt, int32_t nsteps, int32_t K,
  * (re, im) pairs, x fastest; either pointer may be NULL. */
 int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite);
 
+/* Batches of lattices (no counterpart in the reference; the reference runs one lattice per
+ * job).  Advances n independent lattices -- any sizes and precisions, one device -- by nsteps
+ * DKD steps of dt from t1, each exactly as axg_advance_steps would (bit-identical fields and
+ * accumulators).  Every step's scalars go to device tables first, so one HIP graph of `chunk`
+ * steps of all the lattices (each on its own stream, forked from and joined into the first
+ * handle's) is captured once and replayed; the steps left over, and chunk = 0, launch the same
+ * kernels directly.  chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of
+ * the steps.  Errors are reported on hs[0]. */
+int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, int32_t nsteps, int32_t chunk,
+                      double* ms);
+
 /* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);
