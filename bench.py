@@ -205,6 +205,56 @@ def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_en
                 check=dict(steps=check_steps, adjoint_fp32c=ad, fd_fp64=fd, rel_err=rel))
 
 
+def run_batch(capi, steps=600, chunk=10, t_end=16.0):
+    """Independent realizations of uneven sizes advanced together: 16 x 64^3 + 16 x 96^3 +
+    16 x 128^3, dspa = 0.25, seeds 0..47, FP32C, GW on (64 bins), `steps` steps spanning tau
+    1 .. t_end (a fixed 0.2 dspa would pass the leapfrog's stability limit near tau = 23).
+    Times one captured graph of `chunk` steps of all 48 replayed, the same kernels launched
+    directly, and the lattices advanced one after another (axg_advance_steps); then the
+    per-realization GW spectra, averaged per size."""
+    sizes = [64] * 16 + [96] * 16 + [128] * 16
+    dt = (t_end - 1.0) / steps
+    lats = []
+    try:
+        for i, n in enumerate(sizes):
+            P = params(capi, n, 0.25 * n, capi.FP32C, 64)
+            lat = capi.Lattice(P)
+            lat.init_flat(i, 1.0)
+            lats.append(lat)
+        for lat, n in zip(lats, sizes):
+            if n in (64, 128):  # 96 = 3 * 32 has a single configuration
+                lat.tune(3)
+        init = [lat.get_fields() for lat in lats]
+        su = sum(n ** 3 for n in sizes) * steps
+
+        def reset():
+            for lat, (a, b) in zip(lats, init):
+                lat.set_fields(a, b)
+
+        capi.batch_advance(lats, 1.0, dt, 4, chunk)  # warm-up (and graph code paths)
+        reset()
+        t_graph = capi.batch_advance(lats, 1.0, dt, steps, chunk)
+        spec = {}
+        for n in (64, 96, 128):
+            sp = [lat.gw_spectrum()["power"] for lat, m in zip(lats, sizes) if m == n]
+            spec[n] = np.mean(sp, axis=0)
+        reset()
+        t_direct = capi.batch_advance(lats, 1.0, dt, steps, 0)
+        reset()
+        t0 = time.perf_counter()
+        for lat in lats:
+            lat.advance_steps(1.0, dt, steps)
+        t_seq = (time.perf_counter() - t0) * 1e3
+    finally:
+        for lat in lats:
+            lat.close()
+    return dict(workload=f"16 x 64^3 + 16 x 96^3 + 16 x 128^3, dspa 0.25, FP32C, GW on, {steps} steps of {dt:g}",
+                graph_chunk=chunk, graph_ms=round(t_graph, 1), direct_ms=round(t_direct, 1),
+                one_by_one_ms=round(t_seq, 1), su_s_graph=su / (t_graph * 1e-3),
+                su_s_one_by_one=su / (t_seq * 1e-3), speedup_graph_vs_one_by_one=round(t_seq / t_graph, 3),
+                size_mean_gw_power_finite=bool(all(np.all(np.isfinite(v)) for v in spec.values())))
+
+
 def copy_probe(nbytes=4 << 30, reps=10):
     """Device-to-device copy of nbytes (read + write = 2 nbytes per copy): the achievable
     streaming bandwidth on this GPU, measured with torch's copy kernel."""
@@ -253,6 +303,7 @@ def main():
     ap.add_argument("--no-diag", action="store_true", help="skip timing the diagnostics after the FP32C run")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in configuration of the fused pass")
     ap.add_argument("--no-adjoint", action="store_true", help="skip timing the gradient (128^3, 800 steps)")
+    ap.add_argument("--no-batch", action="store_true", help="skip timing the 48-realization batch")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
                          "its kernel is the known-size reference for the FETCH/WRITE counters)")
@@ -286,6 +337,12 @@ def main():
         log("adjoint: 128^3 FP32C, 800 steps, checkpoints every 50")
         adj = run_adjoint(capi)
         log(f"adjoint: {adj}")
+
+    bat = None
+    if not args.no_batch:
+        log("batch: 16 x 64^3 + 16 x 96^3 + 16 x 128^3 FP32C, 600 steps")
+        bat = run_batch(capi)
+        log(f"batch: {bat}")
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
     tc = c["tuned"] or {"choice": 0, "ypass": 0}
@@ -331,6 +388,7 @@ def main():
         "diagnostics": c["diagnostics"],
         "tuning": {"fp32c": c["tuned"], "fp64": None if d is None else d["tuned"]},
         "adjoint": adj,
+        "batch": bat,
     }
     print(json.dumps(out), flush=True)
 
