@@ -59,17 +59,61 @@ __device__ void pol_all(int nx, int ny, int nz, double dwav, double2 pol[6], dou
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ __forceinline__ double2 cconj(double2 a) { return {a.x, -a.y}; }
 
-// Dbar_c at one stored mode (half index hk) for all six components.
+// pol_all in FP32 (for FP32 adjoint sources): the geometry to FP32 precision; q in FP64.
+__device__ void pol_all_f(int nx, int ny, int nz, double dwav, double2 pol[6], double& q)
+{
+#pragma clang fp contract(off)
+    const int n2 = nx * nx + ny * ny + nz * nz;
+    q = sqrt((double)n2) * dwav;
+    if (n2 == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) pol[c] = double2{0, 0};
+        return;
+    }
+    const float inv = rsqrtf((float)n2);
+    const float hx = nx * inv, hy = ny * inv, hz = nz * inv;
+    const int rr2 = nx * nx + ny * ny;
+    float cc = 1, ss = 0, r = 0;
+    if (rr2 > 0) {
+        const float ir = rsqrtf((float)rr2);
+        cc = nx * ir;
+        ss = ny * ir;
+        r = sqrtf((float)rr2) * inv;
+    }
+    const float2 p[3] = {{hz * cc, -ss}, {hz * ss, cc}, {-r, 0}};
+    const int ii[6] = {0, 1, 1, 2, 2, 2}, jj[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const float2 a = p[ii[c]], b = p[jj[c]];
+        float2 v = {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+        if (ii[c] == jj[c]) v = float2{v.x * 0.5f, v.y * 0.5f};
+        pol[c] = double2{v.x, v.y};
+    }
+}
+
+// Dbar_c at one stored mode (half index hk) for all six components; FAST: the mode geometry
+// and the phase's sine and cosine in FP32 (the phase itself reduced in FP64).
+template <bool FAST>
 __device__ void dbar_all(int N, int kx, int ky, int kz, const double2* __restrict__ abar, int64_t ms, int64_t hk,
                          const AdjStepScalars& g, double2 out[6])
 {
 #pragma clang fp contract(off)
     double2 pol[6];
     double q;
-    pol_all(wrapi(kx, N), wrapi(ky, N), wrapi(kz, N), g.dwav, pol, q);
-    const double ph = q * g.tau;
     double s, c;
-    sincos(ph, &s, &c);
+    if constexpr (FAST) {
+        pol_all_f(wrapi(kx, N), wrapi(ky, N), wrapi(kz, N), g.dwav, pol, q);
+        double ph = q * g.tau;
+        ph = ph - 6.283185307179586 * rint(ph * 0.15915494309189535);
+        float sf, cf;
+        sincosf((float)ph, &sf, &cf);
+        s = sf;
+        c = cf;
+    } else {
+        pol_all(wrapi(kx, N), wrapi(ky, N), wrapi(kz, N), g.dwav, pol, q);
+        const double ph = q * g.tau;
+        sincos(ph, &s, &c);
+    }
     const double2 a11 = abar[hk], a21 = abar[ms + hk], a12 = abar[2 * ms + hk], a22 = abar[3 * ms + hk];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -122,7 +166,7 @@ __global__ void adj_modes_kernel(int N, const double2* __restrict__ abar, AdjSte
         const int y = (int)(r % N), z = (int)(r / N);
         const int ym = (N - y) % N, zm = (N - z) % N;
         double2 d[6], H[6];
-        dbar_all(N, x, y, z, abar, ms, hk, g, d);
+        dbar_all<sizeof(TW) == 4>(N, x, y, z, abar, ms, hk, g, d);
         if (x > 0 && x < N / 2) {
 #pragma unroll
             for (int c = 0; c < 6; c++) H[c] = double2{d[c].x / 2, d[c].y / 2};
@@ -132,7 +176,7 @@ __global__ void adj_modes_kernel(int N, const double2* __restrict__ abar, AdjSte
             store_pairs<TW>(W0, W1, W2, ((int64_t)zm * N + ym) * N + (N - x), H);
         } else {
             double2 e[6];
-            dbar_all(N, x, ym, zm, abar, ms, ((int64_t)zm * N + ym) * nh + x, g, e);
+            dbar_all<sizeof(TW) == 4>(N, x, ym, zm, abar, ms, ((int64_t)zm * N + ym) * nh + x, g, e);
 #pragma unroll
             for (int c = 0; c < 6; c++) H[c] = double2{(d[c].x + e[c].x) / 2, (d[c].y - e[c].y) / 2};
             store_pairs<TW>(W0, W1, W2, ((int64_t)z * N + y) * N + x, H);
