@@ -9,6 +9,14 @@ template <typename T> struct V2;
 template <> struct V2<double> { using type = double2; };
 template <> struct V2<float> { using type = float2; };
 
+// The lattice sizes with compile-time kernels (fft_p2.h): N = 2^L in 16..1024 and
+// N = 3 * 2^L in 24..768.  Other sizes run the runtime-plan kernels.
+constexpr bool p2_size(int n)
+{
+    const int pw = n % 3 == 0 ? n / 3 : n;
+    return pw >= 8 && (pw & (pw - 1)) == 0 && n >= 16 && n <= 1024;
+}
+
 template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b)

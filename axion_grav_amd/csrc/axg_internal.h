@@ -69,7 +69,7 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
                        const KickScalars& k, double dspa, double dt_drift, bool gw,
                        const StepEntry* se = nullptr);
 
-// Fused kick + source + x transform + drift for N = 2^k; false (nothing launched) otherwise.
+// Fused kick + source + x transform + drift for p2_size(N); false (nothing launched) otherwise.
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
@@ -87,7 +87,7 @@ template <typename T, int SIGN>
 void launch_fft_rows(hipStream_t s, int N, void* data, int64_t nrows, const FftPlan& plan, const void* tw);
 
 // Layout of the half-spectrum arrays read by the accumulation pass (the y-transformed sources
-// and the accumulators).  zt == 0: rows [c][kz][ky][kx] with pitch P.  zt > 0 (N = 2^k):
+// and the accumulators).  zt == 0: rows [c][kz][ky][kx] with pitch P.  zt > 0 (p2_size(N)):
 // tiles [c][ky][t][kz][zt] with t = kx / zt, so one accumulation block (one ky, one tile)
 // reads and writes contiguous N*zt runs whatever the tile width.
 struct GwLayout {
@@ -104,12 +104,12 @@ struct GwLayout {
 };
 GwLayout gw_layout(int N, bool fp32);
 
-// half_pitch(N) as a compile-time constant for the N = 2^k kernels.
+// half_pitch(N) as a compile-time constant for the compile-time (p2_size) kernels.
 template <int N> constexpr int half_pitch_c() { return ((N / 2 + 1) + 15) / 16 * 16; }
 
-// The tiled GwLayout with compile-time N and tile width ZT (powers of two), so index
-// arithmetic in the N = 2^k kernels reduces to shifts and masks.  Launchers check that the
-// handle's runtime layout is this one.
+// The tiled GwLayout with compile-time N and tile width ZT (a power of two), so index
+// arithmetic in the compile-time kernels reduces to shifts, masks and constant divisions.
+// Launchers check that the handle's runtime layout is this one.
 template <int N, int ZT> struct TiledLayout {
     static constexpr int NT = (N / 2 + 1 + ZT - 1) / ZT;
     static constexpr int64_t COMP = (int64_t)N * NT * N * ZT;

@@ -112,13 +112,13 @@ struct axg_handle {
     void* cpi = nullptr;
     void* src = nullptr;
     void* W = nullptr;
-    void* W2 = nullptr;      // y-transformed sources in the tiled layout (N = 2^k)
+    void* W2 = nullptr;      // y-transformed sources in the tiled layout (p2_size(N))
     void* A = nullptr;
     GwLayout L{};            // layout of W2 (or W) and A as the accumulation pass reads them
     void* tw = nullptr;      // twiddles in the storage precision
     void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
     int cur = 0;
-    bool fuse_x = true;      // x transform fused into the stencil pass when N = 2^k
+    bool fuse_x = true;      // x transform fused into the stencil pass when p2_size(N)
     int kick_variant = 0;    // configuration of the fused pass (axg_tune; 0 = built-in choice)
     int ypass_variant = 0;   // configuration of the y pass (axg_tune; 0 = 128-byte column tiles)
     // diagnostics
@@ -420,7 +420,7 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         }
         if (p.grav_spec) {
             // the packed source planes only exist when the x transform is not fused
-            if ((N & (N - 1)) != 0 || !h->fuse_x) h->src = h->alloc(3 * n3 * h->t2);
+            if (!p2_size(N) || !h->fuse_x) h->src = h->alloc(3 * n3 * h->t2);
             h->L = gw_layout(N, h->fp32);
             h->W = h->alloc((size_t)6 * N * N * h->P * h->t2);
             if (h->L.zt) h->W2 = h->alloc((size_t)6 * h->L.comp() * h->t2);
@@ -747,7 +747,7 @@ extern "C" int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* m
         const int N = h->N;
         const int nv = (int)(sizeof kTuneVariants / sizeof kTuneVariants[0]);
         for (int i = 0; ms && i < nms; i++) ms[i] = 0;
-        const bool tunable = h->p.grav_spec && h->fuse_x && (N & (N - 1)) == 0 && N >= 64;
+        const bool tunable = h->p.grav_spec && h->fuse_x && p2_size(N) && N >= 64;
         if (!tunable) {  // one configuration only
             if (variant) *variant = h->kick_variant | (h->ypass_variant << 8);
             return;

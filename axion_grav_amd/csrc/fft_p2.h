@@ -1,20 +1,42 @@
-// Compile-time radix-8 Stockham FFT for N = 2^L (16 <= N <= 1024), register resident.
+// Compile-time radix-8 Stockham FFT for N = 2^L (16 <= N <= 1024) or N = 3 * 2^L
+// (24 <= N <= 768), register resident.
 //
 // A transform is carried by TPT = N/8 threads; thread j0 holds the 8 elements
 // x[j0 + e*TPT], e = 0..7 (for a batch in LDS, element n of the thread's column sits at
-// s[n*es + off]).  Stages are radix 8, the last one radix 2 or 4 when L is not a multiple
-// of 3.  A stage of radix R and span Ns runs 8/R butterflies per thread: butterfly
-// j = j0 + m*TPT reads x[j + r*N/R] = v[m + r*(8/R)], twiddles by w^(r*(j mod Ns)) with
-// w = exp(SIGN 2 pi i/(Ns R)), and writes y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  Between
-// stages the outputs go through LDS and come back in the x[j0 + e*TPT] layout; after the
-// last stage v[e] already holds output element j0 + e*TPT (natural order), so the first
-// load and the last store go straight to and from HBM, coalesced along j0.
+// s[n*es + off]).  The register stages cover the power-of-two factor PW (= N, or N/3): radix 8,
+// the last one radix 2 or 4 when log2(PW) is not a multiple of 3.  A stage of radix R and span
+// Ns runs 8/R butterflies per thread: butterfly j = j0 + m*TPT reads x[j + r*N/R] =
+// v[m + r*(8/R)], twiddles by w^(r*(j mod Ns)) with w = exp(SIGN 2 pi i/(Ns R)), and writes
+// y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].  Between stages the outputs go through LDS and come back
+// in the x[j0 + e*TPT] layout.  For N = 3 * 2^L a radix-3 step of span PW follows (p3_finish):
+// each thread forms its own 8 outputs from the three LDS values they depend on.  Either way
+// v[e] ends holding output element j0 + e*TPT (natural order), so the first load and the last
+// store go straight to and from HBM, coalesced along j0.
 #pragma once
+#include <utility>
+
 #include "axg_device.h"
 
+constexpr int p2_ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
+// largest power of two p <= lim that divides n (rows per block of the compile-time kernels)
+constexpr int p2_rows_div(int n, int lim)
+{
+    int p = 1;
+    while (2 * p <= lim && n % (2 * p) == 0) p *= 2;
+    return p;
+}
+
 template <int N> struct P2 {
-    static_assert(N >= 16 && N <= 1024 && (N & (N - 1)) == 0, "P2 plans cover 16..1024, powers of two");
-    static constexpr int L = N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 : N == 128 ? 7 : N == 256 ? 8 : N == 512 ? 9 : 10;
+    static_assert(p2_size(N), "P2 plans cover N = 2^L in 16..1024 and N = 3 * 2^L in 24..768");
+    static constexpr bool R3 = N % 3 == 0;  // a radix-3 step after the register stages
+    static constexpr int PW = R3 ? N / 3 : N;
+    static constexpr int L = p2_ilog2(PW);
     static constexpr int N8 = L / 3;
     static constexpr int REM = L % 3;
     static constexpr int NST = N8 + (REM ? 1 : 0);
@@ -28,12 +50,30 @@ template <int N> struct P2 {
     }
 };
 
+// Periodic neighbours and mirror on a ring of N sites (masks when N is a power of two).
+template <int N> __device__ __forceinline__ int ring_dn(int i)
+{
+    if constexpr ((N & (N - 1)) == 0) return (i - 1) & (N - 1);
+    else return i == 0 ? N - 1 : i - 1;
+}
+template <int N, int D = 1> __device__ __forceinline__ int ring_up(int i)
+{
+    if constexpr ((N & (N - 1)) == 0) return (i + D) & (N - 1);
+    else return i + D >= N ? i + D - N : i + D;
+}
+template <int N> __device__ __forceinline__ int ring_neg(int i)
+{
+    if constexpr ((N & (N - 1)) == 0) return (N - i) & (N - 1);
+    else return i == 0 ? 0 : N - i;
+}
+
 // The twiddles a thread needs depend only on j0: stage S >= 1 uses M*(R-1) <= 7 of them.
 // Loading them all before the first stage (next to the data loads, or before a barrier)
 // takes their latency and address arithmetic out of the stages.
 template <typename T, int N> struct P2Tw {
     static constexpr int NS = P2<N>::NST > 1 ? P2<N>::NST - 1 : 1;
     typename V2<T>::type w[NS][7];
+    typename V2<T>::type w3[2];  // w^j0 and w^(2 j0) of the radix-3 step (N = 3 * 2^L)
 };
 
 template <typename T, int N, int SIGN, int S = 1>
@@ -62,6 +102,14 @@ __device__ __forceinline__ P2Tw<T, N> p2_load_twiddles(int j0, const typename V2
 {
     P2Tw<T, N> t;
     p2_twiddles<T, N, SIGN>(t, j0, tw);
+    if constexpr (P2<N>::R3) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            typename V2<T>::type w = tw[(i + 1) * j0];
+            if constexpr (SIGN < 0) w.y = -w.y;
+            t.w3[i] = w;
+        }
+    }
     return t;
 }
 
@@ -88,6 +136,41 @@ __device__ __forceinline__ typename V2<T>::type p2_tw(const P2TwTable<T, N, SIGN
     typename V2<T>::type w = t.tw[r * k * TS];
     if constexpr (SIGN < 0) w.y = -w.y;
     return w;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ typename V2<T>::type p3_tw(const P2Tw<T, N>& t, int i)
+{
+    return t.w3[i];
+}
+
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ typename V2<T>::type p3_tw(const P2TwTable<T, N, SIGN>& t, int i)
+{
+    typename V2<T>::type w = t.tw[(i + 1) * t.j0];
+    if constexpr (SIGN < 0) w.y = -w.y;
+    return w;
+}
+
+// a * exp(SIGN 2 pi i E/8): exact swaps and negations for even E, one (1 +- i)/sqrt(2)
+// product for odd E
+template <int SIGN, int E, typename T2> __device__ __forceinline__ T2 rot8(T2 a)
+{
+    constexpr int e = E & 7;
+    if constexpr (e & 1) {
+        using T = decltype(a.x);
+        const T h = (T)0.70710678118654752440084436210485;
+        const T2 b = SIGN > 0 ? T2{(a.x - a.y) * h, (a.y + a.x) * h} : T2{(a.x + a.y) * h, (a.y - a.x) * h};
+        return rot8<SIGN, e - 1>(b);
+    } else if constexpr (e == 0) {
+        return a;
+    } else if constexpr (e == 4) {
+        return T2{-a.x, -a.y};
+    } else if constexpr ((e == 2) == (SIGN > 0)) {
+        return T2{-a.y, a.x};  // times i
+    } else {
+        return T2{a.y, -a.x};  // times -i
+    }
 }
 
 template <typename T, int N, int S, int SIGN, typename TW>
@@ -154,6 +237,45 @@ __device__ __forceinline__ void p2_exchange_if(bool active, typename V2<T>::type
     __syncthreads();
 }
 
+// The radix-3 step of N = 3 * 2^L after the last register stage (span PW = N/3).  Its
+// butterfly j < PW maps y[j + r PW] to X[j + r' PW] = sum_r y[j + r PW] w^(r j) w3^(r r')
+// with w = exp(SIGN 2 pi i/N), w3 = w^PW, i.e. X[n] = sum_r y[(n mod PW) + r PW] w^(r n).  The
+// thread's outputs n = j0 + e TPT (TPT = N/8) need w^n = w^j0 exp(SIGN 2 pi i e/8) and
+// w^(2n) = w^(2 j0) exp(SIGN 2 pi i 2e/8), so two table twiddles serve all eight.
+template <typename T, int N, int SIGN, typename TW>
+__device__ __forceinline__ void p3_finish(bool active, typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s,
+                                          int es, int off, const TW& tws)
+{
+    using T2 = typename V2<T>::type;
+    constexpr int S = P2<N>::NST - 1, R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
+    constexpr int PW = P2<N>::PW;
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int j = j0 + m * TPT;
+            const int k = j & (Ns - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) s[(base + r * Ns) * es + off] = v[m + r * M];
+        }
+    }
+    __syncthreads();
+    if (active) {
+        const T2 a = p3_tw(tws, 0), b = p3_tw(tws, 1);
+        [&]<int... E>(std::integer_sequence<int, E...>) {
+            (
+                [&] {
+                    const int jj = (j0 + E * TPT) & (PW - 1);
+                    const T2 y0 = s[jj * es + off], y1 = s[(jj + PW) * es + off], y2 = s[(jj + 2 * PW) * es + off];
+                    const T2 u = cmul(y1, rot8<SIGN, E>(a)), w = cmul(y2, rot8<SIGN, 2 * E>(b));
+                    v[E] = T2{y0.x + u.x + w.x, y0.y + u.y + w.y};
+                }(),
+                ...);
+        }(std::make_integer_sequence<int, 8>{});
+    }
+    __syncthreads();
+}
+
 // tws: p2_load_twiddles<T, N, SIGN>(j0, tw) of the same thread (registers), or
 // P2TwTable<T, N, SIGN>{tw, j0} (read in the stages).
 template <typename T, int N, int SIGN, int S = 0, typename TW>
@@ -163,6 +285,7 @@ __device__ __forceinline__ void fft_p2_if(bool active, typename V2<T>::type (&v)
     if constexpr (S < P2<N>::NST) {
         if (active) p2_stage<T, N, S, SIGN>(v, tws);
         if constexpr (S + 1 < P2<N>::NST) p2_exchange_if<T, N, S>(active, v, j0, s, es, off);
+        else if constexpr (P2<N>::R3) p3_finish<T, N, SIGN>(active, v, j0, s, es, off, tws);
         fft_p2_if<T, N, SIGN, S + 1>(active, v, j0, s, es, off, tws);
     }
 }
@@ -174,6 +297,7 @@ __device__ __forceinline__ void fft_p2(typename V2<T>::type (&v)[8], int j0, typ
     if constexpr (S < P2<N>::NST) {
         p2_stage<T, N, S, SIGN>(v, tws);
         if constexpr (S + 1 < P2<N>::NST) p2_exchange<T, N, S>(v, j0, s, es, off);
+        else if constexpr (P2<N>::R3) p3_finish<T, N, SIGN>(true, v, j0, s, es, off, tws);
         fft_p2<T, N, SIGN, S + 1>(v, j0, s, es, off, tws);
     }
 }

@@ -7,7 +7,7 @@
 // kx = 0..N/2.  W[c][z][y][kx] (c = 0..5 in the reference's (i,j) order (1,1),(2,1),(2,2),
 // (3,1),(3,2),(3,3); row pitch P = half_pitch(N)) holds the x-transformed sources.  The y
 // transform writes them out of place into W2 in the tiled GwLayout [c][ky][kx/8][kz][kx%8]
-// (N = 2^k; in place in W with rows [c][kz][ky][kx] otherwise), and the accumulation pass
+// (the sizes of fft_p2.h; in place in W with rows [c][kz][ky][kx] otherwise), and the accumulation pass
 // z-transforms them on the fly.  The accumulators A[m][...] (m = arr_grav (1,1),(2,1),(1,2),
 // (2,2)) use the same GwLayout, so that pass reads and writes contiguous runs.
 #include "axg_internal.h"
@@ -27,7 +27,7 @@ template <typename T> int col_width(int N)
 }
 
 
-// ================================================================== N = 2^L specializations
+// ================================================================== N = 2^L, 3 * 2^L specializations
 // The same three passes with the compile-time radix-8 engine of fft_p2.h: every thread
 // carries 8 elements of one transform in registers, the first load and last store go
 // directly between registers and HBM, and LDS is touched only between stages.
@@ -41,7 +41,7 @@ template <typename T, int N> constexpr int p2_ztx()
     return tx;
 }
 
-template <int N> constexpr int p2_xrows() { return (2048 / N) < N ? (2048 / N < 1 ? 1 : 2048 / N) : N; }
+template <int N> constexpr int p2_xrows() { return (2048 / N) < N ? p2_rows_div(N, 2048 / N < 1 ? 1 : 2048 / N) : N; }
 
 template <typename T, int N>
 __global__ void __launch_bounds__(p2_xrows<N>() * (N / 8)) gw_xpass_p2_kernel(const typename V2<T>::type* __restrict__ src,
@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(p2_xrows<N>() * (N / 8)) gw_xpass_p2_kernel(co
         const int k = j0 + e * TPT;
         if (k <= N / 2) {
             T2 xk = row[k];
-            T2 xm = row[(N - k) & (N - 1)];
+            T2 xm = row[ring_neg<N>(k)];
             wa[k] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
             wb[k] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
         }
@@ -279,10 +279,17 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
     }
 }
 
-// Runs F.template operator()<N>() for a power-of-two N in 16..1024; returns false otherwise.
+// Runs F.template operator()<N>() for N = 2^L in 16..1024 or 3 * 2^L in 24..768 (the sizes of
+// fft_p2.h); returns false otherwise.
 template <typename F> bool p2_dispatch(int N, F&& f)
 {
     switch (N) {
+    case 24: f.template operator()<24>(); return true;
+    case 48: f.template operator()<48>(); return true;
+    case 96: f.template operator()<96>(); return true;
+    case 192: f.template operator()<192>(); return true;
+    case 384: f.template operator()<384>(); return true;
+    case 768: f.template operator()<768>(); return true;
     case 16: f.template operator()<16>(); return true;
     case 32: f.template operator()<32>(); return true;
     case 64: f.template operator()<64>(); return true;
@@ -415,7 +422,7 @@ __global__ void __launch_bounds__(FFT_NT) fft_rows_kernel(int N, int TB, int64_t
     for (int e = threadIdx.x; e < nrow * N; e += FFT_NT) base[e] = s[e];
 }
 
-// Rows of N = 2^k: RB rows per block of RB * N/8 threads, each thread holding 8 elements of its
+// Rows of the compile-time sizes: RB rows per block of RB * N/8 threads, each thread holding 8 elements of its
 // row in registers (fft_p2.h), loads and stores coalesced along the row.
 template <int N> constexpr int p2_rows_per_block() { return (N / 8) >= 256 ? 1 : 256 / (N / 8); }
 
@@ -634,7 +641,7 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
             if (variant == 1) go.template operator()<TXH>();
             else go.template operator()<TXP>();
         });
-        if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k");
+        if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k or 3 * 2^k");
         AXG_HIP(hipGetLastError());
         return;
     }

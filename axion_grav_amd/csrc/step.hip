@@ -148,7 +148,7 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
 }
 
 
-// ------------------------------------------------------------------ fused pass for N = 2^L
+// ------------------------------------------------------------------ fused pass for N = 2^L, 3 * 2^L
 //
 // The same kick + source + drift, with the x transform of the three packed source planes
 // done in the same block (fft_p2.h): block = XR rows y at one z with one thread per site
@@ -160,10 +160,10 @@ void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, 
 // written to W[2m] and W[2m+1] (the y and z passes follow).  The six source reals per site
 // never reach HBM.
 
-// rows per block: blocks of 512 threads (measured at 512^3: one row per block beats two by
-// 5-7 % in both precisions), one row when N >= 512
-template <int N> constexpr int fused_rows() { return N >= 512 ? 1 : (512 / N < N ? 512 / N : N); }
-template <int N> constexpr int fused_rows_wide() { return N >= 1024 ? 1 : (1024 / N < N ? 1024 / N : N); }
+// rows per block: a power of two (it divides N) with blocks of up to 512 threads (measured at
+// 512^3: one row per block beats two by 5-7 % in both precisions), one row when N >= 512
+template <int N> constexpr int fused_rows() { return N >= 512 ? 1 : (512 / N < N ? p2_rows_div(N, 512 / N) : N); }
+template <int N> constexpr int fused_rows_wide() { return N >= 1024 ? 1 : (1024 / N < N ? p2_rows_div(N, 1024 / N) : N); }
 
 // One site of the fused pass: kick of p (evolve.f90:325-330 with potential.f90:43-60, in the
 // reference's order), the next drift f = c + dt_drift p, and the gradient (grav_waves.f90:
@@ -249,9 +249,9 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     {
         const int rr = threadIdx.x / N, x = threadIdx.x % N;
         const int y = yg * XR + rr;
-        const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
-        const int zm = (z - 1) & (N - 1), zp = (z + 1) & (N - 1);
-        const int xm = (x - 1) & (N - 1), xp = (x + 1) & (N - 1);
+        const int ym = ring_dn<N>(y), yp = ring_up<N>(y);
+        const int zm = ring_dn<N>(z), zp = ring_up<N>(z);
+        const int xm = ring_dn<N>(x), xp = ring_up<N>(x);
         const int64_t row = (int64_t)z * NN + (int64_t)y * N;
         const int64_t i = row + x;
         double2 c = ldf<T, COMP>(phi_in, cphi_in, i);
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
         const int y = yg * XR + rr;
         const T2* rowp = s + q * N;
         T2 xk = rowp[kk];
-        T2 xm = rowp[(N - kk) & (N - 1)];
+        T2 xm = rowp[ring_neg<N>(kk)];
         T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
         wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
         wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
@@ -395,13 +395,13 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     {
         const int rr = threadIdx.x / H, x0 = 2 * (threadIdx.x % H);
         const int y = yg * XR + rr;
-        const int ym = (y - 1) & (N - 1), yp = (y + 1) & (N - 1);
-        const int zm = (z - 1) & (N - 1), zp = (z + 1) & (N - 1);
+        const int ym = ring_dn<N>(y), yp = ring_up<N>(y);
+        const int zm = ring_dn<N>(z), zp = ring_up<N>(z);
         const int64_t row = (int64_t)z * NN + (int64_t)y * N;
         double2 c0, c1, l, r, b10, b11, b20, b21, z10, z11, z20, z21, p0, p1;
         ld_pair<T, COMP>(phi_in, cphi_in, row + x0, c0, c1);
-        l = ldf<T, COMP>(phi_in, cphi_in, row + ((x0 - 1) & (N - 1)));
-        r = ldf<T, COMP>(phi_in, cphi_in, row + ((x0 + 2) & (N - 1)));
+        l = ldf<T, COMP>(phi_in, cphi_in, row + ring_dn<N>(x0));
+        r = ldf<T, COMP>(phi_in, cphi_in, row + ring_up<N, 2>(x0));
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)ym * N + x0, b10, b11);
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)yp * N + x0, b20, b21);
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x0, z10, z11);
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
         const int y = yg * XR + rr;
         const T2* rowp = s + qq * N;
         T2 xk = rowp[kk];
-        T2 xm = rowp[(N - kk) & (N - 1)];
+        T2 xm = rowp[ring_neg<N>(kk)];
         T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
         wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
         wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
@@ -489,7 +489,9 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                                        (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
                                        (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift, se);
                 };
-                constexpr int X2W = 2 * XR <= NN ? 2 * XR : XR;
+                // wide: twice the rows when they divide N and the tile stays within 64 KiB of LDS
+                constexpr int X2W =
+                    2 * XR <= NN && NN % (2 * XR) == 0 && 6 * XR * NN * (int)sizeof(T2) <= 65536 ? 2 * XR : XR;
                 const bool strip = (variant & 8) != 0 && (NN / X2W) % 8 == 0;
                 if (variant & 1) {
                     if (strip) launch2.template operator()<X2W, true>();
@@ -523,6 +525,12 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     case 256: go.template operator()<256>(); break;
     case 512: go.template operator()<512>(); break;
     case 1024: go.template operator()<1024>(); break;
+    case 24: go.template operator()<24>(); break;
+    case 48: go.template operator()<48>(); break;
+    case 96: go.template operator()<96>(); break;
+    case 192: go.template operator()<192>(); break;
+    case 384: go.template operator()<384>(); break;
+    case 768: go.template operator()<768>(); break;
     default: return false;
     }
     AXG_HIP(hipGetLastError());

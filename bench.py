@@ -221,9 +221,8 @@ def run_batch(capi, steps=600, chunk=10, t_end=16.0):
             lat = capi.Lattice(P)
             lat.init_flat(i, 1.0)
             lats.append(lat)
-        for lat, n in zip(lats, sizes):
-            if n in (64, 128):  # 96 = 3 * 32 has a single configuration
-                lat.tune(3)
+        for lat in lats:
+            lat.tune(3)
         init = [lat.get_fields() for lat in lats]
         su = sum(n ** 3 for n in sizes) * steps
 

@@ -10,7 +10,8 @@ from tests.conftest import SQ_HALF, phys
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(16, capi.FP64), (24, capi.FP32C), (32, capi.FP32C), (64, capi.FP32), (32, capi.FP64)]
+CASES = [(16, capi.FP64), (24, capi.FP32C), (32, capi.FP32C), (64, capi.FP32), (32, capi.FP64), (36, capi.FP32C),
+         (48, capi.FP64)]
 
 
 def make(N, prec, seed):

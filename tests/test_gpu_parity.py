@@ -117,12 +117,11 @@ def test_time_reversal_residual(prec, tol):
     assert np.abs(b_pi - pi).max() < tol * np.abs(phi).max(), np.abs(b_pi - pi).max()
 
 
-@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
-def test_tune_keeps_fields_and_results(prec):
+@pytest.mark.parametrize("prec,N", [(capi.FP64, 64), (capi.FP32C, 64), (capi.FP32C, 96)])
+def test_tune_keeps_fields_and_results(prec, N):
     """axg_tune times the fused pass's configurations on the live lattice: the fields are the
     same before and after, and the steps after tuning match untuned steps bit for bit."""
-    say("tune", prec)
-    N = 64
+    say("tune", prec, N)
     prm, P = params(N, precision=prec)
     phi, pi = ic(N, 9, 4.0)
     with capi.Lattice(prm) as L:
@@ -146,7 +145,7 @@ def test_tune_keeps_fields_and_results(prec):
 # ---------------------------------------------------------------- GW accumulation
 
 @pytest.mark.parametrize("N,nsteps,fix", [(16, 1, False), (16, 3, True), (24, 2, False), (32, 4, False),
-                                           (64, 2, False)])
+                                           (64, 2, False), (48, 2, True), (36, 2, False), (36, 1, True)])
 def test_gw_accumulators_fp64(N, nsteps, fix):
     say("gw", N, nsteps, fix)
     prm, P = params(N, gw_fix_swap=fix)
@@ -187,7 +186,8 @@ def test_gw_spectrum_fp64(N):
 
 
 def test_gw_radix3_n96_short():
-    """N = 96 exercises the radix-3 FFT stage (96 = 3 * 8 * 4)."""
+    """N = 96 = 3 * 32 runs the compile-time kernels with their radix-3 step (8 * 4, then 3);
+    N = 36 in the accumulator test above runs the runtime-plan kernels."""
     say("gw n96")
     N = 96
     prm, P = params(N)
@@ -480,9 +480,9 @@ def test_thermal_state_rejects_tau_ini_at_or_above_one():
 
 # ---------------------------------------------------------------- FP32C vs FP64
 
-def test_fp32c_tracks_fp64_gw():
-    say("fp32c vs fp64")
-    N = 32
+@pytest.mark.parametrize("N", [32, 96])
+def test_fp32c_tracks_fp64_gw(N):
+    say("fp32c vs fp64", N)
     phi, pi = ic(N, 6, 3.0)
     out = {}
     for prec in (capi.FP64, capi.FP32C, capi.FP32):
