@@ -9,6 +9,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <chrono>
@@ -1248,6 +1249,18 @@ static void fft3d_work(axg_handle* h, double2* d, int sign = +1)
     }
 }
 
+// Runs body(lo, hi) over [0, n) split into contiguous ranges on up to 16 host threads (the
+// one-time spectrum tables: each index writes only its own outputs, so the split changes nothing).
+template <typename F> static void host_parallel(int64_t n, F&& body)
+{
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({(int64_t)(hc ? hc : 1), (int64_t)16, n}));
+    std::vector<std::thread> pool;
+    for (int64_t k = 1; k < nt; k++) pool.emplace_back([&, k] { body(n * k / nt, n * (k + 1) / nt); });
+    body(0, n / nt);
+    for (auto& t : pool) t.join();
+}
+
 static void ensure_spec_tables(axg_handle* h)
 {
     if (h->d_largeq) return;
@@ -1256,22 +1269,24 @@ static void ensure_spec_tables(axg_handle* h)
     // spectrum.f90:74-98
     std::vector<double> largeq((size_t)ncov * (nlq + 1)), mat((size_t)ncov * (nlq + 1)), dqs(ncov);
     const std::vector<double>& wav = h->wav;
-    for (int ij = 1; ij <= ncov; ij++) {
-        int i = (int)lround(sqrt(2.0 * ij));
-        int j = ij - i * (i - 1) / 2;
-        double lj = wav[j], sj = wav[j - 1], li = wav[i], si = wav[i - 1];
-        double maxq = li + lj;
-        double minq = std::max(si, sj) - std::min(li, lj);
-        if (minq < 0) minq = 0;
-        double d = maxq - minq;
-        d = d / nlq;
-        dqs[ij - 1] = d;
-        for (int iq = 0; iq <= nlq; iq++) {
-            double q = minq + d * iq;
-            largeq[(size_t)(ij - 1) * (nlq + 1) + iq] = q;
-            mat[(size_t)(ij - 1) * (nlq + 1) + iq] = integrate_p(200, li, si, lj, sj, q);
+    host_parallel(ncov, [&](int64_t lo, int64_t hi) {
+        for (int ij = (int)lo + 1; ij <= (int)hi; ij++) {
+            int i = (int)lround(sqrt(2.0 * ij));
+            int j = ij - i * (i - 1) / 2;
+            double lj = wav[j], sj = wav[j - 1], li = wav[i], si = wav[i - 1];
+            double maxq = li + lj;
+            double minq = std::max(si, sj) - std::min(li, lj);
+            if (minq < 0) minq = 0;
+            double d = maxq - minq;
+            d = d / nlq;
+            dqs[ij - 1] = d;
+            for (int iq = 0; iq <= nlq; iq++) {
+                double q = minq + d * iq;
+                largeq[(size_t)(ij - 1) * (nlq + 1) + iq] = q;
+                mat[(size_t)(ij - 1) * (nlq + 1) + iq] = integrate_p(200, li, si, lj, sj, q);
+            }
         }
-    }
+    });
     h->d_largeq = (double*)h->alloc(largeq.size() * 8);
     h->d_matspec = (double*)h->alloc(mat.size() * 8);
     h->d_dqs = (double*)h->alloc(dqs.size() * 8);
@@ -1286,11 +1301,18 @@ static void ensure_spec_tables(axg_handle* h)
             int ny = (y < N / 2) ? y : y - N, nz = (z < N / 2) ? z : z - N;
             r2[(size_t)ny * ny + (size_t)nz * nz] += 1;
         }
-    for (int x = 0; x < N; x++) {
-        int nx = (x < N / 2) ? x : x - N;
-        for (size_t m = 0; m < r2.size(); m++)
-            if (r2[m] != 0) cnt[(size_t)nx * nx + m] += r2[m];
-    }
+    // cnt[k] = sum over x of r2[k - nx^2]: each thread owns a range of k (integer counts, exact
+    // in any order)
+    host_parallel(h->nn2, [&](int64_t lo, int64_t hi) {
+        for (int x = 0; x < N; x++) {
+            const int64_t nx = (x < N / 2) ? x : x - N, s2 = nx * nx;
+            const int64_t k0 = std::max<int64_t>(lo, s2), k1 = std::min<int64_t>(hi, s2 + (int64_t)r2.size());
+            for (int64_t k = k0; k < k1; k++) {
+                const double v = r2[(size_t)(k - s2)];
+                if (v != 0) cnt[(size_t)k] += v;
+            }
+        }
+    });
     h->d_cnt = (double*)h->alloc(h->nn2 * 8);
     AXG_HIP(hipMemcpy(h->d_cnt, cnt.data(), h->nn2 * 8, hipMemcpyHostToDevice));
     h->d_hist = (double*)h->alloc(h->nn2 * 8);

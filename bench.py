@@ -254,6 +254,46 @@ def run_batch(capi, steps=600, chunk=10, t_end=16.0):
                 size_mean_gw_power_finite=bool(all(np.all(np.isfinite(v)) for v in spec.values())))
 
 
+def run_largest(capi, N=1024, L=256.0, steps=10, warmup=2, kappa0=1e-6, seed=2024):
+    """The largest single-GPU share: 1024^3, L = 256, FP32C with GW (256 bins), the N_DW = 2 tilt
+    kappa0 tau^4, seed 2024.  Times `steps` steps after `warmup`, then one round of the
+    diagnostics the driver runs at a checkpoint (classification, string and wall fits, GW and
+    field spectra with 256 bins; the first field spectrum also builds the deconvolution tables)."""
+    P = params(capi, N, L, capi.FP32C, 256)
+    P.potential_kappa0 = kappa0
+    P.num_bin = 256
+    dt = 0.2 * (L / N)
+    d = {}
+    with capi.Lattice(P) as lat:
+        lat.init_flat(seed, 1.0)
+        tuned = lat.tune(3)
+        lat.advance_steps(1.0, dt, warmup)
+        t = 1.0 + warmup * dt
+        lat.profile(True)
+        t0 = time.perf_counter()
+        lat.advance_steps(t, dt, steps)
+        el = time.perf_counter() - t0
+        prof = lat.profile(False)
+        t += steps * dt
+        for name, fn in (("gw_spectrum_ms", lat.gw_spectrum), ("defects_ms", lat.defects),
+                         ("strings_ms", lambda: lat.strings(1, 2)), ("walls_ms", lat.walls),
+                         ("field_spectrum_first_ms", lambda: lat.field_spectrum(t)),
+                         ("field_spectrum_ms", lambda: lat.field_spectrum(t))):
+            t1 = time.perf_counter()
+            r = fn()
+            d[name] = round((time.perf_counter() - t1) * 1e3, 2)
+            if name == "defects_ms":
+                d["n_string_cells"], d["n_wall_cells"], d["n_mask_cells"] = (int(v) for v in r)
+            if name == "gw_spectrum_ms":
+                d["gw_power_finite"] = bool(np.all(np.isfinite(r["power"])))
+    kern = {k: round(ms / cnt, 3) for k, (ms, cnt) in prof.items() if cnt}
+    return dict(workload=f"{N}^3, L={L:g}, FP32C, GW on (256 bins), kappa0={kappa0:g}, seed {seed}, "
+                         f"{steps} steps of {dt:g} after {warmup}",
+                su_s=steps * N ** 3 / el, ms_per_step=round(el / steps * 1e3, 3), kernel_ms=kern,
+                tuning={"choice": tuned[0], "ypass": tuned[1], "ms": {k: round(v, 3) for k, v in tuned[2].items()}},
+                diagnostics=d)
+
+
 def copy_probe(nbytes=4 << 30, reps=10):
     """Device-to-device copy of nbytes (read + write = 2 nbytes per copy): the achievable
     streaming bandwidth on this GPU, measured with torch's copy kernel."""
@@ -303,6 +343,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in configuration of the fused pass")
     ap.add_argument("--no-adjoint", action="store_true", help="skip timing the gradient (128^3, 800 steps)")
     ap.add_argument("--no-batch", action="store_true", help="skip timing the 48-realization batch")
+    ap.add_argument("--largest", action="store_true",
+                    help="also time the largest single-GPU share (1024^3 FP32C with GW and walls, ~130 GB)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
                          "its kernel is the known-size reference for the FETCH/WRITE counters)")
@@ -342,6 +384,12 @@ def main():
         log("batch: 16 x 64^3 + 16 x 96^3 + 16 x 128^3 FP32C, 600 steps")
         bat = run_batch(capi)
         log(f"batch: {bat}")
+
+    big = None
+    if args.largest:
+        log("largest: 1024^3 FP32C, GW on, N_DW = 2 tilt")
+        big = run_largest(capi)
+        log(f"largest: {big}")
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
     tc = c["tuned"] or {"choice": 0, "ypass": 0}
@@ -388,6 +436,7 @@ def main():
         "tuning": {"fp32c": c["tuned"], "fp64": None if d is None else d["tuned"]},
         "adjoint": adj,
         "batch": bat,
+        "largest": big,
     }
     print(json.dumps(out), flush=True)
 
