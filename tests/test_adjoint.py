@@ -57,6 +57,8 @@ def test_oracle_adjoint_matches_finite_differences(fix):
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,N,K,fix,tol,units", [("fp64", 16, 4, False, 1e-7, "phys"),
                                                     ("fp64", 16, 5, True, 1e-7, "phys"),
+                                                    ("fp64", 24, 4, False, 1e-7, "phys"),
+                                                    ("fp64", 36, 5, True, 1e-7, "phys"),
                                                     ("fp32c", 32, 4, False, 2e-4, "phys"),
                                                     ("fp32c", 32, 6, False, 2e-4, "unit")])
 def test_gpu_gradient_matches_oracle(prec, N, K, fix, tol, units):
