@@ -261,9 +261,10 @@ int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite)
  * job).  Advances n independent lattices -- any sizes and precisions, one device -- by nsteps
  * DKD steps of dt from t1, each exactly as axg_advance_steps would (bit-identical fields and
  * accumulators).  Every step's scalars go to device tables first, so one HIP graph of `chunk`
- * steps of all the lattices (each on its own stream, forked from and joined into the first
- * handle's) is captured once and replayed; the steps left over, and chunk = 0, launch the same
- * kernels directly.  chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of
+ * steps of all the lattices (dealt to two streams, forked from and joined into the first
+ * handle's; environment variable AXG_BATCH_LANES sets another count) is captured once and
+ * replayed; the steps left over, and chunk = 0, launch the same kernels directly (each lattice
+ * on its own stream).  chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of
  * the steps.  Errors are reported on hs[0]. */
 int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, int32_t nsteps, int32_t chunk,
                       double* ms);

@@ -24,11 +24,15 @@ def make(N, prec, seed):
     return prm, phi, pi
 
 
-@pytest.mark.parametrize("nsteps,chunk", [(7, 2), (8, 4), (5, 0), (3, 4)])
-def test_batch_matches_single_lattices(nsteps, chunk):
+@pytest.mark.parametrize("nsteps,chunk,lanes", [(7, 2, None), (8, 4, None), (5, 0, None), (3, 4, None),
+                                                (6, 2, "1"), (6, 2, "3"), (4, 4, "64")])
+def test_batch_matches_single_lattices(nsteps, chunk, lanes, monkeypatch):
     """Odd step counts leave a tail of direct steps after the graph replays; chunk > nsteps runs
-    no graph at all; chunk = 0 launches every step directly."""
-    print(f"[case] batch nsteps={nsteps} chunk={chunk}", flush=True)
+    no graph at all; chunk = 0 launches every step directly.  AXG_BATCH_LANES deals the lattices
+    to 1, 3 or (clamped) all streams in the graph instead of the default two."""
+    print(f"[case] batch nsteps={nsteps} chunk={chunk} lanes={lanes}", flush=True)
+    if lanes is not None:
+        monkeypatch.setenv("AXG_BATCH_LANES", lanes)
     t1, dt = 1.0, 0.05
     lats, ref = [], []
     for i, (N, prec) in enumerate(CASES):
