@@ -202,6 +202,31 @@ def test_gw_radix3_n96_short():
         assert relerr(A[m], OA[m][:, :, : N // 2 + 1]) < 1e-10
 
 
+@pytest.mark.parametrize("N", [32, 48])
+def test_gw_unfused_x_pass_matches_fused(N, monkeypatch):
+    """AXG_XFUSE=0 runs the x transform as its own pass (kick_drift_kernel + gw_xpass_p2_kernel):
+    with dspa a power of two the FP64 fields equal the fused pass's bit for bit; the accumulators
+    agree to FFT rounding (the two kernels' transforms may fuse multiply-adds differently), and
+    both match the oracle."""
+    say("unfused x", N)
+    prm, P = params(N)
+    phi, pi = ic(N, 8, 4.0)
+    out = []
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("AXG_XFUSE", fuse)
+        with capi.Lattice(prm) as L:
+            L.set_fields(phi, pi)
+            L.advance_steps(1.0, 0.05, 2)
+            out.append((L.get_fields()[0], L.gw_state()))
+    assert np.array_equal(out[0][0], out[1][0])
+    OA = O.new_gw_state(N)
+    O.symplectic(phi, pi, OA, 1.0, 0.05, 2, P)
+    for m in range(4):
+        assert relerr(out[1][1][m], out[0][1][m]) < 1e-13
+        for f in out:
+            assert relerr(f[1][m], OA[m][:, :, : N // 2 + 1]) < 1e-10
+
+
 # ---------------------------------------------------------------- defects / spectra / energy
 
 @pytest.mark.parametrize("N,kmax", [(16, 6.0), (32, 6.0)])
