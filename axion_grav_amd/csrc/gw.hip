@@ -32,12 +32,16 @@ template <typename T> int col_width(int N)
 // carries 8 elements of one transform in registers, the first load and last store go
 // directly between registers and HBM, and LDS is touched only between stages.
 
-// The accumulation pass keeps 8 k-points' projected sums per thread: at most 512 threads per
-// block so the compiler has 256 VGPRs.
+// The accumulation pass keeps 8 k-points' projected sums per thread.  With at most 512 threads
+// per block the compiler has 256 VGPRs for the pipelined FP32 form.  FP32 from N = 512 instead
+// takes tiles of up to 1024 threads (16 columns at 512, 8 at 1024; 128 VGPRs, no pipelining),
+// so the y pass writes 128-byte (512) or 64-byte (1024) tile rows instead of half of that.
+// Measured: y pass 1.80 -> 1.55 ms at 512^3 and 35 -> 15 ms at 1024^3, z pass within 3 %.
+template <typename T, int N> constexpr bool p2_zwide() { return sizeof(T) == 4 && N >= 512; }
 template <typename T, int N> constexpr int p2_ztx()
 {
     int tx = p2_tx<T, N>();
-    while (tx > 1 && tx * (N / 8) > 512) tx /= 2;
+    while (tx > 1 && tx * (N / 8) > (p2_zwide<T, N>() ? 1024 : 512)) tx /= 2;
     return tx;
 }
 
@@ -172,23 +176,26 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
 #pragma unroll
     for (int e = 0; e < 8; e++) sp[e] = sn[e] = T2{0, 0};
     // per wavevector: qz = nz/|n| and r = |n_xy|/|n| (one reciprocal; k = 0 is zeroed after the
-    // component loop)
-    T gz[8], gr[8];
-    {
-        const T rho2 = (T)nx * (T)nx + (T)ny * (T)ny, rho = sqrt(rho2);
+    // component loop); held for the 8 k-points, or recomputed per component in the 1024-thread
+    // form, whose 128 VGPRs cannot hold them
+    const T rho2g = (T)nx * (T)nx + (T)ny * (T)ny, rhog = sqrt(rho2g);
+    auto geom = [&](int e, T& z, T& r) {
+        const int nz = wrapn(j0 + e * TPT, N);
+        const T q = sqrt(rho2g + (T)nz * (T)nz);
+        const T iq = q > 0 ? (T)1 / q : (T)0;
+        z = (T)nz * iq;
+        r = rhog * iq;
+    };
+    constexpr int NG8 = p2_zwide<T, N>() ? 1 : 8;
+    T gz[NG8], gr[NG8];
+    if constexpr (!p2_zwide<T, N>()) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int nz = wrapn(j0 + e * TPT, N);
-            const T q = sqrt(rho2 + (T)nz * (T)nz);
-            const T iq = q > 0 ? (T)1 / q : (T)0;
-            gz[e] = (T)nz * iq;
-            gr[e] = rho * iq;
-        }
+        for (int e = 0; e < 8; e++) geom(e, gz[e], gr[e]);
     }
 
     // FP32: twiddles in registers and component c+1 loaded while component c is transformed
     // (software pipelining); FP64 has no registers to spare for either.
-    constexpr bool PIPE = sizeof(T) == 4, PTW = sizeof(T) == 4;
+    constexpr bool PIPE = sizeof(T) == 4 && !p2_zwide<T, N>(), PTW = PIPE;
     T2 v[8], nxt[8];
     if constexpr (PIPE) {
 #pragma unroll
@@ -222,7 +229,10 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
             const int jj = (c == 0 || c == 1 || c == 3) ? 0 : ((c == 2 || c == 4) ? 1 : 2);
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const T2 p1 = {gz[e] * gc, -gs}, p2 = {gz[e] * gs, gc}, p3 = {-gr[e], 0};
+                T ge, re;
+                if constexpr (p2_zwide<T, N>()) geom(e, ge, re);
+                else { ge = gz[e]; re = gr[e]; }
+                const T2 p1 = {ge * gc, -gs}, p2 = {ge * gs, gc}, p3 = {-re, 0};
                 const T2 a = ii == 0 ? p1 : (ii == 1 ? p2 : p3);
                 const T2 bb = jj == 0 ? p1 : (jj == 1 ? p2 : p3);
                 T2 pol = {a.x * bb.x - a.y * bb.y, a.x * bb.y + a.y * bb.x};

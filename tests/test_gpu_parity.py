@@ -527,6 +527,33 @@ def test_fp32c_tracks_fp64_gw(N):
     assert np.array_equal(c_s["denom"], ref_s["denom"])
 
 
+def test_fp32_wide_tiles_track_fp64_gw_at_512():
+    """From N = 512 the FP32 transforms use 16-column (512) or 8-column (1024) z tiles with
+    1024-thread accumulation blocks: the FP32C and FP32 GW spectra after 6 steps from the same
+    device-made field stay within FP32 transform rounding of FP64's.  Bins holding more than
+    1e-6 of the peak agree to 1e-4 each; the rest (a few steps from a smooth field leave the
+    highest bins ~1e-20 of the peak) sit at the FP32 noise floor, below 1e-6 of the peak."""
+    say("fp32 wide tiles 512")
+    N = 512
+    out = {}
+    for prec in (capi.FP64, capi.FP32C, capi.FP32):
+        prm, _ = params(N, precision=prec, num_bin_grav=64)
+        with capi.Lattice(prm) as L:
+            L.init_flat(5, 4.0)
+            L.advance_steps(1.0, 0.05, 6)
+            out[prec] = L.gw_spectrum()
+    ref = out[capi.FP64]
+    top = ref["numer"].max()
+    big = ref["numer"] > 1e-6 * top
+    assert big.sum() > 30
+    for prec in (capi.FP32C, capi.FP32):
+        got = out[prec]["numer"]
+        err = np.abs(got[big] - ref["numer"][big]) / ref["numer"][big]
+        assert err.max() < 1e-4, (prec, err.max())
+        assert np.abs(got - ref["numer"]).max() < 1e-6 * top
+        assert np.array_equal(out[prec]["denom"], ref["denom"])
+
+
 def test_advance_get_dtau_interval():
     say("advance interval")
     N = 16
