@@ -209,11 +209,35 @@ __device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, dou
     }
 }
 
+// Sub-strips per XCD strip of SG row groups of XR rows: the largest power of two dividing SG that
+// keeps at least 64 rows x 512 sites per sub-strip, so the three planes of a sub-strip an XCD
+// works on (about 1.2 MB in FP32C) fit its 4 MB L2 also at N = 1024.
+constexpr int strip_subs(int SG, int XR, int N)
+{
+    int sub = 1;
+    while (SG % (2 * sub) == 0 && (SG / (2 * sub)) * XR * N >= 64 * 512) sub *= 2;
+    return sub;
+}
+
+// Block order of the XCD strips: block lin goes to XCD lin % 8 as its (lin / 8)-th block.  XCD k
+// takes the strip of row groups [k SG, (k+1) SG), split into SUB sub-strips, and walks each
+// sub-strip plane by plane before the next.
+template <int N, int XR>
+__device__ __forceinline__ void strip_order(int lin, int& yg, int& z)
+{
+    constexpr int NG = N / XR, SG = NG >= 8 ? NG / 8 : 1;  // launched only when NG % 8 == 0
+    constexpr int SUB = strip_subs(SG, XR, N), SS = SG / SUB;
+    const int l = lin >> 3;
+    const int sub = l / (SS * N), m = l - sub * (SS * N);
+    yg = (lin & 7) * SG + sub * SS + m % SS;
+    z = m / SS;
+}
+
 // XCD-aware block orders: the dispatcher deals consecutive blocks to the 8 XCDs in turn.
 // REMAP 1 (planes): XCD k walks a contiguous run of planes, so the blocks it holds at once are
-// neighbouring rows of one plane and share their y halo rows in its L2.  REMAP 2 (strips): XCD
-// k takes a strip of NG/8 row groups and walks it plane by plane, so its blocks share both
-// the y and the z halo rows.
+// neighbouring rows of one plane and share their y halo rows in its L2.  REMAP 2 (strips,
+// strip_order): XCD k takes a strip of NG/8 row groups and walks it plane by plane, so its
+// blocks share both the y and the z halo rows.
 template <typename T, bool COMP, int N, int XR, int REMAP>
 __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
@@ -240,11 +264,7 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
         yg = logical % NG;
         z = logical / NG;
     } else if constexpr (REMAP == 2) {
-        constexpr int SG = NG >= 8 ? NG / 8 : 1;  // launched only when NG % 8 == 0
-        const int lin = blockIdx.y * NG + blockIdx.x;
-        const int l = lin >> 3;
-        yg = (lin & 7) * SG + l % SG;
-        z = l / SG;
+        strip_order<N, XR>(blockIdx.y * NG + blockIdx.x, yg, z);
     }
     {
         const int rr = threadIdx.x / N, x = threadIdx.x % N;
@@ -383,14 +403,9 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const int64_t NN = (int64_t)N * N;
     int yg = blockIdx.x, z = blockIdx.y;
     if constexpr (STRIP) {
-        // XCD-aware strips: block lin goes to XCD lin % 8 as its (lin / 8)-th block; each XCD
-        // takes a strip of NG/8 row groups and walks it plane by plane, so the y and z
-        // neighbour rows its blocks read were fetched by its own recent blocks (its L2)
-        constexpr int NG = N / XR, SG = NG >= 8 ? NG / 8 : 1;  // launched only when NG % 8 == 0
-        const int lin = blockIdx.y * NG + blockIdx.x;
-        const int l = lin >> 3;
-        yg = (lin & 7) * SG + l % SG;
-        z = l / SG;
+        // XCD-aware strips (strip_order): the y and z neighbour rows a block reads were fetched
+        // by its XCD's own recent blocks (its L2)
+        strip_order<N, XR>(blockIdx.y * (N / XR) + blockIdx.x, yg, z);
     }
     {
         const int rr = threadIdx.x / H, x0 = 2 * (threadIdx.x % H);

@@ -554,6 +554,24 @@ def test_fp32_wide_tiles_track_fp64_gw_at_512():
         assert np.array_equal(out[prec]["denom"], ref["denom"])
 
 
+def test_strip_orders_bit_identical_at_768(monkeypatch):
+    """At N = 768 the XCD strips split into sub-strips (strip_order): the fused pass in the
+    dispatcher's order (0), in strips (8) and in strips with wide blocks (9) gives the same
+    energy terms and GW spectrum bit for bit (every site is updated exactly once)."""
+    say("strip orders 768")
+    N = 768
+    prm, _ = params(N, precision=capi.FP32C, num_bin_grav=32)
+    out = []
+    for v in ("0", "8", "9"):
+        monkeypatch.setenv("AXG_KICK_VARIANT", v)
+        with capi.Lattice(prm) as L:
+            L.init_flat(3, 2.0)
+            L.advance_steps(1.0, 0.05, 2)
+            out.append((L.energy(1.1), L.gw_spectrum()["numer"]))
+    for e, g in out[1:]:
+        assert np.array_equal(e, out[0][0]) and np.array_equal(g, out[0][1])
+
+
 def test_advance_get_dtau_interval():
     say("advance interval")
     N = 16
