@@ -185,11 +185,12 @@ def test_gw_spectrum_fp64(N):
     assert np.array_equal(s["qmid"], o["qmid"])
 
 
-def test_gw_radix3_n96_short():
-    """N = 96 = 3 * 32 runs the compile-time kernels with their radix-3 step (8 * 4, then 3);
-    N = 36 in the accumulator test above runs the runtime-plan kernels."""
-    say("gw n96")
-    N = 96
+@pytest.mark.parametrize("N", [96, 192, 384])
+def test_gw_radix3_short(N):
+    """N = 3 * 2^k runs the compile-time kernels with their radix-3 step after the register
+    stages: 96 = (8 * 4) * 3, 192 = (8 * 8) * 3, 384 = (8 * 8 * 2) * 3.  N = 36 in the
+    accumulator test above runs the runtime-plan kernels."""
+    say("gw radix-3", N)
     prm, P = params(N)
     phi, pi = ic(N, 4, 2.0)
     with capi.Lattice(prm) as L:
@@ -482,7 +483,7 @@ def test_string_velocity_of_moving_string():
 
 # ---------------------------------------------------------------- thermal initial state
 
-@pytest.mark.parametrize("N,prec", [(32, capi.FP64), (48, capi.FP64), (32, capi.FP32C)])
+@pytest.mark.parametrize("N,prec", [(32, capi.FP64), (48, capi.FP64), (32, capi.FP32C), (192, capi.FP64)])
 def test_thermal_state_matches_oracle(N, prec):
     """The GPU thermal state (same Philox draws, device exp/log/sqrt, FP64 transform) against
     the oracle: <= 1e-12 relative in FP64, the FP32C storage rounding otherwise."""
