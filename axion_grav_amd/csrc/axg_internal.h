@@ -111,6 +111,7 @@ template <int N> constexpr int half_pitch_c() { return ((N / 2 + 1) + 15) / 16 *
 // arithmetic in the compile-time kernels reduces to shifts, masks and constant divisions.
 // Launchers check that the handle's runtime layout is this one.
 template <int N, int ZT> struct TiledLayout {
+    static constexpr int ZTW = ZT;
     static constexpr int NT = (N / 2 + 1 + ZT - 1) / ZT;
     static constexpr int64_t COMP = (int64_t)N * NT * N * ZT;
     __host__ __device__ static int64_t idx(int c, int kz, int ky, int kx)

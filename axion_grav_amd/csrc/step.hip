@@ -321,16 +321,23 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     // split every transformed row into two half spectra: NR rows x (N/2+1) outputs
     constexpr int NH = N / 2 + 1, PC = half_pitch_c<N>();
     constexpr int64_t cs = (int64_t)N * N * PC;  // component stride of W
-    for (int t = threadIdx.x; t < NR * NH; t += BS) {
-        const int q = t / NH, kk = t - q * NH;
+    // whole rows of pitch PC: the padding columns NH..PC-1 get zeros, so the last 128-byte line
+    // of a row is written whole instead of in part
+    for (int t = threadIdx.x; t < NR * PC; t += BS) {
+        const int q = t / PC, kk = t - q * PC;
         const int m = q / XR, rr = q - m * XR;
         const int y = yg * XR + rr;
-        const T2* rowp = s + q * N;
-        T2 xk = rowp[kk];
-        T2 xm = rowp[ring_neg<N>(kk)];
         T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
-        wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
-        wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        if (kk < NH) {
+            const T2* rowp = s + q * N;
+            T2 xk = rowp[kk];
+            T2 xm = rowp[ring_neg<N>(kk)];
+            wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+            wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        } else {
+            wa[kk] = T2{0, 0};
+            wa[cs + kk] = T2{0, 0};
+        }
     }
 }
 
@@ -462,16 +469,23 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     }
     constexpr int NH = N / 2 + 1, PC = half_pitch_c<N>();
     constexpr int64_t cs = (int64_t)N * N * PC;
-    for (int t = threadIdx.x; t < NR * NH; t += BS) {
-        const int qq = t / NH, kk = t - qq * NH;
+    // whole rows of pitch PC: the padding columns NH..PC-1 get zeros, so the last 128-byte line
+    // of a row is written whole instead of in part
+    for (int t = threadIdx.x; t < NR * PC; t += BS) {
+        const int qq = t / PC, kk = t - qq * PC;
         const int m = qq / XR, rr = qq - m * XR;
         const int y = yg * XR + rr;
-        const T2* rowp = s + qq * N;
-        T2 xk = rowp[kk];
-        T2 xm = rowp[ring_neg<N>(kk)];
         T2* wa = W + (int64_t)(2 * m) * cs + ((int64_t)z * N + y) * PC;
-        wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
-        wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        if (kk < NH) {
+            const T2* rowp = s + qq * N;
+            T2 xk = rowp[kk];
+            T2 xm = rowp[ring_neg<N>(kk)];
+            wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
+            wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+        } else {
+            wa[kk] = T2{0, 0};
+            wa[cs + kk] = T2{0, 0};
+        }
     }
 }
 
