@@ -387,7 +387,7 @@ class Lattice:
     TUNE_CONFIGS = ("fused: built-in", "fused: wide blocks", "fused: other sites per thread",
                     "fused: other sites, wide blocks", "fused: XCD strips", "fused: XCD strips, wide blocks",
                     "fused: other sites, XCD strips", "fused: other sites, XCD strips, wide blocks",
-                    "y: 128-byte tiles", "y: 64-byte tiles")
+                    "y: 128-byte tiles", "y: 64-byte tiles", "y: two planes per block")
 
     def tune(self, reps=5):
         """Autotune the fused pass and the y transform on this lattice:

@@ -801,7 +801,7 @@ extern "C" int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* m
         double ybest = 1e300;
         int ybestv = h->ypass_variant;
         if (h->L.zt) {
-            for (int yv = 0; yv < 2; yv++) {
+            for (int yv = 0; yv < 3; yv++) {
                 const double avg = time_avg([&] {
                     if (h->fp32) launch_gw_ypass<float>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, yv);
                     else launch_gw_ypass<double>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, yv);

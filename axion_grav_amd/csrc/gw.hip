@@ -108,8 +108,11 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
 }
 
 // y transform of the x-transformed sources W[c][z][y][kx] (pitch P) into W2 in the tiled
-// layout of the accumulation pass.  Block = TX kx columns of one (c, z) plane.
-template <typename T, int N, int TX>
+// layout of the accumulation pass.  Block = TX kx columns of KZ consecutive (c, z) planes: the
+// planes are transformed one after the other through the same LDS tile, and their outputs for
+// one (ky, kx) are adjacent tile rows, stored back to back (KZ = 2 turns the 64-byte rows of
+// 8-column FP32 or 4-column FP64 tiles into whole 128-byte lines).
+template <typename T, int N, int TX, int KZ>
 __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                     typename V2<T>::type* __restrict__ W2,
                                                                     const typename V2<T>::type* __restrict__ tw)
@@ -121,20 +124,29 @@ __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typen
     T2* s = reinterpret_cast<T2*>(smem_raw);
     const int b = threadIdx.x % TX, j0 = threadIdx.x / TX;
     const int kx = blockIdx.x * TX + b;
-    const int c = blockIdx.y / N, z = blockIdx.y % N;
+    const int c = blockIdx.y / (N / KZ), z0 = (blockIdx.y % (N / KZ)) * KZ;
     const bool ok = kx < NH;
-    const T2* base = W + ((int64_t)c * N + z) * (int64_t)N * P + kx;
-    T2 v[8];
+    T2 v[KZ][8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
+    for (int q = 0; q < KZ; q++) {
+        const T2* base = W + ((int64_t)c * N + z0 + q) * (int64_t)N * P + kx;
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[q][e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
+    }
     // twiddles read inside the stages (measured: no faster with them held in registers)
-    fft_p2<T, N, +1>(v, j0, s, TX, b, P2TwTable<T, N, +1>{tw, j0});
+#pragma unroll
+    for (int q = 0; q < KZ; q++) {
+        if (q > 0) __syncthreads();
+        fft_p2<T, N, +1>(v[q], j0, s, TX, b, P2TwTable<T, N, +1>{tw, j0});
+    }
     // the padding columns of the last tile (NH <= kx < NT * ZT) get zeros, so its rows are
     // written whole; a block wider than the tile (TX > ZT) also covers columns past the last
     // tile, which do not exist
     if (kx < L::NT * L::ZTW) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) W2[L::idx(c, z, j0 + e * TPT, kx)] = ok ? v[e] : T2{0, 0};
+        for (int e = 0; e < 8; e++)
+#pragma unroll
+            for (int q = 0; q < KZ; q++) W2[L::idx(c, z0 + q, j0 + e * TPT, kx)] = ok ? v[q][e] : T2{0, 0};
     }
 }
 
@@ -657,15 +669,17 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
         bool done = p2_dispatch(N, [&]<int NN>() {
             if (!TiledLayout<NN, p2_ztx<T, NN>()>::matches(L) || pitch != half_pitch_c<NN>())
                 throw Error(AXG_EINVAL, "y pass: layout does not match the compiled tile");
-            // variant 1: half-width column tiles (more, smaller blocks per CU)
-            auto go = [&]<int TXP>() {
-                hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP>), dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN),
-                                   dim3(TXP * (NN / 8)), (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2,
-                                   (const T2*)tw);
+            // variant 1: half-width column tiles (more, smaller blocks per CU); variant 2: two
+            // z planes per block
+            auto go = [&]<int TXP, int KZ>() {
+                hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP, KZ>),
+                                   dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ), dim3(TXP * (NN / 8)),
+                                   (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw);
             };
             constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1;
-            if (variant == 1) go.template operator()<TXH>();
-            else go.template operator()<TXP>();
+            if (variant == 1) go.template operator()<TXH, 1>();
+            else if (variant == 2) go.template operator()<TXP, 2>();
+            else go.template operator()<TXP, 1>();
         });
         if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k or 3 * 2^k");
         AXG_HIP(hipGetLastError());

@@ -281,8 +281,9 @@ int axg_energy(axg_handle* h, double tau, double out[4]);
  * every step).  *variant = fused-pass code | (y-pass code << 8); ms[i] (i < nms, may be NULL)
  * = average ms per launch of: fused pass built-in, wide blocks (twice the rows), other sites
  * per thread, both, and the same four with XCD strips (each XCD walking its own strip of rows
- * plane by plane); y pass 128-byte tiles, 64-byte tiles (all 0 when the lattice has a single
- * configuration: grav_spec off or N not a power of two >= 64). */
+ * plane by plane); y pass 128-byte tiles, 64-byte tiles, 128-byte tiles with two z planes per
+ * block (all 0 when the lattice has a single configuration: grav_spec off, or N below 64 or
+ * without the compile-time kernels). */
 int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms);
 
 /* Kernel timing (no counterpart in the reference; for measurement).  enable = 1 resets the

@@ -130,7 +130,7 @@ def test_tune_keeps_fields_and_results(prec, N):
         v, yv, ms = L.tune(3)
         b_phi, b_pi = L.get_fields()
         assert np.array_equal(a_phi, b_phi) and np.array_equal(a_pi, b_pi)
-        assert v in (0, 1, 4, 5, 8, 9, 12, 13) and yv in (0, 1) and all(t > 0 for t in ms.values())
+        assert v in (0, 1, 4, 5, 8, 9, 12, 13) and yv in (0, 1, 2) and all(t > 0 for t in ms.values())
         L.advance_steps(1.0, 0.05, 3)
         t_phi, t_pi = L.get_fields()
         t_A = L.gw_state()
@@ -201,6 +201,24 @@ def test_gw_radix3_short(N):
     O.symplectic(phi, pi, OA, 1.0, 0.05, 1, P)
     for m in range(4):
         assert relerr(A[m], OA[m][:, :, : N // 2 + 1]) < 1e-10
+
+
+@pytest.mark.parametrize("N,prec", [(64, capi.FP32C), (96, capi.FP64)])
+def test_ypass_variants_bit_identical(N, prec, monkeypatch):
+    """The y pass's three configurations (128-byte tiles, 64-byte tiles, two z planes per block)
+    give the same accumulators bit for bit."""
+    say("y variants", N, prec)
+    prm, _ = params(N, precision=prec)
+    phi, pi = ic(N, 12, 4.0)
+    out = []
+    for v in ("0", "1", "2"):
+        monkeypatch.setenv("AXG_YPASS_VARIANT", v)
+        with capi.Lattice(prm) as L:
+            L.set_fields(phi, pi)
+            L.advance_steps(1.0, 0.05, 2)
+            out.append(L.gw_state())
+    for A in out[1:]:
+        assert np.array_equal(A, out[0])
 
 
 @pytest.mark.parametrize("N", [32, 48])
