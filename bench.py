@@ -61,8 +61,10 @@ def _fused_name(prec, v):
     return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {2 if v & 8 else 1 if v & 2 else 0}>"
 
 
-YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16>", ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8>",
-                ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8>", ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4>"}
+YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16, 1>", ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8, 1>",
+                ("fp32c", 2): "gw_ypass_p2_kernel<float, 512, 16, 2>",
+                ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8, 1>", ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4, 1>",
+                ("fp64", 2): "gw_ypass_p2_kernel<double, 512, 8, 2>"}
 ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512>", "fp64": "gw_zpass_p2_kernel<double, 512>"}
 
 
