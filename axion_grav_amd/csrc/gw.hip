@@ -32,16 +32,17 @@ template <typename T> int col_width(int N)
 // carries 8 elements of one transform in registers, the first load and last store go
 // directly between registers and HBM, and LDS is touched only between stages.
 
-// The accumulation pass keeps 8 k-points' projected sums per thread.  With at most 512 threads
-// per block the compiler has 256 VGPRs for the pipelined FP32 form.  FP32 from N = 512 instead
-// takes tiles of up to 1024 threads (16 columns at 512, 8 at 1024; 128 VGPRs, no pipelining),
-// so the y pass writes 128-byte (512) or 64-byte (1024) tile rows instead of half of that.
-// Measured: y pass 1.80 -> 1.55 ms at 512^3 and 35 -> 15 ms at 1024^3, z pass within 3 %.
+// The accumulation pass keeps 8 k-points' projected sums per thread.  With at most 768 threads
+// per block the compiler has 170 VGPRs for the pipelined FP32 form (it needs about 155).  FP32
+// from N = 512 instead takes tiles of up to 1024 threads (16 columns at 512, 8 at 1024; 128
+// VGPRs, no pipelining), so the y pass writes 128-byte (512) or 64-byte (1024) tile rows
+// instead of half of that.  Measured: y pass 1.80 -> 1.55 ms at 512^3 and 35 -> 15 ms at
+// 1024^3, z pass within 3 %.
 template <typename T, int N> constexpr bool p2_zwide() { return sizeof(T) == 4 && N >= 512; }
 template <typename T, int N> constexpr int p2_ztx()
 {
     int tx = p2_tx<T, N>();
-    while (tx > 1 && tx * (N / 8) > (p2_zwide<T, N>() ? 1024 : 512)) tx /= 2;
+    while (tx > 1 && tx * (N / 8) > (p2_zwide<T, N>() ? 1024 : 768)) tx /= 2;
     return tx;
 }
 

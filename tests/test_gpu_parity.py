@@ -546,14 +546,15 @@ def test_fp32c_tracks_fp64_gw(N):
     assert np.array_equal(c_s["denom"], ref_s["denom"])
 
 
-def test_fp32_wide_tiles_track_fp64_gw_at_512():
-    """From N = 512 the FP32 transforms use 16-column (512) or 8-column (1024) z tiles with
-    1024-thread accumulation blocks: the FP32C and FP32 GW spectra after 6 steps from the same
-    device-made field stay within FP32 transform rounding of FP64's.  Bins holding more than
-    1e-6 of the peak agree to 1e-4 each; the rest (a few steps from a smooth field leave the
-    highest bins ~1e-20 of the peak) sit at the FP32 noise floor, below 1e-6 of the peak."""
-    say("fp32 wide tiles 512")
-    N = 512
+@pytest.mark.parametrize("N", [384, 512])
+def test_fp32_wide_tiles_track_fp64_gw(N):
+    """The FP32 z tiles of the larger lattices: 16 columns with 768-thread accumulation blocks at
+    N = 384, 16 columns with 1024-thread blocks (no register pipelining) from N = 512.  The FP32C
+    and FP32 GW spectra after 6 steps from the same device-made field stay within FP32 transform
+    rounding of FP64's.  Bins holding more than 1e-6 of the peak agree to 1e-4 each; the rest (a
+    few steps from a smooth field leave the highest bins ~1e-20 of the peak) sit at the FP32
+    noise floor, below 1e-6 of the peak."""
+    say("fp32 wide tiles", N)
     out = {}
     for prec in (capi.FP64, capi.FP32C, capi.FP32):
         prm, _ = params(N, precision=prec, num_bin_grav=64)
