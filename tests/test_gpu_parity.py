@@ -524,13 +524,13 @@ def test_thermal_state_rejects_tau_ini_at_or_above_one():
 
 # ---------------------------------------------------------------- FP32C vs FP64
 
-@pytest.mark.parametrize("N", [32, 96])
-def test_fp32c_tracks_fp64_gw(N):
-    say("fp32c vs fp64", N)
+@pytest.mark.parametrize("N,fix", [(32, False), (96, False), (32, True), (48, True)])
+def test_fp32c_tracks_fp64_gw(N, fix):
+    say("fp32c vs fp64", N, fix)
     phi, pi = ic(N, 6, 3.0)
     out = {}
     for prec in (capi.FP64, capi.FP32C, capi.FP32):
-        prm, P = params(N, precision=prec)
+        prm, P = params(N, precision=prec, gw_fix_swap=fix)
         with capi.Lattice(prm) as L:
             L.set_fields(phi, pi)
             L.advance_steps(1.0, 0.05, 20)
