@@ -266,7 +266,7 @@ def test_defects_bitexact(N, kmax):
     assert np.array_equal(ml, exp)
 
 
-@pytest.mark.parametrize("N", [16, 32])
+@pytest.mark.parametrize("N", [16, 32, 48])
 def test_field_spectrum_fp64(N):
     say("field_spectrum", N)
     nb = 8 if N == 16 else 12
@@ -292,7 +292,7 @@ def test_field_spectrum_fp64(N):
 
 
 @pytest.mark.parametrize("N,kmax,min_mc,max_mc,tol", [(16, 2.0, 10, 100, 0.1), (32, 2.0, 4, 12, 0.0),
-                                                       (16, 6.0, 10, 20, 0.1)])
+                                                       (16, 6.0, 10, 20, 0.1), (48, 2.0, 4, 6, 0.0)])
 def test_field_covariance_fp64(N, kmax, min_mc, max_mc, tol):
     """estimate_covmat on the GPU against the oracle's restatement on the same mask, power (the
     GPU's) and deconvolution matrix (the oracle's, which agrees with the GPU's to 1e-8): the
