@@ -384,7 +384,7 @@ class Lattice:
         self._check(lib().axg_get_adjoint(self._h, _ptr(a), _ptr(b), N ** 3), "axg_get_adjoint")
         return a, b
 
-    TUNE_CONFIGS = ("fused: built-in", "fused: wide blocks", "fused: other sites per thread",
+    TUNE_CONFIGS = ("fused: rows in dispatch order", "fused: wide blocks", "fused: other sites per thread",
                     "fused: other sites, wide blocks", "fused: XCD strips", "fused: XCD strips, wide blocks",
                     "fused: other sites, XCD strips", "fused: other sites, XCD strips, wide blocks",
                     "y: 128-byte tiles", "y: 64-byte tiles", "y: two planes per block")

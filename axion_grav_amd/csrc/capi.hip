@@ -120,7 +120,7 @@ struct axg_handle {
     void* tw64 = nullptr;    // FP64 twiddles for the diagnostic transforms
     int cur = 0;
     bool fuse_x = true;      // x transform fused into the stencil pass when p2_size(N)
-    int kick_variant = 0;    // configuration of the fused pass (axg_tune; 0 = built-in choice)
+    int kick_variant = 0;    // configuration of the fused pass (axg_tune codes; set per size at creation)
     int ypass_variant = 0;   // configuration of the y pass (axg_tune; 0 = 128-byte column tiles)
     // diagnostics
     uint16_t* gw_bin = nullptr;
@@ -408,6 +408,11 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         h->plan = make_plan(N);
         // tuning knob: AXG_XFUSE=0 runs the x transform as its own pass (same results)
         if (const char* e = getenv("AXG_XFUSE")) h->fuse_x = strcmp(e, "0") != 0;
+        // the fused pass's configuration before any axg_tune, from the tuner's picks on MI355X:
+        // XCD strips at N = 128, 256, 384, 768 and 1024; at 512 wide blocks with strips for FP32
+        // storage and the dispatcher's order for FP64; code 0 elsewhere (within 10 % of the best)
+        if (N == 512) h->kick_variant = h->fp32 ? 9 : 0;
+        else if (N == 128 || N == 256 || N == 384 || N >= 768) h->kick_variant = 8;
         // tuning knobs AXG_KICK_VARIANT / AXG_YPASS_VARIANT: initial configurations (axg_tune's codes)
         if (const char* e = getenv("AXG_KICK_VARIANT")) h->kick_variant = atoi(e);
         if (const char* e = getenv("AXG_YPASS_VARIANT")) h->ypass_variant = atoi(e);

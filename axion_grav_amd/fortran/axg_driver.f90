@@ -38,6 +38,7 @@ program axion_3d
   type(axg_params) :: prm
   type(c_ptr) :: h
   integer(c_int32_t) :: nsteps
+  integer(c_int32_t) :: tune_code   ! the launch configuration axg_tune picks
   integer(c_int32_t) :: seed(2)   ! the two integers of key ss (subroutines.f90:49-95)
   integer(8) :: clk
   integer(8) :: clock0, clock_rate
@@ -243,6 +244,9 @@ program axion_3d
      flag_defects = .false.
      call write_chk(tau_end, idx, flag_defects)
   end if
+  ! the step kernels' launch configuration for this lattice, timed on the live fields (every
+  ! configuration gives the same results)
+  if (grav_spec) call axg_check(axg_tune(h, 3_c_int32_t, tune_code, c_null_ptr, 0_c_int32_t), h, 'axg_tune')
   do while (idx < n_chk)
      tau_start = tau_end
      idx = idx + 1

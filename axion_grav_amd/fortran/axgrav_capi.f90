@@ -144,6 +144,16 @@ module axgrav_capi
        integer(c_int32_t), intent(out) :: nsteps
      end function axg_advance
 
+     ! the step kernels' launch configuration, timed on the live lattice (results unchanged)
+     integer(c_int) function axg_tune(h, reps, variant, ms, nms) bind(C, name="axg_tune")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: reps
+       integer(c_int32_t), intent(out) :: variant
+       type(c_ptr), value :: ms
+       integer(c_int32_t), value :: nms
+     end function axg_tune
+
      integer(c_int) function axg_advance_steps(h, t1, dt, nsteps) bind(C, name="axg_advance_steps")
        import :: c_int, c_ptr, c_double, c_int32_t
        type(c_ptr), value :: h

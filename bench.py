@@ -49,7 +49,7 @@ MODEL_REALS = 40
 # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (calibrated against a copy of known
 # size) for the 512^3 workload; the profile summary holds the commands and the calibration.
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "lattice_step_512_hbm_traffic.md")
-# the fused pass's kernel for each tuner choice (0 built-in, 1 wide blocks, 4 / 5 the other
+# the fused pass's kernel for each tuner choice (0 rows in dispatch order, 1 wide blocks, 4 / 5 the other
 # sites-per-thread choice), FP32C and FP64 at 512^3
 def _fused_name(prec, v):
     """Kernel name of the fused pass for tuner code v at 512^3 (see launch_kick_source_xfft)."""
@@ -342,7 +342,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-diag", action="store_true", help="skip timing the diagnostics after the FP32C run")
-    ap.add_argument("--no-tune", action="store_true", help="keep the built-in configuration of the fused pass")
+    ap.add_argument("--no-tune", action="store_true", help="keep the fused pass's configuration chosen at creation (by lattice size)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip timing the gradient (128^3, 800 steps)")
     ap.add_argument("--no-batch", action="store_true", help="skip timing the 48-realization batch")
     ap.add_argument("--largest", action="store_true",
@@ -394,8 +394,9 @@ def main():
         log(f"largest: {big}")
 
     dom_name, dom = max(c["kernels"].items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
-    tc = c["tuned"] or {"choice": 0, "ypass": 0}
-    traffic = measured_traffic(dom_name, "fp32c", N, tc["choice"] if dom_name == "kick_source_drift" else tc["ypass"])
+    tc = c["tuned"]  # without tuning the configuration is the one chosen at creation: no counted traffic
+    traffic = None if tc is None else measured_traffic(
+        dom_name, "fp32c", N, tc["choice"] if dom_name == "kick_source_drift" else tc["ypass"])
     roof = dict(bound="hbm", kernel=dom_name, achieved=round(dom["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(dom["gbs"] / HBM_PEAK_GBS, 4), traffic=traffic,
                 alg_bytes_per_launch=dom["alg_bytes"], ms_per_launch=round(dom["ms_avg"], 4),

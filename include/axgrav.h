@@ -279,9 +279,9 @@ int axg_energy(axg_handle* h, double tau, double out[4]);
  * the handle's later steps.  Every configuration gives bit-identical results; the fields are
  * left as they were (pi is saved and restored; the other buffers written are rewritten by
  * every step).  *variant = fused-pass code | (y-pass code << 8); ms[i] (i < nms, may be NULL)
- * = average ms per launch of: fused pass built-in, wide blocks (twice the rows), other sites
- * per thread, both, and the same four with XCD strips (each XCD walking its own strip of rows
- * plane by plane); y pass 128-byte tiles, 64-byte tiles, 128-byte tiles with two z planes per
+ * = average ms per launch of: fused pass rows in dispatch order, wide blocks (twice the
+ * rows), other sites per thread, both, and the same four with XCD strips (each XCD walking its
+ * own strip of rows plane by plane); y pass 128-byte tiles, 64-byte tiles, 128-byte tiles with two z planes per
  * block (all 0 when the lattice has a single configuration: grav_spec off, or N below 64 or
  * without the compile-time kernels). */
 int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t nms);

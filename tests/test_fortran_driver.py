@@ -223,6 +223,28 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
 
 @needs_exe
 @pytest.mark.gpu
+def test_tuned_run_matches_oracle(tmp_path):
+    """At N = 64 the driver tunes the step kernels (axg_tune) before its first interval; the GW
+    spectrum it writes still matches the oracle's (the tuner's trial launches leave no trace)."""
+    print("[case] fortran driver tuned N=64", flush=True)
+    N, nbg = 64, 25
+    d = O.derive(SIGMA, 90.0, 1e-30, 1.5, 8.0, N)
+    phi0, _ = O.ic_flat(N, 78, 1.0, d["dwav"])
+    rng = np.random.default_rng(6)
+    pi0 = 0.5 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    np.concatenate([phi0.ravel(), pi0.ravel()]).tofile(tmp_path / "ic.raw")
+    run(tmp_path, ini_text(str(tmp_path), ic_file=str(tmp_path / "ic.raw"), ngrid=N, n_chk=1, tau_fin=1.5,
+                           boxsize=8.0, spec_est="F", id_strings="F", id_walls="F", n_id=1), "-b", "100")
+    P = dict(d, kappa=1e-30, initial_loop=True)
+    _, _, A = oracle_run(N, P, phi0, pi0, [1.0, 1.5], 1.0)[1]
+    g = read_rows(tmp_path / "run_gravspec_1.dat")
+    o = O.gw_spectrum(A, nbg, d["dspa"], d["dwav"], d["tau_cr"], d["a_now"], d["dv_wav"])
+    assert np.array_equal(g[:, 4], o["denom"])
+    assert np.allclose(g[:, 3], o["power"], rtol=1e-8, atol=0)
+
+
+@needs_exe
+@pytest.mark.gpu
 def test_grouped_networks_and_defect_files(tmp_path):
     """group_strings = group_walls = T with out_strings = out_walls = T: the string and wall
     parameter columns hold the grouped sums (all, long strings / large walls, longer still) of
