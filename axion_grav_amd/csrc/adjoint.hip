@@ -16,6 +16,7 @@
 // Hermitian H_c(k) = (Dbar_c(k)[k in half] + conj Dbar_c(-k)[-k in half]) / 2, so the pair
 // H_a + i H_b transforms to Sbar_a + i Sbar_b in one complex FFT.
 #include "axg_internal.h"
+#include "fft_p2.h"
 
 namespace axg {
 
@@ -250,6 +251,69 @@ __global__ void adj_gbar_kernel(int N, const typename V2<T>::type* __restrict__ 
     }
 }
 
+// The same gbar with the last (x) transform of Sbar done in the block: block = XR rows y at one
+// z, one site per thread.  3*XR*N/8 of the threads transform the rows of W0..W2 (already
+// transformed along y and z) in place in LDS, then every thread forms gbar at its site from
+// the three LDS rows and writes it over W.  Saves the separate row pass (a read and a write of
+// the three arrays).
+template <int N> constexpr int adj_rows() { return N >= 512 ? 1 : p2_rows_div(N, 512 / N); }
+
+template <typename T, bool COMP, typename TW, int N>
+__global__ void __launch_bounds__(adj_rows<N>() * N) adj_gbar_xfft_kernel(
+    const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi, double inv_d,
+    typename V2<TW>::type* __restrict__ W0, typename V2<TW>::type* __restrict__ W1,
+    typename V2<TW>::type* __restrict__ W2, const typename V2<TW>::type* __restrict__ tw)
+{
+#pragma clang fp contract(off)
+    using W2T = typename V2<TW>::type;
+    constexpr int XR = adj_rows<N>(), TPT = N / 8, NR = 3 * XR;
+    extern __shared__ unsigned char smem_raw[];
+    W2T* s = reinterpret_cast<W2T*>(smem_raw);  // [m][rr][x]
+    const int64_t NN = (int64_t)N * N;
+    const int y0 = blockIdx.x * XR, z = blockIdx.y;
+    {
+        const int q = threadIdx.x / TPT, j0 = threadIdx.x % TPT;
+        const bool active = q < NR;
+        const int m = active ? q / XR : 0, rr = active ? q - m * XR : 0;
+        const W2T* src = (m == 0 ? W0 : m == 1 ? W1 : W2) + (int64_t)z * NN + (int64_t)(y0 + rr) * N;
+        W2T v[8];
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = src[j0 + e * TPT];
+        }
+        W2T* rowp = s + (active ? q : 0) * N;
+        fft_p2_if<TW, N, -1>(active, v, j0, rowp, 1, 0, P2TwTable<TW, N, -1>{tw, j0});
+        if (active) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) rowp[j0 + e * TPT] = v[e];
+        }
+        __syncthreads();
+    }
+    const int rr = threadIdx.x / N, x = threadIdx.x - rr * N;
+    const int y = y0 + rr;
+    const int64_t i = (int64_t)z * NN + (int64_t)y * N + x;
+    const int64_t base = i - x;
+    const int xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+    const int yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+    const int zp = z + 1 == N ? 0 : z + 1, zm = z == 0 ? N - 1 : z - 1;
+    const double2 a1 = ldphi<T, COMP>(phi, cphi, base + xm), a2 = ldphi<T, COMP>(phi, cphi, base + xp);
+    const double2 b1 = ldphi<T, COMP>(phi, cphi, (int64_t)z * NN + (int64_t)ym * N + x);
+    const double2 b2 = ldphi<T, COMP>(phi, cphi, (int64_t)z * NN + (int64_t)yp * N + x);
+    const double2 c1 = ldphi<T, COMP>(phi, cphi, (int64_t)zm * NN + (int64_t)y * N + x);
+    const double2 c2 = ldphi<T, COMP>(phi, cphi, (int64_t)zp * NN + (int64_t)y * N + x);
+    const double2 g1 = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+    const double2 g2 = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+    const double2 g3 = {(c2.x - c1.x) * inv_d, (c2.y - c1.y) * inv_d};
+    const W2T w0 = s[rr * N + x], w1 = s[(XR + rr) * N + x], w2 = s[(2 * XR + rr) * N + x];
+    const double s11 = w0.x, s21 = w0.y, s22 = w1.x, s31 = w1.y, s32 = w2.x, s33 = w2.y;
+    W0[i] = W2T{(TW)(4 * s11 * g1.x + 2 * s21 * g2.x + 2 * s31 * g3.x),
+                (TW)(4 * s11 * g1.y + 2 * s21 * g2.y + 2 * s31 * g3.y)};
+    W1[i] = W2T{(TW)(2 * s21 * g1.x + 4 * s22 * g2.x + 2 * s32 * g3.x),
+                (TW)(2 * s21 * g1.y + 4 * s22 * g2.y + 2 * s32 * g3.y)};
+    W2[i] = W2T{(TW)(2 * s31 * g1.x + 2 * s32 * g2.x + 4 * s33 * g3.x),
+                (TW)(2 * s31 * g1.y + 2 * s32 * g2.y + 4 * s33 * g3.y)};
+}
+
 // Adjoint of one kick (and of the source's central differences):
 //   phibar(y) += inv_d sum_i (gbar_i(y - e_i) - gbar_i(y + e_i)) + f2 Lap(pibar)(y) + f3 H(phi(y)) pibar(y),
 // H the symmetric Jacobian of dVdphi as a map of (Re, Im).  Block partial sums of the parameter
@@ -378,6 +442,22 @@ void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, do
 }
 
 template <typename T, bool COMP, typename TW>
+bool launch_adj_gbar_xfft(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, void* W0, void* W1,
+                          void* W2, const void* tw)
+{
+    using T2 = typename V2<T>::type;
+    using W2T = typename V2<TW>::type;
+    const bool done = p2_dispatch(N, [&]<int NN>() {
+        constexpr int XR = adj_rows<NN>();
+        hipLaunchKernelGGL((adj_gbar_xfft_kernel<T, COMP, TW, NN>), dim3(NN / XR, NN), dim3(XR * NN),
+                           (size_t)3 * XR * NN * sizeof(W2T), s, (const T2*)phi, (const short2*)cphi, inv_d,
+                           (W2T*)W0, (W2T*)W1, (W2T*)W2, (const W2T*)tw);
+    });
+    if (done) AXG_HIP(hipGetLastError());
+    return done;
+}
+
+template <typename T, bool COMP, typename TW>
 void launch_adj_kick(hipStream_t s, int N, const void* phi, const void* cphi, const double2* pibar, double2* phibar,
                      double2* pibar_out, const void* G0, const void* G1, const void* G2, const KickScalars& k,
                      double inv_d, double hd, double s0, double s1, bool use_g, double* partial)
@@ -403,6 +483,8 @@ void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi
 
 #define AXG_INST_ADJ(T, C, TW)                                                                                       \
     template void launch_adj_gbar<T, C, TW>(hipStream_t, int, const void*, const void*, double, void*, void*, void*); \
+    template bool launch_adj_gbar_xfft<T, C, TW>(hipStream_t, int, const void*, const void*, double, void*, void*,   \
+                                                 void*, const void*);                                             \
     template void launch_adj_kick<T, C, TW>(hipStream_t, int, const void*, const void*, const double2*, double2*,     \
                                             double2*, const void*, const void*, const void*, const KickScalars&,      \
                                             double, double, double, double, bool, double*);                           \

@@ -138,6 +138,12 @@ void launch_adj_modes(hipStream_t s, int N, const double2* abar, const AdjStepSc
 template <typename T, bool COMP, typename TW>
 void launch_adj_gbar(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, void* W0, void* W1,
                      void* W2);
+// The same with the x transform of W0..W2 (SIGN -1, twiddle table tw of N entries in TW) done
+// first in the kernel, for the sizes of the compile-time FFT engine; false (nothing launched)
+// for the others.
+template <typename T, bool COMP, typename TW>
+bool launch_adj_gbar_xfft(hipStream_t s, int N, const void* phi, const void* cphi, double inv_d, void* W0, void* W1,
+                          void* W2, const void* tw);
 // phibar += adjoint of the kick and of the source's differences; pibar_out = f1 pibar + hd phibar;
 // partial[block][2] += (s0, s1) times the block's sums of <pibar, V'_0(phi)> and <pibar, (-Re, Im)>.
 template <typename T, bool COMP, typename TW>

@@ -1692,6 +1692,12 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
         launch_fft_cols<T, -1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw);
         launch_fft_cols<T, -1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw);
     };
+    // y and z only: the compile-time sizes do the x transform inside the gbar kernel
+    auto fft_back_yz = [&](void* d) {
+        launch_fft_cols<T, -1>(h->stream, N, d, N, (int64_t)N * N, N, N, h->plan, h->tw);
+        launch_fft_cols<T, -1>(h->stream, N, d, (int64_t)N * N, N, N, N, h->plan, h->tw);
+    };
+    const bool fused_x = p2_size(N);
     auto slot = [&](int64_t j) { return (char*)h->ckpt + state * (size_t)j; };
     auto save = [&](int64_t j) {
         char* b = slot(j);
@@ -1777,10 +1783,19 @@ static void gradient_impl(axg_handle* h, double t1, double dt, int32_t nsteps, i
         g.fix_swap = p.gw_fix_swap;
         g.last = s == nsteps - 1;
         launch_adj_modes<T>(h->stream, N, h->abar, g, W0, W1, W2);
-        fft_back(W0);
-        fft_back(W1);
-        fft_back(W2);
-        launch_adj_gbar<T, COMP, T>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2);
+        if (fused_x) {
+            fft_back_yz(W0);
+            fft_back_yz(W1);
+            fft_back_yz(W2);
+            if (!launch_adj_gbar_xfft<T, COMP, T>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2,
+                                                  h->tw))
+                throw Error(AXG_EDEVICE, "gradient: no compile-time transform for this size");
+        } else {
+            fft_back(W0);
+            fft_back(W1);
+            fft_back(W2);
+            launch_adj_gbar<T, COMP, T>(h->stream, N, h->phi[h->cur], h->cphi[h->cur], inv_d, W0, W1, W2);
+        }
         const KickScalars k = kick_scalars(p, t, dt);
         const double hd = s == 0 ? dt / 2 : dt;  // the drift before kick s
         const double t2 = t * t;

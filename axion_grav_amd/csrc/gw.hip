@@ -318,28 +318,6 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
     }
 }
 
-// Runs F.template operator()<N>() for N = 2^L in 16..1024 or 3 * 2^L in 24..768 (the sizes of
-// fft_p2.h); returns false otherwise.
-template <typename F> bool p2_dispatch(int N, F&& f)
-{
-    switch (N) {
-    case 24: f.template operator()<24>(); return true;
-    case 48: f.template operator()<48>(); return true;
-    case 96: f.template operator()<96>(); return true;
-    case 192: f.template operator()<192>(); return true;
-    case 384: f.template operator()<384>(); return true;
-    case 768: f.template operator()<768>(); return true;
-    case 16: f.template operator()<16>(); return true;
-    case 32: f.template operator()<32>(); return true;
-    case 64: f.template operator()<64>(); return true;
-    case 128: f.template operator()<128>(); return true;
-    case 256: f.template operator()<256>(); return true;
-    case 512: f.template operator()<512>(); return true;
-    case 1024: f.template operator()<1024>(); return true;
-    default: return false;
-    }
-}
-
 // ------------------------------------------------------------------ x pass (R2C via pairing)
 // Block = TB rows of one packed component m at one z.  Row FFT of C_m = S_a + i S_b, then
 //   S_a^(k) = (X(k) + conj(X(N-k)))/2,  S_b^(k) = (X(k) - conj(X(N-k)))/(2i),  k = 0..N/2,
