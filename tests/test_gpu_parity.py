@@ -574,6 +574,40 @@ def test_fp32_wide_tiles_track_fp64_gw(N):
         assert np.array_equal(out[prec]["denom"], ref["denom"])
 
 
+@pytest.mark.parametrize("N", [768, 1024])
+def test_largest_lattices_fp32c_tracks_fp64(N):
+    """The largest compile-time sizes: 768^3 (3 * 2^8: radix-3 finish, sub-strips, 8-column FP32
+    and 4-column FP64 tiles) and 1024^3 (the largest single-GPU lattice; its FP64 handle holds
+    about 213 GB), checked through properties that do not depend on the size.  From the same
+    device-made field, the FP32C GW spectrum after 4 steps stays within FP32 transform rounding of
+    FP64's (bins above 1e-6 of the peak to 1e-4 each, the rest below 1e-6 of the peak, bin counts
+    identical).  In each precision, 4 steps forward and 4 steps back return the energy terms to
+    round-off (the integrator is time-reversible; tolerances as the 32^3 field-reversal test)."""
+    out = {}
+    for prec, tol in ((capi.FP64, 1e-11), (capi.FP32C, 1e-8)):
+        say("largest lattices", N, prec)
+        prm, _ = params(N, precision=prec, num_bin_grav=64)
+        with capi.Lattice(prm) as L:
+            L.init_flat(5, 4.0)
+            e0 = L.energy(1.0)
+            L.advance_steps(1.0, 0.05, 4)
+            out[prec] = L.gw_spectrum()
+            e_mid = L.energy(1.2)
+            L.advance_steps(1.2, -0.05, 4)
+            e1 = L.energy(1.0)
+        scale = np.abs(e0).max()
+        assert np.abs(e_mid - e0).max() > 1e-6 * scale  # the fields did move
+        assert np.abs(e1 - e0).max() <= tol * scale, (prec, e0, e1)
+    ref, got = out[capi.FP64], out[capi.FP32C]
+    top = ref["numer"].max()
+    big = ref["numer"] > 1e-6 * top
+    assert big.sum() > 30
+    err = np.abs(got["numer"][big] - ref["numer"][big]) / ref["numer"][big]
+    assert err.max() < 1e-4, err.max()
+    assert np.abs(got["numer"] - ref["numer"]).max() < 1e-6 * top
+    assert np.array_equal(got["denom"], ref["denom"])
+
+
 def test_strip_orders_bit_identical_at_768(monkeypatch):
     """At N = 768 the XCD strips split into sub-strips (strip_order): the fused pass in the
     dispatcher's order (0), in strips (8) and in strips with wide blocks (9) gives the same
