@@ -214,12 +214,16 @@ void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double ta
 // String cells (flag bit 1) in site order `cells`: fit, chord length and velocity per cell.
 // rec: 11 doubles per string (pos(3) in site coordinates counted from 0, dir(3), gof, sl, vel(3));
 // partial: 5 doubles per block of STRING_NT strings (sum sl, n with |v| < 1, sum |v|, sum v^2,
-// sum gamma), to be summed with sum_columns.
+// sum gamma), to be summed with sum_columns.  Searches still running after `split` simplex
+// iterations move to a second kernel (the records do not depend on split); its work space:
+// npend (one int), pend (ns int64) and state (STRING_STATE doubles per string).
 constexpr int STRING_NT = 64;
+constexpr int STRING_STATE = 26;
+constexpr int STRING_SPLIT = 64;  // default split (fastest of 2..1000 at 512^3 on MI355X)
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, double* rec,
-                       double* partial);
+                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, int split,
+                       double* rec, double* partial, int* npend, int64_t* pend, double* state);
 
 void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col);
 

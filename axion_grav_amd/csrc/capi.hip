@@ -139,7 +139,10 @@ struct axg_handle {
     int32_t* str_cells = nullptr;
     double* str_rec = nullptr;
     double* str_part = nullptr;
-    int64_t str_cap = 0;      // strings the three buffers hold
+    int64_t* str_pend = nullptr;   // string fits' second-stage list (launch_string_fit)
+    double* str_state = nullptr;   // its counter (first 8 bytes) and parked searches
+    int64_t str_cap = 0;      // strings the five buffers hold
+    int string_split = STRING_SPLIT;  // simplex iterations before a search moves to the second kernel
     int64_t nstr_rec = -1;    // strings in str_rec (-1: no fit since the fields changed)
     int64_t nwall = 0;        // wall cells of the last classification
     int32_t* wall_cells = nullptr;
@@ -416,6 +419,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         // tuning knobs AXG_KICK_VARIANT / AXG_YPASS_VARIANT: initial configurations (axg_tune's codes)
         if (const char* e = getenv("AXG_KICK_VARIANT")) h->kick_variant = atoi(e);
         if (const char* e = getenv("AXG_YPASS_VARIANT")) h->ypass_variant = atoi(e);
+        // AXG_STRING_SPLIT: simplex iterations of the string fits' first kernel (same records)
+        if (const char* e = getenv("AXG_STRING_SPLIT")) h->string_split = std::max(0, atoi(e));
         AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int64_t n3 = h->n3;
         for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
@@ -985,14 +990,20 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
             void* a = h->str_cells;
             void* b = h->str_rec;
             void* c = h->str_part;
+            void* d = h->str_pend;
+            void* e = h->str_state;
             AXG_HIP(hipStreamSynchronize(h->stream));
             h->release_one(a);
             h->release_one(b);
             h->release_one(c);
+            h->release_one(d);
+            h->release_one(e);
             h->str_cap = 0;
             h->str_cells = (int32_t*)h->alloc((size_t)ns * sizeof(int32_t));
             h->str_rec = (double*)h->alloc((size_t)11 * ns * sizeof(double));
             h->str_part = (double*)h->alloc((size_t)(5 * nsb + 8) * sizeof(double));
+            h->str_pend = (int64_t*)h->alloc((size_t)ns * sizeof(int64_t));
+            h->str_state = (double*)h->alloc((size_t)(STRING_STATE * ns + 1) * sizeof(double));
             h->str_cap = ns;
         }
         double sums[5] = {0, 0, 0, 0, 0};
@@ -1003,15 +1014,20 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
             launch_scan_counts(h->stream, counts, soffs, nblk, stot, 0);
             launch_cell_compact(h->stream, h->N, h->flags, 1, soffs, h->str_cells, 0, nblk);
             double* dsum = h->str_part + 5 * nsb;
+            int* npend = (int*)h->str_state;
+            double* state = h->str_state + 1;
             if (h->comp)
                 launch_string_fit<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi,
-                                               h->str_cells, ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+                                               h->str_cells, ns, seed0, seed1, h->p.dspa, h->string_split, h->str_rec,
+                                               h->str_part, npend, h->str_pend, state);
             else if (h->fp32)
                 launch_string_fit<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, h->str_cells,
-                                                ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+                                                ns, seed0, seed1, h->p.dspa, h->string_split, h->str_rec, h->str_part,
+                                                npend, h->str_pend, state);
             else
                 launch_string_fit<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr,
-                                                 h->str_cells, ns, seed0, seed1, h->p.dspa, h->str_rec, h->str_part);
+                                                 h->str_cells, ns, seed0, seed1, h->p.dspa, h->string_split,
+                                                 h->str_rec, h->str_part, npend, h->str_pend, state);
             sum_columns(h->stream, h->str_part, (int)nsb, 5, dsum);
             AXG_HIP(hipMemcpyAsync(sums, dsum, sizeof sums, hipMemcpyDeviceToHost, h->stream));
             AXG_HIP(hipStreamSynchronize(h->stream));

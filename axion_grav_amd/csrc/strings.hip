@@ -8,6 +8,15 @@
 // corner values of phi and pi per string and writes 11 doubles.  Every array of the search
 // lives in registers: the insertion of a start point and the bubble sort are unrolled over the
 // five simplex slots, so no index into them depends on data.
+//
+// Two stages, because the simplex phase diverges: most fits converge within a few iterations,
+// but about one string in seven never reaches chi2 <= 0.05 and runs all 1000, so nearly every
+// wave of 64 strings would wait for 1000 iterations while most of its lanes idle.  Stage one
+// runs the random starts and the first `split` simplex iterations of every string; a string still
+// searching then parks its simplex (5 points, 5 chi2, iteration count) in a list.  Stage two
+// continues the parked strings, one per thread, so the long searches share waves.  A string's
+// search is the same sequence of operations whichever stage ends it, so the records do not
+// depend on `split`.  The per-block sums are taken afterwards from the records.
 #include "axg_internal.h"
 
 namespace axg {
@@ -153,16 +162,20 @@ __device__ __forceinline__ void copy4(double* d, const double* s)
     for (int k = 0; k < 4; k++) d[k] = s[k];
 }
 
-// position_string: 1001 random starts keep the best five (the first two unsorted, as in the
-// reference), then the reference's simplex iteration until chi2 <= 0.05 or 1000 iterations.
-__device__ void position_string(const Phases& ph, uint32_t k0, uint32_t k1, int64_t s, double pos[3],
-                                double dir[3], double& gof, double& sl)
+// position_string (defects_3d.f90:435-603) in three parts.  The search state of one string:
+// the five simplex points p and their chi2 c2.
+constexpr double STR_TOL = 0.05;  // chi2 at which the simplex stops
+
+// 1001 random starts keep the best five (the first two unsorted, as in the reference).
+// The search arrays are locals here and copied out at the end: with the caller's arrays
+// (references) the compiler moved c2 to scratch memory inside the 1001-start loop.
+__device__ __forceinline__ void fit_starts(const Phases& ph, uint32_t k0, uint32_t k1, int64_t s,
+                                           double (&p_out)[NSIMP + 1][4], double (&c2_out)[NSIMP + 1])
 {
 #pragma clang fp contract(off)
-    const double chi2max = 8 * PI_T * PI_T, tol = 0.05;
     const double mx[4] = {1.0, PI_T, 0.86602540378443865, PI_T};
     const double mn[4] = {-1.0, -PI_T, 0.0, -PI_T};
-    double p[NSIMP + 1][4], c2[NSIMP + 1], t[4], c[4], sv[4];
+    double p[NSIMP + 1][4], c2[NSIMP + 1], t[4];
 #pragma unroll
     for (int j = 0; j <= NSIMP; j++) {
         c2[j] = 0;
@@ -204,8 +217,22 @@ __device__ void position_string(const Phases& ph, uint32_t k0, uint32_t k1, int6
             if (!placed) { c2[0] = chi2; copy4(p[0], t); }
         }
     }
-    int it = 0;
-    while (c2[0] > tol && it < MAXIT) {
+#pragma unroll
+    for (int j = 0; j <= NSIMP; j++) {
+        c2_out[j] = c2[j];
+        copy4(p_out[j], p[j]);
+    }
+}
+
+// The reference's simplex iteration, continued from iteration `it` until chi2 <= 0.05 or
+// iteration itmax (the reference's limit is MAXIT).
+__device__ __forceinline__ void fit_simplex(const Phases& ph, double (&p)[NSIMP + 1][4], double (&c2)[NSIMP + 1],
+                                            int& it, int itmax)
+{
+#pragma clang fp contract(off)
+    const double chi2max = 8 * PI_T * PI_T;
+    double t[4], c[4], sv[4];
+    while (c2[0] > STR_TOL && it < itmax) {
         it = it + 1;
 #pragma unroll
         for (int k = 0; k < 4; k++) c[k] = 0;
@@ -256,8 +283,13 @@ __device__ void position_string(const Phases& ph, uint32_t k0, uint32_t k1, int6
                     c2[j + 1] = q;
                 }
     }
-    (void)mx;
-    (void)mn;
+}
+
+// Position, direction, goodness of fit and chord length of the best point.
+__device__ __forceinline__ void fit_result(const double (&p)[NSIMP + 1][4], const double (&c2)[NSIMP + 1],
+                                           double pos[3], double dir[3], double& gof, double& sl)
+{
+#pragma clang fp contract(off)
     double mu = p[0][0], phi = p[0][1];
     double u0 = p[0][2] * cos(p[0][3]), v0 = p[0][2] * sin(p[0][3]);
     double sm = sqrt(1.0 - mu * mu);
@@ -271,116 +303,217 @@ __device__ void position_string(const Phases& ph, uint32_t k0, uint32_t k1, int6
     sl = string_length(pos, dir);
 }
 
+// Lattice site of string cell s; its corner sites (periodic wrap) are formed where they are
+// read, so no corner index stays live through the fit.
+struct StringCell {
+    int N, x, y, z;
+    __device__ __forceinline__ int64_t site(int c) const
+    {
+        int cx = x + (c & 1), cy = y + ((c >> 1) & 1), cz = z + (c >> 2);
+        cx = cx == N ? 0 : cx;
+        cy = cy == N ? 0 : cy;
+        cz = cz == N ? 0 : cz;
+        return ((int64_t)cz * N + cy) * N + cx;
+    }
+};
+
+__device__ __forceinline__ StringCell string_cell(int N, const int32_t* __restrict__ cells, int64_t s)
+{
+    const int64_t i = cells[s];
+    const int64_t NN = (int64_t)N * N;
+    return StringCell{N, (int)(i % N), (int)((i / N) % N), (int)(i / NN)};
+}
+
+template <typename T, bool COMP>
+__device__ __forceinline__ Phases corner_phases(const typename V2<T>::type* __restrict__ phi,
+                                                const short2* __restrict__ cphi, const StringCell& sc)
+{
+    Phases ph;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        double2 f = ldf<T, COMP>(phi, cphi, sc.site(c));
+        ph.v[c] = atan2(f.y, f.x);
+    }
+    return ph;
+}
+
+// velocity_string and the record of string s (11 doubles: pos in site coordinates counted
+// from 0, dir, gof, sl, vel).
+template <typename T, bool COMP>
+__device__ __forceinline__ void finish_string(const typename V2<T>::type* __restrict__ phi,
+                                              const short2* __restrict__ cphi,
+                                              const typename V2<T>::type* __restrict__ pi,
+                                              const short2* __restrict__ cpi, const StringCell& sc,
+                                              const double pos[3], const double dir[3], double gof, double sl,
+                                              double dspa, double* __restrict__ rec, int64_t s)
+{
+#pragma clang fp contract(off)
+    const int x = sc.x, y = sc.y, z = sc.z;
+    auto corner = [&](int c) { return sc.site(c); };
+    // the corner values are read again here rather than kept live through the fit
+    double2 f[8], q[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        f[c] = ldf<T, COMP>(phi, cphi, corner(c));
+        q[c] = ldf<T, COMP>(pi, cpi, corner(c));
+    }
+
+    // velocity_string: least-squares fit of phi and pi to a + b.d over the corners
+    double wr[4] = {0, 0, 0, 0}, wi[4] = {0, 0, 0, 0}, dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
+    double mat[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        double d[4] = {1.0, (c & 1) - pos[0] - 0.5, ((c >> 1) & 1) - pos[1] - 0.5, (c >> 2) - pos[2] - 0.5};
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            wr[m] = wr[m] + d[m] * f[c].x;
+            wi[m] = wi[m] + d[m] * f[c].y;
+            dr[m] = dr[m] + d[m] * q[c].x;
+            di[m] = di[m] + d[m] * q[c].y;
+        }
+#pragma unroll
+        for (int cc = 1; cc <= 4; cc++)
+#pragma unroll
+            for (int r = 1; r <= cc; r++) mat[r + cc * (cc - 1) / 2 - 1] += d[r - 1] * d[cc - 1];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        wr[m] /= 8;
+        wi[m] /= 8;
+        dr[m] /= 8;
+        di[m] /= 8;
+    }
+#pragma unroll
+    for (int m = 0; m < 10; m++) mat[m] /= 8;
+    pp_inverse4(mat);
+    double w0r[4] = {0, 0, 0, 0}, w0i[4] = {0, 0, 0, 0}, d0r = 0, d0i = 0;
+#pragma unroll
+    for (int ii = 1; ii <= 4; ii++)
+#pragma unroll
+        for (int jj = 1; jj <= 4; jj++) {
+            int kk = ii > jj ? ii : jj;
+            kk = kk * (kk - 1) / 2 + (ii < jj ? ii : jj) - 1;
+            w0r[ii - 1] += mat[kk] * wr[jj - 1];
+            w0i[ii - 1] += mat[kk] * wi[jj - 1];
+            if (ii == 1) {
+                d0r += mat[kk] * dr[jj - 1];
+                d0i += mat[kk] * di[jj - 1];
+            }
+        }
+    double a = 0, b = 0, cc = 0;
+#pragma unroll
+    for (int m = 1; m < 4; m++) {
+        a += w0r[m] * w0r[m];
+        b += w0r[m] * w0i[m];
+        cc += w0i[m] * w0i[m];
+    }
+    double det = a * cc - b * b;
+    double vel[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        double v = (d0r * cc - d0i * b) * w0r[m + 1] + (-d0r * b + d0i * a) * w0i[m + 1];
+        vel[m] = -v / det * dspa;
+    }
+
+    double* o = rec + 11 * s;
+    o[0] = pos[0] + ((double)(x + 1) - 0.5);
+    o[1] = pos[1] + ((double)(y + 1) - 0.5);
+    o[2] = pos[2] + ((double)(z + 1) - 0.5);
+    o[3] = dir[0];
+    o[4] = dir[1];
+    o[5] = dir[2];
+    o[6] = gof;
+    o[7] = sl;
+    o[8] = vel[0];
+    o[9] = vel[1];
+    o[10] = vel[2];
+}
+
+// Searches parked by stage one: slot j holds string pend[j] and, at state + 26 j, its simplex
+// points (20), their chi2 (5) and the iterations done.
+
+// Stage one: random starts and the first `split` simplex iterations of string s = one thread.
 template <typename T, bool COMP>
 __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const typename V2<T>::type* __restrict__ phi,
                                                                const short2* __restrict__ cphi,
                                                                const typename V2<T>::type* __restrict__ pi,
                                                                const short2* __restrict__ cpi,
                                                                const int32_t* __restrict__ cells, int64_t ns,
-                                                               uint32_t k0, uint32_t k1, double dspa,
-                                                               double* __restrict__ rec, double* __restrict__ partial)
+                                                               uint32_t k0, uint32_t k1, double dspa, int split,
+                                                               double* __restrict__ rec, int* __restrict__ npend,
+                                                               int64_t* __restrict__ pend, double* __restrict__ state)
+{
+#pragma clang fp contract(off)
+    const int64_t s = blockIdx.x * (int64_t)STRING_NT + threadIdx.x;
+    if (s >= ns) return;
+    const StringCell sc = string_cell(N, cells, s);
+    const Phases ph = corner_phases<T, COMP>(phi, cphi, sc);
+    double p[NSIMP + 1][4], c2[NSIMP + 1];
+    fit_starts(ph, k0, k1, s, p, c2);
+    int it = 0;
+    fit_simplex(ph, p, c2, it, split < MAXIT ? split : MAXIT);
+    if (c2[0] > STR_TOL && it < MAXIT) {
+        // still searching: park it for stage two
+        const int j = atomicAdd(npend, 1);
+        pend[j] = s;
+        double* st = state + (int64_t)j * STRING_STATE;
+#pragma unroll
+        for (int a = 0; a <= NSIMP; a++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) st[4 * a + k] = p[a][k];
+            st[20 + a] = c2[a];
+        }
+        st[25] = (double)it;
+        return;
+    }
+    double pos[3], dir[3], gof, sl;
+    fit_result(p, c2, pos, dir, gof, sl);
+    finish_string<T, COMP>(phi, cphi, pi, cpi, sc, pos, dir, gof, sl, dspa, rec, s);
+}
+
+// Stage two: the parked searches, one per thread, continued to the end.
+constexpr int STRING_NT2 = 64;
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(STRING_NT2) string_fit_rest_kernel(
+    int N, const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi,
+    const typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cpi, const int32_t* __restrict__ cells,
+    double dspa, double* __restrict__ rec, const int* __restrict__ npend, const int64_t* __restrict__ pend,
+    const double* __restrict__ state)
+{
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * STRING_NT2 + threadIdx.x;
+    if (j >= *npend) return;
+    const int64_t s = pend[j];
+    const double* st = state + (int64_t)j * STRING_STATE;
+    double p[NSIMP + 1][4], c2[NSIMP + 1];
+#pragma unroll
+    for (int a = 0; a <= NSIMP; a++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) p[a][k] = st[4 * a + k];
+        c2[a] = st[20 + a];
+    }
+    int it = (int)st[25];
+    const StringCell sc = string_cell(N, cells, s);
+    const Phases ph = corner_phases<T, COMP>(phi, cphi, sc);
+    fit_simplex(ph, p, c2, it, MAXIT);
+    double pos[3], dir[3], gof, sl;
+    fit_result(p, c2, pos, dir, gof, sl);
+    finish_string<T, COMP>(phi, cphi, pi, cpi, sc, pos, dir, gof, sl, dspa, rec, s);
+}
+
+// add_string_length's sums from the records, per block of STRING_NT strings in string order
+// (the order of the sums does not depend on which stage ended a search).
+__global__ void __launch_bounds__(STRING_NT) string_sum_kernel(const double* __restrict__ rec, int64_t ns,
+                                                               double* __restrict__ partial)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[5][STRING_NT];
     const int64_t s = blockIdx.x * (int64_t)STRING_NT + threadIdx.x;
     double acc[5] = {0, 0, 0, 0, 0};
     if (s < ns) {
-        const int64_t NN = (int64_t)N * N;
-        const int64_t i = cells[s];
-        const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
-        auto corner = [&](int c) {
-            int cx = x + (c & 1), cy = y + ((c >> 1) & 1), cz = z + (c >> 2);
-            cx = cx == N ? 0 : cx;
-            cy = cy == N ? 0 : cy;
-            cz = cz == N ? 0 : cz;
-            return (int64_t)cz * NN + (int64_t)cy * N + cx;
-        };
-        Phases ph;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            double2 f = ldf<T, COMP>(phi, cphi, corner(c));
-            ph.v[c] = atan2(f.y, f.x);
-        }
-        double pos[3], dir[3], gof, sl;
-        position_string(ph, k0, k1, s, pos, dir, gof, sl);
-        // the corner values are read again here rather than kept live through the fit
-        double2 f[8], q[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            f[c] = ldf<T, COMP>(phi, cphi, corner(c));
-            q[c] = ldf<T, COMP>(pi, cpi, corner(c));
-        }
-
-        // velocity_string: least-squares fit of phi and pi to a + b.d over the corners
-        double wr[4] = {0, 0, 0, 0}, wi[4] = {0, 0, 0, 0}, dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
-        double mat[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            double d[4] = {1.0, (c & 1) - pos[0] - 0.5, ((c >> 1) & 1) - pos[1] - 0.5, (c >> 2) - pos[2] - 0.5};
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                wr[m] = wr[m] + d[m] * f[c].x;
-                wi[m] = wi[m] + d[m] * f[c].y;
-                dr[m] = dr[m] + d[m] * q[c].x;
-                di[m] = di[m] + d[m] * q[c].y;
-            }
-#pragma unroll
-            for (int cc = 1; cc <= 4; cc++)
-#pragma unroll
-                for (int r = 1; r <= cc; r++) mat[r + cc * (cc - 1) / 2 - 1] += d[r - 1] * d[cc - 1];
-        }
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            wr[m] /= 8;
-            wi[m] /= 8;
-            dr[m] /= 8;
-            di[m] /= 8;
-        }
-#pragma unroll
-        for (int m = 0; m < 10; m++) mat[m] /= 8;
-        pp_inverse4(mat);
-        double w0r[4] = {0, 0, 0, 0}, w0i[4] = {0, 0, 0, 0}, d0r = 0, d0i = 0;
-#pragma unroll
-        for (int ii = 1; ii <= 4; ii++)
-#pragma unroll
-            for (int jj = 1; jj <= 4; jj++) {
-                int kk = ii > jj ? ii : jj;
-                kk = kk * (kk - 1) / 2 + (ii < jj ? ii : jj) - 1;
-                w0r[ii - 1] += mat[kk] * wr[jj - 1];
-                w0i[ii - 1] += mat[kk] * wi[jj - 1];
-                if (ii == 1) {
-                    d0r += mat[kk] * dr[jj - 1];
-                    d0i += mat[kk] * di[jj - 1];
-                }
-            }
-        double a = 0, b = 0, cc = 0;
-#pragma unroll
-        for (int m = 1; m < 4; m++) {
-            a += w0r[m] * w0r[m];
-            b += w0r[m] * w0i[m];
-            cc += w0i[m] * w0i[m];
-        }
-        double det = a * cc - b * b;
-        double vel[3];
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-            double v = (d0r * cc - d0i * b) * w0r[m + 1] + (-d0r * b + d0i * a) * w0i[m + 1];
-            vel[m] = -v / det * dspa;
-        }
-
-        double* o = rec + 11 * s;
-        o[0] = pos[0] + ((double)(x + 1) - 0.5);
-        o[1] = pos[1] + ((double)(y + 1) - 0.5);
-        o[2] = pos[2] + ((double)(z + 1) - 0.5);
-        o[3] = dir[0];
-        o[4] = dir[1];
-        o[5] = dir[2];
-        o[6] = gof;
-        o[7] = sl;
-        o[8] = vel[0];
-        o[9] = vel[1];
-        o[10] = vel[2];
-        acc[0] = sl;
+        const double* o = rec + 11 * s;
+        const double vel[3] = {o[8], o[9], o[10]};
+        acc[0] = o[7];
         double v2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
         if (v2 < 1.0) {
             acc[1] = 1;
@@ -403,23 +536,35 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
 
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, double* rec,
-                       double* partial)
+                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, int split,
+                       double* rec, double* partial, int* npend, int64_t* pend, double* state)
 {
     using T2 = typename V2<T>::type;
     if (ns <= 0) return;
+    if (ns > INT32_MAX) throw Error(AXG_EINVAL, "string fits: more than 2^31 - 1 string cells");
     const int64_t nblk = (ns + STRING_NT - 1) / STRING_NT;
+    AXG_HIP(hipMemsetAsync(npend, 0, sizeof(int), s));
     hipLaunchKernelGGL((string_fit_kernel<T, COMP>), dim3((unsigned)nblk), dim3(STRING_NT), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, key0, key1, dspa, rec,
-                       partial);
+                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, key0, key1, dspa, split, rec,
+                       npend, pend, state);
+    AXG_HIP(hipGetLastError());
+    // as many threads as strings; those past the parked count return at once
+    hipLaunchKernelGGL((string_fit_rest_kernel<T, COMP>), dim3((unsigned)((ns + STRING_NT2 - 1) / STRING_NT2)),
+                       dim3(STRING_NT2), 0, s, N, (const T2*)phi, (const short2*)cphi, (const T2*)pi,
+                       (const short2*)cpi, cells, dspa, rec, (const int*)npend, (const int64_t*)pend,
+                       (const double*)state);
+    AXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(string_sum_kernel, dim3((unsigned)nblk), dim3(STRING_NT), 0, s, (const double*)rec, ns, partial);
     AXG_HIP(hipGetLastError());
 }
 
-template void launch_string_fit<double, false>(hipStream_t, int, const void*, const void*, const void*, const void*,
-                                               const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
-template void launch_string_fit<float, false>(hipStream_t, int, const void*, const void*, const void*, const void*,
-                                              const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
-template void launch_string_fit<float, true>(hipStream_t, int, const void*, const void*, const void*, const void*,
-                                             const int32_t*, int64_t, uint32_t, uint32_t, double, double*, double*);
+#define AXG_STRING_FIT(T, C)                                                                                        \
+    template void launch_string_fit<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*,    \
+                                          const int32_t*, int64_t, uint32_t, uint32_t, double, int, double*,        \
+                                          double*, int*, int64_t*, double*);
+AXG_STRING_FIT(double, false)
+AXG_STRING_FIT(float, false)
+AXG_STRING_FIT(float, true)
+#undef AXG_STRING_FIT
 
 }  // namespace axg

@@ -405,6 +405,34 @@ def test_strings_match_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_string_fit_split_bit_identical(prec, monkeypatch):
+    """The string fits run in two kernels: a search still running after `split` simplex
+    iterations is parked and continued by the second.  The records and the sums are the same bit
+    for bit whatever the split (0: every unconverged search is parked after the random starts;
+    1000: none is).  Searches that end unconverged (chi2 > 0.05, so gof < exp(-0.025)) ran all
+    1000 iterations, so at the default split they went through the second kernel."""
+    say("string fit split", prec)
+    N = 32
+    prm, P = params(N, grav_spec=False, precision=prec, id_strings=True, id_walls=True, spec_est=True)
+    phi, _ = ic(N, 31, 6.0)
+    rng = np.random.default_rng(4)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    out = []
+    for split in ("0", "1", "32", "1000"):
+        monkeypatch.setenv("AXG_STRING_SPLIT", split)
+        with capi.Lattice(prm) as L:
+            L.set_fields(phi, pi)
+            L.defects()
+            o = L.strings(1234, 99)
+            out.append((o, L.string_records()))
+    o0, r0 = out[0]
+    assert len(r0) > 50 and np.count_nonzero(r0[:, 6] < np.exp(-0.025)) > 0
+    for o, r in out[1:]:
+        assert np.array_equal(r, r0)
+        assert o["length"] == o0["length"] and o["n_vel"] == o0["n_vel"] and np.array_equal(o["vel"], o0["vel"])
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
 def test_walls_match_oracle(prec):
     """Wall fits on the GPU against the oracle on the same field and cell list: no random
     numbers, the same grid search and polygon; records differ by last-bit libm differences
