@@ -82,6 +82,17 @@ def build_fortran(verbose=True):
         raise RuntimeError(f"link of axion_3d failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {exe}")
+    # the reference-side binding with halo-padded arrays, as a runnable check
+    obj = os.path.join(BUILD, "halo_binding_check.o")
+    chk = os.path.join(BINDIR, "halo_binding_check")
+    for cmd in ([FC, "-O2", "-c", os.path.join(fdir, "halo_binding_check.f90"), "-o", obj, "-module-dir", BUILD,
+                 "-I", BUILD],
+                [FC, "-o", chk, objs[0], obj, "-L", LIBDIR, "-laxgrav", f"-Wl,-rpath,$ORIGIN/../lib"]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {chk}")
     return exe
 
 

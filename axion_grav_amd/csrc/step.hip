@@ -167,13 +167,16 @@ template <int N> constexpr int fused_rows_wide() { return N >= 1024 ? 1 : (1024 
 
 // One site of the fused pass: kick of p (evolve.f90:325-330 with potential.f90:43-60, in the
 // reference's order), the next drift f = c + dt_drift p, and the gradient (grav_waves.f90:
-// 331-351, times 1/(2 dspa): exactly the division when dspa is a power of two) with the packed
-// source products of grav_waves.f90:213-223, C0 = S11 + i S21, C1 = S22 + i S31,
-// C2 = S32 + i S33.
+// 331-351) with the packed source products of grav_waves.f90:213-223, C0 = S11 + i S21,
+// C1 = S22 + i S31, C2 = S32 + i S33.  The reference divides the differences by d2 = 2 dspa.
+// Multiplying by inv_d = 1/d2 gives the same bits when d2 is a power of two; for any other
+// dspa the host sets div and the FP64 path divides.  FP32 storage always multiplies: its
+// gradient is rounded to FP32, far coarser than the FP64 quotient's last bit.
 template <typename T>
 __device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, double2 b1, double2 b2, double2 c1v,
                                           double2 c2v, double2& p, double2& f, typename V2<T>::type (&src)[3],
-                                          const KickScalars& k, double inv_d, double dt_drift)
+                                          const KickScalars& k, double inv_d, double d2, bool div,
+                                          double dt_drift)
 {
 #pragma clang fp contract(off)
     using T2 = typename V2<T>::type;
@@ -200,9 +203,16 @@ __device__ __forceinline__ void kick_site(double2 c, double2 a1, double2 a2, dou
         src[1] = T2{2 * (tx * tx + ty * ty), 2 * (ux * sx + uy * sy)};
         src[2] = T2{2 * (ux * tx + uy * ty), 2 * (ux * ux + uy * uy)};
     } else {
-        double2 gx = {(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
-        double2 gy = {(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
-        double2 gz = {(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
+        double2 gx, gy, gz;
+        if (div) {
+            gx = double2{(a2.x - a1.x) / d2, (a2.y - a1.y) / d2};
+            gy = double2{(b2.x - b1.x) / d2, (b2.y - b1.y) / d2};
+            gz = double2{(c2v.x - c1v.x) / d2, (c2v.y - c1v.y) / d2};
+        } else {
+            gx = double2{(a2.x - a1.x) * inv_d, (a2.y - a1.y) * inv_d};
+            gy = double2{(b2.x - b1.x) * inv_d, (b2.y - b1.y) * inv_d};
+            gz = double2{(c2v.x - c1v.x) * inv_d, (c2v.y - c1v.y) * inv_d};
+        }
         src[0] = T2{(T)(2 * (gx.x * gx.x + gx.y * gx.y)), (T)(2 * (gy.x * gx.x + gy.y * gx.y))};
         src[1] = T2{(T)(2 * (gy.x * gy.x + gy.y * gy.y)), (T)(2 * (gz.x * gx.x + gz.y * gx.y))};
         src[2] = T2{(T)(2 * (gz.x * gy.x + gz.y * gy.y)), (T)(2 * (gz.x * gz.x + gz.y * gz.y))};
@@ -243,7 +253,7 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, int P, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double dt_drift, const StepEntry* __restrict__ se)
+    KickScalars k, double inv_d, double d2, int div, double dt_drift, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
     if (se) {
@@ -284,7 +294,7 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
         double2 p = ldf<T, COMP>(pi, cpi, i);
         double2 f;
         T2 src[3];
-        kick_site<T>(c, a1, a2, b1, b2, c1v, c2v, p, f, src, k, inv_d, dt_drift);
+        kick_site<T>(c, a1, a2, b1, b2, c1v, c2v, p, f, src, k, inv_d, d2, div != 0, dt_drift);
         stf<T, COMP>(pi, cpi, i, p);
         stf<T, COMP>(phi_out, cphi_out, i, f);
         s[rr * N + x] = src[0];
@@ -396,7 +406,7 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double dt_drift, const StepEntry* __restrict__ se)
+    KickScalars k, double inv_d, double d2, int div, double dt_drift, const StepEntry* __restrict__ se)
 {
 #pragma clang fp contract(off)
     if (se) {
@@ -431,8 +441,8 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
         ld_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
         double2 f0, f1;
         T2 sa[3], sb[3];
-        kick_site<T>(c0, l, c1, b10, b20, z10, z20, p0, f0, sa, k, inv_d, dt_drift);
-        kick_site<T>(c1, c0, r, b11, b21, z11, z21, p1, f1, sb, k, inv_d, dt_drift);
+        kick_site<T>(c0, l, c1, b10, b20, z10, z20, p0, f0, sa, k, inv_d, d2, div != 0, dt_drift);
+        kick_site<T>(c1, c0, r, b11, b21, z11, z21, p1, f1, sb, k, inv_d, d2, div != 0, dt_drift);
         st_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
         st_pair<T, COMP>(phi_out, cphi_out, row + x0, f0, f1);
 #pragma unroll
@@ -496,6 +506,10 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
 {
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
+    // the gradient's divisor 2 dspa: a power of two (mantissa 1/2) lets the kernel multiply
+    const double d2 = dspa * 2;
+    int d2e;
+    const int div = frexp(d2, &d2e) != 0.5 ? 1 : 0;
     // configuration (same results; the block shape and order only): bit 0 = wide blocks (twice
     // the rows: 1024 threads), bit 1 = XCD remap by planes (one site per thread only), bit 2 =
     // swap the sites-per-thread choice, bit 3 = XCD strips
@@ -503,7 +517,7 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
         hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
                            (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
-                           1.0 / (dspa * 2), dt_drift, se);
+                           1.0 / d2, d2, div, dt_drift, se);
     };
     auto go = [&]<int NN>() {
         constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
@@ -516,7 +530,7 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                     hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN),
                                        dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
                                        (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
-                                       (T2*)W, (const T2*)tw, k, 1.0 / (dspa * 2), dt_drift, se);
+                                       (T2*)W, (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, se);
                 };
                 // wide: twice the rows when they divide N and the tile stays within 64 KiB of LDS
                 constexpr int X2W =

@@ -604,7 +604,7 @@ contains
     integer(c_int32_t) :: iters
     integer(c_int64_t) :: ncov, key
     integer :: u, i, j, ij
-    double precision :: q, dq, q2dq, ddrho, drho
+    double precision :: drho, qc(num_bin), wb(num_bin)
     ncov = int(num_bin, c_int64_t)*(num_bin + 1)/2
     allocate(cov(ncov))
     write(*, '(A)') ' *estimating covariance matrix using MC (Gaussianity assumed)'
@@ -625,107 +625,103 @@ contains
        end do
     end do
     close(u)
-    drho = 0d0
-    do i = 1, num_bin
-       ddrho = 0d0
-       do j = 1, i
-          ij = i*(i - 1)/2 + j
-          q = (wav_edge(j) + wav_edge(j - 1))/2
-          dq = wav_edge(j) - wav_edge(j - 1)
-          q2dq = q*q*dq/pi_t/pi_t
-          if (j == i) q2dq = q2dq/2
-          ddrho = ddrho + cov(ij)*q2dq
-       end do
-       q = (wav_edge(i) + wav_edge(i - 1))/2
-       dq = wav_edge(i) - wav_edge(i - 1)
-       q2dq = q*q*dq/twopi_t/pi_t
-       drho = drho + ddrho*q2dq
-    end do
-    drho = sqrt(drho)
+    ! compute_crho's error (spectrum.f90:316-342): crho = sum_b P_b a_b / (2 pi^2) with
+    ! a_b = qmid_b^2 dq_b, so delta crho = sqrt(a^T C a) / (2 pi^2)
+    call spec_bins(qc, wb)
+    drho = sqrt(packed_form(cov, qc*qc*wb, qc*qc*wb))/(2*pi_t*pi_t)
     crho_spec(1, idx) = rho
     crho_spec(2, idx) = drho
     write(*, '(A,3E16.4)') '  tau, crho, delta crho: ', tau_id(idx), crho_spec(1:2, idx)
     deallocate(cov)
   end subroutine covariance
 
-  ! Bin edge i of the axion spectrum (init_spectrum, spectrum.f90:58-61): i dq, dq = pi/dspa/num_bin.
-  double precision function wav_edge(i)
-    integer, intent(in) :: i
-    integer :: k
-    double precision :: dq
-    dq = pi_t/prm%dspa/num_bin
-    wav_edge = 0d0
-    do k = 1, i
-       wav_edge = wav_edge + dq
+  ! Centres and widths of the axion spectrum's bins.  The edges are those of init_spectrum
+  ! (spectrum.f90:58-61): running sums of dq = pi/dspa/num_bin from 0.
+  subroutine spec_bins(qc, wb)
+    double precision, intent(out) :: qc(num_bin), wb(num_bin)
+    double precision :: lo, hi, step
+    integer :: b
+    step = pi_t/prm%dspa/num_bin
+    hi = 0d0
+    do b = 1, num_bin
+       lo = hi
+       hi = hi + step
+       qc(b) = (lo + hi)/2
+       wb(b) = hi - lo
     end do
-  end function wav_edge
+  end subroutine spec_bins
 
-  ! get_epsilon (spectrum.f90:956-1040): the last n_id spectra and covariances read back from
-  ! their files; for each pair (idx, idx + n_mod_eps) the energy-weighted slope eps of the
-  ! differential spectrum, its error deps, and the changes of crho, into _eps.txt
-  ! '(2I12,7E20.10e3)'.
+  ! x^T C y for the symmetric num_bin x num_bin matrix C stored packed-upper (the covariance
+  ! files' order: entry (i, j <= i) at i(i-1)/2 + j).
+  double precision function packed_form(cp, x, y)
+    double precision, intent(in) :: cp(:), x(num_bin), y(num_bin)
+    integer :: r, c, k
+    packed_form = 0d0
+    k = 0
+    do r = 1, num_bin
+       do c = 1, r
+          k = k + 1
+          if (c == r) then
+             packed_form = packed_form + cp(k)*x(r)*y(r)
+          else
+             packed_form = packed_form + cp(k)*(x(r)*y(c) + x(c)*y(r))
+          end if
+       end do
+    end do
+  end function packed_form
+
+  ! get_epsilon (spectrum.f90:956-1040), at the end of the run.  The last n_id snapshots' power
+  ! (column 2 of _spectrum_<id>.dat) and covariance (column 3 of _covariance_<id>.dat) are read
+  ! back.  For snapshots a and b = a + n_mod_eps, at tau = (tau_a + tau_b)/2 and f = tau/(4 pi):
+  !   slope     eps  = f sum(dP q^2) / sum(dP q),  dP = P_b - P_a, q = the bin centres;
+  !   its error deps = f sqrt(w^T (C_a + C_b) w) / |sum(dP q)|,  w = q^2 - eps q / f
+  !   (eps's linearisation in the two independent spectra);
+  ! with the change of crho and its error in quadrature, as '(2I12,7E20.10e3)' rows of _eps.txt.
   subroutine get_epsilon()
-    integer :: id, i, j, ij, u, lo
-    double precision, allocatable :: ps(:, :), pc(:, :)
-    double precision :: eps, deps, num, den, d, q, tau, dum1, dum2, f
-    integer :: ncov
+    integer :: a, b, k, u, first, ncov
+    double precision, allocatable :: pw(:, :), cv(:, :)
+    double precision :: qc(num_bin), wb(num_bin), dpw(num_bin), w(num_bin)
+    double precision :: tmid, f, slope, eps, deps, c1, c2
     write(*, *)
     write(*, '(A)') ' #epsilon'
-    lo = n_chk - n_id + 1
+    first = n_chk - n_id + 1
     ncov = num_bin*(num_bin + 1)/2
-    allocate(ps(num_bin, lo:n_chk), pc(ncov, lo:n_chk))
-    do id = lo, n_chk
-       open(newunit=u, file=trim(rootname)//'_spectrum_'//trim(itoa(id))//'.dat', action='read', status='old', &
+    allocate(pw(num_bin, first:n_chk), cv(ncov, first:n_chk))
+    do a = first, n_chk
+       open(newunit=u, file=trim(rootname)//'_spectrum_'//trim(itoa(a))//'.dat', action='read', status='old', &
             iostat=info)
-       if (info /= 0) call die('error: cannot open spectrum.dat')
-       do i = 1, num_bin
-          read(u, *, iostat=info) dum1, ps(i, id), dum2
-          if (info /= 0) call die('error: cannot read spectrum.dat')
+       if (info /= 0) call die(' error: spectrum file '//trim(itoa(a))//' is missing')
+       do k = 1, num_bin
+          read(u, *, iostat=info) c1, pw(k, a)
+          if (info /= 0) call die(' error: short spectrum file '//trim(itoa(a)))
        end do
        close(u)
-       open(newunit=u, file=trim(rootname)//'_covariance_'//trim(itoa(id))//'.dat', action='read', status='old', &
+       open(newunit=u, file=trim(rootname)//'_covariance_'//trim(itoa(a))//'.dat', action='read', status='old', &
             iostat=info)
-       if (info /= 0) call die('error: cannot open covariance.dat')
-       do i = 1, num_bin
-          do j = 1, i
-             ij = i*(i - 1)/2 + j
-             read(u, *, iostat=info) dum1, dum2, pc(ij, id)
-             if (info /= 0) call die('error: cannot read spectrum.dat')
-          end do
+       if (info /= 0) call die(' error: covariance file '//trim(itoa(a))//' is missing')
+       do k = 1, ncov
+          read(u, *, iostat=info) c1, c2, cv(k, a)
+          if (info /= 0) call die(' error: short covariance file '//trim(itoa(a)))
        end do
        close(u)
     end do
+    call spec_bins(qc, wb)
     open(newunit=u, file=trim(rootname)//'_eps.txt', action='write', status='replace', iostat=info)
-    if (info /= 0) call die('error: cannot open eps.txt')
-    do id = lo, n_chk - n_mod_eps, n_mod_eps
-       tau = (tau_id(id + n_mod_eps) + tau_id(id))/2
-       f = tau/2/twopi_t
-       num = 0d0
-       den = 0d0
-       do i = 1, num_bin
-          d = ps(i, id + n_mod_eps) - ps(i, id)
-          q = (wav_edge(i - 1) + wav_edge(i))/2
-          num = num + d*q*q
-          den = den + d*q
-       end do
-       eps = num/den*f
-       deps = 0d0
-       do i = 1, num_bin
-          dum1 = (wav_edge(i - 1) + wav_edge(i))/2
-          do j = 1, i
-             ij = i*(i - 1)/2 + j
-             dum2 = (wav_edge(j - 1) + wav_edge(j))/2
-             d = (1 - eps/dum1/f)*(1 - eps/dum2/f)*(pc(ij, id + n_mod_eps) + pc(ij, id))*dum1**2*dum2**2
-             if (j /= i) d = d*2
-             deps = deps + d
-          end do
-       end do
-       deps = sqrt(deps/den/den)*f
-       write(u, '(2I12,7E20.10e3)') id, id + n_mod_eps, tau_id(id), tau_id(id + n_mod_eps), tau, eps, deps, &
-            crho_spec(1, id + n_mod_eps) - crho_spec(1, id), sqrt(crho_spec(2, id + n_mod_eps)**2 + crho_spec(2, id)**2)
+    if (info /= 0) call die(' error: cannot create the eps file')
+    do a = first, n_chk - n_mod_eps, n_mod_eps
+       b = a + n_mod_eps
+       tmid = (tau_id(b) + tau_id(a))/2
+       f = tmid/(2*twopi_t)
+       dpw = pw(:, b) - pw(:, a)
+       slope = sum(dpw*qc)
+       eps = f*sum(dpw*qc*qc)/slope
+       w = qc*qc - eps*qc/f
+       deps = f*sqrt(packed_form(cv(:, a) + cv(:, b), w, w))/abs(slope)
+       write(u, '(2I12,7E20.10e3)') a, b, tau_id(a), tau_id(b), tmid, eps, deps, &
+            crho_spec(1, b) - crho_spec(1, a), hypot(crho_spec(2, b), crho_spec(2, a))
     end do
     close(u)
-    deallocate(ps, pc)
+    deallocate(pw, cv)
   end subroutine get_epsilon
 
 end program axion_3d

@@ -182,6 +182,13 @@ module axgrav_capi
        integer(c_int64_t), intent(out) :: count
      end function axg_get_mask
 
+     integer(c_int) function axg_get_flags(h, flags, nsite) bind(C, name="axg_get_flags")
+       import :: c_int, c_ptr, c_int8_t, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int8_t), intent(out) :: flags(*)
+       integer(c_int64_t), value :: nsite
+     end function axg_get_flags
+
      integer(c_int) function axg_strings(h, seed0, seed1, out) bind(C, name="axg_strings")
        import :: c_int, c_ptr, c_int32_t, axg_string_out
        type(c_ptr), value :: h

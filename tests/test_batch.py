@@ -67,3 +67,37 @@ def test_batch_rejects_bad_arguments():
             capi.batch_advance([L], 1.0, 0.05, 4, 3)  # odd chunk
         with pytest.raises(capi.AxgError):
             capi.batch_advance([L, L], 1.0, 0.05, 4, 2)  # the same lattice twice
+
+
+def test_batch_mixed_workload_sizes():
+    """The mixed-size workload's lattice sizes in one captured graph: two each of 64^3, 96^3 and
+    128^3, FP32C with GW (the kernels the batch benchmark times), bit-identical to the same
+    lattices advanced one by one, and their size-grouped mean GW spectra equal."""
+    print("[case] batch 64/96/128", flush=True)
+    t1, dt, nsteps = 1.0, 0.025, 6
+    lats, ref = [], []
+    try:
+        for i, N in enumerate((64, 96, 128, 64, 96, 128)):
+            P = phys(N)
+            prm = capi.make_params(N, P["dspa"], SQ_HALF, precision=capi.FP32C, num_bin_grav=64)
+            L = capi.Lattice(prm)
+            L.init_flat(i, 4.0)
+            lats.append(L)
+            phi, pi = L.get_fields()
+            with capi.Lattice(prm) as R:
+                R.set_fields(phi, pi)
+                R.advance_steps(t1, dt, nsteps)
+                ref.append((R.get_fields(), R.gw_state(), R.gw_spectrum()["power"]))
+        capi.batch_advance(lats, t1, dt, nsteps, 2)
+        spec = {}
+        for L, ((rphi, rpi), rA, rpow) in zip(lats, ref):
+            phi, pi = L.get_fields()
+            assert np.array_equal(phi, rphi) and np.array_equal(pi, rpi)
+            assert np.array_equal(L.gw_state(), rA)
+            spec.setdefault(L.N, []).append(L.gw_spectrum()["power"])
+        for N, s in spec.items():
+            r = [p for (_, _, p), L in zip(ref, lats) if L.N == N]
+            assert np.array_equal(np.mean(s, axis=0), np.mean(r, axis=0))
+    finally:
+        for L in lats:
+            L.close()

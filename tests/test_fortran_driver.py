@@ -291,3 +291,19 @@ def test_grouped_networks_and_defect_files(tmp_path):
         assert np.allclose(fw[:, 0:8], wrec[:, 0:8], atol=1.5e-3)
         assert np.array_equal(fs[:, 11], ids_s) and np.array_equal(fs[:, 12], ids_s)
         assert np.array_equal(fw[:, 8], ids_w) and np.array_equal(fw[:, 9], ids_w)
+
+
+HALO_EXE = os.path.join(ROOT, "axion_grav_amd", "bin", "halo_binding_check")
+
+
+@pytest.mark.skipif(not os.path.exists(HALO_EXE), reason="halo_binding_check not built (no amdflang)")
+@pytest.mark.gpu
+def test_halo_padded_binding_roundtrip():
+    """The reference's data1/data2 layout (ngrid x ngrid x (ngrid+2), halo planes 1 and ngrid+2,
+    initial.f90:240-285) through the binding INTEGRATION.md shows: the interior sections go in and
+    come back bit for bit, the halo planes are never touched, and a domain wall placed between
+    interior planes ngrid/2 and ngrid/2+1 is flagged on exactly those cells (a one-plane shift of
+    the fields would move it)."""
+    print("[case] halo-padded Fortran binding", flush=True)
+    r = subprocess.run([HALO_EXE], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "halo binding ok" in r.stdout, r.stdout + r.stderr

@@ -858,3 +858,125 @@ def test_walls_records_and_sums():
     # a plane cuts a unit cube in at most ~1.41 (sqrt(2)); for non-planar edge points the triangle
     # fan of the reference can give a slightly negative area
     assert np.all(rec[:, 7] > -0.1) and np.all(rec[:, 7] < 1.5) and np.median(rec[:, 7]) > 0.3
+
+
+# ------------------------------------------------------------------ GW power prefactor, units, maxion2
+
+# potential.f90:13-19, typed in here so that the oracle's constants are checked against them
+BIGH_GEV = 2.1332e-42       # H0/h in GeV
+MSTAR_GEV = 2.435e18        # reduced Planck mass in GeV
+GEV_IN_K = 1.16045250061657e13
+TCMB_K = 2.726
+G_REL_NOWS = 3.9091
+
+
+def _single_mode_state(N, n, amp):
+    """Accumulators with one nonzero mode n = (nx, ny, nz) of component (1,1): |A|^2 = amp^2."""
+    A = O.new_gw_state(N)
+    A[0, n[2] % N, n[1] % N, n[0] % N] = amp
+    return A
+
+
+def test_gw_power_scales_with_tau_cr_a_now_and_dv_wav():
+    """power = numer/denom qmid^3 fact with fact = tau_cr^-6 (BigH/M*)^-2 a_now^-4 / 3 / (2 pi)^2
+    dv_wav (grav_waves.f90:266, 301): doubling tau_cr divides every bin by 2^6, doubling a_now by
+    2^4, doubling dv_wav multiplies by 2.  A wrong exponent fails one of these."""
+    N, nb = 16, 8
+    P = phys(N)
+    A = _single_mode_state(N, (2, 1, 0), 1.5)
+    base = O.gw_spectrum(A, nb, P["dspa"], P["dwav"], 1.0, 1.0, P["dv_wav"])["power"]
+    b = np.nonzero(base)[0]
+    assert len(b) == 1
+    for kw, ratio in ((dict(tau_cr=2.0), 2.0 ** -6), (dict(a_now=2.0), 2.0 ** -4), (dict(dv_wav=2 * P["dv_wav"]), 2.0),
+                      (dict(tau_cr=3.0, a_now=5.0), 3.0 ** -6 * 5.0 ** -4)):
+        args = dict(tau_cr=1.0, a_now=1.0, dv_wav=P["dv_wav"])
+        args.update(kw)
+        p = O.gw_spectrum(A, nb, P["dspa"], P["dwav"], args["tau_cr"], args["a_now"], args["dv_wav"])["power"]
+        assert abs(p[b[0]] / base[b[0]] / ratio - 1) < 1e-14, kw
+
+
+def test_gw_power_of_single_modes_is_cubic_in_qmid():
+    """One unit mode in bin b gives power_b = qmid_b^3 fact / denom_b (denom_b = the lattice
+    vectors in the bin, by brute force): two modes in different bins compare as the cube of their
+    bin centres, and the absolute value matches fact from the constants of potential.f90:13-19
+    (Omega_GW h^2 normalisation: 1/(3 (BigH/M*)^2) for the critical density in units of M*^2
+    (H0/h)^2, a_now^-4 redshift, tau_cr^-6 and (2 pi)^-2 of the reference's units)."""
+    N, nb = 16, 8
+    tau_cr, a_now = 2.5, 3.0e3
+    P = phys(N)
+    dq = PI_T / P["dspa"] / nb
+    idx = np.arange(N)
+    n = np.where(idx < N // 2, idx, idx - N)
+    nz, ny, nx = np.meshgrid(n, n, n, indexing="ij")
+    q = np.sqrt((nx * nx + ny * ny + nz * nz).astype(float)) * P["dwav"]
+    b_of = np.floor(q / dq + 0.5 + 0.5).astype(int)
+    vals = {}
+    for mode in ((1, 0, 0), (3, 2, 1)):
+        A = _single_mode_state(N, mode, 1.0)
+        s = O.gw_spectrum(A, nb, P["dspa"], P["dwav"], tau_cr, a_now, P["dv_wav"])
+        qm = np.sqrt(sum(m * m for m in mode)) * P["dwav"]
+        b = int(np.floor(qm / dq + 0.5 + 0.5))
+        denom = int(np.count_nonzero((b_of == b) & (q > 0)))
+        assert s["denom"][b - 1] == denom and s["numer"][b - 1] == 1.0
+        qmid = (b - 0.5) * dq
+        assert abs(s["qmid"][b - 1] / qmid - 1) < 1e-14
+        vals[b] = (s["power"][b - 1], qmid, denom)
+    (p1, q1, d1), (p2, q2, d2) = vals.values()
+    assert abs((p1 * d1) / (p2 * d2) / (q1 / q2) ** 3 - 1) < 1e-13
+    twopi = 2 * PI_T
+    fact = (P["dv_wav"] / (tau_cr ** 6 * a_now ** 4)) / (3 * (BIGH_GEV / MSTAR_GEV) ** 2 * twopi ** 2)
+    assert abs(p1 / (q1 ** 3 / d1 * fact) - 1) < 1e-13
+
+
+def test_derived_units_from_physics():
+    """settings.f90:274-276 against the physics they encode: a_now T_cmb = T_cr (g_rel /
+    g_s,now)^(1/3) with T_cr = sqrt(6) sigma (entropy conservation, a_cr = 1); tau_cr = 1/H(T_cr)
+    with the radiation-era Friedmann rate H^2 = (pi^2 g_rel T^4 / 30)/3 in units of M*;
+    sigma_cr = sigma tau_cr.  The reference's truncated pi leaves ~1e-11 of difference against
+    numpy's pi in tau_cr."""
+    sigma, g_rel = 1.3e-9, 106.75
+    d = O.derive(sigma, g_rel, 1e-30, 20.0, 2.0, 64)
+    t_cr = np.sqrt(6.0) * sigma
+    t_cmb = TCMB_K / GEV_IN_K / MSTAR_GEV
+    assert abs(d["a_now"] * t_cmb / (t_cr * (g_rel / G_REL_NOWS) ** (1 / 3)) - 1) < 1e-14
+    H = np.sqrt(np.pi ** 2 * g_rel * t_cr ** 4 / 30 / 3)
+    assert abs(d["tau_cr"] * H - 1) < 1e-10
+    assert d["sigma_cr"] == d["tau_cr"] * sigma
+    assert d["dspa"] == 2.0 * 20.0 / 64 and abs(d["dwav"] * d["dspa"] * 64 / (2 * PI_T) - 1) < 1e-15
+
+
+def test_energy_gradient_term_plane_wave():
+    """For phi = a exp(i k.x) with lattice wave numbers k_d = 2 pi n_d / N the forward
+    differences give sum_x sum_d |phi(x+e_d) - phi(x)|^2 / dspa^2 = N^3 |a|^2 sum_d 2 (1 - cos k_d)
+    / dspa^2, times dv_spa = dspa^3 (a wrong neighbour, a missing axis or the 1/dspa^2 scale
+    fails it)."""
+    N, dspa = 12, 0.3
+    a = 0.4 - 0.7j
+    for n in ((1, 0, 0), (0, 2, 0), (1, 3, 5), (6, 0, 2)):
+        z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+        k = [2 * np.pi * v / N for v in n]
+        phi = np.ascontiguousarray(a * np.exp(1j * (k[0] * x + k[1] * y + k[2] * z)))
+        e = O.energy(phi, np.zeros_like(phi), 1.5, dspa, SQ_HALF, 1e-30, True)
+        exp = N ** 3 * abs(a) ** 2 * sum(2 * (1 - np.cos(kd)) for kd in k) / dspa ** 2 * dspa ** 3
+        assert abs(e[1] / exp - 1) < 1e-12, n
+
+
+def test_maxion2_branches():
+    """potential.f90:62-79: kappa^4 c0 below thrm12 = kappa (cT/c0)^(1/n), kappa^4 cT (kappa/T)^n
+    above; the two meet at thrm12 (continuity), and on the power-law side d ln m^2 / d ln T =
+    -n = -6.68."""
+    for kappa in (0.05, 1.0, 1.37, 3.0):
+        k4 = kappa ** 4
+        t12 = kappa * 6.26 ** (1 / 6.68)
+        assert abs(O.maxion2(0.5 * t12, kappa) / k4 - 1) < 1e-15
+        assert O.maxion2(0.0, kappa) == O.maxion2(0.5 * t12, kappa)
+        lo, hi = O.maxion2(t12 * (1 - 1e-9), kappa), O.maxion2(t12 * (1 + 1e-9), kappa)
+        assert abs(hi / lo - 1) < 1e-7 and lo == O.maxion2(0.0, kappa)
+        t = 3.0 * t12
+        h = 1e-5
+        slope = (np.log(O.maxion2(t * (1 + h), kappa)) - np.log(O.maxion2(t * (1 - h), kappa))) / (
+            np.log(1 + h) - np.log(1 - h))
+        assert abs(slope + 6.68) < 1e-6
+        assert abs(O.maxion2(t, kappa) / (k4 * 6.26 * (kappa / t) ** 6.68) - 1) < 1e-14
+    # msquared (potential.f90:34-41): zero at T_cr = sqrt 6, -2 at T = 0
+    assert abs(O.msquared(2.449489742783178)) < 1e-15 and abs(O.msquared(0.0) + 2) < 1e-15
