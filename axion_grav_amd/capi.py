@@ -57,6 +57,11 @@ class DefectOut(C.Structure):
     _fields_ = [("n_string", C.c_int64), ("n_wall", C.c_int64), ("n_mask", C.c_int64)]
 
 
+class PlaquetteOut(C.Structure):
+    _fields_ = [("n_pierced", C.c_int64 * 3), ("n_plus", C.c_int64 * 3), ("n_minus", C.c_int64 * 3),
+                ("n_quadrant", C.c_int64 * 3), ("net", C.c_int64 * 3)]
+
+
 class WallOut(C.Structure):
     _fields_ = [("n_wall", C.c_int64), ("area", C.c_double)]
 
@@ -79,7 +84,9 @@ EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_
            "axg_advance_steps", "axg_gw_spectrum", "axg_defects", "axg_get_mask", "axg_get_flags",
            "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls",
            "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
-           "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_batch_advance", "axg_energy", "axg_tune", "axg_profile")
+           "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_batch_advance", "axg_energy",
+           "axg_tune", "axg_profile", "axg_hbm_probe", "axg_plaquettes",
+           "axg_get_plaquettes")
 
 _LIB = None
 
@@ -109,6 +116,8 @@ def lib():
             "axg_defects": [p, C.POINTER(DefectOut)],
             "axg_get_mask": [p, p, i64, p],
             "axg_get_flags": [p, p, i64],
+            "axg_plaquettes": [p, C.POINTER(PlaquetteOut)],
+            "axg_get_plaquettes": [p, p, i64, p],
             "axg_strings": [p, C.c_uint32, C.c_uint32, C.POINTER(StringOut)],
             "axg_get_strings": [p, p, i64, p],
             "axg_walls": [p, C.POINTER(WallOut)],
@@ -122,6 +131,7 @@ def lib():
             "axg_batch_advance": [p, i32, d, d, i32, i32, p],
             "axg_get_adjoint": [p, p, p, i64],
             "axg_energy": [p, d, p],
+            "axg_hbm_probe": [i32, i64, i32, p],
             "axg_profile": [p, i32, p, p, i32],
             "axg_tune": [p, i32, p, p, i32],
         }
@@ -163,6 +173,16 @@ def batch_advance(lattices, t1, dt, nsteps, chunk=10):
     if rc != 0:
         raise AxgError(f"axg_batch_advance failed ({ERRORS.get(rc, rc)}): {lib().axg_last_error(lattices[0]._h).decode()}")
     return ms.value
+
+
+def hbm_probe(nbytes=4 << 30, reps=10, device=0):
+    """Streaming bandwidth of the device's HBM with the library's own 16-byte-per-lane kernels:
+    dict(read, write, copy, copy_nt) in GB/s (a copy counts its reads and its writes)."""
+    g = np.zeros(4)
+    rc = lib().axg_hbm_probe(device, int(nbytes), int(reps), _ptr(g))
+    if rc != 0:
+        raise AxgError(f"axg_hbm_probe failed ({ERRORS.get(rc, rc)}): {lib().axg_last_error(None).decode()}")
+    return dict(zip(("read", "write", "copy", "copy_nt"), (float(x) for x in g)))
 
 
 def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initial_loop=True, kappa=1e-30,
@@ -304,6 +324,22 @@ class Lattice:
         f = np.empty((self.N,) * 3, np.int8)
         self._check(lib().axg_get_flags(self._h, _ptr(f), self.n3), "axg_get_flags")
         return f
+
+    def plaquettes(self):
+        """Plaquette windings of the current phi: dict of (3,) int64 arrays n_pierced, n_plus,
+        n_minus, n_quadrant, net (faces (x,y), (y,z), (z,x))."""
+        out = PlaquetteOut()
+        self._check(lib().axg_plaquettes(self._h, C.byref(out)), "axg_plaquettes")
+        return {k: np.array(getattr(out, k)[:], np.int64) for k, _ in PlaquetteOut._fields_}
+
+    def plaquette_records(self):
+        """(n, 5) int32 rows (ix, iy, iz, o, n) of the pierced faces of the last plaquettes()."""
+        cnt = C.c_int64()
+        self._check(lib().axg_get_plaquettes(self._h, None, 0, C.byref(cnt)), "axg_get_plaquettes")
+        n = cnt.value
+        buf = np.zeros((max(n, 1), 5), np.int32)
+        self._check(lib().axg_get_plaquettes(self._h, _ptr(buf), n, C.byref(cnt)), "axg_get_plaquettes")
+        return buf[:n]
 
     def strings(self, seed0, seed1):
         """Fit every string cell of the last defects() (needs id_strings): dict(n_string, n_vel,

@@ -35,6 +35,7 @@ struct KickScalars {
     double f1, f2, f3, m2, ma2, kt;  // kt: N_DW=2 tilt coefficient kappa0*t^4 (0 = off)
 };
 KickScalars kick_scalars(const axg_params& p, double t, double dt);
+void set_handleless_error(const std::string& msg);
 // One step's scalars in device memory, for steps replayed from a captured graph: the step
 // kernels take them from here instead of their by-value arguments when the pointer is set.
 struct StepEntry {
@@ -226,6 +227,16 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
                        double* rec, double* partial, int* npend, int64_t* pend, double* state);
 
 void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col);
+
+// Plaquette windings (plaquettes.hip): per-site face codes (uint16: 3 bits of winding + 2 per face
+// o, then the quadrant-test bit of face o at 9 + o), per-block counts [nblk][16] (pierced, +1, -1,
+// quadrant, net per face, then the block's pierced total), their exclusive offsets of the
+// pierced totals and the 16 column totals.  The compaction writes rows (ix, iy, iz, o, n).
+constexpr int PLAQ_BLOCK = 256;
+template <typename T, bool COMP>
+void launch_plaquettes(hipStream_t s, int N, const void* phi, const void* cphi, uint16_t* code, int64_t* counts,
+                       int64_t* offs, int64_t* total);
+void launch_plaquette_compact(hipStream_t s, int N, const uint16_t* code, const int64_t* offs, int32_t* rec);
 
 template <typename T, bool COMP>
 void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags,

@@ -135,6 +135,13 @@ struct axg_handle {
     int64_t nmask = -1;       // -1: no classification yet
     int64_t nstring = 0;      // string cells of the last classification
     bool have_flags = false;
+    // plaquette windings (grown on demand): face codes, per-block counts / offsets / totals, rows
+    uint16_t* plaq_code = nullptr;
+    int64_t* plaq_cnt = nullptr;
+    int32_t* plaq_rec = nullptr;
+    int64_t plaq_rec_cap = 0;
+    int64_t nplaq = 0;        // pierced faces of the last axg_plaquettes
+    bool have_plaq = false;
     // string fits (grown on demand): cell list, 11 doubles per string, per-block sums
     int32_t* str_cells = nullptr;
     double* str_rec = nullptr;
@@ -252,6 +259,12 @@ struct axg_handle {
 };
 
 static thread_local std::string g_create_err;
+
+namespace axg {
+// The message axg_last_error(NULL) returns: the last failure of a call without a handle
+// (axg_create, axg_energy_fields) on the calling thread.
+void set_handleless_error(const std::string& msg) { g_create_err = msg; }
+}  // namespace axg
 
 template <typename F>
 static int guarded(axg_handle* h, F&& f)
@@ -537,6 +550,7 @@ extern "C" int axg_set_fields(axg_handle* h, const double* phi, const double* pi
             store_from_work(h, h->pi, h->cpi);
         }
         h->have_flags = false;
+        h->have_plaq = false;
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
@@ -705,6 +719,7 @@ static void advance(axg_handle* h, double t1, double dt, int32_t nt)
     else if (h->fp32) run_steps<float, false>(h, t1, dt, nt);
     else run_steps<double, false>(h, t1, dt, nt);
     h->have_flags = false;
+    h->have_plaq = false;
     h->have_spec = false;
     h->nstr_rec = -1;
     h->nwall_rec = -1;
@@ -957,6 +972,64 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         h->nstr_rec = -1;
         h->nwall_rec = -1;
         h->have_grp[0] = h->have_grp[1] = false;
+    });
+}
+
+extern "C" int axg_plaquettes(axg_handle* h, axg_plaquette_out* out)
+{
+    return guarded(h, [&] {
+        require(out != nullptr, "null output");
+        const int64_t nblk = (h->n3 + PLAQ_BLOCK - 1) / PLAQ_BLOCK;
+        if (!h->plaq_code) {
+            h->plaq_code = (uint16_t*)h->alloc((size_t)h->n3 * sizeof(uint16_t));
+            h->plaq_cnt = (int64_t*)h->alloc((size_t)(17 * nblk + 16) * sizeof(int64_t));
+        }
+        int64_t* counts = h->plaq_cnt;
+        int64_t* offs = counts + 16 * nblk;
+        int64_t* tot = offs + nblk;
+        if (h->comp)
+            launch_plaquettes<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->plaq_code, counts, offs,
+                                           tot);
+        else if (h->fp32)
+            launch_plaquettes<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot);
+        else
+            launch_plaquettes<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot);
+        int64_t t[16];
+        AXG_HIP(hipMemcpyAsync(t, tot, sizeof(t), hipMemcpyDeviceToHost, h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
+        for (int o = 0; o < 3; o++) {
+            out->n_pierced[o] = t[5 * o + 0];
+            out->n_plus[o] = t[5 * o + 1];
+            out->n_minus[o] = t[5 * o + 2];
+            out->n_quadrant[o] = t[5 * o + 3];
+            out->net[o] = t[5 * o + 4];
+        }
+        h->nplaq = t[15];
+        h->have_plaq = true;
+    });
+}
+
+extern "C" int axg_get_plaquettes(axg_handle* h, int32_t* rec, int64_t cap, int64_t* count)
+{
+    return guarded(h, [&] {
+        require(h->have_plaq, "call axg_plaquettes first");
+        require(count != nullptr, "null count");
+        *count = h->nplaq;
+        if (h->nplaq == 0 || rec == nullptr || cap == 0) return;  // size query
+        require(cap >= h->nplaq, "plaquette buffer too small");
+        if (h->nplaq > h->plaq_rec_cap) {
+            void* a = h->plaq_rec;
+            AXG_HIP(hipStreamSynchronize(h->stream));
+            h->release_one(a);
+            h->plaq_rec_cap = 0;
+            h->plaq_rec = (int32_t*)h->alloc((size_t)5 * h->nplaq * sizeof(int32_t));
+            h->plaq_rec_cap = h->nplaq;
+        }
+        const int64_t nblk = (h->n3 + PLAQ_BLOCK - 1) / PLAQ_BLOCK;
+        launch_plaquette_compact(h->stream, h->N, h->plaq_code, h->plaq_cnt + 16 * nblk, h->plaq_rec);
+        AXG_HIP(hipMemcpyAsync(rec, h->plaq_rec, (size_t)5 * h->nplaq * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               h->stream));
+        AXG_HIP(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -1659,6 +1732,7 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
         for (int i = 0; i < n; i++) {
             axg_handle* h = hs[i];
             h->have_flags = false;
+            h->have_plaq = false;
             h->have_spec = false;
             h->nstr_rec = -1;
             h->nwall_rec = -1;
@@ -1868,6 +1942,7 @@ extern "C" int axg_gradient(axg_handle* h, double t1, double dt, int32_t nsteps,
         else if (h->fp32) gradient_impl<float, false>(h, t1, dt, nsteps, K, qlo, qhi, out);
         else gradient_impl<double, false>(h, t1, dt, nsteps, K, qlo, qhi, out);
         h->have_flags = false;
+        h->have_plaq = false;
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
@@ -1924,6 +1999,7 @@ extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
         }
         AXG_HIP(hipStreamSynchronize(h->stream));
         h->have_flags = false;
+        h->have_plaq = false;
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
@@ -1997,6 +2073,7 @@ extern "C" int axg_init_flat(axg_handle* h, uint64_t seed, double kmax)
         (void)hipFree(pidx);
         (void)hipFree(pval);
         h->have_flags = false;
+        h->have_plaq = false;
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;

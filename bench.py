@@ -16,7 +16,13 @@ Prints ONE JSON line (rank 0).  `roofline` covers the dominant kernel, its achie
 algorithmic bytes per launch over its mean duration measured with HIP events on the stream
 the kernels run on; `traffic` is its counter-measured HBM bytes per launch from the committed
 512^3 profile.  `cpu_baseline` is the CPU FP64 oracle (the reference dataflow restated in C,
-OpenMP) timed on this host on a bounded sample: two steps at 512^3.
+OpenMP) timed on this host on a bounded sample: three steps at 512^3 after one untimed.
+
+Besides the headline line's FP32C and FP64 step rates, the default run also reports (each can be
+skipped): the HBM streaming probe (read / write / copy with the library's own kernels), the
+diagnostics' costs, the adjoint's cost at 128^3 with its finite-difference check, the
+48-realization mixed-size batch, the 1024^3 largest share, and the FP32C-vs-FP64 GW spectrum error
+over 2000 steps at 512^3.
 """
 import argparse
 import json
@@ -27,6 +33,8 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# the CPU baseline's OpenMP threads stay on neighbouring cores (read when the oracle loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
@@ -156,6 +164,48 @@ def run_gpu(capi, N, L, precision, steps, warmup, seed, diag=False, tune=True):
                                                   "ms": {k: round(v, 4) for k, v in tuned[2].items()}})
 
 
+def run_precision(capi, N, L, seed, steps=2000, every=250, dt=0.01, bins=128):
+    """The precision workload: FP64 and FP32C lattices from the same seeded field advanced in
+    lockstep with GW on; every `every` steps the FP32C GW spectrum's per-bin relative error against
+    FP64, the cell counts and the string length (axion_grav_amd/precision_study.py).  dt = 0.01, not
+    0.2 dspa: at lambda = 2 the leapfrog passes its stability bound near tau = 23, which 2000 steps of
+    0.05 would cross (the FP64 oracle diverges between steps 400 and 500 there)."""
+    from axion_grav_amd import precision_study as PS
+
+    L64 = PS.lattice(N, L, capi.FP64, bins)
+    L32 = PS.lattice(N, L, capi.FP32C, bins)
+    try:
+        L64.init_flat(seed, 1.0)
+        phi, pi = L64.get_fields()
+        L32.set_fields(phi, pi)
+        del phi, pi
+        t, rows = 1.0, []
+        t0 = time.perf_counter()
+        for blk in range(steps // every):
+            L64.advance_steps(t, dt, every)
+            L32.advance_steps(t, dt, every)
+            t = 1.0 + (blk + 1) * every * dt
+            step = (blk + 1) * every
+            r = PS.compare(t, step, PS.diag(L64, t, step), PS.diag(L32, t, step))
+            rows.append(r)
+            log(f"precision: step {step} tau {t:.2f} GW max rel err {r['gw_relerr_max']:.3g}, "
+                f"cells {r['cells_fp32c']} vs {r['cells_fp64']}")
+        el = time.perf_counter() - t0
+    finally:
+        L32.close()
+        L64.close()
+    return dict(workload=f"{N}^3, L={L:g}, seed {seed}, {steps} steps of {dt:g} from tau 1, {bins} GW bins, "
+                         f"FP32C vs FP64 every {every}",
+                gw_relerr_max=max(r["gw_relerr_max"] for r in rows),
+                gw_relerr_median_last=rows[-1]["gw_relerr_median"],
+                gw_denom_equal=all(r["gw_denom_equal"] for r in rows),
+                cells_equal=all(r["cells_fp64"] == r["cells_fp32c"] for r in rows),
+                string_length_relerr_max=max(r["string_length_relerr"] for r in rows),
+                energy_relerr_max=max(max(r["energy_relerr"]) for r in rows),
+                per_checkpoint=[{k: r[k] for k in ("step", "tau", "gw_relerr_max", "cells_fp64")} for r in rows],
+                wall_s=round(el, 1))
+
+
 def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_end=16.0, check_steps=400):
     """Cost of the gradient of a GW band energy (k in [0.5, 2]) by the adjoint: a plain FP32C
     forward of `steps` steps, then the same steps with checkpoints every K and the backward sweep.
@@ -204,7 +254,10 @@ def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_en
                 forward_ms=round(fwd, 2), forward_with_checkpoints_ms=round(g["forward_ms"], 2),
                 adjoint_ms=round(g["adjoint_ms"], 2), adjoint_over_forward=round(g["adjoint_ms"] / fwd, 3),
                 gradient_over_forward=round((g["forward_ms"] + g["adjoint_ms"]) / fwd, 3),
-                check=dict(steps=check_steps, adjoint_fp32c=ad, fd_fp64=fd, rel_err=rel))
+                check=dict(steps=check_steps, adjoint_fp32c=ad, fd_fp64=fd, rel_err=rel,
+                           horizon=f"finite differences checked over the first {check_steps} of the {steps} steps: "
+                                   f"over all {steps} J varies too roughly with the parameters for central "
+                                   f"differences to converge (relative steps 1e-4 and 1e-6 disagree by 4x)"))
 
 
 def run_batch(capi, steps=600, chunk=10, t_end=16.0):
@@ -296,28 +349,17 @@ def run_largest(capi, N=1024, L=256.0, steps=10, warmup=2, kappa0=1e-6, seed=202
                 diagnostics=d)
 
 
-def copy_probe(nbytes=4 << 30, reps=10):
-    """Device-to-device copy of nbytes (read + write = 2 nbytes per copy): the achievable
-    streaming bandwidth on this GPU, measured with torch's copy kernel."""
-    import torch
-
-    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    gbs = 2 * nbytes * reps / el / 1e9
-    log(f"copy probe: {gbs:.0f} GB/s ({nbytes >> 20} MiB x {reps})")
-    del a, b
-    torch.cuda.empty_cache()
-    return gbs
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def run_cpu(N, L, seed, steps=2):
+def run_cpu(N, L, seed, steps=3, warmup=1):
     import oracle as O
 
     dspa = L / N
@@ -325,8 +367,10 @@ def run_cpu(N, L, seed, steps=2):
     P = dict(dspa=dspa, dwav=dwav, dv_spa=dspa ** 3, sigma_cr=SIGMA_CR, kappa=1e-30, initial_loop=True)
     phi, pi = O.ic_flat(N, seed, 1.0, dwav)
     A = O.new_gw_state(N)
+    dt = 0.2 * dspa
+    O.symplectic(phi, pi, A, 1.0, dt, warmup, P)
     t0 = time.perf_counter()
-    O.symplectic(phi, pi, A, 1.0, 0.2 * dspa, steps, P)
+    O.symplectic(phi, pi, A, 1.0 + warmup * dt, dt, steps, P)
     el = time.perf_counter() - t0
     return steps * N ** 3 / el, el
 
@@ -345,11 +389,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the fused pass's configuration chosen at creation (by lattice size)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip timing the gradient (128^3, 800 steps)")
     ap.add_argument("--no-batch", action="store_true", help="skip timing the 48-realization batch")
-    ap.add_argument("--largest", action="store_true",
-                    help="also time the largest single-GPU share (1024^3 FP32C with GW and walls, ~130 GB)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="also time a 4 GiB device-to-device copy (the measured HBM copy ceiling; "
-                         "its kernel is the known-size reference for the FETCH/WRITE counters)")
+    ap.add_argument("--no-largest", action="store_true",
+                    help="skip timing the largest single-GPU share (1024^3 FP32C with GW and walls, ~130 GB)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32C-vs-FP64 spectrum error run (512^3, 2000 steps of 0.01)")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the HBM streaming probe (read, write, copy of 4 GiB with the library's own kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the lattice lives in one MI355X's HBM and this build has no "
@@ -358,7 +403,10 @@ def main():
 
     from axion_grav_amd import capi
 
-    copy_gbs = copy_probe() if args.copy_probe else None
+    probe = None
+    if not args.no_probe:
+        probe = {k: round(v, 1) for k, v in capi.hbm_probe(4 << 30, 10).items()}
+        log(f"HBM probe (4 GiB, GB/s): {probe}")
     N, L = args.n, args.box
     log(f"FP32C {N}^3 L={L}: warmup {args.warmup}, timed {args.steps} steps")
     c = run_gpu(capi, N, L, capi.FP32C, args.steps, args.warmup, args.seed, diag=not args.no_diag,
@@ -387,8 +435,14 @@ def main():
         bat = run_batch(capi)
         log(f"batch: {bat}")
 
+    prec = None
+    if not args.no_precision:
+        log(f"precision: {N}^3 FP32C vs FP64, 2000 steps of 0.01, every 250")
+        prec = run_precision(capi, N, L, args.seed)
+        log(f"precision: max GW rel err {prec['gw_relerr_max']:.3g}, cells equal {prec['cells_equal']}")
+
     big = None
-    if args.largest:
+    if not args.no_largest:
         log("largest: 1024^3 FP32C, GW on, N_DW = 2 tilt")
         big = run_largest(capi)
         log(f"largest: {big}")
@@ -406,10 +460,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"CPU oracle: two steps at {N}^3 on {cores} threads")
+        log(f"CPU oracle: 1 + 3 steps at {N}^3 on {cores} threads")
         su, el = run_cpu(N, L, args.seed)
         cpu = dict(value=su, unit="site-updates/s", cores=cores, kind="port",
-                   sample=f"2 DKD steps with GW at {N}^3 (FP64 oracle, unfused reference dataflow), {el:.1f} s")
+                   sample=f"3 DKD steps with GW at {N}^3 after 1 untimed (FP64 oracle, unfused reference dataflow, "
+                          f"-O3 -march=x86-64-v3), {el:.1f} s; {cpu_model()}, OMP_PROC_BIND="
+                          f"{os.environ.get('OMP_PROC_BIND', 'unset')}")
         log(f"CPU {su:.4g} site-updates/s ({el:.1f} s)")
 
     out = {
@@ -434,12 +490,13 @@ def main():
                                         "step_model_frac": round(MODEL_REALS * 8 * d["su_s"] / 8e12, 4)},
         "fp32c_over_fp64": None if d is None else round(c["su_s"] / d["su_s"], 3),
         "kernels": {k: {"ms": round(v["ms_avg"], 4), "GB/s": round(v["gbs"], 1)} for k, v in c["kernels"].items()},
-        "hbm_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
+        "hbm_probe_gbs": probe,
         "diagnostics": c["diagnostics"],
         "tuning": {"fp32c": c["tuned"], "fp64": None if d is None else d["tuned"]},
         "adjoint": adj,
         "batch": bat,
         "largest": big,
+        "precision": prec,
     }
     print(json.dumps(out), flush=True)
 

@@ -79,8 +79,8 @@ typedef struct axg_handle axg_handle;
  * start at zero (grav_waves.f90:77). */
 int axg_create(const axg_params* p, axg_handle** out);
 int axg_destroy(axg_handle* h);
-/* Message of the last failed call on h; h == NULL gives the last axg_create failure of
- * the calling thread. */
+/* Message of the last failed call on h; h == NULL gives the last failure of a call without a
+ * handle (axg_create, axg_hbm_probe) on the calling thread. */
 const char* axg_last_error(const axg_handle* h);
 
 /* Copy phi (data1) and pi = phi' (data2) in / out; nsite must be N^3.  In FP32C mode the
@@ -133,6 +133,30 @@ typedef struct axg_defect_out {
  * counts them and compacts the mask list on the device.  Flags stay resident for the
  * field spectrum. */
 int axg_defects(axg_handle* h, axg_defect_out* out);
+/* Plaquette windings (no counterpart in the reference as a diagnostic; its plaquette test is the
+ * quadrant count of velocity_string_YY, defects_3d.f90:850-879).  Face o = 0, 1, 2 of the site
+ * (x, y, z) is the elementary square spanned by (u, v) = (x, y), (y, z), (z, x) from that site
+ * (the reference's ipla = 1, 2, 3), looped s -> s+u -> s+u+v -> s+v -> s, counterclockwise about
+ * its normal u x v.  Its winding n is the number of times the shorter arcs between consecutive
+ * corners' phases cross arg = pi counterclockwise minus clockwise (= sum of the wrapped phase
+ * differences / 2 pi; an antipodal pair of corners counts 0), decided by sign tests only, so the
+ * counts are exact integers.  n_quadrant counts the faces whose four corners' phases occupy at
+ * least three quadrants ceiling(2 ph / pi) + 2 (the reference's penetration test).  On a periodic
+ * lattice net[o] = 0, and the windings out of the six faces of every cell sum to 0. */
+typedef struct axg_plaquette_out {
+    int64_t n_pierced[3];   /* faces with n != 0, per orientation */
+    int64_t n_plus[3];      /* faces with n = +1 */
+    int64_t n_minus[3];     /* faces with n = -1 (|n| = 2 is counted in n_pierced only) */
+    int64_t n_quadrant[3];  /* faces passing the quadrant test */
+    int64_t net[3];         /* sum of n */
+} axg_plaquette_out;
+
+int axg_plaquettes(axg_handle* h, axg_plaquette_out* out);
+/* The pierced faces of the last axg_plaquettes, compacted on the device in site order (z, then y,
+ * then x fastest) and face order: 5 int32 per face, (ix, iy, iz) 1-based, o + 1, n.  cap = room in
+ * faces; *count = faces written; rec == NULL or cap == 0: size query. */
+int axg_get_plaquettes(axg_handle* h, int32_t* rec, int64_t cap, int64_t* count);
+
 /* Compacted mask cells in the reference's loop order (z, then y, then x fastest), as
  * 1-based (ix, iy, iz) triplets, the '(3I6)' rows of _mask_<idx>-<rank>.dat
  * (defects_3d.f90:265-273).  cap = room in ixyz (triplets); *count = cells written.  With
@@ -292,6 +316,13 @@ int axg_tune(axg_handle* h, int32_t reps, int32_t* variant, double* ms, int32_t 
  * 0 first drift, 1 kick + GW source + drift, 2 x-transform, 3 y-transform,
  * 4 z-transform + accumulation (nslots <= 5; either pointer may be NULL). */
 int axg_profile(axg_handle* h, int32_t enable, double* ms, int64_t* launches, int32_t nslots);
+
+/* Streaming-bandwidth probe of one device's HBM (no counterpart in the reference; for
+ * measurement).  Allocates two buffers of `bytes` (>= 1 MiB) and times `reps` launches each of
+ * a read-only, a write-only, a copy and a copy with nontemporal hints kernel, 16 bytes per lane;
+ * gbs[kind] = bytes moved / time in GB/s (a copy moves 2 x bytes).  Frees everything before
+ * returning.  Errors: message in axg_last_error(NULL). */
+int axg_hbm_probe(int32_t device, int64_t bytes, int32_t reps, double gbs[4]);
 
 #ifdef __cplusplus
 }

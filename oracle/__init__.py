@@ -48,6 +48,7 @@ def lib():
         L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int, d]
         L.orc_gw_spectrum.argtypes = [p, i64, C.c_int, d, d, d, d, d, p, p, p, p]
         L.orc_classify.argtypes = [p, i64, C.c_int, C.c_int, C.c_int, p, p]
+        L.orc_plaquettes.argtypes = [p, i64, p, p]
         L.orc_conv_spectrum.argtypes = [p, p, i64, C.c_int, d, d, d, d, p, p]
         L.orc_integrate_p_check.restype = d
         L.orc_integrate_p_check.argtypes = [C.c_int, d, d, d, d, d]
@@ -185,6 +186,20 @@ def classify(phi, id_strings=True, id_walls=True, spec_est=True):
     counts = np.zeros(3, np.int64)
     lib().orc_classify(_p(phi), N, int(id_strings), int(id_walls), int(spec_est), _p(flags), _p(counts))
     return flags, counts
+
+
+def plaquettes(phi):
+    """Plaquette windings: (code (N, N, N) uint16, counts dict of (3,) arrays n_pierced, n_plus,
+    n_minus, n_quadrant, net, records (n, 5) int32 (ix, iy, iz, o, n) in site then face order)."""
+    N = phi.shape[0]
+    code = np.zeros((N, N, N), np.uint16)
+    c = np.zeros(16, np.int64)
+    lib().orc_plaquettes(_p(np.ascontiguousarray(phi, np.complex128)), N, code.ctypes.data_as(C.c_void_p), _p(c))
+    counts = {k: c[j::5][:3].copy() for j, k in enumerate(("n_pierced", "n_plus", "n_minus", "n_quadrant", "net"))}
+    n = np.stack([((code.astype(np.int32) >> (3 * o)) & 7) - 2 for o in range(3)], axis=-1)  # (z, y, x, o)
+    zz, yy, xx, oo = np.nonzero(n)
+    rec = np.stack([xx + 1, yy + 1, zz + 1, oo + 1, n[zz, yy, xx, oo]], axis=1).astype(np.int32)
+    return code, counts, rec
 
 
 def conv_spectrum(pi, flags, nb, dspa, dwav, dv_spa, dv_wav):

@@ -478,6 +478,74 @@ void orc_classify(const double* phi, int64_t N, int id_strings, int id_walls, in
     counts[0] = ns; counts[1] = nw; counts[2] = nm;
 }
 
+/* ---------------------------------------------------------- plaquette windings */
+
+/* Plaquette windings (a new diagnostic; the reference's plaquette test is the quadrant count of
+ * velocity_string_YY, defects_3d.f90:850-879).  Face o of site s spans (u, v) = (x, y), (y, z),
+ * (z, x) (ipla = o + 1, the cshift order of defects_3d.f90:854); loop s, s+u, s+u+v, s+v.  Edge
+ * a -> b adds +1 when the shorter arc from arg a to arg b crosses arg = pi counterclockwise (a on
+ * the closed upper side Im >= 0, b below, cross product Re a Im b - Im a Re b > 0), -1 when
+ * clockwise, 0 when the cross product vanishes (antipodal or zero corners). */
+static int arc_cut(oc a, oc b)
+{
+    int ua = !(a.im < 0), ub = !(b.im < 0);
+    double c = a.re * b.im - a.im * b.re;
+    if (ua && !ub && c > 0) return 1;
+    if (!ua && ub && c < 0) return -1;
+    return 0;
+}
+
+/* defects_3d.f90:866-873: quadrant m = ceiling(2 ph / pi) + 2 of ph = aimag(log(z)) */
+static int quadrant_of(oc z)
+{
+    double ph = atan2(z.im, z.re);
+    return (int)ceil(2 * ph / PI_T) + 2;
+}
+
+/* code[site] = sum_o (n_o + 2) << 3o | quad_o << (9 + o); counts[5 o + k] = pierced, +1, -1,
+ * quadrant-test hits, net winding of face o; counts[15] = all pierced faces. */
+void orc_plaquettes(const double* phi, int64_t N, uint16_t* code, int64_t* counts)
+{
+    const oc* f = (const oc*)phi;
+    int64_t c[16] = {0};
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : c[:16])
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++) {
+                int64_t p[3] = {x, y, z};
+                int cd = 0;
+                for (int o = 0; o < 3; o++) {
+                    int u = o, v = (o + 1) % 3;
+                    int64_t q[4][3];
+                    for (int k = 0; k < 4; k++) {
+                        q[k][0] = p[0]; q[k][1] = p[1]; q[k][2] = p[2];
+                    }
+                    q[1][u] = (p[u] + 1) % N;
+                    q[2][u] = (p[u] + 1) % N;
+                    q[2][v] = (p[v] + 1) % N;
+                    q[3][v] = (p[v] + 1) % N;
+                    oc w[4];
+                    for (int k = 0; k < 4; k++) w[k] = f[q[k][0] + N * (q[k][1] + N * q[k][2])];
+                    int n = 0, mask = 0;
+                    for (int k = 0; k < 4; k++) {
+                        n += arc_cut(w[k], w[(k + 1) % 4]);
+                        mask |= 1 << quadrant_of(w[k]);
+                    }
+                    int nq = 0;
+                    for (int m = 1; m <= 4; m++) nq += (mask >> m) & 1;
+                    int quad = nq >= 3;
+                    cd |= (n + 2) << (3 * o);
+                    cd |= quad << (9 + o);
+                    if (n != 0) { c[5 * o] += 1; c[5 * o + 4] += n; c[15] += 1; }
+                    if (n == 1) c[5 * o + 1] += 1;
+                    if (n == -1) c[5 * o + 2] += 1;
+                    c[5 * o + 3] += quad;
+                }
+                code[x + N * (y + N * z)] = (uint16_t)cd;
+            }
+    for (int k = 0; k < 16; k++) counts[k] = c[k];
+}
+
 /* ---------------------------------------------------------- field (axion) spectrum */
 
 /* spectrum.f90:466-567 compute_conv_spec (+ masking 900-954, degrade_spec=0 copy 736-792,

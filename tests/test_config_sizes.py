@@ -5,7 +5,8 @@ Each test compares the HIP path (through the C-ABI) with the CPU FP64 oracle on 
 * 512^3 (L = 128): one FP64 DKD step with GW from a GPU-evolved field -- fields bit-exact,
   accumulators within 1e-10 -- and the cell classification of that field: flags and counts
   bit-exact (integer work; the thresholds at pi and 7 pi/4 depend on device atan2 agreeing with
-  the host's over 134 M cells).
+  the host's over 134 M cells); the plaquette windings of that field, bit-exact, with the
+  winding invariants (zero net winding per orientation, none out of any cell).
 * 128^3: FP64 GW accumulators over two steps (the XCD-strip configuration that 128 and 256 run
   by default) and the strip / dispatcher-order configurations bit-identical at 128 and 256.
 * 128^3 adjoint (L = 32, kappa0 = 1e-6, K = 4) against the oracle's adjoint, and its FP32C
@@ -101,6 +102,32 @@ def test_classify_512_matches_oracle(field512):
     assert cnt[0] > 10000 and cnt[1] > 10000 and cnt[2] > 10000
     assert np.array_equal(cnt, ocnt)
     assert np.array_equal(fl, ofl)
+
+
+def test_plaquettes_512_match_oracle(field512):
+    """Plaquette windings of the evolved 512^3 field: counts and the list of pierced faces equal
+    the oracle's, and the size-independent invariants hold on the GPU's own list: the windings of
+    each orientation sum to 0, and the six faces of every cell carry no net winding."""
+    N = 512
+    phi, pi = field512
+    say("512^3 plaquettes vs oracle")
+    P = phys(N, 0.25)
+    prm = capi.make_params(N, P["dspa"], SQ_HALF, precision=capi.FP64, grav_spec=False)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        cnt = L.plaquettes()
+        rec = L.plaquette_records()
+    code, ocnt, orec = O.plaquettes(phi)
+    say(f"  pierced {cnt['n_pierced'].tolist()} quadrant {cnt['n_quadrant'].tolist()}")
+    for k in ocnt:
+        assert np.array_equal(cnt[k], ocnt[k]), k
+    assert np.array_equal(rec, orec)
+    assert cnt["n_pierced"].min() > 10000 and np.all(cnt["net"] == 0)
+    n = np.zeros((N, N, N, 3), np.int8)
+    n[rec[:, 2] - 1, rec[:, 1] - 1, rec[:, 0] - 1, rec[:, 3] - 1] = rec[:, 4]
+    res = (np.roll(n[..., 0], -1, axis=0) - n[..., 0] + np.roll(n[..., 1], -1, axis=2) - n[..., 1]
+           + np.roll(n[..., 2], -1, axis=1) - n[..., 2])
+    assert not np.any(res)
 
 
 # ---------------------------------------------------------------- 128^3 and 256^3

@@ -671,3 +671,27 @@ def test_advance_get_dtau_interval():
     O.symplectic(phi, pi, OA, 1.0, dt, n, P)
     assert np.array_equal(g_phi, phi) and np.array_equal(g_pi, pi)
     assert relerr(A[0], OA[0][:, :, : N // 2 + 1]) < 1e-10
+
+
+# ---------------------------------------------------------------- plaquette windings
+
+@pytest.mark.parametrize("N,prec,kmax", [(16, capi.FP64, 6.0), (32, capi.FP64, 6.0), (24, capi.FP32C, 4.0),
+                                         (48, capi.FP64, 2.0)])
+def test_plaquettes_bitexact(N, prec, kmax):
+    """Face codes, counts and the compacted list of pierced faces equal the oracle's on the field
+    the GPU holds (integer work: bit-exact)."""
+    say("plaquettes", N, prec)
+    prm, P = params(N, grav_spec=False, precision=prec)
+    phi, pi = ic(N, 19, kmax)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        f_phi, _ = L.get_fields()
+        with pytest.raises(capi.AxgError):
+            L.plaquette_records()  # no plaquettes yet
+        cnt = L.plaquettes()
+        rec = L.plaquette_records()
+    code, ocnt, orec = O.plaquettes(f_phi)
+    for k in ocnt:
+        assert np.array_equal(cnt[k], ocnt[k]), k
+    assert cnt["n_pierced"].sum() > 0 and np.all(cnt["net"] == 0)
+    assert np.array_equal(rec, orec)

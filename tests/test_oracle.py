@@ -980,3 +980,110 @@ def test_maxion2_branches():
         assert abs(O.maxion2(t, kappa) / (k4 * 6.26 * (kappa / t) ** 6.68) - 1) < 1e-14
     # msquared (potential.f90:34-41): zero at T_cr = sqrt 6, -2 at T = 0
     assert abs(O.msquared(2.449489742783178)) < 1e-15 and abs(O.msquared(0.0) + 2) < 1e-15
+
+
+# ------------------------------------------------------------------ plaquette windings
+
+def _windings(code):
+    """(z, y, x, 3) windings n_o from the oracle's face codes."""
+    return np.stack([((code.astype(np.int32) >> (3 * o)) & 7) - 2 for o in range(3)], axis=-1)
+
+
+def test_plaquettes_uniform_phase_is_empty():
+    N = 8
+    phi = np.full((N, N, N), np.exp(0.7j))
+    code, cnt, rec = O.plaquettes(phi)
+    assert len(rec) == 0 and all(np.all(v == 0) for v in cnt.values())
+    assert np.all(_windings(code) == 0)
+
+
+@pytest.mark.parametrize("axis,sign", [(2, 1), (2, -1), (0, 1), (1, -1)])
+def test_plaquettes_straight_string(axis, sign):
+    """A straight string along `axis` through a non-lattice point: phi = (U - u0) + i sign (V - v0)
+    in the plane (U, V) normal to it, with (U, V, axis) cyclic.  Exactly one face per plane
+    perpendicular to the string is pierced, all of the orientation whose normal is `axis`, with
+    winding `sign` (counterclockwise about the axis is +1), and it is the face containing the
+    string; every such face also passes the reference's quadrant test."""
+    N = 12
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    X = [x, y, z]
+    u_ax, v_ax = (axis + 1) % 3, (axis + 2) % 3
+    u0, v0 = 5.3, 6.6
+    phi = np.ascontiguousarray((X[u_ax] - u0) + 1j * sign * (X[v_ax] - v0))
+    code, cnt, rec = O.plaquettes(phi)
+    o_normal = {2: 0, 0: 1, 1: 2}[axis]  # faces (x,y), (y,z), (z,x) have normals z, x, y
+    inner = rec[(np.abs(rec[:, u_ax] - 1 - 5) == 0) & (np.abs(rec[:, v_ax] - 1 - 6) == 0)]
+    assert len(inner) == N and np.all(inner[:, 3] == o_normal + 1) and np.all(inner[:, 4] == sign)
+    # the periodic seams of this non-periodic field carry the compensating windings (net 0)
+    assert np.all(cnt["net"] == 0)
+    n = _windings(code)
+    q = np.stack([(code.astype(np.int32) >> (9 + o)) & 1 for o in range(3)], axis=-1)
+    zz, yy, xx = inner[:, 2] - 1, inner[:, 1] - 1, inner[:, 0] - 1
+    assert np.all(q[zz, yy, xx, o_normal] == 1)
+    assert np.all(n[zz, yy, xx, o_normal] == sign)
+
+
+def _gauss_residual(n):
+    """Windings out of the six faces of every cell: n0(s+z) - n0(s) + n1(s+x) - n1(s) + n2(s+y) - n2(s)."""
+    return (np.roll(n[..., 0], -1, axis=0) - n[..., 0] + np.roll(n[..., 1], -1, axis=2) - n[..., 1]
+            + np.roll(n[..., 2], -1, axis=1) - n[..., 2])
+
+
+def test_plaquettes_small_loop_closes():
+    """A string loop of radius 3.2 in the plane z = z0: phi = (rho - R) + i (z - z0) with rho the
+    distance from the loop's axis (the jump of z - z0 at the periodic seam z = N - 1/2 makes a
+    second loop there).  The pierced faces have normals x or y only (the loops lie in xy
+    planes), lie within 1.5 cells of one of the two circles, the windings of each orientation
+    cancel, and no cell has a net outflow (strings do not end)."""
+    N, R = 16, 3.2
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    cx, cy, cz = 7.4, 7.7, 8.5
+    rho = np.hypot(x - cx, y - cy)
+    phi = np.ascontiguousarray((rho - R) + 1j * (z - cz))
+    code, cnt, rec = O.plaquettes(phi)
+    assert cnt["n_pierced"][0] == 0 and cnt["n_pierced"][1] > 6 and cnt["n_pierced"][2] > 6
+    assert np.all(cnt["net"] == 0)
+    # face centres: o=1 (y,z) at (x, y+1/2, z+1/2), o=2 (z,x) at (x+1/2, y, z+1/2)
+    off = {2: (0.0, 0.5, 0.5), 3: (0.5, 0.0, 0.5)}
+    near = 0
+    for r in rec:
+        dx, dy, dz = off[int(r[3])]
+        px, py, pz = r[0] - 1 + dx, r[1] - 1 + dy, r[2] - 1 + dz
+        assert abs(np.hypot(px - cx, py - cy) - R) < 1.5
+        assert abs(pz - cz) < 1.0 or abs(pz - (N - 0.5)) < 1.0
+        near += abs(pz - cz) < 1.0
+    assert near > 12
+    assert np.all(_gauss_residual(_windings(code)) == 0)
+
+
+def test_plaquettes_random_field_matches_phase_winding():
+    """On a random field the sign-based count equals the winding from the phases themselves:
+    round(sum of the four wrapped differences of np.angle / 2 pi); cells obey Gauss's law; the
+    quadrant test matches a direct restatement of defects_3d.f90:856-877."""
+    N = 12
+    rng = np.random.default_rng(8)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    code, cnt, rec = O.plaquettes(phi)
+    th = np.angle(phi)
+    n = _windings(code)
+    axes = {"x": 2, "y": 1, "z": 0}
+    for o, (u, v) in enumerate((("x", "y"), ("y", "z"), ("z", "x"))):
+        c0 = th
+        c1 = np.roll(th, -1, axis=axes[u])
+        c2 = np.roll(c1, -1, axis=axes[v])
+        c3 = np.roll(th, -1, axis=axes[v])
+        tot = 0
+        for a, b in ((c0, c1), (c1, c2), (c2, c3), (c3, c0)):
+            d = b - a
+            tot = tot + (d - 2 * np.pi * np.round(d / (2 * np.pi)))
+        w = np.round(tot / (2 * np.pi)).astype(int)
+        assert np.array_equal(w, n[..., o]), o
+        quads = [np.ceil(2 * q / PI_T).astype(int) + 2 for q in (c0, c1, c2, c3)]
+        filled = sum(((quads[0] == m) | (quads[1] == m) | (quads[2] == m) | (quads[3] == m)).astype(int)
+                     for m in range(1, 5))
+        assert cnt["n_quadrant"][o] == np.count_nonzero(filled >= 3)
+        assert cnt["n_pierced"][o] == np.count_nonzero(w) > 100
+        assert cnt["n_plus"][o] == np.count_nonzero(w == 1) and cnt["n_minus"][o] == np.count_nonzero(w == -1)
+    assert np.all(cnt["net"] == 0)
+    assert np.all(_gauss_residual(n) == 0)
+    assert len(rec) == sum(cnt["n_pierced"])
