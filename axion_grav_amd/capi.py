@@ -85,7 +85,7 @@ EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_
            "axg_strings", "axg_get_strings", "axg_walls", "axg_get_walls",
            "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
            "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_batch_advance", "axg_energy",
-           "axg_tune", "axg_profile", "axg_hbm_probe", "axg_plaquettes",
+           "axg_energy_fields", "axg_tune", "axg_profile", "axg_hbm_probe", "axg_plaquettes",
            "axg_get_plaquettes")
 
 _LIB = None
@@ -131,6 +131,7 @@ def lib():
             "axg_batch_advance": [p, i32, d, d, i32, i32, p],
             "axg_get_adjoint": [p, p, p, i64],
             "axg_energy": [p, d, p],
+            "axg_energy_fields": [C.POINTER(Params), p, p, d, p, p],
             "axg_hbm_probe": [i32, i64, i32, p],
             "axg_profile": [p, i32, p, p, i32],
             "axg_tune": [p, i32, p, p, i32],
@@ -173,6 +174,15 @@ def batch_advance(lattices, t1, dt, nsteps, chunk=10):
     if rc != 0:
         raise AxgError(f"axg_batch_advance failed ({ERRORS.get(rc, rc)}): {lib().axg_last_error(lattices[0]._h).decode()}")
     return ms.value
+
+
+def energy_fields(params, phi_dev, pi_dev, tau, out_dev, stream=None):
+    """axg_energy_fields on device pointers (ints: FP64 complex N^3 fields, 4-double output);
+    launches on `stream` (a hipStream_t as an int, None = default) without synchronizing."""
+    rc = lib().axg_energy_fields(C.byref(params), C.c_void_p(phi_dev), C.c_void_p(pi_dev), tau, C.c_void_p(out_dev),
+                                 C.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise AxgError(f"axg_energy_fields failed ({ERRORS.get(rc, rc)}): {lib().axg_last_error(None).decode()}")
 
 
 def hbm_probe(nbytes=4 << 30, reps=10, device=0):

@@ -80,7 +80,7 @@ typedef struct axg_handle axg_handle;
 int axg_create(const axg_params* p, axg_handle** out);
 int axg_destroy(axg_handle* h);
 /* Message of the last failed call on h; h == NULL gives the last failure of a call without a
- * handle (axg_create, axg_hbm_probe) on the calling thread. */
+ * handle (axg_create, axg_energy_fields, axg_hbm_probe) on the calling thread. */
 const char* axg_last_error(const axg_handle* h);
 
 /* Copy phi (data1) and pi = phi' (data2) in / out; nsite must be N^3.  In FP32C mode the
@@ -296,6 +296,20 @@ int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, in
 /* Comoving lattice energy at time tau (FP64 reduction): out = {kinetic sum|pi|^2,
  * gradient sum|grad phi|^2, potential tau^2 sigma_cr^2 sum V, total}, each times dspa^3. */
 int axg_energy(axg_handle* h, double tau, double out[4]);
+
+/* axg_energy on fields the caller holds in device memory, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  phi and pi: device pointers to
+ * N^3 FP64 complex (re, im) values, x fastest, N = p->ngrid (2..1024; the physics is read from
+ * p: dspa, sigma_cr, kappa, initial_loop, potential_kappa0, device).  out: device pointer to 4
+ * doubles, written as axg_energy writes them and bit-identical to it on the same FP64 fields.
+ * Launches on `stream` (a hipStream_t; NULL = the default stream) and returns without
+ * synchronizing.  The reduction's partial sums go to a fixed 64 KiB device array of the library
+ * (1024 blocks x 3 partials + 3 sums, whatever N), held by every device it runs on until the
+ * process ends; calls that overlap on one device race on it.  Errors: as the other entry
+ * points, with the message in axg_last_error(NULL). */
+int axg_energy_fields(const axg_params* p, const double* phi, const double* pi, double tau, double* out,
+                      void* stream);
 
 /* Autotuner (no counterpart in the reference).  Times each configuration of the fused
  * kick / GW source / x-transform pass (block rows, sites per thread) and of the y transform
