@@ -43,7 +43,33 @@ struct StepEntry {
     double dtd;       // the drift after the kick
     double tau, dw;   // the GW update's time and weight
 };
-void launch_step_select(hipStream_t s, const StepEntry* tab, int* counter, StepEntry* cur);
+// Where a table-driven step finds its scalars: tab[*base + sidx] (base may be null: 0).  The
+// step kernels of a captured graph take a fixed sidx and a device counter `base` that the graph
+// advances once per replay, so no per-step launch selects the entry.
+struct StepRef {
+    const StepEntry* tab = nullptr;
+    const int* base = nullptr;
+    int sidx = 0;
+};
+__device__ __forceinline__ int step_index(const StepRef& r) { return (r.base ? *r.base : 0) + r.sidx; }
+__device__ __forceinline__ const StepEntry* step_entry(const StepRef& r)
+{
+    return r.tab ? r.tab + step_index(r) : nullptr;
+}
+// One lattice of a stacked launch: lattices of one size and precision advanced by one launch per
+// pass, the lattice index in blockIdx.z.  phi[0] / cphi[0] hold the fields at step 0 of the
+// table, so step s reads phi[s & 1] and writes phi[(s & 1) ^ 1].
+struct LatRef {
+    void* phi[2];
+    void* cphi[2];
+    void* pi;
+    void* cpi;
+    void* W;
+    void* W2;
+    void* A;
+    const StepEntry* tab;
+};
+void launch_add_int(hipStream_t s, int* counter, int delta);
 double tilt_coef(const axg_params& p, double t);
 double maxion2(double thrm, double kappa);
 double msquared(double thrm);
@@ -68,13 +94,14 @@ template <typename T, bool COMP>
 void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi,
                        const void* cphi_in, void* cphi_out, void* cpi, void* src,
                        const KickScalars& k, double dspa, double dt_drift, bool gw,
-                       const StepEntry* se = nullptr);
+                       const StepRef& sr = StepRef{});
 
 // Fused kick + source + x transform + drift for p2_size(N); false (nothing launched) otherwise.
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift, int variant, const StepEntry* se = nullptr);
+                             double dspa, double dt_drift, int variant, const StepRef& sr = StepRef{},
+                             const LatRef* lat = nullptr, int nlat = 1);
 
 template <typename T>
 void launch_gw_xpass(hipStream_t s, int N, int pitch, const void* src, void* W, const FftPlan& plan,
@@ -158,12 +185,12 @@ void launch_adj_dot(hipStream_t s, int64_t n3, const void* phi, const void* cphi
 // into W2 in the tiled layout.
 template <typename T>
 void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
-                     const void* tw, int variant = 0);
+                     const void* tw, int variant = 0, const LatRef* lat = nullptr, int nlat = 1);
 
 template <typename T>
 void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
                          const FftPlan& plan, const void* tw, const GwStepScalars& g,
-                         const StepEntry* se = nullptr);
+                         const StepRef& sr = StepRef{}, const LatRef* lat = nullptr, int nlat = 1);
 
 template <typename T>
 void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int nb,

@@ -174,7 +174,9 @@ struct axg_handle {
     // table-driven steps (axg_batch_advance): per-step scalars, the counter and the current entry
     StepEntry* steptab = nullptr;
     int64_t steptab_cap = 0;
-    StepEntry* stepcur = nullptr;  // [0] the current entry; the counter follows it
+    int* stepbase = nullptr;       // the batch's step counter (on the first handle of a batch)
+    LatRef* latref = nullptr;      // the batch's stacked-launch tables (on the first handle)
+    int64_t latref_cap = 0;
     double adj_scale = 1;          // the adjoint fields are held divided by this
     // spectrum tables (init_spectrum)
     std::vector<double> wav;  // num_bin+1 edges
@@ -672,26 +674,25 @@ static void run_steps(axg_handle* h, double t1, double dt, int32_t nt)
     run_steps_hooked<T, COMP>(h, t1, dt, nt, [](int32_t, double) {});
 }
 
-// One DKD step whose scalars come from the handle's step table (the entry the head kernel
-// selects): the same kernels as run_steps, so the fields match it bit for bit.  Launches only,
-// so it can be captured into a graph; flips h->cur like a step does.
+// One DKD step whose scalars come from the handle's step table, entry *base + sidx: the same
+// kernels as run_steps, so the fields match it bit for bit.  Launches only, so it can be
+// captured into a graph; flips h->cur like a step does.
 template <typename T, bool COMP>
-static void table_step(axg_handle* h)
+static void table_step(axg_handle* h, int sidx, const int* base)
 {
     const int N = h->N;
     const axg_params& p = h->p;
-    StepEntry* cur = h->stepcur;
-    launch_step_select(h->stream, h->steptab, (int*)(cur + 1), cur);
+    const StepRef sr{h->steptab, base, sidx};
     const KickScalars k{};  // replaced by the table's entry in the kernels
     const int nxt = 1 - h->cur;
     bool fused = false;
     if (p.grav_spec && h->fuse_x)
         fused = launch_kick_source_xfft<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
                                                  h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k, p.dspa, 0.0,
-                                                 h->kick_variant, cur);
+                                                 h->kick_variant, sr);
     if (!fused)
         launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur], h->cphi[nxt],
-                                   h->cpi, h->src, k, p.dspa, 0.0, p.grav_spec != 0, cur);
+                                   h->cpi, h->src, k, p.dspa, 0.0, p.grav_spec != 0, sr);
     if (p.grav_spec) {
         if (!fused) launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw);
         launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, h->ypass_variant);
@@ -701,16 +702,45 @@ static void table_step(axg_handle* h)
         g.dwav = p.dwav;
         g.fix_swap = p.gw_fix_swap;
         const void* wz = h->L.zt ? h->W2 : h->W;
-        launch_gw_zpass_acc<T>(h->stream, N, h->P, wz, h->A, h->L, h->plan, h->tw, g, cur);
+        launch_gw_zpass_acc<T>(h->stream, N, h->P, wz, h->A, h->L, h->plan, h->tw, g, sr);
     }
     h->cur = nxt;
 }
 
-static void table_step_any(axg_handle* h)
+static void table_step_any(axg_handle* h, int sidx, const int* base)
 {
-    if (h->comp) table_step<float, true>(h);
-    else if (h->fp32) table_step<float, false>(h);
-    else table_step<double, false>(h);
+    if (h->comp) table_step<float, true>(h, sidx, base);
+    else if (h->fp32) table_step<float, false>(h, sidx, base);
+    else table_step<double, false>(h, sidx, base);
+}
+
+// One DKD step of a stack of lattices that share N, precision, dspa, dwav and the index order of
+// the GW update, in the pass configurations of h (the first of them): each pass is one launch
+// over all of them, the lattice in blockIdx.z, each reading its buffers and its step entry *base + sidx from
+// its LatRef.  Launches on h->stream; the handles' cur is left alone (the caller flips it).
+template <typename T, bool COMP>
+static void stacked_step(axg_handle* h, const LatRef* lat, int nlat, int sidx, const int* base)
+{
+    const axg_params& p = h->p;
+    const StepRef sr{nullptr, base, sidx};
+    const KickScalars k{};
+    if (!launch_kick_source_xfft<T, COMP>(h->stream, h->N, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                          nullptr, h->P, h->tw, k, p.dspa, 0.0, h->kick_variant, sr, lat, nlat))
+        throw Error(AXG_EINVAL, "stacked step: no fused pass for this size");
+    launch_gw_ypass<T>(h->stream, h->N, h->P, nullptr, nullptr, h->L, h->plan, h->tw, h->ypass_variant, lat, nlat);
+    GwStepScalars g;
+    g.tau = 0;
+    g.dw = 0;
+    g.dwav = p.dwav;
+    g.fix_swap = p.gw_fix_swap;
+    launch_gw_zpass_acc<T>(h->stream, h->N, h->P, nullptr, nullptr, h->L, h->plan, h->tw, g, sr, lat, nlat);
+}
+
+static void stacked_step_any(axg_handle* h, const LatRef* lat, int nlat, int sidx, const int* base)
+{
+    if (h->comp) stacked_step<float, true>(h, lat, nlat, sidx, base);
+    else if (h->fp32) stacked_step<float, false>(h, lat, nlat, sidx, base);
+    else stacked_step<double, false>(h, lat, nlat, sidx, base);
 }
 
 static void advance(axg_handle* h, double t1, double dt, int32_t nt)
@@ -1664,13 +1694,102 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
                 h->steptab = (StepEntry*)h->alloc((size_t)nsteps * sizeof(StepEntry));
                 h->steptab_cap = nsteps;
             }
-            if (!h->stepcur) h->stepcur = (StepEntry*)h->alloc(2 * sizeof(StepEntry));
             AXG_HIP(hipMemcpyAsync(h->steptab, tab.data(), (size_t)nsteps * sizeof(StepEntry), hipMemcpyHostToDevice,
                                    h->stream));
-            const int zero = 0;
-            AXG_HIP(hipMemcpyAsync(h->stepcur + 1, &zero, sizeof zero, hipMemcpyHostToDevice, h->stream));
             AXG_HIP(hipStreamSynchronize(h->stream));
         }
+        // units of work: a stack of lattices that share N, precision, dspa, dwav and the GW index
+        // order (one launch per pass over all of them, in the pass configurations of the stack's
+        // first lattice: every configuration gives the same bits), or one lattice the stacked
+        // kernels do not cover (no GW, no fused pass, runtime-plan sizes).  AXG_BATCH_STACK=0 makes
+        // every lattice its own unit (same results).
+        bool stack = true;
+        if (const char* e = getenv("AXG_BATCH_STACK")) stack = strcmp(e, "0") != 0;
+        // a stack is closed before its step working set (fields, source transforms, accumulators)
+        // passes 128 MiB, so a stack's passes hand their arrays to each other through the 256 MB
+        // last-level cache (measured on 16 x 64^3 + 16 x 96^3 + 16 x 128^3, 600 steps: 1.38 s at
+        // 96-128 MiB, 1.44-1.45 s at 32-64 MiB, 1.46 s unlimited, 1.51 s one by one); tuning knob
+        // AXG_BATCH_STACK_MB (0 = no limit)
+        double stack_mb = 128;
+        if (const char* e = getenv("AXG_BATCH_STACK_MB")) stack_mb = atof(e);
+        auto step_mb = [](const axg_handle* h) {
+            const double fields = (double)h->n3 * (3 * (double)h->t2 + (h->comp ? 12.0 : 0.0));
+            const double gw = h->A ? ((double)6 * h->N * h->N * h->P + 10.0 * h->L.comp()) * (double)h->t2 : 0.0;
+            return (fields + gw) / (1 << 20);
+        };
+        std::vector<double> unit_mb;
+        struct Unit {
+            std::vector<int> idx;
+            bool stacked = false;
+            int off = 0;  // first LatRef of a stacked unit
+        };
+        std::vector<Unit> units;
+        for (int i = 0; i < n; i++) {
+            axg_handle* h = hs[i];
+            const bool can = stack && h->p.grav_spec && h->fuse_x && p2_size(h->N) && h->L.zt > 0;
+            Unit* u = nullptr;
+            const double mb = step_mb(h);
+            if (can) {
+                for (size_t v = 0; v < units.size(); v++) {
+                    if (!units[v].stacked) continue;
+                    const axg_handle* g = hs[units[v].idx[0]];
+                    if (g->N == h->N && g->prec == h->prec && g->p.dspa == h->p.dspa && g->p.dwav == h->p.dwav &&
+                        g->p.gw_fix_swap == h->p.gw_fix_swap && (stack_mb <= 0 || unit_mb[v] + mb <= stack_mb)) {
+                        u = &units[v];
+                        unit_mb[v] += mb;
+                        break;
+                    }
+                }
+            }
+            if (!u) {
+                units.emplace_back();
+                unit_mb.push_back(mb);
+                u = &units.back();
+                u->stacked = can;
+            }
+            u->idx.push_back(i);
+        }
+        std::vector<LatRef> refs;
+        for (auto& u : units) {
+            if (!u.stacked) continue;
+            u.off = (int)refs.size();
+            for (int i : u.idx) {
+                const axg_handle* h = hs[i];
+                LatRef r{};
+                r.phi[0] = h->phi[h->cur];
+                r.phi[1] = h->phi[1 - h->cur];
+                r.cphi[0] = h->cphi[h->cur];
+                r.cphi[1] = h->cphi[1 - h->cur];
+                r.pi = h->pi;
+                r.cpi = h->cpi;
+                r.W = h->W;
+                r.W2 = h->W2;
+                r.A = h->A;
+                r.tab = h->steptab;
+                refs.push_back(r);
+            }
+        }
+        if ((int64_t)refs.size() > h0->latref_cap) {
+            void* q = h0->latref;
+            h0->release_one(q);
+            h0->latref = nullptr;
+            h0->latref_cap = 0;
+            h0->latref = (LatRef*)h0->alloc(refs.size() * sizeof(LatRef));
+            h0->latref_cap = (int64_t)refs.size();
+        }
+        if (!h0->stepbase) h0->stepbase = (int*)h0->alloc(sizeof(int));
+        if (!refs.empty())
+            AXG_HIP(hipMemcpyAsync(h0->latref, refs.data(), refs.size() * sizeof(LatRef), hipMemcpyHostToDevice,
+                                   h0->stream));
+        const int zero = 0;
+        AXG_HIP(hipMemcpyAsync(h0->stepbase, &zero, sizeof zero, hipMemcpyHostToDevice, h0->stream));
+        AXG_HIP(hipStreamSynchronize(h0->stream));
+        int* base = h0->stepbase;
+        auto step_unit = [&](const Unit& u, int sidx) {
+            if (u.stacked) stacked_step_any(hs[u.idx[0]], h0->latref + u.off, (int)u.idx.size(), sidx, base);
+            else table_step_any(hs[u.idx[0]], sidx, base);
+        };
+        const int nu = (int)units.size();
         const auto c0 = std::chrono::steady_clock::now();
         // first half drifts
         for (int i = 0; i < n; i++) {
@@ -1682,17 +1801,16 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
             else launch_drift<double, false>(h->stream, h->N, h->phi[h->cur], h->phi[h->cur], h->pi, nullptr,
                                              nullptr, nullptr, dt / 2);
         }
+        for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
         int32_t done = 0;
         if (chunk > 0 && nsteps >= chunk) {
-            // one graph of `chunk` steps of every lattice, replayed; an even chunk returns the
-            // ping-pong buffers to where the capture found them.  The lattices are dealt to S
-            // lanes (the streams of the first S handles, forked from and joined into the first):
-            // lattice i runs its chunk on lane i % S after the lane's previous lattice.  Two lanes
-            // by default (measured on 16 x 64^3 + 16 x 96^3 + 16 x 128^3: 2 lanes 1.44 s, 4 to 48
-            // lanes 1.49-1.53 s, 1 lane 1.58 s); tuning knob AXG_BATCH_LANES.
-            int S = std::min(n, 2);
-            if (const char* e = getenv("AXG_BATCH_LANES")) S = std::max(1, std::min(n, atoi(e)));
-            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+            // one graph of `chunk` steps of every unit, replayed; the step counter advances by chunk
+            // at its end, and an even chunk returns the ping-pong buffers to where the capture found
+            // them.  The units are dealt to S lanes (the streams of the first S handles, forked from
+            // and joined into the first): unit u runs on lane u % S after the lane's previous unit.
+            // Two lanes by default; tuning knob AXG_BATCH_LANES.
+            int S = std::min(nu, 2);
+            if (const char* e = getenv("AXG_BATCH_LANES")) S = std::max(1, std::min(nu, atoi(e)));
             hipStream_t s0 = h0->stream;
             EventSet ev((size_t)n + 1);
             GraphPair gp;
@@ -1702,10 +1820,11 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
             AXG_HIP(hipEventRecord(ev.e[n], s0));
             for (int l = 1; l < S; l++) AXG_HIP(hipStreamWaitEvent(own[l], ev.e[n], 0));
             try {
-                for (int i = 0; i < n; i++) {
-                    hs[i]->stream = own[i % S];
-                    for (int32_t c = 0; c < chunk; c++) table_step_any(hs[i]);
-                    hs[i]->stream = own[i];
+                for (int u = 0; u < nu; u++) {
+                    axg_handle* h = hs[units[u].idx[0]];
+                    h->stream = own[u % S];
+                    for (int32_t c = 0; c < chunk; c++) step_unit(units[u], c);
+                    h->stream = own[units[u].idx[0]];
                 }
             } catch (...) {
                 for (int i = 0; i < n; i++) hs[i]->stream = own[i];
@@ -1715,19 +1834,23 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
                 AXG_HIP(hipEventRecord(ev.e[l], own[l]));
                 AXG_HIP(hipStreamWaitEvent(s0, ev.e[l], 0));
             }
+            launch_add_int(s0, base, chunk);
             AXG_HIP(hipStreamEndCapture(s0, &gp.g));
             AXG_HIP(hipGraphInstantiate(&gp.x, gp.g, nullptr, nullptr, 0));
             for (; done + chunk <= nsteps; done += chunk) AXG_HIP(hipGraphLaunch(gp.x, s0));
             AXG_HIP(hipEventRecord(ev.e[0], s0));
             for (int i = 1; i < n; i++) AXG_HIP(hipStreamWaitEvent(hs[i]->stream, ev.e[0], 0));
-            for (int i = 0; i < n; i++)
-                for (int32_t s = done; s < nsteps; s++) table_step_any(hs[i]);
-            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+            // the steps left over: entries *base + s = done + s
+            for (int32_t s = 0; s < nsteps - done; s++)
+                for (const Unit& u : units) step_unit(u, s);
         } else {
             for (int32_t s = 0; s < nsteps; s++)
-                for (int i = 0; i < n; i++) table_step_any(hs[i]);
-            for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+                for (const Unit& u : units) step_unit(u, s);
         }
+        for (int i = 0; i < n; i++) AXG_HIP(hipStreamSynchronize(hs[i]->stream));
+        for (const Unit& u : units)
+            if (u.stacked)
+                for (int i : u.idx) hs[i]->cur ^= (nsteps & 1);
         const auto c1 = std::chrono::steady_clock::now();
         for (int i = 0; i < n; i++) {
             axg_handle* h = hs[i];

@@ -116,9 +116,15 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
 template <typename T, int N, int TX, int KZ>
 __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                     typename V2<T>::type* __restrict__ W2,
-                                                                    const typename V2<T>::type* __restrict__ tw)
+                                                                    const typename V2<T>::type* __restrict__ tw,
+                                                                    const LatRef* __restrict__ lat)
 {
     using T2 = typename V2<T>::type;
+    if (lat) {
+        // stacked launch: this block's lattice in blockIdx.z
+        W = (const T2*)lat[blockIdx.z].W;
+        W2 = (T2*)lat[blockIdx.z].W2;
+    }
     constexpr int TPT = N / 8, NH = N / 2 + 1, P = half_pitch_c<N>();
     using L = TiledLayout<N, p2_ztx<T, N>()>;
     extern __shared__ unsigned char smem_raw[];
@@ -156,11 +162,17 @@ template <typename T, int N>
 __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                            typename V2<T>::type* __restrict__ A,
                                                                            const typename V2<T>::type* __restrict__ tw,
-                                                                           GwStepScalars g,
-                                                                           const StepEntry* __restrict__ se)
+                                                                           GwStepScalars g, StepRef sr,
+                                                                           const LatRef* __restrict__ lat)
 {
 #pragma clang fp contract(off)
-    if (se) {
+    if (lat) {
+        // stacked launch: this block's lattice in blockIdx.z
+        W = (const typename V2<T>::type*)lat[blockIdx.z].W2;
+        A = (typename V2<T>::type*)lat[blockIdx.z].A;
+        sr.tab = lat[blockIdx.z].tab;
+    }
+    if (const StepEntry* se = step_entry(sr)) {
         g.tau = se->tau;
         g.dw = se->dw;
     }
@@ -519,10 +531,10 @@ template <typename T>
 __global__ void __launch_bounds__(ZNT) gw_zpass_acc_kernel(int N, int P, const typename V2<T>::type* __restrict__ W,
                                                           typename V2<T>::type* __restrict__ A, FftPlan plan,
                                                           const typename V2<T>::type* __restrict__ tw,
-                                                          GwStepScalars g, int TX, const StepEntry* __restrict__ se)
+                                                          GwStepScalars g, int TX, StepRef sr)
 {
 #pragma clang fp contract(off)
-    if (se) {
+    if (const StepEntry* se = step_entry(sr)) {
         g.tau = se->tau;
         g.dw = se->dw;
     }
@@ -641,9 +653,10 @@ GwLayout gw_layout(int N, bool fp32)
 
 template <typename T>
 void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const GwLayout& L, const FftPlan& plan,
-                     const void* tw, int variant)
+                     const void* tw, int variant, const LatRef* lat, int nlat)
 {
     using T2 = typename V2<T>::type;
+    if (lat && !L.zt) throw Error(AXG_EINVAL, "stacked y pass needs the tiled layout");
     if (L.zt) {
         bool done = p2_dispatch(N, [&]<int NN>() {
             if (!TiledLayout<NN, p2_ztx<T, NN>()>::matches(L) || pitch != half_pitch_c<NN>())
@@ -652,8 +665,8 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
             // z planes per block
             auto go = [&]<int TXP, int KZ>() {
                 hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP, KZ>),
-                                   dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ), dim3(TXP * (NN / 8)),
-                                   (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw);
+                                   dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ, nlat), dim3(TXP * (NN / 8)),
+                                   (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
             };
             constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1;
             if (variant == 1) go.template operator()<TXH, 1>();
@@ -669,15 +682,17 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
 
 template <typename T>
 void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A, const GwLayout& L,
-                         const FftPlan& plan, const void* tw, const GwStepScalars& g, const StepEntry* se)
+                         const FftPlan& plan, const void* tw, const GwStepScalars& g, const StepRef& sr,
+                         const LatRef* lat, int nlat)
 {
     using T2 = typename V2<T>::type;
+    if (lat && !L.zt) throw Error(AXG_EINVAL, "stacked z pass needs the tiled layout");
     bool done = L.zt && p2_dispatch(N, [&]<int NN>() {
         constexpr int TXP = p2_ztx<T, NN>();
         if (!TiledLayout<NN, TXP>::matches(L))
             throw Error(AXG_EINVAL, "accumulation layout does not match the z pass tile");
-        hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3(L.ntile, NN), dim3(TXP * (NN / 8)),
-                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, (const T2*)tw, g, se);
+        hipLaunchKernelGGL((gw_zpass_p2_kernel<T, NN>), dim3(L.ntile, NN, nlat), dim3(TXP * (NN / 8)),
+                           (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)A, (const T2*)tw, g, sr, lat);
     });
     if (done) { AXG_HIP(hipGetLastError()); return; }
     int TX = col_width<T>(N);
@@ -686,7 +701,7 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
     dim3 grid((nh + TX - 1) / TX, N);
     size_t lds = (size_t)N * TX * sizeof(T2);
     hipLaunchKernelGGL((gw_zpass_acc_kernel<T>), grid, dim3(ZNT), lds, s, N, pitch, (const T2*)W, (T2*)A, plan,
-                       (const T2*)tw, g, TX, se);
+                       (const T2*)tw, g, TX, sr);
     AXG_HIP(hipGetLastError());
 }
 
@@ -804,9 +819,10 @@ void launch_gw_pack(hipStream_t s, int N, const GwLayout& L, void* A, int m, dou
                                          const void*);                                                          \
     template void launch_fft_rows<T, +1>(hipStream_t, int, void*, int64_t, const FftPlan&, const void*);        \
     template void launch_gw_ypass<T>(hipStream_t, int, int, void*, void*, const GwLayout&, const FftPlan&,       \
-                                     const void*, int);                                                              \
+                                     const void*, int, const LatRef*, int);                                     \
     template void launch_gw_zpass_acc<T>(hipStream_t, int, int, const void*, void*, const GwLayout&,             \
-                                         const FftPlan&, const void*, const GwStepScalars&, const StepEntry*);  \
+                                         const FftPlan&, const void*, const GwStepScalars&, const StepRef&,     \
+                                         const LatRef*, int);                                                   \
     template void launch_gw_bin<T>(hipStream_t, int, const GwLayout&, const void*, const uint16_t*, int, double*, \
                                    int, double*);                                                               \
     template void launch_gw_pack<T>(hipStream_t, int, const GwLayout&, void*, int, double2*, int);

@@ -55,10 +55,10 @@ __global__ void kick_drift_kernel(int N, const typename V2<T>::type* __restrict_
                                   typename V2<T>::type* __restrict__ phi_out, typename V2<T>::type* __restrict__ pi,
                                   const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
                                   short2* __restrict__ cpi, typename V2<T>::type* __restrict__ src, KickScalars k,
-                                  double d, double dt_drift, int gw, const StepEntry* __restrict__ se)
+                                  double d, double dt_drift, int gw, StepRef sr)
 {
 #pragma clang fp contract(off)
-    if (se) {
+    if (const StepEntry* se = step_entry(sr)) {
         k = se->k;
         dt_drift = se->dtd;
     }
@@ -116,34 +116,30 @@ __global__ void kick_drift_kernel(int N, const typename V2<T>::type* __restrict_
     stf<T, COMP>(phi_out, cphi_out, i, f);
 }
 
-// cur = tab[*counter]; ++*counter (one thread, vector stores): the head of a table-driven step.
-__global__ void step_select_kernel(const StepEntry* __restrict__ tab, int* __restrict__ counter,
-                                   StepEntry* __restrict__ cur)
+// *counter += delta (one thread, a vector store): advances a captured graph's step counter once
+// per replay.
+__global__ void add_int_kernel(int* __restrict__ counter, int delta)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const int c = *counter;
-        *cur = tab[c];
-        *counter = c + 1;
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = *counter + delta;
 }
 
-void launch_step_select(hipStream_t s, const StepEntry* tab, int* counter, StepEntry* cur)
+void launch_add_int(hipStream_t s, int* counter, int delta)
 {
-    hipLaunchKernelGGL(step_select_kernel, dim3(1), dim3(64), 0, s, tab, counter, cur);
+    hipLaunchKernelGGL(add_int_kernel, dim3(1), dim3(64), 0, s, counter, delta);
     AXG_HIP(hipGetLastError());
 }
 
 template <typename T, bool COMP>
 void launch_kick_drift(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                        void* cphi_out, void* cpi, void* src, const KickScalars& k, double dspa, double dt_drift,
-                       bool gw, const StepEntry* se)
+                       bool gw, const StepRef& sr)
 {
     using T2 = typename V2<T>::type;
     int bx = N < 256 ? N : 256;
     dim3 grid((N + bx - 1) / bx, N, N);
     hipLaunchKernelGGL((kick_drift_kernel<T, COMP>), grid, dim3(bx), 0, s, N, (const T2*)phi_in, (T2*)phi_out,
                        (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)src, k, dspa * 2,
-                       dt_drift, gw ? 1 : 0, se);
+                       dt_drift, gw ? 1 : 0, sr);
     AXG_HIP(hipGetLastError());
 }
 
@@ -253,14 +249,27 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, int P, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double d2, int div, double dt_drift, const StepEntry* __restrict__ se)
+    KickScalars k, double inv_d, double d2, int div, double dt_drift, StepRef sr, const LatRef* __restrict__ lat)
 {
 #pragma clang fp contract(off)
-    if (se) {
+    using T2 = typename V2<T>::type;
+    if (lat) {
+        // stacked launch: this block's lattice, its buffers for this step and its scalars
+        const LatRef& L = lat[blockIdx.z];
+        const int st = step_index(sr) & 1;
+        phi_in = (const T2*)L.phi[st];
+        phi_out = (T2*)L.phi[st ^ 1];
+        cphi_in = (const short2*)L.cphi[st];
+        cphi_out = (short2*)L.cphi[st ^ 1];
+        pi = (T2*)L.pi;
+        cpi = (short2*)L.cpi;
+        W = (T2*)L.W;
+        sr.tab = L.tab;
+    }
+    if (const StepEntry* se = step_entry(sr)) {
         k = se->k;
         dt_drift = se->dtd;
     }
-    using T2 = typename V2<T>::type;
     constexpr int TPT = N / 8, NR = 3 * XR;  // NR rows to transform
     constexpr int BS = XR * N;                 // threads per block
     constexpr int NG = N / XR;                 // row groups per plane
@@ -406,14 +415,27 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
     short2* __restrict__ cpi, typename V2<T>::type* __restrict__ W, const typename V2<T>::type* __restrict__ tw,
-    KickScalars k, double inv_d, double d2, int div, double dt_drift, const StepEntry* __restrict__ se)
+    KickScalars k, double inv_d, double d2, int div, double dt_drift, StepRef sr, const LatRef* __restrict__ lat)
 {
 #pragma clang fp contract(off)
-    if (se) {
+    using T2 = typename V2<T>::type;
+    if (lat) {
+        // stacked launch: this block's lattice, its buffers for this step and its scalars
+        const LatRef& L = lat[blockIdx.z];
+        const int st = step_index(sr) & 1;
+        phi_in = (const T2*)L.phi[st];
+        phi_out = (T2*)L.phi[st ^ 1];
+        cphi_in = (const short2*)L.cphi[st];
+        cphi_out = (short2*)L.cphi[st ^ 1];
+        pi = (T2*)L.pi;
+        cpi = (short2*)L.cpi;
+        W = (T2*)L.W;
+        sr.tab = L.tab;
+    }
+    if (const StepEntry* se = step_entry(sr)) {
         k = se->k;
         dt_drift = se->dtd;
     }
-    using T2 = typename V2<T>::type;
     constexpr int TPT = N / 8, NR = 3 * XR, BS = XR * N / 2, H = N / 2;
     extern __shared__ unsigned char smem_raw[];
     T2* s = reinterpret_cast<T2*>(smem_raw);  // [m][rr][x]
@@ -502,7 +524,8 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
 template <typename T, bool COMP>
 bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi_out, void* pi, const void* cphi_in,
                              void* cphi_out, void* cpi, void* W, int pitch, const void* tw, const KickScalars& k,
-                             double dspa, double dt_drift, int variant, const StepEntry* se)
+                             double dspa, double dt_drift, int variant, const StepRef& sr, const LatRef* lat,
+                             int nlat)
 {
     using T2 = typename V2<T>::type;
     if (pitch != half_pitch(N)) throw Error(AXG_EINVAL, "fused pass: unexpected half-spectrum pitch");
@@ -514,10 +537,10 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     // the rows: 1024 threads), bit 1 = XCD remap by planes (one site per thread only), bit 2 =
     // swap the sites-per-thread choice, bit 3 = XCD strips
     auto launch = [&]<int NN, int XR, int RM>() {
-        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN), dim3(XR * NN),
+        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN, nlat), dim3(XR * NN),
                            (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
                            (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
-                           1.0 / d2, d2, div, dt_drift, se);
+                           1.0 / d2, d2, div, dt_drift, sr, lat);
     };
     auto go = [&]<int NN>() {
         constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
@@ -527,10 +550,10 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
             const bool two = (sizeof(T) == 4) != ((variant & 4) != 0);
             if (two) {
                 auto launch2 = [&]<int X2, bool ST>() {
-                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN),
+                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN, nlat),
                                        dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
                                        (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
-                                       (T2*)W, (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, se);
+                                       (T2*)W, (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, sr, lat);
                 };
                 // wide: twice the rows when they divide N and the tile stays within 64 KiB of LDS
                 constexpr int X2W =
@@ -585,10 +608,10 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
                                      const void*, double);                                                   \
     template void launch_kick_drift<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*,    \
                                           void*, void*, const KickScalars&, double, double, bool,        \
-                                          const StepEntry*);                                              \
+                                          const StepRef&);                                                \
     template bool launch_kick_source_xfft<T, C>(hipStream_t, int, const void*, void*, void*, const void*, void*, \
                                                 void*, void*, int, const void*, const KickScalars&, double, double, \
-                                                int, const StepEntry*);
+                                                int, const StepRef&, const LatRef*, int);
 AXG_INST_STEP(double, false)
 AXG_INST_STEP(float, false)
 AXG_INST_STEP(float, true)

@@ -288,10 +288,17 @@ def run_batch(capi, steps=600, chunk=10, t_end=16.0):
         capi.batch_advance(lats, 1.0, dt, 4, chunk)  # warm-up (and graph code paths)
         reset()
         t_graph = capi.batch_advance(lats, 1.0, dt, steps, chunk)
-        spec = {}
+        # per realization: the 64-bin GW spectrum and the string / wall / mask cell counts; then
+        # their means per size
+        spec, cells = {}, {}
+        per = []
+        for lat, n in zip(lats, sizes):
+            pw = lat.gw_spectrum()["power"]
+            c = lat.defects()
+            per.append((n, pw, c))
         for n in (64, 96, 128):
-            sp = [lat.gw_spectrum()["power"] for lat, m in zip(lats, sizes) if m == n]
-            spec[n] = np.mean(sp, axis=0)
+            spec[n] = np.mean([pw for m, pw, _ in per if m == n], axis=0)
+            cells[n] = np.mean([c for m, _, c in per if m == n], axis=0)
         reset()
         t_direct = capi.batch_advance(lats, 1.0, dt, steps, 0)
         reset()
@@ -306,7 +313,10 @@ def run_batch(capi, steps=600, chunk=10, t_end=16.0):
                 graph_chunk=chunk, graph_ms=round(t_graph, 1), direct_ms=round(t_direct, 1),
                 one_by_one_ms=round(t_seq, 1), su_s_graph=su / (t_graph * 1e-3),
                 su_s_one_by_one=su / (t_seq * 1e-3), speedup_graph_vs_one_by_one=round(t_seq / t_graph, 3),
-                size_mean_gw_power_finite=bool(all(np.all(np.isfinite(v)) for v in spec.values())))
+                size_mean_gw_power_finite=bool(all(np.all(np.isfinite(v)) for v in spec.values())),
+                size_mean_cells={str(n): [round(float(x), 1) for x in cells[n]] for n in cells},
+                size_mean_gw_power={str(n): [float(f"{x:.4g}") for x in spec[n]] for n in spec},
+                realization_string_cells=[int(c[0]) for _, _, c in per])
 
 
 def run_largest(capi, N=1024, L=256.0, steps=10, warmup=2, kappa0=1e-6, seed=2024):

@@ -284,12 +284,16 @@ int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite)
 /* Batches of lattices (no counterpart in the reference; the reference runs one lattice per
  * job).  Advances n independent lattices -- any sizes and precisions, one device -- by nsteps
  * DKD steps of dt from t1, each exactly as axg_advance_steps would (bit-identical fields and
- * accumulators).  Every step's scalars go to device tables first, so one HIP graph of `chunk`
- * steps of all the lattices (dealt to two streams, forked from and joined into the first
+ * accumulators).  Every step's scalars go to a device table per lattice first.  Lattices that
+ * share N, precision, dspa, dwav and the GW index order, with the fused pass, are stacked: each
+ * pass of a step is one launch over all of them (the lattice index in the grid, each reading its
+ * own buffers and table entry).  The others step one by one.  One HIP graph of `chunk` steps of
+ * all the stacks and lattices (dealt to two streams forked from and joined into the first
  * handle's; environment variable AXG_BATCH_LANES sets another count) is captured once and
- * replayed; the steps left over, and chunk = 0, launch the same kernels directly (each lattice
- * on its own stream).  chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of
- * the steps.  Errors are reported on hs[0]. */
+ * replayed, a device step counter advancing once per replay; the steps left over, and chunk = 0,
+ * launch the same kernels directly.  AXG_BATCH_STACK=0 turns stacking off (same results).
+ * chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of the steps.  Errors
+ * are reported on hs[0]. */
 int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, int32_t nsteps, int32_t chunk,
                       double* ms);
 

@@ -11,7 +11,7 @@ from tests.conftest import SQ_HALF, phys
 pytestmark = pytest.mark.gpu
 
 CASES = [(16, capi.FP64), (24, capi.FP32C), (32, capi.FP32C), (64, capi.FP32), (32, capi.FP64), (36, capi.FP32C),
-         (48, capi.FP64)]
+         (48, capi.FP64), (32, capi.FP32C), (64, capi.FP32)]
 
 
 def make(N, prec, seed):
@@ -24,13 +24,17 @@ def make(N, prec, seed):
     return prm, phi, pi
 
 
-@pytest.mark.parametrize("nsteps,chunk,lanes", [(7, 2, None), (8, 4, None), (5, 0, None), (3, 4, None),
-                                                (6, 2, "1"), (6, 2, "3"), (4, 4, "64")])
-def test_batch_matches_single_lattices(nsteps, chunk, lanes, monkeypatch):
+@pytest.mark.parametrize("nsteps,chunk,lanes,stack", [(7, 2, None, "1"), (8, 4, None, "1"), (5, 0, None, "1"),
+                                                      (3, 4, None, "1"), (6, 2, "1", "1"), (6, 2, "3", "1"),
+                                                      (4, 4, "64", "1"), (7, 2, None, "0"), (5, 0, None, "0")])
+def test_batch_matches_single_lattices(nsteps, chunk, lanes, stack, monkeypatch):
     """Odd step counts leave a tail of direct steps after the graph replays; chunk > nsteps runs
-    no graph at all; chunk = 0 launches every step directly.  AXG_BATCH_LANES deals the lattices
-    to 1, 3 or (clamped) all streams in the graph instead of the default two."""
-    print(f"[case] batch nsteps={nsteps} chunk={chunk} lanes={lanes}", flush=True)
+    no graph at all; chunk = 0 launches every step directly.  AXG_BATCH_LANES deals the units
+    to 1, 3 or (clamped) all streams in the graph instead of the default two.  The lattices of
+    the same size and precision (two 32^3 FP32C, with and without the tilt) form a stack, the
+    others step alone; AXG_BATCH_STACK=0 steps every lattice alone."""
+    print(f"[case] batch nsteps={nsteps} chunk={chunk} lanes={lanes} stack={stack}", flush=True)
+    monkeypatch.setenv("AXG_BATCH_STACK", stack)
     if lanes is not None:
         monkeypatch.setenv("AXG_BATCH_LANES", lanes)
     t1, dt = 1.0, 0.05
@@ -74,7 +78,7 @@ def test_batch_mixed_workload_sizes():
     128^3, FP32C with GW (the kernels the batch benchmark times), bit-identical to the same
     lattices advanced one by one, and their size-grouped mean GW spectra equal."""
     print("[case] batch 64/96/128", flush=True)
-    t1, dt, nsteps = 1.0, 0.025, 6
+    t1, dt, nsteps = 1.0, 0.025, 7
     lats, ref = [], []
     try:
         for i, N in enumerate((64, 96, 128, 64, 96, 128)):
