@@ -86,7 +86,7 @@ EXPORTS = ("axg_derive", "axg_get_dtau", "axg_create", "axg_destroy", "axg_last_
            "axg_group_strings", "axg_group_walls", "axg_get_groups", "axg_field_spectrum",
            "axg_field_covariance", "axg_gradient", "axg_get_adjoint", "axg_batch_advance", "axg_energy",
            "axg_energy_fields", "axg_tune", "axg_profile", "axg_hbm_probe", "axg_plaquettes",
-           "axg_get_plaquettes")
+           "axg_get_plaquettes", "axg_set_stream", "axg_get_stream")
 
 _LIB = None
 
@@ -117,6 +117,8 @@ def lib():
             "axg_get_mask": [p, p, i64, p],
             "axg_get_flags": [p, p, i64],
             "axg_plaquettes": [p, C.POINTER(PlaquetteOut)],
+            "axg_set_stream": [p, C.c_uint32, C.c_uint32, C.c_uint64],
+            "axg_get_stream": [p, p],
             "axg_get_plaquettes": [p, p, i64, p],
             "axg_strings": [p, C.c_uint32, C.c_uint32, C.POINTER(StringOut)],
             "axg_get_strings": [p, p, i64, p],
@@ -285,6 +287,16 @@ class Lattice:
         pi = np.empty((self.N,) * 3, np.complex128)
         self._check(lib().axg_get_fields(self._h, _ptr(phi), _ptr(pi), self.n3), "axg_get_fields")
         return phi, pi
+
+    def set_stream(self, seed1, seed2, position=0):
+        """Take the handle's random draws from the reference's MCG59 stream seeded by (seed1, seed2)
+        (its key ss), positioned at `position` draws."""
+        self._check(lib().axg_set_stream(self._h, seed1, seed2, int(position)), "axg_set_stream")
+
+    def stream_position(self):
+        pos = C.c_uint64()
+        self._check(lib().axg_get_stream(self._h, C.byref(pos)), "axg_get_stream")
+        return pos.value
 
     def init_flat(self, seed, kmax):
         self._check(lib().axg_init_flat(self._h, int(seed), float(kmax)), "axg_init_flat")

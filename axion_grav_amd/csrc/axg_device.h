@@ -276,3 +276,45 @@ __device__ __forceinline__ void unit_gauss(uint64_t seed, uint32_t tag, uint32_t
     re = r * cos(th);
     im = r * sin(th);
 }
+
+// ------------------------------------------------------------------ the reference's random stream
+// MKL VSL's MCG59, the generator of the reference (subroutines.f90:16, 49-159): x_{n+1} = 13^13 x_n
+// mod 2^59, uniform u_n = x_n / 2^59; VSL's BOXMULLER Gaussian n is sqrt(-2 ln u_{2n})
+// sin(2 pi u_{2n+1}).  Position p of the stream started at x_0 is x_0 13^(13 p) mod 2^59: 2^59
+// divides 2^64, so wrapping 64-bit products reduced to 59 bits at the end give it exactly.
+constexpr uint64_t MCG59_A = 302875106592253ull;  // 13^13
+constexpr uint64_t MCG59_MASK = (1ull << 59) - 1;
+
+__host__ __device__ __forceinline__ uint64_t mcg59_jump(uint64_t x0, uint64_t p)
+{
+    uint64_t a = MCG59_A, r = 1;
+    while (p) {
+        if (p & 1) r *= a;
+        a *= a;
+        p >>= 1;
+    }
+    return (r * x0) & MCG59_MASK;
+}
+
+__host__ __device__ __forceinline__ uint64_t mcg59_next(uint64_t x) { return (x * MCG59_A) & MCG59_MASK; }
+__host__ __device__ __forceinline__ double mcg59_u(uint64_t x) { return (double)x * 1.7347234759768071e-18; }
+
+// Box-Muller on the next two draws of the stream at x (advanced by two).
+__device__ __forceinline__ double mcg59_gauss(uint64_t& x)
+{
+#pragma clang fp contract(off)
+    const double u1 = mcg59_u(x);
+    x = mcg59_next(x);
+    const double u2 = mcg59_u(x);
+    x = mcg59_next(x);
+    return sqrt(-2.0 * log(u1)) * sin(6.283185307179586 * u2);
+}
+
+// Where a routine's random numbers come from: the reference's MCG59 stream (mcg != 0: its start
+// x0 and the position of the routine's first draw) or the build's counter-based Philox streams
+// keyed by (k0, k1).
+struct Draws {
+    uint64_t x0, pos;
+    uint32_t k0, k1;
+    int mcg;
+};

@@ -233,10 +233,10 @@ double group_prepare(hipStream_t s, int N, const double* rec, int stride, int co
 void group_link(hipStream_t s, int N, const double* rec, int stride, const int32_t* cells, int64_t n,
                 const int32_t* map, int32_t* parent, int32_t* label, double th, double maxoff, int R, int mode,
                 int centre, const int32_t* outliers, int n_out, int* fail);
-void launch_cov_modes(hipStream_t s, int N, uint64_t seed, int iter, const uint16_t* bin_of_n2, const double* sigma,
+void launch_cov_modes(hipStream_t s, int N, const Draws& dr, int iter, const uint16_t* bin_of_n2, const double* sigma,
                       double2* out);
 void launch_mask_zero(hipStream_t s, int64_t n3, const int8_t* flags, double2* data);
-void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+void launch_thermal_modes(hipStream_t s, int N, const Draws& dr, int ip, double tau_ini, double sigma_cr, double dwav,
                           double dv_wav, double thrm_sigma, double2* out);
 
 // String cells (flag bit 1) in site order `cells`: fit, chord length and velocity per cell.
@@ -250,7 +250,7 @@ constexpr int STRING_STATE = 26;
 constexpr int STRING_SPLIT = 64;  // default split (fastest of 2..1000 at 512^3 on MI355X)
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, int split,
+                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split,
                        double* rec, double* partial, int* npend, int64_t* pend, double* state);
 
 void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col);

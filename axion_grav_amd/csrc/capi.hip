@@ -135,6 +135,11 @@ struct axg_handle {
     int64_t nmask = -1;       // -1: no classification yet
     int64_t nstring = 0;      // string cells of the last classification
     bool have_flags = false;
+    // the reference's random stream (axg_set_stream): MCG59 start and the position of the next draw
+    bool stream_on = false;
+    uint64_t stream_x0 = 0, stream_pos = 0;
+    // the random source of the next call: the stream at its position, or Philox keyed by (k0, k1)
+    Draws draws(uint32_t k0, uint32_t k1) const { return Draws{stream_x0, stream_pos, k0, k1, stream_on ? 1 : 0}; }
     // plaquette windings (grown on demand): face codes, per-block counts / offsets / totals, rows
     uint16_t* plaq_code = nullptr;
     int64_t* plaq_cnt = nullptr;
@@ -1117,24 +1122,26 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
             launch_scan_counts(h->stream, counts, soffs, nblk, stot, 0);
             launch_cell_compact(h->stream, h->N, h->flags, 1, soffs, h->str_cells, 0, nblk);
             double* dsum = h->str_part + 5 * nsb;
+            const Draws dr = h->draws(seed0, seed1);
             int* npend = (int*)h->str_state;
             double* state = h->str_state + 1;
             if (h->comp)
                 launch_string_fit<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi,
-                                               h->str_cells, ns, seed0, seed1, h->p.dspa, h->string_split, h->str_rec,
+                                               h->str_cells, ns, dr, h->p.dspa, h->string_split, h->str_rec,
                                                h->str_part, npend, h->str_pend, state);
             else if (h->fp32)
                 launch_string_fit<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, h->str_cells,
-                                                ns, seed0, seed1, h->p.dspa, h->string_split, h->str_rec, h->str_part,
+                                                ns, dr, h->p.dspa, h->string_split, h->str_rec, h->str_part,
                                                 npend, h->str_pend, state);
             else
                 launch_string_fit<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr,
-                                                 h->str_cells, ns, seed0, seed1, h->p.dspa, h->string_split,
+                                                 h->str_cells, ns, dr, h->p.dspa, h->string_split,
                                                  h->str_rec, h->str_part, npend, h->str_pend, state);
             sum_columns(h->stream, h->str_part, (int)nsb, 5, dsum);
             AXG_HIP(hipMemcpyAsync(sums, dsum, sizeof sums, hipMemcpyDeviceToHost, h->stream));
             AXG_HIP(hipStreamSynchronize(h->stream));
         }
+        if (h->stream_on) h->stream_pos += 4004 * (uint64_t)ns;  // 4 draws x 1001 starts per string cell
         out->n_string = ns;
         out->n_vel = (int64_t)sums[1];
         out->length = sums[0];
@@ -1575,8 +1582,9 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
         const std::vector<double>& mat = h->spec_mat;
         int iter = 1;
         bool converged = false;
+        const Draws dr = h->draws((uint32_t)seed, (uint32_t)(seed >> 32));
         for (; iter <= max_mc; iter++) {
-            launch_cov_modes(h->stream, N, seed, iter, h->spec_bin, d_sig, h->work);
+            launch_cov_modes(h->stream, N, dr, iter, h->spec_bin, d_sig, h->work);
             fft3d_work(h, h->work, -1);
             launch_mask_zero(h->stream, h->n3, h->flags, h->work);
             fft3d_work(h, h->work, +1);
@@ -1621,7 +1629,9 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
                 break;
             }
         }
-        // without convergence the loop counter ends at max_mc + 1, which the reference divides by
+        // without convergence the loop counter ends at max_mc + 1, which the reference divides by;
+        // the stream has given 4 N^3 draws to each realisation run
+        if (h->stream_on) h->stream_pos += 4 * (uint64_t)h->n3 * (uint64_t)(converged ? iter : max_mc);
         for (int ij = 0; ij < ncov; ij++) cov[ij] = pc[ij] / iter;
         if (iters) *iters = converged ? iter : -iter;
     });
@@ -2113,20 +2123,41 @@ extern "C" int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini)
         // reference's zero mode has no finite amplitude (osc = 0 or imaginary)
         require(tau_ini > 0 && tau_ini < 1, "the thermal initial state needs 0 < tau_ini < 1 (T above T_cr)");
         const axg_params& p = h->p;
+        const Draws dr = h->draws((uint32_t)seed, (uint32_t)(seed >> 32));
         for (int ip = 1; ip <= 2; ip++) {
-            launch_thermal_modes(h->stream, h->N, seed, ip, tau_ini, p.sigma_cr, p.dwav, p.dv_wav, AXG_THRM_SIGMA,
+            launch_thermal_modes(h->stream, h->N, dr, ip, tau_ini, p.sigma_cr, p.dwav, p.dv_wav, AXG_THRM_SIGMA,
                                  h->work);
             fft3d_work(h, h->work, -1);
             if (ip == 1) store_from_work(h, h->phi[h->cur], h->cphi[h->cur]);
             else store_from_work(h, h->pi, h->cpi);
         }
         AXG_HIP(hipStreamSynchronize(h->stream));
+        if (h->stream_on) h->stream_pos += 8 * (uint64_t)h->n3;  // 2 fields x N^3 modes x 4 draws
         h->have_flags = false;
         h->have_plaq = false;
         h->have_spec = false;
         h->nstr_rec = -1;
         h->nwall_rec = -1;
         h->have_grp[0] = h->have_grp[1] = false;
+    });
+}
+
+extern "C" int axg_set_stream(axg_handle* h, uint32_t seed1, uint32_t seed2, uint64_t position)
+{
+    return guarded(h, [&] {
+        uint64_t x = ((uint64_t)seed1 + ((uint64_t)seed2 << 32)) & MCG59_MASK;
+        h->stream_x0 = x ? x : 1;  // vslNewStreamEx's start for two seed words
+        h->stream_pos = position;
+        h->stream_on = true;
+    });
+}
+
+extern "C" int axg_get_stream(axg_handle* h, uint64_t* position)
+{
+    return guarded(h, [&] {
+        require(position != nullptr, "null position");
+        require(h->stream_on, "no stream set (axg_set_stream)");
+        *position = h->stream_pos;
     });
 }
 

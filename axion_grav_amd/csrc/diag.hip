@@ -527,20 +527,29 @@ void launch_scale(hipStream_t s, int64_t n, double2* data, double scale)
 
 // ------------------------------------------------------------------ thermal initial state
 // One mode per thread (initial.f90:105-135 with initial_phi_phidot 372-390): dv_wav times a
-// unit complex Gaussian (Philox4x32-10 keyed by the seed, counter = (mode index, 'thml', ip);
-// Box-Muller on two 53-bit uniforms in (0,1]) times the Bose-Einstein amplitude of the mode,
-// in the reference's operation order.  out is then transformed with the FORWARD sign.
-__global__ void thermal_modes_kernel(int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+// unit complex Gaussian times the Bose-Einstein amplitude of the mode, in the reference's
+// operation order.  The Gaussian of mode lin of field ip: from the reference's stream, draws
+// pos + 4 N^3 (ip - 1) + 4 lin .. +3 (two Box-Muller pairs, the reference's order); otherwise
+// Philox4x32-10 keyed by the seed, counter = (mode index, 'thml', ip), Box-Muller on two 53-bit
+// uniforms in (0,1].  out is then transformed with the FORWARD sign.
+__global__ void thermal_modes_kernel(int N, Draws dr, int ip, double tau_ini, double sigma_cr, double dwav,
                                      double dv_wav, double thrm_sigma, double2* __restrict__ out)
 {
 #pragma clang fp contract(off)
+    const uint64_t seed = (uint64_t)dr.k0 | ((uint64_t)dr.k1 << 32);
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
     for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
          lin += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(lin % N), y = (int)((lin / N) % N), z = (int)(lin / NN);
         const int64_t nx = x < N / 2 ? x : x - N, ny = y < N / 2 ? y : y - N, nz = z < N / 2 ? z : z - N;
         double g1, g2;
-        unit_gauss(seed, 0x74686d6cu, (uint32_t)ip, (uint64_t)lin, g1, g2);
+        if (dr.mcg) {
+            uint64_t xs = mcg59_jump(dr.x0, dr.pos + 4 * (uint64_t)n3 * (ip - 1) + 4 * (uint64_t)lin);
+            g1 = mcg59_gauss(xs);
+            g2 = mcg59_gauss(xs);
+        } else {
+            unit_gauss(seed, 0x74686d6cu, (uint32_t)ip, (uint64_t)lin, g1, g2);
+        }
         // initial_phi_phidot
         double q = (double)(nx * nx + ny * ny + nz * nz);
         q = sqrt(q) * dwav;
@@ -557,22 +566,25 @@ __global__ void thermal_modes_kernel(int N, uint64_t seed, int ip, double tau_in
     }
 }
 
-void launch_thermal_modes(hipStream_t s, int N, uint64_t seed, int ip, double tau_ini, double sigma_cr, double dwav,
+void launch_thermal_modes(hipStream_t s, int N, const Draws& dr, int ip, double tau_ini, double sigma_cr, double dwav,
                           double dv_wav, double thrm_sigma, double2* out)
 {
-    hipLaunchKernelGGL(thermal_modes_kernel, dim3(4096), dim3(256), 0, s, N, seed, ip, tau_ini, sigma_cr, dwav, dv_wav,
+    hipLaunchKernelGGL(thermal_modes_kernel, dim3(4096), dim3(256), 0, s, N, dr, ip, tau_ini, sigma_cr, dwav, dv_wav,
                        thrm_sigma, out);
     AXG_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ spectrum covariance (MC)
 // One Monte Carlo realisation of estimate_covmat (spectrum.f90:592-641): mode k gets a unit
-// complex Gaussian (Philox4x32-10 keyed by the seed, counter = (mode index, 'cov ', iter)) times
-// sigma[b] of its spectrum bin b, or 0 outside the bins.
-__global__ void cov_modes_kernel(int N, uint64_t seed, int iter, const uint16_t* __restrict__ bin_of_n2,
+// complex Gaussian times sigma[b] of its spectrum bin b, or 0 outside the bins.  The Gaussian of
+// mode k in realisation iter (1-based): from the reference's stream, draws pos + 4 N^3 (iter - 1)
+// + 4 k .. +3 (every mode draws, binned or not, as there); otherwise Philox4x32-10 keyed by the
+// seed, counter = (mode index, 'cov ', iter).
+__global__ void cov_modes_kernel(int N, Draws dr, int iter, const uint16_t* __restrict__ bin_of_n2,
                                  const double* __restrict__ sigma, double2* __restrict__ out)
 {
 #pragma clang fp contract(off)
+    const uint64_t seed = (uint64_t)dr.k0 | ((uint64_t)dr.k1 << 32);
     const int64_t NN = (int64_t)N * N, n3 = NN * N;
     for (int64_t lin = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; lin < n3;
          lin += (int64_t)gridDim.x * blockDim.x) {
@@ -584,7 +596,13 @@ __global__ void cov_modes_kernel(int N, uint64_t seed, int iter, const uint16_t*
             continue;
         }
         double g1, g2;
-        unit_gauss(seed, 0x636f7620u, (uint32_t)iter, (uint64_t)lin, g1, g2);
+        if (dr.mcg) {
+            uint64_t xs = mcg59_jump(dr.x0, dr.pos + 4 * (uint64_t)n3 * (iter - 1) + 4 * (uint64_t)lin);
+            g1 = mcg59_gauss(xs);
+            g2 = mcg59_gauss(xs);
+        } else {
+            unit_gauss(seed, 0x636f7620u, (uint32_t)iter, (uint64_t)lin, g1, g2);
+        }
         const double sg = sigma[b - 1];
         out[lin] = double2{g1 * sg, g2 * sg};
     }
@@ -597,10 +615,10 @@ __global__ void mask_zero_kernel(int64_t n3, const int8_t* __restrict__ flags, d
         if (flags[i] & 4) data[i] = double2{0.0, 0.0};
 }
 
-void launch_cov_modes(hipStream_t s, int N, uint64_t seed, int iter, const uint16_t* bin_of_n2, const double* sigma,
+void launch_cov_modes(hipStream_t s, int N, const Draws& dr, int iter, const uint16_t* bin_of_n2, const double* sigma,
                       double2* out)
 {
-    hipLaunchKernelGGL(cov_modes_kernel, dim3(4096), dim3(256), 0, s, N, seed, iter, bin_of_n2, sigma, out);
+    hipLaunchKernelGGL(cov_modes_kernel, dim3(4096), dim3(256), 0, s, N, dr, iter, bin_of_n2, sigma, out);
     AXG_HIP(hipGetLastError());
 }
 

@@ -28,12 +28,22 @@ constexpr int MAXIT = 1000;    // max_iter_string_pos
 constexpr double PI_T = 3.14159265359;
 constexpr double TWOPI_T = 2 * PI_T;
 
-// The four uniforms of start point i of string s: one Philox block keyed by the seed, counter
-// (i, s, 'strg').  The reference reads them from an MKL VSL MCG59 stream positioned at
-// s * 4004 (defects_3d.f90:137-162, 465); a counter-based generator keeps one stream per string
-// independent of how the strings are scheduled.
-__device__ __forceinline__ void uniforms(uint32_t k0, uint32_t k1, int64_t s, int i, double u[4])
+// The four uniforms of start point i of string s.  From the reference's stream (dr.mcg): its
+// next four draws at xs, which fit_starts positions at pos + 4004 s (defects_3d.f90:137-162,
+// 465: each string cell takes 4 x 1001 draws, in site order).  Otherwise one Philox block keyed
+// by (k0, k1), counter (i, s, 'strg'): one stream per string, independent of how the strings are
+// scheduled.
+__device__ __forceinline__ void uniforms(const Draws& dr, uint64_t& xs, int64_t s, int i, double u[4])
 {
+    if (dr.mcg) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            u[k] = mcg59_u(xs);
+            xs = mcg59_next(xs);
+        }
+        return;
+    }
+    const uint32_t k0 = dr.k0, k1 = dr.k1;
     uint32_t c0 = (uint32_t)i, c1 = (uint32_t)s, c2 = (uint32_t)((uint64_t)s >> 32), c3 = 0x73747267u;
     philox4x32(c0, c1, c2, c3, k0, k1);
     const double sc = 2.3283064365386963e-10;  // 2^-32
@@ -169,10 +179,11 @@ constexpr double STR_TOL = 0.05;  // chi2 at which the simplex stops
 // 1001 random starts keep the best five (the first two unsorted, as in the reference).
 // The search arrays are locals here and copied out at the end: with the caller's arrays
 // (references) the compiler moved c2 to scratch memory inside the 1001-start loop.
-__device__ __forceinline__ void fit_starts(const Phases& ph, uint32_t k0, uint32_t k1, int64_t s,
+__device__ __forceinline__ void fit_starts(const Phases& ph, const Draws& dr, int64_t s,
                                            double (&p_out)[NSIMP + 1][4], double (&c2_out)[NSIMP + 1])
 {
 #pragma clang fp contract(off)
+    uint64_t xs = dr.mcg ? mcg59_jump(dr.x0, dr.pos + 4004 * (uint64_t)s) : 0;
     const double mx[4] = {1.0, PI_T, 0.86602540378443865, PI_T};
     const double mn[4] = {-1.0, -PI_T, 0.0, -PI_T};
     double p[NSIMP + 1][4], c2[NSIMP + 1], t[4];
@@ -185,7 +196,7 @@ __device__ __forceinline__ void fit_starts(const Phases& ph, uint32_t k0, uint32
     int n0 = -1;
     for (int i = 0; i <= MAXIT; i++) {
         double x[4];
-        uniforms(k0, k1, s, i, x);
+        uniforms(dr, xs, s, i, x);
 #pragma unroll
         for (int k = 0; k < 4; k++) t[k] = mx[k] * x[k] + mn[k] * (1 - x[k]);
         double chi2 = chi2_string(ph, t);
@@ -439,7 +450,7 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
                                                                const typename V2<T>::type* __restrict__ pi,
                                                                const short2* __restrict__ cpi,
                                                                const int32_t* __restrict__ cells, int64_t ns,
-                                                               uint32_t k0, uint32_t k1, double dspa, int split,
+                                                               Draws dr, double dspa, int split,
                                                                double* __restrict__ rec, int* __restrict__ npend,
                                                                int64_t* __restrict__ pend, double* __restrict__ state)
 {
@@ -449,7 +460,7 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
     const StringCell sc = string_cell(N, cells, s);
     const Phases ph = corner_phases<T, COMP>(phi, cphi, sc);
     double p[NSIMP + 1][4], c2[NSIMP + 1];
-    fit_starts(ph, k0, k1, s, p, c2);
+    fit_starts(ph, dr, s, p, c2);
     int it = 0;
     fit_simplex(ph, p, c2, it, split < MAXIT ? split : MAXIT);
     if (c2[0] > STR_TOL && it < MAXIT) {
@@ -536,7 +547,7 @@ __global__ void __launch_bounds__(STRING_NT) string_sum_kernel(const double* __r
 
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, uint32_t key0, uint32_t key1, double dspa, int split,
+                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split,
                        double* rec, double* partial, int* npend, int64_t* pend, double* state)
 {
     using T2 = typename V2<T>::type;
@@ -545,7 +556,7 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
     const int64_t nblk = (ns + STRING_NT - 1) / STRING_NT;
     AXG_HIP(hipMemsetAsync(npend, 0, sizeof(int), s));
     hipLaunchKernelGGL((string_fit_kernel<T, COMP>), dim3((unsigned)nblk), dim3(STRING_NT), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, key0, key1, dspa, split, rec,
+                       (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, dr, dspa, split, rec,
                        npend, pend, state);
     AXG_HIP(hipGetLastError());
     // as many threads as strings; those past the parked count return at once
@@ -560,7 +571,7 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
 
 #define AXG_STRING_FIT(T, C)                                                                                        \
     template void launch_string_fit<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*,    \
-                                          const int32_t*, int64_t, uint32_t, uint32_t, double, int, double*,        \
+                                          const int32_t*, int64_t, const Draws&, double, int, double*,              \
                                           double*, int*, int64_t*, double*);
 AXG_STRING_FIT(double, false)
 AXG_STRING_FIT(float, false)

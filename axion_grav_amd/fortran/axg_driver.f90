@@ -10,10 +10,11 @@
 ! file _param.chk ('idx tau flag') and _phi_<idx>.chk (raw FP64 complex phi then pi).
 !
 ! Keys this build adds (all optional): precision = fp64 | fp32c, num_bin_grav (25),
-! gw_fix_swap (F), dtau_fixed (0: use get_dtau), potential_kappa0 (0), ic_seed (seed of a
-! fresh start: the thermal state at tau_ini when initial_loop = F, else the flat-spectrum field
-! up to ic_kmax), ic_file (raw phi,pi file to start from instead), chk_gw (F: also checkpoint the GW accumulators, which the reference
-! does not), device (0).
+! gw_fix_swap (F), dtau_fixed (0: use get_dtau), potential_kappa0 (0), ic_seed (seed of the
+! flat-spectrum field of a fresh start with initial_loop = T, up to ic_kmax; the thermal state of
+! initial_loop = F draws from the reference's stream seeded by ss, like the string fits and the
+! covariance), ic_file (raw phi,pi file to start from instead), chk_gw (F: also checkpoint the GW
+! accumulators, which the reference does not), device (0).
 program axion_3d
   use, intrinsic :: iso_c_binding
   use axgrav_capi
@@ -208,6 +209,10 @@ program axion_3d
   end if
 
   call axg_check(axg_create(prm, h), c_null_ptr, 'axg_create')
+  ! init_rand_gen (subroutines.f90:49-95): one MCG59 stream seeded by ss, from its first draw at
+  ! every start (a resume re-seeds it too); the thermal state, the string fits and the
+  ! covariance draw from it in the reference's order
+  call axg_check(axg_set_stream(h, seed(1), seed(2), 0_c_int64_t), h, 'axg_set_stream')
 
   ! ------------------------------------------------------------ initial field (initial.f90:21-78)
   call set_initial_condition(chk)

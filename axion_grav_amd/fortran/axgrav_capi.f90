@@ -109,6 +109,19 @@ module axgrav_capi
        integer(c_int64_t), value :: nsite
      end function axg_get_fields
 
+     integer(c_int) function axg_set_stream(h, seed1, seed2, position) bind(C, name="axg_set_stream")
+       import :: c_int, c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: seed1, seed2   ! uint32_t in C: the same bits
+       integer(c_int64_t), value :: position
+     end function axg_set_stream
+
+     integer(c_int) function axg_get_stream(h, position) bind(C, name="axg_get_stream")
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int64_t), intent(out) :: position
+     end function axg_get_stream
+
      integer(c_int) function axg_init_flat(h, seed, kmax) bind(C, name="axg_init_flat")
        import :: c_int, c_ptr, c_int64_t, c_double
        type(c_ptr), value :: h

@@ -88,6 +88,21 @@ const char* axg_last_error(const axg_handle* h);
 int axg_set_fields(axg_handle* h, const double* phi, const double* pi, int64_t nsite);
 int axg_get_fields(axg_handle* h, double* phi, double* pi, int64_t nsite);
 
+/* The reference's random stream (subroutines.f90:49-159): from this call on, the handle's random
+ * draws come from Intel MKL VSL's MCG59 generator (x_{n+1} = 13^13 x_n mod 2^59, uniform
+ * x/2^59, Gaussians by Box-Muller on consecutive pairs, as VSL_RNG_METHOD_GAUSSIAN_BOXMULLER)
+ * started as vslNewStreamEx(VSL_BRNG_MCG59, 2, {seed1, seed2}) does -- the reference's key ss --
+ * and positioned at `position` draws, in the reference's order: axg_init_thermal takes
+ * 8 N^3 draws (initial.f90:102-126: per site 2 Gaussians of phi, then those of phidot),
+ * axg_strings 4004 per string cell (defects_3d.f90:137-162, 465) and axg_field_covariance 4 N^3
+ * per realisation run (spectrum.f90:592-630); each advances the position by what it took, as the
+ * reference's stream0 advances.  The seed arguments of those calls are then unused.  Draws equal
+ * MKL's: uniforms bit for bit, Gaussians to MKL's own ln/sin accuracy (~1e-8 relative).  Without
+ * this call the handle uses its counter-based Philox streams keyed by the calls' seeds. */
+int axg_set_stream(axg_handle* h, uint32_t seed1, uint32_t seed2, uint64_t position);
+/* The position of the stream's next draw (AXG_ESTATE-like AXG_EINVAL when no stream is set). */
+int axg_get_stream(axg_handle* h, uint64_t* position);
+
 /* Seeded flat-spectrum initial field: complex Gaussian modes (Philox4x32-10 + Box-Muller)
  * for |k| < kmax, zero above, BACKWARD FFT, rescaled to <|phi|^2> = 1; pi = 0. */
 int axg_init_flat(axg_handle* h, uint64_t seed, double kmax);
@@ -95,9 +110,9 @@ int axg_init_flat(axg_handle* h, uint64_t seed, double kmax);
 /* Thermal initial state before the transition (get_initial_thermal, initial.f90:80-238, with
  * initial_phi_phidot 372-390): every mode of phi and phidot gets dv_wav times a unit complex
  * Gaussian times its Bose-Einstein amplitude at temperature thrm_sigma/tau_ini, then the
- * unnormalised FORWARD transform.  The Gaussians come from Philox4x32-10 keyed by seed, one
- * draw per mode (the reference's MKL VSL stream is not reproduced: the state agrees in
- * distribution, not draw for draw).  Needs 0 < tau_ini < 1 (AXG_EINVAL otherwise: at
+ * unnormalised FORWARD transform.  The Gaussians come from the reference's stream when
+ * axg_set_stream was called (draw for draw as the reference's), otherwise from Philox4x32-10
+ * keyed by seed, one draw per mode.  Needs 0 < tau_ini < 1 (AXG_EINVAL otherwise: at
  * tau_ini >= 1 the reference's zero mode is not finite). */
 int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini);
 
@@ -176,10 +191,10 @@ typedef struct axg_string_out {
 /* String length and velocity (defects_3d.f90:167-205 with position_string 435-603,
  * calc_chi2_string 384-433, string_length 1027-1056, velocity_string 605-690 of the VEL = 0
  * build, and add_string_length 1654-1733 with group_strings = F), one GPU thread per string
- * cell of the last axg_defects.  Each fit draws its 1001 starting points from Philox4x32-10
- * keyed by (seed0, seed1) with the string's index in site order as counter (the reference's
- * MKL VSL stream, positioned per string, is not reproduced: results agree with the reference's
- * algorithm to the fit's tolerance, not bit for bit).  The driver's string parameter is
+ * cell of the last axg_defects.  Each fit draws its 1001 starting points from the reference's
+ * stream when axg_set_stream was called (string s in site order at position + 4004 s, as the
+ * reference positions it), otherwise from Philox4x32-10 keyed by (seed0, seed1) with the
+ * string's index in site order as counter.  The driver's string parameter is
  * (tau/dspa)^2 * length / N^3 / 4 (evolve.f90:191). */
 int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_string_out* out);
 /* The per-string records of the last axg_strings: 11 doubles per string, pos(3), dir(3), gof,
@@ -244,8 +259,9 @@ int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, d
 /* estimate_covmat (spectrum.f90:569-734; spectrum_estimation calls it with min_mc = 10,
  * max_mc = 100, tol = 0.1): Monte Carlo covariance of the power of the last
  * axg_field_spectrum (an ESTATE error if the fields changed since).  Each realisation draws
- * Gaussian modes of that power (Philox4x32-10 keyed by seed; the reference uses its MCG59
- * stream, so realisations differ from its own), masks them in real space and deconvolves.
+ * Gaussian modes of that power (from the reference's stream when axg_set_stream was called,
+ * realisation after realisation as the reference draws them; otherwise Philox4x32-10 keyed by
+ * seed), masks them in real space and deconvolves.
  * cov: the packed-upper num_bin x num_bin matrix (ncov = num_bin (num_bin+1)/2 entries, pair
  * (i, j <= i) at i(i-1)/2 + j - 1), the reference's power_cov(:, idx).  iters: the number of
  * realisations when the jackknife test met tol, else -(max_mc + 1), the count the reference

@@ -49,6 +49,18 @@ def lib():
         L.orc_gw_spectrum.argtypes = [p, i64, C.c_int, d, d, d, d, d, p, p, p, p]
         L.orc_classify.argtypes = [p, i64, C.c_int, C.c_int, C.c_int, p, p]
         L.orc_plaquettes.argtypes = [p, i64, p, p]
+        u32, u64 = C.c_uint32, C.c_uint64
+        L.orc_mcg59_seed.restype = u64
+        L.orc_mcg59_seed.argtypes = [u32, u32]
+        L.orc_mcg59_jump.restype = u64
+        L.orc_mcg59_jump.argtypes = [u64, u64]
+        L.orc_mcg59_uniforms.argtypes = [u32, u32, u64, i64, p]
+        L.orc_mcg59_gauss.argtypes = [u32, u32, u64, i64, p]
+        L.orc_strings_mcg.restype = i64
+        L.orc_strings_mcg.argtypes = [p, p, i64, p, u64, u64, d, p, p]
+        L.orc_ic_thermal_mcg.argtypes = [p, p, i64, u64, u64, d, d, d, d]
+        L.orc_field_covariance_mcg.restype = C.c_int
+        L.orc_field_covariance_mcg.argtypes = [p, i64, C.c_int, d, d, p, p, u64, u64, C.c_int, C.c_int, d, p, p]
         L.orc_conv_spectrum.argtypes = [p, p, i64, C.c_int, d, d, d, d, p, p]
         L.orc_integrate_p_check.restype = d
         L.orc_integrate_p_check.argtypes = [C.c_int, d, d, d, d, d]
@@ -300,15 +312,20 @@ def velocity_string(phi, pi, x, y, z, pos, dspa):
     return vel
 
 
-def strings(phi, pi, flags, key0, key1, dspa):
+def strings(phi, pi, flags, key0, key1, dspa, stream=None):
     """All string cells (flag bit 1) in site order: records (ns, 11) = pos(3, site coordinates
-    counted from 0), dir(3), gof, sl, vel(3); sums = (sum sl, n |v|<1, sum |v|, sum v^2, sum gamma)."""
+    counted from 0), dir(3), gof, sl, vel(3); sums = (sum sl, n |v|<1, sum |v|, sum v^2, sum gamma).
+    stream = (w0, w1, pos): starting points from the reference's MCG59 stream (keys unused)."""
     N = phi.shape[0]
     ns = int(np.count_nonzero(flags & 1))
     rec = np.zeros((max(ns, 1), 11))
     sums = np.zeros(5)
-    n = lib().orc_strings(_p(phi), _p(pi), N, _p(np.ascontiguousarray(flags, np.int8)), key0, key1, dspa,
-                          _p(rec), _p(sums))
+    fl = _p(np.ascontiguousarray(flags, np.int8))
+    if stream is None:
+        n = lib().orc_strings(_p(phi), _p(pi), N, fl, key0, key1, dspa, _p(rec), _p(sums))
+    else:
+        n = lib().orc_strings_mcg(_p(phi), _p(pi), N, fl, mcg59_seed(stream[0], stream[1]), stream[2], dspa,
+                                  _p(rec), _p(sums))
     return rec[:n], sums
 
 
@@ -324,11 +341,39 @@ def thermal_gauss(seed, ip, lin):
     return complex(re.value, im.value)
 
 
-def ic_thermal(N, seed, tau_ini, sigma_cr, dwav, dv_wav):
+def ic_thermal(N, seed, tau_ini, sigma_cr, dwav, dv_wav, stream=None):
+    """Thermal state; stream = (w0, w1, pos): the reference's MCG59 stream seeded by the two words
+    and positioned at pos (seed is then unused)."""
     phi = np.zeros((N, N, N), np.complex128)
     pi = np.zeros((N, N, N), np.complex128)
-    lib().orc_ic_thermal(_p(phi), _p(pi), N, seed, tau_ini, sigma_cr, dwav, dv_wav)
+    if stream is None:
+        lib().orc_ic_thermal(_p(phi), _p(pi), N, seed, tau_ini, sigma_cr, dwav, dv_wav)
+    else:
+        x0 = mcg59_seed(stream[0], stream[1])
+        lib().orc_ic_thermal_mcg(_p(phi), _p(pi), N, x0, stream[2], tau_ini, sigma_cr, dwav, dv_wav)
     return phi, pi
+
+
+# ------------------------------------------------------------------ the reference's random stream
+
+def mcg59_seed(w0, w1):
+    return lib().orc_mcg59_seed(w0, w1)
+
+
+def mcg59_jump(x0, p):
+    return lib().orc_mcg59_jump(x0, p)
+
+
+def mcg59_uniforms(w0, w1, pos, n):
+    out = np.zeros(n)
+    lib().orc_mcg59_uniforms(w0, w1, pos, n, _p(out))
+    return out
+
+
+def mcg59_gauss(w0, w1, pos, n):
+    out = np.zeros(n)
+    lib().orc_mcg59_gauss(w0, w1, pos, n, _p(out))
+    return out
 
 
 # ------------------------------------------------------------------ spectrum covariance
@@ -339,17 +384,21 @@ def cov_gauss(seed, it, lin):
     return complex(re.value, im.value)
 
 
-def field_covariance(flags, nb, dspa, dwav, power, matn, seed, min_mc, max_mc, tol, want_pows=False):
+def field_covariance(flags, nb, dspa, dwav, power, matn, seed, min_mc, max_mc, tol, want_pows=False, stream=None):
     """Monte Carlo covariance of the deconvolved spectrum: (packed cov, iteration counter
-    (negative without convergence), per-realisation pow (max_mc, nb) or None)."""
+    (negative without convergence), per-realisation pow (max_mc, nb) or None).  stream =
+    (w0, w1, pos): realisations from the reference's MCG59 stream (seed unused)."""
     N = flags.shape[0]
     ncov = nb * (nb + 1) // 2
     cov = np.zeros(ncov)
     pows = np.zeros((max_mc, nb)) if want_pows else None
-    it = lib().orc_field_covariance(_p(np.ascontiguousarray(flags, np.int8)), N, nb, dspa, dwav,
-                                    _p(np.ascontiguousarray(power, np.float64)),
-                                    _p(np.ascontiguousarray(matn, np.float64)), seed, min_mc, max_mc, tol,
-                                    _p(cov), _p(pows) if want_pows else None)
+    args = (_p(np.ascontiguousarray(flags, np.int8)), N, nb, dspa, dwav, _p(np.ascontiguousarray(power, np.float64)),
+            _p(np.ascontiguousarray(matn, np.float64)))
+    tail = (min_mc, max_mc, tol, _p(cov), _p(pows) if want_pows else None)
+    if stream is None:
+        it = lib().orc_field_covariance(*args, seed, *tail)
+    else:
+        it = lib().orc_field_covariance_mcg(*args, mcg59_seed(stream[0], stream[1]), stream[2], *tail)
     return cov, it, pows
 
 

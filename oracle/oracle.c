@@ -901,12 +901,86 @@ double orc_string_length(const double* pos, const double* dir)
     return (tp < atmax0 && tn > -atmax0) ? tp - tn : 0.0;
 }
 
-/* The uniforms of start point i of string s: one Philox4x32-10 block keyed by the seed.  The
- * reference draws them from an MKL VSL MCG59 stream positioned at s*4004 (defects_3d.f90:
- * 137-162, 465); this keeps the same "stream per string" structure with a counter-based
- * generator, so the result does not depend on the thread count. */
-static void string_uniforms(uint32_t key0, uint32_t key1, int64_t s, int i, double* u)
+/* ------------------------------------------------ the reference's random stream (MCG59) */
+
+/* The reference draws every random number from one MKL VSL stream of the MCG59 generator
+ * (subroutines.f90:16, 49-159; Intel MKL VS notes): x_{n+1} = a x_n mod 2^59 with a = 13^13,
+ * uniform u_n = x_n / 2^59 on [0, 1); vslNewStreamEx(MCG59, 2, {w0, w1}) starts at
+ * x_0 = (w0 + 2^32 w1) mod 2^59 (1 if that is 0), and u_0 is the first draw.  VSL's
+ * BOXMULLER Gaussian n uses the pair (u_{2n}, u_{2n+1}): sqrt(-2 ln u_{2n}) sin(2 pi u_{2n+1}).
+ * Position p of the stream is x_0 a^p mod 2^59; as 2^59 divides 2^64, wrapping 64-bit products
+ * reduced to 59 bits at the end give it exactly. */
+#define MCG59_A 302875106592253ull /* 13^13 */
+#define MCG59_MASK ((1ull << 59) - 1)
+
+uint64_t orc_mcg59_seed(uint32_t w0, uint32_t w1)
 {
+    uint64_t x = ((uint64_t)w0 + ((uint64_t)w1 << 32)) & MCG59_MASK;
+    return x ? x : 1;
+}
+
+uint64_t orc_mcg59_jump(uint64_t x0, uint64_t p)
+{
+    uint64_t a = MCG59_A, r = 1;
+    while (p) {
+        if (p & 1) r *= a;
+        a *= a;
+        p >>= 1;
+    }
+    return (r * x0) & MCG59_MASK;
+}
+
+static double mcg59_u(uint64_t x) { return (double)x * 1.7347234759768071e-18; /* 2^-59 */ }
+static uint64_t mcg59_next(uint64_t x) { return (x * MCG59_A) & MCG59_MASK; }
+
+/* Box-Muller on the next two draws of the stream at x (advanced by two). */
+static double mcg59_gauss(uint64_t* x)
+{
+    double u1 = mcg59_u(*x);
+    *x = mcg59_next(*x);
+    double u2 = mcg59_u(*x);
+    *x = mcg59_next(*x);
+    return sqrt(-2.0 * log(u1)) * sin(6.283185307179586 * u2);
+}
+
+/* n uniforms / n Gaussians of the stream (w0, w1) from position pos (test vectors). */
+void orc_mcg59_uniforms(uint32_t w0, uint32_t w1, uint64_t pos, int64_t n, double* out)
+{
+    uint64_t x = orc_mcg59_jump(orc_mcg59_seed(w0, w1), pos);
+    for (int64_t i = 0; i < n; i++) {
+        out[i] = mcg59_u(x);
+        x = mcg59_next(x);
+    }
+}
+
+void orc_mcg59_gauss(uint32_t w0, uint32_t w1, uint64_t pos, int64_t n, double* out)
+{
+    uint64_t x = orc_mcg59_jump(orc_mcg59_seed(w0, w1), pos);
+    for (int64_t i = 0; i < n; i++) out[i] = mcg59_gauss(&x);
+}
+
+/* Where a routine's random numbers come from: the reference's stream (on != 0: x0 and the
+ * position of the routine's first draw) or, when on = 0 or the pointer is NULL, the build's
+ * own counter-based Philox streams keyed by the routine's seed. */
+typedef struct {
+    int on;
+    uint64_t x0, pos;
+} orc_stream;
+
+/* The uniforms of start point i of string s.  The reference's stream: draws pos + 4004 s + 4 i
+ * .. +3 (defects_3d.f90:137-162, 465: each string cell takes mult_string_pos = 4 x 1001 draws,
+ * in site order).  Otherwise one Philox4x32-10 block keyed by the seed, counter (i, s, 'strg'):
+ * the same "stream per string" structure, independent of the thread count. */
+static void string_uniforms(uint32_t key0, uint32_t key1, const orc_stream* st, int64_t s, int i, double* u)
+{
+    if (st && st->on) {
+        uint64_t x = orc_mcg59_jump(st->x0, st->pos + 4004 * (uint64_t)s + 4 * (uint64_t)i);
+        for (int k = 0; k < 4; k++) {
+            u[k] = mcg59_u(x);
+            x = mcg59_next(x);
+        }
+        return;
+    }
     uint32_t c[4] = {(uint32_t)i, (uint32_t)s, (uint32_t)((uint64_t)s >> 32), 0x73747267u};
     philox4x32_10(c, key0, key1);
     for (int k = 0; k < 4; k++) u[k] = ((double)c[k] + 0.5) * 2.3283064365386963e-10;
@@ -924,8 +998,17 @@ static int str_out_of_box(const double* t, const double* mx, const double* mn)
  * 2c - p, expansion 3c - 4p, contraction 1.5c - 0.5p, else shrink towards the best point,
  * bubble sort) until chi2 <= 0.05 or 1000 iterations; then position, direction, goodness of
  * fit and the chord length.  s = the string's index in site order (its RNG stream). */
+static void position_string_st(const double* ph, uint32_t key0, uint32_t key1, const orc_stream* st, int64_t s,
+                               double* pos, double* dir, double* gof, double* sl);
+
 void orc_position_string(const double* ph, uint32_t key0, uint32_t key1, int64_t s, double* pos,
                          double* dir, double* gof, double* sl)
+{
+    position_string_st(ph, key0, key1, NULL, s, pos, dir, gof, sl);
+}
+
+static void position_string_st(const double* ph, uint32_t key0, uint32_t key1, const orc_stream* st, int64_t s,
+                               double* pos, double* dir, double* gof, double* sl)
 {
     const double chi2max = 8 * PI_T * PI_T, tol = 0.05;
     const double mx[STR_NSIMP] = {1.0, PI_T, 0.86602540378443865, PI_T};
@@ -936,7 +1019,7 @@ void orc_position_string(const double* ph, uint32_t key0, uint32_t key1, int64_t
     int n0 = -1;
     for (int i = 0; i <= STR_MAXIT; i++) {
         double x[4];
-        string_uniforms(key0, key1, s, i, x);
+        string_uniforms(key0, key1, st, s, i, x);
         for (int k = 0; k < STR_NSIMP; k++) t[k] = mx[k] * x[k] + mn[k] * (1 - x[k]);
         double chi2 = orc_chi2_string(ph, t);
         if (n0 <= 0) {
@@ -1076,8 +1159,26 @@ void orc_velocity_string(const double* phi, const double* pi, int64_t N, int64_t
  * the 1-based corner, i.e. site coordinates counted from 0) and add_string_length (1654-1733, group_strings = F).  rec (or NULL)
  * receives 11 doubles per string: pos(3), dir(3), gof, sl, vel(3).  sums = {sum sl, n with
  * |v| < 1, sum |v|, sum v^2, sum gamma} over those.  Returns the string count. */
+int64_t orc_strings_st(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
+                       uint32_t key1, const orc_stream* st, double dspa, double* rec, double* sums);
+
 int64_t orc_strings(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
                     uint32_t key1, double dspa, double* rec, double* sums)
+{
+    return orc_strings_st(phi, pi, N, flags, key0, key1, NULL, dspa, rec, sums);
+}
+
+/* orc_strings with the reference's stream: x0, pos = the stream's start and the position of the
+ * first string's first draw. */
+int64_t orc_strings_mcg(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint64_t x0,
+                        uint64_t pos, double dspa, double* rec, double* sums)
+{
+    orc_stream st = {1, x0, pos};
+    return orc_strings_st(phi, pi, N, flags, 0, 0, &st, dspa, rec, sums);
+}
+
+int64_t orc_strings_st(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
+                       uint32_t key1, const orc_stream* st, double dspa, double* rec, double* sums)
 {
     const oc* f = (const oc*)phi;
     int64_t n3 = N * N * N, ns = 0;
@@ -1097,7 +1198,7 @@ int64_t orc_strings(const double* phi, const double* pi, int64_t N, const int8_t
             ph[c] = atan2(v.im, v.re);
         }
         double* o = r + 11 * s;
-        orc_position_string(ph, key0, key1, s, o, o + 3, o + 6, o + 7);
+        position_string_st(ph, key0, key1, st, s, o, o + 3, o + 6, o + 7);
         orc_velocity_string(phi, pi, N, x, y, z, o, dspa, o + 8);
         o[0] += (double)(x + 1) - 0.5;
         o[1] += (double)(y + 1) - 0.5;
@@ -1179,8 +1280,27 @@ void orc_cov_gauss(uint64_t seed, int iter, uint64_t lin, double* re, double* im
  * wavevector grid (x fastest) gets dv_wav times a unit complex Gaussian times
  * orc_thermal_factor, then the unnormalised FORWARD transform (FFTW_FORWARD, 332-333) gives
  * the field in real space. */
+static void ic_thermal_st(double* phi, double* pi, int64_t N, uint64_t seed, const orc_stream* st, double tau_ini,
+                          double sigma_cr, double dwav, double dv_wav);
+
 void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double tau_ini, double sigma_cr,
                     double dwav, double dv_wav)
+{
+    ic_thermal_st(phi, pi, N, seed, NULL, tau_ini, sigma_cr, dwav, dv_wav);
+}
+
+/* The same from the reference's stream at position pos: mode lin of field ip takes draws
+ * pos + 4 N^3 (ip - 1) + 4 lin .. +3, i.e. Gaussians (re, im) of those two pairs
+ * (initial.f90:102-126: mult = 2 Gaussians x 2 draws per site, phi's N^3 sites then phidot's). */
+void orc_ic_thermal_mcg(double* phi, double* pi, int64_t N, uint64_t x0, uint64_t pos, double tau_ini,
+                        double sigma_cr, double dwav, double dv_wav)
+{
+    orc_stream st = {1, x0, pos};
+    ic_thermal_st(phi, pi, N, 0, &st, tau_ini, sigma_cr, dwav, dv_wav);
+}
+
+static void ic_thermal_st(double* phi, double* pi, int64_t N, uint64_t seed, const orc_stream* st, double tau_ini,
+                          double sigma_cr, double dwav, double dv_wav)
 {
     int64_t n3 = N * N * N;
     oc* buf = (oc*)malloc(sizeof(oc) * n3);
@@ -1192,7 +1312,13 @@ void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double ta
                     int64_t lin = x + N * (y + N * z);
                     int64_t nx = wrapn(x, N), ny = wrapn(y, N), nz = wrapn(z, N);
                     double g1, g2;
-                    orc_thermal_gauss(seed, ip, (uint64_t)lin, &g1, &g2);
+                    if (st && st->on) {
+                        uint64_t x = orc_mcg59_jump(st->x0, st->pos + 4 * (uint64_t)n3 * (ip - 1) + 4 * (uint64_t)lin);
+                        g1 = mcg59_gauss(&x);
+                        g2 = mcg59_gauss(&x);
+                    } else {
+                        orc_thermal_gauss(seed, ip, (uint64_t)lin, &g1, &g2);
+                    }
                     double f = orc_thermal_factor(ip, nx * nx + ny * ny + nz * nz, tau_ini, sigma_cr, dwav, dv_wav);
                     buf[lin].re = (g1 * dv_wav) * f;
                     buf[lin].im = (g2 * dv_wav) * f;
@@ -1213,9 +1339,31 @@ void orc_ic_thermal(double* phi, double* pi, int64_t N, uint64_t seed, double ta
  * on, stop when max over the diagonal of sd(tmp/(iter-1))/(cov/iter) < tol.  cov is divided
  * by the final loop counter, max_mc + 1 without convergence.  Returns the counter, negated
  * without convergence.  pows (optional, max_mc x nb) receives each realisation's pow. */
+static int field_covariance_st(const int8_t* flags, int64_t N, int nb, double dspa, double dwav,
+                               const double* power, const double* matn, uint64_t seed, const orc_stream* st,
+                               int min_mc, int max_mc, double tol, double* cov, double* pows);
+
 int orc_field_covariance(const int8_t* flags, int64_t N, int nb, double dspa, double dwav, const double* power,
                          const double* matn, uint64_t seed, int min_mc, int max_mc, double tol, double* cov,
                          double* pows)
+{
+    return field_covariance_st(flags, N, nb, dspa, dwav, power, matn, seed, NULL, min_mc, max_mc, tol, cov, pows);
+}
+
+/* The same from the reference's stream at position pos: realisation iter (1-based) draws
+ * pos + 4 N^3 (iter - 1) + 4 k .. +3 for mode k (spectrum.f90:592-630, all N^3 modes, binned or
+ * not). */
+int orc_field_covariance_mcg(const int8_t* flags, int64_t N, int nb, double dspa, double dwav, const double* power,
+                             const double* matn, uint64_t x0, uint64_t pos, int min_mc, int max_mc, double tol,
+                             double* cov, double* pows)
+{
+    orc_stream st = {1, x0, pos};
+    return field_covariance_st(flags, N, nb, dspa, dwav, power, matn, 0, &st, min_mc, max_mc, tol, cov, pows);
+}
+
+static int field_covariance_st(const int8_t* flags, int64_t N, int nb, double dspa, double dwav,
+                               const double* power, const double* matn, uint64_t seed, const orc_stream* st,
+                               int min_mc, int max_mc, double tol, double* cov, double* pows)
 {
     const int64_t n3 = N * N * N;
     const int ncov = nb * (nb + 1) / 2;
@@ -1243,9 +1391,15 @@ int orc_field_covariance(const int8_t* flags, int64_t N, int nb, double dspa, do
     for (int ij = 0; ij < ncov; ij++) cov[ij] = 0;
     int iter, converged = 0;
     for (iter = 1; iter <= max_mc; iter++) {
+        uint64_t xs = st && st->on ? orc_mcg59_jump(st->x0, st->pos + 4 * (uint64_t)n3 * (iter - 1)) : 0;
         for (int64_t k = 0; k < n3; k++) {
             double g1, g2;
-            orc_cov_gauss(seed, iter, (uint64_t)k, &g1, &g2);
+            if (st && st->on) {
+                g1 = mcg59_gauss(&xs);
+                g2 = mcg59_gauss(&xs);
+            } else {
+                orc_cov_gauss(seed, iter, (uint64_t)k, &g1, &g2);
+            }
             if (bin[k]) {
                 w[2 * k] = g1 * sigma[bin[k] - 1];
                 w[2 * k + 1] = g2 * sigma[bin[k] - 1];

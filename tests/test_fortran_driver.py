@@ -106,6 +106,10 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
     P = dict(d, kappa=1e-30, initial_loop=True)
     taus = [1.0, 1.5, 2.0]
     states = oracle_run(N, P, phi0, pi0, taus, 1.0)
+    # the reference's stream seeded by ss = "1 2" (an ic_file start draws nothing): at each snapshot
+    # the string fits take 4004 draws per string cell, then the covariance 4 N^3 per realisation
+    pos = 0
+    str_pos = {}
     for idx in (1, 2):
         phi, pi, A = states[idx]
         # gravspec: tau_id, q, q f_cr, power, denom (grav_waves.f90:313-314)
@@ -139,8 +143,11 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
         assert c.shape == (nb * (nb + 1) // 2, 3)
         assert np.array_equal(c[:, 0], ii[order] + 1) and np.array_equal(c[:, 1], jj[order] + 1)
         matn = [(O.sym_from_packed(inv, nb) / np.outer(num, num))[j, i] for i, j in zip(ii[order], jj[order])]
-        ocov, oit, _ = O.field_covariance(flags, nb, d["dspa"], d["dwav"], power, np.array(matn), (1 << 32) | (2 + idx),
-                                          10, 100, 0.1)
+        str_pos[idx] = pos
+        pos += 4004 * int(counts[0])
+        ocov, oit, _ = O.field_covariance(flags, nb, d["dspa"], d["dwav"], power, np.array(matn), 0, 10, 100, 0.1,
+                                          stream=(1, 2, pos))
+        pos += 4 * N ** 3 * (oit if oit > 0 else 100)
         assert power.min() > 0 and np.all(np.isfinite(ocov))
         assert np.allclose(c[:, 2], ocov, rtol=1e-5, atol=1e-6 * np.abs(ocov).max())
     # get_epsilon (spectrum.f90:956-1040) from the files: eps = f sum d q^2 / sum d q with
@@ -175,11 +182,11 @@ def test_run_outputs_match_oracle_and_resume(tmp_path):
     assert np.allclose(e[0, 8], np.hypot(drho[1], drho[2]), rtol=1e-7)
     sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
     assert sw.shape == (2, 12) and np.allclose(sw[:, 1], taus[1:])
-    # string parameter (evolve.f90:191) and velocities from the fits seeded by ss = "1 2" and idx
+    # string parameter (evolve.f90:191) and velocities from the fits' draws of the stream
     for idx in (1, 2):
         phi, pi, _ = states[idx]
         flags, _ = O.classify(phi)
-        _, sums = O.strings(phi, pi, flags, 1, 2 + idx, d["dspa"])
+        _, sums = O.strings(phi, pi, flags, 0, 0, d["dspa"], stream=(1, 2, str_pos[idx]))
         xi = (taus[idx] / d["dspa"]) ** 2 * sums[0] / N ** 3 / 4
         assert np.allclose(sw[idx - 1, 3:6], xi, rtol=0.02), (sw[idx - 1, 3:6], xi)
         assert np.allclose(sw[idx - 1, 9:12], sums[2:5] / sums[1], rtol=0.02)
@@ -264,11 +271,13 @@ def test_grouped_networks_and_defect_files(tmp_path):
     taus = [1.0, 1.5, 2.0]
     states = oracle_run(N, P, phi0, pi0, taus, 1.0)
     sw = read_rows(tmp_path / "run_string_wall_parameters.txt")
+    pos = 0  # the reference's stream seeded by ss: 4004 draws per string cell, no covariance
     for idx in (1, 2):
         phi, pi, _ = states[idx]
         tau = taus[idx]
         flags, counts = O.classify(phi)
-        srec, _ = O.strings(phi, pi, flags, 1, 2 + idx, d["dspa"])
+        srec, _ = O.strings(phi, pi, flags, 0, 0, d["dspa"], stream=(1, 2, pos))
+        pos += 4004 * int(counts[0])
         wrec, _ = O.walls(phi, flags)
         hor = tau / d["dspa"]
         th_s = O.group_threshold(d["width_cr"] / d["dspa"] / tau, hor)

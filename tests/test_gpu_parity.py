@@ -695,3 +695,71 @@ def test_plaquettes_bitexact(N, prec, kmax):
         assert np.array_equal(cnt[k], ocnt[k]), k
     assert cnt["n_pierced"].sum() > 0 and np.all(cnt["net"] == 0)
     assert np.array_equal(rec, orec)
+
+
+# ---------------------------------------------------------------- the reference's random stream
+
+@pytest.mark.parametrize("N,prec", [(32, capi.FP64), (48, capi.FP64), (32, capi.FP32C)])
+def test_thermal_state_from_reference_stream(N, prec):
+    """axg_set_stream: the thermal state's Gaussians come from the reference's MCG59 stream (draws
+    pos + 4 N^3 (ip - 1) + 4 lin, initial.f90:102-126) and the stream advances by 8 N^3.  Against
+    the oracle on the same stream: <= 1e-12 in FP64 (device vs host ln/sin/exp), FP32C storage
+    rounding otherwise."""
+    say("thermal stream", N, prec)
+    prm, P = params(N, grav_spec=False, initial_loop=False, precision=prec)
+    with capi.Lattice(prm) as L:
+        L.set_stream(12345, 678, 1000)
+        L.init_thermal(0, 0.2)
+        phi, pi = L.get_fields()
+        pos = L.stream_position()
+    assert pos == 1000 + 8 * N ** 3
+    ophi, opi = O.ic_thermal(N, 0, 0.2, P["sigma_cr"], P["dwav"], P["dv_wav"], stream=(12345, 678, 1000))
+    tol = 1e-12 if prec == capi.FP64 else 1e-10
+    assert relerr(phi, ophi) < tol and relerr(pi, opi) < tol
+    # not the Philox state
+    xphi, _ = O.ic_thermal(N, 0, 0.2, P["sigma_cr"], P["dwav"], P["dv_wav"])
+    assert relerr(phi, xphi) > 0.1
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_strings_and_covariance_from_reference_stream(prec):
+    """A snapshot's draws in the reference's order from one stream: the string fits take 4004
+    draws per string cell (string s at position + 4004 s, defects_3d.f90:137-162, 465), then the
+    covariance 4 N^3 per realisation (spectrum.f90:592-630).  Records and sums as the oracle's on
+    the same stream positions (bars of the Philox tests), the covariance with the same number of
+    realisations within 1e-6, and the position advanced by exactly what was drawn."""
+    say("strings + covariance stream", prec)
+    N, nb = 32, 12
+    prm, P = params(N, grav_spec=False, precision=prec, id_strings=True, id_walls=True, spec_est=True, num_bin=nb)
+    phi, _ = ic(N, 31, 2.0)  # a mask of ~25 % of the cells keeps every deconvolved bin positive
+    rng = np.random.default_rng(4)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    p0 = 8 * N ** 3
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        f_phi, f_pi = L.get_fields()
+        L.set_stream(7, 9, p0)
+        cnt = L.defects()
+        out = L.strings(0, 0)
+        rec = L.string_records()
+        p1 = L.stream_position()
+        s = L.field_spectrum(3.0)
+        cov, it = L.field_covariance(0, 4, 12, 0.0)
+        p2 = L.stream_position()
+    ns = int(cnt[0])
+    assert ns > 50 and p1 == p0 + 4004 * ns
+    assert it == -13 and p2 == p1 + 4 * N ** 3 * 12
+    oflags, _ = O.classify(f_phi)
+    orec, osums = O.strings(f_phi, f_pi, oflags, 0, 0, P["dspa"], stream=(7, 9, p0))
+    rel = np.abs(rec - orec) / np.maximum(1, np.abs(orec))
+    assert rel.max() < 1e-5
+    assert abs(out["length"] - osums[0]) <= 1e-9 * osums[0] and out["n_vel"] == osums[1]
+    prec_rec, _ = O.strings(f_phi, f_pi, oflags, 0, 0, P["dspa"])
+    assert np.abs(prec_rec - orec).max() > 1e-3  # the Philox starts give other fits
+    tab = O.init_spectrum(nb, P["dspa"])
+    pw, num = O.conv_spectrum(f_pi, oflags, nb, P["dspa"], P["dwav"], P["dv_spa"], P["dv_wav"])
+    _, inv, info = O.matrix_deconv(oflags, nb, P["dwav"], P["dv_spa"], P["dv_wav"], tab)
+    M = O.sym_from_packed(inv, nb) / np.outer(num, num)
+    ocov, oit, _ = O.field_covariance(oflags, nb, P["dspa"], P["dwav"], s["power"], packed(M), 0, 4, 12, 0.0,
+                                      stream=(7, 9, p1))
+    assert oit == it and relerr(cov, ocov) < 1e-6

@@ -1087,3 +1087,58 @@ def test_plaquettes_random_field_matches_phase_winding():
     assert np.all(cnt["net"] == 0)
     assert np.all(_gauss_residual(n) == 0)
     assert len(rec) == sum(cnt["n_pierced"])
+
+
+# ------------------------------------------------------------------ the reference's random stream
+
+def test_mcg59_matches_mkl_golden_vectors():
+    """The oracle's MCG59 against MKL's own draws (tests/golden/mcg59_mkl.npz, written by
+    make_mcg59.py from the MKL runtime): uniforms equal bit for bit from five seeds (including
+    the all-zero seed, which MKL starts at x = 1) and after five skip-ahead distances (up to
+    ~4e8 draws); Box-Muller Gaussians from the same positions agree to 5e-8 relative.  MKL
+    evaluates ln and sin to about 27 bits there (its draws differ from correctly rounded
+    sqrt(-2 ln u1) sin(2 pi u2) by <= 1.2e-8 relative; a swapped pair, cos for sin or another
+    pairing differs by O(1))."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "mcg59_mkl.npz"))
+    for i, (w0, w1) in enumerate(d["seeds"]):
+        for j, skip in enumerate(d["skips"]):
+            u = O.mcg59_uniforms(int(w0), int(w1), int(skip), d["uniform"].shape[2])
+            assert np.array_equal(u, d["uniform"][i, j]), (w0, w1, skip)
+            g = O.mcg59_gauss(int(w0), int(w1), int(skip), d["gauss"].shape[2])  # the skip counts draws
+            ref = d["gauss"][i, j]
+            assert np.all(np.abs(g - ref) <= 5e-8 * np.maximum(np.abs(ref), 1.0)), (w0, w1, skip)
+
+
+def test_mcg59_jump_is_the_recurrence():
+    """Skip-ahead by p equals p steps of x <- 13^13 x mod 2^59 (Python integers), and the seed is
+    (w0 + 2^32 w1) mod 2^59 with 0 mapped to 1."""
+    a, m = 13 ** 13, 2 ** 59
+    for w0, w1, p in ((3, 5, 0), (3, 5, 1), (99, 0, 977), (2 ** 32 - 1, 2 ** 31, 12345)):
+        x0 = (w0 + 2 ** 32 * w1) % m or 1
+        assert O.mcg59_seed(w0, w1) == x0
+        x = x0
+        for _ in range(p):
+            x = a * x % m
+        assert O.mcg59_jump(x0, p) == x
+    assert O.mcg59_seed(0, 2 ** 27) == 1  # 2^59 mod 2^59 = 0 -> 1
+
+
+def test_thermal_state_from_reference_stream_uses_its_draws():
+    """With the reference's stream the thermal state's mode lin of field ip holds dv_wav times the
+    Gaussians of draws 4 N^3 (ip - 1) + 4 lin .. +3 (initial.f90:102-126) times the Bose-Einstein
+    factor: one mode of each field rebuilt from the stream's own Gaussians, through the transform
+    of a single mode."""
+    N, tau = 8, 0.3
+    P = phys(N)
+    phi, pi = O.ic_thermal(N, 0, tau, P["sigma_cr"], P["dwav"], P["dv_wav"], stream=(11, 22, 0))
+    # back to Fourier space: FORWARD was applied, so the BACKWARD transform / N^3 inverts it
+    modes = [O.fft3d(f, +1) / N ** 3 for f in (phi, pi)]
+    for ip in (1, 2):
+        for lin in (0, 5, 77, N ** 3 - 1):
+            x, y, z = lin % N, (lin // N) % N, lin // N ** 2
+            nx, ny, nz = [v if v < N // 2 else v - N for v in (x, y, z)]
+            g = O.mcg59_gauss(11, 22, 4 * N ** 3 * (ip - 1) + 4 * lin, 2)
+            f = O.thermal_factor(ip, nx * nx + ny * ny + nz * nz, tau, P["sigma_cr"], P["dwav"], P["dv_wav"])
+            exp = (g[0] * P["dv_wav"]) * f + 1j * (g[1] * P["dv_wav"]) * f
+            got = modes[ip - 1][z, y, x]
+            assert abs(got - exp) <= 1e-12 * abs(exp), (ip, lin)
