@@ -253,16 +253,21 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
                        const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split,
                        double* rec, double* partial, int* npend, int64_t* pend, double* state);
 
-void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col);
+// Exclusive offsets of column col of per-block counts in[n][ncol] (ncol 3 or PLAQ_COLS) and the
+// column totals; part: scan_scratch(n, ncol) values of device scratch.
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col, int ncol,
+                        int64_t* part);
+int64_t scan_scratch(int64_t n, int ncol);
 
 // Plaquette windings (plaquettes.hip): per-site face codes (uint16: 3 bits of winding + 2 per face
 // o, then the quadrant-test bit of face o at 9 + o), per-block counts [nblk][16] (pierced, +1, -1,
 // quadrant, net per face, then the block's pierced total), their exclusive offsets of the
 // pierced totals and the 16 column totals.  The compaction writes rows (ix, iy, iz, o, n).
 constexpr int PLAQ_BLOCK = 256;
+constexpr int PLAQ_COLS = 16;  // per block: [pierced, plus, minus, quadrant, net] x 3, then pierced total
 template <typename T, bool COMP>
 void launch_plaquettes(hipStream_t s, int N, const void* phi, const void* cphi, uint16_t* code, int64_t* counts,
-                       int64_t* offs, int64_t* total);
+                       int64_t* offs, int64_t* total, int64_t* part);
 void launch_plaquette_compact(hipStream_t s, int N, const uint16_t* code, const int64_t* offs, int32_t* rec);
 
 template <typename T, bool COMP>

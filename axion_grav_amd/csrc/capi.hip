@@ -496,7 +496,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         h->work = (double2*)h->alloc(n3 * 16);
         h->flags = (int8_t*)h->alloc(n3);
         int64_t nblk = (n3 + 255) / 256;
-        h->blk = (int64_t*)h->alloc((3 * nblk + 8) * sizeof(int64_t) * 2);
+        // counts [nblk][3], offsets [nblk] + totals, string / wall offsets [nblk] + totals, scan scratch
+        h->blk = (int64_t*)h->alloc((6 * nblk + 16 + scan_scratch(nblk, 3)) * sizeof(int64_t));
         AXG_HIP(hipStreamSynchronize(h->stream));
     } catch (const Error& e) {
         g_create_err = e.msg;
@@ -993,7 +994,7 @@ extern "C" int axg_defects(axg_handle* h, axg_defect_out* out)
         else
             launch_classify<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, p.id_strings, p.id_walls,
                                            p.spec_est, h->flags, counts, nblk);
-        launch_scan_counts(h->stream, counts, offs, nblk, tot, 2);
+        launch_scan_counts(h->stream, counts, offs, nblk, tot, 2, 3, h->blk + 6 * nblk + 16);
         int64_t t[4];
         AXG_HIP(hipMemcpyAsync(t, tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
@@ -1017,18 +1018,20 @@ extern "C" int axg_plaquettes(axg_handle* h, axg_plaquette_out* out)
         const int64_t nblk = (h->n3 + PLAQ_BLOCK - 1) / PLAQ_BLOCK;
         if (!h->plaq_code) {
             h->plaq_code = (uint16_t*)h->alloc((size_t)h->n3 * sizeof(uint16_t));
-            h->plaq_cnt = (int64_t*)h->alloc((size_t)(17 * nblk + 16) * sizeof(int64_t));
+            h->plaq_cnt = (int64_t*)h->alloc((size_t)(17 * nblk + 16 + scan_scratch(nblk, PLAQ_COLS)) * sizeof(int64_t));
         }
         int64_t* counts = h->plaq_cnt;
         int64_t* offs = counts + 16 * nblk;
         int64_t* tot = offs + nblk;
         if (h->comp)
             launch_plaquettes<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->plaq_code, counts, offs,
-                                           tot);
+                                           tot, tot + 16);
         else if (h->fp32)
-            launch_plaquettes<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot);
+            launch_plaquettes<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot,
+                                            tot + 16);
         else
-            launch_plaquettes<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot);
+            launch_plaquettes<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->plaq_code, counts, offs, tot,
+                                            tot + 16);
         int64_t t[16];
         AXG_HIP(hipMemcpyAsync(t, tot, sizeof(t), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
@@ -1119,7 +1122,7 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
             int64_t* counts = h->blk;               // [nblk][3] from the classification
             int64_t* soffs = h->blk + 4 * nblk + 8;  // [nblk] exclusive string offsets
             int64_t* stot = soffs + nblk;            // [4]
-            launch_scan_counts(h->stream, counts, soffs, nblk, stot, 0);
+            launch_scan_counts(h->stream, counts, soffs, nblk, stot, 0, 3, h->blk + 6 * nblk + 16);
             launch_cell_compact(h->stream, h->N, h->flags, 1, soffs, h->str_cells, 0, nblk);
             double* dsum = h->str_part + 5 * nsb;
             const Draws dr = h->draws(seed0, seed1);
@@ -1179,7 +1182,7 @@ extern "C" int axg_walls(axg_handle* h, axg_wall_out* out)
             int64_t* counts = h->blk;
             int64_t* woffs = h->blk + 4 * nblk + 8;  // shares the string offsets' room
             int64_t* wtot = woffs + nblk;
-            launch_scan_counts(h->stream, counts, woffs, nblk, wtot, 1);
+            launch_scan_counts(h->stream, counts, woffs, nblk, wtot, 1, 3, h->blk + 6 * nblk + 16);
             launch_cell_compact(h->stream, h->N, h->flags, 2, woffs, h->wall_cells, 0, nblk);
             double* dsum = h->wall_part + 2 * nwb;
             if (h->comp)

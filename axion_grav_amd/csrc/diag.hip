@@ -103,13 +103,21 @@ void launch_energy(hipStream_t s, int N, const void* phi, const void* cphi, cons
 
 __device__ __forceinline__ int id_with_phase(double* p, int id_strings, int id_walls, int spec_est)
 {
-    // defects_3d.f90:309-349 (bubble sort as there; same ordered result as any sort)
-    for (int i = 7; i >= 1; i--)
-        for (int j = 0; j < i; j++)
-            if (p[j] > p[j + 1]) { double t = p[j]; p[j] = p[j + 1]; p[j + 1] = t; }
+    // defects_3d.f90:309-349 sorts the eight phases by bubble sort; Batcher's odd-even merge
+    // network (19 min/max pairs) gives the same ordered list (the phases are never NaN, and a
+    // swapped +0 / -0 pair changes no comparison below).
+    constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+    for (int k = 0; k < 19; k++) {
+        const double a = p[net[k][0]], b = p[net[k][1]];
+        p[net[k][0]] = fmin(a, b);
+        p[net[k][1]] = fmax(a, b);
+    }
     int flag = 0;
     double t = p[0] - p[7] + AXG_TWOPI_T;
     double tmax = p[1] - p[0];
+#pragma unroll
     for (int i = 2; i < 8; i++) { double d = p[i] - p[i - 1]; tmax = d > tmax ? d : tmax; }
     if (id_walls && tmax > t) flag += 2;
     else tmax = tmax > t ? tmax : t;
@@ -119,35 +127,88 @@ __device__ __forceinline__ int id_with_phase(double* p, int id_strings, int id_w
 }
 
 template <typename T, bool COMP>
-__global__ void __launch_bounds__(256) classify_kernel(int N, const typename V2<T>::type* __restrict__ phi,
-                                                       const short2* __restrict__ cphi, int id_strings, int id_walls,
-                                                       int spec_est, int8_t* __restrict__ flags,
-                                                       int64_t* __restrict__ counts)
+__device__ __forceinline__ double site_phase(const typename V2<T>::type* __restrict__ phi,
+                                             const short2* __restrict__ cphi, int64_t i)
+{
+    const double2 v = ldf<T, COMP>(phi, cphi, i);
+    return atan2(v.y, v.x);  // aimag(log(phi)), defects_3d.f90:92
+}
+
+// A block classifies the cells of a CL_TX x CL_TY tile of (x, y) over zc consecutive planes,
+// marching in z.  The phases of one site plane of the tile, (CL_TX+1) x (CL_TY+1) with the
+// periodic upper neighbours, sit in LDS; each step computes the next plane's phases once and
+// classifies the cells between the two planes, so a site's atan2 is evaluated about
+// (1 + 1/CL_TX)(1 + 1/CL_TY)(1 + 1/zc) times instead of 8.
+constexpr int CL_TX = 32, CL_TY = 8, CL_NT = CL_TX * CL_TY, CL_ZC = 32;
+
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(CL_NT) classify_kernel(int N, int zc, const typename V2<T>::type* __restrict__ phi,
+                                                         const short2* __restrict__ cphi, int id_strings,
+                                                         int id_walls, int spec_est, int8_t* __restrict__ flags)
 {
 #pragma clang fp contract(off)
-    __shared__ int sc[3];
-    if (threadIdx.x < 3) sc[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t NN = (int64_t)N * N, n3 = NN * N;
-    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
-    if (i < n3) {
-        int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
-        int x2 = x + 1 == N ? 0 : x + 1, y2 = y + 1 == N ? 0 : y + 1, z2 = z + 1 == N ? 0 : z + 1;
-        double ph[8];
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            int xx = (c & 1) ? x2 : x, yy = (c & 2) ? y2 : y, zz = (c & 4) ? z2 : z;
-            double2 v = ldf<T, COMP>(phi, cphi, (int64_t)zz * NN + (int64_t)yy * N + xx);
-            ph[c] = atan2(v.y, v.x);  // aimag(log(phi)), defects_3d.f90:92
+    __shared__ double ph[2][CL_TY + 1][CL_TX + 1];
+    const int x0 = blockIdx.x * CL_TX, y0 = blockIdx.y * CL_TY, z0 = blockIdx.z * zc;
+    const int nx = min(CL_TX, N - x0), ny = min(CL_TY, N - y0), nz = min(zc, N - z0);
+    const int64_t NN = (int64_t)N * N;
+    const int t = threadIdx.x, lx = t % CL_TX, ly = t / CL_TX;
+    auto load_plane = [&](int buf, int z) {
+        for (int k = t; k < (CL_TY + 1) * (CL_TX + 1); k += CL_NT) {
+            const int py = k / (CL_TX + 1), px = k - py * (CL_TX + 1);
+            if (px <= nx && py <= ny) {
+                const int gx = x0 + px == N ? 0 : x0 + px, gy = y0 + py == N ? 0 : y0 + py;
+                ph[buf][py][px] = site_phase<T, COMP>(phi, cphi, z * NN + (int64_t)gy * N + gx);
+            }
         }
-        int fl = id_with_phase(ph, id_strings, id_walls, spec_est);
-        flags[i] = (int8_t)fl;
-        if (fl & 1) atomicAdd(&sc[0], 1);
-        if (fl & 2) atomicAdd(&sc[1], 1);
-        if (fl & 4) atomicAdd(&sc[2], 1);
+    };
+    load_plane(0, z0);
+    for (int k = 0; k < nz; k++) {
+        const int z = z0 + k, a = k & 1, b = a ^ 1;
+        load_plane(b, z + 1 == N ? 0 : z + 1);
+        __syncthreads();
+        if (lx < nx && ly < ny) {
+            // corner c has offsets (bit0, bit1, bit2) in (x, y, z), defects_3d.f90:38-40
+            double p[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) p[c] = ph[(c & 4) ? b : a][ly + ((c >> 1) & 1)][lx + (c & 1)];
+            flags[z * NN + (int64_t)(y0 + ly) * N + x0 + lx] = (int8_t)id_with_phase(p, id_strings, id_walls, spec_est);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < 3) counts[blockIdx.x * 3 + threadIdx.x] = sc[threadIdx.x];
+}
+
+// Per 256-cell block of the site order (the granularity of cell_compact_kernel), the number of
+// cells with flag bit 1, 2 and 4; one wave per block, the three counts packed 10 bits apart.
+__global__ void __launch_bounds__(256) flag_count_kernel(int64_t n3, int64_t nblk, const int8_t* __restrict__ flags,
+                                                         int64_t* __restrict__ counts)
+{
+    const int64_t b = blockIdx.x * (int64_t)4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (b >= nblk) return;  // uniform per wave
+    int packed = 0;
+    if (b * 256 + 256 <= n3) {  // a whole block: four flags per lane in one load
+        const uint32_t f4 = *(const uint32_t*)(flags + b * 256 + 4 * lane);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t f = f4 >> (8 * k);
+            packed += (f & 1) | (((f >> 1) & 1) << 10) | (((f >> 2) & 1) << 20);
+        }
+    } else {
+        for (int k = 0; k < 4; k++) {
+            const int64_t i = b * 256 + k * 64 + lane;
+            if (i < n3) {
+                const int f = flags[i];
+                packed += (f & 1) | (((f >> 1) & 1) << 10) | (((f >> 2) & 1) << 20);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) packed += __shfl_xor(packed, o);
+    if (lane == 0) {
+        counts[b * 3 + 0] = packed & 1023;
+        counts[b * 3 + 1] = (packed >> 10) & 1023;
+        counts[b * 3 + 2] = packed >> 20;
+    }
 }
 
 template <typename T, bool COMP>
@@ -155,54 +216,138 @@ void launch_classify(hipStream_t s, int N, const void* phi, const void* cphi, in
                      int8_t* flags, int64_t* block_counts, int64_t nblk)
 {
     using T2 = typename V2<T>::type;
-    hipLaunchKernelGGL((classify_kernel<T, COMP>), dim3((unsigned)nblk), dim3(256), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, id_strings, id_walls, spec_est, flags, block_counts);
+    const int zc = min(N, CL_ZC);
+    const dim3 grid((unsigned)((N + CL_TX - 1) / CL_TX), (unsigned)((N + CL_TY - 1) / CL_TY), (unsigned)((N + zc - 1) / zc));
+    hipLaunchKernelGGL((classify_kernel<T, COMP>), grid, dim3(CL_NT), 0, s, N, zc, (const T2*)phi, (const short2*)cphi,
+                       id_strings, id_walls, spec_est, flags);
+    AXG_HIP(hipGetLastError());
+    const int64_t n3 = (int64_t)N * N * N;
+    hipLaunchKernelGGL(flag_count_kernel, dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, n3, nblk,
+                       (const int8_t*)flags, block_counts);
     AXG_HIP(hipGetLastError());
 }
 
-// Exclusive scan of the per-block counts of column col of in[nblk][3] (0 strings, 1 walls,
-// 2 mask) and the totals of all three columns, by one block of 1024 threads over contiguous
-// chunks.
-__global__ void __launch_bounds__(1024) scan_counts_kernel(const int64_t* __restrict__ in, int64_t* __restrict__ out,
-                                                           int64_t n, int64_t* __restrict__ total, int col)
+// Exclusive scan of column col of the per-block counts in[n][C] and the totals of all C columns
+// (the classifier's C = 3: strings, walls, mask; the plaquettes' C = 16), over the whole chip:
+// one block per SCAN_ROWS rows sums its rows, one block scans those partial sums, and every
+// block then scans its own rows from its partial's offset.  Integer sums, so the result does
+// not depend on the split.  part holds (C + 1) * ceil(n / SCAN_ROWS) values.
+constexpr int SCAN_ROWS = 1024;
+
+template <int C>
+__global__ void __launch_bounds__(256) count_partials_kernel(const int64_t* __restrict__ in, int64_t n,
+                                                             int64_t* __restrict__ part)
+{
+    __shared__ int64_t sh[4][C];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int64_t s[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) s[c] = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ROWS / 256; k++) {
+        const int64_t r = blockIdx.x * (int64_t)SCAN_ROWS + k * 256 + t;
+        if (r < n)
+#pragma unroll
+            for (int c = 0; c < C; c++) s[c] += in[r * C + c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[c] += __shfl_xor(s[c], o);
+        if (lane == 0) sh[w][c] = s[c];
+    }
+    __syncthreads();
+    if (t < C) part[blockIdx.x * (int64_t)C + t] = sh[0][t] + sh[1][t] + sh[2][t] + sh[3][t];
+}
+
+template <int C>
+__global__ void __launch_bounds__(1024) partials_scan_kernel(const int64_t* __restrict__ part, int64_t G, int col,
+                                                             int64_t* __restrict__ poff, int64_t* __restrict__ total)
 {
     __shared__ int64_t sh[1024];
-    __shared__ int64_t tot[3][1024];
     const int t = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b0 = t * chunk, b1 = min(n, b0 + chunk);
-    int64_t s = 0, s0 = 0, s1 = 0;
-    int64_t s2 = 0;
-    for (int64_t b = b0; b < b1; b++) {
-        s += in[b * 3 + col];
-        s0 += in[b * 3 + 0];
-        s1 += in[b * 3 + 1];
-        s2 += in[b * 3 + 2];
+    const int64_t chunk = (G + 1023) / 1024, g0 = t * chunk, g1 = min(G, g0 + chunk);
+    for (int c = 0; c < C; c++) {
+        int64_t s = 0;
+        for (int64_t g = g0; g < g1; g++) s += part[g * C + c];
+        sh[t] = s;
+        __syncthreads();
+        for (int h = 512; h > 0; h >>= 1) {
+            if (t < h) sh[t] += sh[t + h];
+            __syncthreads();
+        }
+        if (t == 0) total[c] = sh[0];
+        __syncthreads();
     }
+    int64_t s = 0;
+    for (int64_t g = g0; g < g1; g++) s += part[g * C + col];
     sh[t] = s;
-    tot[0][t] = s0;
-    tot[1][t] = s1;
-    tot[2][t] = s2;
     __syncthreads();
-    if (t == 0) {
-        int64_t run = 0;
-        for (int i = 0; i < 1024; i++) { int64_t v = sh[i]; sh[i] = run; run += v; }
-        int64_t a = 0, b = 0, c = 0;
-        for (int i = 0; i < 1024; i++) { a += tot[0][i]; b += tot[1][i]; c += tot[2][i]; }
-        total[0] = a; total[1] = b; total[2] = c; total[3] = 0;
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+        const int64_t v = t >= d ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
     }
-    __syncthreads();
-    int64_t run = sh[t];
-    for (int64_t b = b0; b < b1; b++) {
-        out[b] = run;
-        run += in[b * 3 + col];
+    int64_t run = sh[t] - s;
+    for (int64_t g = g0; g < g1; g++) {
+        poff[g] = run;
+        run += part[g * C + col];
     }
 }
 
-void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col)
+template <int C>
+__global__ void __launch_bounds__(256) scan_rows_kernel(const int64_t* __restrict__ in, int64_t n, int col,
+                                                        const int64_t* __restrict__ poff, int64_t* __restrict__ out)
 {
-    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, in, out, n, total, col);
+    __shared__ int64_t ws[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t r0 = blockIdx.x * (int64_t)SCAN_ROWS + 4 * t;  // four consecutive rows per thread
+    int64_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[k] = r0 + k < n ? in[(r0 + k) * C + col] : 0;
+        s += v[k];
+    }
+    int64_t inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t u = __shfl_up(inc, d);
+        if (lane >= d) inc += u;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    int64_t run = poff[blockIdx.x] + inc - s;
+    for (int k = 0; k < w; k++) run += ws[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (r0 + k < n) out[r0 + k] = run;
+        run += v[k];
+    }
+}
+
+int64_t scan_scratch(int64_t n, int ncol) { return (ncol + 1) * ((n + SCAN_ROWS - 1) / SCAN_ROWS); }
+
+template <int C>
+static void scan_counts_c(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col,
+                          int64_t* part)
+{
+    const int64_t G = (n + SCAN_ROWS - 1) / SCAN_ROWS;
+    int64_t* poff = part + C * G;
+    hipLaunchKernelGGL(count_partials_kernel<C>, dim3((unsigned)G), dim3(256), 0, s, in, n, part);
     AXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(partials_scan_kernel<C>, dim3(1), dim3(1024), 0, s, (const int64_t*)part, G, col, poff, total);
+    AXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scan_rows_kernel<C>, dim3((unsigned)G), dim3(256), 0, s, in, n, col, (const int64_t*)poff, out);
+    AXG_HIP(hipGetLastError());
+}
+
+void launch_scan_counts(hipStream_t s, const int64_t* in, int64_t* out, int64_t n, int64_t* total, int col, int ncol,
+                        int64_t* part)
+{
+    if (ncol == 3) scan_counts_c<3>(s, in, out, n, total, col, part);
+    else if (ncol == PLAQ_COLS) scan_counts_c<PLAQ_COLS>(s, in, out, n, total, col, part);
+    else throw std::runtime_error("scan over an unsupported column count");
 }
 
 // Cells of block b with flag bit `bit` go to offs[b] + (rank among the block's such cells), so

@@ -15,7 +15,6 @@
 namespace axg {
 
 constexpr int PLAQ_NT = PLAQ_BLOCK;
-constexpr int PLAQ_COLS = 16;  // per block: [pierced, plus, minus, quadrant, net] x 3, then pierced total
 
 // Crossing of the negative real axis by the shorter arc from arg a to arg b: +1 counterclockwise,
 // -1 clockwise, 0 otherwise.  The cut belongs to the upper side (Im z >= 0, as atan2 = pi).
@@ -86,53 +85,6 @@ __global__ void __launch_bounds__(PLAQ_NT) plaquette_kernel(int N, const typenam
     if (threadIdx.x < PLAQ_COLS) counts[blockIdx.x * (int64_t)PLAQ_COLS + threadIdx.x] = sc[threadIdx.x];
 }
 
-// Exclusive offsets of the per-block pierced totals (column 15) and the totals of all columns,
-// by one block of 1024 threads over contiguous chunks of blocks, then fixed-shape trees
-// (deterministic).
-__global__ void __launch_bounds__(1024) plaquette_scan_kernel(const int64_t* __restrict__ counts, int64_t nblk,
-                                                              int64_t* __restrict__ offs, int64_t* __restrict__ total)
-{
-    __shared__ int64_t sh[1024];
-    const int t = threadIdx.x;
-    const int64_t chunk = (nblk + 1023) / 1024;
-    const int64_t b0 = t * chunk, b1 = min(nblk, b0 + chunk);
-    int64_t part[PLAQ_COLS];
-#pragma unroll
-    for (int c = 0; c < PLAQ_COLS; c++) part[c] = 0;
-    for (int64_t b = b0; b < b1; b++)
-#pragma unroll
-        for (int c = 0; c < PLAQ_COLS; c++) part[c] += counts[b * PLAQ_COLS + c];
-    // column totals
-#pragma unroll
-    for (int c = 0; c < PLAQ_COLS; c++) {
-        sh[t] = part[c];
-        __syncthreads();
-        for (int h = 512; h > 0; h >>= 1) {
-            if (t < h) sh[t] += sh[t + h];
-            __syncthreads();
-        }
-        if (t == 0) total[c] = sh[0];
-        __syncthreads();
-    }
-    // exclusive scan of the chunk totals of column 15
-    sh[t] = part[15];
-    __syncthreads();
-    if (t == 0) {
-        int64_t run = 0;
-        for (int k = 0; k < 1024; k++) {
-            const int64_t v = sh[k];
-            sh[k] = run;
-            run += v;
-        }
-    }
-    __syncthreads();
-    int64_t run = sh[t];
-    for (int64_t b = b0; b < b1; b++) {
-        offs[b] = run;
-        run += counts[b * PLAQ_COLS + 15];
-    }
-}
-
 // Pierced faces in site order, then face order: rows (ix, iy, iz, o, n), 1-based site and o.
 __global__ void __launch_bounds__(PLAQ_NT) plaquette_compact_kernel(int N, const uint16_t* __restrict__ code,
                                                                     const int64_t* __restrict__ offs,
@@ -178,15 +130,15 @@ __global__ void __launch_bounds__(PLAQ_NT) plaquette_compact_kernel(int N, const
 
 template <typename T, bool COMP>
 void launch_plaquettes(hipStream_t s, int N, const void* phi, const void* cphi, uint16_t* code, int64_t* counts,
-                       int64_t* offs, int64_t* total)
+                       int64_t* offs, int64_t* total, int64_t* part)
 {
     using T2 = typename V2<T>::type;
     const int64_t n3 = (int64_t)N * N * N, nblk = (n3 + PLAQ_NT - 1) / PLAQ_NT;
     hipLaunchKernelGGL((plaquette_kernel<T, COMP>), dim3((unsigned)nblk), dim3(PLAQ_NT), 0, s, N, (const T2*)phi,
                        (const short2*)cphi, code, counts);
     AXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(plaquette_scan_kernel, dim3(1), dim3(1024), 0, s, (const int64_t*)counts, nblk, offs, total);
-    AXG_HIP(hipGetLastError());
+    // offsets of the pierced totals (column 15) and the totals of all 16 columns
+    launch_scan_counts(s, counts, offs, nblk, total, 15, PLAQ_COLS, part);
 }
 
 void launch_plaquette_compact(hipStream_t s, int N, const uint16_t* code, const int64_t* offs, int32_t* rec)
@@ -197,10 +149,10 @@ void launch_plaquette_compact(hipStream_t s, int N, const uint16_t* code, const 
 }
 
 template void launch_plaquettes<double, false>(hipStream_t, int, const void*, const void*, uint16_t*, int64_t*,
-                                               int64_t*, int64_t*);
+                                               int64_t*, int64_t*, int64_t*);
 template void launch_plaquettes<float, false>(hipStream_t, int, const void*, const void*, uint16_t*, int64_t*,
-                                              int64_t*, int64_t*);
+                                              int64_t*, int64_t*, int64_t*);
 template void launch_plaquettes<float, true>(hipStream_t, int, const void*, const void*, uint16_t*, int64_t*,
-                                             int64_t*, int64_t*);
+                                             int64_t*, int64_t*, int64_t*);
 
 }  // namespace axg
