@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libaxgrav.so")
-SOURCES = ["capi.hip", "step.hip", "gw.hip", "diag.hip", "strings.hip", "walls.hip", "groups.hip", "adjoint.hip", "noalloc.hip", "probe.hip", "plaquettes.hip"]
+SOURCES = ["capi.hip", "step.hip", "gw.hip", "diag.hip", "strings.hip", "walls.hip", "groups.hip", "adjoint.hip", "noalloc.hip", "probe.hip", "plaquettes.hip", "degrade.hip"]
 FORTRAN = ["axgrav_capi.f90", "axg_ini.f90", "axg_driver.f90"]
 BINDIR = os.path.join(HERE, "bin")
 FC = os.environ.get("AXG_FC", "/opt/rocm/bin/amdflang")

@@ -44,7 +44,8 @@ class Params(C.Structure):
         ("eps_dtau", C.c_double),
         ("dtau_fixed", C.c_double),
         ("potential_kappa0", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("degrade_spec", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -199,8 +200,10 @@ def hbm_probe(nbytes=4 << 30, reps=10, device=0):
 
 def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initial_loop=True, kappa=1e-30,
                 num_bin_grav=25, num_bin=50, id_strings=True, id_walls=True, spec_est=True, tau_cr=1.0,
-                a_now=1.0, eps_dtau=0.0, dtau_fixed=0.0, gw_fix_swap=False, potential_kappa0=0.0, device=0):
-    """Params from a lattice spacing and sigma_cr (dwav, dv_spa, dv_wav as settings.f90:295-297)."""
+                a_now=1.0, eps_dtau=0.0, dtau_fixed=0.0, gw_fix_swap=False, potential_kappa0=0.0, degrade_spec=0,
+                device=0):
+    """Params from a lattice spacing and sigma_cr (dwav, dv_spa, dv_wav as settings.f90:295-297).
+    degrade_spec > 0 puts the GW source and the axion spectrum on the grid ngrid / 2^degrade_spec."""
     twopi_t = 2 * 3.14159265359
     P = Params()
     P.ngrid = int(ngrid)
@@ -224,6 +227,7 @@ def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initia
     P.eps_dtau = eps_dtau
     P.dtau_fixed = dtau_fixed
     P.potential_kappa0 = potential_kappa0
+    P.degrade_spec = int(degrade_spec)
     return P
 
 
@@ -306,7 +310,9 @@ class Lattice:
         self._check(lib().axg_init_thermal(self._h, seed, tau_ini), "axg_init_thermal")
 
     def gw_state(self):
-        N = self.N
+        """The four GW accumulators on the half spectrum of the GW grid (the lattice's, or
+        ngrid / 2^degrade_spec): (4, Ng, Ng, Ng/2 + 1) complex."""
+        N = self.N >> int(self.params.degrade_spec)
         A = np.empty((4, N, N, N // 2 + 1), np.complex128)
         self._check(lib().axg_get_gw_state(self._h, _ptr(A), A.size), "axg_get_gw_state")
         return A

@@ -270,6 +270,15 @@ void launch_plaquettes(hipStream_t s, int N, const void* phi, const void* cphi, 
                        int64_t* offs, int64_t* total, int64_t* part);
 void launch_plaquette_compact(hipStream_t s, int N, const uint16_t* code, const int64_t* offs, int32_t* rec);
 
+// The reduced grid of degrade_spec > 0 (degrade.hip), deg = 2^degrade_spec: the block sums of
+// the lattice gradient into the packed source planes of the N/deg grid, the block mean of a
+// field, and the block's mask bit.
+template <typename T, bool COMP>
+void launch_degrade_source(hipStream_t s, int N, int deg, const void* phi, const void* cphi, double dspa, double2* src);
+template <typename T, bool COMP>
+void launch_degrade_field(hipStream_t s, int N, int deg, const void* f, const void* cf, double2* out);
+void launch_degrade_mask(hipStream_t s, int N, int deg, const int8_t* flags, int8_t* out);
+
 template <typename T, bool COMP>
 void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags,
                          double2* out, double fill_unmasked, bool use_pi);

@@ -105,6 +105,11 @@ struct axg_handle {
     bool fp32 = false, comp = false;
     size_t t2 = 16;   // bytes of one stored complex
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false for the reduced-grid handle, which runs on its lattice's stream
+    // degrade_spec > 0: deg = 2^degrade_spec and the handle of the reduced grid N/deg that holds
+    // the GW state and the axion spectrum machinery (FP64, sharing this handle's stream)
+    int deg = 1;
+    axg_handle* gs = nullptr;
     FftPlan plan{};
     std::vector<DevBuf> bufs;
     void* phi[2] = {nullptr, nullptr};
@@ -256,11 +261,16 @@ struct axg_handle {
     }
     void release()
     {
+        if (gs) {
+            gs->release();
+            delete gs;
+            gs = nullptr;
+        }
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
         for (auto& b : bufs) (void)hipFree(b.p);
         bufs.clear();
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
 };
@@ -402,6 +412,41 @@ static double integrate_p(int nint_, double li, double si, double lj, double sj,
 
 // ------------------------------------------------------------------ create / destroy
 
+// The reduced grid of degrade_spec > 0 as init_grav and init_spectrum set it up
+// (grav_waves.f90:37-45, spectrum.f90:42-49): ngrid / deg sites per axis of spacing dspa * deg,
+// the wavenumber unit and cell volumes from those in the reference's expressions.
+static axg_params reduced_params(const axg_params& p)
+{
+    axg_params q = p;
+    const int deg = 1 << p.degrade_spec;
+    q.ngrid = p.ngrid / deg;
+    q.precision = AXG_FP64;
+    q.degrade_spec = 0;
+    q.dspa = p.dspa * deg;
+    q.dwav = AXG_TWOPI_T / q.dspa / (double)q.ngrid;
+    q.dv_spa = q.dspa * q.dspa * q.dspa;
+    const double r = q.dwav / AXG_TWOPI_T;
+    q.dv_wav = r * r * r;
+    return q;
+}
+
+static void make_reduced_grid(axg_handle* h)
+{
+    const axg_params q = reduced_params(h->p);
+    axg_handle* g = nullptr;
+    const int rc = axg_create(&q, &g);
+    if (rc != AXG_OK) throw Error(rc, "reduced grid: " + g_create_err);
+    h->gs = g;
+    // the packed source planes the degraded gradient fills (the x transform is its own pass)
+    g->fuse_x = false;
+    if (q.grav_spec && !g->src) g->src = g->alloc(3 * g->n3 * g->t2);
+    // one stream for both grids, so the coarse passes follow the lattice's kernels in order
+    AXG_HIP(hipStreamSynchronize(g->stream));
+    AXG_HIP(hipStreamDestroy(g->stream));
+    g->stream = h->stream;
+    g->own_stream = false;
+}
+
 extern "C" int axg_create(const axg_params* pin, axg_handle** out)
 {
     g_create_err.clear();
@@ -417,6 +462,9 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
                 "precision must be AXG_FP64 or AXG_FP32C");
         require(p.dspa > 0 && p.dwav > 0 && p.dv_spa > 0 && p.dv_wav > 0 && p.sigma_cr > 0, "derived units missing");
         require(p.num_bin_grav >= 1 && p.num_bin_grav <= 4096, "num_bin_grav out of range");
+        require(p.degrade_spec >= 0 && p.degrade_spec <= 7 && supported_n(p.ngrid >> p.degrade_spec) &&
+                    ((p.ngrid >> p.degrade_spec) << p.degrade_spec) == p.ngrid,
+                "degrade_spec must leave an even 2^a 3^b grid of at least 8 (ngrid / 2^degrade_spec)");
         require(!p.spec_est || (p.num_bin >= 1 && p.num_bin <= 1024), "num_bin out of range");
         AXG_HIP(hipSetDevice(p.device));
         h->N = (int)p.ngrid;
@@ -441,6 +489,13 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         if (const char* e = getenv("AXG_YPASS_VARIANT")) h->ypass_variant = atoi(e);
         // AXG_STRING_SPLIT: simplex iterations of the string fits' first kernel (same records)
         if (const char* e = getenv("AXG_STRING_SPLIT")) h->string_split = std::max(0, atoi(e));
+        // degrade_spec > 0: the GW passes run on the reduced grid's handle (made below); the
+        // lattice's own step is the plain kick + drift
+        const bool reduced = p.degrade_spec > 0 && (p.grav_spec || p.spec_est);
+        if (reduced) {
+            h->deg = 1 << p.degrade_spec;
+            h->fuse_x = false;
+        }
         AXG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         const int64_t n3 = h->n3;
         for (int i = 0; i < 2; i++) h->phi[i] = h->alloc(n3 * h->t2);
@@ -449,7 +504,7 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
             for (int i = 0; i < 2; i++) h->cphi[i] = h->alloc(n3 * 4);
             h->cpi = h->alloc(n3 * 4);
         }
-        if (p.grav_spec) {
+        if (p.grav_spec && !reduced) {
             // the packed source planes only exist when the x transform is not fused
             if (!p2_size(N) || !h->fuse_x) h->src = h->alloc(3 * n3 * h->t2);
             h->L = gw_layout(N, h->fp32);
@@ -498,6 +553,7 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         int64_t nblk = (n3 + 255) / 256;
         // counts [nblk][3], offsets [nblk] + totals, string / wall offsets [nblk] + totals, scan scratch
         h->blk = (int64_t*)h->alloc((6 * nblk + 16 + scan_scratch(nblk, 3)) * sizeof(int64_t));
+        if (reduced) make_reduced_grid(h.get());
         AXG_HIP(hipStreamSynchronize(h->stream));
     } catch (const Error& e) {
         g_create_err = e.msg;
@@ -587,9 +643,16 @@ extern "C" int axg_get_fields(axg_handle* h, double* phi, double* pi, int64_t ns
 
 // ------------------------------------------------------------------ GW state
 
+// The result of an entry point run on the reduced grid's handle, reported as h's.
+static void forward(axg_handle* h, int rc)
+{
+    if (rc != AXG_OK) throw Error(rc, h->gs->err);
+}
+
 extern "C" int axg_get_gw_state(axg_handle* h, double* A, int64_t count)
 {
     return guarded(h, [&] {
+        if (h->gs) return forward(h, axg_get_gw_state(h->gs, A, count));
         const int N = h->N;
         require(h->A != nullptr, "grav_spec is off: no GW state");
         require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
@@ -606,6 +669,7 @@ extern "C" int axg_get_gw_state(axg_handle* h, double* A, int64_t count)
 extern "C" int axg_set_gw_state(axg_handle* h, const double* A, int64_t count)
 {
     return guarded(h, [&] {
+        if (h->gs) return forward(h, axg_set_gw_state(h->gs, A, count));
         const int N = h->N;
         require(h->A != nullptr, "grav_spec is off: no GW state");
         require(A && count == (int64_t)4 * N * N * h->nh, "count must be 4*N*N*(N/2+1)");
@@ -620,6 +684,31 @@ extern "C" int axg_set_gw_state(axg_handle* h, const double* A, int64_t count)
 }
 
 // ------------------------------------------------------------------ time stepping
+
+// GW sourcing on the reduced grid (degrade_spec > 0; grav_waves.f90:202-206, 358-507) for the
+// kick at t with step dt: the block sums of the gradient of phi at t (phi[cur], which the
+// out-of-place kick + drift has not overwritten) into the coarse handle's packed source planes,
+// then the coarse grid's x, y and z passes with dw = tau dtau dv_spa_grav sigma_cr^2.
+template <typename T, bool COMP>
+static void reduced_gw_step(axg_handle* h, double t, double dt)
+{
+    axg_handle* g = h->gs;
+    h->timed(2, [&] {
+        launch_degrade_source<T, COMP>(h->stream, h->N, h->deg, h->phi[h->cur], h->cphi[h->cur], h->p.dspa,
+                                       (double2*)g->src);
+        launch_gw_xpass<double>(g->stream, g->N, g->P, g->src, g->W, g->plan, g->tw);
+    });
+    h->timed(3, [&] {
+        launch_gw_ypass<double>(g->stream, g->N, g->P, g->W, g->W2, g->L, g->plan, g->tw, g->ypass_variant);
+    });
+    GwStepScalars s;
+    s.tau = t;
+    s.dw = t * dt * g->p.dv_spa * (g->p.sigma_cr * g->p.sigma_cr);
+    s.dwav = g->p.dwav;
+    s.fix_swap = g->p.gw_fix_swap;
+    const void* wz = g->L.zt ? g->W2 : g->W;
+    h->timed(4, [&] { launch_gw_zpass_acc<double>(g->stream, g->N, g->P, wz, g->A, g->L, g->plan, g->tw, s); });
+}
 
 // The DKD loop; before_kick(s, t) runs before kick s (0-based) at its time t, when phi has
 // reached t and pi is still the previous kick's.
@@ -643,7 +732,8 @@ static void run_steps_hooked(axg_handle* h, double t1, double dt, int32_t nt, Ho
         double dtd = (it == nt) ? dt / 2 : dt;
         int nxt = 1 - h->cur;
         bool fused = false;
-        if (p.grav_spec && h->fuse_x) {
+        const bool lattice_gw = p.grav_spec && !h->gs;  // GW sourced on the lattice's own grid
+        if (lattice_gw && h->fuse_x) {
             h->timed(1, [&] {
                 fused = launch_kick_source_xfft<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi,
                                                          h->cphi[h->cur], h->cphi[nxt], h->cpi, h->W, h->P, h->tw, k,
@@ -653,10 +743,11 @@ static void run_steps_hooked(axg_handle* h, double t1, double dt, int32_t nt, Ho
         if (!fused) {
             h->timed(1, [&] {
                 launch_kick_drift<T, COMP>(h->stream, N, h->phi[h->cur], h->phi[nxt], h->pi, h->cphi[h->cur],
-                                           h->cphi[nxt], h->cpi, h->src, k, p.dspa, dtd, p.grav_spec != 0);
+                                           h->cphi[nxt], h->cpi, h->src, k, p.dspa, dtd, lattice_gw);
             });
         }
-        if (p.grav_spec) {
+        if (p.grav_spec && h->gs) reduced_gw_step<T, COMP>(h, t, dt);
+        if (lattice_gw) {
             if (!fused) h->timed(2, [&] { launch_gw_xpass<T>(h->stream, N, h->P, h->src, h->W, h->plan, h->tw); });
             h->timed(3, [&] {
                 launch_gw_ypass<T>(h->stream, N, h->P, h->W, h->W2, h->L, h->plan, h->tw, h->ypass_variant);
@@ -909,6 +1000,7 @@ extern "C" int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_
 extern "C" int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, double* numer, double* denom, int32_t nbins)
 {
     return guarded(h, [&] {
+        if (h->gs) return forward(h, axg_gw_spectrum(h->gs, qmid, power, numer, denom, nbins));
         const axg_params& p = h->p;
         const int nb = p.num_bin_grav;
         require(h->A != nullptr, "grav_spec is off: no GW state");
@@ -1462,6 +1554,20 @@ extern "C" int axg_field_spectrum(axg_handle* h, double tau, double* qmid, doubl
         require(p.spec_est, "spec_est is off");
         require(nbins == nb, "nbins must equal num_bin");
         require(h->have_flags, "call axg_defects first (the mask)");
+        if (h->gs) {
+            // compute_conv_spec on the reduced grid: the block means of pi
+            // (data2_to_data_spec_degrade) and the blocks' mask, then the same spectrum there
+            axg_handle* g = h->gs;
+            if (h->comp) launch_degrade_field<float, true>(h->stream, h->N, h->deg, h->pi, h->cpi, (double2*)g->pi);
+            else if (h->fp32) launch_degrade_field<float, false>(h->stream, h->N, h->deg, h->pi, nullptr, (double2*)g->pi);
+            else launch_degrade_field<double, false>(h->stream, h->N, h->deg, h->pi, nullptr, (double2*)g->pi);
+            launch_degrade_mask(h->stream, h->N, h->deg, h->flags, g->flags);
+            g->have_flags = true;
+            g->have_spec = false;
+            forward(h, axg_field_spectrum(g, tau, qmid, power, conv, num, rho, nbins));
+            h->have_spec = true;
+            return;
+        }
         ensure_spec_tables(h);
         const int ncov = nb * (nb + 1) / 2;
         const int nwg = 1024;
@@ -1570,6 +1676,16 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
         const int ncov = nb * (nb + 1) / 2;
         require(h->have_spec && h->have_flags, "call axg_field_spectrum first (after the last change of the fields)");
         require(cov != nullptr && ncov_out == ncov, "cov needs num_bin*(num_bin+1)/2 entries");
+        if (h->gs) {
+            // estimate_covmat on the reduced grid, drawing from this handle's stream
+            axg_handle* g = h->gs;
+            g->stream_on = h->stream_on;
+            g->stream_x0 = h->stream_x0;
+            g->stream_pos = h->stream_pos;
+            forward(h, axg_field_covariance(g, seed, min_mc, max_mc, tol, cov, ncov_out, iters));
+            h->stream_pos = g->stream_pos;
+            return;
+        }
         require(min_mc >= 2 && max_mc >= min_mc && max_mc <= 100000, "need 2 <= min_mc <= max_mc");
         const int N = h->N;
         const double ntot = (double)h->n3;
@@ -1681,6 +1797,7 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
                 "chunk: 0 (no graph) or an even number of steps from 2 to 64");
         for (int i = 0; i < n; i++) {
             require(hs[i] != nullptr, "null handle");
+            require(!hs[i]->gs, "degrade_spec > 0: advance such a lattice with axg_advance_steps");
             require(hs[i]->p.device == h0->p.device, "all handles must be on one device");
             for (int j = 0; j < i; j++) require(hs[j] != hs[i], "a handle appears twice");
         }
@@ -2069,6 +2186,7 @@ extern "C" int axg_gradient(axg_handle* h, double t1, double dt, int32_t nsteps,
 {
     return guarded(h, [&] {
         require(out != nullptr, "null output");
+        require(!h->gs, "axg_gradient differentiates the lattice-grid GW spectrum (degrade_spec = 0)");
         require(h->p.grav_spec && h->A != nullptr, "grav_spec is off: no GW spectrum to differentiate");
         require(nsteps >= 1 && K >= 1, "need nsteps >= 1 and K >= 1");
         require(dt > 0 && t1 > 0 && std::isfinite(dt) && std::isfinite(t1), "need t1 > 0 and dt > 0");

@@ -145,7 +145,7 @@ program axion_3d
   if (spec_est .or. grav_spec) then
      allocate(tau_id(0:n_chk))
      degrade_spec = ini_int('degrade_spec')
-     if (degrade_spec /= 0) call die('error: degrade_spec > 0 is not supported by this build')
+     if (degrade_spec < 0) call die('error: degrade_spec must be >= 0')
   end if
   if (spec_est) then
      if (n_mod_eps < 1) call die('error: n_mod_eps must be positive')
@@ -157,7 +157,8 @@ program axion_3d
   ! memory estimate (settings.f90:241-260), one task
   b = byte_dc*2
   if (spec_est) then
-     b = b + 2*byte_dc + byte_dc
+     b = b + 2*byte_dc/2**(degrade_spec*3)
+     b = b + max(byte_dc/2**(degrade_spec*3), 1d0)
   else if (id_strings .or. id_walls) then
      b = b + 1d0
   end if
@@ -199,6 +200,7 @@ program axion_3d
   prm%eps_dtau = eps_dtau
   prm%dtau_fixed = ini_double('dtau_fixed', 0d0)
   prm%potential_kappa0 = ini_double('potential_kappa0', 0d0)
+  prm%degrade_spec = degrade_spec
   call ini_close()
   write(*, '(A,I0)') ' ngrid = ', ngrid
   write(*, '(A,ES22.14)') ' sigma_cr = ', prm%sigma_cr
@@ -403,7 +405,7 @@ contains
     integer, intent(in) :: idx
     complex(c_double_complex), allocatable :: phi(:), pi(:), a(:)
     integer :: u, ios
-    integer(c_int64_t) :: na
+    integer(c_int64_t) :: na, ng
     allocate(phi(n3), pi(n3))
     if (rd) then
        open(newunit=u, file=trim(chk_name(idx)), access='stream', form='unformatted', status='old', &
@@ -422,7 +424,9 @@ contains
     end if
     deallocate(phi, pi)
     if (chk_gw .and. grav_spec) then
-       na = 4*ngrid*ngrid*(ngrid/2 + 1)
+       ! the accumulators live on the GW grid, ngrid / 2^degrade_spec per axis
+       ng = ngrid/2**degrade_spec
+       na = 4*ng*ng*(ng/2 + 1)
        allocate(a(na))
        if (rd) then
           inquire(file=trim(grav_chk_name(idx)), exist=exists)
@@ -641,12 +645,13 @@ contains
   end subroutine covariance
 
   ! Centres and widths of the axion spectrum's bins.  The edges are those of init_spectrum
-  ! (spectrum.f90:58-61): running sums of dq = pi/dspa/num_bin from 0.
+  ! (spectrum.f90:58-61): running sums of dq = pi/dspa_spec/num_bin from 0, dspa_spec = dspa
+  ! 2^degrade_spec (spectrum.f90:46).
   subroutine spec_bins(qc, wb)
     double precision, intent(out) :: qc(num_bin), wb(num_bin)
     double precision :: lo, hi, step
     integer :: b
-    step = pi_t/prm%dspa/num_bin
+    step = pi_t/(prm%dspa*2**degrade_spec)/num_bin
     hi = 0d0
     do b = 1, num_bin
        lo = hi

@@ -55,7 +55,11 @@ typedef struct axg_params {
     double dtau_fixed;                   /* > 0: fixed step instead of get_dtau (new key) */
     double potential_kappa0;             /* N_DW=2 tilt -kappa0 tau^4 (phi^2+phi*^2)/2 beside
                                             lambda/4 (|phi|^2-1)^2, lambda = 4 sigma_cr^2; 0 = off */
-    int32_t reserved[8];
+    int32_t degrade_spec;   /* key degrade_spec (settings.f90:45): the GW source and the axion
+                               spectrum live on the reduced grid N / 2^degrade_spec (>= 8), as
+                               init_grav / init_spectrum set it up (grav_waves.f90:37-45,
+                               spectrum.f90:42-49); 0 = the lattice grid */
+    int32_t reserved[7];
 } axg_params;
 
 /* The ini keys axg_derive() needs (settings.f90:269-297). */
@@ -116,16 +120,19 @@ int axg_init_flat(axg_handle* h, uint64_t seed, double kmax);
  * tau_ini >= 1 the reference's zero mode is not finite). */
 int axg_init_thermal(axg_handle* h, uint64_t seed, double tau_ini);
 
-/* GW accumulators arr_grav (grav_waves.f90:12,74) on the half spectrum kx = 0..N/2:
- * count = 4 * N * N * (N/2+1) complex, ordered [m][kz][ky][kx] with m = (1,1),(2,1),(1,2),
- * (2,2) of arr_grav(a,b,...), interleaved (re, im).  The other half is the complex
- * conjugate (exact for every mode that reaches a spectrum bin). */
+/* GW accumulators arr_grav (grav_waves.f90:12,74) on the half spectrum kx = 0..Ng/2 of the GW
+ * grid, Ng = N / 2^degrade_spec: count = 4 * Ng * Ng * (Ng/2+1) complex, ordered [m][kz][ky][kx]
+ * with m = (1,1),(2,1),(1,2),(2,2) of arr_grav(a,b,...), interleaved (re, im).  The other half
+ * is the complex conjugate (exact for every mode that reaches a spectrum bin).  With
+ * degrade_spec > 0 they are FP64 whatever the field precision. */
 int axg_get_gw_state(axg_handle* h, double* A, int64_t count);
 int axg_set_gw_state(axg_handle* h, const double* A, int64_t count);
 
 /* symplectic_integration(t1, t2) (evolve.f90:238-257): get_dtau (or dtau_fixed), a drift
  * of dt/2, then n x (kick at the midpoint with the GW accumulation of grav_waves.f90:193-255
- * when grav_spec, drift dt), the last drift being dt/2.  Returns the step count. */
+ * when grav_spec, drift dt), the last drift being dt/2.  Returns the step count.  With
+ * degrade_spec > 0 the GW source is data1_to_arr_dphi_degrade's block-summed gradient
+ * (grav_waves.f90:358-507) and the accumulation runs on the GW grid. */
 int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_out);
 /* The same DKD schedule with an explicit step: nsteps steps of dt from t1.  dt < 0 runs it
  * backwards (t1 + nsteps*dt > 0): the leapfrog with the Crank-Nicolson friction of
@@ -134,7 +141,8 @@ int axg_advance(axg_handle* h, double t1, double t2, int32_t* nsteps_out);
 int axg_advance_steps(axg_handle* h, double t1, double dt, int32_t nsteps);
 
 /* grav_spec_estimation (grav_waves.f90:257-321) without the file write: qmid, power,
- * numer, denom per bin (nbins must equal num_bin_grav).  Any pointer may be NULL. */
+ * numer, denom per bin (nbins must equal num_bin_grav) on the GW grid (bin edges from
+ * pi / dspa_grav, dspa_grav = dspa 2^degrade_spec).  Any pointer may be NULL. */
 int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, double* numer, double* denom,
                     int32_t nbins);
 
@@ -248,11 +256,14 @@ int axg_group_walls(axg_handle* h, double thickness, double hor_size, axg_group_
 int axg_get_groups(axg_handle* h, int32_t kind, int32_t* ids, int64_t cap, int64_t* count);
 
 /* spectrum_estimation's deterministic part (spectrum.f90:242-286 with compute_matrix_deconv
- * 344-444 and compute_conv_spec 466-567, degrade_spec = 0): the masked pi spectrum
- * deconvolved with the mask window.  tau is tau_id(idx).  power = the deconvolved
- * spectrum, conv = conv_spec (already times tau^2), num = num_conv_spec; rho = the
- * compute_crho energy (spectrum.f90:316-342, its first output).  Needs axg_defects first
- * when spec_est (the mask); nbins must equal num_bin. */
+ * 344-444 and compute_conv_spec 466-567): the masked pi spectrum deconvolved with the mask
+ * window.  tau is tau_id(idx).  power = the deconvolved spectrum, conv = conv_spec (already
+ * times tau^2), num = num_conv_spec; rho = the compute_crho energy (spectrum.f90:316-342, its
+ * first output).  Needs axg_defects first when spec_est (the mask); nbins must equal num_bin.
+ * With degrade_spec > 0 it runs on the grid N / 2^degrade_spec: pi's block means
+ * (data2_to_data_spec_degrade, spectrum.f90:736-898) and a coarse cell masked when any of its
+ * lattice cells is (masking, spectrum.f90:900-954); axg_field_covariance follows it there
+ * (4 (N / 2^degrade_spec)^3 stream draws per realisation). */
 int axg_field_spectrum(axg_handle* h, double tau, double* qmid, double* power, double* conv,
                        double* num, double* rho, int32_t nbins);
 
@@ -281,7 +292,7 @@ int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc, int32_t m
  * units sigma_cr^2 = lambda/4); dJ_dkappa0: d/d(potential_kappa0); dJ_dlnamp: d/d(ln A) for
  * the initial (phi, pi) scaled by A.  On return the fields and accumulators are those at the
  * end of the run; the adjoints of the initial fields are available from axg_get_adjoint.
- * forward_ms / adjoint_ms: wall time of the two sweeps. */
+ * forward_ms / adjoint_ms: wall time of the two sweeps.  Needs degrade_spec = 0. */
 typedef struct {
     double J;
     double dJ_dsigma2;
@@ -309,7 +320,8 @@ int axg_get_adjoint(axg_handle* h, double* phibar, double* pibar, int64_t nsite)
  * replayed, a device step counter advancing once per replay; the steps left over, and chunk = 0,
  * launch the same kernels directly.  AXG_BATCH_STACK=0 turns stacking off (same results).
  * chunk: 0 or an even number from 2 to 64.  *ms (may be NULL): wall time of the steps.  Errors
- * are reported on hs[0]. */
+ * are reported on hs[0].  Handles with degrade_spec > 0 are refused (AXG_EINVAL): advance them
+ * with axg_advance_steps. */
 int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, double dt, int32_t nsteps, int32_t chunk,
                       double* ms);
 

@@ -45,7 +45,11 @@ def lib():
         L.orc_fft3d.argtypes = [p, p, i64, C.c_int]
         L.orc_gradient.argtypes = [p, p, i64, d]
         L.orc_gw_integrate.argtypes = [p, p, i64, d, d, d, d, d, d, C.c_int]
-        L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int, d]
+        L.orc_symplectic.argtypes = [p, p, p, i64, d, d, i32, d, d, d, d, d, C.c_int, C.c_int, d, C.c_int]
+        L.orc_gw_accumulate.argtypes = [p, p, i64, d, d, d, d, d, C.c_int]
+        L.orc_degrade_gradient.argtypes = [p, i64, i64, d, p]
+        L.orc_degrade_field.argtypes = [p, i64, i64, p]
+        L.orc_degrade_mask.argtypes = [p, i64, i64, p]
         L.orc_gw_spectrum.argtypes = [p, i64, C.c_int, d, d, d, d, d, p, p, p, p]
         L.orc_classify.argtypes = [p, i64, C.c_int, C.c_int, C.c_int, p, p]
         L.orc_plaquettes.argtypes = [p, i64, p, p]
@@ -172,14 +176,61 @@ def gw_integrate(phi, A, tau, dtau, dspa, dwav, dv_spa, sigma_cr, fix_swap=False
 
 def symplectic(phi, pi, A, t1, dt, nsteps, P, fix_swap=False):
     """Advance (phi, pi[, A]) in place by nsteps DKD steps of dt from t1.  P is a dict with
-    dspa, dwav, dv_spa, sigma_cr, kappa, initial_loop and optionally potential_kappa0."""
+    dspa, dwav, dv_spa, sigma_cr, kappa, initial_loop and optionally potential_kappa0 and
+    degrade_spec (then A lives on the grid N / 2^degrade_spec)."""
     lib().orc_symplectic(_p(phi), _p(pi), _p(A) if A is not None else None, phi.shape[0], t1, dt,
                          int(nsteps), P["dspa"], P["dwav"], P["dv_spa"], P["sigma_cr"], P["kappa"],
-                         int(P["initial_loop"]), int(fix_swap), float(P.get("potential_kappa0", 0.0)))
+                         int(P["initial_loop"]), int(fix_swap), float(P.get("potential_kappa0", 0.0)),
+                         int(P.get("degrade_spec", 0)))
 
 
 def new_gw_state(N):
     return np.zeros((4, N, N, N), np.complex128)
+
+
+def gw_accumulate(dphi, A, tau, dtau, dwav, dv_spa, sigma_cr, fix_swap=False):
+    """grav_wave_integration's accumulation from arr_dphi (3, N, N, N) into A (4, N, N, N)."""
+    dphi = np.ascontiguousarray(dphi, np.complex128)
+    lib().orc_gw_accumulate(_p(dphi), _p(A), dphi.shape[1], tau, dtau, dwav, dv_spa, sigma_cr, int(fix_swap))
+
+
+def reduced_units(P, degrade_spec):
+    """The GW / spectrum grid of degrade_spec > 0 as init_grav and init_spectrum derive it
+    (grav_waves.f90:37-45, spectrum.f90:42-49): ngrid, dspa, dwav, dv_spa, dv_wav."""
+    deg = 2 ** degrade_spec
+    twopi_t = 2 * 3.14159265359
+    n = int(P["ngrid"] if "ngrid" in P else P["N"]) // deg
+    dspa = P["dspa"] * deg
+    dwav = twopi_t / dspa / n
+    r = dwav / twopi_t
+    return dict(ngrid=n, dspa=dspa, dwav=dwav, dv_spa=dspa * dspa * dspa, dv_wav=r * r * r)
+
+
+def degrade_gradient(phi, degrade_spec, dspa):
+    """data1_to_arr_dphi_degrade: (3, Nc, Nc, Nc) block-summed central differences."""
+    phi = np.ascontiguousarray(phi, np.complex128)
+    N, deg = phi.shape[0], 2 ** degrade_spec
+    g = np.empty((3, N // deg, N // deg, N // deg), np.complex128)
+    lib().orc_degrade_gradient(_p(phi), N, deg, dspa, _p(g))
+    return g
+
+
+def degrade_field(f, degrade_spec):
+    """data2_to_data_spec_degrade: the block means (Nc, Nc, Nc)."""
+    f = np.ascontiguousarray(f, np.complex128)
+    N, deg = f.shape[0], 2 ** degrade_spec
+    out = np.empty((N // deg,) * 3, np.complex128)
+    lib().orc_degrade_field(_p(f), N, deg, _p(out))
+    return out
+
+
+def degrade_mask(flags, degrade_spec):
+    """masking with degrade: bit 4 on the coarse cells that hold a masked lattice cell."""
+    flags = np.ascontiguousarray(flags, np.int8)
+    N, deg = flags.shape[0], 2 ** degrade_spec
+    out = np.empty((N // deg,) * 3, np.int8)
+    lib().orc_degrade_mask(_p(flags), N, deg, _p(out))
+    return out
 
 
 # ------------------------------------------------------------------ diagnostics

@@ -305,13 +305,25 @@ void orc_gradient(const double* phi, double* dphi, int64_t N, double dspa)
  * m=0 (1,1) cos,+   m=1 (2,1) sin,+   m=2 (1,2) cos,-   m=3 (2,2) sin,-   (full N^3 each).
  * fix_swap=0 reproduces lines 239-242 exactly (A(2,1) <- A(1,2)+..., A(1,2) <- new A(2,1)+...);
  * fix_swap=1 applies the evidently intended A(2,1) += s*P*D, A(1,2) += c*N*D. */
+void orc_gw_accumulate(const double* dphi, double* A, int64_t N, double tau, double dtau, double dwav,
+                       double dv_spa, double sigma_cr, int fix_swap);
+
 void orc_gw_integrate(const double* phi, double* A, int64_t N, double tau, double dtau,
                       double dspa, double dwav, double dv_spa, double sigma_cr, int fix_swap)
 {
-    int64_t n3 = N * N * N;
-    double* dphi = (double*)malloc(sizeof(double) * 6 * n3);
-    double* src = (double*)malloc(sizeof(double) * 2 * n3);
+    double* dphi = (double*)malloc(sizeof(double) * 6 * N * N * N);
     orc_gradient(phi, dphi, N, dspa);
+    orc_gw_accumulate(dphi, A, N, tau, dtau, dwav, dv_spa, sigma_cr, fix_swap);
+    free(dphi);
+}
+
+/* grav_waves.f90:209-247: the accumulation of grav_wave_integration from arr_dphi (dphi[d*N^3 +
+ * site] on the GW grid of N sites per axis, wavenumber unit dwav and cell volume dv_spa). */
+void orc_gw_accumulate(const double* dphi, double* A, int64_t N, double tau, double dtau, double dwav,
+                       double dv_spa, double sigma_cr, int fix_swap)
+{
+    int64_t n3 = N * N * N;
+    double* src = (double*)malloc(sizeof(double) * 2 * n3);
     const oc* g = (const oc*)dphi;
     oc* a = (oc*)A;
     double dw = tau * dtau * dv_spa * (sigma_cr * sigma_cr);
@@ -357,8 +369,81 @@ void orc_gw_integrate(const double* phi, double* A, int64_t N, double tau, doubl
                         a22->re = a22->re + sn * nd.re; a22->im = a22->im + sn * nd.im;
                     }
         }
-    free(dphi);
     free(src);
+}
+
+/* ------------------------------------------------------ reduced grid (degrade_spec > 0) */
+
+/* grav_waves.f90:384-407 data1_to_arr_dphi_degrade (one rank): on the grid Nc = N/deg, per axis
+ * the sum over the block of deg^3 sites (z outermost, x innermost) of phi(s+e) - phi(s-e),
+ * accumulated as (sum + a) - b from zero, divided by d = deg^3 * dspa * 2.  dphi: 3 x Nc^3. */
+void orc_degrade_gradient(const double* phi, int64_t N, int64_t deg, double dspa, double* dphi)
+{
+    const oc* f = (const oc*)phi;
+    oc* g = (oc*)dphi;
+    const int64_t Nc = N / deg, nc3 = Nc * Nc * Nc;
+    const double d = (double)(deg * deg * deg) * dspa * 2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t cz = 0; cz < Nc; cz++)
+        for (int64_t cy = 0; cy < Nc; cy++)
+            for (int64_t cx = 0; cx < Nc; cx++) {
+                oc s[3] = {{0, 0}, {0, 0}, {0, 0}};
+                for (int64_t z = cz * deg; z < (cz + 1) * deg; z++) {
+                    const int64_t z1 = (z + N - 1) % N, z2 = (z + 1) % N;
+                    for (int64_t y = cy * deg; y < (cy + 1) * deg; y++) {
+                        const int64_t y1 = (y + N - 1) % N, y2 = (y + 1) % N;
+                        for (int64_t x = cx * deg; x < (cx + 1) * deg; x++) {
+                            const int64_t x1 = (x + N - 1) % N, x2 = (x + 1) % N;
+                            const oc ax = f[x2 + N * (y + N * z)], bx = f[x1 + N * (y + N * z)];
+                            const oc ay = f[x + N * (y2 + N * z)], by = f[x + N * (y1 + N * z)];
+                            const oc az = f[x + N * (y + N * z2)], bz = f[x + N * (y + N * z1)];
+                            s[0].re = s[0].re + ax.re - bx.re; s[0].im = s[0].im + ax.im - bx.im;
+                            s[1].re = s[1].re + ay.re - by.re; s[1].im = s[1].im + ay.im - by.im;
+                            s[2].re = s[2].re + az.re - bz.re; s[2].im = s[2].im + az.im - bz.im;
+                        }
+                    }
+                }
+                const int64_t c = cx + Nc * (cy + Nc * cz);
+                for (int k = 0; k < 3; k++) {
+                    g[k * nc3 + c].re = s[k].re / d;
+                    g[k * nc3 + c].im = s[k].im / d;
+                }
+            }
+}
+
+/* spectrum.f90:761-786 data2_to_data_spec_degrade (one rank): the block mean of f on the grid
+ * N/deg, summed in loop order from zero and divided by deg^3. */
+void orc_degrade_field(const double* f, int64_t N, int64_t deg, double* out)
+{
+    const oc* a = (const oc*)f;
+    oc* o = (oc*)out;
+    const int64_t Nc = N / deg;
+    const double dd = (double)(deg * deg * deg);
+    for (int64_t cz = 0; cz < Nc; cz++)
+        for (int64_t cy = 0; cy < Nc; cy++)
+            for (int64_t cx = 0; cx < Nc; cx++) {
+                oc s = {0, 0};
+                for (int64_t z = cz * deg; z < (cz + 1) * deg; z++)
+                    for (int64_t y = cy * deg; y < (cy + 1) * deg; y++)
+                        for (int64_t x = cx * deg; x < (cx + 1) * deg; x++) {
+                            s.re = s.re + a[x + N * (y + N * z)].re;
+                            s.im = s.im + a[x + N * (y + N * z)].im;
+                        }
+                o[cx + Nc * (cy + Nc * cz)].re = s.re / dd;
+                o[cx + Nc * (cy + Nc * cz)].im = s.im / dd;
+            }
+}
+
+/* spectrum.f90:900-954 masking with deg > 1: the mask file's cells (ix0,iy0,iz0) zero the coarse
+ * cell ((ix0-1)/deg+1, ...); here the coarse flag gets bit 4 from any fine cell's bit 4. */
+void orc_degrade_mask(const int8_t* flags, int64_t N, int64_t deg, int8_t* out)
+{
+    const int64_t Nc = N / deg;
+    memset(out, 0, (size_t)(Nc * Nc * Nc));
+    for (int64_t z = 0; z < N; z++)
+        for (int64_t y = 0; y < N; y++)
+            for (int64_t x = 0; x < N; x++)
+                if (flags[x + N * (y + N * z)] & 4) out[x / deg + Nc * (y / deg + Nc * (z / deg))] = 4;
 }
 
 /* evolve.f90:238-257 symplectic_integration over [t1, t2] with nt steps of dt (from get_dtau
@@ -366,17 +451,28 @@ void orc_gw_integrate(const double* phi, double* A, int64_t N, double tau, doubl
  * drift dt/2.  A may be NULL when grav_spec is off. */
 void orc_symplectic(double* phi, double* pi, double* A, int64_t N, double t1, double dt, int32_t nt,
                     double dspa, double dwav, double dv_spa, double sigma_cr, double kappa,
-                    int initial_loop, int fix_swap, double kappa0)
+                    int initial_loop, int fix_swap, double kappa0, int degrade_spec)
 {
     int64_t n3 = N * N * N;
     double t = t1;
+    /* init_grav (grav_waves.f90:37-45): the GW grid of degrade_spec > 0 */
+    const int64_t deg = (int64_t)1 << degrade_spec, Ng = N / deg;
+    const double dspa_g = dspa * (double)deg, dwav_g = TWOPI_T / dspa_g / (double)Ng;
+    const double dv_spa_g = dspa_g * dspa_g * dspa_g;
+    double* dphi = (A && deg > 1) ? (double*)malloc(sizeof(double) * 6 * Ng * Ng * Ng) : NULL;
     for (int32_t it = 1; it <= nt; it++) {
         if (it == 1) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
         orc_kick(phi, pi, N, t, dt, dspa, sigma_cr, kappa, initial_loop, kappa0);
-        if (A) orc_gw_integrate(phi, A, N, t, dt, dspa, dwav, dv_spa, sigma_cr, fix_swap);
+        if (A && deg > 1) {
+            orc_degrade_gradient(phi, N, deg, dspa, dphi);
+            orc_gw_accumulate(dphi, A, Ng, t, dt, dwav_g, dv_spa_g, sigma_cr, fix_swap);
+        } else if (A) {
+            orc_gw_integrate(phi, A, N, t, dt, dspa, dwav, dv_spa, sigma_cr, fix_swap);
+        }
         if (it == nt) { orc_drift(phi, pi, n3, dt / 2); t = t + dt / 2; }
         else { orc_drift(phi, pi, n3, dt); t = t + dt; }
     }
+    free(dphi);
 }
 
 /* ---------------------------------------------------------------- GW spectrum */

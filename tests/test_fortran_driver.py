@@ -316,3 +316,66 @@ def test_halo_padded_binding_roundtrip():
     print("[case] halo-padded Fortran binding", flush=True)
     r = subprocess.run([HALO_EXE], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "halo binding ok" in r.stdout, r.stdout + r.stderr
+
+
+@needs_exe
+@pytest.mark.gpu
+def test_degraded_run_matches_oracle(tmp_path):
+    """degrade_spec = 1: the gravspec, spectrum and covariance files come from the 16^3 grid of
+    the 32^3 lattice (grav_waves.f90:37-45, 358-507; spectrum.f90:42-49, 736-954), the GW
+    checkpoint holds that grid's accumulators, and the field checkpoint the lattice's.  A coarse
+    cell is masked when any of its 8 lattice cells is, so the reduced grid's mask is about twice
+    the lattice's: at the first snapshot one deconvolved bin is negative and the covariance NaN on
+    both sides (as in the reference: sqrt of a negative power), at the second all are finite."""
+    print("[case] fortran driver N=32 degrade_spec=1", flush=True)
+    N, nb, nbg, ds = 32, 8, 25, 1
+    Ng = N >> ds
+    d = O.derive(SIGMA, 90.0, 1e-30, 2.0, 4.0, N)
+    phi0, _ = O.ic_flat(N, 5, 1.0, d["dwav"])
+    rng = np.random.default_rng(5)
+    pi0 = 2.0 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    np.concatenate([phi0.ravel(), pi0.ravel()]).tofile(tmp_path / "ic.raw")
+    run(tmp_path, ini_text(str(tmp_path), ic_file=str(tmp_path / "ic.raw"), degrade_spec=ds, chk_gw="T"),
+        "-b", "100")
+    P = dict(d, kappa=1e-30, initial_loop=True, degrade_spec=ds, N=N)
+    u = O.reduced_units(P, ds)
+    phi, pi = phi0.copy(), pi0.copy()
+    A = O.new_gw_state(Ng)
+    taus = [1.0, 1.5, 2.0]
+    pos = 0
+    for idx in (1, 2):
+        dt, n = O.get_dtau(taus[idx - 1], taus[idx], P["sigma_cr"], P["dwav"], N, 1.0)
+        O.symplectic(phi, pi, A, taus[idx - 1], dt, n, P)
+        g = read_rows(tmp_path / f"run_gravspec_{idx}.dat")
+        o = O.gw_spectrum(A, nbg, u["dspa"], u["dwav"], d["tau_cr"], d["a_now"], u["dv_wav"])
+        assert np.allclose(g[:, 1], o["qmid"], rtol=1e-9) and np.array_equal(g[:, 4], o["denom"])
+        assert np.allclose(g[:, 3], o["power"], rtol=1e-8, atol=0)
+        flags, counts = O.classify(phi)
+        cf, cpi = O.degrade_mask(flags, ds), O.degrade_field(pi, ds)
+        s = read_rows(tmp_path / f"run_spectrum_{idx}.dat")
+        pw, num = O.conv_spectrum(cpi, cf, nb, u["dspa"], u["dwav"], u["dv_spa"], u["dv_wav"])
+        conv = pw * taus[idx] ** 2
+        tab = O.init_spectrum(nb, u["dspa"])
+        _, inv, info = O.matrix_deconv(cf, nb, u["dwav"], u["dv_spa"], u["dv_wav"], tab)
+        M = O.sym_from_packed(inv, nb) / np.outer(num, num)
+        power = M @ conv
+        assert np.array_equal(s[:, 3], num)
+        assert np.allclose(s[:, 0], (np.arange(nb) + 0.5) * PI_T / u["dspa"] / nb, rtol=1e-12)
+        assert np.allclose(s[:, 1], power, rtol=1e-6, atol=1e-12 * np.abs(power).max())
+        c = read_rows(tmp_path / f"run_covariance_{idx}.dat")
+        ii, jj = np.tril_indices(nb)
+        order = np.lexsort((jj, ii))
+        pos += 4004 * int(counts[0])
+        ocov, oit, _ = O.field_covariance(cf, nb, u["dspa"], u["dwav"], power,
+                                          np.array([M[j, i] for i, j in zip(ii[order], jj[order])]), 0, 10, 100,
+                                          0.1, stream=(1, 2, pos))
+        pos += 4 * Ng ** 3 * (oit if oit > 0 else 100)
+        fin = np.isfinite(ocov)
+        assert np.array_equal(np.isfinite(c[:, 2]), fin) and fin.all() == (idx == 2) == (power.min() > 0)
+        if idx == 2:
+            assert np.allclose(c[:, 2], ocov, rtol=1e-5, atol=1e-6 * np.abs(ocov).max())
+    raw = np.fromfile(tmp_path / "run_phi_2.chk", dtype=np.complex128)
+    assert np.array_equal(raw[: N ** 3].reshape(N, N, N), phi) and np.array_equal(raw[N ** 3:].reshape(N, N, N), pi)
+    ga = np.fromfile(tmp_path / "run_grav_2.chk", dtype=np.complex128).reshape(4, Ng, Ng, Ng // 2 + 1)
+    assert np.max(np.abs(ga - A[:, :, :, : Ng // 2 + 1])) <= 1e-10 * np.abs(A).max()
+    assert read_rows(tmp_path / "run_eps.txt").shape == (1, 9)

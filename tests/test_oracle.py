@@ -1142,3 +1142,116 @@ def test_thermal_state_from_reference_stream_uses_its_draws():
             exp = (g[0] * P["dv_wav"]) * f + 1j * (g[1] * P["dv_wav"]) * f
             got = modes[ip - 1][z, y, x]
             assert abs(got - exp) <= 1e-12 * abs(exp), (ip, lin)
+
+
+# ------------------------------------------------------------- reduced grid (degrade_spec > 0)
+
+def _blocks(a, deg):
+    """(N,N,N) -> (Nc, Nc, Nc, deg^3) view of the deg^3 blocks (z, y, x order inside a block)."""
+    N = a.shape[0]
+    Nc = N // deg
+    return a.reshape(Nc, deg, Nc, deg, Nc, deg).transpose(0, 2, 4, 1, 3, 5).reshape(Nc, Nc, Nc, deg ** 3)
+
+
+@pytest.mark.parametrize("ds", [1, 2])
+def test_degrade_field_is_the_block_mean(ds):
+    """data2_to_data_spec_degrade is the mean over each 2^ds cube: exact on small integers (every
+    partial sum representable), and to 1e-15 on a random field."""
+    N, deg = 16, 2 ** ds
+    rng = np.random.default_rng(3)
+    f = (rng.integers(-50, 50, (N, N, N)) + 1j * rng.integers(-50, 50, (N, N, N))).astype(np.complex128)
+    assert np.array_equal(O.degrade_field(f, ds), _blocks(f, deg).sum(-1) / deg ** 3)
+    g = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    ref = _blocks(g, deg).mean(-1)
+    assert np.max(np.abs(O.degrade_field(g, ds) - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
+def test_degrade_mask_marks_blocks_with_a_masked_cell():
+    N = 16
+    rng = np.random.default_rng(4)
+    flags = np.zeros((N, N, N), np.int8)
+    flags[rng.random((N, N, N)) < 0.02] = 4
+    flags[rng.random((N, N, N)) < 0.1] |= 3  # string / wall bits alone mask nothing
+    for ds in (1, 2):
+        exp = np.where((_blocks(flags, 2 ** ds) & 4).any(-1), 4, 0).astype(np.int8)
+        assert np.array_equal(O.degrade_mask(flags, ds), exp)
+
+
+def test_degrade_gradient_reduces_to_the_gradient():
+    """deg = 1: (0 + a) - b divided by 1 * dspa * 2 is data1_to_arr_dphi's (a - b) / (2 dspa)."""
+    N = 12
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    assert np.array_equal(O.degrade_gradient(phi, 0, 0.3), O.gradient(phi, 0.3))
+
+
+@pytest.mark.parametrize("ds", [1, 2])
+def test_degrade_gradient_plane_wave(ds):
+    """phi = e^{i k.x}: phi(s + e_d) - phi(s - e_d) = 2 i sin(k_d dspa) phi(s), so the coarse
+    component d is 2 i sin(k_d dspa) / (deg^3 2 dspa) times the block sum of phi, and the block
+    sum is phi(block corner) prod_e sum_{j < deg} e^{i k_e j dspa}."""
+    N, dspa, deg = 16, 0.35, 2 ** ds
+    m = np.array([1, -3, 5])
+    k = 2 * np.pi * m / (N * dspa)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    phi = np.exp(1j * dspa * (k[0] * x + k[1] * y + k[2] * z))
+    g = O.degrade_gradient(phi, ds, dspa)
+    Nc = N // deg
+    cz, cy, cx = np.meshgrid(np.arange(Nc), np.arange(Nc), np.arange(Nc), indexing="ij")
+    corner = np.exp(1j * dspa * deg * (k[0] * cx + k[1] * cy + k[2] * cz))
+    geo = np.prod([np.exp(1j * k[e] * dspa * np.arange(deg)).sum() for e in range(3)])
+    for d in range(3):
+        exp = 2j * np.sin(k[d] * dspa) / (deg ** 3 * 2 * dspa) * corner * geo
+        assert np.max(np.abs(g[d] - exp)) < 1e-12, d
+
+
+def test_degrade_gradient_of_a_periodic_field_sums_to_zero():
+    """Summed over the whole periodic lattice the central differences cancel."""
+    N = 16
+    rng = np.random.default_rng(6)
+    phi = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    g = O.degrade_gradient(phi, 1, 0.25)
+    assert np.all(np.abs(g.sum(axis=(1, 2, 3))) < 1e-11)
+    assert np.all(np.abs(g).max(axis=(1, 2, 3)) > 0.1)
+
+
+def test_reduced_units():
+    """init_grav / init_spectrum's reduced grid: spacing deg * dspa, N / deg sites, so the
+    wavenumber unit and dv_wav are the lattice's (to rounding) and dv_spa is deg^3 times it."""
+    P = phys(32, 0.3)
+    for ds in (1, 2, 3):
+        u = O.reduced_units(P, ds)
+        deg = 2 ** ds
+        assert u["ngrid"] == 32 // deg and u["dspa"] == 0.3 * deg
+        assert abs(u["dwav"] / P["dwav"] - 1) < 4e-16
+        assert abs(u["dv_wav"] / P["dv_wav"] - 1) < 1e-15
+        assert abs(u["dv_spa"] / (deg ** 3 * 0.3 ** 3) - 1) < 1e-15
+
+
+def test_symplectic_with_degrade_runs_the_reduced_grid():
+    """orc_symplectic with degrade_spec = 1: the fields as without it (bit for bit), and the GW
+    accumulators those of data1_to_arr_dphi_degrade + the accumulation on the N/2 grid at every
+    kick, composed here from the pieces with the reduced units."""
+    N, dt, n = 16, 0.05, 3
+    P = phys(N)
+    P["degrade_spec"] = 1
+    phi0, pi0 = O.ic_flat(N, 7, 3.0, P["dwav"])
+    phi, pi = phi0.copy(), pi0.copy()
+    A = O.new_gw_state(N // 2)
+    O.symplectic(phi, pi, A, 1.0, dt, n, P)
+    phi1, pi1 = phi0.copy(), pi0.copy()
+    O.symplectic(phi1, pi1, None, 1.0, dt, n, P)
+    assert np.array_equal(phi, phi1) and np.array_equal(pi, pi1)
+    u = O.reduced_units(P, 1)
+    f, p = phi0.copy(), pi0.copy()
+    B = O.new_gw_state(N // 2)
+    t = 1.0
+    O.drift(f, p, dt / 2)
+    t += dt / 2
+    for it in range(n):
+        O.kick(f, p, t, dt, P["dspa"], P["sigma_cr"], P["kappa"], P["initial_loop"])
+        O.gw_accumulate(O.degrade_gradient(f, 1, P["dspa"]), B, t, dt, u["dwav"], u["dv_spa"], P["sigma_cr"])
+        O.drift(f, p, dt / 2 if it == n - 1 else dt)
+        t += dt / 2 if it == n - 1 else dt
+    assert np.array_equal(A, B)
+    assert np.max(np.abs(A)) > 0
