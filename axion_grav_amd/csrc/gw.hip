@@ -138,7 +138,7 @@ __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typen
     for (int q = 0; q < KZ; q++) {
         const T2* base = W + ((int64_t)c * N + z0 + q) * (int64_t)N * P + kx;
 #pragma unroll
-        for (int e = 0; e < 8; e++) v[q][e] = ok ? base[(int64_t)(j0 + e * TPT) * P] : T2{0, 0};
+        for (int e = 0; e < 8; e++) v[q][e] = ok ? ld_stream(base + (int64_t)(j0 + e * TPT) * P) : T2{0, 0};
     }
     // twiddles read inside the stages (measured: no faster with them held in registers)
 #pragma unroll
@@ -240,12 +240,12 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
             if (c + 1 < 6) {
                 const T2* base = W + (c + 1) * cst + tile0;
 #pragma unroll
-                for (int e = 0; e < 8; e++) nxt[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+                for (int e = 0; e < 8; e++) nxt[e] = ok ? ld_stream(base + (int64_t)(j0 + e * TPT) * zs) : T2{0, 0};
             }
         } else {
             const T2* base = W + c * cst + tile0;
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[e] = ok ? base[(int64_t)(j0 + e * TPT) * zs] : T2{0, 0};
+            for (int e = 0; e < 8; e++) v[e] = ok ? ld_stream(base + (int64_t)(j0 + e * TPT) * zs) : T2{0, 0};
         }
         if (c > 0) __syncthreads();
         if constexpr (PTW) fft_p2<T, N, +1>(v, j0, s, TX, b, twr);
@@ -313,18 +313,19 @@ __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel
         const T2 SP = {sp[e].x * dw, sp[e].y * dw};
         const T2 SN = {sn[e].x * dw, sn[e].y * dw};
         const int64_t ai = tile0 + (int64_t)kz * zs;
-        T2 a11 = A[ai], a22 = A[3 * ms + ai], a12 = A[2 * ms + ai];
-        A[ai] = T2{a11.x + cq * SP.x, a11.y + cq * SP.y};
-        A[3 * ms + ai] = T2{a22.x + sq * SN.x, a22.y + sq * SN.y};
+        // the accumulators are streamed: each is read and written once per step
+        T2 a11 = ld_stream(A + ai), a22 = ld_stream(A + 3 * ms + ai), a12 = ld_stream(A + 2 * ms + ai);
+        st_stream(A + ai, T2{a11.x + cq * SP.x, a11.y + cq * SP.y});
+        st_stream(A + 3 * ms + ai, T2{a22.x + sq * SN.x, a22.y + sq * SN.y});
         if (!g.fix_swap) {
             T2 r12 = {a12.x + (sq * SP.x + cq * SN.x), a12.y + (sq * SP.y + cq * SN.y)};
             const T n6 = n2 > 0 ? ((T)rho2 / (T)n2) / 2 * dw : (T)0;  // pol_33 dw = r^2/2 dw
-            A[2 * ms + ai] = r12;
-            A[ms + ai] = T2{r12.x - cq * (n6 * v[e].x), r12.y - cq * (n6 * v[e].y)};
+            st_stream(A + 2 * ms + ai, r12);
+            st_stream(A + ms + ai, T2{r12.x - cq * (n6 * v[e].x), r12.y - cq * (n6 * v[e].y)});
         } else {
-            T2 a21 = A[ms + ai];
-            A[ms + ai] = T2{a21.x + sq * SP.x, a21.y + sq * SP.y};
-            A[2 * ms + ai] = T2{a12.x + cq * SN.x, a12.y + cq * SN.y};
+            T2 a21 = ld_stream(A + ms + ai);
+            st_stream(A + ms + ai, T2{a21.x + sq * SP.x, a21.y + sq * SP.y});
+            st_stream(A + 2 * ms + ai, T2{a12.x + cq * SN.x, a12.y + cq * SN.y});
         }
         __builtin_amdgcn_sched_barrier(0);  // keep the k-points' loads from piling up in registers
     }

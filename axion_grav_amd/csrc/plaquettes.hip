@@ -34,55 +34,102 @@ __device__ __forceinline__ int quadrant(double2 z)
     return (int)ceil(2 * ph / AXG_PI_T) + 2;
 }
 
-// defects_3d.f90:856-877: the four corners' phases fill at least three of the four quadrants.
-__device__ __forceinline__ int quad_hit(double2 a, double2 b, double2 c, double2 d)
-{
-    const int m = (1 << quadrant(a)) | (1 << quadrant(b)) | (1 << quadrant(c)) | (1 << quadrant(d));
-    return __popc(m) >= 3 ? 1 : 0;
-}
+// A block takes a PQ_TX x PQ_TY tile of (x, y) over zc consecutive planes, marching in z: the
+// values and quadrants of one site plane of the tile, with the periodic upper neighbours, sit in
+// LDS, so every site is loaded and its atan2 taken about (1 + 1/PQ_TX)(1 + 1/PQ_TY)(1 + 1/zc)
+// times instead of 7 and 12.  The three faces of a site need its plane and the next.
+constexpr int PQ_TX = 32, PQ_TY = 8, PQ_NT = PQ_TX * PQ_TY, PQ_ZC = 32;
 
 template <typename T, bool COMP>
-__global__ void __launch_bounds__(PLAQ_NT) plaquette_kernel(int N, const typename V2<T>::type* __restrict__ phi,
-                                                            const short2* __restrict__ cphi,
-                                                            uint16_t* __restrict__ code, int64_t* __restrict__ counts)
+__global__ void __launch_bounds__(PQ_NT) plaquette_kernel(int N, int zc, const typename V2<T>::type* __restrict__ phi,
+                                                          const short2* __restrict__ cphi, uint16_t* __restrict__ code)
 {
-    __shared__ int sc[PLAQ_COLS];
-    if (threadIdx.x < PLAQ_COLS) sc[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t NN = (int64_t)N * N, n3 = NN * N;
-    const int64_t i = blockIdx.x * (int64_t)PLAQ_NT + threadIdx.x;
-    if (i < n3) {
-        const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / NN);
-        const int64_t dx = (x + 1 == N ? 0 : x + 1) - x, dy = ((y + 1 == N ? 0 : y + 1) - y) * (int64_t)N,
-                      dz = ((z + 1 == N ? 0 : z + 1) - z) * NN;
-        const double2 s = ldf<T, COMP>(phi, cphi, i);
-        const double2 sx = ldf<T, COMP>(phi, cphi, i + dx), sy = ldf<T, COMP>(phi, cphi, i + dy);
-        const double2 sz = ldf<T, COMP>(phi, cphi, i + dz);
-        const double2 sxy = ldf<T, COMP>(phi, cphi, i + dx + dy), syz = ldf<T, COMP>(phi, cphi, i + dy + dz);
-        const double2 szx = ldf<T, COMP>(phi, cphi, i + dz + dx);
-        // (u, v) = (x, y), (y, z), (z, x): corners s, s+u, s+u+v, s+v
-        const double2 c1[3] = {sx, sy, sz}, c2[3] = {sxy, syz, szx}, c3[3] = {sy, sz, sx};
-        int cd = 0, pierced = 0;
-#pragma unroll
-        for (int o = 0; o < 3; o++) {
-            const int n = arc_cut(s, c1[o]) + arc_cut(c1[o], c2[o]) + arc_cut(c2[o], c3[o]) + arc_cut(c3[o], s);
-            const int q = quad_hit(s, c1[o], c2[o], c3[o]);
-            cd |= (n + 2) << (3 * o);
-            cd |= q << (9 + o);
-            if (n != 0) {
-                pierced++;
-                atomicAdd(&sc[5 * o + 0], 1);
-                atomicAdd(&sc[5 * o + 4], n);
+    __shared__ double2 val[2][PQ_TY + 1][PQ_TX + 1];
+    __shared__ int8_t qd[2][PQ_TY + 1][PQ_TX + 1];
+    const int x0 = blockIdx.x * PQ_TX, y0 = blockIdx.y * PQ_TY, z0 = blockIdx.z * zc;
+    const int nx = min(PQ_TX, N - x0), ny = min(PQ_TY, N - y0), nz = min(zc, N - z0);
+    const int64_t NN = (int64_t)N * N;
+    const int t = threadIdx.x, lx = t % PQ_TX, ly = t / PQ_TX;
+    auto load_plane = [&](int buf, int z) {
+        for (int k = t; k < (PQ_TY + 1) * (PQ_TX + 1); k += PQ_NT) {
+            const int py = k / (PQ_TX + 1), px = k - py * (PQ_TX + 1);
+            if (px <= nx && py <= ny) {
+                const int gx = x0 + px == N ? 0 : x0 + px, gy = y0 + py == N ? 0 : y0 + py;
+                const double2 v = ldf<T, COMP>(phi, cphi, z * NN + (int64_t)gy * N + gx);
+                val[buf][py][px] = v;
+                qd[buf][py][px] = (int8_t)quadrant(v);
             }
-            if (n == 1) atomicAdd(&sc[5 * o + 1], 1);
-            if (n == -1) atomicAdd(&sc[5 * o + 2], 1);
-            if (q) atomicAdd(&sc[5 * o + 3], 1);
         }
-        code[i] = (uint16_t)cd;
-        if (pierced) atomicAdd(&sc[15], pierced);
+    };
+    load_plane(0, z0);
+    for (int k = 0; k < nz; k++) {
+        const int z = z0 + k, a = k & 1, b = a ^ 1;
+        load_plane(b, z + 1 == N ? 0 : z + 1);
+        __syncthreads();
+        if (lx < nx && ly < ny) {
+            const double2 s = val[a][ly][lx], sx = val[a][ly][lx + 1], sy = val[a][ly + 1][lx];
+            const double2 sxy = val[a][ly + 1][lx + 1], sz = val[b][ly][lx];
+            const double2 syz = val[b][ly + 1][lx], szx = val[b][ly][lx + 1];
+            const int q = qd[a][ly][lx], qx = qd[a][ly][lx + 1], qy = qd[a][ly + 1][lx];
+            const int qxy = qd[a][ly + 1][lx + 1], qz = qd[b][ly][lx], qyz = qd[b][ly + 1][lx];
+            const int qzx = qd[b][ly][lx + 1];
+            // (u, v) = (x, y), (y, z), (z, x): corners s, s+u, s+u+v, s+v
+            const double2 c1[3] = {sx, sy, sz}, c2[3] = {sxy, syz, szx}, c3[3] = {sy, sz, sx};
+            const int q1[3] = {qx, qy, qz}, q2[3] = {qxy, qyz, qzx}, q3[3] = {qy, qz, qx};
+            int cd = 0;
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                const int n = arc_cut(s, c1[o]) + arc_cut(c1[o], c2[o]) + arc_cut(c2[o], c3[o]) + arc_cut(c3[o], s);
+                // defects_3d.f90:856-877: the four corners fill at least three of the quadrants
+                const int m = (1 << q) | (1 << q1[o]) | (1 << q2[o]) | (1 << q3[o]);
+                cd |= (n + 2) << (3 * o);
+                cd |= (__popc(m) >= 3 ? 1 : 0) << (9 + o);
+            }
+            code[z * NN + (int64_t)(y0 + ly) * N + x0 + lx] = (uint16_t)cd;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < PLAQ_COLS) counts[blockIdx.x * (int64_t)PLAQ_COLS + threadIdx.x] = sc[threadIdx.x];
+}
+
+// Per 256-site block of the site order (the granularity of the compaction): the 16 count
+// columns from the face codes, one wave per block.
+__global__ void __launch_bounds__(256) plaquette_count_kernel(int64_t n3, int64_t nblk, const uint16_t* __restrict__ code,
+                                                              int64_t* __restrict__ counts)
+{
+    const int64_t b = blockIdx.x * (int64_t)4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (b >= nblk) return;  // uniform per wave
+    int c[PLAQ_COLS];
+#pragma unroll
+    for (int j = 0; j < PLAQ_COLS; j++) c[j] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t i = b * PLAQ_NT + k * 64 + lane;
+        if (i < n3) {
+            const int cd = code[i];
+#pragma unroll
+            for (int o = 0; o < 3; o++) {
+                const int n = ((cd >> (3 * o)) & 7) - 2, q = (cd >> (9 + o)) & 1;
+                c[5 * o + 0] += n != 0;
+                c[5 * o + 1] += n == 1;
+                c[5 * o + 2] += n == -1;
+                c[5 * o + 3] += q;
+                c[5 * o + 4] += n;
+                c[15] += n != 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PLAQ_COLS; j++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c[j] += __shfl_xor(c[j], o);
+    }
+    if (lane < PLAQ_COLS) {
+        int v = 0;
+#pragma unroll
+        for (int j = 0; j < PLAQ_COLS; j++) v = lane == j ? c[j] : v;
+        counts[b * PLAQ_COLS + lane] = v;
+    }
 }
 
 // Pierced faces in site order, then face order: rows (ix, iy, iz, o, n), 1-based site and o.
@@ -134,8 +181,13 @@ void launch_plaquettes(hipStream_t s, int N, const void* phi, const void* cphi, 
 {
     using T2 = typename V2<T>::type;
     const int64_t n3 = (int64_t)N * N * N, nblk = (n3 + PLAQ_NT - 1) / PLAQ_NT;
-    hipLaunchKernelGGL((plaquette_kernel<T, COMP>), dim3((unsigned)nblk), dim3(PLAQ_NT), 0, s, N, (const T2*)phi,
-                       (const short2*)cphi, code, counts);
+    const int zc = min(N, PQ_ZC);
+    const dim3 grid((unsigned)((N + PQ_TX - 1) / PQ_TX), (unsigned)((N + PQ_TY - 1) / PQ_TY), (unsigned)((N + zc - 1) / zc));
+    hipLaunchKernelGGL((plaquette_kernel<T, COMP>), grid, dim3(PQ_NT), 0, s, N, zc, (const T2*)phi, (const short2*)cphi,
+                       code);
+    AXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(plaquette_count_kernel, dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, n3, nblk,
+                       (const uint16_t*)code, counts);
     AXG_HIP(hipGetLastError());
     // offsets of the pierced totals (column 15) and the totals of all 16 columns
     launch_scan_counts(s, counts, offs, nblk, total, 15, PLAQ_COLS, part);

@@ -351,11 +351,11 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
             const T2* rowp = s + q * N;
             T2 xk = rowp[kk];
             T2 xm = rowp[ring_neg<N>(kk)];
-            wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
-            wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+            st_stream(wa + kk, T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)});
+            st_stream(wa + cs + kk, T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)});
         } else {
-            wa[kk] = T2{0, 0};
-            wa[cs + kk] = T2{0, 0};
+            st_stream(wa + kk, T2{0, 0});
+            st_stream(wa + cs + kk, T2{0, 0});
         }
     }
 }
@@ -371,16 +371,37 @@ struct alignas(8) RPair {
     short2 a, b;
 };
 
-template <typename T, bool COMP>
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// NT: non-temporal (streamed) accesses, for the pairs a pass touches once
+template <typename T, bool COMP, bool NT = false>
 __device__ __forceinline__ void ld_pair(const typename V2<T>::type* __restrict__ f, const short2* __restrict__ cf,
                                         int64_t i, double2& u, double2& w)
 {
-    const CPair<T> v = reinterpret_cast<const CPair<T>*>(f + i)[0];
+    CPair<T> v;
+    if constexpr (NT && sizeof(T) == 4) {
+        const f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(f + i));
+        v.a = {q.x, q.y};
+        v.b = {q.z, q.w};
+    } else if constexpr (NT) {
+        v.a = ld_stream(f + i);
+        v.b = ld_stream(f + i + 1);
+    } else {
+        v = reinterpret_cast<const CPair<T>*>(f + i)[0];
+    }
     if constexpr (sizeof(T) == 8) {
         u = v.a;
         w = v.b;
     } else if constexpr (COMP) {
-        const RPair r = reinterpret_cast<const RPair*>(cf + i)[0];
+        RPair r;
+        if constexpr (NT) {
+            const s16x4 q = __builtin_nontemporal_load(reinterpret_cast<const s16x4*>(cf + i));
+            r.a = {q.x, q.y};
+            r.b = {q.z, q.w};
+        } else {
+            r = reinterpret_cast<const RPair*>(cf + i)[0];
+        }
         u = double2{comp_get(v.a.x, r.a.x), comp_get(v.a.y, r.a.y)};
         w = double2{comp_get(v.b.x, r.b.x), comp_get(v.b.y, r.b.y)};
     } else {
@@ -389,7 +410,7 @@ __device__ __forceinline__ void ld_pair(const typename V2<T>::type* __restrict__
     }
 }
 
-template <typename T, bool COMP>
+template <typename T, bool COMP, bool NT = false>
 __device__ __forceinline__ void st_pair(typename V2<T>::type* __restrict__ f, short2* __restrict__ cf, int64_t i,
                                         double2 u, double2 w)
 {
@@ -402,12 +423,22 @@ __device__ __forceinline__ void st_pair(typename V2<T>::type* __restrict__ f, sh
         RPair r;
         v.a = T2{comp_put(u.x, r.a.x), comp_put(u.y, r.a.y)};
         v.b = T2{comp_put(w.x, r.b.x), comp_put(w.y, r.b.y)};
-        reinterpret_cast<RPair*>(cf + i)[0] = r;
+        if constexpr (NT)
+            __builtin_nontemporal_store(s16x4{r.a.x, r.a.y, r.b.x, r.b.y}, reinterpret_cast<s16x4*>(cf + i));
+        else
+            reinterpret_cast<RPair*>(cf + i)[0] = r;
     } else {
         v.a = T2{(float)u.x, (float)u.y};
         v.b = T2{(float)w.x, (float)w.y};
     }
-    reinterpret_cast<CPair<T>*>(f + i)[0] = v;
+    if constexpr (NT && sizeof(T) == 4) {
+        __builtin_nontemporal_store(f32x4{v.a.x, v.a.y, v.b.x, v.b.y}, reinterpret_cast<f32x4*>(f + i));
+    } else if constexpr (NT) {
+        st_stream(f + i, v.a);
+        st_stream(f + i + 1, v.b);
+    } else {
+        reinterpret_cast<CPair<T>*>(f + i)[0] = v;
+    }
 }
 
 template <typename T, bool COMP, int N, int XR, bool STRIP>
@@ -460,13 +491,13 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)z * NN + (int64_t)yp * N + x0, b20, b21);
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)zm * NN + (int64_t)y * N + x0, z10, z11);
         ld_pair<T, COMP>(phi_in, cphi_in, (int64_t)zp * NN + (int64_t)y * N + x0, z20, z21);
-        ld_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
+        ld_pair<T, COMP, true>(pi, cpi, row + x0, p0, p1);
         double2 f0, f1;
         T2 sa[3], sb[3];
         kick_site<T>(c0, l, c1, b10, b20, z10, z20, p0, f0, sa, k, inv_d, d2, div != 0, dt_drift);
         kick_site<T>(c1, c0, r, b11, b21, z11, z21, p1, f1, sb, k, inv_d, d2, div != 0, dt_drift);
-        st_pair<T, COMP>(pi, cpi, row + x0, p0, p1);
-        st_pair<T, COMP>(phi_out, cphi_out, row + x0, f0, f1);
+        st_pair<T, COMP, true>(pi, cpi, row + x0, p0, p1);
+        st_pair<T, COMP, true>(phi_out, cphi_out, row + x0, f0, f1);
 #pragma unroll
         for (int m = 0; m < 3; m++) {
             CPair<T> v;
@@ -512,11 +543,11 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
             const T2* rowp = s + qq * N;
             T2 xk = rowp[kk];
             T2 xm = rowp[ring_neg<N>(kk)];
-            wa[kk] = T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)};
-            wa[cs + kk] = T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)};
+            st_stream(wa + kk, T2{(T)0.5 * (xk.x + xm.x), (T)0.5 * (xk.y - xm.y)});
+            st_stream(wa + cs + kk, T2{(T)0.5 * (xk.y + xm.y), (T)0.5 * (xm.x - xk.x)});
         } else {
-            wa[kk] = T2{0, 0};
-            wa[cs + kk] = T2{0, 0};
+            st_stream(wa + kk, T2{0, 0});
+            st_stream(wa + cs + kk, T2{0, 0});
         }
     }
 }
