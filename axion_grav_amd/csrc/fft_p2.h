@@ -302,6 +302,50 @@ __device__ __forceinline__ void fft_p2(typename V2<T>::type (&v)[8], int j0, typ
     }
 }
 
+// Two transforms per thread, the LDS tile's columns off0 and off1, through the same stages: the
+// exchanges of both share one pair of barriers.
+template <typename T, int N, int S>
+__device__ __forceinline__ void p2_exchange2(typename V2<T>::type (&v0)[8], typename V2<T>::type (&v1)[8], int j0,
+                                             typename V2<T>::type* s, int es, int off0, int off1)
+{
+    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int j = j0 + m * TPT;
+        const int k = j & (Ns - 1);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            s[(base + r * Ns) * es + off0] = v0[m + r * M];
+            s[(base + r * Ns) * es + off1] = v1[m + r * M];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        v0[e] = s[(j0 + e * TPT) * es + off0];
+        v1[e] = s[(j0 + e * TPT) * es + off1];
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int SIGN, int S = 0, typename TW>
+__device__ __forceinline__ void fft_p2_2(typename V2<T>::type (&v0)[8], typename V2<T>::type (&v1)[8], int j0,
+                                         typename V2<T>::type* s, int es, int off0, int off1, const TW& tws)
+{
+    if constexpr (S < P2<N>::NST) {
+        p2_stage<T, N, S, SIGN>(v0, tws);
+        p2_stage<T, N, S, SIGN>(v1, tws);
+        if constexpr (S + 1 < P2<N>::NST) {
+            p2_exchange2<T, N, S>(v0, v1, j0, s, es, off0, off1);
+        } else if constexpr (P2<N>::R3) {
+            p3_finish<T, N, SIGN>(true, v0, j0, s, es, off0, tws);
+            p3_finish<T, N, SIGN>(true, v1, j0, s, es, off1, tws);
+        }
+        fft_p2_2<T, N, SIGN, S + 1>(v0, v1, j0, s, es, off0, off1, tws);
+    }
+}
+
 // Column-tile width of the specialized passes: 128 bytes of complex per row, bounded so the
 // block (TX * N/8 threads) stays within 1024 threads and the tile within 64 KiB of LDS.
 template <typename T, int N> constexpr int p2_tx()

@@ -158,6 +158,88 @@ __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typen
 }
 
 
+// Column-tile width of the y pass with two adjacent kx columns per thread: a whole 128-byte line
+// of complex per row, for the sizes whose one-column tile is narrower (the 1024-thread block
+// limit at N = 768 and 1024 in FP32, the 64 KiB tile of the one-column kernels in FP64); 0 where
+// it does not apply.  Its LDS tile, TX * N complex (up to 128 KiB), fits gfx950's 160 KiB per
+// workgroup.
+template <typename T, int N> constexpr int p2_tx2()
+{
+    constexpr int bytes = (int)sizeof(typename V2<T>::type), tx = 128 / bytes;
+    return (p2_tx<T, N>() < tx && (tx / 2) * (N / 8) <= 1024 && tx * N * bytes <= 160 * 1024) ? tx : 0;
+}
+
+// The y pass of gw_ypass_p2_kernel with two adjacent columns kx, kx + 1 per thread: W is read in
+// whole 128-byte lines (16-byte FP32 / 32-byte FP64 accesses per thread), and each thread writes
+// its column pair as one access into the tiled W2 (pairs never straddle a tile: kx is even).
+// The two columns' transforms share the stages and barriers (fft_p2_2).
+template <typename T, int N, int TX, int KZ>
+__global__ void __launch_bounds__((TX / 2) * (N / 8)) gw_ypass_p2w_kernel(const typename V2<T>::type* __restrict__ W,
+                                                                         typename V2<T>::type* __restrict__ W2,
+                                                                         const typename V2<T>::type* __restrict__ tw,
+                                                                         const LatRef* __restrict__ lat)
+{
+    using T2 = typename V2<T>::type;
+    if (lat) {
+        W = (const T2*)lat[blockIdx.z].W;
+        W2 = (T2*)lat[blockIdx.z].W2;
+    }
+    constexpr int TPT = N / 8, NH = N / 2 + 1, P = half_pitch_c<N>(), H = TX / 2;
+    using L = TiledLayout<N, p2_ztx<T, N>()>;
+    static_assert(L::ZTW % 2 == 0 && P % 2 == 0, "column pairs stay within a tile and a 16-byte line");
+    __shared__ __align__(16) unsigned char smem[TX * N * sizeof(T2)];
+    T2* s = reinterpret_cast<T2*>(smem);
+    const int b = threadIdx.x % H, j0 = threadIdx.x / H;
+    const int kx = blockIdx.x * TX + 2 * b;
+    const int c = blockIdx.y / (N / KZ), z0 = (blockIdx.y % (N / KZ)) * KZ;
+    T2 v0[KZ][8], v1[KZ][8];
+#pragma unroll
+    for (int q = 0; q < KZ; q++) {
+        const T2* base = W + ((int64_t)c * N + z0 + q) * (int64_t)N * P + kx;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const T2* r = base + (int64_t)(j0 + e * TPT) * P;
+            if (kx + 1 < NH) {
+                if constexpr (sizeof(T) == 4) {
+                    typedef float f32x4 __attribute__((ext_vector_type(4)));
+                    const f32x4 u = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(r));
+                    v0[q][e] = T2{u.x, u.y};
+                    v1[q][e] = T2{u.z, u.w};
+                } else {
+                    v0[q][e] = ld_stream(r);
+                    v1[q][e] = ld_stream(r + 1);
+                }
+            } else {
+                v0[q][e] = kx < NH ? ld_stream(r) : T2{0, 0};
+                v1[q][e] = T2{0, 0};
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KZ; q++) {
+        if (q > 0) __syncthreads();
+        fft_p2_2<T, N, +1>(v0[q], v1[q], j0, s, TX, 2 * b, 2 * b + 1, P2TwTable<T, N, +1>{tw, j0});
+    }
+    // the padding columns of the last tile get zeros, as in gw_ypass_p2_kernel; columns past the
+    // last tile do not exist
+    if (kx < L::NT * L::ZTW) {
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+#pragma unroll
+            for (int q = 0; q < KZ; q++) {
+                T2* o = W2 + L::idx(c, z0 + q, j0 + e * TPT, kx);
+                const T2 a = kx < NH ? v0[q][e] : T2{0, 0}, bb = kx + 1 < NH ? v1[q][e] : T2{0, 0};
+                if constexpr (sizeof(T) == 4) {
+                    typedef float f32x4 __attribute__((ext_vector_type(4)));
+                    *reinterpret_cast<f32x4*>(o) = f32x4{a.x, a.y, bb.x, bb.y};
+                } else {
+                    o[0] = a;
+                    o[1] = bb;
+                }
+            }
+    }
+}
+
 template <typename T, int N>
 __global__ void __launch_bounds__((p2_ztx<T, N>() * (N / 8))) gw_zpass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                            typename V2<T>::type* __restrict__ A,
@@ -663,16 +745,30 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
             if (!TiledLayout<NN, p2_ztx<T, NN>()>::matches(L) || pitch != half_pitch_c<NN>())
                 throw Error(AXG_EINVAL, "y pass: layout does not match the compiled tile");
             // variant 1: half-width column tiles (more, smaller blocks per CU); variant 2: two
-            // z planes per block
+            // z planes per block.  Where the one-column tile is narrower than a 128-byte line
+            // (p2_tx2), variants 0 and 2 run the two-column kernel instead and variant 1 the
+            // one-column tile.
             auto go = [&]<int TXP, int KZ>() {
                 hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP, KZ>),
                                    dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ, nlat), dim3(TXP * (NN / 8)),
                                    (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
             };
-            constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1;
-            if (variant == 1) go.template operator()<TXH, 1>();
-            else if (variant == 2) go.template operator()<TXP, 2>();
-            else go.template operator()<TXP, 1>();
+            constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1, TX2 = p2_tx2<T, NN>();
+            if constexpr (TX2 > 0) {
+                auto go2 = [&]<int KZ>() {
+                    using L = TiledLayout<NN, p2_ztx<T, NN>()>;
+                    constexpr int cols = L::NT * L::ZTW > NN / 2 + 1 ? L::NT * L::ZTW : NN / 2 + 1;
+                    hipLaunchKernelGGL((gw_ypass_p2w_kernel<T, NN, TX2, KZ>), dim3((cols + TX2 - 1) / TX2, 6 * NN / KZ, nlat),
+                                       dim3((TX2 / 2) * (NN / 8)), 0, s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
+                };
+                if (variant == 1) go.template operator()<TXP, 1>();
+                else if (variant == 2) go2.template operator()<2>();
+                else go2.template operator()<1>();
+            } else {
+                if (variant == 1) go.template operator()<TXH, 1>();
+                else if (variant == 2) go.template operator()<TXP, 2>();
+                else go.template operator()<TXP, 1>();
+            }
         });
         if (!done) throw Error(AXG_EINVAL, "tiled y pass needs N = 2^k or 3 * 2^k");
         AXG_HIP(hipGetLastError());

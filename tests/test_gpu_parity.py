@@ -221,6 +221,27 @@ def test_ypass_variants_bit_identical(N, prec, monkeypatch):
         assert np.array_equal(A, out[0])
 
 
+@pytest.mark.parametrize("N,prec", [(768, capi.FP32C), (768, capi.FP64), (1024, capi.FP32C)])
+def test_ypass_line_tiles_large(N, prec, monkeypatch):
+    """At N = 768 and 1024 the one-column y-pass tile is narrower than a 128-byte line (8 FP32 or
+    4 FP64 columns: the 1024-thread block, the 64 KiB tile); variants 0 and 2 then run two
+    adjacent columns per thread over a 128-byte-wide tile (96-128 KiB of LDS), variant 1 the
+    one-column tile.  One step from the same field gives the same GW spectrum sums bit for bit."""
+    say("y line tiles", N, prec)
+    prm, _ = params(N, precision=prec, num_bin_grav=64)
+    out = []
+    for v in ("0", "1", "2"):
+        monkeypatch.setenv("AXG_YPASS_VARIANT", v)
+        with capi.Lattice(prm) as L:
+            L.init_flat(3, 1.0)
+            L.advance_steps(1.0, 0.05, 1)
+            s = L.gw_spectrum()
+        out.append((s["numer"], s["denom"]))
+    assert np.any(out[0][0] > 0) and np.all(np.isfinite(out[0][0]))
+    for nu, de in out[1:]:
+        assert np.array_equal(nu, out[0][0]) and np.array_equal(de, out[0][1])
+
+
 @pytest.mark.parametrize("N", [32, 48])
 def test_gw_unfused_x_pass_matches_fused(N, monkeypatch):
     """AXG_XFUSE=0 runs the x transform as its own pass (kick_drift_kernel + gw_xpass_p2_kernel):
