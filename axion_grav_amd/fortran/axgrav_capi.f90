@@ -44,6 +44,11 @@ module axgrav_capi
      integer(c_int64_t) :: n_string = 0, n_wall = 0, n_mask = 0
   end type axg_defect_out
 
+  ! struct axg_plaquette_out: per face orientation (x,y), (y,z), (z,x)
+  type, bind(C) :: axg_plaquette_out
+     integer(c_int64_t) :: n_pierced(3) = 0, n_plus(3) = 0, n_minus(3) = 0, n_quadrant(3) = 0, net(3) = 0
+  end type axg_plaquette_out
+
   ! struct axg_string_out
   type, bind(C) :: axg_string_out
      integer(c_int64_t) :: n_string = 0, n_vel = 0
@@ -122,6 +127,21 @@ module axgrav_capi
        type(c_ptr), value :: h
        integer(c_int64_t), intent(out) :: position
      end function axg_get_stream
+
+     integer(c_int) function axg_plaquettes(h, out) bind(C, name="axg_plaquettes")
+       import :: c_int, c_ptr, axg_plaquette_out
+       type(c_ptr), value :: h
+       type(axg_plaquette_out), intent(out) :: out
+     end function axg_plaquettes
+
+     ! rec(5, cap): (ix, iy, iz, o, n) per pierced face; cap = 0 is a size query
+     integer(c_int) function axg_get_plaquettes(h, rec, cap, count) bind(C, name="axg_get_plaquettes")
+       import :: c_int, c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(out) :: rec(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: count
+     end function axg_get_plaquettes
 
      integer(c_int) function axg_init_flat(h, seed, kmax) bind(C, name="axg_init_flat")
        import :: c_int, c_ptr, c_int64_t, c_double

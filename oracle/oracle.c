@@ -17,6 +17,16 @@
  * Layout: a complex field is interleaved (re, im) doubles, N^3 sites, x fastest:
  * site (x, y, z) -> x + N*(y + N*z), exactly the Fortran complex(C_DOUBLE_COMPLEX)
  * array data1(1:N,1:N,1:N) without halo planes.
+ *
+ * Parity unpinned (no reference run, golden vector or closed form fixes them):
+ *  - FFT rounding: orc_fft3d is pinned to the DFT (brute force, numpy), not to the
+ *    unversioned FFTW/MKL cdft the reference links;
+ *  - LAPACK rounding: orc_pp_inverse is pinned to numpy's inverse, not to MKL's
+ *    dpptrf/dpptri;
+ *  - the positions in the MCG59 stream at which the *_mcg functions place their
+ *    draws (4 N^3 per thermal field, 4004 per string cell, 4 N^3 per covariance
+ *    realisation) follow the reference's skip-ahead calls as read; the generator
+ *    itself is pinned to MKL's own draws (tests/golden/mcg59_mkl.npz).
  */
 #include <math.h>
 #include <stdint.h>
