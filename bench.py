@@ -27,6 +27,7 @@ over 2000 steps at 512^3.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -369,6 +370,20 @@ def cpu_model():
     return "unknown"
 
 
+def oracle_for_this_host():
+    """oracle/oracle.c compiled for this host's CPU (-march=native, the same -ffp-contract=off
+    FP64 order) into a temporary directory; None when the host has no compiler for it."""
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="axg_oracle_"), "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+           "-std=c11", "-shared", "-o", out, os.path.join(HERE, "oracle", "oracle.c"), "-lm"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    return out if r.returncode == 0 else None
+
+
 def run_cpu(N, L, seed, steps=3, warmup=1):
     import oracle as O
 
@@ -470,11 +485,16 @@ def main():
     cpu = None
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"CPU oracle: 1 + 3 steps at {N}^3 on {cores} threads")
+        native = oracle_for_this_host()
+        march = "-march=native" if native else "-march=x86-64-v3"
+        if native:
+            import oracle as O
+            O.use_library(native)
+        log(f"CPU oracle ({march}): 1 + 3 steps at {N}^3 on {cores} threads")
         su, el = run_cpu(N, L, args.seed)
         cpu = dict(value=su, unit="site-updates/s", cores=cores, kind="port",
                    sample=f"3 DKD steps with GW at {N}^3 after 1 untimed (FP64 oracle, unfused reference dataflow, "
-                          f"-O3 -march=x86-64-v3), {el:.1f} s; {cpu_model()}, OMP_PROC_BIND="
+                          f"-O3 {march} -ffp-contract=off), {el:.1f} s; {cpu_model()}, OMP_PROC_BIND="
                           f"{os.environ.get('OMP_PROC_BIND', 'unset')}")
         log(f"CPU {su:.4g} site-updates/s ({el:.1f} s)")
 

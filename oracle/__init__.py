@@ -22,15 +22,27 @@ _LIB = None
 PI_T = 3.14159265359  # potential.f90:8
 
 
+_LIB_PATH = None  # another build of oracle.c to load instead of liboracle.so (use_library)
+
+
 def build():
     subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def use_library(path):
+    """Load the oracle from `path` (a build of the same oracle.c, e.g. for the host's own CPU)
+    instead of liboracle.so; before the first call only."""
+    global _LIB_PATH
+    if _LIB is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
 
 
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
+        path = _LIB_PATH or os.path.join(_HERE, "liboracle.so")
+        if _LIB_PATH is None and not os.path.exists(path):
             build()
         L = C.CDLL(path)
         d, i64, i32, p = C.c_double, C.c_int64, C.c_int32, C.c_void_p
