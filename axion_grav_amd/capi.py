@@ -45,7 +45,8 @@ class Params(C.Structure):
         ("dtau_fixed", C.c_double),
         ("potential_kappa0", C.c_double),
         ("degrade_spec", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("string_vel", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -201,9 +202,10 @@ def hbm_probe(nbytes=4 << 30, reps=10, device=0):
 def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initial_loop=True, kappa=1e-30,
                 num_bin_grav=25, num_bin=50, id_strings=True, id_walls=True, spec_est=True, tau_cr=1.0,
                 a_now=1.0, eps_dtau=0.0, dtau_fixed=0.0, gw_fix_swap=False, potential_kappa0=0.0, degrade_spec=0,
-                device=0):
+                string_vel=0, device=0):
     """Params from a lattice spacing and sigma_cr (dwav, dv_spa, dv_wav as settings.f90:295-297).
-    degrade_spec > 0 puts the GW source and the axion spectrum on the grid ngrid / 2^degrade_spec."""
+    degrade_spec > 0 puts the GW source and the axion spectrum on the grid ngrid / 2^degrade_spec;
+    string_vel = 1 takes string velocities from velocity_string_YY (the reference's VEL = 1 build)."""
     twopi_t = 2 * 3.14159265359
     P = Params()
     P.ngrid = int(ngrid)
@@ -228,6 +230,7 @@ def make_params(ngrid, dspa, sigma_cr, *, precision=FP64, grav_spec=True, initia
     P.dtau_fixed = dtau_fixed
     P.potential_kappa0 = potential_kappa0
     P.degrade_spec = int(degrade_spec)
+    P.string_vel = int(string_vel)
     return P
 
 

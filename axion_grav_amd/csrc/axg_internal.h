@@ -250,7 +250,7 @@ constexpr int STRING_STATE = 26;
 constexpr int STRING_SPLIT = 64;  // default split (fastest of 2..1000 at 512^3 on MI355X)
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split,
+                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split, int vel_yy,
                        double* rec, double* partial, int* npend, int64_t* pend, double* state);
 
 // Exclusive offsets of column col of per-block counts in[n][ncol] (ncol 3 or PLAQ_COLS) and the

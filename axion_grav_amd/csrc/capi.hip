@@ -462,6 +462,7 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
                 "precision must be AXG_FP64 or AXG_FP32C");
         require(p.dspa > 0 && p.dwav > 0 && p.dv_spa > 0 && p.dv_wav > 0 && p.sigma_cr > 0, "derived units missing");
         require(p.num_bin_grav >= 1 && p.num_bin_grav <= 4096, "num_bin_grav out of range");
+        require(p.string_vel == 0 || p.string_vel == 1, "string_vel must be 0 (VEL = 0 build) or 1 (VEL = 1)");
         require(p.degrade_spec >= 0 && p.degrade_spec <= 7 && supported_n(p.ngrid >> p.degrade_spec) &&
                     ((p.ngrid >> p.degrade_spec) << p.degrade_spec) == p.ngrid,
                 "degrade_spec must leave an even 2^a 3^b grid of at least 8 (ngrid / 2^degrade_spec)");
@@ -1225,15 +1226,15 @@ extern "C" int axg_strings(axg_handle* h, uint32_t seed0, uint32_t seed1, axg_st
             double* state = h->str_state + 1;
             if (h->comp)
                 launch_string_fit<float, true>(h->stream, h->N, h->phi[h->cur], h->cphi[h->cur], h->pi, h->cpi,
-                                               h->str_cells, ns, dr, h->p.dspa, h->string_split, h->str_rec,
+                                               h->str_cells, ns, dr, h->p.dspa, h->string_split, h->p.string_vel, h->str_rec,
                                                h->str_part, npend, h->str_pend, state);
             else if (h->fp32)
                 launch_string_fit<float, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr, h->str_cells,
-                                                ns, dr, h->p.dspa, h->string_split, h->str_rec, h->str_part,
+                                                ns, dr, h->p.dspa, h->string_split, h->p.string_vel, h->str_rec, h->str_part,
                                                 npend, h->str_pend, state);
             else
                 launch_string_fit<double, false>(h->stream, h->N, h->phi[h->cur], nullptr, h->pi, nullptr,
-                                                 h->str_cells, ns, dr, h->p.dspa, h->string_split,
+                                                 h->str_cells, ns, dr, h->p.dspa, h->string_split, h->p.string_vel,
                                                  h->str_rec, h->str_part, npend, h->str_pend, state);
             sum_columns(h->stream, h->str_part, (int)nsb, 5, dsum);
             AXG_HIP(hipMemcpyAsync(sums, dsum, sizeof sums, hipMemcpyDeviceToHost, h->stream));

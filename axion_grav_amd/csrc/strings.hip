@@ -512,6 +512,147 @@ __global__ void __launch_bounds__(STRING_NT2) string_fit_rest_kernel(
     finish_string<T, COMP>(phi, cphi, pi, cpi, sc, pos, dir, gof, sl, dspa, rec, s);
 }
 
+// velocity_string_YY (defects_3d.f90:824-982, the reference's VEL = 1 build) for string cell s,
+// written over the record's velocity.  For each face orientation (u, v, w) = cshift(x, y, z) and
+// each of the cell's two faces normal to w: the quadrant test of its four corners (three of the
+// four quadrants of arg phi filled); on a pierced face, the least-squares plane of phi over the
+// corners (whose normal matrix is diag(4, 1, 1), so the packed inverse is diag(1/4, 1, 1)
+// exactly) gives the penetration point (du, dv), clamped to the face; the corners' pi and central
+// differences of phi, weighted (0.5 + sign(du, i - 1/2))(0.5 + sign(dv, j - 1/2)) as the
+// reference writes them, give the face's velocity.  The cell's velocity is the reference's
+// average over the pierced faces, which adds the last face's velocity npen times (its index
+// npen in place of the loop's), or 1e-10 per component when no face is pierced.  In z the
+// corners of the upper face of the top cell plane lie in the halo plane: the same values as
+// plane 0, but a one-sided difference.
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(STRING_NT) string_vel_yy_kernel(int N, const typename V2<T>::type* __restrict__ phi,
+                                                                  const short2* __restrict__ cphi,
+                                                                  const typename V2<T>::type* __restrict__ pi,
+                                                                  const short2* __restrict__ cpi,
+                                                                  const int32_t* __restrict__ cells, int64_t ns,
+                                                                  double dspa, double* __restrict__ rec)
+{
+#pragma clang fp contract(off)
+    const int64_t s = blockIdx.x * (int64_t)STRING_NT + threadIdx.x;
+    if (s >= ns) return;
+    const int64_t cell = cells[s], NN = (int64_t)N * N;
+    const int c0[3] = {(int)(cell % N), (int)((cell / N) % N), (int)(cell / NN)};
+    auto site = [&](int x, int y, int z) { return (int64_t)z * NN + (int64_t)y * N + x; };
+    int npen = 0;
+    double last[3] = {0, 0, 0};
+    for (int pl = 0; pl < 3; pl++) {
+        const int au = pl, av = (pl + 1) % 3, aw = (pl + 2) % 3;
+        for (int k = 0; k < 2; k++) {
+            // corner (i, j): x and y wrapped, z left at N for the halo plane
+            auto corner = [&](int i, int j, int c[3]) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) c[a] = c0[a] + (a == au ? i : 0) + (a == av ? j : 0) + (a == aw ? k : 0);
+                if (c[0] >= N) c[0] -= N;
+                if (c[1] >= N) c[1] -= N;
+            };
+            int mask = 0, n = 0;
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    int c[3];
+                    corner(i, j, c);
+                    const double2 f = ldf<T, COMP>(phi, cphi, site(c[0], c[1], c[2] == N ? 0 : c[2]));
+                    const int m = (int)ceil(2 * atan2(f.y, f.x) / AXG_PI_T) + 2;
+                    if (!((mask >> m) & 1)) {
+                        mask |= 1 << m;
+                        n++;
+                    }
+                }
+            if (n < 3) continue;
+            npen++;
+            double2 vp[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    int c[3];
+                    corner(i, j, c);
+                    const double2 f = ldf<T, COMP>(phi, cphi, site(c[0], c[1], c[2] == N ? 0 : c[2]));
+                    const double w[3] = {1.0, i - 0.5, j - 0.5};
+#pragma unroll
+                    for (int m = 0; m < 3; m++) {
+                        vp[m].x = vp[m].x + w[m] * f.x;
+                        vp[m].y = vp[m].y + w[m] * f.y;
+                    }
+                }
+            const double d01 = 0.25 * vp[0].x, d02 = 0.25 * vp[0].y;
+            const double d11 = vp[1].x, d12 = vp[1].y, d21 = vp[2].x, d22 = vp[2].y;
+            const double det = d11 * d22 - d21 * d12;
+            double du = -((d22 * d01 - d21 * d02) / det);
+            double dv = -((-(d12 * d01) + d11 * d02) / det);
+            du = fmax(fmin(du, 0.5), -0.5);
+            dv = fmax(fmin(dv, 0.5), -0.5);
+            double2 dp[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    const double wt = (0.5 + (i ? fabs(du) : -fabs(du))) * (0.5 + (j ? fabs(dv) : -fabs(dv)));
+                    int c[3];
+                    corner(i, j, c);
+                    const int cz = c[2] == N ? 0 : c[2];
+                    const double2 q = ldf<T, COMP>(pi, cpi, site(c[0], c[1], cz));
+                    dp[0].x = dp[0].x + wt * q.x;
+                    dp[0].y = dp[0].y + wt * q.y;
+                    const int xp = c[0] + 1 == N ? 0 : c[0] + 1, xm = c[0] == 0 ? N - 1 : c[0] - 1;
+                    const int yp = c[1] + 1 == N ? 0 : c[1] + 1, ym = c[1] == 0 ? N - 1 : c[1] - 1;
+                    double2 a = ldf<T, COMP>(phi, cphi, site(xp, c[1], cz)), b = ldf<T, COMP>(phi, cphi, site(xm, c[1], cz));
+                    dp[1].x = dp[1].x + wt * (a.x - b.x) / 2;
+                    dp[1].y = dp[1].y + wt * (a.y - b.y) / 2;
+                    a = ldf<T, COMP>(phi, cphi, site(c[0], yp, cz));
+                    b = ldf<T, COMP>(phi, cphi, site(c[0], ym, cz));
+                    dp[2].x = dp[2].x + wt * (a.x - b.x) / 2;
+                    dp[2].y = dp[2].y + wt * (a.y - b.y) / 2;
+                    // z: central but one-sided from the halo plane (c[2] == N)
+                    int zp, zm;
+                    double dz;
+                    if (c[2] == N) {
+                        zp = 0;
+                        zm = N - 1;
+                        dz = 1;
+                    } else {
+                        zp = c[2] + 1 == N ? 0 : c[2] + 1;
+                        zm = c[2] == 0 ? N - 1 : c[2] - 1;
+                        dz = 2;
+                    }
+                    a = ldf<T, COMP>(phi, cphi, site(c[0], c[1], zp));
+                    b = ldf<T, COMP>(phi, cphi, site(c[0], c[1], zm));
+                    dp[3].x = dp[3].x + wt * (a.x - b.x) / dz;
+                    dp[3].y = dp[3].y + wt * (a.y - b.y) / dz;
+                }
+            double a11 = 0, a12 = 0, a22 = 0;
+#pragma unroll
+            for (int m = 1; m <= 3; m++) {
+                a11 = a11 + dp[m].x * dp[m].x;
+                a12 = a12 + dp[m].x * dp[m].y;
+                a22 = a22 + dp[m].y * dp[m].y;
+            }
+            const double dt = a11 * a22 - a12 * a12;
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const double v = dp[0].x * (a22 * dp[m + 1].x - a12 * dp[m + 1].y) +
+                                 dp[0].y * (a11 * dp[m + 1].y - a12 * dp[m + 1].x);
+                last[m] = -(v / dt * dspa);
+            }
+        }
+    }
+    double vel[3];
+    if (npen != 0) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            double t = 0;
+            for (int i = 0; i < npen; i++) t = t + last[m];
+            vel[m] = t / npen;
+        }
+    } else {
+        vel[0] = vel[1] = vel[2] = 1.e-10;
+    }
+    double* o = rec + 11 * s;
+    o[8] = vel[0];
+    o[9] = vel[1];
+    o[10] = vel[2];
+}
+
 // add_string_length's sums from the records, per block of STRING_NT strings in string order
 // (the order of the sums does not depend on which stage ended a search).
 __global__ void __launch_bounds__(STRING_NT) string_sum_kernel(const double* __restrict__ rec, int64_t ns,
@@ -547,7 +688,7 @@ __global__ void __launch_bounds__(STRING_NT) string_sum_kernel(const double* __r
 
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
-                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split,
+                       const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split, int vel_yy,
                        double* rec, double* partial, int* npend, int64_t* pend, double* state)
 {
     using T2 = typename V2<T>::type;
@@ -565,13 +706,18 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
                        (const short2*)cpi, cells, dspa, rec, (const int*)npend, (const int64_t*)pend,
                        (const double*)state);
     AXG_HIP(hipGetLastError());
+    if (vel_yy) {
+        hipLaunchKernelGGL((string_vel_yy_kernel<T, COMP>), dim3((unsigned)nblk), dim3(STRING_NT), 0, s, N,
+                           (const T2*)phi, (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, dspa, rec);
+        AXG_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(string_sum_kernel, dim3((unsigned)nblk), dim3(STRING_NT), 0, s, (const double*)rec, ns, partial);
     AXG_HIP(hipGetLastError());
 }
 
 #define AXG_STRING_FIT(T, C)                                                                                        \
     template void launch_string_fit<T, C>(hipStream_t, int, const void*, const void*, const void*, const void*,    \
-                                          const int32_t*, int64_t, const Draws&, double, int, double*,              \
+                                          const int32_t*, int64_t, const Draws&, double, int, int, double*,         \
                                           double*, int*, int64_t*, double*);
 AXG_STRING_FIT(double, false)
 AXG_STRING_FIT(float, false)

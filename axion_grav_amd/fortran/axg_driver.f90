@@ -201,6 +201,8 @@ program axion_3d
   prm%dtau_fixed = ini_double('dtau_fixed', 0d0)
   prm%potential_kappa0 = ini_double('potential_kappa0', 0d0)
   prm%degrade_spec = degrade_spec
+  ! the reference's build option VEL (Makefile:11) as a key: 1 = velocity_string_YY
+  prm%string_vel = ini_int('string_vel', 0)
   call ini_close()
   write(*, '(A,I0)') ' ngrid = ', ngrid
   write(*, '(A,ES22.14)') ' sigma_cr = ', prm%sigma_cr

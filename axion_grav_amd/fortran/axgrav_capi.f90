@@ -30,7 +30,8 @@ module axgrav_capi
      real(c_double) :: dtau_fixed = 0
      real(c_double) :: potential_kappa0 = 0
      integer(c_int32_t) :: degrade_spec = 0
-     integer(c_int32_t) :: reserved(7) = 0
+     integer(c_int32_t) :: string_vel = 0
+     integer(c_int32_t) :: reserved(6) = 0
   end type axg_params
 
   ! struct axg_keys

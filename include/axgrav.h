@@ -59,7 +59,10 @@ typedef struct axg_params {
                                spectrum live on the reduced grid N / 2^degrade_spec (>= 8), as
                                init_grav / init_spectrum set it up (grav_waves.f90:37-45,
                                spectrum.f90:42-49); 0 = the lattice grid */
-    int32_t reserved[7];
+    int32_t string_vel;     /* the string velocity estimator of the reference's build option VEL
+                               (Makefile:11): 0 = velocity_string (defects_3d.f90:605-690, the
+                               default build), 1 = velocity_string_YY (824-982) */
+    int32_t reserved[6];
 } axg_params;
 
 /* The ini keys axg_derive() needs (settings.f90:269-297). */

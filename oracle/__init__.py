@@ -73,7 +73,8 @@ def lib():
         L.orc_mcg59_uniforms.argtypes = [u32, u32, u64, i64, p]
         L.orc_mcg59_gauss.argtypes = [u32, u32, u64, i64, p]
         L.orc_strings_mcg.restype = i64
-        L.orc_strings_mcg.argtypes = [p, p, i64, p, u64, u64, d, p, p]
+        L.orc_strings_mcg.argtypes = [p, p, i64, p, u64, u64, d, C.c_int, p, p]
+        L.orc_velocity_string_yy.argtypes = [p, p, i64, i64, i64, i64, d, p]
         L.orc_ic_thermal_mcg.argtypes = [p, p, i64, u64, u64, d, d, d, d]
         L.orc_field_covariance_mcg.restype = C.c_int
         L.orc_field_covariance_mcg.argtypes = [p, i64, C.c_int, d, d, p, p, u64, u64, C.c_int, C.c_int, d, p, p]
@@ -96,7 +97,7 @@ def lib():
         L.orc_position_string.argtypes = [p, C.c_uint32, C.c_uint32, i64, p, p, p, p]
         L.orc_velocity_string.argtypes = [p, p, i64, i64, i64, i64, p, d, p]
         L.orc_strings.restype = i64
-        L.orc_strings.argtypes = [p, p, i64, p, C.c_uint32, C.c_uint32, d, p, p]
+        L.orc_strings.argtypes = [p, p, i64, p, C.c_uint32, C.c_uint32, d, C.c_int, p, p]
         L.orc_thermal_factor.restype = d
         L.orc_thermal_factor.argtypes = [C.c_int, i64, d, d, d, d]
         L.orc_thermal_gauss.argtypes = [C.c_uint64, C.c_int, C.c_uint64, p, p]
@@ -375,20 +376,28 @@ def velocity_string(phi, pi, x, y, z, pos, dspa):
     return vel
 
 
-def strings(phi, pi, flags, key0, key1, dspa, stream=None):
+def velocity_string_yy(phi, pi, x, y, z, dspa):
+    """velocity_string_YY (the VEL = 1 build) of the cell with corner (x, y, z), 0-based."""
+    vel = np.zeros(3)
+    lib().orc_velocity_string_yy(_p(phi), _p(pi), phi.shape[0], x, y, z, dspa, _p(vel))
+    return vel
+
+
+def strings(phi, pi, flags, key0, key1, dspa, stream=None, vel_yy=False):
     """All string cells (flag bit 1) in site order: records (ns, 11) = pos(3, site coordinates
     counted from 0), dir(3), gof, sl, vel(3); sums = (sum sl, n |v|<1, sum |v|, sum v^2, sum gamma).
-    stream = (w0, w1, pos): starting points from the reference's MCG59 stream (keys unused)."""
+    stream = (w0, w1, pos): starting points from the reference's MCG59 stream (keys unused).
+    vel_yy: velocities from velocity_string_YY (the VEL = 1 build)."""
     N = phi.shape[0]
     ns = int(np.count_nonzero(flags & 1))
     rec = np.zeros((max(ns, 1), 11))
     sums = np.zeros(5)
     fl = _p(np.ascontiguousarray(flags, np.int8))
     if stream is None:
-        n = lib().orc_strings(_p(phi), _p(pi), N, fl, key0, key1, dspa, _p(rec), _p(sums))
+        n = lib().orc_strings(_p(phi), _p(pi), N, fl, key0, key1, dspa, int(vel_yy), _p(rec), _p(sums))
     else:
         n = lib().orc_strings_mcg(_p(phi), _p(pi), N, fl, mcg59_seed(stream[0], stream[1]), stream[2], dspa,
-                                  _p(rec), _p(sums))
+                                  int(vel_yy), _p(rec), _p(sums))
     return rec[:n], sums
 
 

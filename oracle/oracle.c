@@ -1260,31 +1260,140 @@ void orc_velocity_string(const double* phi, const double* pi, int64_t N, int64_t
     }
 }
 
+/* defects_3d.f90:824-982 velocity_string_YY (the VEL = 1 build) for the cell with corner (x,y,z)
+ * (0-based), one rank: for ipla = 1..3, (u,v,w) = cshift((1,2,3), ipla-1), and k = 0, 1, the
+ * quadrant test of the face's corners (i, j) in i-outer order; on a pierced face vphi = sum of
+ * (1, i-1/2, j-1/2) phi, d = invmat vphi with invmat = diag(1/4, 1, 1) (the packed inverse of
+ * diag(4, 1, 1), exact), det, du, dv (clamped to [-1/2, 1/2]), the weights (1/2 + sign(du, i-1/2))
+ * (1/2 + sign(dv, j-1/2)) applied to pi and the central differences of phi (one-sided in z from
+ * the halo plane above the top plane), a11, a12, a22, vpen = -(...)/det*dspa; then vel = the
+ * sum over the pierced faces of vpen(:, npen) -- the last face's, as the reference indexes it --
+ * divided by npen, or 1e-10 without a pierced face. */
+void orc_velocity_string_yy(const double* phi, const double* pi, int64_t N, int64_t x, int64_t y, int64_t z,
+                            double dspa, double* vel)
+{
+    const oc* f = (const oc*)phi;
+    const oc* q = (const oc*)pi;
+    const int64_t c0[3] = {x, y, z};
+#define AT(a, cx, cy, cz) (a)[(cx) + N * ((cy) + N * (cz))]
+    int npen = 0;
+    double last[3] = {0, 0, 0};
+    for (int pl = 0; pl < 3; pl++) {
+        const int au = pl, av = (pl + 1) % 3, aw = (pl + 2) % 3;
+        for (int k = 0; k < 2; k++) {
+            int64_t cc[2][2][3];
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    for (int a = 0; a < 3; a++)
+                        cc[i][j][a] = c0[a] + (a == au ? i : 0) + (a == av ? j : 0) + (a == aw ? k : 0);
+                    if (cc[i][j][0] >= N) cc[i][j][0] -= N;
+                    if (cc[i][j][1] >= N) cc[i][j][1] -= N;
+                }
+            int quad[5] = {0, 0, 0, 0, 0}, n = 0;
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    const int64_t *c = cc[i][j];
+                    const oc v = AT(f, c[0], c[1], c[2] % N);
+                    const double ph = atan2(v.im, v.re);
+                    const int m = (int)ceil(2 * ph / PI_T) + 2;
+                    if (!quad[m]) { quad[m] = 1; n++; }
+                }
+            if (n < 3) continue;
+            npen++;
+            oc vp[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    const int64_t *c = cc[i][j];
+                    const oc v = AT(f, c[0], c[1], c[2] % N);
+                    const double w[3] = {1.0, i - 0.5, j - 0.5};
+                    for (int m = 0; m < 3; m++) { vp[m].re = vp[m].re + w[m] * v.re; vp[m].im = vp[m].im + w[m] * v.im; }
+                }
+            const double d01 = 0.25 * vp[0].re, d02 = 0.25 * vp[0].im;
+            const double d11 = vp[1].re, d12 = vp[1].im, d21 = vp[2].re, d22 = vp[2].im;
+            const double det = d11 * d22 - d21 * d12;
+            double du = -((d22 * d01 - d21 * d02) / det);
+            double dv = -((-(d12 * d01) + d11 * d02) / det);
+            du = fmax(fmin(du, 0.5), -0.5);
+            dv = fmax(fmin(dv, 0.5), -0.5);
+            oc dp[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    const double wt = (0.5 + (i ? fabs(du) : -fabs(du))) * (0.5 + (j ? fabs(dv) : -fabs(dv)));
+                    const int64_t *c = cc[i][j];
+                    const int64_t cz = c[2] % N;
+                    const oc p = AT(q, c[0], c[1], cz);
+                    dp[0].re = dp[0].re + wt * p.re;
+                    dp[0].im = dp[0].im + wt * p.im;
+                    oc a = AT(f, (c[0] + 1) % N, c[1], cz), b = AT(f, (c[0] + N - 1) % N, c[1], cz);
+                    dp[1].re = dp[1].re + wt * (a.re - b.re) / 2;
+                    dp[1].im = dp[1].im + wt * (a.im - b.im) / 2;
+                    a = AT(f, c[0], (c[1] + 1) % N, cz);
+                    b = AT(f, c[0], (c[1] + N - 1) % N, cz);
+                    dp[2].re = dp[2].re + wt * (a.re - b.re) / 2;
+                    dp[2].im = dp[2].im + wt * (a.im - b.im) / 2;
+                    /* evolve/defects z indexing on one rank: l1 = iz+1 unless it passes the upper
+                     * halo plane N+1 (1-based), then l1 = iz; divisor l1 - l2 */
+                    int64_t zp, zm;
+                    double dz;
+                    if (c[2] == N) { zp = 0; zm = N - 1; dz = 1; }
+                    else { zp = (c[2] + 1) % N; zm = (c[2] + N - 1) % N; dz = 2; }
+                    a = AT(f, c[0], c[1], zp);
+                    b = AT(f, c[0], c[1], zm);
+                    dp[3].re = dp[3].re + wt * (a.re - b.re) / dz;
+                    dp[3].im = dp[3].im + wt * (a.im - b.im) / dz;
+                }
+            double a11 = 0, a12 = 0, a22 = 0;
+            for (int m = 1; m <= 3; m++) {
+                a11 = a11 + dp[m].re * dp[m].re;
+                a12 = a12 + dp[m].re * dp[m].im;
+                a22 = a22 + dp[m].im * dp[m].im;
+            }
+            const double dt = a11 * a22 - a12 * a12;
+            for (int m = 0; m < 3; m++) {
+                const double v = dp[0].re * (a22 * dp[m + 1].re - a12 * dp[m + 1].im) +
+                                 dp[0].im * (a11 * dp[m + 1].im - a12 * dp[m + 1].re);
+                last[m] = -(v / dt * dspa);
+            }
+        }
+    }
+#undef AT
+    if (npen != 0) {
+        for (int m = 0; m < 3; m++) {
+            double t = 0;
+            for (int i = 0; i < npen; i++) t = t + last[m];
+            vel[m] = t / npen;
+        }
+    } else {
+        vel[0] = vel[1] = vel[2] = 1.e-10;
+    }
+}
+
 /* id_defects' string loop (defects_3d.f90:167-205: cells with the string flag in site order,
  * 8 corner phases atan2(Im, Re), position_string, velocity_string, pos + ixyz - 1/2 with ixyz
  * the 1-based corner, i.e. site coordinates counted from 0) and add_string_length (1654-1733, group_strings = F).  rec (or NULL)
  * receives 11 doubles per string: pos(3), dir(3), gof, sl, vel(3).  sums = {sum sl, n with
  * |v| < 1, sum |v|, sum v^2, sum gamma} over those.  Returns the string count. */
 int64_t orc_strings_st(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
-                       uint32_t key1, const orc_stream* st, double dspa, double* rec, double* sums);
+                       uint32_t key1, const orc_stream* st, double dspa, int vel_yy, double* rec, double* sums);
 
+/* vel_yy != 0: the velocities of velocity_string_YY (the VEL = 1 build) instead */
 int64_t orc_strings(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
-                    uint32_t key1, double dspa, double* rec, double* sums)
+                    uint32_t key1, double dspa, int vel_yy, double* rec, double* sums)
 {
-    return orc_strings_st(phi, pi, N, flags, key0, key1, NULL, dspa, rec, sums);
+    return orc_strings_st(phi, pi, N, flags, key0, key1, NULL, dspa, vel_yy, rec, sums);
 }
 
 /* orc_strings with the reference's stream: x0, pos = the stream's start and the position of the
  * first string's first draw. */
 int64_t orc_strings_mcg(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint64_t x0,
-                        uint64_t pos, double dspa, double* rec, double* sums)
+                        uint64_t pos, double dspa, int vel_yy, double* rec, double* sums)
 {
     orc_stream st = {1, x0, pos};
-    return orc_strings_st(phi, pi, N, flags, 0, 0, &st, dspa, rec, sums);
+    return orc_strings_st(phi, pi, N, flags, 0, 0, &st, dspa, vel_yy, rec, sums);
 }
 
 int64_t orc_strings_st(const double* phi, const double* pi, int64_t N, const int8_t* flags, uint32_t key0,
-                       uint32_t key1, const orc_stream* st, double dspa, double* rec, double* sums)
+                       uint32_t key1, const orc_stream* st, double dspa, int vel_yy, double* rec, double* sums)
 {
     const oc* f = (const oc*)phi;
     int64_t n3 = N * N * N, ns = 0;
@@ -1305,7 +1414,8 @@ int64_t orc_strings_st(const double* phi, const double* pi, int64_t N, const int
         }
         double* o = r + 11 * s;
         position_string_st(ph, key0, key1, st, s, o, o + 3, o + 6, o + 7);
-        orc_velocity_string(phi, pi, N, x, y, z, o, dspa, o + 8);
+        if (vel_yy) orc_velocity_string_yy(phi, pi, N, x, y, z, dspa, o + 8);
+        else orc_velocity_string(phi, pi, N, x, y, z, o, dspa, o + 8);
         o[0] += (double)(x + 1) - 0.5;
         o[1] += (double)(y + 1) - 0.5;
         o[2] += (double)(z + 1) - 0.5;

@@ -30,10 +30,11 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout():
-    # the C struct: int64 + 10 int32 + 11 doubles + degrade_spec + 7 int32 reserved
+    # the C struct: int64 + 10 int32 + 11 doubles + degrade_spec + string_vel + 6 int32 reserved
     assert C.sizeof(capi.Params) == 8 + 10 * 4 + 11 * 8 + 8 * 4
     assert capi.Params.dspa.offset == 48
     assert capi.Params.degrade_spec.offset == 8 + 10 * 4 + 11 * 8
+    assert capi.Params.string_vel.offset == 8 + 10 * 4 + 11 * 8 + 4
 
 
 @pytest.mark.parametrize("sigma,g_rel,kappa,tau_fin,boxsize,ngrid", [

@@ -426,6 +426,37 @@ def test_strings_match_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
+def test_strings_vel_yy_matches_oracle(prec):
+    """string_vel = 1, the reference's VEL = 1 build: the velocities come from velocity_string_YY
+    (defects_3d.f90:824-982) -- quadrant tests, a plane fit per pierced face, weighted central
+    differences, the last pierced face's velocity -- against the oracle's restatement on the
+    values the GPU holds: velocities within 1e-12 relative (no transcendental but the quadrants'
+    atan2), the fits as in the VEL = 0 test, and the velocity sums from the same records."""
+    say("strings vel_yy", prec)
+    N = 32
+    prm, P = params(N, grav_spec=False, precision=prec, id_strings=True, id_walls=True, spec_est=True,
+                    string_vel=1)
+    phi, _ = ic(N, 31, 6.0)
+    rng = np.random.default_rng(4)
+    pi = 0.3 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        f_phi, f_pi = L.get_fields()
+        L.defects()
+        out = L.strings(1234, 99)
+        rec = L.string_records()
+    oflags, _ = O.classify(f_phi)
+    orec, osums = O.strings(f_phi, f_pi, oflags, 1234, 99, P["dspa"], vel_yy=True)
+    assert rec.shape == orec.shape and len(rec) > 50
+    vrel = np.abs(rec[:, 8:] - orec[:, 8:]) / np.maximum(1e-3, np.abs(orec[:, 8:]))
+    assert vrel.max() < 1e-12, vrel.max()
+    assert np.max(np.abs(rec[:, :8] - orec[:, :8]) / np.maximum(1, np.abs(orec[:, :8]))) < 1e-5
+    assert out["n_vel"] == osums[1]
+    ovel = osums[2:5] / osums[1]
+    assert np.all(np.abs(out["vel"] - ovel) <= 1e-10 * np.abs(ovel))
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
 def test_string_fit_split_bit_identical(prec, monkeypatch):
     """The string fits run in two kernels: a search still running after `split` simplex
     iterations is parked and continued by the second.  The records and the sums are the same bit

@@ -1255,3 +1255,69 @@ def test_symplectic_with_degrade_runs_the_reduced_grid():
         t += dt / 2 if it == n - 1 else dt
     assert np.array_equal(A, B)
     assert np.max(np.abs(A)) > 0
+
+
+# ------------------------------------------- velocity_string_YY (the reference's VEL = 1 build)
+
+def _moving_string(N, axis, vel, dspa, center=(7.3, 8.6), sign=1.0):
+    """A straight string along `axis` through `center` of the two other axes (u, v) = the next
+    two axes cyclically: phi = (U - u0) + i sign (V - v0) in cell units, and pi = -(vu + i sign vv)
+    / dspa, the time derivative of that field moving with velocity (vu, vv) in units of c."""
+    u_ax, v_ax = (axis + 1) % 3, (axis + 2) % 3
+    idx = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")  # z, y, x
+    coord = {0: idx[2], 1: idx[1], 2: idx[0]}
+    phi = (coord[u_ax] - center[0]) + 1j * sign * (coord[v_ax] - center[1])
+    pi = np.full((N, N, N), -(vel[0] + 1j * sign * vel[1]) / dspa)
+    return phi.astype(np.complex128), pi.astype(np.complex128), u_ax, v_ax
+
+
+@pytest.mark.parametrize("axis,sign", [(2, 1.0), (2, -1.0), (0, 1.0), (1, -1.0)])
+def test_velocity_yy_moving_straight_string(axis, sign):
+    """A straight string moving with (vu, vv) across its axis: the two faces normal to the axis
+    are pierced (the other four see two quadrants) and give the velocity exactly (the field is
+    linear, so the weights -- which sum to 1 -- do not matter)."""
+    N, dspa, v = 16, 0.3, (0.31, -0.17)
+    phi, pi, u_ax, v_ax = _moving_string(N, axis, v, dspa, sign=sign)
+    c = [0, 0, 0]
+    c[u_ax], c[v_ax], c[axis] = 7, 8, 5
+    vel = O.velocity_string_yy(phi, pi, c[0], c[1], c[2], dspa)
+    exp = np.zeros(3)
+    exp[u_ax], exp[v_ax] = v
+    assert np.all(np.abs(vel - exp) < 1e-12), (vel, exp)
+
+
+def test_velocity_yy_uses_the_last_pierced_face():
+    """The reference averages vpen(:, npen) -- the last pierced face's velocity, npen times --
+    not the faces' mean: with pi changing along the string between the two pierced faces the
+    result is the upper face's velocity."""
+    N, dspa = 16, 0.25
+    phi, pi, _, _ = _moving_string(N, 2, (0.2, 0.1), dspa)
+    z = np.arange(N)[:, None, None]
+    pi = pi - (0.05 * (z - 5)) / dspa  # vx grows by 0.05 per plane
+    vel = O.velocity_string_yy(phi, pi.astype(np.complex128), 7, 8, 5, dspa)
+    assert abs(vel[0] - 0.25) < 1e-12 and abs(vel[1] - 0.1) < 1e-12 and abs(vel[2]) < 1e-12
+    assert abs(vel[0] - 0.225) > 1e-3  # not the two faces' mean
+
+
+def test_velocity_yy_without_a_pierced_face():
+    N = 8
+    phi = np.full((N, N, N), 0.6 + 0.8j)
+    pi = np.zeros((N, N, N), np.complex128)
+    assert np.array_equal(O.velocity_string_yy(phi, pi, 3, 3, 3, 0.25), np.full(3, 1e-10))
+
+
+def test_strings_vel_yy_keeps_the_fits():
+    """vel_yy changes only the velocity columns of the records (and the velocity sums)."""
+    N = 16
+    P = phys(N)
+    phi, _ = O.ic_flat(N, 5, 6.0, P["dwav"])
+    rng = np.random.default_rng(2)
+    pi = 0.2 * (rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N)))
+    flags, cnt = O.classify(phi)
+    assert cnt[0] > 10
+    r0, s0 = O.strings(phi, pi, flags, 3, 4, P["dspa"])
+    r1, s1 = O.strings(phi, pi, flags, 3, 4, P["dspa"], vel_yy=True)
+    assert np.array_equal(r0[:, :8], r1[:, :8]) and s0[0] == s1[0]
+    zz, yy, xx = np.nonzero(flags & 1)
+    for k in range(0, len(xx), 7):
+        assert np.array_equal(r1[k, 8:], O.velocity_string_yy(phi, pi, xx[k], yy[k], zz[k], P["dspa"]))
