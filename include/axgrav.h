@@ -201,7 +201,8 @@ typedef struct axg_string_out {
 
 /* String length and velocity (defects_3d.f90:167-205 with position_string 435-603,
  * calc_chi2_string 384-433, string_length 1027-1056, velocity_string 605-690 of the VEL = 0
- * build, and add_string_length 1654-1733 with group_strings = F), one GPU thread per string
+ * build -- or velocity_string_YY 824-982 of the VEL = 1 build when params.string_vel = 1 -- and
+ * add_string_length 1654-1733 with group_strings = F), one GPU thread per string
  * cell of the last axg_defects.  Each fit draws its 1001 starting points from the reference's
  * stream when axg_set_stream was called (string s in site order at position + 4004 s, as the
  * reference positions it), otherwise from Philox4x32-10 keyed by (seed0, seed1) with the
