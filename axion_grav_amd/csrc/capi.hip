@@ -458,8 +458,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
     const axg_params& p = h->p;
     try {
         require(supported_n(p.ngrid), "ngrid must be even, 2^a 3^b, 8..1024");
-        require(p.precision == AXG_FP64 || p.precision == AXG_FP32C || p.precision == 2,
-                "precision must be AXG_FP64 or AXG_FP32C");
+        require(p.precision == AXG_FP64 || p.precision == AXG_FP32C || p.precision == AXG_FP32,
+                "precision must be AXG_FP64, AXG_FP32C or AXG_FP32");
         require(p.dspa > 0 && p.dwav > 0 && p.dv_spa > 0 && p.dv_wav > 0 && p.sigma_cr > 0, "derived units missing");
         require(p.num_bin_grav >= 1 && p.num_bin_grav <= 4096, "num_bin_grav out of range");
         require(p.string_vel == 0 || p.string_vel == 1, "string_vel must be 0 (VEL = 0 build) or 1 (VEL = 1)");

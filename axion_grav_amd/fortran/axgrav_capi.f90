@@ -8,7 +8,7 @@ module axgrav_capi
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: AXG_OK = 0, AXG_FP64 = 0, AXG_FP32C = 1
+  integer(c_int), parameter :: AXG_OK = 0, AXG_FP64 = 0, AXG_FP32C = 1, AXG_FP32 = 2
 
   ! struct axg_params
   type, bind(C) :: axg_params

@@ -31,14 +31,17 @@ enum {
     AXG_ENUMERIC = -5 /* numerical failure (e.g. deconvolution matrix not positive definite) */
 };
 
-enum { AXG_FP64 = 0, AXG_FP32C = 1 };
+/* Field storage: FP64; FP32C = FP32 value + int16 residual (about 39 significant bits) with FP64
+ * arithmetic; FP32 = plain FP32 storage, FP64 arithmetic (a reference point for FP32C's
+ * precision, not a drop-in mode).  GW transforms and accumulators are FP32 in both FP32 modes. */
+enum { AXG_FP64 = 0, AXG_FP32C = 1, AXG_FP32 = 2 };
 
 /* Parameters: the settings.f90 globals the hot path reads (settings.f90:17-66,
  * potential.f90:24) plus the keys this build adds.  Derive the physical ones with
  * axg_derive() from the ini keys, or fill them in from an existing settings module. */
 typedef struct axg_params {
     int64_t ngrid;          /* N (settings key ngrid); even, 2^a * 3^b, 8 <= N <= 1024 */
-    int32_t precision;      /* AXG_FP64 or AXG_FP32C (new key precision) */
+    int32_t precision;      /* AXG_FP64, AXG_FP32C or AXG_FP32 (new key precision) */
     int32_t grav_spec;      /* accumulate GW each kick (key grav_spec) */
     int32_t initial_loop;   /* thermal mass off (key initial_loop; evolve.f90:292-296) */
     int32_t gw_fix_swap;    /* 0: reproduce grav_waves.f90:239-242 exactly; 1: intended form */
