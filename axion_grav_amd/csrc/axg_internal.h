@@ -247,7 +247,7 @@ void launch_thermal_modes(hipStream_t s, int N, const Draws& dr, int ip, double 
 // npend (one int), pend (ns int64) and state (STRING_STATE doubles per string).
 constexpr int STRING_NT = 64;
 constexpr int STRING_STATE = 26;
-constexpr int STRING_SPLIT = 64;  // default split (fastest of 2..1000 at 512^3 on MI355X)
+constexpr int STRING_SPLIT = 32;  // default split (fastest of 0..128 at 512^3 on MI355X)
 template <typename T, bool COMP>
 void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, const void* pi, const void* cpi,
                        const int32_t* cells, int64_t ns, const Draws& dr, double dspa, int split, int vel_yy,

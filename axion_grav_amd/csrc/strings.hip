@@ -17,6 +17,12 @@
 // continues the parked strings, one per thread, so the long searches share waves.  A string's
 // search is the same sequence of operations whichever stage ends it, so the records do not
 // depend on `split`.  The per-block sums are taken afterwards from the records.
+//
+// Within a wave the simplex iterations run in lockstep, and a shrink (four chi2) on any lane
+// used to cost the whole wave four chi2 rounds; the searches that never converge shrink at almost
+// every iteration late in their run.  The four chi2 of every shrinking search are now shared out
+// over the wave's 64 lanes (up to 16 shrinks per chi2 round), so an iteration costs three rounds
+// (reflection, expansion or contraction, shrink) where it cost up to six.
 #include "axg_internal.h"
 
 namespace axg {
@@ -235,74 +241,145 @@ __device__ __forceinline__ void fit_starts(const Phases& ph, const Draws& dr, in
     }
 }
 
+// Centroid of the simplex without its worst point.
+__device__ __forceinline__ void centroid(const double (&p)[NSIMP + 1][4], double c[4])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = 0;
+#pragma unroll
+    for (int i0 = 0; i0 < NSIMP; i0++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = c[k] + p[i0][k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = c[k] / NSIMP;
+}
+
+// The reference's bubble sort of the simplex by chi2 (stable: equal chi2 keep their order).
+__device__ __forceinline__ void sort_simplex(double (&p)[NSIMP + 1][4], double (&c2)[NSIMP + 1])
+{
+#pragma unroll
+    for (int i0 = NSIMP - 1; i0 >= 0; i0--)
+#pragma unroll
+        for (int j = 0; j <= i0; j++)
+            if (c2[j] > c2[j + 1]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    double q = p[j][k];
+                    p[j][k] = p[j + 1][k];
+                    p[j + 1][k] = q;
+                }
+                double q = c2[j];
+                c2[j] = c2[j + 1];
+                c2[j + 1] = q;
+            }
+}
+
+// LDS of the shared shrink evaluation: SHR_SLOTS searches per chi2 round, each with its four
+// shrunk points and its corner phases, and the chi2 results.
+constexpr int SHR_SLOTS = 16;
+struct ShrinkLds {
+    double pt[SHR_SLOTS][4][4];
+    double ph[SHR_SLOTS][8];
+    double res[SHR_SLOTS][4];
+};
+
+// The four chi2 of every shrinking search of the wave, shared out over all 64 lanes: up to 16
+// searches (64 evaluations) per chi2 round instead of four rounds per search.  Called by every
+// lane of the wave (the block is one wave) with shr set on the lanes whose search shrinks.
+__device__ __forceinline__ void shrink_chi2(ShrinkLds& L, bool shr, const Phases& ph,
+                                            const double (&p)[NSIMP + 1][4], double (&c2)[NSIMP + 1])
+{
+    const uint64_t m = __ballot(shr);
+    if (m == 0) return;
+    const int k = __popcll(m);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int lane = (int)(threadIdx.x & 63);
+    for (int c0 = 0; c0 < k; c0 += SHR_SLOTS) {
+        const int sl = rank - c0;
+        const bool mine = shr && sl >= 0 && sl < SHR_SLOTS;
+        if (mine) {
+#pragma unroll
+            for (int a = 0; a < NSIMP; a++)
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) L.pt[sl][a][kk] = p[a + 1][kk];
+#pragma unroll
+            for (int c = 0; c < 8; c++) L.ph[sl][c] = ph.v[c];
+        }
+        __syncthreads();
+        const int ts = lane >> 2, tp = lane & 3;
+        if (c0 + ts < k) {
+            Phases q;
+#pragma unroll
+            for (int c = 0; c < 8; c++) q.v[c] = L.ph[ts][c];
+            L.res[ts][tp] = chi2_string_v(q, L.pt[ts][tp][0], L.pt[ts][tp][1], L.pt[ts][tp][2], L.pt[ts][tp][3]);
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+            for (int a = 0; a < NSIMP; a++) c2[a + 1] = L.res[sl][a];
+        }
+        __syncthreads();
+    }
+}
+
 // The reference's simplex iteration, continued from iteration `it` until chi2 <= 0.05 or
-// iteration itmax (the reference's limit is MAXIT).
+// iteration itmax (the reference's limit is MAXIT).  Every lane of the wave calls it and stays
+// in the loop until the wave's last search ends (`valid` false: no search on this lane), because
+// the four chi2 of a shrink are evaluated by the whole wave.  A search's arithmetic is the
+// reference's operation for operation (the shrunk points do not depend on each other's chi2),
+// so where its chi2 are evaluated does not change its bits.
 __device__ __forceinline__ void fit_simplex(const Phases& ph, double (&p)[NSIMP + 1][4], double (&c2)[NSIMP + 1],
-                                            int& it, int itmax)
+                                            int& it, int itmax, bool valid, ShrinkLds& L)
 {
 #pragma clang fp contract(off)
     const double chi2max = 8 * PI_T * PI_T;
     double t[4], c[4], sv[4];
-    while (c2[0] > STR_TOL && it < itmax) {
-        it = it + 1;
+    for (;;) {
+        const bool act = valid && c2[0] > STR_TOL && it < itmax;
+        if (!__any(act)) break;
+        bool shr = false;
+        if (act) {
+            it = it + 1;
+            centroid(p, c);
 #pragma unroll
-        for (int k = 0; k < 4; k++) c[k] = 0;
+            for (int k = 0; k < 4; k++) t[k] = 2 * c[k] - p[NSIMP][k];
+            double chi2 = out_of_box(t) ? chi2max : chi2_string(ph, t);
+            if (chi2 < c2[NSIMP]) {
 #pragma unroll
-        for (int i0 = 0; i0 < NSIMP; i0++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) c[k] = c[k] + p[i0][k];
-#pragma unroll
-        for (int k = 0; k < 4; k++) c[k] = c[k] / NSIMP;
-#pragma unroll
-        for (int k = 0; k < 4; k++) t[k] = 2 * c[k] - p[NSIMP][k];
-        double chi2 = out_of_box(t) ? chi2max : chi2_string(ph, t);
-        if (chi2 < c2[NSIMP]) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) sv[k] = 3 * c[k] - 4 * p[NSIMP][k];
-            double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
-            if (chi2 <= chi2s) { c2[NSIMP] = chi2; copy4(p[NSIMP], t); }
-            else { c2[NSIMP] = chi2s; copy4(p[NSIMP], sv); }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) sv[k] = 1.5 * c[k] - 0.5 * p[NSIMP][k];
-            double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
-            if (chi2s <= c2[NSIMP]) {
-                c2[NSIMP] = chi2s;
-                copy4(p[NSIMP], sv);
+                for (int k = 0; k < 4; k++) sv[k] = 3 * c[k] - 4 * p[NSIMP][k];
+                double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+                if (chi2 <= chi2s) { c2[NSIMP] = chi2; copy4(p[NSIMP], t); }
+                else { c2[NSIMP] = chi2s; copy4(p[NSIMP], sv); }
             } else {
 #pragma unroll
-                for (int i0 = 1; i0 <= NSIMP; i0++) {
+                for (int k = 0; k < 4; k++) sv[k] = 1.5 * c[k] - 0.5 * p[NSIMP][k];
+                double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+                if (chi2s <= c2[NSIMP]) {
+                    c2[NSIMP] = chi2s;
+                    copy4(p[NSIMP], sv);
+                } else {
+                    // shrink towards the best point
 #pragma unroll
-                    for (int k = 0; k < 4; k++) p[i0][k] = 0.5 * (p[i0][k] + p[0][k]);
-                    c2[i0] = chi2_string(ph, p[i0]);
+                    for (int i0 = 1; i0 <= NSIMP; i0++)
+#pragma unroll
+                        for (int k = 0; k < 4; k++) p[i0][k] = 0.5 * (p[i0][k] + p[0][k]);
+                    shr = true;
                 }
             }
         }
-#pragma unroll
-        for (int i0 = NSIMP - 1; i0 >= 0; i0--)
-#pragma unroll
-            for (int j = 0; j <= i0; j++)
-                if (c2[j] > c2[j + 1]) {
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        double q = p[j][k];
-                        p[j][k] = p[j + 1][k];
-                        p[j + 1][k] = q;
-                    }
-                    double q = c2[j];
-                    c2[j] = c2[j + 1];
-                    c2[j + 1] = q;
-                }
+        shrink_chi2(L, shr, ph, p, c2);
+        if (act) sort_simplex(p, c2);
     }
 }
 
-// Position, direction, goodness of fit and chord length of the best point.
-__device__ __forceinline__ void fit_result(const double (&p)[NSIMP + 1][4], const double (&c2)[NSIMP + 1],
-                                           double pos[3], double dir[3], double& gof, double& sl)
+// Position, direction, goodness of fit and chord length of the best point p0 (chi2 c20).
+__device__ __forceinline__ void fit_result(const double p0[4], double c20, double pos[3], double dir[3], double& gof,
+                                           double& sl)
 {
 #pragma clang fp contract(off)
-    double mu = p[0][0], phi = p[0][1];
-    double u0 = p[0][2] * cos(p[0][3]), v0 = p[0][2] * sin(p[0][3]);
+    double mu = p0[0], phi = p0[1];
+    double u0 = p0[2] * cos(p0[3]), v0 = p0[2] * sin(p0[3]);
     double sm = sqrt(1.0 - mu * mu);
     dir[0] = sm * cos(phi);
     dir[1] = sm * sin(phi);
@@ -310,7 +387,7 @@ __device__ __forceinline__ void fit_result(const double (&p)[NSIMP + 1][4], cons
     pos[0] = u0 * mu * cos(phi) - v0 * sin(phi);
     pos[1] = u0 * mu * sin(phi) + v0 * cos(phi);
     pos[2] = -u0 * sm;
-    gof = exp(-c2[0] / 2);
+    gof = exp(-c20 / 2);
     sl = string_length(pos, dir);
 }
 
@@ -455,14 +532,20 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
                                                                int64_t* __restrict__ pend, double* __restrict__ state)
 {
 #pragma clang fp contract(off)
+    __shared__ ShrinkLds lds;
     const int64_t s = blockIdx.x * (int64_t)STRING_NT + threadIdx.x;
-    if (s >= ns) return;
-    const StringCell sc = string_cell(N, cells, s);
-    const Phases ph = corner_phases<T, COMP>(phi, cphi, sc);
+    // lanes past the last string stay for the wave's shared shrink evaluations
+    const bool valid = s < ns;
+    const StringCell sc = string_cell(N, cells, valid ? s : 0);
+    Phases ph;
     double p[NSIMP + 1][4], c2[NSIMP + 1];
-    fit_starts(ph, dr, s, p, c2);
     int it = 0;
-    fit_simplex(ph, p, c2, it, split < MAXIT ? split : MAXIT);
+    if (valid) {
+        ph = corner_phases<T, COMP>(phi, cphi, sc);
+        fit_starts(ph, dr, s, p, c2);
+    }
+    fit_simplex(ph, p, c2, it, split < MAXIT ? split : MAXIT, valid, lds);
+    if (!valid) return;
     if (c2[0] > STR_TOL && it < MAXIT) {
         // still searching: park it for stage two
         const int j = atomicAdd(npend, 1);
@@ -478,7 +561,7 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
         return;
     }
     double pos[3], dir[3], gof, sl;
-    fit_result(p, c2, pos, dir, gof, sl);
+    fit_result(p[0], c2[0], pos, dir, gof, sl);
     finish_string<T, COMP>(phi, cphi, pi, cpi, sc, pos, dir, gof, sl, dspa, rec, s);
 }
 
@@ -492,23 +575,33 @@ __global__ void __launch_bounds__(STRING_NT2) string_fit_rest_kernel(
     const double* __restrict__ state)
 {
 #pragma clang fp contract(off)
+    __shared__ ShrinkLds lds;
+    const int np = *npend;
+    if (blockIdx.x * STRING_NT2 >= (unsigned)np) return;  // the whole wave has no search
     const int j = blockIdx.x * STRING_NT2 + threadIdx.x;
-    if (j >= *npend) return;
-    const int64_t s = pend[j];
-    const double* st = state + (int64_t)j * STRING_STATE;
+    const bool valid = j < np;
     double p[NSIMP + 1][4], c2[NSIMP + 1];
+    int it = 0;
+    Phases ph;
+    StringCell sc{N, 0, 0, 0};
+    int64_t s = 0;
+    if (valid) {
+        s = pend[j];
+        const double* st = state + (int64_t)j * STRING_STATE;
 #pragma unroll
-    for (int a = 0; a <= NSIMP; a++) {
+        for (int a = 0; a <= NSIMP; a++) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) p[a][k] = st[4 * a + k];
-        c2[a] = st[20 + a];
+            for (int k = 0; k < 4; k++) p[a][k] = st[4 * a + k];
+            c2[a] = st[20 + a];
+        }
+        it = (int)st[25];
+        sc = string_cell(N, cells, s);
+        ph = corner_phases<T, COMP>(phi, cphi, sc);
     }
-    int it = (int)st[25];
-    const StringCell sc = string_cell(N, cells, s);
-    const Phases ph = corner_phases<T, COMP>(phi, cphi, sc);
-    fit_simplex(ph, p, c2, it, MAXIT);
+    fit_simplex(ph, p, c2, it, MAXIT, valid, lds);
+    if (!valid) return;
     double pos[3], dir[3], gof, sl;
-    fit_result(p, c2, pos, dir, gof, sl);
+    fit_result(p[0], c2[0], pos, dir, gof, sl);
     finish_string<T, COMP>(phi, cphi, pi, cpi, sc, pos, dir, gof, sl, dspa, rec, s);
 }
 
@@ -700,7 +793,7 @@ void launch_string_fit(hipStream_t s, int N, const void* phi, const void* cphi, 
                        (const short2*)cphi, (const T2*)pi, (const short2*)cpi, cells, ns, dr, dspa, split, rec,
                        npend, pend, state);
     AXG_HIP(hipGetLastError());
-    // as many threads as strings; those past the parked count return at once
+    // as many threads as strings; waves past the parked count return at once
     hipLaunchKernelGGL((string_fit_rest_kernel<T, COMP>), dim3((unsigned)((ns + STRING_NT2 - 1) / STRING_NT2)),
                        dim3(STRING_NT2), 0, s, N, (const T2*)phi, (const short2*)cphi, (const T2*)pi,
                        (const short2*)cpi, cells, dspa, rec, (const int*)npend, (const int64_t*)pend,
