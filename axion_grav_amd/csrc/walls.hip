@@ -27,8 +27,22 @@ __device__ __forceinline__ void edge_point(int e, double t, double& px, double& 
     else { px = b - 0.5; py = t; pz = a - 0.5; }               // y-edge: slot 8+iz*2+ix
 }
 
+// The search grid of position_wall: 20 x 20 directions (mu, phi) with mu = (2 imu - 21) / 20 and
+// phi = pi_t (2 iphi - 21) / 20, the same for every cell.  A block evaluates sqrt(1 - mu^2) and
+// cos / sin of phi once into LDS (the same library calls as the per-cell loop made, so the same
+// bits); the loop then costs 12 dot products per direction instead of three transcendentals.
+constexpr int WALL_GRID = 20;
+struct WallGrid {
+    double smu[WALL_GRID];         // sqrt(1 - mu^2) of mu index imu - 1
+    double cs[WALL_GRID][2];       // cos, sin of phi index iphi - 1
+};
+
+__device__ __forceinline__ double grid_mu(int imu) { return (-20 + 2 * imu - 1.0) / 20; }
+__device__ __forceinline__ double grid_phi(int iphi) { return PI_T * (2 * iphi - 1.0 - 20) / 20; }
+
 // position_wall; returns the number m of edge points (m < 3: no fit, outputs untouched).
-__device__ int position_wall(const double ph[8], double pos[3], double dir[3], double& gof, double& wa)
+__device__ int position_wall(const double ph[8], const WallGrid& G, double pos[3], double dir[3], double& gof,
+                             double& wa)
 {
 #pragma clang fp contract(off)
     double sn[8];
@@ -52,12 +66,12 @@ __device__ int position_wall(const double ph[8], double pos[3], double dir[3], d
     for (int e = 0; e < 12; e++) m += ok[e] ? 1 : 0;
     if (m < 3) return m;
 
-    double chi2min = 6.0, mut = 0, phit = 0, wt = 0;
-    for (int imu = 1; imu <= 20; imu++) {
-        const double mu = (-20 + 2 * imu - 1.0) / 20;
-        for (int iphi = 1; iphi <= 20; iphi++) {
-            const double phi = PI_T * (2 * iphi - 1.0 - 20) / 20;
-            const double d0 = sqrt(1 - mu * mu) * cos(phi), d1 = sqrt(1 - mu * mu) * sin(phi), d2 = mu;
+    double chi2min = 6.0, mut = 0, wt = 0;
+    int bphi = 0;  // phi index of the best direction (0: none below chi2 = 6)
+    for (int imu = 1; imu <= WALL_GRID; imu++) {
+        const double mu = grid_mu(imu), smu = G.smu[imu - 1];
+        for (int iphi = 1; iphi <= WALL_GRID; iphi++) {
+            const double d0 = smu * G.cs[iphi - 1][0], d1 = smu * G.cs[iphi - 1][1], d2 = mu;
             double w = 0, chi2 = 0;
 #pragma unroll
             for (int e = 0; e < 12; e++) {
@@ -74,11 +88,12 @@ __device__ int position_wall(const double ph[8], double pos[3], double dir[3], d
             if (chi2 < chi2min) {
                 chi2min = chi2;
                 mut = mu;
-                phit = phi;
+                bphi = iphi;
                 wt = w;
             }
         }
     }
+    const double phit = bphi > 0 ? grid_phi(bphi) : 0.0;
     double dr[3] = {sqrt(1 - mut * mut) * cos(phit), sqrt(1 - mut * mut) * sin(phit), mut};
     int num = 0;
 #pragma unroll
@@ -161,6 +176,15 @@ __global__ void __launch_bounds__(WALL_NT) wall_fit_kernel(int N, const typename
 {
 #pragma clang fp contract(off)
     __shared__ double sh[2][WALL_NT];
+    __shared__ WallGrid grid;
+    static_assert(WALL_NT >= WALL_GRID, "one thread per grid value");
+    if (threadIdx.x < WALL_GRID) {
+        const double mu = grid_mu(threadIdx.x + 1), phi = grid_phi(threadIdx.x + 1);
+        grid.smu[threadIdx.x] = sqrt(1 - mu * mu);
+        grid.cs[threadIdx.x][0] = cos(phi);
+        grid.cs[threadIdx.x][1] = sin(phi);
+    }
+    __syncthreads();
     const int64_t s = blockIdx.x * (int64_t)WALL_NT + threadIdx.x;
     double acc[2] = {0, 0};
     if (s < nw) {
@@ -179,7 +203,7 @@ __global__ void __launch_bounds__(WALL_NT) wall_fit_kernel(int N, const typename
         }
         double pos[3] = {0, 0, 0}, dir[3] = {0, 0, 0}, gof = 0, wa = 0;
         double* o = rec + 8 * s;
-        if (position_wall(ph, pos, dir, gof, wa) < 3) {
+        if (position_wall(ph, grid, pos, dir, gof, wa) < 3) {
             acc[1] = 1;
 #pragma unroll
             for (int k = 0; k < 8; k++) o[k] = 0;

@@ -425,6 +425,38 @@ def test_strings_match_oracle(prec):
     assert np.all(np.abs(out["vel"] - ovel) <= 1e-8 * np.abs(ovel))
 
 
+def test_strings_match_oracle_128():
+    """The string fits at 128^3 on an evolved field (the oracle's own FP64 evolution, 12 DKD
+    steps of dt = 0.05 from a flat-spectrum start: 11,622 string cells, 47 % of them still
+    searching after 64 simplex iterations, so most long searches run in the second kernel), FP64
+    storage so that the GPU holds the oracle's values: classification bit-exact, then the same
+    bars as at 32^3 -- every
+    record within 1e-5 (a search that took another path differs by O(0.1)), the summed length
+    within 1e-9, the mean velocities within 1e-8, n_vel equal."""
+    N = 128
+    say("strings 128", N)
+    prm, P = params(N, grav_spec=False, precision=capi.FP64, id_strings=True, id_walls=True, spec_est=True)
+    phi, pi = O.ic_flat(N, 3, 1.0, P["dwav"])
+    O.symplectic(phi, pi, None, 1.0, 0.05, 12, P)
+    with capi.Lattice(prm) as L:
+        L.set_fields(phi, pi)
+        cnt = L.defects()
+        out = L.strings(1234, 99)
+        rec = L.string_records()
+        flags = L.flags()
+    oflags, ocnt = O.classify(phi)
+    assert np.array_equal(flags, oflags) and cnt[0] == ocnt[0] == out["n_string"] > 10000
+    orec, osums = O.strings(phi, pi, oflags, 1234, 99, P["dspa"])
+    rel = np.abs(rec - orec) / np.maximum(1, np.abs(orec))
+    print(f"  strings {len(rec)}: max rel diff {rel.max():.2e}, {np.count_nonzero(rel.max(1) > 1e-9)} records "
+          f"above 1e-9; length {out['length']:.10f} vs {osums[0]:.10f}")
+    assert rel.max() < 1e-5
+    assert abs(out["length"] - osums[0]) <= 1e-9 * osums[0]
+    assert out["n_vel"] == osums[1]
+    ovel = osums[2:5] / osums[1]
+    assert np.all(np.abs(out["vel"] - ovel) <= 1e-8 * np.abs(ovel))
+
+
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32C])
 def test_strings_vel_yy_matches_oracle(prec):
     """string_vel = 1, the reference's VEL = 1 build: the velocities come from velocity_string_YY
