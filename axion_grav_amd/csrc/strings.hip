@@ -345,16 +345,16 @@ __device__ __forceinline__ void fit_simplex(const Phases& ph, double (&p)[NSIMP 
 #pragma unroll
             for (int k = 0; k < 4; k++) t[k] = 2 * c[k] - p[NSIMP][k];
             double chi2 = out_of_box(t) ? chi2max : chi2_string(ph, t);
-            if (chi2 < c2[NSIMP]) {
+            // expansion or contraction point, evaluated at one call site (one chi2 round for
+            // the wave whichever its lanes take)
+            const bool expand = chi2 < c2[NSIMP];
 #pragma unroll
-                for (int k = 0; k < 4; k++) sv[k] = 3 * c[k] - 4 * p[NSIMP][k];
-                double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+            for (int k = 0; k < 4; k++) sv[k] = expand ? 3 * c[k] - 4 * p[NSIMP][k] : 1.5 * c[k] - 0.5 * p[NSIMP][k];
+            const double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
+            if (expand) {
                 if (chi2 <= chi2s) { c2[NSIMP] = chi2; copy4(p[NSIMP], t); }
                 else { c2[NSIMP] = chi2s; copy4(p[NSIMP], sv); }
             } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) sv[k] = 1.5 * c[k] - 0.5 * p[NSIMP][k];
-                double chi2s = out_of_box(sv) ? chi2max : chi2_string(ph, sv);
                 if (chi2s <= c2[NSIMP]) {
                     c2[NSIMP] = chi2s;
                     copy4(p[NSIMP], sv);
@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
 // Stage two: the parked searches, one per thread, continued to the end.
 constexpr int STRING_NT2 = 64;
 template <typename T, bool COMP>
-__global__ void __launch_bounds__(STRING_NT2) string_fit_rest_kernel(
+__global__ void __launch_bounds__(STRING_NT2, 3) string_fit_rest_kernel(
     int N, const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi,
     const typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cpi, const int32_t* __restrict__ cells,
     double dspa, double* __restrict__ rec, const int* __restrict__ npend, const int64_t* __restrict__ pend,
