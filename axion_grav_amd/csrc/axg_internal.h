@@ -59,14 +59,18 @@ __device__ __forceinline__ const StepEntry* step_entry(const StepRef& r)
 // One lattice of a stacked launch: lattices of one size and precision advanced by one launch per
 // pass, the lattice index in blockIdx.z.  phi[0] / cphi[0] hold the fields at step 0 of the
 // table, so step s reads phi[s & 1] and writes phi[(s & 1) ^ 1].
+// The buffers are global-memory pointers by type, so that the stacked kernels, which read them
+// from this table, address HBM with global instructions rather than flat ones (a flat access also
+// counts against the LDS wait counter, so each barrier of a transform would wait for it).
+using gptr = __attribute__((address_space(1))) void*;
 struct LatRef {
-    void* phi[2];
-    void* cphi[2];
-    void* pi;
-    void* cpi;
-    void* W;
-    void* W2;
-    void* A;
+    gptr phi[2];
+    gptr cphi[2];
+    gptr pi;
+    gptr cpi;
+    gptr W;
+    gptr W2;
+    gptr A;
     const StepEntry* tab;
 };
 void launch_add_int(hipStream_t s, int* counter, int delta);

@@ -1892,15 +1892,15 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
             for (int i : u.idx) {
                 const axg_handle* h = hs[i];
                 LatRef r{};
-                r.phi[0] = h->phi[h->cur];
-                r.phi[1] = h->phi[1 - h->cur];
-                r.cphi[0] = h->cphi[h->cur];
-                r.cphi[1] = h->cphi[1 - h->cur];
-                r.pi = h->pi;
-                r.cpi = h->cpi;
-                r.W = h->W;
-                r.W2 = h->W2;
-                r.A = h->A;
+                r.phi[0] = (gptr)h->phi[h->cur];
+                r.phi[1] = (gptr)h->phi[1 - h->cur];
+                r.cphi[0] = (gptr)h->cphi[h->cur];
+                r.cphi[1] = (gptr)h->cphi[1 - h->cur];
+                r.pi = (gptr)h->pi;
+                r.cpi = (gptr)h->cpi;
+                r.W = (gptr)h->W;
+                r.W2 = (gptr)h->W2;
+                r.A = (gptr)h->A;
                 r.tab = h->steptab;
                 refs.push_back(r);
             }

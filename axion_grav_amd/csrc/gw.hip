@@ -113,14 +113,14 @@ __global__ void __launch_bounds__((p2_tx<T, N>() * (N / 8))) fft_cols_p2_kernel(
 // planes are transformed one after the other through the same LDS tile, and their outputs for
 // one (ky, kx) are adjacent tile rows, stored back to back (KZ = 2 turns the 64-byte rows of
 // 8-column FP32 or 4-column FP64 tiles into whole 128-byte lines).
-template <typename T, int N, int TX, int KZ>
+template <typename T, int N, int TX, int KZ, bool STK>
 __global__ void __launch_bounds__((TX * (N / 8))) gw_ypass_p2_kernel(const typename V2<T>::type* __restrict__ W,
                                                                     typename V2<T>::type* __restrict__ W2,
                                                                     const typename V2<T>::type* __restrict__ tw,
                                                                     const LatRef* __restrict__ lat)
 {
     using T2 = typename V2<T>::type;
-    if (lat) {
+    if constexpr (STK) {
         // stacked launch: this block's lattice in blockIdx.z
         W = (const T2*)lat[blockIdx.z].W;
         W2 = (T2*)lat[blockIdx.z].W2;
@@ -173,14 +173,14 @@ template <typename T, int N> constexpr int p2_tx2()
 // whole 128-byte lines (16-byte FP32 / 32-byte FP64 accesses per thread), and each thread writes
 // its column pair as one access into the tiled W2 (pairs never straddle a tile: kx is even).
 // The two columns' transforms share the stages and barriers (fft_p2_2).
-template <typename T, int N, int TX, int KZ>
+template <typename T, int N, int TX, int KZ, bool STK>
 __global__ void __launch_bounds__((TX / 2) * (N / 8)) gw_ypass_p2w_kernel(const typename V2<T>::type* __restrict__ W,
                                                                          typename V2<T>::type* __restrict__ W2,
                                                                          const typename V2<T>::type* __restrict__ tw,
                                                                          const LatRef* __restrict__ lat)
 {
     using T2 = typename V2<T>::type;
-    if (lat) {
+    if constexpr (STK) {
         W = (const T2*)lat[blockIdx.z].W;
         W2 = (T2*)lat[blockIdx.z].W2;
     }
@@ -748,18 +748,28 @@ void launch_gw_ypass(hipStream_t s, int N, int pitch, void* W, void* W2, const G
             // z planes per block.  Where the one-column tile is narrower than a 128-byte line
             // (p2_tx2), variants 0 and 2 run the two-column kernel instead and variant 1 the
             // one-column tile.
+            // buffers as arguments or from the stack's LatRef table (global or flat addressing)
             auto go = [&]<int TXP, int KZ>() {
-                hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP, KZ>),
-                                   dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ, nlat), dim3(TXP * (NN / 8)),
-                                   (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
+                auto g1 = [&]<bool STK>() {
+                    hipLaunchKernelGGL((gw_ypass_p2_kernel<T, NN, TXP, KZ, STK>),
+                                       dim3((NN / 2 + 1 + TXP - 1) / TXP, 6 * NN / KZ, nlat), dim3(TXP * (NN / 8)),
+                                       (size_t)NN * TXP * sizeof(T2), s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
+                };
+                if (lat) g1.template operator()<true>();
+                else g1.template operator()<false>();
             };
             constexpr int TXP = p2_tx<T, NN>(), TXH = TXP > 1 ? TXP / 2 : 1, TX2 = p2_tx2<T, NN>();
             if constexpr (TX2 > 0) {
                 auto go2 = [&]<int KZ>() {
                     using L = TiledLayout<NN, p2_ztx<T, NN>()>;
                     constexpr int cols = L::NT * L::ZTW > NN / 2 + 1 ? L::NT * L::ZTW : NN / 2 + 1;
-                    hipLaunchKernelGGL((gw_ypass_p2w_kernel<T, NN, TX2, KZ>), dim3((cols + TX2 - 1) / TX2, 6 * NN / KZ, nlat),
-                                       dim3((TX2 / 2) * (NN / 8)), 0, s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
+                    auto g2 = [&]<bool STK>() {
+                        hipLaunchKernelGGL((gw_ypass_p2w_kernel<T, NN, TX2, KZ, STK>),
+                                           dim3((cols + TX2 - 1) / TX2, 6 * NN / KZ, nlat), dim3((TX2 / 2) * (NN / 8)), 0,
+                                           s, (const T2*)W, (T2*)W2, (const T2*)tw, lat);
+                    };
+                    if (lat) g2.template operator()<true>();
+                    else g2.template operator()<false>();
                 };
                 if (variant == 1) go.template operator()<TXP, 1>();
                 else if (variant == 2) go2.template operator()<2>();

@@ -244,7 +244,7 @@ __device__ __forceinline__ void strip_order(int lin, int& yg, int& z)
 // neighbouring rows of one plane and share their y halo rows in its L2.  REMAP 2 (strips,
 // strip_order): XCD k takes a strip of NG/8 row groups and walks it plane by plane, so its
 // blocks share both the y and the z halo rows.
-template <typename T, bool COMP, int N, int XR, int REMAP>
+template <typename T, bool COMP, int N, int XR, int REMAP, bool STK>
 __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(XR * N) kick_source_xfft_kernel(
 {
 #pragma clang fp contract(off)
     using T2 = typename V2<T>::type;
-    if (lat) {
+    if constexpr (STK) {
         // stacked launch: this block's lattice, its buffers for this step and its scalars
         const LatRef& L = lat[blockIdx.z];
         const int st = step_index(sr) & 1;
@@ -441,7 +441,7 @@ __device__ __forceinline__ void st_pair(typename V2<T>::type* __restrict__ f, sh
     }
 }
 
-template <typename T, bool COMP, int N, int XR, bool STRIP>
+template <typename T, bool COMP, int N, int XR, bool STRIP, bool STK>
 __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
     const typename V2<T>::type* __restrict__ phi_in, typename V2<T>::type* __restrict__ phi_out,
     typename V2<T>::type* __restrict__ pi, const short2* __restrict__ cphi_in, short2* __restrict__ cphi_out,
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(XR * N / 2) kick_source_xfft2_kernel(
 {
 #pragma clang fp contract(off)
     using T2 = typename V2<T>::type;
-    if (lat) {
+    if constexpr (STK) {
         // stacked launch: this block's lattice, its buffers for this step and its scalars
         const LatRef& L = lat[blockIdx.z];
         const int st = step_index(sr) & 1;
@@ -567,11 +567,18 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
     // configuration (same results; the block shape and order only): bit 0 = wide blocks (twice
     // the rows: 1024 threads), bit 1 = XCD remap by planes (one site per thread only), bit 2 =
     // swap the sites-per-thread choice, bit 3 = XCD strips
+    // a kernel with the buffers as arguments (lat == nullptr) or from the stack's LatRef table: two
+    // instantiations, so that the single-lattice form addresses HBM with global instructions
+    // (reached through the table, the buffers take flat ones)
     auto launch = [&]<int NN, int XR, int RM>() {
-        hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM>), dim3(NN / XR, NN, nlat), dim3(XR * NN),
-                           (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out, (T2*)pi,
-                           (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch, (const T2*)tw, k,
-                           1.0 / d2, d2, div, dt_drift, sr, lat);
+        auto go1 = [&]<bool STK>() {
+            hipLaunchKernelGGL((kick_source_xfft_kernel<T, COMP, NN, XR, RM, STK>), dim3(NN / XR, NN, nlat),
+                               dim3(XR * NN), (size_t)3 * XR * NN * sizeof(T2), s, (const T2*)phi_in, (T2*)phi_out,
+                               (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi, (T2*)W, pitch,
+                               (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, sr, lat);
+        };
+        if (lat) go1.template operator()<true>();
+        else go1.template operator()<false>();
     };
     auto go = [&]<int NN>() {
         constexpr int XR = fused_rows<NN>(), XW = fused_rows_wide<NN>();
@@ -581,10 +588,15 @@ bool launch_kick_source_xfft(hipStream_t s, int N, const void* phi_in, void* phi
             const bool two = (sizeof(T) == 4) != ((variant & 4) != 0);
             if (two) {
                 auto launch2 = [&]<int X2, bool ST>() {
-                    hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST>), dim3(NN / X2, NN, nlat),
-                                       dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
-                                       (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out, (short2*)cpi,
-                                       (T2*)W, (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, sr, lat);
+                    auto go2 = [&]<bool STK>() {
+                        hipLaunchKernelGGL((kick_source_xfft2_kernel<T, COMP, NN, X2, ST, STK>), dim3(NN / X2, NN, nlat),
+                                           dim3(X2 * NN / 2), (size_t)3 * X2 * NN * sizeof(T2), s, (const T2*)phi_in,
+                                           (T2*)phi_out, (T2*)pi, (const short2*)cphi_in, (short2*)cphi_out,
+                                           (short2*)cpi, (T2*)W, (const T2*)tw, k, 1.0 / d2, d2, div, dt_drift, sr,
+                                           lat);
+                    };
+                    if (lat) go2.template operator()<true>();
+                    else go2.template operator()<false>();
                 };
                 // wide: twice the rows when they divide N and the tile stays within 64 KiB of LDS
                 constexpr int X2W =

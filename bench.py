@@ -66,15 +66,17 @@ def _fused_name(prec, v):
     two = (prec == "fp32c") != bool(v & 4)
     xr = 2 if v & 1 else 1
     if two:
-        return f"kick_source_xfft2_kernel<{T}, {C}, 512, {xr}, {'true' if v & 8 else 'false'}>"
-    return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {2 if v & 8 else 1 if v & 2 else 0}>"
+        return f"kick_source_xfft2_kernel<{T}, {C}, 512, {xr}, {'true' if v & 8 else 'false'}, false>"
+    return f"kick_source_xfft_kernel<{T}, {C}, 512, {xr}, {2 if v & 8 else 1 if v & 2 else 0}, false>"
 
 
-YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16, 1>", ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8, 1>",
-                ("fp32c", 2): "gw_ypass_p2_kernel<float, 512, 16, 2>",
-                ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8, 1>", ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4, 1>",
-                ("fp64", 2): "gw_ypass_p2_kernel<double, 512, 8, 2>"}
-ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512>", "fp64": "gw_zpass_p2_kernel<double, 512>"}
+YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16, 1, false>",
+                ("fp32c", 1): "gw_ypass_p2_kernel<float, 512, 8, 1, false>",
+                ("fp32c", 2): "gw_ypass_p2_kernel<float, 512, 16, 2, false>",
+                ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8, 1, false>",
+                ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4, 1, false>",
+                ("fp64", 2): "gw_ypass_p2_kernel<double, 512, 8, 2, false>"}
+ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512, false>", "fp64": "gw_zpass_p2_kernel<double, 512, false>"}
 
 
 def measured_traffic(slot, prec, N, choice=0):
