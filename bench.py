@@ -76,7 +76,7 @@ YPASS_KERNEL = {("fp32c", 0): "gw_ypass_p2_kernel<float, 512, 16, 1, false>",
                 ("fp64", 0): "gw_ypass_p2_kernel<double, 512, 8, 1, false>",
                 ("fp64", 1): "gw_ypass_p2_kernel<double, 512, 4, 1, false>",
                 ("fp64", 2): "gw_ypass_p2_kernel<double, 512, 8, 2, false>"}
-ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512, false>", "fp64": "gw_zpass_p2_kernel<double, 512, false>"}
+ZPASS_KERNEL = {"fp32c": "gw_zpass_p2_kernel<float, 512>", "fp64": "gw_zpass_p2_kernel<double, 512>"}
 
 
 def measured_traffic(slot, prec, N, choice=0):
