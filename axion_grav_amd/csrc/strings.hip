@@ -276,8 +276,11 @@ __device__ __forceinline__ void sort_simplex(double (&p)[NSIMP + 1][4], double (
 }
 
 // LDS of the shared shrink evaluation: SHR_SLOTS searches per chi2 round, each with its four
-// shrunk points and its corner phases, and the chi2 results.
+// shrunk points and its corner phases, and the chi2 results.  The search loop's trip count is
+// decided per wave (__any), and its rounds synchronise the block, so a fit block must be exactly
+// one wave.
 constexpr int SHR_SLOTS = 16;
+static_assert(STRING_NT == 64, "string fit blocks are one wave");
 struct ShrinkLds {
     double pt[SHR_SLOTS][4][4];
     double ph[SHR_SLOTS][8];
@@ -567,6 +570,7 @@ __global__ void __launch_bounds__(STRING_NT) string_fit_kernel(int N, const type
 
 // Stage two: the parked searches, one per thread, continued to the end.
 constexpr int STRING_NT2 = 64;
+static_assert(STRING_NT2 == 64, "string fit blocks are one wave (see ShrinkLds)");
 template <typename T, bool COMP>
 __global__ void __launch_bounds__(STRING_NT2, 3) string_fit_rest_kernel(
     int N, const typename V2<T>::type* __restrict__ phi, const short2* __restrict__ cphi,
