@@ -302,6 +302,45 @@ __device__ __forceinline__ void fft_p2(typename V2<T>::type (&v)[8], int j0, typ
     }
 }
 
+// A workgroup barrier for LDS traffic only: this wave's LDS operations complete, then s_barrier.
+// __syncthreads() also waits (vmcnt 0) for every direct global-to-LDS copy in flight, since such a
+// copy writes LDS; a transform that runs while its block copies its next input into another LDS
+// tile synchronises with this barrier instead (fft_p2_rb), which leaves the copies in flight.
+__device__ __forceinline__ void lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <typename T, int N, int S>
+__device__ __forceinline__ void p2_exchange_rb(typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s, int es, int off)
+{
+    constexpr int R = P2<N>::radix(S), Ns = P2<N>::span(S), TPT = P2<N>::TPT, M = 8 / R;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int j = j0 + m * TPT;
+        const int k = j & (Ns - 1);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) s[(base + r * Ns) * es + off] = v[m + r * M];
+    }
+    lds_sync();
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = s[(j0 + e * TPT) * es + off];
+    lds_sync();
+}
+
+// fft_p2 with lds_sync barriers (2^k sizes).
+template <typename T, int N, int SIGN, int S = 0, typename TW>
+__device__ __forceinline__ void fft_p2_rb(typename V2<T>::type (&v)[8], int j0, typename V2<T>::type* s, int es, int off,
+                                          const TW& tws)
+{
+    if constexpr (S < P2<N>::NST) {
+        p2_stage<T, N, S, SIGN>(v, tws);
+        if constexpr (S + 1 < P2<N>::NST) p2_exchange_rb<T, N, S>(v, j0, s, es, off);
+        fft_p2_rb<T, N, SIGN, S + 1>(v, j0, s, es, off, tws);
+    }
+}
+
 // Two transforms per thread, the LDS tile's columns off0 and off1, through the same stages: the
 // exchanges of both share one pair of barriers.
 template <typename T, int N, int S>
