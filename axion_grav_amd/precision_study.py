@@ -3,13 +3,15 @@
 
     python -m axion_grav_amd.precision_study [--n 512] [--box 128] [--steps 2000]
                                              [--every 250] [--dt 0.01] [--seed 7] [--bins 128]
+                                             [--storage fp32c|fp32]
 
 Both precisions start from the same seeded flat-spectrum field (|k| < 1, <|phi|^2> = 1,
 pi = 0, lambda = 2 at zero temperature, radiation era from tau = 1) and advance in lockstep
 with GW accumulation on.  Every `--every` steps each lattice reports its 128-bin GW spectrum,
 its energy (FP64 reductions in both precisions), its string / wall / mask cell counts, its
 string length and velocities and its wall area, and
-one JSON line gives the FP32C error against FP64: the per-bin relative error of the GW power
+one JSON line gives the FP32C error against FP64 (`--storage fp32`: plain FP32 storage in
+place of FP32C, the keys keep their names): the per-bin relative error of the GW power
 (max and median over the bins that hold modes), the relative error of the energy terms, the
 cell counts, and the relative error of the summed string length and mean string velocities
 (fits seeded alike in both precisions) and of the summed wall area.
@@ -73,11 +75,12 @@ def main():
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--bins", type=int, default=128)
+    ap.add_argument("--storage", choices=("fp32c", "fp32"), default="fp32c")
     a = ap.parse_args()
     if a.steps % a.every:
         sys.exit("--steps must be a multiple of --every")
     L64 = lattice(a.n, a.box, capi.FP64, a.bins)
-    L32 = lattice(a.n, a.box, capi.FP32C, a.bins)
+    L32 = lattice(a.n, a.box, capi.FP32C if a.storage == "fp32c" else capi.FP32, a.bins)
     L64.init_flat(a.seed, 1.0)
     phi, pi = L64.get_fields()
     L32.set_fields(phi, pi)
@@ -95,7 +98,7 @@ def main():
             worst = max(worst, r["gw_relerr_max"])
         print(json.dumps(r), flush=True)
     print(json.dumps(dict(summary=True, n=a.n, box=a.box, steps=a.steps, dt=a.dt, seed=a.seed, bins=a.bins,
-                          gw_relerr_max_over_run=worst)), flush=True)
+                          storage=a.storage, gw_relerr_max_over_run=worst)), flush=True)
     L32.close()
     L64.close()
 
