@@ -1840,13 +1840,14 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
         bool stack = true;
         if (const char* e = getenv("AXG_BATCH_STACK")) stack = strcmp(e, "0") != 0;
         // a stack is closed before its step working set (fields, source transforms, accumulators)
-        // passes 160 MiB, so a stack's passes hand their arrays to each other through the 256 MB
+        // passes 210 MiB, so a stack's passes hand their arrays to each other through the 256 MB
         // last-level cache (measured on 16 x 64^3 + 16 x 96^3 + 16 x 128^3, 600 steps, graph of
-        // 20 steps: 1.344 s at 160-200 MiB, where the 33 MiB 64^3 lattices go four or six to a stack
-        // instead of three (a 96^3 lattice takes 102 MiB, a 128^3 one 232 MiB), 1.359-1.360 s
-        // at 128 MiB, 1.365-1.366 s at 96; with 10-step graphs, earlier: 1.44-1.45 s at 32-64 MiB,
-        // 1.46 s unlimited); tuning knob AXG_BATCH_STACK_MB (0 = no limit)
-        double stack_mb = 160;
+        // 20 steps: 1.319-1.321 s at 210-240 MiB, where the 102 MiB 96^3 lattices go two to a
+        // stack and the 33 MiB 64^3 ones six or seven, against 1.332-1.334 s at 160 MiB on the same
+        // box (1.344 s at 160-200 MiB on another, 1.359-1.360 s at 128, 1.365-1.366 s at 96); a
+        // 128^3 lattice takes 232 MiB and stays alone; with 10-step graphs, earlier: 1.44-1.45 s
+        // at 32-64 MiB, 1.46 s unlimited); tuning knob AXG_BATCH_STACK_MB (0 = no limit)
+        double stack_mb = 210;
         if (const char* e = getenv("AXG_BATCH_STACK_MB")) stack_mb = atof(e);
         auto step_mb = [](const axg_handle* h) {
             const double fields = (double)h->n3 * (3 * (double)h->t2 + (h->comp ? 12.0 : 0.0));

@@ -263,7 +263,7 @@ def run_adjoint(capi, N=128, L=32.0, steps=800, K=50, kappa0=1e-6, seed=99, t_en
                                    f"differences to converge (relative steps 1e-4 and 1e-6 disagree by 4x)"))
 
 
-def run_batch(capi, steps=600, chunk=20, t_end=16.0):
+def run_batch(capi, steps=600, chunk=40, t_end=16.0):
     """Independent realizations of uneven sizes advanced together: 16 x 64^3 + 16 x 96^3 +
     16 x 128^3, dspa = 0.25, seeds 0..47, FP32C, GW on (64 bins), `steps` steps spanning tau
     1 .. t_end (a fixed 0.2 dspa would pass the leapfrog's stability limit near tau = 23).
