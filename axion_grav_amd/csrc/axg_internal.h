@@ -197,7 +197,7 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
                          const StepRef& sr = StepRef{}, const LatRef* lat = nullptr, int nlat = 1);
 
 template <typename T>
-void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int nb,
+void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const uint16_t* bin_of_n2, int64_t n2max, int nb,
                    double* partial, int nwg, double* out);
 
 // one accumulator component between the device layout and the compact [kz][ky][kx<=N/2]

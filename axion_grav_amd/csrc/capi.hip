@@ -129,6 +129,7 @@ struct axg_handle {
     int ypass_variant = 0;   // configuration of the y pass (axg_tune; 0 = 128-byte column tiles)
     // diagnostics
     uint16_t* gw_bin = nullptr;
+    int64_t gw_n2max = 0;     // largest |n|^2 with a GW bin
     uint16_t* spec_bin = nullptr;
     int64_t nn2 = 0;          // entries of the n^2 tables (3 (N/2)^2 + 1)
     double* red = nullptr;    // reduction scratch
@@ -542,6 +543,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
         {
             std::vector<uint16_t> tab;
             bin_table(h->nn2, p.num_bin_grav, p.dspa, p.dwav, tab, nullptr);
+            for (int64_t n2 = 0; n2 < h->nn2; n2++)
+                if (tab[n2]) h->gw_n2max = n2;
             h->gw_bin = (uint16_t*)h->alloc(h->nn2 * 2);
             AXG_HIP(hipMemcpy(h->gw_bin, tab.data(), h->nn2 * 2, hipMemcpyHostToDevice));
             if (p.spec_est) {
@@ -1012,8 +1015,8 @@ extern "C" int axg_gw_spectrum(axg_handle* h, double* qmid, double* power, doubl
         const int nwg = 1024;
         require((size_t)(nwg + 1) * 2 * nb * sizeof(double) <= h->red_bytes, "reduction scratch too small");
         double* out = h->red + (size_t)nwg * 2 * nb;
-        if (h->fp32) launch_gw_bin<float>(h->stream, h->N, h->L, h->A, h->gw_bin, nb, h->red, nwg, out);
-        else launch_gw_bin<double>(h->stream, h->N, h->L, h->A, h->gw_bin, nb, h->red, nwg, out);
+        if (h->fp32) launch_gw_bin<float>(h->stream, h->N, h->L, h->A, h->gw_bin, h->gw_n2max, nb, h->red, nwg, out);
+        else launch_gw_bin<double>(h->stream, h->N, h->L, h->A, h->gw_bin, h->gw_n2max, nb, h->red, nwg, out);
         std::vector<double> r(2 * nb);
         AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
