@@ -287,7 +287,7 @@ template <typename T, bool COMP>
 void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, const int8_t* flags,
                          double2* out, double fill_unmasked, bool use_pi);
 
-void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int nb,
+void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int64_t n2max, int nb,
                      double scale, double* partial, int nwg, double* out);
 
 void launch_n2_hist(hipStream_t s, int N, const double2* data, double* plane_part, double* hist);

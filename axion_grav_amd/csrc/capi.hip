@@ -131,6 +131,7 @@ struct axg_handle {
     uint16_t* gw_bin = nullptr;
     int64_t gw_n2max = 0;     // largest |n|^2 with a GW bin
     uint16_t* spec_bin = nullptr;
+    int64_t spec_n2max = 0;   // largest |n|^2 with a field-spectrum bin
     int64_t nn2 = 0;          // entries of the n^2 tables (3 (N/2)^2 + 1)
     double* red = nullptr;    // reduction scratch
     size_t red_bytes = 0;
@@ -549,6 +550,8 @@ extern "C" int axg_create(const axg_params* pin, axg_handle** out)
             AXG_HIP(hipMemcpy(h->gw_bin, tab.data(), h->nn2 * 2, hipMemcpyHostToDevice));
             if (p.spec_est) {
                 bin_table(h->nn2, p.num_bin, p.dspa, p.dwav, tab, &h->wav);
+                for (int64_t n2 = 0; n2 < h->nn2; n2++)
+                    if (tab[n2]) h->spec_n2max = n2;
                 h->spec_bin = (uint16_t*)h->alloc(h->nn2 * 2);
                 AXG_HIP(hipMemcpy(h->spec_bin, tab.data(), h->nn2 * 2, hipMemcpyHostToDevice));
             }
@@ -1599,7 +1602,7 @@ extern "C" int axg_field_spectrum(axg_handle* h, double tau, double* qmid, doubl
         fft3d_work(h, h->work);
         double* out = h->red + (size_t)nwg * 2 * nb;
         require((size_t)(nwg + 1) * 2 * nb * 8 <= h->red_bytes, "reduction scratch too small");
-        launch_spec_bin(h->stream, h->N, h->work, h->spec_bin, nb, p.dv_spa, h->red, nwg, out);
+        launch_spec_bin(h->stream, h->N, h->work, h->spec_bin, h->spec_n2max, nb, p.dv_spa, h->red, nwg, out);
         std::vector<double> r(2 * nb);
         AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * 8, hipMemcpyDeviceToHost, h->stream));
         AXG_HIP(hipStreamSynchronize(h->stream));
@@ -1714,7 +1717,7 @@ extern "C" int axg_field_covariance(axg_handle* h, uint64_t seed, int32_t min_mc
             fft3d_work(h, h->work, -1);
             launch_mask_zero(h->stream, h->n3, h->flags, h->work);
             fft3d_work(h, h->work, +1);
-            launch_spec_bin(h->stream, N, h->work, h->spec_bin, nb, 1.0, part, nwg, out);
+            launch_spec_bin(h->stream, N, h->work, h->spec_bin, h->spec_n2max, nb, 1.0, part, nwg, out);
             AXG_HIP(hipMemcpyAsync(r.data(), out, 2 * nb * 8, hipMemcpyDeviceToHost, h->stream));
             AXG_HIP(hipStreamSynchronize(h->stream));
             // pow = mat_deconv pow0 - power (dspmv with beta = 1, spectrum.f90:675-677)

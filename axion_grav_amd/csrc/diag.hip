@@ -421,48 +421,80 @@ void launch_masked_field(hipStream_t s, int N, const void* pi, const void* cpi, 
 // j = |nx| = 0..N/2 (kx = j and kx = N-j), along which the bin is non-decreasing, then every
 // run of equal bins is summed by the thread at its start (same deterministic scheme as the
 // GW binning).  pow[b] += |d*amp|^2 (times dv_wav on the host), num[b] += 1 per mode.
-__global__ void spec_bin_kernel(int N, const double2* __restrict__ d, const uint16_t* __restrict__ bin_of_n2, int nb,
-                                double amp, double* __restrict__ partial)
+// As in the GW binning, each thread loads its values of the next row while the current row is
+// binned (only modes up to the last bin's |n|^2, without waiting for the bin lookup), and each run
+// is summed over the length its last element recorded; the sums and their order are unchanged.
+constexpr int SPEC_BIN_NT = 256, SPEC_BIN_KPT = 3;  // folded j values per thread: N / 2 + 1 <= 768
+
+__global__ void __launch_bounds__(SPEC_BIN_NT) spec_bin_kernel(int N, const double2* __restrict__ d,
+                                                               const uint16_t* __restrict__ bin_of_n2, int64_t n2max,
+                                                               int nb, double amp, double* __restrict__ partial)
 {
     extern __shared__ double sh[];
     double* bins = sh;
     double* v = sh + 2 * nb;
     double* cnt = v + (N / 2 + 1);
     int* bb = (int*)(cnt + (N / 2 + 1));
+    int* rend = bb + (N / 2 + 1);  // nb: one past the last j of bin b's run in the row
     const int nh = N / 2 + 1;
     const int64_t NN = (int64_t)N * N;
     for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) bins[b] = 0;
-    __syncthreads();
-    for (int64_t row = blockIdx.x; row < NN; row += gridDim.x) {
-        int kz = (int)(row / N), ky = (int)(row - (int64_t)kz * N);
-        int ny = wrapn(ky, N), nz = wrapn(kz, N);
+    double2 a[SPEC_BIN_KPT], e[SPEC_BIN_KPT];
+    int bq[SPEC_BIN_KPT];
+    auto fetch = [&](int64_t row) {
+        const int kz = (int)(row / N), ky = (int)(row - (int64_t)kz * N);
+        const int ny = wrapn(ky, N), nz = wrapn(kz, N);
         const double2* r = d + row * N;
-        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
-            int64_t n2 = (int64_t)j * j + (int64_t)ny * ny + (int64_t)nz * nz;
-            int b = bin_of_n2[n2];
-            double s = 0, c = 0;
-            if (b > 0) {
-                double2 a = r[j];
-                double ax = a.x * amp, ay = a.y * amp;
-                s = ax * ax + ay * ay;
-                c = 1;
-                if (j > 0 && j < N / 2) {
-                    double2 e = r[N - j];
-                    double ex = e.x * amp, ey = e.y * amp;
-                    s += ex * ex + ey * ey;
-                    c = 2;
+#pragma unroll
+        for (int i = 0; i < SPEC_BIN_KPT; i++) {
+            const int j = threadIdx.x + i * SPEC_BIN_NT;
+            if (j < nh) {
+                const int64_t n2 = (int64_t)j * j + (int64_t)ny * ny + (int64_t)nz * nz;
+                bq[i] = bin_of_n2[n2];
+                if (n2 <= n2max) {
+                    a[i] = r[j];
+                    if (j > 0 && j < N / 2) e[i] = r[N - j];
                 }
             }
-            v[j] = s;
-            cnt[j] = c;
-            bb[j] = b;
+        }
+    };
+    int64_t row = blockIdx.x;
+    if (row < NN) fetch(row);
+    __syncthreads();
+    for (; row < NN; row += gridDim.x) {
+#pragma unroll
+        for (int i = 0; i < SPEC_BIN_KPT; i++) {
+            const int j = threadIdx.x + i * SPEC_BIN_NT;
+            if (j < nh) {
+                double s = 0, c = 0;
+                if (bq[i] > 0) {
+                    double ax = a[i].x * amp, ay = a[i].y * amp;
+                    s = ax * ax + ay * ay;
+                    c = 1;
+                    if (j > 0 && j < N / 2) {
+                        double ex = e[i].x * amp, ey = e[i].y * amp;
+                        s += ex * ex + ey * ey;
+                        c = 2;
+                    }
+                }
+                v[j] = s;
+                cnt[j] = c;
+                bb[j] = bq[i];
+            }
+        }
+        if (row + gridDim.x < NN) fetch(row + gridDim.x);
+        __syncthreads();
+        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+            const int b = bb[j];
+            if (b > 0 && (j == nh - 1 || bb[j + 1] != b)) rend[b - 1] = j + 1;
         }
         __syncthreads();
         for (int j = threadIdx.x; j < nh; j += blockDim.x) {
             int b = bb[j];
             if (b > 0 && (j == 0 || bb[j - 1] != b)) {
+                const int end = rend[b - 1];
                 double sn = 0, sd = 0;
-                for (int k = j; k < nh && bb[k] == b; k++) { sn += v[k]; sd += cnt[k]; }
+                for (int k = j; k < end; k++) { sn += v[k]; sd += cnt[k]; }
                 bins[b - 1] += sn;
                 bins[nb + b - 1] += sd;
             }
@@ -472,12 +504,14 @@ __global__ void spec_bin_kernel(int N, const double2* __restrict__ d, const uint
     for (int b = threadIdx.x; b < 2 * nb; b += blockDim.x) partial[(int64_t)blockIdx.x * 2 * nb + b] = bins[b];
 }
 
-void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int nb, double scale,
-                     double* partial, int nwg, double* out)
+void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* bin_of_n2, int64_t n2max, int nb,
+                     double scale, double* partial, int nwg, double* out)
 {
     int nh = N / 2 + 1;
-    size_t lds = (size_t)(2 * nb + 2 * nh) * sizeof(double) + (size_t)nh * sizeof(int);
-    hipLaunchKernelGGL(spec_bin_kernel, dim3(nwg), dim3(256), lds, s, N, data, bin_of_n2, nb, scale, partial);
+    if (nh > SPEC_BIN_KPT * SPEC_BIN_NT) throw Error(AXG_EINVAL, "field spectrum: N too large for the binning kernel");
+    size_t lds = (size_t)(2 * nb + 2 * nh) * sizeof(double) + (size_t)(nh + nb) * sizeof(int);
+    hipLaunchKernelGGL(spec_bin_kernel, dim3(nwg), dim3(SPEC_BIN_NT), lds, s, N, data, bin_of_n2, n2max, nb, scale,
+                       partial);
     AXG_HIP(hipGetLastError());
     sum_columns(s, partial, nwg, 2 * nb, out);
 }
