@@ -516,29 +516,64 @@ void launch_spec_bin(hipStream_t s, int N, const double2* data, const uint16_t* 
     sum_columns(s, partial, nwg, 2 * nb, out);
 }
 
-// Histogram of |W|^2 by n^2 = |n|^2, deterministic: block kz walks its plane row by row
-// (fixed order); within one row the folded j = |nx| hit distinct m = j^2 + ny^2, so the
-// per-plane partial plane[kz][m] needs no atomics.  hist[n2] = sum over kz (in order) of
-// plane[kz][n2 - nz^2].
-__global__ void n2_plane_kernel(int N, const double2* __restrict__ d, double* __restrict__ plane)
+// Histogram of |W|^2 by n^2 = |n|^2, deterministic: block kz walks its plane by a = |ny|
+// (a = 0 .. N/2, in order).  The rows ky = a and ky = N - a share a, so their values at each
+// folded j = |nx| land on the same m = j^2 + a^2 and are added together first; within one a the
+// j hit distinct m, so the per-plane partial plane[kz][m] needs no atomics.  Each thread loads
+// its values of the next a while the current one is added (the input is read-only).
+// hist[n2] = sum over kz (in order) of plane[kz][n2 - nz^2].
+constexpr int N2P_NT = 256, N2P_KPT = 3;  // folded j values per thread: N / 2 + 1 <= 768
+
+__global__ void __launch_bounds__(N2P_NT) n2_plane_kernel(int N, const double2* __restrict__ d,
+                                                          double* __restrict__ plane)
 {
     const int nh = N / 2 + 1;
     const int64_t mm = (int64_t)2 * (N / 2) * (N / 2) + 1;
     const int kz = blockIdx.x;
     double* pp = plane + (int64_t)kz * mm;
     for (int64_t m = threadIdx.x; m < mm; m += blockDim.x) pp[m] = 0;
-    __syncthreads();
-    for (int ky = 0; ky < N; ky++) {
-        int ny = wrapn(ky, N);
-        const double2* r = d + ((int64_t)kz * N + ky) * N;
-        for (int j = threadIdx.x; j < nh; j += blockDim.x) {
-            double2 a = r[j];
-            double s = a.x * a.x + a.y * a.y;
-            if (j > 0 && j < N / 2) {
-                double2 e = r[N - j];
-                s += e.x * e.x + e.y * e.y;
+    double2 u[N2P_KPT][4];  // rows a and N - a, each at j and N - j
+    auto fetch = [&](int a) {
+        const double2* r1 = d + ((int64_t)kz * N + a) * N;
+        const double2* r2 = d + ((int64_t)kz * N + (N - a)) * N;
+        const bool two = a > 0 && a < N / 2;
+#pragma unroll
+        for (int i = 0; i < N2P_KPT; i++) {
+            const int j = threadIdx.x + i * N2P_NT;
+            if (j < nh) {
+                const bool mir = j > 0 && j < N / 2;
+                u[i][0] = r1[j];
+                if (mir) u[i][1] = r1[N - j];
+                if (two) {
+                    u[i][2] = r2[j];
+                    if (mir) u[i][3] = r2[N - j];
+                }
             }
-            pp[(int64_t)j * j + (int64_t)ny * ny] += s;
+        }
+    };
+    fetch(0);
+    __syncthreads();
+    for (int a = 0; a <= N / 2; a++) {
+        const bool two = a > 0 && a < N / 2;
+        double s[N2P_KPT];
+#pragma unroll
+        for (int i = 0; i < N2P_KPT; i++) {
+            const int j = threadIdx.x + i * N2P_NT;
+            const bool mir = j > 0 && j < N / 2;
+            double t = u[i][0].x * u[i][0].x + u[i][0].y * u[i][0].y;
+            if (mir) t += u[i][1].x * u[i][1].x + u[i][1].y * u[i][1].y;
+            if (two) {
+                double t2 = u[i][2].x * u[i][2].x + u[i][2].y * u[i][2].y;
+                if (mir) t2 += u[i][3].x * u[i][3].x + u[i][3].y * u[i][3].y;
+                t += t2;
+            }
+            s[i] = t;
+        }
+        if (a < N / 2) fetch(a + 1);
+#pragma unroll
+        for (int i = 0; i < N2P_KPT; i++) {
+            const int j = threadIdx.x + i * N2P_NT;
+            if (j < nh) pp[(int64_t)j * j + (int64_t)a * a] += s[i];
         }
         __syncthreads();
     }
@@ -561,7 +596,8 @@ __global__ void n2_merge_kernel(int N, const double* __restrict__ plane, double*
 
 void launch_n2_hist(hipStream_t s, int N, const double2* data, double* plane_part, double* hist)
 {
-    hipLaunchKernelGGL(n2_plane_kernel, dim3(N), dim3(256), 0, s, N, data, plane_part);
+    if (N / 2 + 1 > N2P_KPT * N2P_NT) throw Error(AXG_EINVAL, "field spectrum: N too large for the window histogram");
+    hipLaunchKernelGGL(n2_plane_kernel, dim3(N), dim3(N2P_NT), 0, s, N, data, plane_part);
     AXG_HIP(hipGetLastError());
     int64_t nn2 = (int64_t)3 * (N / 2) * (N / 2) + 1;
     hipLaunchKernelGGL(n2_merge_kernel, dim3((unsigned)((nn2 + 255) / 256)), dim3(256), 0, s, N, plane_part, hist);
