@@ -586,10 +586,16 @@ __global__ void n2_merge_kernel(int N, const double* __restrict__ plane, double*
     int64_t n2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (n2 >= nn2) return;
     double s = 0;
+    // unrolled, every load made (a plane without this n2 reads its entry 0 and adds +0, which
+    // leaves the non-negative sum unchanged), so the planes' loads are in flight together; the
+    // sum keeps its order
+#pragma unroll 8
     for (int kz = 0; kz < N; kz++) {
         int nz = wrapn(kz, N);
         int64_t m = n2 - (int64_t)nz * nz;
-        if (m >= 0 && m < mm) s += plane[(int64_t)kz * mm + m];
+        const bool in = m >= 0 && m < mm;
+        const double v = plane[(int64_t)kz * mm + (in ? m : 0)];
+        s += in ? v : 0.0;
     }
     hist[n2] = s;
 }
