@@ -207,7 +207,7 @@ __device__ __forceinline__ float comp_put(double x, short& lo)
 // Streaming accesses: data a pass touches once (its outputs, pi, the transform inputs and the
 // accumulators) moves with non-temporal loads and stores, which leave L2 to the re-read halo
 // rows of phi.  Measured at 512^3 FP32C: fused pass 2.49 -> 2.35 ms, y pass 1.40 -> 1.34 ms,
-// z pass 1.65 -> 1.62 ms, step 5.62 -> 5.38 ms (same box, results bit-identical).
+// z pass 1.65 -> 1.62 ms, step 5.62 -> 5.38 ms (same GPU, results bit-identical).
 template <typename T2>
 __device__ __forceinline__ T2 ld_stream(const T2* p)
 {

@@ -1850,7 +1850,7 @@ extern "C" int axg_batch_advance(axg_handle* const* hs, int32_t n, double t1, do
         // last-level cache (measured on 16 x 64^3 + 16 x 96^3 + 16 x 128^3, 600 steps, graph of
         // 20 steps: 1.319-1.321 s at 210-240 MiB, where the 102 MiB 96^3 lattices go two to a
         // stack and the 33 MiB 64^3 ones six or seven, against 1.332-1.334 s at 160 MiB on the same
-        // box (1.344 s at 160-200 MiB on another, 1.359-1.360 s at 128, 1.365-1.366 s at 96); a
+        // GPU (1.344 s at 160-200 MiB on another, 1.359-1.360 s at 128, 1.365-1.366 s at 96); a
         // 128^3 lattice takes 232 MiB and stays alone; with 10-step graphs, earlier: 1.44-1.45 s
         // at 32-64 MiB, 1.46 s unlimited); tuning knob AXG_BATCH_STACK_MB (0 = no limit)
         double stack_mb = 210;
