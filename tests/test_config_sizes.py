@@ -3,7 +3,8 @@
 Each test compares the HIP path (through the C-ABI) with the CPU FP64 oracle on the same inputs:
 
 * 512^3 (L = 128): one FP64 DKD step with GW from a GPU-evolved field -- fields bit-exact,
-  accumulators within 1e-10 -- and the cell classification of that field: flags and counts
+  accumulators within 1e-10, GW spectrum bin counts bit-exact and bin sums within 1e-10 -- and
+  the cell classification of that field: flags and counts
   bit-exact (integer work; the thresholds at pi and 7 pi/4 depend on device atan2 agreeing with
   the host's over 134 M cells); the plaquette windings of that field, bit-exact, with the
   winding invariants (zero net winding per orientation, none out of any cell).
@@ -60,7 +61,10 @@ def field512():
 
 def test_fp64_512_step_matches_oracle(field512):
     """One DKD step with GW at 512^3 in FP64 (the precision form's reference path): phi and pi
-    bit-exact, the four accumulators within 1e-10 of the oracle's full-spectrum ones."""
+    bit-exact, the four accumulators within 1e-10 of the oracle's full-spectrum ones, and the
+    128-bin GW spectrum of the device accumulators (257 folded kx per row, so a block thread
+    bins more than one; rows whose modes lie past the last bin are skipped) against the oracle's
+    binning of its own: mode counts bit-exact, power sums within 1e-10."""
     N = 512
     phi0, pi0 = field512
     say("512^3 FP64 step vs oracle")
@@ -72,6 +76,7 @@ def test_fp64_512_step_matches_oracle(field512):
         L.advance_steps(t1, dt, 1)
         g_phi, g_pi = L.get_fields()
         A = L.gw_state()
+        spec = L.gw_spectrum()
     phi, pi = phi0.copy(), pi0.copy()
     OA_ = O.new_gw_state(N)
     O.symplectic(phi, pi, OA_, t1, dt, 1, P)
@@ -83,6 +88,12 @@ def test_fp64_512_step_matches_oracle(field512):
         e = relerr(A[m], OA_[m][:, :, :h])
         say(f"  accumulator {m}: rel err {e:.2e}")
         assert e < 1e-10, (m, e)
+    del A
+    o = O.gw_spectrum(OA_, 128, P["dspa"], P["dwav"], P["tau_cr"], P["a_now"], P["dv_wav"])
+    assert np.array_equal(spec["denom"], o["denom"])
+    e = relerr(spec["numer"], o["numer"])
+    say(f"  GW spectrum: rel err {e:.2e}")
+    assert e < 1e-10, e
 
 
 def test_classify_512_matches_oracle(field512):
