@@ -867,14 +867,19 @@ void launch_gw_zpass_acc(hipStream_t s, int N, int pitch, const void* W, void* A
 // the same bit for bit.  512^3, 128 bins: kernel 1.13 -> 0.77 ms in FP32C (its 1.18 GB now move
 // with the lookups and run sums overlapped), whole spectrum call 1.43 -> 0.83 ms (FP64 1.36 ->
 // 0.84-0.87 ms).
-constexpr int GW_BIN_NT = 256, GW_BIN_KPT = 3;  // kx values per thread: N / 2 + 1 <= 768
+// block size: 512 threads for FP32 accumulators, 256 for FP64 (512^3: 0.83 -> 0.72 ms per FP32C
+// spectrum with 512, FP64 0.84-0.87 -> 1.04 ms, so FP64 keeps 256); kx values per thread up to
+// N / 2 + 1 <= 768
+template <typename T> constexpr int gw_bin_nt() { return sizeof(T) == 4 ? 512 : 256; }
+template <typename T> constexpr int gw_bin_kpt() { return (768 + gw_bin_nt<T>() - 1) / gw_bin_nt<T>(); }
 
 template <typename T>
-__global__ void __launch_bounds__(GW_BIN_NT) gw_bin_kernel(int N, GwLayout L, const typename V2<T>::type* __restrict__ A,
+__global__ void __launch_bounds__(gw_bin_nt<T>()) gw_bin_kernel(int N, GwLayout L, const typename V2<T>::type* __restrict__ A,
                                                            const uint16_t* __restrict__ bin_of_n2, int64_t n2max,
                                                            int nb, double* __restrict__ partial)
 {
     using T2 = typename V2<T>::type;
+    constexpr int GW_BIN_NT = gw_bin_nt<T>(), GW_BIN_KPT = gw_bin_kpt<T>();
     extern __shared__ double sh[];
     double* bins = sh;              // 2*nb: numer, denom
     double* v = sh + 2 * nb;        // N/2+1 values
@@ -969,8 +974,8 @@ void launch_gw_bin(hipStream_t s, int N, const GwLayout& L, const void* A, const
     using T2 = typename V2<T>::type;
     int nh = N / 2 + 1;
     size_t lds = (size_t)(2 * nb + nh) * sizeof(double) + (size_t)(nh + nb) * sizeof(int);
-    if (nh > GW_BIN_KPT * GW_BIN_NT) throw Error(AXG_EINVAL, "GW spectrum: N too large for the binning kernel");
-    hipLaunchKernelGGL((gw_bin_kernel<T>), dim3(nwg), dim3(GW_BIN_NT), lds, s, N, L, (const T2*)A, bin_of_n2, n2max,
+    if (nh > gw_bin_kpt<T>() * gw_bin_nt<T>()) throw Error(AXG_EINVAL, "GW spectrum: N too large for the binning kernel");
+    hipLaunchKernelGGL((gw_bin_kernel<T>), dim3(nwg), dim3(gw_bin_nt<T>()), lds, s, N, L, (const T2*)A, bin_of_n2, n2max,
                        nb, partial);
     AXG_HIP(hipGetLastError());
     hipLaunchKernelGGL(sum_partials_kernel, dim3((2 * nb + 255) / 256), dim3(256), 0, s, partial, nwg, 2 * nb, out);
